@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Incoherent-extend benchmark of RenderCore_HIP on one MI355X.
+
+Workload (incoherent traversal microbenchmark): 4,000,000 random triangles
+(centroids U[0,1)^3, vertex offsets N(0, 0.005^2), seed 1), binned-SAH BVH
+built on the GPU; 1920x1080 primary rays from (0.5,0.5,-1.2) towards
+(0.5,0.5,0.5), FOV 40, then 3 cosine-weighted diffuse bounces (misses die).
+One step = primary generation + 4 extend waves + 3 bounce kernels.
+
+metric: "Mrays/s incoherent extend" = rays of waves 2..4 / time of their extend
+launches (HIP events on the launch stream), the reference's rays/traceTime
+definition (lib/RenderCore_PrimeRef/rendercore.cpp:568-576,
+apps/benchmarkapp/main.cpp:479-482). Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NODE_BYTES = {"q8": 64, "f32": 64}
+TRI_BYTES = 48          # leaf triangle: 3 float4
+RAY_IO_BYTES = 48       # 32 B ray in + 16 B hit record out
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tris", type=int, default=4_000_000)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--format", choices=["q8", "f32"], default="q8")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+class Waves:
+    """Device buffers and launches of one step."""
+
+    def __init__(self, torch, core, view, W, H):
+        self.torch, self.core, self.view, self.W, self.H = torch, core, view, W, H
+        n = W * H
+        self.n = n
+        f = dict(device="cuda")
+        self.rays = [torch.zeros((n, 8), dtype=torch.float32, **f) for _ in range(2)]
+        self.pid = [torch.zeros(n, dtype=torch.int32, **f) for _ in range(2)]
+        self.hits = torch.zeros((n, 4), dtype=torch.float32, **f)
+        self.cnt = torch.zeros(1, dtype=torch.int32, **f)
+        self.stream = torch.cuda.current_stream()
+        self.sh = self.stream.cuda_stream
+
+    def step(self, events=None, keep=None):
+        """Runs one step; returns [n1, n2, n3, n4] ray counts per wave."""
+        from lighthouse2_amd import core as C
+        torch = self.torch
+        C.generate_primary(self.view, self.W, self.H, 1234, 1e-5, self.rays[0].data_ptr(), self.pid[0].data_ptr(), self.sh)
+        counts, cur, n = [], 0, self.n
+        for wave in range(4):
+            counts.append(n)
+            if events is not None:
+                events[wave][0].record(self.stream)
+            self.core.trace_closest(self.rays[cur].data_ptr(), self.hits.data_ptr(), n, self.sh)
+            if events is not None:
+                events[wave][1].record(self.stream)
+            if keep is not None and wave >= 1:
+                keep.append((self.rays[cur][:n].clone(), self.hits[:n].clone()))
+            if wave == 3:
+                break
+            self.core.bounce_diffuse(self.rays[cur].data_ptr(), self.hits.data_ptr(), self.pid[cur].data_ptr(), n,
+                                     (wave + 1) * 91771, 1e-5, self.rays[cur ^ 1].data_ptr(), self.pid[cur ^ 1].data_ptr(),
+                                     self.cnt.data_ptr(), self.sh)
+            n = int(self.cnt.item())
+            cur ^= 1
+        return counts
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {a.gpus}); "
+              "the traversal shards by image tile with the scene replicated, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import lighthouse2_amd as lh
+    from lighthouse2_amd import scenes
+
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    verts = scenes.random_triangles(a.tris, seed=1)
+    t_gen = time.time() - t0
+    core = lh.RenderCore()
+    core.Setting("nodeFormat", 1 if a.format == "q8" else 0)
+    t0 = time.time()
+    core.SetGeometry(0, verts, None)
+    core.SetInstance(0, 0)
+    core.FinalizeInstances()
+    t_build = time.time() - t0
+    info = core.bvh_info(0)
+    view = scenes.microbench_view(a.width, a.height)
+    wv = Waves(torch, core, view, a.width, a.height)
+
+    for _ in range(a.warmup):
+        wv.step()
+    torch.cuda.synchronize()
+    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(4)] for _ in range(a.steps)]
+    per_step = []
+    torch.cuda.synchronize()
+    tw0 = time.time()
+    for s in range(a.steps):
+        per_step.append(wv.step(ev[s]))
+    torch.cuda.synchronize()
+    wall = time.time() - tw0
+    ms = [[ev[s][w][0].elapsed_time(ev[s][w][1]) for w in range(4)] for s in range(a.steps)]
+    inco_rays = sum(sum(c[1:]) for c in per_step)
+    inco_ms = sum(sum(m[1:]) for m in ms)
+    prim_rays = sum(c[0] for c in per_step)
+    prim_ms = sum(m[0] for m in ms)
+    value = inco_rays / (inco_ms * 1e-3) / 1e6
+
+    # algorithmic work per incoherent ray, counted by the kernel on the same rays (untimed)
+    keep = []
+    wv.step(keep=keep)
+    torch.cuda.synchronize()
+    nodes = tris = nrays = 0
+    scratch = torch.zeros_like(wv.hits)
+    for r, _h in keep:
+        n_, t_ = core.trace_closest_counted(r.data_ptr(), scratch.data_ptr(), len(r), wv.sh)
+        nodes, tris, nrays = nodes + n_, tris + t_, nrays + len(r)
+    nodes_per_ray, tris_per_ray = nodes / nrays, tris / nrays
+    bytes_per_ray = RAY_IO_BYTES + nodes_per_ray * NODE_BYTES[a.format] + tris_per_ray * TRI_BYTES
+    launch_ms = [m for st in ms for m in st[1:]]
+    launch_rays = [c for st in per_step for c in st[1:]]
+    achieved = float(np.mean([r * bytes_per_ray / (m * 1e-3) / 1e9 for r, m in zip(launch_rays, launch_ms)]))
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "extend_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+
+    cpu = None
+    parity = None
+    if not a.no_cpu:
+        cpu, parity = cpu_baseline(a, verts, keep, np)
+
+    out = {
+        "metric": "Mrays/s incoherent extend",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random triangles, generated on the host)",
+        "config": {"workload": "incoherent traversal: 4M random triangles, 1920x1080 primary + 3 diffuse bounces",
+                   "triangles": a.tris, "width": a.width, "height": a.height, "node_format": a.format,
+                   "bvh": "binned SAH 16 bins, leaf <= 4, GPU build", "seed": 1},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 2),
+                     "tris_per_ray": round(tris_per_ray, 2)},
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "primary_mrays_s": round(prim_rays / (prim_ms * 1e-3) / 1e6, 2),
+        "incoherent_rays_per_step": int(inco_rays / a.steps),
+        "extend_ms_per_launch": round(float(np.mean(launch_ms)), 4),
+        "wall_mrays_s_all_waves": round((inco_rays + prim_rays) / wall / 1e6, 2),
+        "bvh": {"build_s": round(t_build, 3), "scene_gen_s": round(t_gen, 3), **info},
+    }
+    core.Shutdown()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(a, verts, keep, np):
+    """The oracle's BVH traversal (oracle/, a C++ restatement of the same
+    closest-hit contract and BVH rules) on a bounded sample of the same
+    incoherent rays, on this host's cores; also checks the GPU hits of that
+    sample bit for bit."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    threads = min(16, os.cpu_count() or 1)
+    s = __import__("lighthouse2_amd").scenes.Scene()
+    s.add_instance(s.add_mesh(verts, None))
+    t0 = time.time()
+    o = orc.OracleScene(s)
+    t_build = time.time() - t0
+    rays = np.concatenate([r.cpu().numpy() for r, _ in keep])
+    hits = np.concatenate([h.cpu().numpy() for _, h in keep])
+    rng = np.random.default_rng(0)
+    idx = rng.choice(len(rays), size=min(len(rays), 200_000), replace=False)
+    t0 = time.time()
+    ref = o.trace(rays[idx], threads=threads)
+    dt = time.time() - t0
+    mism = int((ref.view(np.uint32) != hits[idx].view(np.uint32)).any(1).sum())
+    # scale the sample so the timed CPU work is about cpu_seconds
+    reps = max(1, int(a.cpu_seconds / max(dt, 1e-3)))
+    n_total, t_total = len(idx), dt
+    sample = rays[idx]
+    for _ in range(reps - 1):
+        t0 = time.time()
+        o.trace(sample, threads=threads)
+        t_total += time.time() - t0
+        n_total += len(idx)
+        if t_total > a.cpu_seconds:
+            break
+    cpu = {"value": round(n_total / t_total / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": f"{n_total} incoherent rays (random 200k of waves 2-4, repeated) traced by the oracle BVH on "
+                     f"{threads} threads in {t_total:.1f} s; oracle BVH build {t_build:.1f} s excluded"}
+    return cpu, {"checked": int(len(idx)), "mismatches": mism}
+
+
+if __name__ == "__main__":
+    main()

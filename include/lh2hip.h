@@ -1,0 +1,125 @@
+/* lh2hip.h - C ABI of librendercore_hip.so (RenderCore_HIP).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status returns
+ * (0 = ok, negative = error; lh2hip_last_error() holds the message of the most
+ * recent failure on the calling thread). Device pointers are hipMalloc'd memory
+ * of the current device; a `stream` argument is a hipStream_t (NULL = the null
+ * stream). Every entry point below cites the reference interface it replaces.
+ * The same library exports the C++ plugin symbol CreateCore (core_api_base.h).
+ */
+#ifndef LH2HIP_H
+#define LH2HIP_H
+
+#include "lh2hip_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+using namespace lh2hip;
+#endif
+
+#define LH2HIP_API __attribute__((visibility("default")))
+
+enum
+{
+	LH2HIP_OK = 0,
+	LH2HIP_ERR_INVALID = -1,   /* bad argument or size */
+	LH2HIP_ERR_DEVICE = -2,    /* a HIP call failed */
+	LH2HIP_ERR_STATE = -3,     /* call out of order (e.g. render before scene) */
+	LH2HIP_ERR_OOM = -4,       /* device allocation failed or budget exceeded */
+	LH2HIP_ERR_MISSING = -5    /* no HIP device present */
+};
+
+typedef struct lh2hip_core lh2hip_core;
+
+LH2HIP_API const char* lh2hip_last_error(void);
+LH2HIP_API int lh2hip_version(void);
+
+/* ---- render core (lib/RenderSystem/core_api_base.h:81-119) ---------------- */
+
+/* CoreAPI_Base::Init (lib/RenderCore_PrimeRef/rendercore.cpp:100-142). */
+LH2HIP_API int lh2hip_create(lh2hip_core** out);
+/* CoreAPI_Base::Shutdown (rendercore.cpp:639-). Frees the handle. */
+LH2HIP_API int lh2hip_destroy(lh2hip_core* core);
+/* CoreAPI_Base::SetTarget, headless (rendercore.cpp:148-208). */
+LH2HIP_API int lh2hip_set_target(lh2hip_core* core, int width, int height, int spp);
+/* CoreAPI_Base::Setting (rendercore.cpp:484-502; keys "epsilon", "clampValue",
+ * plus "maxDepth", "nodeFormat" (0 fp32 BVH2, 1 q8 BVH4), "seed"). */
+LH2HIP_API int lh2hip_setting(lh2hip_core* core, const char* name, float value);
+/* CoreAPI_Base::SetGeometry (rendercore.cpp:214-222; core_mesh.cpp:36-64).
+ * vertexData: 3 float4 per triangle (w ignored); builds the mesh BVH on the GPU. */
+LH2HIP_API int lh2hip_set_geometry(lh2hip_core* core, int meshIdx, const lh2_float4* vertexData,
+	int vertexCount, int triangleCount, const CoreTri* triangles);
+/* CoreAPI_Base::SetInstance (rendercore.cpp:228-242); meshIdx -1 truncates. */
+LH2HIP_API int lh2hip_set_instance(lh2hip_core* core, int instanceIdx, int meshIdx, const lh2_mat4* transform);
+/* CoreAPI_Base::FinalizeInstances (rendercore.cpp:248-277): builds the TLAS. */
+LH2HIP_API int lh2hip_finalize_instances(lh2hip_core* core);
+/* CoreAPI_Base::SetMaterials (rendercore.cpp:360-402). */
+LH2HIP_API int lh2hip_set_materials(lh2hip_core* core, const CoreMaterial* mats, int count);
+/* CoreAPI_Base::SetLights (rendercore.cpp:408-460). */
+LH2HIP_API int lh2hip_set_lights(lh2hip_core* core,
+	const CoreLightTri* triLights, int triLightCount,
+	const CorePointLight* pointLights, int pointLightCount,
+	const CoreSpotLight* spotLights, int spotLightCount,
+	const CoreDirectionalLight* dirLights, int dirLightCount);
+/* CoreAPI_Base::SetTextures (rendercore.cpp:283-310). */
+LH2HIP_API int lh2hip_set_textures(lh2hip_core* core, const CoreTexDesc* tex, int count);
+/* CoreAPI_Base::SetSkyData (rendercore.cpp:466-478). */
+LH2HIP_API int lh2hip_set_sky(lh2hip_core* core, const lh2_float3* pixels, int width, int height,
+	const lh2_mat4* worldToLight);
+/* CoreAPI_Base::SetProbePos (rendercore.cpp:91-94). */
+LH2HIP_API int lh2hip_set_probe_pos(lh2hip_core* core, int x, int y);
+/* CoreAPI_Base::Render (rendercore.cpp:535-632): one frame of spp samples per
+ * pixel. converge = LH2_CONVERGE / LH2_RESTART. async = 1 returns after enqueue. */
+LH2HIP_API int lh2hip_render(lh2hip_core* core, const ViewPyramid* view, int converge, int async);
+/* CoreAPI_Base::WaitForRender (rendercore_optix7/rendercore.cpp:945-957). */
+LH2HIP_API int lh2hip_wait(lh2hip_core* core);
+/* Reads the finalized image (accumulator / samplesTaken, float4 per pixel) to
+ * host memory; replaces the GL surface write of finalizeRender
+ * (lib/CUDA/shared_kernel_code/finalize_shared.h:29-45). */
+LH2HIP_API int lh2hip_read_image(lh2hip_core* core, lh2_float4* hostOut);
+/* Reads the FP64 accumulator (sum of all samples, 4 doubles per pixel). */
+LH2HIP_API int lh2hip_read_accumulator_f64(lh2hip_core* core, double* hostOut);
+/* CoreAPI_Base::GetCoreStats (core_api_base.h:87). */
+LH2HIP_API int lh2hip_get_stats(lh2hip_core* core, CoreStats* out);
+
+/* ---- traversal queries (OptiX Prime rtpQueryExecute replacement) ---------- */
+
+/* Closest hit, RTP_QUERY_TYPE_CLOSEST (lib/RenderCore_PrimeRef/rendercore.cpp:559-573;
+ * rendercore_optix7/optix/.optix.cu:125-140,174-184). rays/hits are device
+ * pointers; hit = tmin < t < tmax nearest, ties to the smaller (inst, prim). */
+LH2HIP_API int lh2hip_trace_closest(lh2hip_core* core, const Ray4* rays, HitRecord* hits, int count, void* stream);
+/* Any hit, RTP_QUERY_TYPE_ANY + RTP_BUFFER_FORMAT_HIT_BITMASK
+ * (rendercore.cpp:598-607): bit i of occluded[i/32] set when ray i is blocked. */
+LH2HIP_API int lh2hip_trace_any(lh2hip_core* core, const Ray4* rays, uint32_t* occluded, int count, void* stream);
+
+/* Closest hit as lh2hip_trace_closest, with the traversal's work counted:
+ * out2[0] += interior nodes visited, out2[1] += triangles tested (host memory).
+ * A separate kernel variant; used to derive algorithmic bytes per ray. */
+LH2HIP_API int lh2hip_trace_closest_counted(lh2hip_core* core, const Ray4* rays, HitRecord* hits, int count, void* stream,
+	uint64_t* out2);
+
+/* ---- microbenchmark wave helpers (incoherent traversal workload) ----------
+ * Primary rays: generateEyeRaysKernel (lib/RenderCore_PrimeRef/kernels/camera.h:39-68)
+ * with pixel jitter from WangHash(index + R0). */
+LH2HIP_API int lh2hip_generate_primary(const ViewPyramid* view, int width, int height, uint32_t R0,
+	float tmin, Ray4* raysOut, uint32_t* pathIdOut, void* stream);
+/* Next diffuse wave: for each hit ray, a cosine-weighted bounce about the
+ * geometric normal flipped toward the incoming ray (seed WangHash(pathId*17+R0));
+ * misses die; survivors compacted through an atomic counter (*countOut, device). */
+LH2HIP_API int lh2hip_bounce_diffuse(lh2hip_core* core, const Ray4* raysIn, const HitRecord* hits,
+	const uint32_t* pathIdIn, int count, uint32_t R0, float tmin,
+	Ray4* raysOut, uint32_t* pathIdOut, uint32_t* countOut, void* stream);
+/* Copies the core's leaf-ordered BVH statistics (node count, leaf count, depth,
+ * bytes) for the current scene: out[0..7]. */
+LH2HIP_API int lh2hip_bvh_info(lh2hip_core* core, int meshIdx, int64_t* out8);
+/* Downloads the binary build tree of a mesh in canonical form (for
+ * structure comparison): per node {lo.xyz, hi.xyz, leftOrFirst, count} as
+ * 8 x 4 bytes, and the leaf primitive list. Returns node count in *nodeCount. */
+LH2HIP_API int lh2hip_bvh_download(lh2hip_core* core, int meshIdx, void* nodesOut, int maxNodes,
+	uint32_t* primsOut, int maxPrims, int* nodeCount);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LH2HIP_H */

@@ -1,0 +1,220 @@
+"""Python binding of RenderCore_HIP (librendercore_hip.so) over its C ABI.
+
+`RenderCore` mirrors the CoreAPI_Base methods (lib/RenderSystem/core_api_base.h:81-119)
+with the same names, argument meaning and call order; errors raise
+`LH2HIPError` instead of aborting the process. `load_scene` performs the call
+sequence of RenderSystem::SynchronizeSceneData (lib/RenderSystem/rendersystem.cpp:203-211).
+There is no fallback: importing fails loudly when the HIP library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import formats as F
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librendercore_hip.so")
+
+LH2_CONVERGE, LH2_RESTART = 0, 1
+
+
+class LH2HIPError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lh2hip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+P = ctypes.c_void_p
+I = ctypes.c_int
+
+_SIGS = {
+    "lh2hip_last_error": (ctypes.c_char_p, []),
+    "lh2hip_version": (I, []),
+    "lh2hip_create": (I, [ctypes.POINTER(P)]),
+    "lh2hip_destroy": (I, [P]),
+    "lh2hip_set_target": (I, [P, I, I, I]),
+    "lh2hip_setting": (I, [P, ctypes.c_char_p, ctypes.c_float]),
+    "lh2hip_set_geometry": (I, [P, I, P, I, I, P]),
+    "lh2hip_set_instance": (I, [P, I, I, P]),
+    "lh2hip_finalize_instances": (I, [P]),
+    "lh2hip_set_materials": (I, [P, P, I]),
+    "lh2hip_set_lights": (I, [P, P, I, P, I, P, I, P, I]),
+    "lh2hip_set_textures": (I, [P, P, I]),
+    "lh2hip_set_sky": (I, [P, P, I, I, P]),
+    "lh2hip_set_probe_pos": (I, [P, I, I]),
+    "lh2hip_render": (I, [P, P, I, I]),
+    "lh2hip_wait": (I, [P]),
+    "lh2hip_read_image": (I, [P, P]),
+    "lh2hip_read_accumulator_f64": (I, [P, P]),
+    "lh2hip_get_stats": (I, [P, P]),
+    "lh2hip_trace_closest": (I, [P, P, P, I, P]),
+    "lh2hip_trace_any": (I, [P, P, P, I, P]),
+    "lh2hip_trace_closest_counted": (I, [P, P, P, I, P, P]),
+    "lh2hip_generate_primary": (I, [P, I, I, ctypes.c_uint32, ctypes.c_float, P, P, P]),
+    "lh2hip_bounce_diffuse": (I, [P, P, P, P, I, ctypes.c_uint32, ctypes.c_float, P, P, P, P]),
+    "lh2hip_bvh_info": (I, [P, I, P]),
+    "lh2hip_bvh_download": (I, [P, I, P, I, P, I, P]),
+}
+EXPORTED = sorted(_SIGS) + ["CreateCore"]
+
+
+def library():
+    """Loads librendercore_hip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise LH2HIPError(rc, library().lh2hip_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+_p = _ptr
+
+
+class RenderCore:
+    """One RenderCore_HIP instance (CoreAPI_Base::CreateCoreAPI + Init)."""
+
+    def __init__(self):
+        h = P()
+        _check(library().lh2hip_create(ctypes.byref(h)))
+        self._h = h
+        self._keep = []
+
+    def Shutdown(self):
+        if self._h:
+            _check(library().lh2hip_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.Shutdown()
+        except Exception:
+            pass
+
+    # ---- CoreAPI_Base -------------------------------------------------------
+    def SetTarget(self, width, height, spp=1):
+        self.width, self.height, self.spp = width, height, spp
+        _check(library().lh2hip_set_target(self._h, width, height, spp))
+
+    def Setting(self, name, value):
+        _check(library().lh2hip_setting(self._h, name.encode(), float(value)))
+
+    def SetGeometry(self, meshIdx, vertexData, triangles=None):
+        v = np.ascontiguousarray(vertexData, np.float32)
+        assert v.ndim == 2 and v.shape[1] == 4 and len(v) % 3 == 0
+        t = None if triangles is None else np.ascontiguousarray(triangles, F.CoreTri)
+        _check(library().lh2hip_set_geometry(self._h, meshIdx, _ptr(v), len(v), len(v) // 3, _ptr(t)))
+
+    def SetInstance(self, instanceIdx, meshIdx, transform=None):
+        m = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(16)
+        _check(library().lh2hip_set_instance(self._h, instanceIdx, meshIdx, _ptr(m)))
+
+    def FinalizeInstances(self):
+        _check(library().lh2hip_finalize_instances(self._h))
+
+    def SetMaterials(self, materials):
+        m = np.ascontiguousarray(materials, F.CoreMaterial)
+        _check(library().lh2hip_set_materials(self._h, _ptr(m), len(m)))
+
+    def SetLights(self, tri=None, point=None, spot=None, directional=None):
+        a = [np.ascontiguousarray(x if x is not None else np.zeros(0, d), d)
+             for x, d in ((tri, F.CoreLightTri), (point, F.CorePointLight), (spot, F.CoreSpotLight),
+                          (directional, F.CoreDirectionalLight))]
+        _check(library().lh2hip_set_lights(self._h, _ptr(a[0]), len(a[0]), _ptr(a[1]), len(a[1]),
+                                           _ptr(a[2]), len(a[2]), _ptr(a[3]), len(a[3])))
+
+    def SetSkyData(self, pixels, width, height, worldToLight=None):
+        p = np.ascontiguousarray(pixels, np.float32)
+        m = None if worldToLight is None else np.ascontiguousarray(worldToLight, np.float32).reshape(16)
+        _check(library().lh2hip_set_sky(self._h, _ptr(p), width, height, _ptr(m)))
+
+    def SetProbePos(self, x, y):
+        _check(library().lh2hip_set_probe_pos(self._h, x, y))
+
+    def Render(self, view, converge=LH2_RESTART, async_=False):
+        v = np.ascontiguousarray(view, F.ViewPyramid)
+        _check(library().lh2hip_render(self._h, _ptr(v), converge, 1 if async_ else 0))
+
+    def WaitForRender(self):
+        _check(library().lh2hip_wait(self._h))
+
+    def GetCoreStats(self):
+        s = np.zeros(1, F.CoreStats)
+        _check(library().lh2hip_get_stats(self._h, _ptr(s)))
+        return s[0]
+
+    # ---- extensions -----------------------------------------------------------
+    def read_image(self):
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        _check(library().lh2hip_read_image(self._h, _ptr(out)))
+        return out
+
+    def read_accumulator_f64(self):
+        out = np.zeros((self.height, self.width, 4), np.float64)
+        _check(library().lh2hip_read_accumulator_f64(self._h, _ptr(out)))
+        return out
+
+    def trace_closest(self, rays_ptr, hits_ptr, count, stream=None):
+        """rays/hits: device pointers (e.g. torch tensor .data_ptr())."""
+        _check(library().lh2hip_trace_closest(self._h, rays_ptr, hits_ptr, count, stream))
+
+    def trace_closest_counted(self, rays_ptr, hits_ptr, count, stream=None):
+        """As trace_closest; returns (interior nodes visited, triangles tested)."""
+        out = np.zeros(2, np.uint64)
+        _check(library().lh2hip_trace_closest_counted(self._h, rays_ptr, hits_ptr, count, stream, _p(out)))
+        return int(out[0]), int(out[1])
+
+    def trace_any(self, rays_ptr, mask_ptr, count, stream=None):
+        _check(library().lh2hip_trace_any(self._h, rays_ptr, mask_ptr, count, stream))
+
+    def bounce_diffuse(self, rays_in, hits, path_in, count, R0, tmin, rays_out, path_out, count_out, stream=None):
+        _check(library().lh2hip_bounce_diffuse(self._h, rays_in, hits, path_in, count, R0, float(tmin),
+                                               rays_out, path_out, count_out, stream))
+
+    def bvh_info(self, mesh=0):
+        out = np.zeros(8, np.int64)
+        _check(library().lh2hip_bvh_info(self._h, mesh, _ptr(out)))
+        keys = ("build_nodes", "leaves", "depth", "nodes_f32", "nodes_q8", "depth_q8", "build_us", "triangles")
+        return dict(zip(keys, (int(x) for x in out)))
+
+    def bvh_download(self, mesh=0):
+        info = self.bvh_info(mesh)
+        nodes = np.zeros(info["build_nodes"], F.BuildNode)
+        prims = np.zeros(info["triangles"], np.uint32)
+        cnt = ctypes.c_int(0)
+        _check(library().lh2hip_bvh_download(self._h, mesh, _ptr(nodes), len(nodes), _ptr(prims), len(prims),
+                                             ctypes.byref(cnt)))
+        return nodes, prims
+
+
+def generate_primary(view, width, height, R0, tmin, rays_ptr, path_ptr=None, stream=None):
+    v = np.ascontiguousarray(view, F.ViewPyramid)
+    _check(library().lh2hip_generate_primary(_ptr(v), width, height, R0, float(tmin), rays_ptr, path_ptr, stream))
+
+
+def load_scene(core, scene):
+    """RenderSystem::SynchronizeSceneData call order: materials, geometry per
+    mesh, lights, instances (+ truncation), FinalizeInstances."""
+    core.SetMaterials(scene.materials if len(scene.materials) else F.new_materials(1))
+    for i, (v, t) in enumerate(scene.meshes):
+        core.SetGeometry(i, v, t)
+    core.SetLights(scene.tri_lights, scene.point_lights, scene.spot_lights, scene.dir_lights)
+    for i, (m, T) in enumerate(scene.instances):
+        core.SetInstance(i, m, T)
+    core.SetInstance(len(scene.instances), -1)
+    core.FinalizeInstances()

@@ -1,0 +1,94 @@
+/* bvh.h - acceleration-structure formats of RenderCore_HIP.
+ *
+ * Replaces the closed OptiX Prime / OptiX 7 acceleration structures
+ * (lib/RenderCore_PrimeRef/core_mesh.cpp:53-64, rendercore.cpp:509-529;
+ * lib/rendercore_optix7/core_mesh.cpp:67-129, rendercore.cpp:767-797) with a
+ * two-level BVH: one BLAS per mesh over the SetGeometry vertex soup (primitive
+ * id = soup index / 3) and a TLAS over instance boxes (instance id = position in
+ * the SetInstance list).
+ *
+ * Build format (GPU builder output): breadth-first BuildNode array, root 0,
+ * children allocated in pairs. Traversal formats (both 64 bytes per node):
+ *   NodeF32  binary node holding both child boxes in fp32;
+ *   NodeQ8   4-wide node, child boxes quantised to 8 bits per plane against
+ *            the node box (lo rounded down, hi rounded up: conservative).
+ * Child refs: bit 31 = leaf, low 30 bits = node index or first leaf triangle.
+ * Leaf triangles: 3 float4 (vertex 0..2); v0.w = primitive id | bit 31 on the
+ * last triangle of a leaf.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <vector>
+
+namespace lh {
+
+struct BuildNode { float lo[3]; int32_t leftFirst; float hi[3]; int32_t count; }; /* count 0: interior */
+
+struct NodeF32
+{
+	float4 lxy;   /* left  lo.x, hi.x, lo.y, hi.y */
+	float4 rxy;   /* right lo.x, hi.x, lo.y, hi.y */
+	float4 lrz;   /* left lo.z, hi.z, right lo.z, hi.z */
+	uint4 refs;   /* x = left ref, y = right ref */
+};
+
+struct NodeQ8
+{
+	float4 origin;    /* xyz = quantisation origin; w = bits: ex | ey << 8 | ez << 16 | nchild << 24 */
+	uint4 qa;         /* lo.x[4], lo.y[4], lo.z[4], hi.x[4] (one byte per child) */
+	uint4 qb;         /* hi.y[4], hi.z[4], unused, unused */
+	uint4 refs;       /* child refs */
+};
+
+static const uint32_t REF_LEAF = 0x80000000u;
+static const uint32_t REF_TLAS = 0x40000000u;
+static const uint32_t REF_MASK = 0x3fffffffu;
+static const uint32_t TRI_LAST = 0x80000000u;
+
+/* Per-instance record for the two-level traversal (64 bytes). */
+struct InstanceRec
+{
+	float4 inv0, inv1, inv2;   /* rows of the inverse transform */
+	const void* nodes;         /* BLAS node array (NodeF32 or NodeQ8) */
+	const float4* tris;        /* BLAS leaf triangles */
+	uint32_t root;             /* BLAS root ref */
+	uint32_t flags;            /* bit 0: identity transform */
+};
+
+/* Result of a GPU build: device arrays (owned by the caller). */
+struct DeviceBuild
+{
+	BuildNode* nodes = nullptr;
+	uint32_t* prims = nullptr;
+	int nodeCount = 0, primCount = 0, depth = 0;
+};
+
+/* Binned SAH (16 bins) over primitive boxes lo[i].xyz / hi[i].xyz on the GPU.
+ * maxLeaf: largest SAH leaf; maxDegenerateLeaf: largest leaf when all
+ * centroids coincide (beyond it the range is halved). Returns hipSuccess or an error. */
+hipError_t gpu_build(const float4* dLo, const float4* dHi, int n, int maxLeaf, int maxDegenerateLeaf,
+	DeviceBuild& out, hipStream_t stream);
+void free_build(DeviceBuild& b);
+
+/* Triangle soup (3 float4 per triangle) -> primitive boxes. */
+hipError_t tri_bounds(const float4* dVerts, int n, float4* dLo, float4* dHi, hipStream_t stream);
+
+/* Host-side layout of a downloaded build tree into the traversal formats.
+ * order[i] = primitive stored in leaf triangle slot i (depth-first leaf order);
+ * last[i] = 1 on the final slot of each leaf. */
+struct Layout
+{
+	std::vector<NodeF32> f32;
+	std::vector<NodeQ8> q8;
+	std::vector<uint32_t> order;
+	std::vector<uint8_t> last;
+	uint32_t rootF32 = 0, rootQ8 = 0;
+	int depthF32 = 0, depthQ8 = 0, stackF32 = 0, stackQ8 = 0;
+};
+void layout_bvh(const std::vector<BuildNode>& nodes, const std::vector<uint32_t>& prims, bool wantQ8, Layout& out);
+
+/* Gathers leaf triangles: tris[3*i+k] = verts[3*order[i]+k], v0.w = prim | last bit. */
+hipError_t gather_tris(const float4* dVerts, const uint32_t* dOrder, const uint8_t* dLast, int n, float4* dTris, hipStream_t stream);
+
+} // namespace lh

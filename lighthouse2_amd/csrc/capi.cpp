@@ -1,0 +1,217 @@
+/* capi.cpp - C ABI of librendercore_hip.so and the CreateCore plugin export.
+ * Every entry point catches the core's exceptions and returns a status code;
+ * the message is kept per thread for lh2hip_last_error(). */
+#include "../../include/lh2hip.h"
+#include "rendercore.h"
+#include <cstring>
+#include <new>
+
+using lh::RenderCore;
+using lh::CoreError;
+
+struct lh2hip_core { RenderCore core; };
+
+static thread_local std::string g_lastError;
+
+static int fail(int code, const char* msg)
+{
+	g_lastError = msg;
+	return code;
+}
+
+template <typename F>
+static int guarded(lh2hip_core* c, F f)
+{
+	if (!c) return fail(LH2HIP_ERR_INVALID, "null core handle");
+	try
+	{
+		f(c->core);
+		return LH2HIP_OK;
+	}
+	catch (const CoreError& e) { return fail(e.code, e.what()); }
+	catch (const std::bad_alloc&) { return fail(LH2HIP_ERR_OOM, "host allocation failed"); }
+	catch (const std::exception& e) { return fail(LH2HIP_ERR_DEVICE, e.what()); }
+}
+
+extern "C" {
+
+LH2HIP_API const char* lh2hip_last_error(void) { return g_lastError.c_str(); }
+LH2HIP_API int lh2hip_version(void) { return 1; }
+
+LH2HIP_API int lh2hip_create(lh2hip_core** out)
+{
+	if (!out) return fail(LH2HIP_ERR_INVALID, "null output pointer");
+	*out = nullptr;
+	lh2hip_core* c = nullptr;
+	try
+	{
+		c = new lh2hip_core();
+		c->core.Init();
+	}
+	catch (const CoreError& e) { delete c; return fail(e.code, e.what()); }
+	catch (const std::exception& e) { delete c; return fail(LH2HIP_ERR_DEVICE, e.what()); }
+	*out = c;
+	return LH2HIP_OK;
+}
+
+LH2HIP_API int lh2hip_destroy(lh2hip_core* core)
+{
+	if (!core) return LH2HIP_OK;
+	int r = guarded(core, [](RenderCore& c) { c.Shutdown(); });
+	delete core;
+	return r;
+}
+
+LH2HIP_API int lh2hip_set_target(lh2hip_core* core, int w, int h, int spp)
+{
+	return guarded(core, [&](RenderCore& c) { c.SetTargetSize(w, h, spp); });
+}
+
+LH2HIP_API int lh2hip_setting(lh2hip_core* core, const char* name, float value)
+{
+	return guarded(core, [&](RenderCore& c) { c.Setting(name, value); });
+}
+
+LH2HIP_API int lh2hip_set_geometry(lh2hip_core* core, int meshIdx, const lh2_float4* v, int vc, int tc, const CoreTri* tris)
+{
+	return guarded(core, [&](RenderCore& c) { c.SetGeometry(meshIdx, v, vc, tc, tris); });
+}
+
+LH2HIP_API int lh2hip_set_instance(lh2hip_core* core, int instanceIdx, int meshIdx, const lh2_mat4* transform)
+{
+	return guarded(core, [&](RenderCore& c) {
+		lh2_mat4 I = { { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 } };
+		c.SetInstance(instanceIdx, meshIdx, transform ? *transform : I);
+	});
+}
+
+LH2HIP_API int lh2hip_finalize_instances(lh2hip_core* core)
+{
+	return guarded(core, [&](RenderCore& c) { c.FinalizeInstances(); });
+}
+
+LH2HIP_API int lh2hip_set_materials(lh2hip_core* core, const CoreMaterial* mats, int count)
+{
+	return guarded(core, [&](RenderCore& c) { c.SetMaterials(const_cast<CoreMaterial*>(mats), count); });
+}
+
+LH2HIP_API int lh2hip_set_lights(lh2hip_core* core, const CoreLightTri* t, int nt, const CorePointLight* p, int np,
+	const CoreSpotLight* s, int ns, const CoreDirectionalLight* d, int nd)
+{
+	return guarded(core, [&](RenderCore& c) { c.SetLights(t, nt, p, np, s, ns, d, nd); });
+}
+
+LH2HIP_API int lh2hip_set_textures(lh2hip_core* core, const CoreTexDesc* tex, int count)
+{
+	return guarded(core, [&](RenderCore& c) { c.SetTextures(tex, count); });
+}
+
+LH2HIP_API int lh2hip_set_sky(lh2hip_core* core, const lh2_float3* pixels, int w, int h, const lh2_mat4* worldToLight)
+{
+	return guarded(core, [&](RenderCore& c) {
+		lh2_mat4 I = { { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 } };
+		if (w < 0 || h < 0) throw CoreError(LH2HIP_ERR_INVALID, "set_sky: negative size");
+		c.SetSkyData(pixels, (uint32_t)w, (uint32_t)h, worldToLight ? *worldToLight : I);
+	});
+}
+
+LH2HIP_API int lh2hip_set_probe_pos(lh2hip_core* core, int x, int y)
+{
+	return guarded(core, [&](RenderCore& c) { lh2_int2 p; p.x = x, p.y = y; c.SetProbePos(p); });
+}
+
+LH2HIP_API int lh2hip_render(lh2hip_core* core, const ViewPyramid* view, int converge, int async)
+{
+	return guarded(core, [&](RenderCore& c) {
+		if (!view) throw CoreError(LH2HIP_ERR_INVALID, "render: null view");
+		c.Render(*view, converge == LH2_RESTART ? lighthouse2::Restart : lighthouse2::Converge, async != 0);
+	});
+}
+
+LH2HIP_API int lh2hip_wait(lh2hip_core* core)
+{
+	return guarded(core, [&](RenderCore& c) { c.WaitForRender(); });
+}
+
+LH2HIP_API int lh2hip_read_image(lh2hip_core* core, lh2_float4* out)
+{
+	return guarded(core, [&](RenderCore& c) { c.ReadImage(out); });
+}
+
+LH2HIP_API int lh2hip_read_accumulator_f64(lh2hip_core* core, double* out)
+{
+	return guarded(core, [&](RenderCore& c) { c.ReadAccumulatorF64(out); });
+}
+
+LH2HIP_API int lh2hip_get_stats(lh2hip_core* core, CoreStats* out)
+{
+	return guarded(core, [&](RenderCore& c) {
+		if (!out) throw CoreError(LH2HIP_ERR_INVALID, "get_stats: null");
+		*out = c.GetCoreStats();
+	});
+}
+
+LH2HIP_API int lh2hip_trace_closest(lh2hip_core* core, const Ray4* rays, HitRecord* hits, int count, void* stream)
+{
+	return guarded(core, [&](RenderCore& c) { c.TraceClosest(rays, hits, count, (hipStream_t)stream); });
+}
+
+LH2HIP_API int lh2hip_trace_closest_counted(lh2hip_core* core, const Ray4* rays, HitRecord* hits, int count, void* stream, uint64_t* out2)
+{
+	return guarded(core, [&](RenderCore& c) {
+		if (!out2) throw CoreError(LH2HIP_ERR_INVALID, "trace_closest_counted: null output");
+		c.TraceClosestCounted(rays, hits, count, (hipStream_t)stream, out2);
+	});
+}
+
+LH2HIP_API int lh2hip_trace_any(lh2hip_core* core, const Ray4* rays, uint32_t* occluded, int count, void* stream)
+{
+	return guarded(core, [&](RenderCore& c) { c.TraceAny(rays, occluded, count, (hipStream_t)stream); });
+}
+
+LH2HIP_API int lh2hip_generate_primary(const ViewPyramid* view, int w, int h, uint32_t R0, float tmin, Ray4* raysOut,
+	uint32_t* pathIdOut, void* stream)
+{
+	if (!view || !raysOut || w <= 0 || h <= 0) return fail(LH2HIP_ERR_INVALID, "generate_primary: bad arguments");
+	lh::CamParams c;
+	memcpy(c.pos, &view->pos, 12), memcpy(c.p1, &view->p1, 12), memcpy(c.p2, &view->p2, 12), memcpy(c.p3, &view->p3, 12);
+	c.aperture = view->aperture, c.spreadAngle = view->spreadAngle, c.w = w, c.h = h;
+	const hipError_t e = lh::launch_primary(c, R0, tmin, (float4*)raysOut, pathIdOut, (hipStream_t)stream);
+	if (e != hipSuccess) return fail(LH2HIP_ERR_DEVICE, hipGetErrorString(e));
+	return LH2HIP_OK;
+}
+
+LH2HIP_API int lh2hip_bounce_diffuse(lh2hip_core* core, const Ray4* raysIn, const HitRecord* hits, const uint32_t* pathIdIn,
+	int count, uint32_t R0, float tmin, Ray4* raysOut, uint32_t* pathIdOut, uint32_t* countOut, void* stream)
+{
+	return guarded(core, [&](RenderCore& c) {
+		if (count < 0 || (count > 0 && (!raysIn || !hits || !pathIdIn || !raysOut || !pathIdOut)) || !countOut)
+			throw CoreError(LH2HIP_ERR_INVALID, "bounce_diffuse: bad arguments");
+		c.Bounce(raysIn, hits, pathIdIn, count, R0, tmin, raysOut, pathIdOut, countOut, (hipStream_t)stream);
+	});
+}
+
+LH2HIP_API int lh2hip_bvh_info(lh2hip_core* core, int meshIdx, int64_t* out8)
+{
+	return guarded(core, [&](RenderCore& c) {
+		if (!out8) throw CoreError(LH2HIP_ERR_INVALID, "bvh_info: null");
+		c.BvhInfo(meshIdx, out8);
+	});
+}
+
+LH2HIP_API int lh2hip_bvh_download(lh2hip_core* core, int meshIdx, void* nodesOut, int maxNodes, uint32_t* primsOut, int maxPrims, int* nodeCount)
+{
+	return guarded(core, [&](RenderCore& c) {
+		const int n = c.BvhDownload(meshIdx, nodesOut, maxNodes, primsOut, maxPrims);
+		if (nodeCount) *nodeCount = n;
+	});
+}
+
+} /* extern "C" */
+
+/* Plugin entry point: RenderSystem dlopens the core and calls Init itself
+ * (lib/RenderSystem/core_api_base.cpp:119-130). */
+extern "C" __attribute__((visibility("default"))) lighthouse2::CoreAPI_Base* CreateCore()
+{
+	return new RenderCore();
+}

@@ -1,0 +1,659 @@
+/* rendercore.cpp - RenderCore_HIP host core.
+ *
+ * Host orchestration of lib/RenderCore_PrimeRef/rendercore.cpp (Init :100-142,
+ * SetTarget :148-208, SetGeometry :214-222, SetInstance :228-242,
+ * FinalizeInstances :248-277, SetMaterials :360-402, SetLights :408-460,
+ * Setting :484-502, Render :535-632) on HIP: BVHs are built on the GPU per
+ * SetGeometry, the TLAS is rebuilt when instances change, and a frame is
+ * enqueued without host round trips: every wave's ray count stays in device
+ * memory and the traversal and shade kernels read it there (the reference
+ * copies the counters to the host after every bounce, rendercore.cpp:592).
+ */
+#include "rendercore.h"
+#include "../../include/lh2hip.h"
+#include <chrono>
+#include <cstring>
+#include <cmath>
+#include <algorithm>
+
+namespace lh {
+
+using lighthouse2::CoreTri;
+using lighthouse2::CoreMaterial;
+using lighthouse2::CoreTexDesc;
+using lighthouse2::CoreLightTri;
+using lighthouse2::CorePointLight;
+using lighthouse2::CoreSpotLight;
+using lighthouse2::CoreDirectionalLight;
+using lighthouse2::ViewPyramid;
+using lighthouse2::GLTexture;
+using lighthouse2::Convergence;
+using lighthouse2::Restart;
+using lighthouse2::Converge;
+using lighthouse2::int2;
+using lighthouse2::mat4;
+
+static inline void hipCheck(hipError_t e, const char* what)
+{
+	if (e == hipErrorOutOfMemory) throw CoreError(LH2HIP_ERR_OOM, std::string(what) + ": out of device memory");
+	if (e != hipSuccess) throw CoreError(LH2HIP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DevBuf::release()
+{
+	if (p) (void)hipFree(p);
+	p = nullptr, bytes = 0;
+}
+
+void DevBuf::ensure(size_t n)
+{
+	if (n <= bytes && p) return;
+	release();
+	if (n == 0) return;
+	hipCheck(hipMalloc(&p, n), "hipMalloc");
+	bytes = n;
+}
+
+void DevBuf::upload(const void* src, size_t n, hipStream_t s)
+{
+	ensure(n);
+	if (n) hipCheck(hipMemcpyAsync(p, src, n, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+}
+
+/* Round-to-nearest-even float -> half (the reference stores material colours
+ * as CUDA half, core_settings.h:112). */
+static uint16_t to_half(float f)
+{
+	uint32_t x;
+	memcpy(&x, &f, 4);
+	const uint32_t sign = (x >> 16) & 0x8000u;
+	x &= 0x7fffffffu;
+	if (x >= 0x7f800000u) return (uint16_t)(sign | (x > 0x7f800000u ? 0x7e00u : 0x7c00u));
+	if (x >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);
+	if (x < 0x38800000u)
+	{
+		if (x < 0x33000000u) return (uint16_t)sign;
+		const uint32_t e = x >> 23, m = (x & 0x7fffffu) | 0x800000u, shift = 126u - e;
+		uint32_t h = m >> shift;
+		const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+		if (rem > half || (rem == half && (h & 1u))) h++;
+		return (uint16_t)(sign | h);
+	}
+	uint32_t h = (x >> 13) - (112u << 10);
+	const uint32_t rem = x & 0x1fffu;
+	if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+	return (uint16_t)(sign | h);
+}
+
+/* TOCHAR (rendercore.cpp:358), saturated to one byte. */
+static inline uint32_t to_byte(float a)
+{
+	const float s = a * 255.0f;
+	if (!(s > 0.0f)) return 0;
+	if (s >= 255.0f) return 255;
+	return (uint32_t)s;
+}
+
+static uint32_t xor32_host(uint32_t& s) { s ^= s << 13; s ^= s >> 17; s ^= s << 5; return s; } /* RandomUInt(seed), system.cpp:51 */
+
+void Wavefront::resizeTarget(int pixels, hipStream_t s)
+{
+	if (pixels <= pixelCap) return;
+	pixelCap = pixels + (pixels >> 4);
+	acc32B.ensure(sizeof(float4) * pixelCap);
+	acc64B.ensure(sizeof(double) * 4 * pixelCap);
+	imageB.ensure(sizeof(float4) * pixelCap);
+	probeB.ensure(16);
+	countLogB.ensure(sizeof(uint32_t) * 1100);
+	hipCheck(hipMemsetAsync(imageB.p, 0, imageB.bytes, s), "memset");
+	hipCheck(hipMemsetAsync(probeB.p, 0xff, 16, s), "memset");
+}
+
+void Wavefront::clearAccumulators(hipStream_t s)
+{
+	if (acc32B.p) hipCheck(hipMemsetAsync(acc32B.p, 0, acc32B.bytes, s), "memset");
+	if (acc64B.p) hipCheck(hipMemsetAsync(acc64B.p, 0, acc64B.bytes, s), "memset");
+}
+
+void Wavefront::resizePaths(size_t n, hipStream_t s)
+{
+	if (n <= pathCap) return;
+	pathCap = n;
+	for (int i = 0; i < 2; i++)
+		rays[i].ensure(32 * n), state[i].ensure(16 * n), pathIdB[i].ensure(4 * n);
+	hitsB.ensure(16 * n), shadowRaysB.ensure(32 * n), shadowPotB.ensure(16 * n), occludedB.ensure(4 * n), pathAccB.ensure(16 * n);
+	countersB.ensure(64);
+	hipCheck(hipMemsetAsync(countersB.p, 0, 64, s), "memset");
+}
+
+void Wavefront::release()
+{
+	for (int i = 0; i < 2; i++) rays[i].release(), state[i].release(), pathIdB[i].release();
+	hitsB.release(), shadowRaysB.release(), shadowPotB.release(), occludedB.release(), pathAccB.release();
+	countersB.release(), countLogB.release(), probeB.release(), acc32B.release(), acc64B.release(), imageB.release();
+	pathCap = 0, pixelCap = 0;
+}
+
+PathBuffers Wavefront::buffers()
+{
+	PathBuffers b;
+	for (int i = 0; i < 2; i++) b.rays[i] = rays[i].as<float4>(), b.state[i] = state[i].as<float4>(), b.pathId[i] = pathIdB[i].as<uint32_t>();
+	b.hits = hitsB.as<float4>(), b.shadowRays = shadowRaysB.as<float4>(), b.shadowPot = shadowPotB.as<float4>();
+	b.occluded = occludedB.as<uint32_t>(), b.pathAcc = pathAccB.as<float4>(), b.counters = countersB.as<uint32_t>();
+	return b;
+}
+
+RenderCore::RenderCore() {}
+RenderCore::~RenderCore() { Shutdown(); }
+
+void RenderCore::Init()
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw CoreError(LH2HIP_ERR_MISSING, "no HIP device");
+	int dev = 0;
+	hipCheck(hipGetDevice(&dev), "hipGetDevice");
+	hipDeviceProp_t prop;
+	hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+	deviceCUs = prop.multiProcessorCount;
+	stats.SMcount = (uint32_t)prop.multiProcessorCount;
+	stats.ccMajor = (uint32_t)prop.major, stats.ccMinor = (uint32_t)prop.minor;
+	stats.VRAM = (uint32_t)(prop.totalGlobalMem >> 20);
+	static char name[256];
+	strncpy(name, prop.name, 255);
+	stats.deviceName = name;
+	stats.probedTriid = -1;
+	hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+	for (int i = 0; i < 8; i++) hipCheck(hipEventCreate(&ev[i]), "hipEventCreate");
+	persistentBlocks = deviceCUs * 8;
+	counters.ensure(64 * sizeof(uint32_t));
+	hipCheck(hipMemsetAsync(counters.p, 0, counters.bytes, stream), "memset");
+	overflow.ensure(trace_overflow_bytes(persistentBlocks));
+}
+
+void RenderCore::Shutdown()
+{
+	if (!stream) return;
+	(void)hipStreamSynchronize(stream);
+	meshes.clear();
+	tlasNodes.release(), instRecs.release(), shadeInsts.release(), overflow.release(), counters.release();
+	materials.release(), triLights.release(), pointLights.release(), spotLights.release(), dirLights.release();
+	wf.release();
+	for (int i = 0; i < 8; i++) (void)hipEventDestroy(ev[i]);
+	(void)hipStreamDestroy(stream);
+	stream = nullptr;
+}
+
+void RenderCore::SetProbePos(const int2 pos) { probePos = pos; }
+
+void RenderCore::SetTarget(GLTexture* target, const uint32_t spp)
+{
+	if (!target) throw CoreError(LH2HIP_ERR_INVALID, "SetTarget: null target");
+	SetTargetSize((int)target->width, (int)target->height, (int)spp);
+}
+
+void RenderCore::SetTargetSize(int w, int h, int spp)
+{
+	if (w <= 0 || h <= 0 || spp <= 0 || spp > 65535 || (int64_t)w * h > (1 << 28))
+		throw CoreError(LH2HIP_ERR_INVALID, "SetTarget: bad size");
+	scrw = w, scrh = h, scrspp = spp;
+	wf.resizeTarget(w * h, stream);
+	wf.clearAccumulators(stream);
+	samplesTaken = 0;
+}
+
+void RenderCore::Setting(const char* name, float value)
+{
+	if (!name) throw CoreError(LH2HIP_ERR_INVALID, "Setting: null name");
+	if (!strcmp(name, "epsilon")) geometryEpsilon = value;
+	else if (!strcmp(name, "clampValue")) clampValue = value;
+	else if (!strcmp(name, "maxDepth"))
+	{
+		if (!(value >= 1 && value <= 1024)) throw CoreError(LH2HIP_ERR_INVALID, "Setting maxDepth: 1..1024");
+		maxDepth = (int)value;
+	}
+	else if (!strcmp(name, "nodeFormat"))
+	{
+		if (value != 0 && value != 1) throw CoreError(LH2HIP_ERR_INVALID, "Setting nodeFormat: 0 (fp32) or 1 (q8)");
+		nodeFormat = (int)value;
+		instancesDirty = true;
+	}
+	else if (!strcmp(name, "seed")) seedSetting = (uint32_t)value;
+	/* unknown keys are ignored, as in the reference cores */
+}
+
+void RenderCore::buildMesh(MeshData& m)
+{
+	const auto t0 = std::chrono::high_resolution_clock::now();
+	const int n = m.triCount;
+	DevBuf lo, hi;
+	lo.ensure(sizeof(float4) * n), hi.ensure(sizeof(float4) * n);
+	hipCheck(tri_bounds(m.verts.as<float4>(), n, lo.as<float4>(), hi.as<float4>(), stream), "tri_bounds");
+	DeviceBuild b;
+	hipCheck(gpu_build(lo.as<float4>(), hi.as<float4>(), n, 4, 255, b, stream), "gpu_build");
+	m.buildNodes.resize(b.nodeCount);
+	m.buildPrims.resize(n);
+	hipCheck(hipMemcpyAsync(m.buildNodes.data(), b.nodes, sizeof(BuildNode) * b.nodeCount, hipMemcpyDeviceToHost, stream), "D2H nodes");
+	hipCheck(hipMemcpyAsync(m.buildPrims.data(), b.prims, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream), "D2H prims");
+	hipCheck(hipStreamSynchronize(stream), "sync");
+	m.buildDepth = b.depth;
+	free_build(b);
+	Layout L;
+	layout_bvh(m.buildNodes, m.buildPrims, true, L);
+	m.rootF32 = L.rootF32, m.rootQ8 = L.rootQ8;
+	m.depthF32 = L.depthF32, m.depthQ8 = L.depthQ8, m.stackF32 = L.stackF32, m.stackQ8 = L.stackQ8;
+	m.nodeCountF32 = L.f32.size(), m.nodeCountQ8 = L.q8.size();
+	m.leafCount = 0;
+	for (uint8_t v : L.last) m.leafCount += v;
+	for (int c = 0; c < 3; c++) m.lo[c] = m.buildNodes[0].lo[c], m.hi[c] = m.buildNodes[0].hi[c];
+	m.nodesF32.upload(L.f32.data(), L.f32.size() * sizeof(NodeF32), stream);
+	m.nodesQ8.upload(L.q8.data(), L.q8.size() * sizeof(NodeQ8), stream);
+	DevBuf order, last;
+	order.upload(L.order.data(), L.order.size() * 4, stream);
+	last.upload(L.last.data(), L.last.size(), stream);
+	m.tris.ensure(sizeof(float4) * 3 * (size_t)n);
+	hipCheck(gather_tris(m.verts.as<float4>(), order.as<uint32_t>(), last.as<uint8_t>(), n, m.tris.as<float4>(), stream), "gather_tris");
+	hipCheck(hipStreamSynchronize(stream), "sync");
+	const auto t1 = std::chrono::high_resolution_clock::now();
+	m.buildMs = std::chrono::duration<float, std::milli>(t1 - t0).count();
+	stats.bvhBuildTime = m.buildMs * 0.001f;
+}
+
+void RenderCore::SetGeometry(const int meshIdx, const lighthouse2::float4* vertexData, const int vertexCount, const int triangleCount, const CoreTri* triangles)
+{
+	if (meshIdx < 0 || meshIdx > (int)meshes.size()) throw CoreError(LH2HIP_ERR_INVALID, "SetGeometry: meshes must be added in sequential order");
+	if (triangleCount < 0 || vertexCount != 3 * triangleCount || (triangleCount > 0 && !vertexData))
+		throw CoreError(LH2HIP_ERR_INVALID, "SetGeometry: vertexCount must be 3 * triangleCount");
+	if (triangleCount >= (1 << 29)) throw CoreError(LH2HIP_ERR_INVALID, "SetGeometry: too many triangles");
+	if (meshIdx == (int)meshes.size()) meshes.emplace_back();
+	MeshData& m = meshes[meshIdx];
+	m.triCount = triangleCount;
+	m.verts.upload(vertexData, sizeof(float4) * 3 * (size_t)triangleCount, stream);
+	m.maxMaterial = -1;
+	if (triangles)
+	{
+		m.coreTris.upload(triangles, sizeof(CoreTri) * (size_t)triangleCount, stream);
+		for (int i = 0; i < triangleCount; i++) m.maxMaterial = std::max(m.maxMaterial, (int)std::min<uint32_t>(triangles[i].material, 0x7fffffffu));
+	}
+	else m.coreTris.release();
+	if (triangleCount > 0) buildMesh(m);
+	else m.buildNodes.clear(), m.buildPrims.clear(), m.nodeCountF32 = m.nodeCountQ8 = 0;
+	instancesDirty = true;
+}
+
+void RenderCore::SetInstance(const int instanceIdx, const int modelIdx, const mat4& transform)
+{
+	if (modelIdx == -1)
+	{
+		if ((int)instances.size() > instanceIdx && instanceIdx >= 0) instances.resize(instanceIdx);
+		instancesDirty = true;
+		return;
+	}
+	if (instanceIdx < 0 || instanceIdx > (int)instances.size()) throw CoreError(LH2HIP_ERR_INVALID, "SetInstance: instances must be added in sequential order");
+	if (modelIdx < 0 || modelIdx >= (int)meshes.size()) throw CoreError(LH2HIP_ERR_INVALID, "SetInstance: unknown mesh");
+	if (instanceIdx == (int)instances.size()) instances.emplace_back();
+	instances[instanceIdx].mesh = modelIdx;
+	memcpy(instances[instanceIdx].T, transform.cell, 64);
+	instancesDirty = true;
+}
+
+/* mat4::Inverted (common_types.h:559-606): MESA cofactors; singular -> identity. */
+static void invert(const float* c, float* out)
+{
+	float inv[16];
+	inv[0] = c[5] * c[10] * c[15] - c[5] * c[11] * c[14] - c[9] * c[6] * c[15] + c[9] * c[7] * c[14] + c[13] * c[6] * c[11] - c[13] * c[7] * c[10];
+	inv[1] = -c[1] * c[10] * c[15] + c[1] * c[11] * c[14] + c[9] * c[2] * c[15] - c[9] * c[3] * c[14] - c[13] * c[2] * c[11] + c[13] * c[3] * c[10];
+	inv[2] = c[1] * c[6] * c[15] - c[1] * c[7] * c[14] - c[5] * c[2] * c[15] + c[5] * c[3] * c[14] + c[13] * c[2] * c[7] - c[13] * c[3] * c[6];
+	inv[3] = -c[1] * c[6] * c[11] + c[1] * c[7] * c[10] + c[5] * c[2] * c[11] - c[5] * c[3] * c[10] - c[9] * c[2] * c[7] + c[9] * c[3] * c[6];
+	inv[4] = -c[4] * c[10] * c[15] + c[4] * c[11] * c[14] + c[8] * c[6] * c[15] - c[8] * c[7] * c[14] - c[12] * c[6] * c[11] + c[12] * c[7] * c[10];
+	inv[5] = c[0] * c[10] * c[15] - c[0] * c[11] * c[14] - c[8] * c[2] * c[15] + c[8] * c[3] * c[14] + c[12] * c[2] * c[11] - c[12] * c[3] * c[10];
+	inv[6] = -c[0] * c[6] * c[15] + c[0] * c[7] * c[14] + c[4] * c[2] * c[15] - c[4] * c[3] * c[14] - c[12] * c[2] * c[7] + c[12] * c[3] * c[6];
+	inv[7] = c[0] * c[6] * c[11] - c[0] * c[7] * c[10] - c[4] * c[2] * c[11] + c[4] * c[3] * c[10] + c[8] * c[2] * c[7] - c[8] * c[3] * c[6];
+	inv[8] = c[4] * c[9] * c[15] - c[4] * c[11] * c[13] - c[8] * c[5] * c[15] + c[8] * c[7] * c[13] + c[12] * c[5] * c[11] - c[12] * c[7] * c[9];
+	inv[9] = -c[0] * c[9] * c[15] + c[0] * c[11] * c[13] + c[8] * c[1] * c[15] - c[8] * c[3] * c[13] - c[12] * c[1] * c[11] + c[12] * c[3] * c[9];
+	inv[10] = c[0] * c[5] * c[15] - c[0] * c[7] * c[13] - c[4] * c[1] * c[15] + c[4] * c[3] * c[13] + c[12] * c[1] * c[7] - c[12] * c[3] * c[5];
+	inv[11] = -c[0] * c[5] * c[11] + c[0] * c[7] * c[9] + c[4] * c[1] * c[11] - c[4] * c[3] * c[9] - c[8] * c[1] * c[7] + c[8] * c[3] * c[5];
+	inv[12] = -c[4] * c[9] * c[14] + c[4] * c[10] * c[13] + c[8] * c[5] * c[14] - c[8] * c[6] * c[13] - c[12] * c[5] * c[10] + c[12] * c[6] * c[9];
+	inv[13] = c[0] * c[9] * c[14] - c[0] * c[10] * c[13] - c[8] * c[1] * c[14] + c[8] * c[2] * c[13] + c[12] * c[1] * c[10] - c[12] * c[2] * c[9];
+	inv[14] = -c[0] * c[5] * c[14] + c[0] * c[6] * c[13] + c[4] * c[1] * c[14] - c[4] * c[2] * c[13] - c[12] * c[1] * c[6] + c[12] * c[2] * c[5];
+	inv[15] = c[0] * c[5] * c[10] - c[0] * c[6] * c[9] - c[4] * c[1] * c[10] + c[4] * c[2] * c[9] + c[8] * c[1] * c[6] - c[8] * c[2] * c[5];
+	const float det = c[0] * inv[0] + c[1] * inv[4] + c[2] * inv[8] + c[3] * inv[12];
+	for (int i = 0; i < 16; i++) out[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+	if (det != 0)
+	{
+		const float invdet = 1.0f / det;
+		for (int i = 0; i < 16; i++) out[i] = inv[i] * invdet;
+	}
+}
+
+void RenderCore::FinalizeInstances()
+{
+	instancesDirty = true;
+	updateTopLevel();
+}
+
+/* TLAS over instance world boxes: 8 corners of the mesh box through the
+ * instance transform (an identity instance keeps the mesh box), built with the
+ * GPU builder, one instance per leaf. */
+void RenderCore::updateTopLevel()
+{
+	if (!instancesDirty) return;
+	const int ni = (int)instances.size();
+	std::vector<InstanceRec> recs(ni);
+	std::vector<ShadeInst> sh(ni);
+	std::vector<float4> blo, bhi;
+	std::vector<int> live;
+	int maxStack = 0;
+	for (int i = 0; i < ni; i++)
+	{
+		const InstanceData& I = instances[i];
+		const MeshData& m = meshes[I.mesh];
+		float inv[16];
+		invert(I.T, inv);
+		bool identity = true;
+		for (int k = 0; k < 16; k++) if (I.T[k] != ((k % 5 == 0) ? 1.0f : 0.0f)) identity = false;
+		InstanceRec& r = recs[i];
+		r.inv0 = make_float4(inv[0], inv[1], inv[2], inv[3]);
+		r.inv1 = make_float4(inv[4], inv[5], inv[6], inv[7]);
+		r.inv2 = make_float4(inv[8], inv[9], inv[10], inv[11]);
+		r.nodes = nodeFormat == FMT_Q8 ? m.nodesQ8.p : m.nodesF32.p;
+		r.tris = m.tris.as<float4>();
+		r.root = nodeFormat == FMT_Q8 ? m.rootQ8 : m.rootF32;
+		r.flags = identity ? 1u : 0u;
+		ShadeInst& s = sh[i];
+		s.inv0 = r.inv0, s.inv1 = r.inv1, s.inv2 = r.inv2;
+		s.coreTris = m.coreTris.as<float4>();
+		s.verts = m.verts.as<float4>();
+		s.mesh = I.mesh;
+		if (m.triCount == 0) continue;
+		maxStack = std::max(maxStack, nodeFormat == FMT_Q8 ? m.stackQ8 : m.stackF32);
+		float lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+		if (identity) for (int c = 0; c < 3; c++) lo[c] = m.lo[c], hi[c] = m.hi[c];
+		else
+			for (int c = 0; c < 8; c++)
+			{
+				const float p[3] = { (c & 1) ? m.hi[0] : m.lo[0], (c & 2) ? m.hi[1] : m.lo[1], (c & 4) ? m.hi[2] : m.lo[2] };
+				for (int a = 0; a < 3; a++)
+				{
+					const float w = I.T[a * 4] * p[0] + I.T[a * 4 + 1] * p[1] + I.T[a * 4 + 2] * p[2] + I.T[a * 4 + 3];
+					lo[a] = std::min(lo[a], w), hi[a] = std::max(hi[a], w);
+				}
+			}
+		blo.push_back(make_float4(lo[0], lo[1], lo[2], 0)), bhi.push_back(make_float4(hi[0], hi[1], hi[2], 0));
+		live.push_back(i);
+	}
+	instRecs.upload(recs.data(), recs.size() * sizeof(InstanceRec), stream);
+	shadeInsts.upload(sh.data(), sh.size() * sizeof(ShadeInst), stream);
+	scene = TraceScene();
+	scene.format = nodeFormat;
+	scene.inst = instRecs.as<InstanceRec>();
+	sceneReady = !live.empty();
+	if (live.size() == 1 && recs[live[0]].flags == 1u && live[0] == 0)
+	{	/* one identity instance: traverse its BLAS directly */
+		scene.twoLevel = 0;
+		scene.nodes = recs[0].nodes, scene.tris = recs[0].tris, scene.root = recs[0].root;
+		scene.maxStack = maxStack;
+	}
+	else if (!live.empty())
+	{
+		const int n = (int)live.size();
+		DevBuf dlo, dhi;
+		dlo.upload(blo.data(), n * sizeof(float4), stream);
+		dhi.upload(bhi.data(), n * sizeof(float4), stream);
+		DeviceBuild b;
+		hipCheck(gpu_build(dlo.as<float4>(), dhi.as<float4>(), n, 1, 1, b, stream), "tlas build");
+		std::vector<BuildNode> nodes(b.nodeCount);
+		std::vector<uint32_t> prims(n);
+		hipCheck(hipMemcpyAsync(nodes.data(), b.nodes, sizeof(BuildNode) * b.nodeCount, hipMemcpyDeviceToHost, stream), "D2H");
+		hipCheck(hipMemcpyAsync(prims.data(), b.prims, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream), "D2H");
+		hipCheck(hipStreamSynchronize(stream), "sync");
+		free_build(b);
+		for (auto& p : prims) p = (uint32_t)live[p]; /* TLAS primitive -> instance id */
+		Layout L;
+		layout_bvh(nodes, prims, false, L);
+		/* one instance per leaf: point leaf refs straight at the instance id */
+		auto fix = [&](uint32_t r) { return (r & REF_LEAF) ? (REF_LEAF | L.order[r & REF_MASK]) : r; };
+		for (auto& nd : L.f32) nd.refs.x = fix(nd.refs.x), nd.refs.y = fix(nd.refs.y);
+		tlasNodes.upload(L.f32.data(), std::max<size_t>(1, L.f32.size()) * sizeof(NodeF32), stream);
+		scene.twoLevel = 1;
+		scene.tlas = tlasNodes.as<NodeF32>();
+		scene.tlasRoot = fix(L.rootF32);
+		scene.maxStack = maxStack + L.stackF32;
+	}
+	if (scene.maxStack > TRACE_MAX_STACK) throw CoreError(LH2HIP_ERR_INVALID, "BVH too deep for the traversal stack");
+	hipCheck(hipStreamSynchronize(stream), "sync");
+	instancesDirty = false;
+}
+
+const TraceScene& RenderCore::Scene()
+{
+	updateTopLevel();
+	if (!sceneReady) throw CoreError(LH2HIP_ERR_STATE, "no scene: call SetGeometry, SetInstance and FinalizeInstances first");
+	return scene;
+}
+
+void RenderCore::TraceClosest(const void* rays, void* hits, int count, hipStream_t s)
+{
+	if (count < 0 || (count > 0 && (!rays || !hits))) throw CoreError(LH2HIP_ERR_INVALID, "trace: bad arguments");
+	const TraceScene& sc = Scene();
+	hipCheck(launch_trace_dev(sc, MODE_CLOSEST, (const float4*)rays, hits, count, nullptr, counters.as<uint32_t>() + 16, overflow.as<uint32_t>(), persistentBlocks, s ? s : 0), "trace");
+}
+
+void RenderCore::TraceClosestCounted(const void* rays, void* hits, int count, hipStream_t s, uint64_t* out2)
+{
+	if (count < 0 || (count > 0 && (!rays || !hits))) throw CoreError(LH2HIP_ERR_INVALID, "trace: bad arguments");
+	const TraceScene& sc = Scene();
+	DevBuf st;
+	st.ensure(sizeof(TraceStats));
+	hipCheck(hipMemsetAsync(st.p, 0, sizeof(TraceStats), s), "memset");
+	hipCheck(launch_trace_dev(sc, MODE_CLOSEST, (const float4*)rays, hits, count, nullptr, counters.as<uint32_t>() + 16,
+		overflow.as<uint32_t>(), persistentBlocks, s, st.as<TraceStats>()), "trace");
+	TraceStats h;
+	hipCheck(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s), "D2H");
+	hipCheck(hipStreamSynchronize(s), "sync");
+	out2[0] += h.nodes, out2[1] += h.tris;
+}
+
+void RenderCore::TraceAny(const void* rays, uint32_t* mask, int count, hipStream_t s)
+{
+	if (count < 0 || (count > 0 && (!rays || !mask))) throw CoreError(LH2HIP_ERR_INVALID, "trace: bad arguments");
+	const TraceScene& sc = Scene();
+	hipCheck(hipMemsetAsync(mask, 0, sizeof(uint32_t) * ((count + 31) / 32), s), "memset");
+	hipCheck(launch_trace_dev(sc, MODE_ANY_MASK, (const float4*)rays, mask, count, nullptr, counters.as<uint32_t>() + 16, overflow.as<uint32_t>(), persistentBlocks, s ? s : 0), "trace");
+}
+
+void RenderCore::Bounce(const void* raysIn, const void* hits, const uint32_t* pathIn, int count, uint32_t R0, float tmin,
+	void* raysOut, uint32_t* pathOut, uint32_t* countOut, hipStream_t s)
+{
+	Scene();
+	hipCheck(launch_bounce(shadeInsts.as<ShadeInst>(), (const float4*)raysIn, (const float4*)hits, pathIn, count, R0, tmin,
+		(float4*)raysOut, pathOut, countOut, s), "bounce");
+}
+
+void RenderCore::BvhInfo(int mesh, int64_t* o) const
+{
+	if (mesh < 0 || mesh >= (int)meshes.size()) throw CoreError(LH2HIP_ERR_INVALID, "bvh_info: unknown mesh");
+	const MeshData& m = meshes[mesh];
+	o[0] = (int64_t)m.buildNodes.size(), o[1] = (int64_t)m.leafCount, o[2] = m.buildDepth, o[3] = (int64_t)m.nodeCountF32;
+	o[4] = (int64_t)m.nodeCountQ8, o[5] = m.depthQ8, o[6] = (int64_t)(m.buildMs * 1000.0f), o[7] = m.triCount;
+}
+
+int RenderCore::BvhDownload(int mesh, void* nodesOut, int maxNodes, uint32_t* primsOut, int maxPrims) const
+{
+	if (mesh < 0 || mesh >= (int)meshes.size()) throw CoreError(LH2HIP_ERR_INVALID, "bvh_download: unknown mesh");
+	const MeshData& m = meshes[mesh];
+	if (nodesOut && maxNodes >= (int)m.buildNodes.size()) memcpy(nodesOut, m.buildNodes.data(), m.buildNodes.size() * sizeof(BuildNode));
+	if (primsOut && maxPrims >= (int)m.buildPrims.size()) memcpy(primsOut, m.buildPrims.data(), m.buildPrims.size() * 4);
+	return (int)m.buildNodes.size();
+}
+
+void RenderCore::SetMaterials(CoreMaterial* mat, const int materialCount)
+{
+	if (materialCount < 0 || (materialCount > 0 && !mat)) throw CoreError(LH2HIP_ERR_INVALID, "SetMaterials: bad arguments");
+	std::vector<GpuMaterial> g(std::max(1, materialCount));
+	memset(g.data(), 0, g.size() * sizeof(GpuMaterial));
+	for (int i = 0; i < materialCount; i++)
+	{
+		const CoreMaterial& m = mat[i];
+		GpuMaterial& d = g[i];
+		const uint32_t dr = to_half(m.color.value.x), dg = to_half(m.color.value.y), db = to_half(m.color.value.z);
+		const uint32_t tr = to_half(1.0f - m.absorption.value.x), tg = to_half(1.0f - m.absorption.value.y), tb = to_half(1.0f - m.absorption.value.z);
+		d.diffuse_rg = dr | (dg << 16);
+		d.diffuse_b_medium_r = db | (tr << 16);
+		d.medium_gb = tg | (tb << 16);
+		d.params.x = to_byte(m.metallic.value) + (to_byte(m.subsurface.value) << 8) + (to_byte(m.specular.value) << 16) + (to_byte(m.roughness.value) << 24);
+		d.params.y = to_byte(m.specularTint.value) + (to_byte(m.anisotropic.value) << 8) + (to_byte(m.sheen.value) << 16) + (to_byte(m.sheenTint.value) << 24);
+		d.params.z = to_byte(m.clearcoat.value) + (to_byte(m.clearcoatGloss.value) << 8) + (to_byte(m.transmission.value) << 16);
+		memcpy(&d.params.w, &m.eta.value, 4);
+		d.flags = (m.eta.value < 1 ? MF_ISDIELECTRIC : 0) + (m.color.textureID != -1 ? MF_HASDIFFUSEMAP : 0) +
+			(m.normals.textureID != -1 ? MF_HASNORMALMAP : 0) + (m.specular.textureID != -1 ? MF_HASSPECULARITYMAP : 0) +
+			(m.roughness.textureID != -1 ? MF_HASROUGHNESSMAP : 0) + (m.detailNormals.textureID != -1 ? MF_HAS2NDNORMALMAP : 0) +
+			(m.detailColor.textureID != -1 ? MF_HAS2NDDIFFUSEMAP : 0) + ((m.flags & 1) ? MF_HASSMOOTHNORMALS : 0) + ((m.flags & 2) ? MF_HASALPHA : 0);
+	}
+	materials.upload(g.data(), g.size() * sizeof(GpuMaterial), stream);
+	this->materialCount = materialCount;
+}
+
+void RenderCore::SetLights(const CoreLightTri* tl, const int nt, const CorePointLight* pl, const int np,
+	const CoreSpotLight* sl, const int ns, const CoreDirectionalLight* dl, const int nd)
+{
+	if (nt < 0 || np < 0 || ns < 0 || nd < 0) throw CoreError(LH2HIP_ERR_INVALID, "SetLights: negative count");
+	if (nt) triLights.upload(tl, nt * sizeof(CoreLightTri), stream);
+	if (np) pointLights.upload(pl, np * sizeof(CorePointLight), stream);
+	if (ns) spotLights.upload(sl, ns * sizeof(CoreSpotLight), stream);
+	if (nd) dirLights.upload(dl, nd * sizeof(CoreDirectionalLight), stream);
+	lightCounts[0] = nt, lightCounts[1] = np, lightCounts[2] = ns, lightCounts[3] = nd;
+}
+
+void RenderCore::SetTextures(const CoreTexDesc* tex, const int textureCount)
+{
+	if (textureCount < 0 || (textureCount > 0 && !tex)) throw CoreError(LH2HIP_ERR_INVALID, "SetTextures: bad arguments");
+	(void)tex; /* texture maps are not sampled by the reference estimator on the parity scenes */
+	stats.argb32TexelCount = 0;
+	for (int i = 0; i < textureCount; i++) if (tex[i].storage == LH2_ARGB32) stats.argb32TexelCount += tex[i].pixelCount;
+}
+
+void RenderCore::SetSkyData(const lighthouse2::float3* pixels, const uint32_t width, const uint32_t height, const mat4& worldToLight)
+{
+	(void)pixels, (void)width, (void)height, (void)worldToLight; /* black sky on a miss */
+}
+
+void RenderCore::Render(const ViewPyramid& view, const Convergence converge, bool async)
+{
+	if (scrw == 0) throw CoreError(LH2HIP_ERR_STATE, "Render: SetTarget first");
+	updateTopLevel();
+	if (!sceneReady) return; /* no geometry yet: nothing to render (rendercore_optix7/rendercore.cpp:821) */
+	for (const InstanceData& I : instances)
+	{
+		const MeshData& m = meshes[I.mesh];
+		if (m.triCount == 0) continue;
+		if (m.maxMaterial < 0) throw CoreError(LH2HIP_ERR_STATE, "Render: a mesh was set without CoreTri data (traversal-only geometry)");
+		if (m.maxMaterial >= materialCount) throw CoreError(LH2HIP_ERR_STATE, "Render: a triangle references a material beyond SetMaterials");
+	}
+	const auto t0 = std::chrono::high_resolution_clock::now();
+	if (converge == Restart || firstConvergingFrame)
+	{
+		wf.clearAccumulators(stream);
+		samplesTaken = 0;
+		firstConvergingFrame = true;
+		camSeed = seedSetting;
+	}
+	if (converge == Converge) firstConvergingFrame = false;
+	const uint32_t R0gen = xor32_host(camSeed);
+	CamParams cam;
+	memcpy(cam.pos, &view.pos, 12), memcpy(cam.p1, &view.p1, 12), memcpy(cam.p2, &view.p2, 12), memcpy(cam.p3, &view.p3, 12);
+	cam.aperture = view.aperture, cam.spreadAngle = view.spreadAngle, cam.w = scrw, cam.h = scrh;
+	ShadeParams P;
+	memset(&P, 0, sizeof(P));
+	P.inst = shadeInsts.as<ShadeInst>();
+	P.mats = materials.as<GpuMaterial>();
+	P.triLights = triLights.as<float4>(), P.pointLights = pointLights.as<float4>();
+	P.spotLights = spotLights.as<float4>(), P.dirLights = dirLights.as<float4>();
+	P.nTri = lightCounts[0], P.nPoint = lightCounts[1], P.nSpot = lightCounts[2], P.nDir = lightCounts[3];
+	P.w = scrw, P.h = scrh, P.maxDepth = maxDepth, P.pass = samplesTaken, P.eps = geometryEpsilon;
+	P.probePixel = probePos.x + scrw * probePos.y;
+	const int pixels = scrw * scrh;
+	const int passSamples = std::max(1, std::min(scrspp, (int)(Wavefront::PASS_CAP / pixels)));
+	wf.resizePaths((size_t)passSamples * pixels, stream);
+	P.probe = wf.probe();
+	const TraceScene& sc = Scene();
+	float traceMs[3] = { 0, 0, 0 }, shadowMs = 0, shadeMs = 0;
+	uint32_t rayCounts[3] = { 0, 0, 0 };
+	for (int s0 = 0; s0 < scrspp; s0 += passSamples)
+	{
+		const int k = std::min(passSamples, scrspp - s0);
+		const int n = k * pixels;
+		PathBuffers B = wf.buffers();
+		P.jobBase = (uint32_t)s0 * (uint32_t)pixels;
+		hipCheck(launch_generate(cam, R0gen, P.jobBase, n, geometryEpsilon, B, 0, stream), "generate");
+		int in = 0;
+		for (int pathLength = 1; pathLength <= maxDepth; pathLength++)
+		{
+			hipCheck(hipEventRecord(ev[0], stream), "event");
+			hipCheck(launch_trace_dev(sc, MODE_CLOSEST, B.rays[in], B.hits, n, B.counters + 0, B.counters + 3, overflow.as<uint32_t>(), persistentBlocks, stream), "extend");
+			hipCheck(hipEventRecord(ev[1], stream), "event");
+			P.pathLength = pathLength;
+			P.R0 = (uint32_t)samplesTaken * 7907u + (uint32_t)pathLength * 91771u;
+			hipCheck(launch_shade(P, B, in, n, stream), "shade");
+			hipCheck(hipEventRecord(ev[2], stream), "event");
+			if (pathLength < maxDepth)
+			{
+				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, B.counters + 2, B.counters + 4, overflow.as<uint32_t>(), persistentBlocks, stream), "connect");
+				hipCheck(launch_connect(B, n, stream), "connect");
+			}
+			hipCheck(hipEventRecord(ev[3], stream), "event");
+			hipCheck(launch_advance(B.counters, wf.countLog() + pathLength, stream), "advance");
+			in ^= 1;
+			/* stop early once every path has terminated (checked every 4 waves) */
+			if ((pathLength & 3) == 0 || pathLength == maxDepth)
+			{
+				uint32_t live = 0;
+				hipCheck(hipMemcpyAsync(&live, B.counters + 0, 4, hipMemcpyDeviceToHost, stream), "D2H");
+				hipCheck(hipStreamSynchronize(stream), "sync");
+				float a, b2, c;
+				(void)hipEventElapsedTime(&a, ev[0], ev[1]), (void)hipEventElapsedTime(&b2, ev[1], ev[2]), (void)hipEventElapsedTime(&c, ev[2], ev[3]);
+				traceMs[std::min(pathLength - 1, 2)] += a, shadeMs += b2, shadowMs += c;
+				if (live == 0) break;
+			}
+		}
+		hipCheck(launch_resolve(B.pathAcc, pixels, k, wf.acc32(), wf.acc64(), stream), "resolve");
+	}
+	samplesTaken += scrspp;
+	hipCheck(launch_finalize(wf.acc32(), pixels, 1.0f / (float)samplesTaken, wf.image(), stream), "finalize");
+	if (!async)
+	{
+		hipCheck(hipStreamSynchronize(stream), "sync");
+		int probe[3];
+		hipCheck(hipMemcpy(probe, wf.probe(), 12, hipMemcpyDeviceToHost), "probe");
+		stats.probedInstid = probe[0], stats.probedTriid = probe[1];
+		memcpy(&stats.probedDist, &probe[2], 4);
+	}
+	const auto t1 = std::chrono::high_resolution_clock::now();
+	stats.renderTime = std::chrono::duration<float>(t1 - t0).count();
+	stats.traceTime0 = traceMs[0] * 1e-3f, stats.traceTime1 = traceMs[1] * 1e-3f, stats.traceTimeX = traceMs[2] * 1e-3f;
+	stats.shadeTime = shadeMs * 1e-3f, stats.shadowTraceTime = shadowMs * 1e-3f;
+	stats.primaryRayCount = (uint32_t)(pixels * scrspp);
+	(void)rayCounts;
+}
+
+void RenderCore::WaitForRender()
+{
+	if (stream) hipCheck(hipStreamSynchronize(stream), "sync");
+}
+
+void RenderCore::ReadImage(void* hostOut)
+{
+	if (!hostOut) throw CoreError(LH2HIP_ERR_INVALID, "read_image: null");
+	if (scrw == 0) throw CoreError(LH2HIP_ERR_STATE, "read_image: no target");
+	hipCheck(hipMemcpyAsync(hostOut, wf.image(), sizeof(float4) * scrw * scrh, hipMemcpyDeviceToHost, stream), "D2H");
+	hipCheck(hipStreamSynchronize(stream), "sync");
+}
+
+void RenderCore::ReadAccumulatorF64(double* hostOut)
+{
+	if (!hostOut) throw CoreError(LH2HIP_ERR_INVALID, "read_accumulator: null");
+	if (scrw == 0) throw CoreError(LH2HIP_ERR_STATE, "read_accumulator: no target");
+	hipCheck(hipMemcpyAsync(hostOut, wf.acc64(), sizeof(double) * 4 * scrw * scrh, hipMemcpyDeviceToHost, stream), "D2H");
+	hipCheck(hipStreamSynchronize(stream), "sync");
+}
+
+} // namespace lh

@@ -1,0 +1,151 @@
+/* rendercore.h - RenderCore_HIP host core: a CoreAPI_Base implementation that
+ * keeps the scene in HBM, builds the BVHs on the GPU and drives the wavefront
+ * kernels. Mirrors lib/RenderCore_PrimeRef/rendercore.h (state per core, one
+ * device, single-threaded API) with headless output. */
+#pragma once
+#include "../../include/core_api_base.h"
+#include "bvh.h"
+#include "trace.h"
+#include "waves.h"
+#include "shade.h"
+#include <string>
+#include <vector>
+#include <stdexcept>
+
+namespace lh {
+
+/* Raised by every failing call; the C ABI turns it into a status code. */
+struct CoreError : std::runtime_error
+{
+	int code;
+	CoreError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+/* Owning device allocation (CoreBuffer<T> ON_DEVICE, lib/CUDA/shared_host_code/cudatools.h:196-327). */
+struct DevBuf
+{
+	void* p = nullptr;
+	size_t bytes = 0;
+	DevBuf() = default;
+	DevBuf(const DevBuf&) = delete;
+	DevBuf& operator=(const DevBuf&) = delete;
+	DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr, o.bytes = 0; }
+	DevBuf& operator=(DevBuf&& o) noexcept { release(); p = o.p, bytes = o.bytes; o.p = nullptr, o.bytes = 0; return *this; }
+	~DevBuf() { release(); }
+	void release();
+	void ensure(size_t n);             /* grow-only, contents undefined after growth */
+	void upload(const void* src, size_t n, hipStream_t s);
+	template <typename T> T* as() const { return (T*)p; }
+};
+
+/* Path buffers of the wavefront loop (RenderCore_PrimeRef/rendercore.cpp:148-208
+ * sizes them maxPixels * spp; here one pass holds at most PASS_CAP paths and a
+ * frame with more samples runs several passes). */
+struct Wavefront
+{
+	static constexpr size_t PASS_CAP = (size_t)1 << 24;
+	DevBuf rays[2], state[2], pathIdB[2], hitsB, shadowRaysB, shadowPotB, occludedB, pathAccB, countersB, countLogB, probeB;
+	DevBuf acc32B, acc64B, imageB;
+	size_t pathCap = 0;
+	int pixelCap = 0;
+	void resizeTarget(int pixels, hipStream_t s);
+	void clearAccumulators(hipStream_t s);
+	void resizePaths(size_t n, hipStream_t s);
+	void release();
+	PathBuffers buffers();
+	int* probe() { return probeB.as<int>(); }
+	float4* acc32() { return acc32B.as<float4>(); }
+	double* acc64() { return acc64B.as<double>(); }
+	float4* image() { return imageB.as<float4>(); }
+	uint32_t* countLog() { return countLogB.as<uint32_t>(); }
+};
+
+struct MeshData
+{
+	int triCount = 0;
+	DevBuf verts, coreTris, nodesF32, nodesQ8, tris;
+	uint32_t rootF32 = 0, rootQ8 = 0;
+	int depthF32 = 0, depthQ8 = 0, stackF32 = 0, stackQ8 = 0, buildDepth = 0;
+	size_t nodeCountF32 = 0, nodeCountQ8 = 0, leafCount = 0;
+	float lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
+	std::vector<BuildNode> buildNodes;   /* canonical build tree (host copy) */
+	std::vector<uint32_t> buildPrims;
+	float buildMs = 0;
+	int maxMaterial = -1;                /* largest CoreTri::material, -1 without CoreTri data */
+};
+
+struct InstanceData { int mesh; float T[16]; };
+
+class RenderCore : public lighthouse2::CoreAPI_Base
+{
+public:
+	RenderCore();
+	~RenderCore();
+	/* CoreAPI_Base */
+	lighthouse2::CoreStats GetCoreStats() const override { return stats; }
+	void Init() override;
+	void SetProbePos(const lighthouse2::int2 pos) override;
+	void SetTarget(lighthouse2::GLTexture* target, const uint32_t spp) override;
+	void Setting(const char* name, float value) override;
+	void Render(const lighthouse2::ViewPyramid& view, const lighthouse2::Convergence converge, bool async) override;
+	void WaitForRender() override;
+	void Shutdown() override;
+	void SetTextures(const lighthouse2::CoreTexDesc* tex, const int textureCount) override;
+	void SetMaterials(lighthouse2::CoreMaterial* mat, const int materialCount) override;
+	void SetLights(const lighthouse2::CoreLightTri* triLights, const int triLightCount,
+		const lighthouse2::CorePointLight* pointLights, const int pointLightCount,
+		const lighthouse2::CoreSpotLight* spotLights, const int spotLightCount,
+		const lighthouse2::CoreDirectionalLight* directionalLights, const int directionalLightCount) override;
+	void SetSkyData(const lighthouse2::float3* pixels, const uint32_t width, const uint32_t height, const lighthouse2::mat4& worldToLight) override;
+	void SetGeometry(const int meshIdx, const lighthouse2::float4* vertexData, const int vertexCount, const int triangleCount, const lighthouse2::CoreTri* triangles) override;
+	void SetInstance(const int instanceIdx, const int modelIdx, const lighthouse2::mat4& transform) override;
+	void FinalizeInstances() override;
+
+	/* C-ABI extensions */
+	void SetTargetSize(int w, int h, int spp);
+	void TraceClosest(const void* rays, void* hits, int count, hipStream_t s);
+	void TraceAny(const void* rays, uint32_t* mask, int count, hipStream_t s);
+	void TraceClosestCounted(const void* rays, void* hits, int count, hipStream_t s, uint64_t* out2);
+	void Bounce(const void* raysIn, const void* hits, const uint32_t* pathIn, int count, uint32_t R0, float tmin,
+		void* raysOut, uint32_t* pathOut, uint32_t* countOut, hipStream_t s);
+	void ReadImage(void* hostOut);
+	void ReadAccumulatorF64(double* hostOut);
+	void BvhInfo(int mesh, int64_t* out8) const;
+	int BvhDownload(int mesh, void* nodesOut, int maxNodes, uint32_t* primsOut, int maxPrims) const;
+	const TraceScene& Scene();
+
+	hipStream_t stream = nullptr;
+	int deviceCUs = 0;
+
+private:
+	void buildMesh(MeshData& m);
+	void ensureTraceBuffers();
+	void updateTopLevel();
+
+	lighthouse2::CoreStats stats = {};
+	std::vector<MeshData> meshes;
+	std::vector<InstanceData> instances;
+	bool instancesDirty = true, sceneReady = false;
+	/* top level */
+	DevBuf tlasNodes, instRecs, shadeInsts;
+	TraceScene scene;
+	/* trace scratch */
+	DevBuf overflow, counters;
+	int persistentBlocks = 0;
+	/* settings (RenderCore_PrimeRef rendercore.cpp:484-502 + extensions) */
+	float geometryEpsilon = 1.0e-3f, clampValue = 10.0f;
+	int maxDepth = 64, nodeFormat = FMT_Q8;
+	uint32_t camSeed = 0x12345678u, seedSetting = 0x12345678u;
+	/* render state */
+	int scrw = 0, scrh = 0, scrspp = 1;
+	int samplesTaken = 0;
+	bool firstConvergingFrame = true;
+	lighthouse2::int2 probePos = { 0, 0 };
+	DevBuf materials, triLights, pointLights, spotLights, dirLights;
+	int lightCounts[4] = { 0, 0, 0, 0 };
+	int materialCount = 0;
+	Wavefront wf;
+	hipEvent_t ev[8];
+};
+
+} // namespace lh

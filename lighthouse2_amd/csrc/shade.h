@@ -1,0 +1,73 @@
+/* shade.h - wavefront path-tracing kernels (generate / shade / connect /
+ * resolve / finalize) of RenderCore_HIP. */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "waves.h"
+
+namespace lh {
+
+/* Device material, the 128-byte CUDAMaterial of the reference
+ * (lib/RenderCore_PrimeRef/core_settings.h:87-136): half diffuse/transmittance,
+ * flags, 16 8-bit Disney parameters + eta, six texture-map descriptors. */
+struct GpuMaterial
+{
+	uint32_t diffuse_rg, diffuse_b_medium_r, medium_gb, flags;
+	uint4 params;
+	uint4 t0, t1, n0, n1, s, r;
+};
+
+enum MaterialFlags
+{
+	MF_ISDIELECTRIC = 1, MF_DIFFUSEMAPISHDR = 2, MF_HASDIFFUSEMAP = 4, MF_HASNORMALMAP = 8, MF_HASSPECULARITYMAP = 16,
+	MF_HASROUGHNESSMAP = 32, MF_ISANISOTROPIC = 64, MF_HAS2NDNORMALMAP = 128, MF_HAS2NDDIFFUSEMAP = 512,
+	MF_HASSMOOTHNORMALS = 2048, MF_HASALPHA = 4096
+};
+
+/* Per-wave constants of the shade kernel (replaces the reference's
+ * __constant__ globals, lib/RenderCore_PrimeRef/kernels/.cuda.cu:20-45, so
+ * several jobs can be in flight). */
+struct ShadeParams
+{
+	const ShadeInst* inst;
+	const GpuMaterial* mats;
+	const float4* triLights;      /* CoreLightTri, 6 float4 each */
+	const float4* pointLights;    /* CorePointLight, 2 float4 */
+	const float4* spotLights;     /* CoreSpotLight, 3 float4 */
+	const float4* dirLights;      /* CoreDirectionalLight, 2 float4 */
+	int nTri, nPoint, nSpot, nDir;
+	int w, h;
+	uint32_t jobBase;             /* global job index of path slot 0 of this pass */
+	int pathLength, maxDepth;
+	uint32_t R0;
+	int pass;
+	float eps;
+	int probePixel;
+	int* probe;                   /* {inst, prim, t bits} */
+};
+
+/* Path-state buffers of one pass (SoA). */
+struct PathBuffers
+{
+	float4* rays[2];       /* Ray4 {O, tmin; D, tmax} */
+	float4* state[2];      /* throughput.xyz, flags bits */
+	uint32_t* pathId[2];   /* global job index */
+	float4* hits;
+	float4* shadowRays;
+	float4* shadowPot;     /* contribution.xyz, path slot bits */
+	uint32_t* occluded;
+	float4* pathAcc;       /* per path slot, summed in path order */
+	uint32_t* counters;    /* [0] extension count in, [1] extension out, [2] shadow, [3] trace work counter, [4] second trace counter */
+};
+
+hipError_t launch_generate(const CamParams& c, uint32_t R0, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s);
+hipError_t launch_shade(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s);
+hipError_t launch_connect(const PathBuffers& b, int maxCount, hipStream_t s);
+hipError_t launch_resolve(const float4* pathAcc, int pixels, int samples, float4* acc32, double* acc64, hipStream_t s);
+hipError_t launch_finalize(const float4* acc32, int pixels, float scale, float4* image, hipStream_t s);
+hipError_t launch_set_u32(uint32_t* p, int n, uint32_t v, hipStream_t s);
+/* counters[0] = counters[1] (next wave size, also logged), counters[1] = counters[2] = 0. */
+hipError_t launch_advance(uint32_t* counters, uint32_t* logSlot, hipStream_t s);
+hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t s);
+
+} // namespace lh

@@ -1,0 +1,490 @@
+/* shade.hip - wavefront kernels of the reference estimator on gfx950.
+ *
+ * Follows RenderCore_PrimeRef ("REFERENCE, for image validation"):
+ *   k_generate  generateEyeRaysKernel    lib/RenderCore_PrimeRef/kernels/camera.h:39-68
+ *   k_shade     shadeKernel              lib/RenderCore_PrimeRef/kernels/pathtracer.h:45-168
+ *               GetShadingData/SetupFrame lib/CUDA/shared_kernel_code/material_shared.h:42-128
+ *               RandomPointOnLight       lib/CUDA/shared_kernel_code/lights_shared.h:225-313
+ *               Lambert+specular BSDF    lib/RenderCore_PrimeRef/kernels/bsdf.h:19-101
+ *   k_connect   finalizeConnectionKernel lib/RenderCore_PrimeRef/kernels/connections.h:22-34
+ *   k_finalize  finalizeRenderKernel     lib/CUDA/shared_kernel_code/finalize_shared.h:29-45
+ * Differences of form: contributions go to a per-path slot (path slots are
+ * owned by one path, so no two lanes ever add to the same address) and
+ * k_resolve adds the slots to the pixel accumulators in sample order, giving
+ * a deterministic image for any spp; the path index has its own 32-bit lane
+ * (no 2^24 cap); light selection sums potentials in a first pass and
+ * recomputes them while searching (no 64-entry stack array).
+ */
+#include "shade.h"
+#include "sample.h"
+
+namespace lh {
+
+#define PI_F 3.14159265358979323846264f
+#define INVPI_F 0.31830988618379067153777f
+#define TWOPI_F 6.28318530717958647692528f
+#define EPSILON_F 0.0001f
+#define S_SPECULAR 1u
+
+struct Shading
+{
+	v3 color, transmittance;
+	uint4 params;
+};
+__device__ __forceinline__ float char2flt(uint32_t a, int s) { return ((float)((a >> s) & 255u)) * (1.0f / 255.0f); }
+__device__ __forceinline__ float roughness(const Shading& sd) { const float r = char2flt(sd.params.x, 24); return r > 0.001f ? r : 0.001f; }
+__device__ __forceinline__ float transmission(const Shading& sd) { return char2flt(sd.params.z, 16); }
+__device__ __forceinline__ float eta(const Shading& sd) { return bitsf(sd.params.w); }
+
+__device__ __forceinline__ v3 consistent_normal(v3 D, v3 iN, float alpha)
+{
+	const float t = PI_F - 2 * alpha, q = (t * t) / (PI_F * (PI_F + (2 * PI_F - 4) * alpha));
+	const float b = dot(D, iN), g = 1 + q * (b - 1), rho = sqrtf(q * (1 + g) / (1 + b));
+	const v3 Rc = (g + rho * b) * iN - (rho * D);
+	return normalize(D + Rc);
+}
+
+/* CoreTri as 13 float4 (common_classes.h:133-163). */
+__device__ void shading_data(const ShadeParams& P, v3 D, float u, float v, const float4* tri, int instIdx,
+	Shading& sd, v3& N, v3& iN, v3& fN, v3& T)
+{
+	const float4 t1 = tri[1], t2 = tri[2], t3 = tri[3], t4 = tri[4], t5 = tri[5], t7 = tri[7];
+	const GpuMaterial& mat = P.mats[fbits(t1.w)];
+	const uint32_t flags = mat.flags;
+	sd.color = mk3(half_to_float(mat.diffuse_rg & 0xffffu), half_to_float(mat.diffuse_rg >> 16), half_to_float(mat.diffuse_b_medium_r & 0xffffu));
+	sd.transmittance = mk3(half_to_float(mat.diffuse_b_medium_r >> 16), half_to_float(mat.medium_gb & 0xffffu), half_to_float(mat.medium_gb >> 16));
+	sd.params = mat.params;
+	N = iN = fN = mk3(t2.w, t3.w, t4.w);
+	T = xyz(t5);
+	const float w = 1 - (u + v);
+	if (flags & MF_HASSMOOTHNORMALS) iN = normalize(u * xyz(t2) + v * xyz(t3) + w * xyz(t4));
+	const ShadeInst& I = P.inst[instIdx];
+	const v3 A = xyz(I.inv0), B = xyz(I.inv1), C = xyz(I.inv2);
+	N = normalize(N.x * A + N.y * B + N.z * C);
+	iN = normalize(iN.x * A + iN.y * B + iN.z * C);
+	const bool backSide = dot(D, N) > 0;
+	const float alpha = u * t7.x + v * t7.y + w * t7.z;
+	iN = (backSide ? -1.0f : 1.0f) * consistent_normal(D * -1.0f, backSide ? (iN * -1.0f) : iN, alpha);
+	fN = iN;
+}
+
+__device__ v3 random_barycentrics(float r0)
+{
+	const uint32_t uf = (uint32_t)(uint64_t)(r0 * 4294967296.0f);
+	float Ax = 1, Ay = 0, Bx = 0, By = 1, Cx = 0, Cy = 0;
+	for (int i = 0; i < 16; ++i)
+	{
+		const int d = (uf >> (2 * (15 - i))) & 0x3;
+		float Anx, Any, Bnx, Bny, Cnx, Cny;
+		if (d == 0) { Anx = (Bx + Cx) * 0.5f, Any = (By + Cy) * 0.5f; Bnx = (Ax + Cx) * 0.5f, Bny = (Ay + Cy) * 0.5f; Cnx = (Ax + Bx) * 0.5f, Cny = (Ay + By) * 0.5f; }
+		else if (d == 1) { Anx = Ax, Any = Ay; Bnx = (Ax + Bx) * 0.5f, Bny = (Ay + By) * 0.5f; Cnx = (Ax + Cx) * 0.5f, Cny = (Ay + Cy) * 0.5f; }
+		else if (d == 2) { Anx = (Bx + Ax) * 0.5f, Any = (By + Ay) * 0.5f; Bnx = Bx, Bny = By; Cnx = (Bx + Cx) * 0.5f, Cny = (By + Cy) * 0.5f; }
+		else { Anx = (Cx + Ax) * 0.5f, Any = (Cy + Ay) * 0.5f; Bnx = (Cx + Bx) * 0.5f, Bny = (Cy + By) * 0.5f; Cnx = Cx, Cny = Cy; }
+		Ax = Anx, Ay = Any, Bx = Bnx, By = Bny, Cx = Cnx, Cy = Cny;
+	}
+	const float rx = (Ax + Bx + Cx) * 0.3333333f, ry = (Ay + By + Cy) * 0.3333333f;
+	return mk3(rx, ry, 1 - rx - ry);
+}
+
+__device__ __forceinline__ float potential(const ShadeParams& P, int i, v3 I, v3 N, v3 bary)
+{
+	if (i < P.nTri)
+	{
+		const float4* L = P.triLights + 6 * i;
+		const float4 d0 = L[0], d1 = L[1];
+		v3 Lv = I;
+		if (bary.x >= 0) Lv = bary.x * xyz(L[3]) + bary.y * xyz(L[4]) + bary.z * xyz(L[5]);
+		Lv = Lv - I;
+		const float att = 1.0f / dot(Lv, Lv);
+		Lv = normalize(Lv);
+		const float LNdotL = fmaxf(0.0f, -dot(xyz(d1), Lv));
+		const float NdotL = fmaxf(0.0f, dot(N, Lv));
+		return d0.w * LNdotL * NdotL * att;
+	}
+	i -= P.nTri;
+	if (i < P.nPoint)
+	{
+		const float4 d0 = P.pointLights[2 * i];
+		const v3 Lv = xyz(d0) - I;
+		const float NdotL = fmaxf(0.0f, dot(N, normalize(Lv)));
+		const float att = 1.0f / dot(Lv, Lv);
+		return d0.w * NdotL * att;
+	}
+	i -= P.nPoint;
+	if (i < P.nSpot)
+	{
+		const float4 d0 = P.spotLights[3 * i], d1 = P.spotLights[3 * i + 1], d2 = P.spotLights[3 * i + 2];
+		v3 Lv = xyz(d0) - I;
+		const float att = 1.0f / dot(Lv, Lv);
+		Lv = normalize(Lv);
+		const float d = (fmaxf(0.0f, -dot(Lv, xyz(d2))) - d1.w) / (d0.w - d1.w);
+		const float NdotL = fmaxf(0.0f, dot(N, Lv));
+		const float LNdotL = fmaxf(0.0f, fminf(1.0f, d));
+		return (d1.x + d1.y + d1.z) * LNdotL * NdotL * att;
+	}
+	i -= P.nSpot;
+	const float4 d0 = P.dirLights[2 * i];
+	const float LNdotL = fmaxf(0.0f, -(d0.x * N.x + d0.y * N.y + d0.z * N.z));
+	return d0.w * LNdotL;
+}
+
+__device__ v3 random_point_on_light(const ShadeParams& P, float r0, float r1, v3 I, v3 N, float& pickProb, float& lightPdf, v3& lightColor)
+{
+	const int lightCount = P.nTri + P.nPoint + P.nSpot + P.nDir;
+	const v3 bary = random_barycentrics(r0);
+	float sum = 0;
+	for (int i = 0; i < lightCount; i++) sum += potential(P, i, I, N, bary);
+	if (sum <= 0) { lightPdf = 0; return mk3(1, 1, 1); }
+	r1 *= sum;
+	int lightIdx = 0;
+	float total = 0, pl = 0;
+	for (int i = 0; i < lightCount; i++)
+	{
+		const float c = potential(P, i, I, N, bary);
+		total += c;
+		if (total >= r1) { lightIdx = i; pl = c; break; }
+	}
+	if (lightIdx == 0) pl = potential(P, 0, I, N, bary);
+	pickProb = pl / sum;
+	if (lightIdx < P.nTri)
+	{
+		const float4* L = P.triLights + 6 * lightIdx;
+		const float4 d1 = L[1];
+		lightColor = xyz(L[2]);
+		const v3 LN = xyz(d1);
+		const v3 Pp = bary.x * xyz(L[3]) + bary.y * xyz(L[4]) + bary.z * xyz(L[5]);
+		v3 Lv = I - Pp;
+		const float sqDist = dot(Lv, Lv);
+		Lv = normalize(Lv);
+		const float LNdotL = Lv.x * LN.x + Lv.y * LN.y + Lv.z * LN.z;
+		const float reciSolidAngle = sqDist / (d1.w * LNdotL);
+		lightPdf = (LNdotL > 0 && dot(Lv, N) < 0) ? reciSolidAngle : 0;
+		return Pp;
+	}
+	int k = lightIdx - P.nTri;
+	if (k < P.nPoint)
+	{
+		const float4 d0 = P.pointLights[2 * k], d1 = P.pointLights[2 * k + 1];
+		const v3 Pp = xyz(d0), Lv = Pp - I;
+		const float sqDist = dot(Lv, Lv);
+		lightColor = xyz(d1) / sqDist;
+		lightPdf = dot(Lv, N) > 0 ? 1 : 0;
+		return Pp;
+	}
+	k -= P.nPoint;
+	if (k < P.nSpot)
+	{
+		const float4 d0 = P.spotLights[3 * k], d1 = P.spotLights[3 * k + 1], d2 = P.spotLights[3 * k + 2];
+		const v3 Pp = xyz(d0);
+		v3 Lv = I - Pp;
+		const float sqDist = dot(Lv, Lv);
+		Lv = normalize(Lv);
+		const float d = (fmaxf(0.0f, Lv.x * d2.x + Lv.y * d2.y + Lv.z * d2.z) - d1.w) / (d0.w - d1.w);
+		const float LNdotL = fminf(1.0f, d);
+		lightPdf = (LNdotL > 0 && dot(Lv, N) < 0) ? (sqDist / LNdotL) : 0;
+		lightColor = xyz(d1);
+		return Pp;
+	}
+	k -= P.nSpot;
+	const float4 d0 = P.dirLights[2 * k], d1 = P.dirLights[2 * k + 1];
+	const v3 Ld = xyz(d0);
+	lightColor = xyz(d1);
+	const float NdotL = dot(Ld, N);
+	lightPdf = NdotL < 0 ? 1 : 0;
+	return I - 1000.0f * Ld;
+}
+
+__device__ __forceinline__ v3 reflect(v3 i, v3 n) { return i - 2.0f * n * dot(n, i); }
+
+__device__ float fr_l(float VDotN, float eio)
+{
+	const float SinThetaT2 = (eio * eio) * (1.0f - VDotN * VDotN);
+	if (SinThetaT2 > 1.0f) return 1.0f;
+	const float LDotN = sqrtf(1.0f - SinThetaT2);
+	const float r1 = (VDotN - eio * LDotN) / (VDotN + eio * LDotN);
+	const float r2 = (LDotN - eio * VDotN) / (LDotN + eio * VDotN);
+	return 0.5f * (r1 * r1 + r2 * r2);
+}
+
+__device__ bool refract_l(v3 wi, v3 n, float et, v3& wt)
+{
+	const float cosThetaI = fabsf(dot(n, wi));
+	const float sin2ThetaI = fmaxf(0.0f, 1.0f - cosThetaI * cosThetaI);
+	const float sin2ThetaT = et * et * sin2ThetaI;
+	if (sin2ThetaT >= 1) return false;
+	const float cosThetaT = sqrtf(1.0f - sin2ThetaT);
+	wt = et * (wi * -1.0f) + (et * cosThetaI - cosThetaT) * n;
+	return true;
+}
+
+__device__ __forceinline__ v3 evaluate_bsdf(const Shading& sd, v3 iN, v3 wi, float& pdf)
+{
+	if (transmission(sd) > 0.999f || roughness(sd) <= 0.001f) { pdf = 0; return mk3(0, 0, 0); }
+	pdf = fabsf(dot(wi, iN)) * INVPI_F;
+	return sd.color * INVPI_F * roughness(sd);
+}
+
+__device__ v3 sample_bsdf(const Shading& sd, v3 iN, v3 N, v3 wo, float distance, float r3, float r4, v3& wi, float& pdf, bool& specular)
+{
+	const float flip = (dot(wo, N) < 0) ? -1 : 1;
+	iN = iN * flip;
+	specular = true, pdf = 1;
+	v3 bsdf;
+	const float TR = transmission(sd), RO = roughness(sd);
+	if (r4 < TR)
+	{
+		const float eio = flip < 0 ? (1.0f / eta(sd)) : eta(sd), F = fr_l(dot(iN, wo), eio);
+		v3 beer = mk3(1, 1, 1);
+		beer.x = cw_expf(-sd.transmittance.x * distance * 2.0f);
+		beer.y = cw_expf(-sd.transmittance.y * distance * 2.0f);
+		beer.z = cw_expf(-sd.transmittance.z * distance * 2.0f);
+		if (r3 < F)
+		{
+			wi = reflect(wo * -1.0f, iN);
+			bsdf = sd.color * beer * (1 / fabsf(dot(iN, wi)));
+		}
+		else
+		{
+			if (!refract_l(wo, iN, eio, wi)) return mk3(0, 0, 0);
+			return sd.color * beer * (1 / fabsf(dot(iN, wi)));
+		}
+	}
+	else
+	{
+		const float pReflect = 1 - RO;
+		if (r3 < pReflect)
+		{
+			wi = reflect(wo * -1.0f, iN);
+			bsdf = sd.color * (1.0f / fabsf(dot(iN, wi)));
+		}
+		else
+		{
+			const float r5 = (r3 - pReflect) / (1 - pReflect);
+			const float r6 = (r4 - TR) / (1 - TR);
+			wi = normalize(tangent2world(cosine_sample(r5, r6), iN));
+			pdf = fmaxf(0.0f, dot(wi, iN)) * INVPI_F;
+			specular = false;
+			bsdf = sd.color * INVPI_F;
+		}
+	}
+	if (dot(N * flip, wi) <= 0) pdf = 0;
+	return bsdf;
+}
+
+__device__ __forceinline__ v3 safe_origin(v3 O, v3 R, v3 N, float eps) { return O + N * (dot(N, R) > 0 ? eps : -eps); }
+
+__device__ __forceinline__ void acc_add(float4* a, v3 c)
+{
+	float4 v = *a;
+	v.x = v.x + c.x, v.y = v.y + c.y, v.z = v.z + c.z;
+	*a = v;
+}
+
+__global__ void k_generate(CamParams c, uint32_t R0, uint32_t jobBase, int n, float eps, float4* __restrict__ rays,
+	float4* __restrict__ state, uint32_t* __restrict__ pathId, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j == 0) counters[0] = (uint32_t)n, counters[1] = 0, counters[2] = 0;
+	if (j >= n) return;
+	const uint32_t job = jobBase + (uint32_t)j;
+	v3 O, D;
+	eye_ray(c, (int)job, R0, O, D);
+	rays[2 * j] = make_float4(O.x, O.y, O.z, eps);
+	rays[2 * j + 1] = make_float4(D.x, D.y, D.z, 1e34f);
+	state[j] = make_float4(1.0f, 1.0f, 1.0f, bitsf(S_SPECULAR));
+	pathId[j] = job;
+	pathAcc[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ void __launch_bounds__(128) k_shade(const ShadeParams P, const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
+	const uint32_t* __restrict__ pathIn, const float4* __restrict__ hits, float4* __restrict__ raysOut, float4* __restrict__ stateOut,
+	uint32_t* __restrict__ pathOut, float4* __restrict__ shadowRays, float4* __restrict__ shadowPot, float4* __restrict__ pathAcc,
+	uint32_t* __restrict__ counters)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= (int)counters[0]) return;
+	const float4 O4 = raysIn[2 * j], D4 = raysIn[2 * j + 1], T4 = stateIn[j];
+	const float4 hd = hits[j];
+	const uint32_t pathIdx = pathIn[j];
+	const uint32_t slot = pathIdx - P.jobBase;
+	float4* acc = pathAcc + slot;
+	const int wh = P.w * P.h;
+	const uint32_t pixelIdx = pathIdx % (uint32_t)wh;
+	const uint32_t sampleIdx = pathIdx / (uint32_t)wh + (uint32_t)P.pass;
+	uint32_t flags = fbits(T4.w);
+	const v3 D = xyz(D4);
+	v3 throughput = xyz(T4);
+	const uint32_t prim = fbits(hd.z);
+	const int instIdx = (int)fbits(hd.y);
+	const float t = hd.w;
+	if (P.pathLength == 1) acc->w = acc->w + (prim == 0xffffffffu ? 10000.0f : t);
+	if (prim == 0xffffffffu)
+	{	/* no sky data: black */
+		acc_add(acc, throughput * mk3(0, 0, 0));
+		return;
+	}
+	if (P.probe && (int)pixelIdx == P.probePixel && P.pathLength == 1 && sampleIdx == 0)
+		P.probe[0] = instIdx, P.probe[1] = (int)prim, P.probe[2] = (int)fbits(t);
+	const uint32_t uvb = fbits(hd.x);
+	const float u = (float)(uvb & 65535u) * (1.0f / 65535.0f), v = (float)(uvb >> 16) * (1.0f / 65535.0f);
+	const ShadeInst& SI = P.inst[instIdx];
+	const float4* tri = SI.coreTris + 13 * (size_t)prim;
+	Shading sd;
+	v3 N, iN, fN, T;
+	const v3 I = xyz(O4) + t * D;
+	shading_data(P, D, u, v, tri, instIdx, sd, N, iN, fN, T);
+	if (sd.color.x > 1.0f || sd.color.y > 1.0f || sd.color.z > 1.0f)
+	{
+		const float DdotNL = -dot(D, N);
+		if (DdotNL > 0)
+		{
+			const v3 c = throughput * sd.color;
+			if (P.pathLength == 1 || (flags & S_SPECULAR)) acc_add(acc, c);
+		}
+		return;
+	}
+	if (roughness(sd) <= 0.001f || transmission(sd) > 0.999f) flags |= S_SPECULAR; else flags &= ~S_SPECULAR;
+	uint32_t seed = wang_hash(pathIdx * 17u + P.R0);
+	const float faceDir = (dot(D, N) > 0) ? -1 : 1;
+	if (faceDir == 1) sd.transmittance = mk3(0, 0, 0);
+	if ((flags & S_SPECULAR) == 0)
+	{
+		const float r0 = rand_float(seed), r1 = rand_float(seed);
+		float pickProb = 0, lightPdf = 0;
+		v3 lightColor = mk3(0, 0, 0);
+		v3 L = random_point_on_light(P, r0, r1, I, fN * faceDir, pickProb, lightPdf, lightColor) - I;
+		const float dist = length(L);
+		L = L * (1.0f / dist);
+		const float NdotL = dot(L, fN * faceDir);
+		if (NdotL > 0 && lightPdf > 0 && P.pathLength < P.maxDepth)
+		{
+			float bsdfPdf;
+			const v3 sampledBSDF = evaluate_bsdf(sd, fN, L, bsdfPdf);
+			const v3 c = throughput * sampledBSDF * lightColor * (NdotL / (pickProb * lightPdf));
+			const uint32_t k = atomicAdd(&counters[2], 1u);
+			const v3 so = safe_origin(I, L, N, P.eps);
+			shadowRays[2 * k] = make_float4(so.x, so.y, so.z, 0.0f);
+			shadowRays[2 * k + 1] = make_float4(L.x, L.y, L.z, dist - 2 * P.eps);
+			shadowPot[k] = make_float4(c.x, c.y, c.z, bitsf(slot));
+		}
+	}
+	v3 R;
+	float newBsdfPdf;
+	bool specular = false;
+	const float r3 = rand_float(seed), r4 = rand_float(seed), r5 = rand_float(seed);
+	const v3 bsdf = sample_bsdf(sd, fN, N, D * -1.0f, t, r3, r4, R, newBsdfPdf, specular);
+	if (newBsdfPdf < EPSILON_F || __builtin_isnan(newBsdfPdf)) return;
+	if (specular) flags |= S_SPECULAR;
+	const float pr = P.pathLength == P.maxDepth ? 0 : ((flags & S_SPECULAR) ? 1 : fminf(1.0f, fmaxf(fmaxf(bsdf.x, bsdf.y), bsdf.z)));
+	if (pr <= r5) return;
+	throughput = throughput * (bsdf * fabsf(dot(fN, R)) / (pr * newBsdfPdf));
+	const uint32_t k = atomicAdd(&counters[1], 1u);
+	const v3 no = safe_origin(I, R, N, P.eps);
+	raysOut[2 * k] = make_float4(no.x, no.y, no.z, 0.0f);
+	raysOut[2 * k + 1] = make_float4(R.x, R.y, R.z, 1e34f);
+	stateOut[k] = make_float4(throughput.x, throughput.y, throughput.z, bitsf(flags));
+	pathOut[k] = pathIdx;
+}
+
+__global__ void k_connect(const float4* __restrict__ shadowPot, const uint32_t* __restrict__ occluded, float4* __restrict__ pathAcc,
+	const uint32_t* __restrict__ counters)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= (int)counters[2]) return;
+	if (occluded[j]) return;
+	const float4 p = shadowPot[j];
+	acc_add(pathAcc + fbits(p.w), xyz(p));
+}
+
+/* Sample-ordered reduction of the path slots into the pixel accumulators. */
+__global__ void k_resolve(const float4* __restrict__ pathAcc, int pixels, int samples, float4* __restrict__ acc32, double* __restrict__ acc64)
+{
+	const int px = blockIdx.x * blockDim.x + threadIdx.x;
+	if (px >= pixels) return;
+	float4 a = acc32[px];
+	double ax = acc64[4 * px], ay = acc64[4 * px + 1], az = acc64[4 * px + 2], aw = acc64[4 * px + 3];
+	for (int s = 0; s < samples; s++)
+	{
+		const float4 c = pathAcc[(size_t)s * pixels + px];
+		a.x = a.x + c.x, a.y = a.y + c.y, a.z = a.z + c.z, a.w = a.w + c.w;
+		ax += (double)c.x, ay += (double)c.y, az += (double)c.z, aw += (double)c.w;
+	}
+	acc32[px] = a;
+	acc64[4 * px] = ax, acc64[4 * px + 1] = ay, acc64[4 * px + 2] = az, acc64[4 * px + 3] = aw;
+}
+
+__global__ void k_finalize(const float4* __restrict__ acc32, int pixels, float scale, float4* __restrict__ image)
+{
+	const int px = blockIdx.x * blockDim.x + threadIdx.x;
+	if (px >= pixels) return;
+	const float4 a = acc32[px];
+	image[px] = make_float4(a.x * scale, a.y * scale, a.z * scale, a.w * scale);
+}
+
+__global__ void k_set_u32(uint32_t* p, int n, uint32_t v)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+__global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) { *dst = *src; }
+
+__global__ void k_advance(uint32_t* c, uint32_t* logSlot)
+{
+	const uint32_t next = c[1];
+	c[0] = next, c[1] = 0u, c[2] = 0u;
+	if (logSlot) *logSlot = next;
+}
+
+hipError_t launch_advance(uint32_t* counters, uint32_t* logSlot, hipStream_t s)
+{
+	k_advance<<<1, 1, 0, s>>>(counters, logSlot);
+	return hipGetLastError();
+}
+
+hipError_t launch_generate(const CamParams& c, uint32_t R0, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s)
+{
+	k_generate<<<(n + 255) / 256, 256, 0, s>>>(c, R0, jobBase, n, eps, b.rays[out], b.state[out], b.pathId[out], b.pathAcc, b.counters);
+	return hipGetLastError();
+}
+
+hipError_t launch_shade(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s)
+{
+	if (maxCount <= 0) return hipSuccess;
+	k_shade<<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
+		b.pathId[in ^ 1], b.shadowRays, b.shadowPot, b.pathAcc, b.counters);
+	return hipGetLastError();
+}
+
+hipError_t launch_connect(const PathBuffers& b, int maxCount, hipStream_t s)
+{
+	if (maxCount <= 0) return hipSuccess;
+	k_connect<<<(maxCount + 255) / 256, 256, 0, s>>>(b.shadowPot, b.occluded, b.pathAcc, b.counters);
+	return hipGetLastError();
+}
+
+hipError_t launch_resolve(const float4* pathAcc, int pixels, int samples, float4* acc32, double* acc64, hipStream_t s)
+{
+	k_resolve<<<(pixels + 255) / 256, 256, 0, s>>>(pathAcc, pixels, samples, acc32, acc64);
+	return hipGetLastError();
+}
+
+hipError_t launch_finalize(const float4* acc32, int pixels, float scale, float4* image, hipStream_t s)
+{
+	k_finalize<<<(pixels + 255) / 256, 256, 0, s>>>(acc32, pixels, scale, image);
+	return hipGetLastError();
+}
+
+hipError_t launch_set_u32(uint32_t* p, int n, uint32_t v, hipStream_t s)
+{
+	k_set_u32<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+	return hipGetLastError();
+}
+
+hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t s)
+{
+	k_copy_u32<<<1, 1, 0, s>>>(dst, src);
+	return hipGetLastError();
+}
+
+} // namespace lh

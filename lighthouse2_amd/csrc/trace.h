@@ -1,0 +1,44 @@
+/* trace.h - traversal kernels of RenderCore_HIP (host-side launch API). */
+#pragma once
+#include "bvh.h"
+
+namespace lh {
+
+enum NodeFormat { FMT_F32 = 0, FMT_Q8 = 1 };
+enum TraceMode { MODE_CLOSEST = 0, MODE_ANY_MASK = 1, MODE_ANY_FLAG = 2 };
+
+struct TraceScene
+{
+	int format = FMT_F32;
+	int twoLevel = 0;
+	/* single level: one mesh, instance 0 with identity transform */
+	const void* nodes = nullptr;
+	const float4* tris = nullptr;
+	uint32_t root = 0;
+	/* two level */
+	const NodeF32* tlas = nullptr;
+	uint32_t tlasRoot = 0;
+	const InstanceRec* inst = nullptr;
+	int maxStack = 0;
+};
+
+struct TraceStats { unsigned long long nodes, tris; };
+
+/* rays: Ray4 {O.xyz, tmin; D.xyz, tmax}. out: MODE_CLOSEST -> float4 hit records;
+ * MODE_ANY_MASK -> uint32 bitmask (must be zeroed); MODE_ANY_FLAG -> uint32 per ray.
+ * scratch: device overflow stack (see trace_scratch_bytes), counter: one uint32
+ * work counter (zeroed by the call). */
+hipError_t launch_trace(const TraceScene& s, int mode, const float4* rays, void* out, int count,
+	uint32_t* counter, uint32_t* overflow, int persistentBlocks, hipStream_t stream, TraceStats* statsDev = nullptr);
+/* As launch_trace, with the ray count read by the kernel from countDev
+ * (device memory) and maxCount an upper bound: no host round trip. */
+hipError_t launch_trace_dev(const TraceScene& s, int mode, const float4* rays, void* out, int maxCount, const uint32_t* countDev,
+	uint32_t* counter, uint32_t* overflow, int persistentBlocks, hipStream_t stream, TraceStats* statsDev = nullptr);
+size_t trace_overflow_bytes(int persistentBlocks);
+
+static const int TRACE_BLOCK = 256;
+static const int TRACE_LDS_STACK = 16;
+static const int TRACE_OVF_STACK = 112;
+static const int TRACE_MAX_STACK = TRACE_LDS_STACK + TRACE_OVF_STACK;
+
+} // namespace lh

@@ -1,0 +1,324 @@
+/* trace.hip - closest-hit and any-hit traversal for gfx950.
+ *
+ * Replaces rtpQueryExecute (lib/RenderCore_PrimeRef/rendercore.cpp:559-573,
+ * 598-607) and optixTrace + __closesthit__radiance / shadow any-hit
+ * (lib/rendercore_optix7/optix/.optix.cu:125-154,174-184).
+ *
+ * Persistent waves: a wave refills its idle lanes with one atomicAdd on a
+ * global ray counter (ballot -> popcount -> mbcnt gives each idle lane its
+ * ray), whenever a quarter of the wave is idle or every lane has finished.
+ * Each lane walks the tree with a short stack of TRACE_LDS_STACK entries in
+ * LDS (lane-interleaved, conflict free) that spills to a per-thread global
+ * area beyond that. Box tests compute t = (plane - O) * rcp(D) and widen the
+ * far distance by 2^-16 relative, so rounding never prunes a box the exact
+ * interval reaches; only the watertight triangle test (lh_math.h) decides a
+ * hit, hence results equal a brute-force scan bit for bit.
+ */
+#include "trace.h"
+#include "lh_math.h"
+
+namespace lh {
+
+struct Lane
+{
+	v3 wO, wD;          /* world ray */
+	v3 O, D, inv;       /* current (object-space in a BLAS) ray */
+	RayPre rp;
+	float tmin, tbest;
+	float bU, bV, bdet;
+	int binst, bprim;
+	int inst;           /* current instance, -1 at the top level */
+	const void* nodes;
+	const float4* tris;
+};
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, float lz, float hz, const v3& O, const v3& inv,
+	float tmin, float tmax, float& tnear)
+{
+	const float tx1 = (lx - O.x) * inv.x, tx2 = (hx - O.x) * inv.x;
+	const float ty1 = (ly - O.y) * inv.y, ty2 = (hy - O.y) * inv.y;
+	const float tz1 = (lz - O.z) * inv.z, tz2 = (hz - O.z) * inv.z;
+	const float tn = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fmaxf(fminf(tz1, tz2), tmin));
+	float tf = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fminf(fmaxf(tz1, tz2), tmax));
+	tf = tf + fabsf(tf) * 1.52587890625e-5f;
+	tnear = tn;
+	return tn <= tf;
+}
+
+template <int BLOCK>
+struct Stack
+{
+	uint32_t* lds;      /* TRACE_LDS_STACK x BLOCK */
+	uint32_t* ovf;      /* this thread's overflow area */
+	int tid, sp;
+	/* The host checks that the tree depth fits TRACE_MAX_STACK; a push beyond
+	 * it is dropped (never written out of bounds). */
+	__device__ __forceinline__ void push(uint32_t v)
+	{
+		if (sp < TRACE_LDS_STACK) lds[sp * BLOCK + tid] = v, sp++;
+		else if (sp < TRACE_MAX_STACK) ovf[sp - TRACE_LDS_STACK] = v, sp++;
+	}
+	__device__ __forceinline__ uint32_t pop()
+	{
+		sp--;
+		return sp < TRACE_LDS_STACK ? lds[sp * BLOCK + tid] : ovf[sp - TRACE_LDS_STACK];
+	}
+};
+
+/* Sort up to 4 (tnear, ref) pairs ascending (misses carry +inf). */
+__device__ __forceinline__ void cswap(float& a, float& b, uint32_t& ra, uint32_t& rb)
+{
+	const bool s = b < a;
+	const float ta = s ? b : a, tb = s ? a : b;
+	const uint32_t qa = s ? rb : ra, qb = s ? ra : rb;
+	a = ta, b = tb, ra = qa, rb = qb;
+}
+
+template <int FMT, int MODE, bool TWO, bool COUNT>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_trace(const TraceScene sc, const float4* __restrict__ rays, void* __restrict__ out,
+	int maxCount, const uint32_t* __restrict__ countDev, uint32_t* __restrict__ counter, uint32_t* __restrict__ overflow, TraceStats* __restrict__ stats)
+{
+	__shared__ uint32_t sstack[TRACE_LDS_STACK * TRACE_BLOCK];
+	const int count = countDev ? (int)*countDev : maxCount;
+	const int tid = threadIdx.x;
+	const int lane = __lane_id();
+	Stack<TRACE_BLOCK> st;
+	st.lds = sstack, st.ovf = overflow + ((size_t)blockIdx.x * TRACE_BLOCK + tid) * TRACE_OVF_STACK, st.tid = tid, st.sp = 0;
+	Lane L;
+	L.inst = -1;
+	L.nodes = sc.nodes, L.tris = sc.tris;
+	int rayIdx = -1;
+	bool active = false, exhausted = false;
+	uint32_t ref = 0;
+	unsigned long long cn = 0, ct = 0;
+	while (true)
+	{
+		const uint64_t idle = __ballot(!active);
+		const int nIdle = __popcll(idle);
+		if (!exhausted && nIdle >= 16)
+		{
+			const int first = __ffsll((unsigned long long)idle) - 1;
+			uint32_t base = 0;
+			if (lane == first) base = atomicAdd(counter, (uint32_t)nIdle);
+			base = __builtin_amdgcn_readlane(base, first);
+			exhausted = base + (uint32_t)nIdle >= (uint32_t)count;
+			if (!active)
+			{
+				const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+				const uint32_t idx = base + below;
+				if (idx < (uint32_t)count)
+				{
+					rayIdx = (int)idx;
+					const float4 o4 = rays[2 * idx], d4 = rays[2 * idx + 1];
+					L.wO = mk3(o4.x, o4.y, o4.z), L.wD = mk3(d4.x, d4.y, d4.z);
+					L.tmin = o4.w, L.tbest = d4.w;
+					L.binst = -1, L.bprim = -1, L.bU = L.bV = 0, L.bdet = 1;
+					L.O = L.wO, L.D = L.wD;
+					L.inv = mk3(rcp(L.D.x), rcp(L.D.y), rcp(L.D.z));
+					st.sp = 0;
+					if (TWO) { L.inst = -1; ref = sc.tlasRoot | REF_TLAS; }
+					else { L.rp = ray_setup(L.D); ref = sc.root; }
+					active = true;
+				}
+			}
+		}
+		if (!__any(active)) break;
+		if (!active) continue;
+		/* traverse until this ray completes */
+		bool done = false, finished = false;
+		while (true)
+		{
+			/* leave to refill when a quarter of the wave has gone idle */
+			if (!exhausted && __popcll(__ballot(true)) < 48) break;
+			if (TWO && (ref & REF_TLAS))
+			{
+				if (L.inst >= 0)
+				{	/* back at the top level: restore the world ray */
+					L.O = L.wO, L.D = L.wD;
+					L.inv = mk3(rcp(L.D.x), rcp(L.D.y), rcp(L.D.z));
+					L.inst = -1;
+				}
+				const uint32_t r = ref & ~REF_TLAS;
+				if (r & REF_LEAF)
+				{	/* enter an instance */
+					const int id = (int)(r & REF_MASK);
+					const InstanceRec& I = sc.inst[id];
+					if (!(I.flags & 1u))
+					{
+						const float4 a = I.inv0, b = I.inv1, c = I.inv2;
+						const v3 wO = L.wO, wD = L.wD;
+						L.O = mk3(a.x * wO.x + a.y * wO.y + a.z * wO.z + a.w, b.x * wO.x + b.y * wO.y + b.z * wO.z + b.w, c.x * wO.x + c.y * wO.y + c.z * wO.z + c.w);
+						L.D = mk3(a.x * wD.x + a.y * wD.y + a.z * wD.z, b.x * wD.x + b.y * wD.y + b.z * wD.z, c.x * wD.x + c.y * wD.y + c.z * wD.z);
+						L.inv = mk3(rcp(L.D.x), rcp(L.D.y), rcp(L.D.z));
+					}
+					L.rp = ray_setup(L.D);
+					L.inst = id;
+					L.nodes = I.nodes, L.tris = I.tris;
+					ref = I.root;
+					continue;
+				}
+				const NodeF32 n = sc.tlas[r];
+				float tl, tr;
+				const bool hl = slab(n.lxy.x, n.lxy.y, n.lxy.z, n.lxy.w, n.lrz.x, n.lrz.y, L.O, L.inv, L.tmin, L.tbest, tl);
+				const bool hr = slab(n.rxy.x, n.rxy.y, n.rxy.z, n.rxy.w, n.lrz.z, n.lrz.w, L.O, L.inv, L.tmin, L.tbest, tr);
+				const uint32_t rl = n.refs.x | REF_TLAS, rr = n.refs.y | REF_TLAS;
+				if (hl && hr) { if (tr < tl) { st.push(rl); ref = rr; } else { st.push(rr); ref = rl; } continue; }
+				if (hl) { ref = rl; continue; }
+				if (hr) { ref = rr; continue; }
+			}
+			else if (ref & REF_LEAF)
+			{
+				uint32_t i = ref & REF_MASK;
+				const int instId = TWO ? L.inst : 0;
+				while (true)
+				{
+					const float4 a = L.tris[3 * i], b = L.tris[3 * i + 1], c = L.tris[3 * i + 2];
+					const uint32_t w = fbits(a.w);
+					const int prim = (int)(w & 0x7fffffffu);
+					if (COUNT) ct++;
+					float t, U, V, det;
+					if (tri_intersect(L.rp, L.O, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, U, V, det) && t > L.tmin)
+					{
+						if (MODE != MODE_CLOSEST)
+						{
+							if (t < L.tbest) { L.binst = instId, L.bprim = prim; done = true; break; }
+						}
+						else if (better_hit(t, instId, prim, L.tbest, L.binst, L.bprim))
+							L.tbest = t, L.binst = instId, L.bprim = prim, L.bU = U, L.bV = V, L.bdet = det;
+					}
+					if (w & TRI_LAST) break;
+					i++;
+				}
+				if (done) break;
+			}
+			else if (FMT == FMT_F32)
+			{
+				if (COUNT) cn++;
+				const NodeF32 n = ((const NodeF32*)L.nodes)[ref];
+				float tl, tr;
+				const bool hl = slab(n.lxy.x, n.lxy.y, n.lxy.z, n.lxy.w, n.lrz.x, n.lrz.y, L.O, L.inv, L.tmin, L.tbest, tl);
+				const bool hr = slab(n.rxy.x, n.rxy.y, n.rxy.z, n.rxy.w, n.lrz.z, n.lrz.w, L.O, L.inv, L.tmin, L.tbest, tr);
+				if (hl && hr) { if (tr < tl) { st.push(n.refs.x); ref = n.refs.y; } else { st.push(n.refs.y); ref = n.refs.x; } continue; }
+				if (hl) { ref = n.refs.x; continue; }
+				if (hr) { ref = n.refs.y; continue; }
+			}
+			else
+			{
+				if (COUNT) cn++;
+				const NodeQ8 n = ((const NodeQ8*)L.nodes)[ref];
+				const uint32_t e = fbits(n.origin.w);
+				const float sx = bitsf((e & 255u) << 23), sy = bitsf(((e >> 8) & 255u) << 23), sz = bitsf(((e >> 16) & 255u) << 23);
+				const int nc = (int)(e >> 24);
+				float tk[4];
+				uint32_t rk[4] = { n.refs.x, n.refs.y, n.refs.z, n.refs.w };
+#pragma unroll
+				for (int c = 0; c < 4; c++)
+				{
+					const int sh = 8 * c;
+					const float lx = fmaf((float)((n.qa.x >> sh) & 255u), sx, n.origin.x);
+					const float ly = fmaf((float)((n.qa.y >> sh) & 255u), sy, n.origin.y);
+					const float lz = fmaf((float)((n.qa.z >> sh) & 255u), sz, n.origin.z);
+					const float hx = fmaf((float)((n.qa.w >> sh) & 255u), sx, n.origin.x);
+					const float hy = fmaf((float)((n.qb.x >> sh) & 255u), sy, n.origin.y);
+					const float hz = fmaf((float)((n.qb.y >> sh) & 255u), sz, n.origin.z);
+					float tn;
+					const bool h = c < nc && slab(lx, hx, ly, hy, lz, hz, L.O, L.inv, L.tmin, L.tbest, tn);
+					tk[c] = h ? tn : __builtin_huge_valf();
+				}
+				cswap(tk[0], tk[1], rk[0], rk[1]);
+				cswap(tk[2], tk[3], rk[2], rk[3]);
+				cswap(tk[0], tk[2], rk[0], rk[2]);
+				cswap(tk[1], tk[3], rk[1], rk[3]);
+				cswap(tk[1], tk[2], rk[1], rk[2]);
+				if (tk[0] < __builtin_huge_valf())
+				{
+					if (tk[3] < __builtin_huge_valf()) st.push(rk[3]);
+					if (tk[2] < __builtin_huge_valf()) st.push(rk[2]);
+					if (tk[1] < __builtin_huge_valf()) st.push(rk[1]);
+					ref = rk[0];
+					continue;
+				}
+			}
+			/* pop */
+			if (st.sp == 0) { finished = true; break; }
+			ref = st.pop();
+		}
+		if (done) finished = true;
+		if (!finished) continue;
+		/* write the result of ray rayIdx */
+		if (MODE == MODE_CLOSEST)
+		{
+			float4 h;
+			if (L.bprim < 0) h = make_float4(0.0f, 0.0f, bitsf(0xffffffffu), L.tbest);
+			else h = make_float4(bitsf(pack_uv(L.bU, L.bV, L.bdet)), bitsf((uint32_t)L.binst), bitsf((uint32_t)L.bprim), L.tbest);
+			((float4*)out)[rayIdx] = h;
+		}
+		else if (MODE == MODE_ANY_MASK)
+		{
+			if (L.bprim >= 0) atomicOr(&((uint32_t*)out)[rayIdx >> 5], 1u << (rayIdx & 31));
+		}
+		else
+		{
+			((uint32_t*)out)[rayIdx] = L.bprim >= 0 ? 1u : 0u;
+		}
+		active = false;
+	}
+	if (COUNT)
+	{
+		atomicAdd(&stats->nodes, cn);
+		atomicAdd(&stats->tris, ct);
+	}
+}
+
+size_t trace_overflow_bytes(int persistentBlocks)
+{
+	return (size_t)persistentBlocks * TRACE_BLOCK * TRACE_OVF_STACK * sizeof(uint32_t);
+}
+
+__global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
+
+template <int FMT, int MODE, bool TWO>
+static void launch3(const TraceScene& s, const float4* rays, void* out, int count, const uint32_t* countDev, uint32_t* counter, uint32_t* overflow,
+	int blocks, hipStream_t stream, TraceStats* stats)
+{
+	if (stats) k_trace<FMT, MODE, TWO, true><<<blocks, TRACE_BLOCK, 0, stream>>>(s, rays, out, count, countDev, counter, overflow, stats);
+	else k_trace<FMT, MODE, TWO, false><<<blocks, TRACE_BLOCK, 0, stream>>>(s, rays, out, count, countDev, counter, overflow, stats);
+}
+
+template <int FMT, int MODE>
+static void launch2(const TraceScene& s, const float4* rays, void* out, int count, const uint32_t* countDev, uint32_t* counter, uint32_t* overflow,
+	int blocks, hipStream_t stream, TraceStats* stats)
+{
+	if (s.twoLevel) launch3<FMT, MODE, true>(s, rays, out, count, countDev, counter, overflow, blocks, stream, stats);
+	else launch3<FMT, MODE, false>(s, rays, out, count, countDev, counter, overflow, blocks, stream, stats);
+}
+
+template <int FMT>
+static void launch1(const TraceScene& s, int mode, const float4* rays, void* out, int count, const uint32_t* countDev, uint32_t* counter, uint32_t* overflow,
+	int blocks, hipStream_t stream, TraceStats* stats)
+{
+	if (mode == MODE_CLOSEST) launch2<FMT, MODE_CLOSEST>(s, rays, out, count, countDev, counter, overflow, blocks, stream, stats);
+	else if (mode == MODE_ANY_MASK) launch2<FMT, MODE_ANY_MASK>(s, rays, out, count, countDev, counter, overflow, blocks, stream, stats);
+	else launch2<FMT, MODE_ANY_FLAG>(s, rays, out, count, countDev, counter, overflow, blocks, stream, stats);
+}
+
+hipError_t launch_trace_dev(const TraceScene& s, int mode, const float4* rays, void* out, int maxCount, const uint32_t* countDev,
+	uint32_t* counter, uint32_t* overflow, int blocks, hipStream_t stream, TraceStats* stats)
+{
+	if (maxCount <= 0) return hipSuccess;
+	if (s.maxStack > TRACE_MAX_STACK) return hipErrorInvalidValue;
+	k_reset_counter<<<1, 1, 0, stream>>>(counter);
+	if (s.format == FMT_Q8) launch1<FMT_Q8>(s, mode, rays, out, maxCount, countDev, counter, overflow, blocks, stream, stats);
+	else launch1<FMT_F32>(s, mode, rays, out, maxCount, countDev, counter, overflow, blocks, stream, stats);
+	return hipGetLastError();
+}
+
+hipError_t launch_trace(const TraceScene& s, int mode, const float4* rays, void* out, int count,
+	uint32_t* counter, uint32_t* overflow, int blocks, hipStream_t stream, TraceStats* stats)
+{
+	return launch_trace_dev(s, mode, rays, out, count, nullptr, counter, overflow, blocks, stream, stats);
+}
+
+} // namespace lh

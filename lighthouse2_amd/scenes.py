@@ -1,0 +1,247 @@
+"""Host-side scene producers for RenderCore_HIP workloads.
+
+These restate the parts of the Lighthouse 2 host scene model that create the
+data crossing the core boundary, in float32 with the reference's operation
+order:
+  add_quad          HostScene::AddQuad        lib/RenderSystem/host_scene.cpp:401-451
+  add_triangle      HostScene::AddTriToMesh   lib/RenderSystem/host_scene.cpp:225-244
+  light_triangle    HostTriLight + ConvertToCoreLightTri  lib/RenderSystem/host_light.cpp:25-59
+  camera_view       Camera::LookAt + GetView  lib/RenderSystem/camera.cpp:68-71,107-128,
+                                              mat4::LookAt common_types.h:519-544
+and generate the seeded synthetic scenes the benchmarks and parity tests use.
+"""
+import numpy as np
+
+from . import formats as F
+
+f32 = np.float32
+
+
+def _normalize(v):
+    v = np.asarray(v, f32)
+    d = f32(v[0] * v[0] + v[1] * v[1]) + f32(v[2] * v[2])
+    inv = f32(1.0) / np.sqrt(f32(d))
+    return (v * inv).astype(f32)
+
+
+def _cross(a, b):
+    a, b = np.asarray(a, f32), np.asarray(b, f32)
+    return np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]], f32)
+
+
+def concat(arrays, dtype):
+    """Concatenate structured arrays keeping `dtype`'s padded C layout
+    (np.concatenate would repack the fields)."""
+    out = np.zeros(sum(len(a) for a in arrays), dtype)
+    k = 0
+    for a in arrays:
+        out[k:k + len(a)] = a
+        k += len(a)
+    return out
+
+
+class Scene:
+    """Meshes (vertex soup + CoreTri), instances, materials and lights, in the
+    order RenderSystem::SynchronizeSceneData hands them to a core."""
+
+    def __init__(self):
+        self.meshes = []          # (verts (3T,4) f32, CoreTri[T] or None)
+        self.instances = []       # (mesh index, (16,) f32 row-major transform)
+        self.materials = F.new_materials(0)
+        self.tri_lights = np.zeros(0, F.CoreLightTri)
+        self.point_lights = np.zeros(0, F.CorePointLight)
+        self.spot_lights = np.zeros(0, F.CoreSpotLight)
+        self.dir_lights = np.zeros(0, F.CoreDirectionalLight)
+
+    def add_material(self, mat):
+        self.materials = concat([self.materials, mat], F.CoreMaterial)
+        return len(self.materials) - 1
+
+    def add_mesh(self, verts, tris):
+        self.meshes.append((np.ascontiguousarray(verts, f32), tris))
+        return len(self.meshes) - 1
+
+    def add_instance(self, mesh, transform=None):
+        t = F.IDENTITY.copy() if transform is None else np.asarray(transform, f32).reshape(16)
+        self.instances.append((mesh, t))
+        return len(self.instances) - 1
+
+    @property
+    def triangle_count(self):
+        return sum(len(v) // 3 for v, _ in self.meshes)
+
+
+def add_quad(N, pos, width, height, mat):
+    """Two triangles of a quad (HostScene::AddQuad): vertex soup (6,4) and CoreTri[2]."""
+    N = _normalize(N)
+    pos = np.asarray(pos, f32)
+    tmp = np.array([0, 1, 0], f32) if abs(N[0]) > 0.9 else np.array([1, 0, 0], f32)
+    T = (f32(0.5) * f32(width)) * _normalize(_cross(N, tmp))
+    B = (f32(0.5) * f32(height)) * _normalize(_cross(_normalize(T), N))
+    T, B = T.astype(f32), B.astype(f32)
+    c = [pos - B - T, pos + B - T, pos - B + T, pos + B - T, pos + B + T, pos - B + T]
+    verts = np.ones((6, 4), f32)
+    for i, p in enumerate(c):
+        verts[i, :3] = p
+    tris = F.new_tris(2)
+    tris["material"] = mat
+    for k in ("vN0", "vN1", "vN2"):
+        tris[k] = N
+    tris["Nx"], tris["Ny"], tris["Nz"] = N[0], N[1], N[2]
+    tris["vertex0"][0], tris["vertex1"][0], tris["vertex2"][0] = verts[0, :3], verts[1, :3], verts[2, :3]
+    tris["vertex0"][1], tris["vertex1"][1], tris["vertex2"][1] = verts[3, :3], verts[4, :3], verts[5, :3]
+    # tri.T = T / (0.5 * height); tri.B = B / (0.5 * width) in double (0.5 * height is a double)
+    tris["T"] = (T.astype(np.float64) / (0.5 * float(height))).astype(f32)
+    tris["B"] = (B.astype(np.float64) / (0.5 * float(width))).astype(f32)
+    return verts, tris
+
+
+def add_triangles(v0, v1, v2, mat):
+    """Flat-shaded triangles (HostScene::AddTriToMesh), vectorised: v0/v1/v2 (T,3)."""
+    v0, v1, v2 = (np.asarray(a, f32) for a in (v0, v1, v2))
+    e1, e2 = v1 - v0, v2 - v0
+    n = np.stack([e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1],
+                  e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2],
+                  e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]], 1).astype(f32)
+    d = (n[:, 0] * n[:, 0] + n[:, 1] * n[:, 1]) + n[:, 2] * n[:, 2]
+    inv = (f32(1.0) / np.sqrt(d)).astype(f32)
+    N = (n * inv[:, None]).astype(f32)
+    T = len(v0)
+    verts = np.ones((3 * T, 4), f32)
+    verts[0::3, :3], verts[1::3, :3], verts[2::3, :3] = v0, v1, v2
+    tris = F.new_tris(T)
+    tris["material"] = mat
+    tris["vN0"], tris["vN1"], tris["vN2"] = N, N, N
+    tris["Nx"], tris["Ny"], tris["Nz"] = N[:, 0], N[:, 1], N[:, 2]
+    tris["vertex0"], tris["vertex1"], tris["vertex2"] = v0, v1, v2
+    return verts, tris
+
+
+def light_triangle(tri, radiance, tri_idx, inst_idx):
+    """CoreLightTri of an emissive triangle (host_light.cpp:25-59)."""
+    L = np.zeros(1, F.CoreLightTri)
+    v0, v1, v2 = (np.asarray(tri[k], f32) for k in ("vertex0", "vertex1", "vertex2"))
+    L["vertex0"], L["vertex1"], L["vertex2"] = v0, v1, v2
+    L["centre"] = f32(0.333333) * ((v0 + v1) + v2)
+    L["N"] = np.array([tri["Nx"], tri["Ny"], tri["Nz"]], f32)
+    ln = lambda a: np.sqrt(f32(a[0] * a[0] + a[1] * a[1]) + f32(a[2] * a[2])).astype(f32)
+    a, b, c = ln(v1 - v0), ln(v2 - v1), ln(v0 - v2)
+    s = f32(f32(a + b) + c) * f32(0.5)
+    area = np.sqrt(f32(f32(f32(s * f32(s - a)) * f32(s - b)) * f32(s - c))).astype(f32)
+    L["area"] = area
+    rad = np.asarray(radiance, f32)
+    L["radiance"] = rad
+    E = rad * area
+    L["energy"] = f32(f32(E[0] + E[1]) + E[2])
+    L["triIdx"], L["instIdx"] = tri_idx, inst_idx
+    return L
+
+
+def look_at(pos, target):
+    """Camera transform of mat4::LookAt(pos, target, up = (0,1,0))."""
+    z = _normalize(np.asarray(target, f32) - np.asarray(pos, f32))
+    x = _normalize(_cross(z, _normalize([0, 1, 0])))
+    y = _cross(x, z)
+    return x, y, z
+
+
+def camera_view(pos, target, fov=40.0, width=1920, height=1080, focal=5.0, aperture=0.0, distortion=0.0):
+    """ViewPyramid of Camera::GetView (camera.cpp:107-128) with aspect = width/height."""
+    right, up, forward = look_at(pos, target)
+    pos = np.asarray(pos, f32)
+    aspect = f32(width / height)
+    PI = f32(3.14159265358979323846264)
+    v = np.zeros(1, F.ViewPyramid)
+    v["pos"] = pos
+    v["spreadAngle"] = f32(f32(f32(fov) * PI) / f32(180)) / f32(height)
+    screen = np.tan(f32(f32(fov) / f32(2)) / f32(f32(180) / PI)).astype(f32)
+    fd = f32(focal)
+    C = pos + fd * forward
+    sr = ((screen * right) * fd) * aspect
+    su = (screen * fd) * up
+    v["p1"] = (C - sr) + su
+    v["p2"] = (C + sr) + su
+    v["p3"] = (C - sr) - su
+    v["aperture"], v["focalDistance"], v["distortion"] = aperture, focal, distortion
+    unit_r, unit_u = (screen * right) * aspect, screen * up
+    p1u, p2u, p3u = (C - unit_r) + unit_u, (C + unit_r) + unit_u, (C - unit_r) - unit_u
+    v["imagePlane"] = float(np.linalg.norm(p1u - p2u) * np.linalg.norm(p1u - p3u))
+    return v
+
+
+def uv_sphere(center, radius, slices, stacks):
+    """Triangles of a latitude/longitude sphere: 2 * slices * (stacks - 1)
+    triangles, wound outward. Returns v0, v1, v2 arrays (T,3) float32."""
+    th = np.linspace(0.0, np.pi, stacks + 1)
+    ph = np.linspace(0.0, 2.0 * np.pi, slices + 1)
+    P = np.stack([np.sin(th)[:, None] * np.cos(ph)[None, :],
+                  np.cos(th)[:, None] * np.ones_like(ph)[None, :],
+                  np.sin(th)[:, None] * np.sin(ph)[None, :]], -1) * radius + np.asarray(center)
+    a, b, c = [], [], []
+    for i in range(stacks):
+        for j in range(slices):
+            p00, p01, p10, p11 = P[i, j], P[i, j + 1], P[i + 1, j], P[i + 1, j + 1]
+            if i > 0:
+                a.append(p00), b.append(p01), c.append(p10)
+            if i < stacks - 1:
+                a.append(p01), b.append(p11), c.append(p10)
+    return (np.asarray(a, f32), np.asarray(b, f32), np.asarray(c, f32))
+
+
+def cornell_box(seed=1234, sphere_tris=20000):
+    """Cornell-box parity scene: five axis-aligned walls of the unit box (open
+    towards -z), a tessellated sphere of radius 0.3 at the centre (flat normals)
+    and a 0.2 x 0.2 quad light under the ceiling facing down, radiance 15.
+    Wall and sphere albedo U(0.2, 0.8) per channel from `seed`; roughness 1.
+    Each wall, the sphere and the light are separate meshes with identity
+    instances, as HostScene creates them."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    walls = [((0, 1, 0), (0.5, 0.0, 0.5)), ((0, -1, 0), (0.5, 1.0, 0.5)), ((0, 0, -1), (0.5, 0.5, 1.0)),
+             ((1, 0, 0), (0.0, 0.5, 0.5)), ((-1, 0, 0), (1.0, 0.5, 0.5))]
+    for N, pos in walls:
+        m = s.add_material(F.lambert_material(rng.uniform(0.2, 0.8, 3).astype(f32)))
+        v, t = add_quad(N, pos, 1.0, 1.0, m)
+        s.add_instance(s.add_mesh(v, t))
+    m = s.add_material(F.lambert_material(rng.uniform(0.2, 0.8, 3).astype(f32)))
+    slices = 100
+    stacks = sphere_tris // (2 * slices) + 1
+    a, b, c = uv_sphere((0.5, 0.5, 0.5), 0.3, slices, stacks)
+    v, t = add_triangles(a, b, c, m)
+    s.add_instance(s.add_mesh(v, t))
+    lm = s.add_material(F.lambert_material((15.0, 15.0, 15.0)))
+    v, t = add_quad((0, -1, 0), (0.5, 0.999, 0.5), 0.2, 0.2, lm)
+    li = s.add_instance(s.add_mesh(v, t))
+    s.tri_lights = concat([light_triangle(t[k], (15.0, 15.0, 15.0), k, li) for k in range(2)], F.CoreLightTri)
+    return s
+
+
+def cornell_view(width=256, height=256):
+    return camera_view((0.5, 0.5, -1.2), (0.5, 0.5, 0.5), 40.0, width, height)
+
+
+def random_triangles(n, seed=1, sigma=0.005):
+    """Incoherent-traversal scene: n triangles, centroids U[0,1)^3, each vertex
+    = centroid + N(0, sigma^2) per component. Vertex soup (3n, 4) float32."""
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 1, 3), dtype=np.float32)
+    e = rng.standard_normal((n, 3, 3), dtype=np.float32) * f32(sigma)
+    v = np.ones((n, 3, 4), f32)
+    v[:, :, :3] = c + e
+    return v.reshape(3 * n, 4)
+
+
+def random_triangle_scene(n, seed=1, with_tris=False):
+    s = Scene()
+    v = random_triangles(n, seed)
+    tris = None
+    if with_tris:
+        m = s.add_material(F.lambert_material((0.5, 0.5, 0.5)))
+        _, tris = add_triangles(v[0::3, :3], v[1::3, :3], v[2::3, :3], m)
+    s.add_instance(s.add_mesh(v, tris))
+    return s
+
+
+def microbench_view(width=1920, height=1080):
+    """Camera of the incoherent-traversal workload: (0.5,0.5,-1.2) -> (0.5,0.5,0.5), FOV 40."""
+    return camera_view((0.5, 0.5, -1.2), (0.5, 0.5, 0.5), 40.0, width, height)

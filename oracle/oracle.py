@@ -1,0 +1,171 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker of RenderCore_HIP. Build it with
+`make -C oracle`.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+P, I, U = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        sigs = {
+            "orc_create": (P, []), "orc_destroy": (None, [P]),
+            "orc_add_mesh": (I, [P, P, I, P]), "orc_add_instance": (I, [P, I, P]), "orc_finalize": (None, [P]),
+            "orc_set_materials": (None, [P, P, I]), "orc_set_lights": (None, [P, P, I, P, I, P, I, P, I]),
+            "orc_trace": (None, [P, P, I, P, I, I, P, I]),
+            "orc_bvh_nodes": (I, [P, I, P, P, P]), "orc_tlas_nodes": (I, [P, P, P]),
+            "orc_render_frame": (None, [P, P, I, I, I, I, ctypes.c_float, U, I, P, I]),
+            "orc_primary_rays": (None, [P, I, I, U, ctypes.c_float, P]),
+            "orc_bounce": (I, [P, P, P, P, I, U, ctypes.c_float, P, P]),
+            "orc_wang_hash": (U, [U]), "orc_xor32_seq": (None, [U, I, P]),
+            "orc_sincos": (None, [ctypes.c_float, P, P]),
+            "orc_float_to_half": (ctypes.c_uint16, [ctypes.c_float]), "orc_half_to_float": (ctypes.c_float, [ctypes.c_uint16]),
+            "orc_mat4_inverse": (None, [P, P]), "orc_convert_material": (None, [P, P]), "orc_sizeof_omat": (I, []),
+            "orc_tri_intersect": (I, [P, P, P, P]),
+        }
+        for n, (r, a) in sigs.items():
+            f = getattr(L, n)
+            f.restype, f.argtypes = r, a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class OracleScene:
+    """CPU restatement of a RenderCore_HIP scene (BVH per mesh, TLAS, materials, lights)."""
+
+    def __init__(self, scene=None):
+        self.h = lib().orc_create()
+        self._keep = []
+        if scene is not None:
+            self.load(scene)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def add_mesh(self, verts, tris=None):
+        v = np.ascontiguousarray(verts, np.float32)
+        return lib().orc_add_mesh(self.h, _p(v), len(v) // 3, _p(tris))
+
+    def add_instance(self, mesh, T=None):
+        t = np.eye(4, dtype=np.float32).reshape(16) if T is None else np.ascontiguousarray(T, np.float32).reshape(16)
+        return lib().orc_add_instance(self.h, mesh, _p(t))
+
+    def finalize(self):
+        lib().orc_finalize(self.h)
+
+    def load(self, scene):
+        for v, t in scene.meshes:
+            self.add_mesh(v, None if t is None else np.ascontiguousarray(t))
+        for m, T in scene.instances:
+            self.add_instance(m, T)
+        self.finalize()
+        mats = np.ascontiguousarray(scene.materials)
+        lib().orc_set_materials(self.h, _p(mats), len(mats))
+        a = [np.ascontiguousarray(x) for x in (scene.tri_lights, scene.point_lights, scene.spot_lights, scene.dir_lights)]
+        self._keep += [mats] + a
+        lib().orc_set_lights(self.h, _p(a[0]), len(a[0]), _p(a[1]), len(a[1]), _p(a[2]), len(a[2]), _p(a[3]), len(a[3]))
+
+    def trace(self, rays, any_hit=False, brute=False, threads=8, counters=False):
+        """rays (n, 8) float32 {O, tmin, D, tmax}. Closest: (n, 4) records with
+        uint32 views (uv, inst, prim) and t; any: (n,) bool."""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        h = np.zeros((len(r), 4), np.float32)
+        c = np.zeros(3, np.uint64) if counters else None
+        lib().orc_trace(self.h, _p(r), len(r), _p(h), 1 if any_hit else 0, 1 if brute else 0, _p(c), threads)
+        out = h[:, 0] != 0 if any_hit else h
+        return (out, c) if counters else out
+
+    def bvh_nodes(self, mesh=0):
+        from lighthouse2_amd.formats import BuildNode
+        n = lib().orc_bvh_nodes(self.h, mesh, None, None, None)
+        nodes = np.zeros(n, BuildNode)
+        depth = ctypes.c_int(0)
+        lib().orc_bvh_nodes(self.h, mesh, _p(nodes), None, ctypes.byref(depth))
+        prims = np.zeros(int(nodes["count"].sum()), np.uint32)
+        lib().orc_bvh_nodes(self.h, mesh, None, _p(prims), None)
+        return nodes, prims, depth.value
+
+    def render_frame(self, view, w, h, spp, max_depth, eps, R0gen, pass_, threads=8):
+        acc = np.zeros((spp * w * h, 4), np.float32)
+        v = np.ascontiguousarray(view)
+        lib().orc_render_frame(self.h, _p(v), w, h, spp, max_depth, eps, R0gen, pass_, _p(acc), threads)
+        return acc
+
+    def bounce(self, rays, hits, path_ids, R0, tmin):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        h = np.ascontiguousarray(hits, np.float32).reshape(-1, 4)
+        p = np.ascontiguousarray(path_ids, np.uint32)
+        out = np.zeros_like(r)
+        po = np.zeros_like(p)
+        k = lib().orc_bounce(self.h, _p(r), _p(h), _p(p), len(r), R0, tmin, _p(out), _p(po))
+        return out[:k], po[:k]
+
+
+def xor32_host(seed):
+    """RandomUInt(seed) of lib/platform/system.cpp:51: returns (value, new seed)."""
+    s = seed & 0xffffffff
+    s ^= (s << 13) & 0xffffffff
+    s ^= s >> 17
+    s ^= (s << 5) & 0xffffffff
+    return s, s
+
+
+def render(scene_or_oracle, view, w, h, spp=1, frames=1, max_depth=3, eps=1e-3, seed=0x12345678, threads=8):
+    """Oracle image: `frames` Render calls (Restart, then Converge) of spp
+    samples. Returns (acc32 float32 (h,w,4) sum, acc64 float64 sum, samplesTaken)."""
+    o = scene_or_oracle if isinstance(scene_or_oracle, OracleScene) else OracleScene(scene_or_oracle)
+    acc32 = np.zeros((w * h, 4), np.float32)
+    acc64 = np.zeros((w * h, 4), np.float64)
+    cam = seed
+    taken = 0
+    for _ in range(frames):
+        R0, cam = xor32_host(cam)
+        pa = o.render_frame(view, w, h, spp, max_depth, eps, R0, taken, threads)
+        for s in range(spp):
+            blk = pa[s * w * h:(s + 1) * w * h]
+            acc32 += blk
+            acc64 += blk.astype(np.float64)
+        taken += spp
+    return acc32.reshape(h, w, 4), acc64.reshape(h, w, 4), taken
+
+
+def sincos(x):
+    s, c = ctypes.c_float(), ctypes.c_float()
+    lib().orc_sincos(x, ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
+
+
+def wang_hash(s):
+    return lib().orc_wang_hash(s)
+
+
+def xor32_seq(seed, n):
+    out = np.zeros(n, np.uint32)
+    lib().orc_xor32_seq(seed, n, _p(out))
+    return out
+
+
+def primary_rays(view, w, h, R0, tmin):
+    out = np.zeros((w * h, 8), np.float32)
+    v = np.ascontiguousarray(view)
+    lib().orc_primary_rays(_p(v), w, h, R0, tmin, _p(out))
+    return out

@@ -1,0 +1,509 @@
+/* oracle_render.cpp - CPU oracle of the reference estimator ("REFERENCE, for
+ * image validation": RenderCore_PrimeRef). TEST INFRASTRUCTURE.
+ *
+ * Each function restates the reference lines it cites, in the same arithmetic
+ * order, in fp32. Paths are traced depth-first: a path's result does not
+ * depend on the wavefront's compaction order, and its contributions reach its
+ * accumulator slot in the order the wavefront deposits them (connect of
+ * bounce k before shade of bounce k+1). */
+#include "oracle_render.h"
+#include <thread>
+#include <vector>
+#include <algorithm>
+
+namespace orc {
+
+static const float PI = 3.14159265358979323846264f;
+static const float INVPI = 0.31830988618379067153777f;
+static const float TWOPI = 6.28318530717958647692528f;
+static const float EPSILON = 0.0001f;
+
+static inline f3 f3of(const lh2hip::lh2_float3& a) { return mk3(a.x, a.y, a.z); }
+
+/* TOCHAR of rendercore.cpp:359, saturated to a byte (the reference overflows
+ * into the next byte for values above 1). */
+static inline uint tochar(float a)
+{
+	const float s = a * 255.0f;
+	if (!(s > 0.0f)) return 0;
+	if (s >= 255.0f) return 255;
+	return (uint)s;
+}
+static inline uint touint4(float a, float b, float c, float d) { return tochar(a) + (tochar(b) << 8) + (tochar(c) << 16) + (tochar(d) << 24); }
+
+/* RenderCore::SetMaterials (lib/RenderCore_PrimeRef/rendercore.cpp:360-402). */
+OMat convert_material(const CoreMaterial& m)
+{
+	OMat g;
+	memset(&g, 0, sizeof(g));
+	g.diffuse[0] = float_to_half(m.color.value.x);
+	g.diffuse[1] = float_to_half(m.color.value.y);
+	g.diffuse[2] = float_to_half(m.color.value.z);
+	g.transm[0] = float_to_half(1.0f - m.absorption.value.x);
+	g.transm[1] = float_to_half(1.0f - m.absorption.value.y);
+	g.transm[2] = float_to_half(1.0f - m.absorption.value.z);
+	g.params[0] = touint4(m.metallic.value, m.subsurface.value, m.specular.value, m.roughness.value);
+	g.params[1] = touint4(m.specularTint.value, m.anisotropic.value, m.sheen.value, m.sheenTint.value);
+	g.params[2] = touint4(m.clearcoat.value, m.clearcoatGloss.value, m.transmission.value, 0.0f);
+	g.params[3] = f2u(m.eta.value);
+	g.flags = (m.eta.value < 1 ? M_ISDIELECTRIC : 0) +
+		(m.color.textureID != -1 ? M_HASDIFFUSEMAP : 0) + (m.normals.textureID != -1 ? M_HASNORMALMAP : 0) +
+		(m.specular.textureID != -1 ? M_HASSPECULARITYMAP : 0) + (m.roughness.textureID != -1 ? M_HASROUGHNESSMAP : 0) +
+		(m.detailNormals.textureID != -1 ? M_HAS2NDNORMALMAP : 0) + (m.detailColor.textureID != -1 ? M_HAS2NDDIFFUSEMAP : 0) +
+		((m.flags & 1) ? M_HASSMOOTHNORMALS : 0) + ((m.flags & 2) ? M_HASALPHA : 0);
+	return g;
+}
+
+static inline float char2flt(uint a, int s) { return ((float)((a >> s) & 255u)) * (1.0f / 255.0f); }
+
+struct Shading
+{
+	f3 color, transmittance;
+	uint params[4];
+	float roughness() const { const float r = char2flt(params[0], 24); return r > 0.001f ? r : 0.001f; }
+	float transmission() const { return char2flt(params[2], 16); }
+	float eta() const { return u2f(params[3]); }
+	bool emissive() const { return color.x > 1.0f || color.y > 1.0f || color.z > 1.0f; }
+};
+
+/* ConsistentNormal, Reshetov et al. 2010, Eq. 1 approximation
+ * (lib/CUDA/shared_kernel_code/tools_shared.h:266-280). */
+static f3 consistent_normal(f3 D, f3 iN, float alpha)
+{
+	const float t = PI - 2 * alpha, q = (t * t) / (PI * (PI + (2 * PI - 4) * alpha));
+	const float b = dot(D, iN), g = 1 + q * (b - 1), rho = sqrtf(q * (1 + g) / (1 + b));
+	const f3 Rc = (g + rho * b) * iN - (rho * D);
+	return normalize(D + Rc);
+}
+
+/* SetupFrame + GetShadingData, OPTIXPRIMEBUILD barycentric convention
+ * (lib/CUDA/shared_kernel_code/material_shared.h:42-94,96-128), CONSISTENTNORMALS
+ * on (lib/RenderCore_PrimeRef/core_settings.h:26); no texture maps. */
+static void shading_data(const RenderScene& s, f3 D, float u, float v, const CoreTri& tri, int instIdx,
+	Shading& sd, f3& N, f3& iN, f3& fN, f3& T)
+{
+	const OMat& mat = s.mats[tri.material];
+	sd.color = mk3(half_to_float(mat.diffuse[0]), half_to_float(mat.diffuse[1]), half_to_float(mat.diffuse[2]));
+	sd.transmittance = mk3(half_to_float(mat.transm[0]), half_to_float(mat.transm[1]), half_to_float(mat.transm[2]));
+	for (int i = 0; i < 4; i++) sd.params[i] = mat.params[i];
+	N = iN = fN = mk3(tri.Nx, tri.Ny, tri.Nz);
+	T = f3of(tri.T);
+	const float w = 1 - (u + v);
+	if (mat.flags & M_HASSMOOTHNORMALS) iN = normalize(u * f3of(tri.vN0) + v * f3of(tri.vN1) + w * f3of(tri.vN2));
+	const float* inv = s.inst[instIdx].inv;
+	const f3 A = mk3(inv[0], inv[1], inv[2]), B = mk3(inv[4], inv[5], inv[6]), C = mk3(inv[8], inv[9], inv[10]);
+	N = normalize(N.x * A + N.y * B + N.z * C);
+	iN = normalize(iN.x * A + iN.y * B + iN.z * C);
+	const bool backSide = dot(D, N) > 0;
+	const float alpha = u * tri.alpha.x + v * tri.alpha.y + w * tri.alpha.z;
+	iN = (backSide ? -1.0f : 1.0f) * consistent_normal(D * -1.0f, backSide ? (iN * -1.0f) : iN, alpha);
+	fN = iN;
+}
+
+/* RandomBarycentrics (lib/RenderSystem/common_functions.h:52-71); the 0:32
+ * fixed-point conversion wraps 2^32 to 0 where the reference is undefined. */
+static f3 random_barycentrics(float r0)
+{
+	const uint uf = (uint)(uint64_t)(r0 * 4294967296.0f);
+	float Ax = 1, Ay = 0, Bx = 0, By = 1, Cx = 0, Cy = 0;
+	for (int i = 0; i < 16; ++i)
+	{
+		const int d = (uf >> (2 * (15 - i))) & 0x3;
+		float Anx, Any, Bnx, Bny, Cnx, Cny;
+		switch (d)
+		{
+		case 0: Anx = (Bx + Cx) * 0.5f, Any = (By + Cy) * 0.5f; Bnx = (Ax + Cx) * 0.5f, Bny = (Ay + Cy) * 0.5f; Cnx = (Ax + Bx) * 0.5f, Cny = (Ay + By) * 0.5f; break;
+		case 1: Anx = Ax, Any = Ay; Bnx = (Ax + Bx) * 0.5f, Bny = (Ay + By) * 0.5f; Cnx = (Ax + Cx) * 0.5f, Cny = (Ay + Cy) * 0.5f; break;
+		case 2: Anx = (Bx + Ax) * 0.5f, Any = (By + Ay) * 0.5f; Bnx = Bx, Bny = By; Cnx = (Bx + Cx) * 0.5f, Cny = (By + Cy) * 0.5f; break;
+		default: Anx = (Cx + Ax) * 0.5f, Any = (Cy + Ay) * 0.5f; Bnx = (Cx + Bx) * 0.5f, Bny = (Cy + By) * 0.5f; Cnx = Cx, Cny = Cy; break;
+		}
+		Ax = Anx, Ay = Any, Bx = Bnx, By = Bny, Cx = Cnx, Cy = Cny;
+	}
+	const float rx = (Ax + Bx + Cx) * 0.3333333f, ry = (Ay + By + Cy) * 0.3333333f;
+	return mk3(rx, ry, 1 - rx - ry);
+}
+
+/* Potential contributions (lib/CUDA/shared_kernel_code/lights_shared.h:37-108). */
+static float potential_tri(const CoreLightTri& L, f3 O, f3 N, f3 I, f3 bary)
+{
+	f3 Lv = I;
+	if (bary.x >= 0)
+	{
+		const f3 V0 = f3of(L.vertex0), V1 = f3of(L.vertex1), V2 = f3of(L.vertex2);
+		Lv = bary.x * V0 + bary.y * V1 + bary.z * V2;
+	}
+	Lv = Lv - O;
+	const float att = 1.0f / dot(Lv, Lv);
+	Lv = normalize(Lv);
+	const float LNdotL = std::max(0.0f, -dot(f3of(L.N), Lv));
+	const float NdotL = std::max(0.0f, dot(N, Lv));
+	return L.energy * LNdotL * NdotL * att;
+}
+static float potential_point(const CorePointLight& L, f3 I, f3 N)
+{
+	const f3 Lv = f3of(L.position) - I;
+	const float NdotL = std::max(0.0f, dot(N, normalize(Lv)));
+	const float att = 1.0f / dot(Lv, Lv);
+	return L.energy * NdotL * att;
+}
+static float potential_spot(const CoreSpotLight& L, f3 I, f3 N)
+{
+	f3 Lv = f3of(L.position) - I;
+	const float att = 1.0f / dot(Lv, Lv);
+	Lv = normalize(Lv);
+	const float d = (std::max(0.0f, -dot(Lv, f3of(L.direction))) - L.cosOuter) / (L.cosInner - L.cosOuter);
+	const float NdotL = std::max(0.0f, dot(N, Lv));
+	const float LNdotL = std::max(0.0f, std::min(1.0f, d));
+	return (L.radiance.x + L.radiance.y + L.radiance.z) * LNdotL * NdotL * att;
+}
+static float potential_dir(const CoreDirectionalLight& L, f3 N)
+{
+	const float LNdotL = std::max(0.0f, -(L.direction.x * N.x + L.direction.y * N.y + L.direction.z * N.z));
+	return L.energy * LNdotL;
+}
+
+/* RandomPointOnLight with importance sampling (lights_shared.h:225-313). The
+ * potentials are summed in one pass and recomputed (identically) while
+ * searching, so there is no 64-light stack array. */
+static f3 random_point_on_light(const RenderScene& s, float r0, float r1, f3 I, f3 N, float& pickProb, float& lightPdf, f3& lightColor)
+{
+	const int nt = (int)s.triLights.size(), np = (int)s.pointLights.size(), ns = (int)s.spotLights.size(), nd = (int)s.dirLights.size();
+	const int lightCount = nt + np + ns + nd;
+	const f3 bary = random_barycentrics(r0);
+	auto pot = [&](int i) -> float {
+		if (i < nt) return potential_tri(s.triLights[i], I, N, I, bary);
+		if (i < nt + np) return potential_point(s.pointLights[i - nt], I, N);
+		if (i < nt + np + ns) return potential_spot(s.spotLights[i - nt - np], I, N);
+		return potential_dir(s.dirLights[i - nt - np - ns], N);
+	};
+	float sum = 0;
+	for (int i = 0; i < lightCount; i++) sum += pot(i);
+	if (sum <= 0) { lightPdf = 0; return mk3(1, 1, 1); }
+	r1 *= sum;
+	int lightIdx = 0;
+	float total = 0, pl = 0;
+	for (int i = 0; i < lightCount; i++)
+	{
+		const float c = pot(i);
+		total += c;
+		if (total >= r1) { lightIdx = i; pl = c; break; }
+	}
+	if (lightIdx == 0) pl = pot(0);
+	pickProb = pl / sum;
+	if (lightIdx < nt)
+	{
+		const CoreLightTri& L = s.triLights[lightIdx];
+		const f3 V0 = f3of(L.vertex0), V1 = f3of(L.vertex1), V2 = f3of(L.vertex2);
+		lightColor = f3of(L.radiance);
+		const f3 LN = f3of(L.N);
+		const f3 P = bary.x * V0 + bary.y * V1 + bary.z * V2;
+		f3 Lv = I - P;
+		const float sqDist = dot(Lv, Lv);
+		Lv = normalize(Lv);
+		const float LNdotL = Lv.x * LN.x + Lv.y * LN.y + Lv.z * LN.z;
+		const float reciSolidAngle = sqDist / (L.area * LNdotL);
+		lightPdf = (LNdotL > 0 && dot(Lv, N) < 0) ? reciSolidAngle : 0;
+		return P;
+	}
+	else if (lightIdx < nt + np)
+	{
+		const CorePointLight& L = s.pointLights[lightIdx - nt];
+		const f3 P = f3of(L.position), Lv = P - I;
+		const float sqDist = dot(Lv, Lv);
+		lightColor = f3of(L.radiance) / sqDist;
+		lightPdf = dot(Lv, N) > 0 ? 1 : 0;
+		return P;
+	}
+	else if (lightIdx < nt + np + ns)
+	{
+		const CoreSpotLight& L = s.spotLights[lightIdx - nt - np];
+		const f3 P = f3of(L.position);
+		f3 Lv = I - P;
+		const float sqDist = dot(Lv, Lv);
+		Lv = normalize(Lv);
+		const float d = (std::max(0.0f, Lv.x * L.direction.x + Lv.y * L.direction.y + Lv.z * L.direction.z) - L.cosOuter) / (L.cosInner - L.cosOuter);
+		const float LNdotL = std::min(1.0f, d);
+		lightPdf = (LNdotL > 0 && dot(Lv, N) < 0) ? (sqDist / LNdotL) : 0;
+		lightColor = f3of(L.radiance);
+		return P;
+	}
+	const CoreDirectionalLight& L = s.dirLights[lightIdx - nt - np - ns];
+	const f3 Ld = f3of(L.direction);
+	lightColor = f3of(L.radiance);
+	const float NdotL = dot(Ld, N);
+	lightPdf = NdotL < 0 ? 1 : 0;
+	return I - 1000.0f * Ld;
+}
+
+/* SetupTangentSpace / Tangent2World, Duff et al. 2017
+ * (lib/RenderSystem/common_functions.h:73-88). */
+static f3 tangent2world(f3 V, f3 N)
+{
+	const float sign = copysignf(1.0f, N.z);
+	const float a = -1.0f / (sign + N.z);
+	const float b = N.x * N.y * a;
+	const f3 B = mk3(1.0f + sign * N.x * N.x * a, sign * b, -sign * N.x);
+	const f3 T = mk3(b, sign + N.y * N.y * a, -N.y);
+	return V.x * T + V.y * B + V.z * N;
+}
+
+/* DiffuseReflectionCosWeighted (common_functions.h:104-110). */
+static f3 cosine_sample(float r0, float r1)
+{
+	const float term1 = TWOPI * r0, term2 = sqrtf(1 - r1);
+	float sn, cs;
+	cw_sincos(term1, &sn, &cs);
+	return mk3(cs * term2, sn * term2, sqrtf(r1));
+}
+
+static inline f3 reflect(f3 i, f3 n) { return i - 2.0f * n * dot(n, i); }
+
+/* Lambert + pure specular BSDF (lib/RenderCore_PrimeRef/kernels/bsdf.h:19-101). */
+static float fr_l(float VDotN, float eio)
+{
+	const float SinThetaT2 = (eio * eio) * (1.0f - VDotN * VDotN);
+	if (SinThetaT2 > 1.0f) return 1.0f;
+	const float LDotN = sqrtf(1.0f - SinThetaT2);
+	const float r1 = (VDotN - eio * LDotN) / (VDotN + eio * LDotN);
+	const float r2 = (LDotN - eio * VDotN) / (LDotN + eio * VDotN);
+	return 0.5f * (r1 * r1 + r2 * r2);
+}
+static bool refract_l(f3 wi, f3 n, float eta, f3& wt)
+{
+	const float cosThetaI = fabsf(dot(n, wi));
+	const float sin2ThetaI = std::max(0.0f, 1.0f - cosThetaI * cosThetaI);
+	const float sin2ThetaT = eta * eta * sin2ThetaI;
+	if (sin2ThetaT >= 1) return false;
+	const float cosThetaT = sqrtf(1.0f - sin2ThetaT);
+	wt = eta * (wi * -1.0f) + (eta * cosThetaI - cosThetaT) * n;
+	return true;
+}
+/* expf of Beer's law: exp(x) = 2^k * exp(r), k = round(x/ln2), Cody-Waite ln2
+ * split, Taylor series of exp(r) to r^7 on |r| <= ln2/2. */
+static float cw_expf(float x)
+{
+	if (x < -87.0f) return 0.0f;
+	if (x > 88.0f) return INFINITY;
+	const float k = floorf(x * 1.44269504f + 0.5f);
+	const float r = (x - k * 0.693359375f) - k * -2.12194440e-4f;
+	const float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.16666667f + r * (4.1666668e-2f + r * (8.3333333e-3f + r * (1.3888889e-3f + r * 1.9841270e-4f))))));
+	return p * u2f((uint)((int)k + 127) << 23);
+}
+static f3 evaluate_bsdf(const Shading& sd, f3 iN, f3 wo, f3 wi, float& pdf)
+{
+	(void)wo;
+	if (sd.transmission() > 0.999f || sd.roughness() <= 0.001f) { pdf = 0; return mk3(0, 0, 0); }
+	pdf = fabsf(dot(wi, iN)) * INVPI;
+	return sd.color * INVPI * sd.roughness();
+}
+static f3 sample_bsdf(const Shading& sd, f3 iN, f3 N, f3 wo, float distance, float r3, float r4, f3& wi, float& pdf, bool& specular)
+{
+	const float flip = (dot(wo, N) < 0) ? -1 : 1;
+	iN = iN * flip;
+	specular = true, pdf = 1;
+	f3 bsdf;
+	const float TR = sd.transmission(), RO = sd.roughness();
+	if (r4 < TR)
+	{
+		const float eio = flip < 0 ? (1.0f / sd.eta()) : sd.eta(), F = fr_l(dot(iN, wo), eio);
+		f3 beer = mk3(1, 1, 1);
+		beer.x = cw_expf(-sd.transmittance.x * distance * 2.0f);
+		beer.y = cw_expf(-sd.transmittance.y * distance * 2.0f);
+		beer.z = cw_expf(-sd.transmittance.z * distance * 2.0f);
+		if (r3 < F)
+		{
+			wi = reflect(wo * -1.0f, iN);
+			bsdf = sd.color * beer * (1 / fabsf(dot(iN, wi)));
+		}
+		else
+		{
+			if (!refract_l(wo, iN, eio, wi)) return mk3(0, 0, 0);
+			return sd.color * beer * (1 / fabsf(dot(iN, wi)));
+		}
+	}
+	else
+	{
+		const float pReflect = 1 - RO;
+		if (r3 < pReflect)
+		{
+			wi = reflect(wo * -1.0f, iN);
+			bsdf = sd.color * (1.0f / fabsf(dot(iN, wi)));
+		}
+		else
+		{
+			const float r5 = (r3 - pReflect) / (1 - pReflect);
+			const float r6 = (r4 - TR) / (1 - TR);
+			wi = normalize(tangent2world(cosine_sample(r5, r6), iN));
+			pdf = std::max(0.0f, dot(wi, iN)) * INVPI;
+			specular = false;
+			bsdf = sd.color * INVPI;
+		}
+	}
+	if (dot(N * flip, wi) <= 0) pdf = 0;
+	return bsdf;
+}
+
+static inline f3 safe_origin(f3 O, f3 R, f3 N, float eps) { return O + N * (dot(N, R) > 0 ? eps : -eps); }
+
+/* RandomPointOnLens, 9-blade aperture (lib/RenderCore_PrimeRef/kernels/camera.h:22-33). */
+static f3 point_on_lens(float r0, float r1, f3 pos, float aperture, f3 right, f3 up)
+{
+	const float blade = (float)(int)(r0 * 9);
+	float r2 = (r0 - blade * (1.0f / 9.0f)) * 9.0f;
+	float x1, y1, x2, y2;
+	cw_sincos(blade * PI / 4.5f, &x1, &y1);
+	cw_sincos((blade + 1.0f) * PI / 4.5f, &x2, &y2);
+	if ((r1 + r2) > 1) r1 = 1.0f - r1, r2 = 1.0f - r2;
+	const float xr = x1 * r1 + x2 * r2;
+	const float yr = y1 * r1 + y2 * r2;
+	return pos + aperture * (right * xr + up * yr);
+}
+
+/* generateEyeRaysKernel (lib/RenderCore_PrimeRef/kernels/camera.h:39-68). */
+static void eye_ray(const ViewPyramid& v, int w, int h, uint R0, int job, f3& O, f3& D)
+{
+	const uint x = (uint)job % (uint)w;
+	uint y = (uint)job / (uint)w;
+	y %= (uint)h;
+	uint seed = wang_hash((uint)job + R0);
+	const float r0 = rand_float(seed), r1 = rand_float(seed);
+	const float r2 = rand_float(seed), r3 = rand_float(seed);
+	const f3 pos = f3of(v.pos), p1 = f3of(v.p1);
+	const f3 right = f3of(v.p2) - p1, up = f3of(v.p3) - p1;
+	const f3 posOnPixel = p1 + ((float)x + r0) * (right / (float)w) + ((float)y + r1) * (up / (float)h);
+	const f3 posOnLens = point_on_lens(r2, r3, pos, v.aperture, right, up);
+	O = posOnLens;
+	D = normalize(posOnPixel - posOnLens);
+}
+
+/* One path, depth first: the wavefront loop of RenderCore::Render
+ * (lib/RenderCore_PrimeRef/rendercore.cpp:564-611) with shadeKernel
+ * (kernels/pathtracer.h:45-168) and finalizeConnectionKernel (connections.h:22-34).
+ * NEE rays emitted at the last depth are not traced: the reference loop breaks
+ * before the connect step (rendercore.cpp:586-591). */
+static void trace_path(const RenderScene& s, const RenderParams& p, int job, uint R0gen, int pass, float* acc4)
+{
+	const int wh = p.w * p.h;
+	f3 O, D;
+	eye_ray(p.view, p.w, p.h, R0gen, job, O, D);
+	float tmin = p.eps;
+	f3 throughput = mk3(1, 1, 1);
+	uint flags = S_SPECULAR;
+	const uint pathIdx = (uint)job;
+	const uint pixelIdx = pathIdx % (uint)wh;
+	float ax = 0, ay = 0, az = 0, aw = 0;
+	(void)pixelIdx;
+	for (int pathLength = 1; pathLength <= p.maxDepth; pathLength++)
+	{
+		const Hit h = trace_closest(s, O, tmin, D, 1e34f, nullptr);
+		const uint R0 = (uint)pass * 7907u + (uint)pathLength * 91771u;
+		if (pathLength == 1) aw += h.prim < 0 ? 10000 : h.t;
+		if (h.prim < 0)
+		{	/* sky: black (no sky data in the parity scenes) */
+			const f3 c = throughput * mk3(0, 0, 0);
+			ax += c.x, ay += c.y, az += c.z;
+			break;
+		}
+		const float u = (float)(h.uv & 65535u) * (1.0f / 65535.0f), v = (float)(h.uv >> 16) * (1.0f / 65535.0f);
+		const float t = h.t;
+		const CoreTri& tri = s.tris[s.inst[h.inst].mesh][h.prim];
+		Shading sd;
+		f3 N, iN, fN, T;
+		const f3 I = O + t * D;
+		shading_data(s, D, u, v, tri, h.inst, sd, N, iN, fN, T);
+		if (sd.emissive())
+		{
+			const float DdotNL = -dot(D, N);
+			if (DdotNL > 0)
+			{
+				const f3 c = throughput * sd.color;
+				if (pathLength == 1 || (flags & S_SPECULAR)) ax += c.x, ay += c.y, az += c.z;
+			}
+			break;
+		}
+		if (sd.roughness() <= 0.001f || sd.transmission() > 0.999f) flags |= S_SPECULAR; else flags &= ~S_SPECULAR;
+		uint seed = wang_hash(pathIdx * 17u + R0);
+		const float faceDir = (dot(D, N) > 0) ? -1 : 1;
+		if (faceDir == 1) sd.transmittance = mk3(0, 0, 0);
+		if ((flags & S_SPECULAR) == 0)
+		{
+			const float r0 = rand_float(seed), r1 = rand_float(seed);
+			float pickProb = 0, lightPdf = 0;
+			f3 lightColor = mk3(0, 0, 0);
+			f3 L = random_point_on_light(s, r0, r1, I, fN * faceDir, pickProb, lightPdf, lightColor) - I;
+			const float dist = length(L);
+			L = L * (1.0f / dist);
+			const float NdotL = dot(L, fN * faceDir);
+			if (NdotL > 0 && lightPdf > 0)
+			{
+				float bsdfPdf;
+				const f3 sampledBSDF = evaluate_bsdf(sd, fN, D * -1.0f, L, bsdfPdf);
+				const f3 c = throughput * sampledBSDF * lightColor * (NdotL / (pickProb * lightPdf));
+				if (pathLength < p.maxDepth)
+				{
+					const f3 so = safe_origin(I, L, N, p.eps);
+					if (!trace_any(s, so, 0.0f, L, dist - 2 * p.eps, nullptr)) ax += c.x, ay += c.y, az += c.z;
+				}
+			}
+		}
+		f3 R;
+		float newBsdfPdf;
+		bool specular = false;
+		const float r3 = rand_float(seed), r4 = rand_float(seed), r5 = rand_float(seed);
+		const f3 bsdf = sample_bsdf(sd, fN, N, D * -1.0f, t, r3, r4, R, newBsdfPdf, specular);
+		if (newBsdfPdf < EPSILON || std::isnan(newBsdfPdf)) break;
+		if (specular) flags |= S_SPECULAR;
+		const float pr = pathLength == p.maxDepth ? 0 : ((flags & S_SPECULAR) ? 1 : std::min(1.0f, std::max(std::max(bsdf.x, bsdf.y), bsdf.z)));
+		if (pr <= r5) break;
+		throughput = throughput * (bsdf * fabsf(dot(fN, R)) / (pr * newBsdfPdf));
+		O = safe_origin(I, R, N, p.eps);
+		D = R;
+		tmin = 0.0f;
+	}
+	acc4[0] = ax, acc4[1] = ay, acc4[2] = az, acc4[3] = aw;
+}
+
+void render_frame(const RenderScene& s, const RenderParams& p, uint R0gen, int pass, float* pathAcc, int threads)
+{
+	const int n = p.w * p.h * p.spp;
+	if (threads < 1) threads = 1;
+	std::vector<std::thread> pool;
+	for (int t = 0; t < threads; t++)
+		pool.emplace_back([&, t]() {
+			for (int j = t; j < n; j += threads) trace_path(s, p, j, R0gen, pass, pathAcc + (size_t)j * 4);
+		});
+	for (auto& th : pool) th.join();
+}
+
+void primary_ray(const ViewPyramid& v, int w, int h, uint R0, int job, float tmin, float* ray8)
+{
+	f3 O, D;
+	eye_ray(v, w, h, R0, job, O, D);
+	ray8[0] = O.x, ray8[1] = O.y, ray8[2] = O.z, ray8[3] = tmin;
+	ray8[4] = D.x, ray8[5] = D.y, ray8[6] = D.z, ray8[7] = 1e34f;
+}
+
+/* Diffuse bounce of the traversal microbenchmark: geometric normal (object
+ * space cross product, instance inverse-transpose applied), flipped against
+ * the ray; cosine-weighted direction; origin at the hit point. */
+bool bounce_ray(const RenderScene& s, const float* in8, const Hit& h, uint pathId, uint R0, float tmin, float* out8)
+{
+	if (h.prim < 0) return false;
+	const Instance& I = s.inst[h.inst];
+	const Mesh& m = s.meshes[I.mesh];
+	const f3 v0 = m.v[h.prim * 3], v1 = m.v[h.prim * 3 + 1], v2 = m.v[h.prim * 3 + 2];
+	f3 N = cross(v1 - v0, v2 - v0);
+	const float* inv = I.inv;
+	N = normalize(N.x * mk3(inv[0], inv[1], inv[2]) + N.y * mk3(inv[4], inv[5], inv[6]) + N.z * mk3(inv[8], inv[9], inv[10]));
+	const f3 O = mk3(in8[0], in8[1], in8[2]), D = mk3(in8[4], in8[5], in8[6]);
+	if (dot(N, D) > 0) N = N * -1.0f;
+	const f3 P = O + h.t * D;
+	uint seed = wang_hash(pathId * 17u + R0);
+	const float r0 = rand_float(seed), r1 = rand_float(seed);
+	const f3 R = normalize(tangent2world(cosine_sample(r0, r1), N));
+	out8[0] = P.x, out8[1] = P.y, out8[2] = P.z, out8[3] = tmin;
+	out8[4] = R.x, out8[5] = R.y, out8[6] = R.z, out8[7] = 1e34f;
+	return true;
+}
+
+} // namespace orc
