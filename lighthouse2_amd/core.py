@@ -64,6 +64,12 @@ def library():
     """Loads librendercore_hip.so (built in-tree by __graft_entry__.build())."""
     global _lib
     if _lib is None:
+        # PyTorch ships its own libamdhip64 (same soname): load it first so the
+        # core binds to that runtime and device pointers are shared.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
         lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
