@@ -217,7 +217,7 @@ void RenderCore::Setting(const char* name, float value)
 		nodeFormat = (int)value;
 		instancesDirty = true;
 	}
-	else if (!strcmp(name, "seed")) seedSetting = (uint32_t)value;
+	else if (!strcmp(name, "seed")) seedSetting = camSeed = (uint32_t)value; /* replaces camRNGseed's 0x12345678 */
 	/* unknown keys are ignored, as in the reference cores */
 }
 

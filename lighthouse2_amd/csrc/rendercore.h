@@ -139,7 +139,7 @@ private:
 	/* render state */
 	int scrw = 0, scrh = 0, scrspp = 1;
 	int samplesTaken = 0;
-	bool firstConvergingFrame = true;
+	bool firstConvergingFrame = false;   /* rendercore.h:134 */
 	lighthouse2::int2 probePos = { 0, 0 };
 	DevBuf materials, triLights, pointLights, spotLights, dirLights;
 	int lightCounts[4] = { 0, 0, 0, 0 };

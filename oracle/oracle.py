@@ -129,15 +129,30 @@ def xor32_host(seed):
     return s, s
 
 
-def render(scene_or_oracle, view, w, h, spp=1, frames=1, max_depth=3, eps=1e-3, seed=0x12345678, threads=8):
-    """Oracle image: `frames` Render calls (Restart, then Converge) of spp
-    samples. Returns (acc32 float32 (h,w,4) sum, acc64 float64 sum, samplesTaken)."""
+RESTART, CONVERGE = 1, 0
+
+
+def render(scene_or_oracle, view, w, h, spp=1, frames=1, max_depth=3, eps=1e-3, seed=0x12345678, threads=8, modes=None):
+    """Oracle of a sequence of Render calls. `modes` lists Restart/Converge per
+    call (default: one Restart, then `frames` Converge calls). Restates the
+    convergence state machine of RenderCore::Render
+    (lib/RenderCore_PrimeRef/rendercore.cpp:543-550; rendercore.h:129-134):
+    a Restart, or the first Converge call after it, clears the accumulator and
+    reseeds the camera RNG. Returns (acc32 (h,w,4) float32, acc64 float64,
+    samplesTaken) as they stand after the last call."""
     o = scene_or_oracle if isinstance(scene_or_oracle, OracleScene) else OracleScene(scene_or_oracle)
+    if modes is None:
+        modes = [RESTART] + [CONVERGE] * frames
     acc32 = np.zeros((w * h, 4), np.float32)
     acc64 = np.zeros((w * h, 4), np.float64)
-    cam = seed
-    taken = 0
-    for _ in range(frames):
+    cam, taken, first = seed, 0, False
+    for m in modes:
+        if m == RESTART or first:
+            acc32[:] = 0
+            acc64[:] = 0
+            taken, first, cam = 0, True, seed
+        if m == CONVERGE:
+            first = False
         R0, cam = xor32_host(cam)
         pa = o.render_frame(view, w, h, spp, max_depth, eps, R0, taken, threads)
         for s in range(spp):
