@@ -21,7 +21,8 @@ def render_gpu(scene, view, w, h, spp, frames, depth, eps=1e-3, seed=0x12345678,
     core.SetTarget(w, h, spp)
     core.Setting("maxDepth", depth)
     core.Setting("epsilon", eps)
-    core.Setting("seed", seed)
+    if seed != 0x12345678:
+        core.Setting("seed", seed)         # a float setting: seeds up to 2^24 are exact
     core.Render(view, lh.LH2_RESTART)          # camera moved: preview frame
     for f in range(frames):
         core.Render(view, lh.LH2_CONVERGE)     # the first converging frame restarts accumulation
