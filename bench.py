@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--format", choices=["q8", "f32"], default="q8")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent traversal blocks per CU (0: core default)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (FETCH_SIZE/WRITE_SIZE calibration)")
     return ap.parse_args()
 
 
@@ -93,11 +96,21 @@ def main():
     from lighthouse2_amd import scenes
 
     torch.cuda.set_device(0)
+    if a.copy_bytes:
+        src = torch.ones(a.copy_bytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.copy_bytes, "dispatches": 3}))
+        return
     t0 = time.time()
     verts = scenes.random_triangles(a.tris, seed=1)
     t_gen = time.time() - t0
     core = lh.RenderCore()
     core.Setting("nodeFormat", 1 if a.format == "q8" else 0)
+    if a.blocks_per_cu:
+        core.Setting("traceBlocksPerCU", a.blocks_per_cu)
     t0 = time.time()
     core.SetGeometry(0, verts, None)
     core.SetInstance(0, 0)
@@ -165,7 +178,8 @@ def main():
         "data": "synthetic (seeded random triangles, generated on the host)",
         "config": {"workload": "incoherent traversal: 4M random triangles, 1920x1080 primary + 3 diffuse bounces",
                    "triangles": a.tris, "width": a.width, "height": a.height, "node_format": a.format,
-                   "bvh": "binned SAH 16 bins, leaf <= 4, GPU build", "seed": 1},
+                   "bvh": "binned SAH 16 bins, leaf <= 4, GPU build", "seed": 1,
+                   "blocks_per_cu": a.blocks_per_cu or 8},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 2),

@@ -44,7 +44,8 @@ LH2HIP_API int lh2hip_destroy(lh2hip_core* core);
 LH2HIP_API int lh2hip_set_target(lh2hip_core* core, int width, int height, int spp);
 /* CoreAPI_Base::Setting (rendercore.cpp:484-502; keys "epsilon", "clampValue",
  * plus "maxDepth", "nodeFormat" (0 fp32 BVH2, 1 q8 BVH4), "seed": camera RNG
- * seed replacing 0x12345678, exact for values up to 2^24). */
+ * seed replacing 0x12345678, exact for values up to 2^24), "traceBlocksPerCU":
+ * persistent traversal blocks of 256 threads per compute unit, default 8). */
 LH2HIP_API int lh2hip_setting(lh2hip_core* core, const char* name, float value);
 /* CoreAPI_Base::SetGeometry (rendercore.cpp:214-222; core_mesh.cpp:36-64).
  * vertexData: 3 float4 per triangle (w ignored); builds the mesh BVH on the GPU. */

@@ -285,37 +285,69 @@ __global__ void k_decide(BuildNode* __restrict__ nodes, int levelBegin, int leve
 	cursors[i] = make_int2(start, start + leftCount);
 }
 
+/* Partition of split nodes. Primitives of a node are contiguous, so most
+ * waves hold one node: they take their left and right slots with one atomic
+ * per side (ballot + mbcnt ranks the lanes); mixed waves use one atomic per
+ * lane. Positions inside a child are therefore arbitrary; the child's set is
+ * not. */
 __global__ void __launch_bounds__(BLOCK) k_partition(int* __restrict__ posNode, const uint32_t* __restrict__ prims,
 	const float4* __restrict__ lo, const float4* __restrict__ hi, int n, int levelBegin, int levelCount,
 	const int* __restrict__ scr, const int4* __restrict__ dec, int2* __restrict__ cursors, const BuildNode* __restrict__ nodes,
 	uint32_t* __restrict__ tmpPrims, int* __restrict__ tmpNode)
 {
 	const int p = blockIdx.x * BLOCK + threadIdx.x;
-	if (p >= n) return;
-	const int nd = posNode[p];
+	const int lane = __lane_id();
+	const int nd = p < n ? posNode[p] : -1;
 	const int local = nd - levelBegin;
-	if (nd < 0 || local < 0 || local >= levelCount) return;
-	const int4 dc = dec[local];
-	if (dc.x < 0) { posNode[p] = -1; return; }
-	const uint32_t prim = prims[p];
-	bool left;
-	int np;
-	if (dc.x == 3)
+	const bool in = nd >= 0 && local >= 0 && local < levelCount;
+	int4 dc = make_int4(-1, 0, 0, 0);
+	if (in) dc = dec[local];
+	if (in && dc.x < 0) posNode[p] = -1; /* became a leaf: finished */
+	const bool part = in && dc.x >= 0;
+	uint32_t prim = 0;
+	bool left = false;
+	if (part)
 	{
-		left = (p - nodes[dc.w].leftFirst) < dc.z;
-		np = p;
+		prim = prims[p];
+		if (dc.x == 3) left = (p - nodes[dc.w].leftFirst) < dc.z;
+		else
+		{
+			const int* d = scr + (int64_t)local * S_STRIDE;
+			float cmin, scale;
+			axis_scale(d, dc.x, cmin, scale);
+			const v3 c = centroid(lo, hi, prim);
+			left = bin_of(comp(c, dc.x), cmin, scale) <= dc.y;
+		}
 	}
-	else
+	const bool binned = part && dc.x != 3;
+	const uint64_t bm = __ballot(binned);
+	int np = p;
+	if (bm)
 	{
-		const int* d = scr + (int64_t)local * S_STRIDE;
-		float cmin, scale;
-		axis_scale(d, dc.x, cmin, scale);
-		const v3 c = centroid(lo, hi, prim);
-		left = bin_of(comp(c, dc.x), cmin, scale) <= dc.y;
-		np = left ? atomicAdd(&cursors[local].x, 1) : atomicAdd(&cursors[local].y, 1);
+		const int leader = __ffsll((unsigned long long)bm) - 1;
+		const int l0 = __builtin_amdgcn_readlane(local, leader);
+		const bool same = __ballot(binned && local == l0) == bm;
+		if (same)
+		{
+			const uint64_t lm = __ballot(binned && left), rm = bm & ~lm;
+			int bl = 0, br = 0;
+			if (lane == leader)
+			{
+				if (lm) bl = atomicAdd(&cursors[l0].x, __popcll(lm));
+				if (rm) br = atomicAdd(&cursors[l0].y, __popcll(rm));
+			}
+			bl = __builtin_amdgcn_readlane(bl, leader), br = __builtin_amdgcn_readlane(br, leader);
+			const uint64_t mine = left ? lm : rm;
+			const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mine >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
+			if (binned) np = (left ? bl : br) + rank;
+		}
+		else if (binned) np = left ? atomicAdd(&cursors[local].x, 1) : atomicAdd(&cursors[local].y, 1);
 	}
-	tmpPrims[np] = prim;
-	tmpNode[np] = left ? dc.w : dc.w + 1;
+	if (part)
+	{
+		tmpPrims[np] = prim;
+		tmpNode[np] = left ? dc.w : dc.w + 1;
+	}
 }
 
 __global__ void __launch_bounds__(BLOCK) k_scatter(int* __restrict__ posNode, uint32_t* __restrict__ prims, int n,

@@ -218,6 +218,13 @@ void RenderCore::Setting(const char* name, float value)
 		instancesDirty = true;
 	}
 	else if (!strcmp(name, "seed")) seedSetting = camSeed = (uint32_t)value; /* replaces camRNGseed's 0x12345678 */
+	else if (!strcmp(name, "traceBlocksPerCU"))
+	{
+		if (!(value >= 1 && value <= 64)) throw CoreError(LH2HIP_ERR_INVALID, "Setting traceBlocksPerCU: 1..64");
+		hipCheck(hipStreamSynchronize(stream), "sync");
+		persistentBlocks = deviceCUs * (int)value;
+		overflow.ensure(trace_overflow_bytes(persistentBlocks));
+	}
 	/* unknown keys are ignored, as in the reference cores */
 }
 

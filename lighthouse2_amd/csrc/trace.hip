@@ -76,8 +76,19 @@ __device__ __forceinline__ void cswap(float& a, float& b, uint32_t& ra, uint32_t
 	a = ta, b = tb, ra = qa, rb = qb;
 }
 
+static const uint32_t REF_DONE = 0xffffffffu;
+
+/* Direction component kept away from zero for the box tests only (the
+ * triangle test uses the exact direction): |D| >= 2^-66 keeps 1/D finite. */
+__device__ __forceinline__ float safe_rcp(float d) { return rcp(fabsf(d) < 1.3552527e-20f ? copysignf(1.3552527e-20f, d) : d); }
+
+__device__ __forceinline__ float ubyte(uint32_t w, int c)
+{
+	return (float)((w >> (8 * c)) & 255u);
+}
+
 template <int FMT, int MODE, bool TWO, bool COUNT>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_trace(const TraceScene sc, const float4* __restrict__ rays, void* __restrict__ out,
+__global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceScene sc, const float4* __restrict__ rays, void* __restrict__ out,
 	int maxCount, const uint32_t* __restrict__ countDev, uint32_t* __restrict__ counter, uint32_t* __restrict__ overflow, TraceStats* __restrict__ stats)
 {
 	__shared__ uint32_t sstack[TRACE_LDS_STACK * TRACE_BLOCK];
@@ -91,10 +102,11 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(const TraceScene sc, cons
 	L.nodes = sc.nodes, L.tris = sc.tris;
 	int rayIdx = -1;
 	bool active = false, exhausted = false;
-	uint32_t ref = 0;
+	uint32_t ref = REF_DONE, leaf = REF_DONE;
 	unsigned long long cn = 0, ct = 0;
 	while (true)
 	{
+		/* refill: one atomic per wave hands out rays to the idle lanes */
 		const uint64_t idle = __ballot(!active);
 		const int nIdle = __popcll(idle);
 		if (!exhausted && nIdle >= 16)
@@ -116,8 +128,9 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(const TraceScene sc, cons
 					L.tmin = o4.w, L.tbest = d4.w;
 					L.binst = -1, L.bprim = -1, L.bU = L.bV = 0, L.bdet = 1;
 					L.O = L.wO, L.D = L.wD;
-					L.inv = mk3(rcp(L.D.x), rcp(L.D.y), rcp(L.D.z));
+					L.inv = mk3(safe_rcp(L.D.x), safe_rcp(L.D.y), safe_rcp(L.D.z));
 					st.sp = 0;
+					leaf = REF_DONE;
 					if (TWO) { L.inst = -1; ref = sc.tlasRoot | REF_TLAS; }
 					else { L.rp = ray_setup(L.D); ref = sc.root; }
 					active = true;
@@ -126,51 +139,133 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(const TraceScene sc, cons
 		}
 		if (!__any(active)) break;
 		if (!active) continue;
-		/* traverse until this ray completes */
 		bool done = false, finished = false;
 		while (true)
 		{
 			/* leave to refill when a quarter of the wave has gone idle */
 			if (!exhausted && __popcll(__ballot(true)) < 48) break;
-			if (TWO && (ref & REF_TLAS))
+			/* node phase: walk interior nodes; the first leaf reached is
+			 * postponed and the lane keeps walking until every lane of the
+			 * wave holds a leaf (Aila & Laine 2009, "while-while" with
+			 * postponed leaves), so leaf tests run with the wave converged.
+			 * A lane never crosses into the top level holding a leaf. */
+			while (true)
 			{
-				if (L.inst >= 0)
-				{	/* back at the top level: restore the world ray */
-					L.O = L.wO, L.D = L.wD;
-					L.inv = mk3(rcp(L.D.x), rcp(L.D.y), rcp(L.D.z));
-					L.inst = -1;
-				}
-				const uint32_t r = ref & ~REF_TLAS;
-				if (r & REF_LEAF)
-				{	/* enter an instance */
-					const int id = (int)(r & REF_MASK);
-					const InstanceRec& I = sc.inst[id];
-					if (!(I.flags & 1u))
-					{
-						const float4 a = I.inv0, b = I.inv1, c = I.inv2;
-						const v3 wO = L.wO, wD = L.wD;
-						L.O = mk3(a.x * wO.x + a.y * wO.y + a.z * wO.z + a.w, b.x * wO.x + b.y * wO.y + b.z * wO.z + b.w, c.x * wO.x + c.y * wO.y + c.z * wO.z + c.w);
-						L.D = mk3(a.x * wD.x + a.y * wD.y + a.z * wD.z, b.x * wD.x + b.y * wD.y + b.z * wD.z, c.x * wD.x + c.y * wD.y + c.z * wD.z);
-						L.inv = mk3(rcp(L.D.x), rcp(L.D.y), rcp(L.D.z));
+				const bool top = TWO && (ref & REF_TLAS) && ref != REF_DONE;
+				const bool inner = !(ref & (REF_LEAF | REF_TLAS));
+				if (!(inner || (top && leaf == REF_DONE))) break;
+				if (top)
+				{
+					if (L.inst >= 0)
+					{	/* back at the top level: restore the world ray */
+						L.O = L.wO, L.D = L.wD;
+						L.inv = mk3(safe_rcp(L.D.x), safe_rcp(L.D.y), safe_rcp(L.D.z));
+						L.inst = -1;
 					}
-					L.rp = ray_setup(L.D);
-					L.inst = id;
-					L.nodes = I.nodes, L.tris = I.tris;
-					ref = I.root;
-					continue;
+					const uint32_t r = ref & ~REF_TLAS;
+					if (r & REF_LEAF)
+					{	/* enter an instance */
+						const int id = (int)(r & REF_MASK);
+						const InstanceRec& I = sc.inst[id];
+						if (!(I.flags & 1u))
+						{
+							const float4 a = I.inv0, b = I.inv1, c = I.inv2;
+							const v3 wO = L.wO, wD = L.wD;
+							L.O = mk3(a.x * wO.x + a.y * wO.y + a.z * wO.z + a.w, b.x * wO.x + b.y * wO.y + b.z * wO.z + b.w, c.x * wO.x + c.y * wO.y + c.z * wO.z + c.w);
+							L.D = mk3(a.x * wD.x + a.y * wD.y + a.z * wD.z, b.x * wD.x + b.y * wD.y + b.z * wD.z, c.x * wD.x + c.y * wD.y + c.z * wD.z);
+							L.inv = mk3(safe_rcp(L.D.x), safe_rcp(L.D.y), safe_rcp(L.D.z));
+						}
+						L.rp = ray_setup(L.D);
+						L.inst = id;
+						L.nodes = I.nodes, L.tris = I.tris;
+						ref = I.root;
+					}
+					else
+					{
+						const NodeF32 n = sc.tlas[r];
+						float tl, tr;
+						const bool hl = slab(n.lxy.x, n.lxy.y, n.lxy.z, n.lxy.w, n.lrz.x, n.lrz.y, L.O, L.inv, L.tmin, L.tbest, tl);
+						const bool hr = slab(n.rxy.x, n.rxy.y, n.rxy.z, n.rxy.w, n.lrz.z, n.lrz.w, L.O, L.inv, L.tmin, L.tbest, tr);
+						const uint32_t rl = n.refs.x | REF_TLAS, rr = n.refs.y | REF_TLAS;
+						if (hl && hr) { if (tr < tl) { st.push(rl); ref = rr; } else { st.push(rr); ref = rl; } }
+						else if (hl) ref = rl;
+						else if (hr) ref = rr;
+						else ref = st.sp ? st.pop() : REF_DONE;
+					}
 				}
-				const NodeF32 n = sc.tlas[r];
-				float tl, tr;
-				const bool hl = slab(n.lxy.x, n.lxy.y, n.lxy.z, n.lxy.w, n.lrz.x, n.lrz.y, L.O, L.inv, L.tmin, L.tbest, tl);
-				const bool hr = slab(n.rxy.x, n.rxy.y, n.rxy.z, n.rxy.w, n.lrz.z, n.lrz.w, L.O, L.inv, L.tmin, L.tbest, tr);
-				const uint32_t rl = n.refs.x | REF_TLAS, rr = n.refs.y | REF_TLAS;
-				if (hl && hr) { if (tr < tl) { st.push(rl); ref = rr; } else { st.push(rr); ref = rl; } continue; }
-				if (hl) { ref = rl; continue; }
-				if (hr) { ref = rr; continue; }
+				else if (FMT == FMT_F32)
+				{
+					if (COUNT) cn++;
+					const NodeF32 n = ((const NodeF32*)L.nodes)[ref];
+					float tl, tr;
+					const bool hl = slab(n.lxy.x, n.lxy.y, n.lxy.z, n.lxy.w, n.lrz.x, n.lrz.y, L.O, L.inv, L.tmin, L.tbest, tl);
+					const bool hr = slab(n.rxy.x, n.rxy.y, n.rxy.z, n.rxy.w, n.lrz.z, n.lrz.w, L.O, L.inv, L.tmin, L.tbest, tr);
+					if (hl && hr) { if (tr < tl) { st.push(n.refs.x); ref = n.refs.y; } else { st.push(n.refs.y); ref = n.refs.x; } }
+					else if (hl) ref = n.refs.x;
+					else if (hr) ref = n.refs.y;
+					else ref = st.sp ? st.pop() : REF_DONE;
+				}
+				else
+				{
+					/* 4-wide node, child planes = origin + q * 2^e per axis. Per node
+					 * and axis: b = (origin - O) / D and d = 2^e / D; a plane is at
+					 * t = q * d + b. Near and far planes are picked by the sign of
+					 * D; e = 2^-20 (|b| + 255 |d|) bounds the rounding of the
+					 * reformulation and is subtracted from near and added to far
+					 * planes, so every box the exact interval reaches is kept. */
+					if (COUNT) cn++;
+					const NodeQ8 n = ((const NodeQ8*)L.nodes)[ref];
+					const uint32_t e = fbits(n.origin.w);
+					const float dx = bitsf((e & 255u) << 23) * L.inv.x;
+					const float dy = bitsf(((e >> 8) & 255u) << 23) * L.inv.y;
+					const float dz = bitsf(((e >> 16) & 255u) << 23) * L.inv.z;
+					const float bx = (n.origin.x - L.O.x) * L.inv.x;
+					const float by = (n.origin.y - L.O.y) * L.inv.y;
+					const float bz = (n.origin.z - L.O.z) * L.inv.z;
+					const float ex = (fabsf(bx) + 255.0f * fabsf(dx)) * 9.5367431640625e-7f;
+					const float ey = (fabsf(by) + 255.0f * fabsf(dy)) * 9.5367431640625e-7f;
+					const float ez = (fabsf(bz) + 255.0f * fabsf(dz)) * 9.5367431640625e-7f;
+					const float bnx = bx - ex, bfx = bx + ex, bny = by - ey, bfy = by + ey, bnz = bz - ez, bfz = bz + ez;
+					const bool px = L.inv.x >= 0.0f, py = L.inv.y >= 0.0f, pz = L.inv.z >= 0.0f;
+					const uint32_t qnx = px ? n.qa.x : n.qa.w, qfx = px ? n.qa.w : n.qa.x;
+					const uint32_t qny = py ? n.qa.y : n.qb.x, qfy = py ? n.qb.x : n.qa.y;
+					const uint32_t qnz = pz ? n.qa.z : n.qb.y, qfz = pz ? n.qb.y : n.qa.z;
+					const int nc = (int)(e >> 24);
+					float tk[4];
+					uint32_t rk[4] = { n.refs.x, n.refs.y, n.refs.z, n.refs.w };
+#pragma unroll
+					for (int c = 0; c < 4; c++)
+					{
+						const float tn = fmaxf(fmaxf(fmaf(ubyte(qnx, c), dx, bnx), fmaf(ubyte(qny, c), dy, bny)), fmaxf(fmaf(ubyte(qnz, c), dz, bnz), L.tmin));
+						const float tf = fminf(fminf(fmaf(ubyte(qfx, c), dx, bfx), fmaf(ubyte(qfy, c), dy, bfy)), fminf(fmaf(ubyte(qfz, c), dz, bfz), L.tbest));
+						tk[c] = (c < nc && tn <= tf) ? tn : __builtin_huge_valf();
+					}
+					cswap(tk[0], tk[1], rk[0], rk[1]);
+					cswap(tk[2], tk[3], rk[2], rk[3]);
+					cswap(tk[0], tk[2], rk[0], rk[2]);
+					cswap(tk[1], tk[3], rk[1], rk[3]);
+					cswap(tk[1], tk[2], rk[1], rk[2]);
+					if (tk[0] < __builtin_huge_valf())
+					{
+						if (tk[3] < __builtin_huge_valf()) st.push(rk[3]);
+						if (tk[2] < __builtin_huge_valf()) st.push(rk[2]);
+						if (tk[1] < __builtin_huge_valf()) st.push(rk[1]);
+						ref = rk[0];
+					}
+					else ref = st.sp ? st.pop() : REF_DONE;
+				}
+				/* postpone the first leaf reached */
+				if (leaf == REF_DONE && (ref & (REF_LEAF | REF_TLAS)) == REF_LEAF)
+				{
+					leaf = ref;
+					ref = st.sp ? st.pop() : REF_DONE;
+				}
+				if (!__any(leaf == REF_DONE)) break;
 			}
-			else if (ref & REF_LEAF)
+			/* leaf phase: the postponed leaf, then any leaves that follow */
+			while (leaf != REF_DONE)
 			{
-				uint32_t i = ref & REF_MASK;
+				uint32_t i = leaf & REF_MASK;
 				const int instId = TWO ? L.inst : 0;
 				while (true)
 				{
@@ -192,60 +287,15 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(const TraceScene sc, cons
 					i++;
 				}
 				if (done) break;
-			}
-			else if (FMT == FMT_F32)
-			{
-				if (COUNT) cn++;
-				const NodeF32 n = ((const NodeF32*)L.nodes)[ref];
-				float tl, tr;
-				const bool hl = slab(n.lxy.x, n.lxy.y, n.lxy.z, n.lxy.w, n.lrz.x, n.lrz.y, L.O, L.inv, L.tmin, L.tbest, tl);
-				const bool hr = slab(n.rxy.x, n.rxy.y, n.rxy.z, n.rxy.w, n.lrz.z, n.lrz.w, L.O, L.inv, L.tmin, L.tbest, tr);
-				if (hl && hr) { if (tr < tl) { st.push(n.refs.x); ref = n.refs.y; } else { st.push(n.refs.y); ref = n.refs.x; } continue; }
-				if (hl) { ref = n.refs.x; continue; }
-				if (hr) { ref = n.refs.y; continue; }
-			}
-			else
-			{
-				if (COUNT) cn++;
-				const NodeQ8 n = ((const NodeQ8*)L.nodes)[ref];
-				const uint32_t e = fbits(n.origin.w);
-				const float sx = bitsf((e & 255u) << 23), sy = bitsf(((e >> 8) & 255u) << 23), sz = bitsf(((e >> 16) & 255u) << 23);
-				const int nc = (int)(e >> 24);
-				float tk[4];
-				uint32_t rk[4] = { n.refs.x, n.refs.y, n.refs.z, n.refs.w };
-#pragma unroll
-				for (int c = 0; c < 4; c++)
+				leaf = REF_DONE;
+				if ((ref & (REF_LEAF | REF_TLAS)) == REF_LEAF)
 				{
-					const int sh = 8 * c;
-					const float lx = fmaf((float)((n.qa.x >> sh) & 255u), sx, n.origin.x);
-					const float ly = fmaf((float)((n.qa.y >> sh) & 255u), sy, n.origin.y);
-					const float lz = fmaf((float)((n.qa.z >> sh) & 255u), sz, n.origin.z);
-					const float hx = fmaf((float)((n.qa.w >> sh) & 255u), sx, n.origin.x);
-					const float hy = fmaf((float)((n.qb.x >> sh) & 255u), sy, n.origin.y);
-					const float hz = fmaf((float)((n.qb.y >> sh) & 255u), sz, n.origin.z);
-					float tn;
-					const bool h = c < nc && slab(lx, hx, ly, hy, lz, hz, L.O, L.inv, L.tmin, L.tbest, tn);
-					tk[c] = h ? tn : __builtin_huge_valf();
-				}
-				cswap(tk[0], tk[1], rk[0], rk[1]);
-				cswap(tk[2], tk[3], rk[2], rk[3]);
-				cswap(tk[0], tk[2], rk[0], rk[2]);
-				cswap(tk[1], tk[3], rk[1], rk[3]);
-				cswap(tk[1], tk[2], rk[1], rk[2]);
-				if (tk[0] < __builtin_huge_valf())
-				{
-					if (tk[3] < __builtin_huge_valf()) st.push(rk[3]);
-					if (tk[2] < __builtin_huge_valf()) st.push(rk[2]);
-					if (tk[1] < __builtin_huge_valf()) st.push(rk[1]);
-					ref = rk[0];
-					continue;
+					leaf = ref;
+					ref = st.sp ? st.pop() : REF_DONE;
 				}
 			}
-			/* pop */
-			if (st.sp == 0) { finished = true; break; }
-			ref = st.pop();
+			if (done || ref == REF_DONE) { finished = true; break; }
 		}
-		if (done) finished = true;
 		if (!finished) continue;
 		/* write the result of ray rayIdx */
 		if (MODE == MODE_CLOSEST)
@@ -264,6 +314,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace(const TraceScene sc, cons
 			((uint32_t*)out)[rayIdx] = L.bprim >= 0 ? 1u : 0u;
 		}
 		active = false;
+		leaf = REF_DONE;
 	}
 	if (COUNT)
 	{
