@@ -40,5 +40,6 @@ static const int TRACE_BLOCK = 256;
 static const int TRACE_LDS_STACK = 16;
 static const int TRACE_OVF_STACK = 112;
 static const int TRACE_MAX_STACK = TRACE_LDS_STACK + TRACE_OVF_STACK;
+static const uint32_t TRACE_MAX_STEPS = 1u << 26;   /* node visits per ray (safety bound) */
 
 } // namespace lh

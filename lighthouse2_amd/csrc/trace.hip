@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 	L.nodes = sc.nodes, L.tris = sc.tris;
 	int rayIdx = -1;
 	bool active = false, exhausted = false;
-	uint32_t ref = REF_DONE, leaf = REF_DONE;
+	uint32_t ref = REF_DONE, leaf = REF_DONE, steps = 0;
 	unsigned long long cn = 0, ct = 0;
 	while (true)
 	{
@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 					L.O = L.wO, L.D = L.wD;
 					L.inv = mk3(safe_rcp(L.D.x), safe_rcp(L.D.y), safe_rcp(L.D.z));
 					st.sp = 0;
+					steps = 0;
 					leaf = REF_DONE;
 					if (TWO) { L.inst = -1; ref = sc.tlasRoot | REF_TLAS; }
 					else { L.rp = ray_setup(L.D); ref = sc.root; }
@@ -144,6 +145,13 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 		{
 			/* leave to refill when a quarter of the wave has gone idle */
 			if (!exhausted && __popcll(__ballot(true)) < 48) break;
+			/* a leaf as the current ref (a root that is a leaf, or a ref
+			 * popped as the node loop ended) is handed to the leaf phase */
+			if (leaf == REF_DONE && (ref & (REF_LEAF | REF_TLAS)) == REF_LEAF)
+			{
+				leaf = ref;
+				ref = st.sp ? st.pop() : REF_DONE;
+			}
 			/* node phase: walk interior nodes; the first leaf reached is
 			 * postponed and the lane keeps walking until every lane of the
 			 * wave holds a leaf (Aila & Laine 2009, "while-while" with
@@ -154,6 +162,11 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 				const bool top = TWO && (ref & REF_TLAS) && ref != REF_DONE;
 				const bool inner = !(ref & (REF_LEAF | REF_TLAS));
 				if (!(inner || (top && leaf == REF_DONE))) break;
+				if (++steps > TRACE_MAX_STEPS)
+				{	/* safety bound: a ray never visits more nodes than the trees hold */
+					ref = REF_DONE, leaf = REF_DONE, st.sp = 0;
+					break;
+				}
 				if (top)
 				{
 					if (L.inst >= 0)
