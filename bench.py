@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent traversal blocks per CU (0: core default)")
+    ap.add_argument("--min-rays-per-lane", type=float, default=0.0, help="shrink the grid of small waves (0: off)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (FETCH_SIZE/WRITE_SIZE calibration)")
     return ap.parse_args()
@@ -111,6 +112,8 @@ def main():
     core.Setting("nodeFormat", 1 if a.format == "q8" else 0)
     if a.blocks_per_cu:
         core.Setting("traceBlocksPerCU", a.blocks_per_cu)
+    if a.min_rays_per_lane:
+        core.Setting("traceMinRaysPerLane", a.min_rays_per_lane)
     t0 = time.time()
     core.SetGeometry(0, verts, None)
     core.SetInstance(0, 0)
@@ -143,10 +146,12 @@ def main():
     wv.step(keep=keep)
     torch.cuda.synchronize()
     nodes = tris = nrays = 0
+    it = np.zeros(4)
     scratch = torch.zeros_like(wv.hits)
     for r, _h in keep:
-        n_, t_ = core.trace_closest_counted(r.data_ptr(), scratch.data_ptr(), len(r), wv.sh)
-        nodes, tris, nrays = nodes + n_, tris + t_, nrays + len(r)
+        c = core.trace_closest_counted(r.data_ptr(), scratch.data_ptr(), len(r), wv.sh)
+        nodes, tris, nrays = nodes + c[0], tris + c[1], nrays + len(r)
+        it += np.array(c[2:], np.float64)
     nodes_per_ray, tris_per_ray = nodes / nrays, tris / nrays
     bytes_per_ray = RAY_IO_BYTES + nodes_per_ray * NODE_BYTES[a.format] + tris_per_ray * TRI_BYTES
     launch_ms = [m for st in ms for m in st[1:]]
@@ -179,7 +184,7 @@ def main():
         "config": {"workload": "incoherent traversal: 4M random triangles, 1920x1080 primary + 3 diffuse bounces",
                    "triangles": a.tris, "width": a.width, "height": a.height, "node_format": a.format,
                    "bvh": "binned SAH 16 bins, leaf <= 4, GPU build", "seed": 1,
-                   "blocks_per_cu": a.blocks_per_cu or 8},
+                   "blocks_per_cu": a.blocks_per_cu or 8, "min_rays_per_lane": a.min_rays_per_lane},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 2),
@@ -189,6 +194,12 @@ def main():
         "primary_mrays_s": round(prim_rays / (prim_ms * 1e-3) / 1e6, 2),
         "incoherent_rays_per_step": int(inco_rays / a.steps),
         "extend_ms_per_launch": round(float(np.mean(launch_ms)), 4),
+        "per_wave": [{"wave": w + 1, "rays": int(np.mean([c[w] for c in per_step])),
+                      "ms": round(float(np.mean([m[w] for m in ms])), 4),
+                      "mrays_s": round(float(np.mean([c[w] for c in per_step]) / np.mean([m[w] for m in ms]) / 1e3), 1)}
+                     for w in range(4)],
+        "simd_utilisation": {"node_loop": round(it[1] / max(64 * it[0], 1), 3), "leaf_loop": round(it[3] / max(64 * it[2], 1), 3),
+                             "node_iters_per_ray": round(it[0] * 64 / nrays, 2), "leaf_iters_per_ray": round(it[2] * 64 / nrays, 2)},
         "wall_mrays_s_all_waves": round((inco_rays + prim_rays) / wall / 1e6, 2),
         "bvh": {"build_s": round(t_build, 3), "scene_gen_s": round(t_gen, 3), **info},
     }

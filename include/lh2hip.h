@@ -45,7 +45,9 @@ LH2HIP_API int lh2hip_set_target(lh2hip_core* core, int width, int height, int s
 /* CoreAPI_Base::Setting (rendercore.cpp:484-502; keys "epsilon", "clampValue",
  * plus "maxDepth", "nodeFormat" (0 fp32 BVH2, 1 q8 BVH4), "seed": camera RNG
  * seed replacing 0x12345678, exact for values up to 2^24), "traceBlocksPerCU":
- * persistent traversal blocks of 256 threads per compute unit, default 8). */
+ * persistent traversal blocks of 256 threads per compute unit, default 8;
+ * "traceMinRaysPerLane": shrink the persistent grid of a small wave so each
+ * lane gets at least this many rays, 0 = off). */
 LH2HIP_API int lh2hip_setting(lh2hip_core* core, const char* name, float value);
 /* CoreAPI_Base::SetGeometry (rendercore.cpp:214-222; core_mesh.cpp:36-64).
  * vertexData: 3 float4 per triangle (w ignored); builds the mesh BVH on the GPU. */
@@ -94,11 +96,13 @@ LH2HIP_API int lh2hip_trace_closest(lh2hip_core* core, const Ray4* rays, HitReco
  * (rendercore.cpp:598-607): bit i of occluded[i/32] set when ray i is blocked. */
 LH2HIP_API int lh2hip_trace_any(lh2hip_core* core, const Ray4* rays, uint32_t* occluded, int count, void* stream);
 
-/* Closest hit as lh2hip_trace_closest, with the traversal's work counted:
- * out2[0] += interior nodes visited, out2[1] += triangles tested (host memory).
- * A separate kernel variant; used to derive algorithmic bytes per ray. */
+/* Closest hit as lh2hip_trace_closest, with the traversal's work counted
+ * (host memory, accumulated): out6[0] interior nodes visited, [1] triangles
+ * tested, [2]/[3] node-loop iterations of all waves / lanes active in them,
+ * [4]/[5] the same for the leaf loop. A separate kernel variant; used to derive
+ * algorithmic bytes per ray and SIMD utilisation. */
 LH2HIP_API int lh2hip_trace_closest_counted(lh2hip_core* core, const Ray4* rays, HitRecord* hits, int count, void* stream,
-	uint64_t* out2);
+	uint64_t* out6);
 
 /* ---- microbenchmark wave helpers (incoherent traversal workload) ----------
  * Primary rays: generateEyeRaysKernel (lib/RenderCore_PrimeRef/kernels/camera.h:39-68)

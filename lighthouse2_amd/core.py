@@ -180,10 +180,11 @@ class RenderCore:
         _check(library().lh2hip_trace_closest(self._h, rays_ptr, hits_ptr, count, stream))
 
     def trace_closest_counted(self, rays_ptr, hits_ptr, count, stream=None):
-        """As trace_closest; returns (interior nodes visited, triangles tested)."""
-        out = np.zeros(2, np.uint64)
+        """As trace_closest; returns (nodes visited, triangles tested, node-loop
+        wave iterations, active lanes in them, leaf-loop iterations, lanes)."""
+        out = np.zeros(6, np.uint64)
         _check(library().lh2hip_trace_closest_counted(self._h, rays_ptr, hits_ptr, count, stream, _p(out)))
-        return int(out[0]), int(out[1])
+        return tuple(int(x) for x in out)
 
     def trace_any(self, rays_ptr, mask_ptr, count, stream=None):
         _check(library().lh2hip_trace_any(self._h, rays_ptr, mask_ptr, count, stream))

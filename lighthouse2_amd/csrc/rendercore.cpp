@@ -218,6 +218,11 @@ void RenderCore::Setting(const char* name, float value)
 		instancesDirty = true;
 	}
 	else if (!strcmp(name, "seed")) seedSetting = camSeed = (uint32_t)value; /* replaces camRNGseed's 0x12345678 */
+	else if (!strcmp(name, "traceMinRaysPerLane"))
+	{
+		if (!(value >= 0 && value <= 64)) throw CoreError(LH2HIP_ERR_INVALID, "Setting traceMinRaysPerLane: 0..64");
+		minRaysPerLane = value;
+	}
 	else if (!strcmp(name, "traceBlocksPerCU"))
 	{
 		if (!(value >= 1 && value <= 64)) throw CoreError(LH2HIP_ERR_INVALID, "Setting traceBlocksPerCU: 1..64");
@@ -431,6 +436,15 @@ void RenderCore::updateTopLevel()
 	instancesDirty = false;
 }
 
+/* Persistent grid for a wave of `count` rays: the full grid, or fewer blocks
+ * (not below one per CU) so that each lane gets at least minRaysPerLane rays. */
+int RenderCore::blocksFor(int count) const
+{
+	if (minRaysPerLane <= 0.0f) return persistentBlocks;
+	const int want = (int)((double)count / (TRACE_BLOCK * (double)minRaysPerLane)) + 1;
+	return std::max(std::min(deviceCUs, persistentBlocks), std::min(persistentBlocks, want));
+}
+
 const TraceScene& RenderCore::Scene()
 {
 	updateTopLevel();
@@ -442,7 +456,7 @@ void RenderCore::TraceClosest(const void* rays, void* hits, int count, hipStream
 {
 	if (count < 0 || (count > 0 && (!rays || !hits))) throw CoreError(LH2HIP_ERR_INVALID, "trace: bad arguments");
 	const TraceScene& sc = Scene();
-	hipCheck(launch_trace_dev(sc, MODE_CLOSEST, (const float4*)rays, hits, count, nullptr, counters.as<uint32_t>() + 16, overflow.as<uint32_t>(), persistentBlocks, s ? s : 0), "trace");
+	hipCheck(launch_trace_dev(sc, MODE_CLOSEST, (const float4*)rays, hits, count, nullptr, counters.as<uint32_t>() + 16, overflow.as<uint32_t>(), blocksFor(count), s ? s : 0), "trace");
 }
 
 void RenderCore::TraceClosestCounted(const void* rays, void* hits, int count, hipStream_t s, uint64_t* out2)
@@ -453,11 +467,12 @@ void RenderCore::TraceClosestCounted(const void* rays, void* hits, int count, hi
 	st.ensure(sizeof(TraceStats));
 	hipCheck(hipMemsetAsync(st.p, 0, sizeof(TraceStats), s), "memset");
 	hipCheck(launch_trace_dev(sc, MODE_CLOSEST, (const float4*)rays, hits, count, nullptr, counters.as<uint32_t>() + 16,
-		overflow.as<uint32_t>(), persistentBlocks, s, st.as<TraceStats>()), "trace");
+		overflow.as<uint32_t>(), blocksFor(count), s, st.as<TraceStats>()), "trace");
 	TraceStats h;
 	hipCheck(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s), "D2H");
 	hipCheck(hipStreamSynchronize(s), "sync");
 	out2[0] += h.nodes, out2[1] += h.tris;
+	out2[2] += h.nodeIters, out2[3] += h.nodeLanes, out2[4] += h.leafIters, out2[5] += h.leafLanes;
 }
 
 void RenderCore::TraceAny(const void* rays, uint32_t* mask, int count, hipStream_t s)
@@ -465,7 +480,7 @@ void RenderCore::TraceAny(const void* rays, uint32_t* mask, int count, hipStream
 	if (count < 0 || (count > 0 && (!rays || !mask))) throw CoreError(LH2HIP_ERR_INVALID, "trace: bad arguments");
 	const TraceScene& sc = Scene();
 	hipCheck(hipMemsetAsync(mask, 0, sizeof(uint32_t) * ((count + 31) / 32), s), "memset");
-	hipCheck(launch_trace_dev(sc, MODE_ANY_MASK, (const float4*)rays, mask, count, nullptr, counters.as<uint32_t>() + 16, overflow.as<uint32_t>(), persistentBlocks, s ? s : 0), "trace");
+	hipCheck(launch_trace_dev(sc, MODE_ANY_MASK, (const float4*)rays, mask, count, nullptr, counters.as<uint32_t>() + 16, overflow.as<uint32_t>(), blocksFor(count), s ? s : 0), "trace");
 }
 
 void RenderCore::Bounce(const void* raysIn, const void* hits, const uint32_t* pathIn, int count, uint32_t R0, float tmin,

@@ -132,6 +132,8 @@ private:
 	/* trace scratch */
 	DevBuf overflow, counters;
 	int persistentBlocks = 0;
+	float minRaysPerLane = 0.0f;       /* 0: always launch persistentBlocks */
+	int blocksFor(int count) const;
 	/* settings (RenderCore_PrimeRef rendercore.cpp:484-502 + extensions) */
 	float geometryEpsilon = 1.0e-3f, clampValue = 10.0f;
 	int maxDepth = 64, nodeFormat = FMT_Q8;

@@ -22,7 +22,10 @@ struct TraceScene
 	int maxStack = 0;
 };
 
-struct TraceStats { unsigned long long nodes, tris; };
+/* Work counters of the counting kernel variant: node visits and triangle
+ * tests per ray, and per wave the loop iterations and the lanes active in
+ * them (SIMD utilisation = lanes / (64 * iterations)). */
+struct TraceStats { unsigned long long nodes, tris, nodeIters, nodeLanes, leafIters, leafLanes; };
 
 /* rays: Ray4 {O.xyz, tmin; D.xyz, tmax}. out: MODE_CLOSEST -> float4 hit records;
  * MODE_ANY_MASK -> uint32 bitmask (must be zeroed); MODE_ANY_FLAG -> uint32 per ray.

@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 	int rayIdx = -1;
 	bool active = false, exhausted = false;
 	uint32_t ref = REF_DONE, leaf = REF_DONE, steps = 0;
-	unsigned long long cn = 0, ct = 0;
+	unsigned long long cn = 0, ct = 0, wn = 0, wl = 0, fn = 0, fl = 0;
 	while (true)
 	{
 		/* refill: one atomic per wave hands out rays to the idle lanes */
@@ -162,6 +162,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 				const bool top = TWO && (ref & REF_TLAS) && ref != REF_DONE;
 				const bool inner = !(ref & (REF_LEAF | REF_TLAS));
 				if (!(inner || (top && leaf == REF_DONE))) break;
+				if (COUNT && __lane_id() == __ffsll((unsigned long long)__ballot(true)) - 1)
+					wn++, wl += __popcll(__ballot(true));
 				if (++steps > TRACE_MAX_STEPS)
 				{	/* safety bound: a ray never visits more nodes than the trees hold */
 					ref = REF_DONE, leaf = REF_DONE, st.sp = 0;
@@ -278,6 +280,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 			/* leaf phase: the postponed leaf, then any leaves that follow */
 			while (leaf != REF_DONE)
 			{
+				if (COUNT && __lane_id() == __ffsll((unsigned long long)__ballot(true)) - 1)
+					fn++, fl += __popcll(__ballot(true));
 				uint32_t i = leaf & REF_MASK;
 				const int instId = TWO ? L.inst : 0;
 				while (true)
@@ -333,6 +337,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 	{
 		atomicAdd(&stats->nodes, cn);
 		atomicAdd(&stats->tris, ct);
+		if (wn) atomicAdd(&stats->nodeIters, wn), atomicAdd(&stats->nodeLanes, wl);
+		if (fn) atomicAdd(&stats->leafIters, fn), atomicAdd(&stats->leafLanes, fl);
 	}
 }
 
