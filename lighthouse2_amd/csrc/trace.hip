@@ -230,6 +230,10 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 					 * planes, so every box the exact interval reaches is kept. */
 					if (COUNT) cn++;
 					const NodeQ8 n = ((const NodeQ8*)L.nodes)[ref];
+					/* keep the child refs in the same memory round trip as the
+					 * boxes (the compiler would otherwise fetch them after the
+					 * box test, adding a dependent load per node visit) */
+					asm volatile("" : : "v"(n.refs.x), "v"(n.refs.y), "v"(n.refs.z), "v"(n.refs.w));
 					const uint32_t e = fbits(n.origin.w);
 					const float dx = bitsf((e & 255u) << 23) * L.inv.x;
 					const float dy = bitsf(((e >> 8) & 255u) << 23) * L.inv.y;
@@ -284,10 +288,14 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 					fn++, fl += __popcll(__ballot(true));
 				uint32_t i = leaf & REF_MASK;
 				const int instId = TWO ? L.inst : 0;
+				float4 a = L.tris[3 * i], b = L.tris[3 * i + 1], c = L.tris[3 * i + 2];
 				while (true)
 				{
-					const float4 a = L.tris[3 * i], b = L.tris[3 * i + 1], c = L.tris[3 * i + 2];
 					const uint32_t w = fbits(a.w);
+					const bool last = (w & TRI_LAST) != 0;
+					/* fetch the next triangle of the leaf while testing this one */
+					float4 na = a, nb = b, nc = c;
+					if (!last) na = L.tris[3 * i + 3], nb = L.tris[3 * i + 4], nc = L.tris[3 * i + 5];
 					const int prim = (int)(w & 0x7fffffffu);
 					if (COUNT) ct++;
 					float t, U, V, det;
@@ -300,7 +308,8 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace(const TraceS
 						else if (better_hit(t, instId, prim, L.tbest, L.binst, L.bprim))
 							L.tbest = t, L.binst = instId, L.bprim = prim, L.bU = U, L.bV = V, L.bdet = det;
 					}
-					if (w & TRI_LAST) break;
+					if (last) break;
+					a = na, b = nb, c = nc;
 					i++;
 				}
 				if (done) break;
