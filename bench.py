@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--format", choices=["q8", "f32"], default="q8")
+    ap.add_argument("--loop", choices=["merged", "ww", "greedy"], default="merged", help="traversal loop kind")
+    ap.add_argument("--tri-bias", type=int, default=16, help="greedy loop weight of leaf lanes, in 1/16")
+    ap.add_argument("--refill-lanes", type=int, default=16, help="idle lanes that make a wave fetch rays")
+    ap.add_argument("--chunk", type=int, default=32, help="rays a wave reserves per atomic")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent traversal blocks per CU (0: core default)")
@@ -110,6 +114,10 @@ def main():
     t_gen = time.time() - t0
     core = lh.RenderCore()
     core.Setting("nodeFormat", 1 if a.format == "q8" else 0)
+    core.Setting("traceLoop", {"merged": 0, "ww": 1, "greedy": 2}[a.loop])
+    core.Setting("traceTriBias", a.tri_bias)
+    core.Setting("traceRefillLanes", a.refill_lanes)
+    core.Setting("traceChunk", a.chunk)
     if a.blocks_per_cu:
         core.Setting("traceBlocksPerCU", a.blocks_per_cu)
     if a.min_rays_per_lane:
@@ -146,7 +154,7 @@ def main():
     wv.step(keep=keep)
     torch.cuda.synchronize()
     nodes = tris = nrays = 0
-    it = np.zeros(4)
+    it = np.zeros(10)
     scratch = torch.zeros_like(wv.hits)
     for r, _h in keep:
         c = core.trace_closest_counted(r.data_ptr(), scratch.data_ptr(), len(r), wv.sh)
@@ -199,7 +207,10 @@ def main():
                       "mrays_s": round(float(np.mean([c[w] for c in per_step]) / np.mean([m[w] for m in ms]) / 1e3), 1)}
                      for w in range(4)],
         "simd_utilisation": {"node_loop": round(it[1] / max(64 * it[0], 1), 3), "leaf_loop": round(it[3] / max(64 * it[2], 1), 3),
-                             "node_iters_per_ray": round(it[0] * 64 / nrays, 2), "leaf_iters_per_ray": round(it[2] * 64 / nrays, 2)},
+                             "node_iters_per_ray": round(it[0] * 64 / nrays, 2), "leaf_iters_per_ray": round(it[2] * 64 / nrays, 2),
+                             "lanes_holding_ray": round(it[5] / max(64 * it[4], 1), 3), "loop_iters_per_ray": round(it[4] * 64 / nrays, 2),
+                             "tail_lanes_holding_ray": round(it[7] / max(64 * it[6], 1), 3), "tail_iters_per_ray": round(it[6] * 64 / nrays, 2),
+                             "tail_time_share": round(1.0 - it[8] / max(it[9], 1), 3)},
         "wall_mrays_s_all_waves": round((inco_rays + prim_rays) / wall / 1e6, 2),
         "bvh": {"build_s": round(t_build, 3), "scene_gen_s": round(t_gen, 3), **info},
     }

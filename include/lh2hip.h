@@ -47,7 +47,14 @@ LH2HIP_API int lh2hip_set_target(lh2hip_core* core, int width, int height, int s
  * seed replacing 0x12345678, exact for values up to 2^24), "traceBlocksPerCU":
  * persistent traversal blocks of 256 threads per compute unit, default 8;
  * "traceMinRaysPerLane": shrink the persistent grid of a small wave so each
- * lane gets at least this many rays, 0 = off). */
+ * lane gets at least this many rays, 0 = off; "traceLoop": 0 merged loop
+ * (one node visit and one leaf triangle per lane and iteration, default),
+ * 1 while-while loop with postponed leaves, 2 greedy loop (each iteration a
+ * node step or a leaf step, whichever more lanes wait for); "traceTriBias":
+ * greedy loop weight of waiting leaf lanes in 1/16, default 16;
+ * "traceRefillLanes": idle lanes of a wave that trigger fetching new rays,
+ * 1..64, default 16; "traceChunk": rays a wave reserves per atomic, 1..1024,
+ * default 32). */
 LH2HIP_API int lh2hip_setting(lh2hip_core* core, const char* name, float value);
 /* CoreAPI_Base::SetGeometry (rendercore.cpp:214-222; core_mesh.cpp:36-64).
  * vertexData: 3 float4 per triangle (w ignored); builds the mesh BVH on the GPU. */
@@ -98,11 +105,15 @@ LH2HIP_API int lh2hip_trace_any(lh2hip_core* core, const Ray4* rays, uint32_t* o
 
 /* Closest hit as lh2hip_trace_closest, with the traversal's work counted
  * (host memory, accumulated): out6[0] interior nodes visited, [1] triangles
- * tested, [2]/[3] node-loop iterations of all waves / lanes active in them,
- * [4]/[5] the same for the leaf loop. A separate kernel variant; used to derive
- * algorithmic bytes per ray and SIMD utilisation. */
+ * tested, [2]/[3] node steps of all waves / lanes visiting a node in them,
+ * [4]/[5] the same for leaf steps, [6]/[7] traversal-loop iterations / lanes
+ * holding a ray in them while rays remain to be fetched, [8]/[9] the same
+ * once all rays are handed out, [10] ns from the first wave's start until the
+ * last ray was handed out, [11] ns from the first start to the last wave's
+ * end. A separate kernel variant; used to derive algorithmic bytes per ray,
+ * SIMD utilisation and the share of the launch spent after the feed ran dry. */
 LH2HIP_API int lh2hip_trace_closest_counted(lh2hip_core* core, const Ray4* rays, HitRecord* hits, int count, void* stream,
-	uint64_t* out6);
+	uint64_t* out12);
 
 /* ---- microbenchmark wave helpers (incoherent traversal workload) ----------
  * Primary rays: generateEyeRaysKernel (lib/RenderCore_PrimeRef/kernels/camera.h:39-68)

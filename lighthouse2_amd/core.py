@@ -180,9 +180,12 @@ class RenderCore:
         _check(library().lh2hip_trace_closest(self._h, rays_ptr, hits_ptr, count, stream))
 
     def trace_closest_counted(self, rays_ptr, hits_ptr, count, stream=None):
-        """As trace_closest; returns (nodes visited, triangles tested, node-loop
-        wave iterations, active lanes in them, leaf-loop iterations, lanes)."""
-        out = np.zeros(6, np.uint64)
+        """As trace_closest; returns (nodes visited, triangles tested, node
+        steps of all waves, lanes visiting a node in them, leaf steps, lanes
+        testing a triangle in them, loop iterations and lanes holding a ray
+        while rays remain, the same after the last ray was handed out, ns
+        from the first wave's start to the feed running dry, ns to the end)."""
+        out = np.zeros(12, np.uint64)
         _check(library().lh2hip_trace_closest_counted(self._h, rays_ptr, hits_ptr, count, stream, _p(out)))
         return tuple(int(x) for x in out)
 

@@ -165,8 +165,10 @@ void RenderCore::Init()
 	hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
 	for (int i = 0; i < 8; i++) hipCheck(hipEventCreate(&ev[i]), "hipEventCreate");
 	persistentBlocks = deviceCUs * 8;
-	counters.ensure(64 * sizeof(uint32_t));
+	counters.ensure(2 * TRACE_QUEUE_WORDS * sizeof(uint32_t));   /* ray feeds: direct queries, render waves */
 	hipCheck(hipMemsetAsync(counters.p, 0, counters.bytes, stream), "memset");
+	int clockKHz = 0;
+	if (hipDeviceGetAttribute(&clockKHz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && clockKHz > 0) wallClockNs = 1e6 / clockKHz;
 	overflow.ensure(trace_overflow_bytes(persistentBlocks));
 }
 
@@ -216,6 +218,30 @@ void RenderCore::Setting(const char* name, float value)
 		if (value != 0 && value != 1) throw CoreError(LH2HIP_ERR_INVALID, "Setting nodeFormat: 0 (fp32) or 1 (q8)");
 		nodeFormat = (int)value;
 		instancesDirty = true;
+	}
+	else if (!strcmp(name, "traceLoop"))
+	{
+		if (value != 0 && value != 1 && value != 2) throw CoreError(LH2HIP_ERR_INVALID, "Setting traceLoop: 0 (merged), 1 (while-while) or 2 (greedy)");
+		traceLoop = (int)value;
+		scene.loop = traceLoop;
+	}
+	else if (!strcmp(name, "traceRefillLanes"))
+	{
+		if (!(value >= 1 && value <= 64)) throw CoreError(LH2HIP_ERR_INVALID, "Setting traceRefillLanes: 1..64");
+		refillLanes = (int)value;
+		scene.refillLanes = refillLanes;
+	}
+	else if (!strcmp(name, "traceChunk"))
+	{
+		if (!(value >= 1 && value <= 1024)) throw CoreError(LH2HIP_ERR_INVALID, "Setting traceChunk: 1..1024");
+		chunk = (int)value;
+		scene.chunk = chunk;
+	}
+	else if (!strcmp(name, "traceTriBias"))
+	{
+		if (!(value >= 1 && value <= 256)) throw CoreError(LH2HIP_ERR_INVALID, "Setting traceTriBias: 1..256");
+		triBias = (int)value;
+		scene.triBias = triBias;
 	}
 	else if (!strcmp(name, "seed")) seedSetting = camSeed = (uint32_t)value; /* replaces camRNGseed's 0x12345678 */
 	else if (!strcmp(name, "traceMinRaysPerLane"))
@@ -397,6 +423,10 @@ void RenderCore::updateTopLevel()
 	shadeInsts.upload(sh.data(), sh.size() * sizeof(ShadeInst), stream);
 	scene = TraceScene();
 	scene.format = nodeFormat;
+	scene.loop = traceLoop;
+	scene.triBias = triBias;
+	scene.refillLanes = refillLanes;
+	scene.chunk = chunk;
 	scene.inst = instRecs.as<InstanceRec>();
 	sceneReady = !live.empty();
 	if (live.size() == 1 && recs[live[0]].flags == 1u && live[0] == 0)
@@ -456,7 +486,7 @@ void RenderCore::TraceClosest(const void* rays, void* hits, int count, hipStream
 {
 	if (count < 0 || (count > 0 && (!rays || !hits))) throw CoreError(LH2HIP_ERR_INVALID, "trace: bad arguments");
 	const TraceScene& sc = Scene();
-	hipCheck(launch_trace_dev(sc, MODE_CLOSEST, (const float4*)rays, hits, count, nullptr, counters.as<uint32_t>() + 16, overflow.as<uint32_t>(), blocksFor(count), s ? s : 0), "trace");
+	hipCheck(launch_trace_dev(sc, MODE_CLOSEST, (const float4*)rays, hits, count, nullptr, counters.as<uint32_t>(), overflow.as<uint32_t>(), blocksFor(count), s ? s : 0), "trace");
 }
 
 void RenderCore::TraceClosestCounted(const void* rays, void* hits, int count, hipStream_t s, uint64_t* out2)
@@ -466,13 +496,17 @@ void RenderCore::TraceClosestCounted(const void* rays, void* hits, int count, hi
 	DevBuf st;
 	st.ensure(sizeof(TraceStats));
 	hipCheck(hipMemsetAsync(st.p, 0, sizeof(TraceStats), s), "memset");
-	hipCheck(launch_trace_dev(sc, MODE_CLOSEST, (const float4*)rays, hits, count, nullptr, counters.as<uint32_t>() + 16,
+	hipCheck(hipMemsetAsync((char*)st.p + offsetof(TraceStats, tStart), 0xff, 2 * sizeof(unsigned long long), s), "memset");
+	hipCheck(launch_trace_dev(sc, MODE_CLOSEST, (const float4*)rays, hits, count, nullptr, counters.as<uint32_t>(),
 		overflow.as<uint32_t>(), blocksFor(count), s, st.as<TraceStats>()), "trace");
 	TraceStats h;
 	hipCheck(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s), "D2H");
 	hipCheck(hipStreamSynchronize(s), "sync");
 	out2[0] += h.nodes, out2[1] += h.tris;
 	out2[2] += h.nodeIters, out2[3] += h.nodeLanes, out2[4] += h.leafIters, out2[5] += h.leafLanes;
+	out2[6] += h.loopIters, out2[7] += h.loopLanes, out2[8] += h.tailIters, out2[9] += h.tailLanes;
+	if (h.tEnd > h.tStart && h.tDrained >= h.tStart && h.tDrained != ~0ull)
+		out2[10] += (uint64_t)((h.tDrained - h.tStart) * wallClockNs), out2[11] += (uint64_t)((h.tEnd - h.tStart) * wallClockNs);
 }
 
 void RenderCore::TraceAny(const void* rays, uint32_t* mask, int count, hipStream_t s)
@@ -480,7 +514,7 @@ void RenderCore::TraceAny(const void* rays, uint32_t* mask, int count, hipStream
 	if (count < 0 || (count > 0 && (!rays || !mask))) throw CoreError(LH2HIP_ERR_INVALID, "trace: bad arguments");
 	const TraceScene& sc = Scene();
 	hipCheck(hipMemsetAsync(mask, 0, sizeof(uint32_t) * ((count + 31) / 32), s), "memset");
-	hipCheck(launch_trace_dev(sc, MODE_ANY_MASK, (const float4*)rays, mask, count, nullptr, counters.as<uint32_t>() + 16, overflow.as<uint32_t>(), blocksFor(count), s ? s : 0), "trace");
+	hipCheck(launch_trace_dev(sc, MODE_ANY_MASK, (const float4*)rays, mask, count, nullptr, counters.as<uint32_t>(), overflow.as<uint32_t>(), blocksFor(count), s ? s : 0), "trace");
 }
 
 void RenderCore::Bounce(const void* raysIn, const void* hits, const uint32_t* pathIn, int count, uint32_t R0, float tmin,
@@ -611,7 +645,7 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 		for (int pathLength = 1; pathLength <= maxDepth; pathLength++)
 		{
 			hipCheck(hipEventRecord(ev[0], stream), "event");
-			hipCheck(launch_trace_dev(sc, MODE_CLOSEST, B.rays[in], B.hits, n, B.counters + 0, B.counters + 3, overflow.as<uint32_t>(), persistentBlocks, stream), "extend");
+			hipCheck(launch_trace_dev(sc, MODE_CLOSEST, B.rays[in], B.hits, n, B.counters + 0, counters.as<uint32_t>() + TRACE_QUEUE_WORDS, overflow.as<uint32_t>(), persistentBlocks, stream), "extend");
 			hipCheck(hipEventRecord(ev[1], stream), "event");
 			P.pathLength = pathLength;
 			P.R0 = (uint32_t)samplesTaken * 7907u + (uint32_t)pathLength * 91771u;
@@ -619,7 +653,7 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 			hipCheck(hipEventRecord(ev[2], stream), "event");
 			if (pathLength < maxDepth)
 			{
-				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, B.counters + 2, B.counters + 4, overflow.as<uint32_t>(), persistentBlocks, stream), "connect");
+				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, B.counters + 2, counters.as<uint32_t>() + TRACE_QUEUE_WORDS, overflow.as<uint32_t>(), persistentBlocks, stream), "connect");
 				hipCheck(launch_connect(B, n, stream), "connect");
 			}
 			hipCheck(hipEventRecord(ev[3], stream), "event");

@@ -131,12 +131,13 @@ private:
 	TraceScene scene;
 	/* trace scratch */
 	DevBuf overflow, counters;
+	double wallClockNs = 10.0;         /* period of the device wall clock */
 	int persistentBlocks = 0;
 	float minRaysPerLane = 0.0f;       /* 0: always launch persistentBlocks */
 	int blocksFor(int count) const;
 	/* settings (RenderCore_PrimeRef rendercore.cpp:484-502 + extensions) */
 	float geometryEpsilon = 1.0e-3f, clampValue = 10.0f;
-	int maxDepth = 64, nodeFormat = FMT_Q8;
+	int maxDepth = 64, nodeFormat = FMT_Q8, traceLoop = LOOP_MERGED, triBias = 16, refillLanes = 16, chunk = 32;
 	uint32_t camSeed = 0x12345678u, seedSetting = 0x12345678u;
 	/* render state */
 	int scrw = 0, scrh = 0, scrspp = 1;

@@ -30,9 +30,10 @@ def gpu_any(torch, core, rays):
     return ((bits[idx >> 5] >> (idx & 31)) & 1).astype(bool)
 
 
-def make_core(scene, fmt):
+def make_core(scene, fmt, loop=0):
     core = lh.RenderCore()
     core.Setting("nodeFormat", fmt)
+    core.Setting("traceLoop", loop)
     lh.load_scene(core, scene)
     return core
 
@@ -45,12 +46,13 @@ def workload_rays(o, seed, n_random=20000, res=96):
     return np.concatenate([prim, b1, _random_rays(rng, n_random)])
 
 
+@pytest.mark.parametrize("loop", [0, 1, 2])
 @pytest.mark.parametrize("fmt", [0, 1])
-def test_closest_and_any_match_oracle(torch_cuda, fmt):
+def test_closest_and_any_match_oracle(torch_cuda, fmt, loop):
     s = scenes.random_triangle_scene(20000, seed=5)
     o = orc.OracleScene(s)
     rays = workload_rays(o, 5)
-    core = make_core(s, fmt)
+    core = make_core(s, fmt, loop)
     got = gpu_closest(torch_cuda, core, rays)
     ref = o.trace(rays)
     print(f"fmt {fmt}: {len(rays)} rays, {(ref.view(np.int32)[:, 2] >= 0).sum()} hits")
@@ -61,12 +63,13 @@ def test_closest_and_any_match_oracle(torch_cuda, fmt):
     core.Shutdown()
 
 
+@pytest.mark.parametrize("loop", [0, 1, 2])
 @pytest.mark.parametrize("fmt", [0, 1])
-def test_two_level_matches_brute_force(torch_cuda, fmt):
+def test_two_level_matches_brute_force(torch_cuda, fmt, loop):
     s = instanced_scene(seed=21, meshes=4, instances=40, tris=600)
     o = orc.OracleScene(s)
     rays = _random_rays(np.random.default_rng(2), 30000, -3, 3)
-    core = make_core(s, fmt)
+    core = make_core(s, fmt, loop)
     got = gpu_closest(torch_cuda, core, rays)
     ref = o.trace(rays, brute=True)
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
@@ -132,11 +135,12 @@ def test_wave_kernels_match_oracle(torch_cuda):
     core.Shutdown()
 
 
-def test_edge_cases(torch_cuda):
+@pytest.mark.parametrize("loop", [0, 1, 2])
+def test_edge_cases(torch_cuda, loop):
     s = scenes.Scene()
     v = np.array([[-1, -1, 2, 1], [3, -1, 2, 1], [-1, 3, 2, 1]], np.float32)
     s.add_instance(s.add_mesh(np.concatenate([v, v]), None))  # duplicate triangle: tie -> prim 0
-    core = make_core(s, 1)
+    core = make_core(s, 1, loop)
     r = np.array([[0, 0, 0, 0, 0, 0, 1, 1e34], [0, 0, 0, 0, 0, 0, 1, 2.0], [0, 0, 0, 2.0, 0, 0, 1, 1e34],
                   [0, 0, 0, 0, 0, 0, -1, 1e34], [0, 0, 0, 0, 0, 0, 1, np.nextafter(np.float32(2), np.float32(3))]], np.float32)
     h = gpu_closest(torch_cuda, core, r).view(np.int32)
@@ -161,6 +165,12 @@ def test_large_scene_against_oracle_bvh(torch_cuda):
     got = gpu_closest(torch_cuda, core, rays)
     ref = o.trace(rays, threads=16)
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    core.Setting("traceLoop", 1)
+    assert np.array_equal(gpu_closest(torch_cuda, core, rays).view(np.uint32), ref.view(np.uint32))
     core.Setting("nodeFormat", 0)
+    assert np.array_equal(gpu_closest(torch_cuda, core, rays).view(np.uint32), ref.view(np.uint32))
+    core.Setting("traceLoop", 0)
+    assert np.array_equal(gpu_closest(torch_cuda, core, rays).view(np.uint32), ref.view(np.uint32))
+    core.Setting("traceLoop", 2)
     assert np.array_equal(gpu_closest(torch_cuda, core, rays).view(np.uint32), ref.view(np.uint32))
     core.Shutdown()
