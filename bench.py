@@ -167,7 +167,7 @@ def main():
     achieved = float(np.mean([r * bytes_per_ray / (m * 1e-3) / 1e9 for r, m in zip(launch_rays, launch_ms)]))
 
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "extend_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "k_trace_merged_incoherent_4m_traffic.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("bytes_per_launch")
 
@@ -192,7 +192,8 @@ def main():
         "config": {"workload": "incoherent traversal: 4M random triangles, 1920x1080 primary + 3 diffuse bounces",
                    "triangles": a.tris, "width": a.width, "height": a.height, "node_format": a.format,
                    "bvh": "binned SAH 16 bins, leaf <= 4, GPU build", "seed": 1,
-                   "blocks_per_cu": a.blocks_per_cu or 8, "min_rays_per_lane": a.min_rays_per_lane},
+                   "blocks_per_cu": f"{a.blocks_per_cu or 8} requested, capped at the kernel occupancy", "min_rays_per_lane": a.min_rays_per_lane,
+                   "loop": a.loop, "ray_chunk": a.chunk, "refill_lanes": a.refill_lanes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 2),
