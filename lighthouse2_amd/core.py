@@ -144,6 +144,12 @@ class RenderCore:
         _check(library().lh2hip_set_lights(self._h, _ptr(a[0]), len(a[0]), _ptr(a[1]), len(a[1]),
                                            _ptr(a[2]), len(a[2]), _ptr(a[3]), len(a[3])))
 
+    def SetTextures(self, textures):
+        """CoreAPI_Base::SetTextures: list of texture dicts (formats.texture_descs)."""
+        d, keep = F.texture_descs(textures)
+        _check(library().lh2hip_set_textures(self._h, _ptr(d) if len(d) else None, len(d)))
+        del keep
+
     def SetSkyData(self, pixels, width, height, worldToLight=None):
         p = np.ascontiguousarray(pixels, np.float32)
         m = None if worldToLight is None else np.ascontiguousarray(worldToLight, np.float32).reshape(16)
@@ -218,8 +224,14 @@ def generate_primary(view, width, height, R0, tmin, rays_ptr, path_ptr=None, str
 
 
 def load_scene(core, scene):
-    """RenderSystem::SynchronizeSceneData call order: materials, geometry per
-    mesh, lights, instances (+ truncation), FinalizeInstances."""
+    """RenderSystem::SynchronizeSceneData call order (rendersystem.cpp:46-211):
+    sky, textures, materials, geometry per mesh, lights, instances (+
+    truncation), FinalizeInstances."""
+    if getattr(scene, "sky", None) is not None:
+        px, T = scene.sky
+        core.SetSkyData(px.reshape(-1, 3), px.shape[1], px.shape[0], T)
+    if getattr(scene, "textures", None):
+        core.SetTextures(scene.textures)
     core.SetMaterials(scene.materials if len(scene.materials) else F.new_materials(1))
     for i, (v, t) in enumerate(scene.meshes):
         core.SetGeometry(i, v, t)

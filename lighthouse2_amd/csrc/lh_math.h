@@ -65,6 +65,51 @@ LH_HD void cw_sincos(float x, float* s, float* c)
 	else { *s = -cr; *c = sr; }
 }
 
+/* atan2(y, x): the octant reduction atan(min/max) (swap and reflect), then
+ * atan(x) = pi/6 + atan((sqrt3 x - 1) / (sqrt3 + x)) for x > tan(pi/12), and
+ * the Taylor series of atan to x^13 on |x| <= tan(pi/12) (truncation error
+ * < 3e-10). atan2(0, 0) = 0; a negative zero y gives +0. */
+LH_HD float lh_atan_unit(float x)
+{
+	const bool big = x > 0.26794919f;
+	const float r = big ? (x * 1.7320508f - 1.0f) / (x + 1.7320508f) : x;
+	const float z = r * r;
+	const float p = r + r * z * (-0.33333333f + z * (0.2f + z * (-0.14285714f + z * (0.11111111f + z * (-0.09090909f + z * 0.07692308f)))));
+	return big ? p + 0.52359878f : p;
+}
+LH_HD float lh_atan2f(float y, float x)
+{
+	const float ax = fabsf(x), ay = fabsf(y);
+	const float mx = ay > ax ? ay : ax, mn = ay > ax ? ax : ay;
+	float a = mx == 0.0f ? 0.0f : lh_atan_unit(mn / mx);
+	if (ay > ax) a = 1.5707964f - a;
+	if (x < 0.0f) a = 3.1415927f - a;
+	return y < 0.0f ? -a : a;
+}
+/* acos(x), |x| <= 1: atan2(sqrt((1 - x)(1 + x)), x). */
+LH_HD float lh_acosf(float x) { return lh_atan2f(sqrtf((1.0f - x) * (1.0f + x)), x); }
+/* log2(x): x = m 2^e with m in [sqrt(1/2), sqrt(2)), ln(m) = 2 atanh(s),
+ * s = (m - 1) / (m + 1), |s| <= 0.1716, series to s^9 (truncation < 2e-9). */
+LH_HD float lh_log2f(float x)
+{
+	if (!(x > 0.0f)) return x == 0.0f ? -__builtin_huge_valf() : __builtin_nanf("");
+	if (x == __builtin_huge_valf()) return x;
+	uint32_t b = fbits(x);
+	int e = 0;
+	if (b < 0x00800000u) { x = x * 8388608.0f; b = fbits(x); e = -23; }
+	e += (int)(b >> 23) - 127;
+	float m = bitsf((b & 0x007fffffu) | 0x3f800000u);
+	if (m > 1.4142135f) { m = m * 0.5f; e += 1; }
+	const float s = (m - 1.0f) / (m + 1.0f), z = s * s;
+	const float ln = 2.0f * s * (1.0f + z * (0.33333333f + z * (0.2f + z * (0.14285714f + z * 0.11111111f))));
+	return (float)e + ln * 1.4426950f;
+}
+/* float -> uint32 with the saturation of the GPU's conversion (NaN and
+ * negatives give 0). */
+LH_HD uint32_t f2u_sat(float x) { return x > 0.0f ? (x >= 4294967040.0f ? 0xffffffffu : (uint32_t)x) : 0u; }
+/* float -> int32 saturating (NaN gives 0). */
+LH_HD int32_t f2i_sat(float x) { return x >= 2147483520.0f ? 0x7fffffff : (x <= -2147483648.0f ? (int32_t)0x80000000 : (x == x ? (int32_t)x : 0)); }
+
 /* exp(x): k = round(x / ln2), Cody-Waite split of ln2 (0.693359375 exact in
  * 10 bits), Taylor series of exp(r) to r^7 on |r| <= ln2/2, scaled by 2^k. */
 LH_HD float cw_expf(float x)

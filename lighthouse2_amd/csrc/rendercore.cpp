@@ -542,6 +542,14 @@ int RenderCore::BvhDownload(int mesh, void* nodesOut, int maxNodes, uint32_t* pr
 	return (int)m.buildNodes.size();
 }
 
+/* HostTexture::PixelsNeeded (host_texture.cpp:117-122). */
+static uint64_t mip_texels(uint32_t w, uint32_t h, int levels)
+{
+	uint64_t n = 0;
+	for (int i = 0; i < levels; i++) n += (uint64_t)w * h, w >>= 1, h >>= 1;
+	return n;
+}
+
 void RenderCore::SetMaterials(CoreMaterial* mat, const int materialCount)
 {
 	if (materialCount < 0 || (materialCount > 0 && !mat)) throw CoreError(LH2HIP_ERR_INVALID, "SetMaterials: bad arguments");
@@ -560,7 +568,32 @@ void RenderCore::SetMaterials(CoreMaterial* mat, const int materialCount)
 		d.params.y = to_byte(m.specularTint.value) + (to_byte(m.anisotropic.value) << 8) + (to_byte(m.sheen.value) << 16) + (to_byte(m.sheenTint.value) << 24);
 		d.params.z = to_byte(m.clearcoat.value) + (to_byte(m.clearcoatGloss.value) << 8) + (to_byte(m.transmission.value) << 16);
 		memcpy(&d.params.w, &m.eta.value, 4);
-		d.flags = (m.eta.value < 1 ? MF_ISDIELECTRIC : 0) + (m.color.textureID != -1 ? MF_HASDIFFUSEMAP : 0) +
+		/* CUDAMaterial::Map (rendercore.h:82-89): width | height << 16, half
+		 * uv scale, half uv offset, first texel */
+		const auto map = [&](uint4& q, int id, const decltype(m.color.uvscale)& sc, const decltype(m.color.uvoffset)& of, bool mipmapped, int storage = LH2_ARGB32) {
+			if (id < 0) return;
+			if (id >= (int)textures.size()) throw CoreError(LH2HIP_ERR_STATE, "SetMaterials: texture id beyond SetTextures (set textures first)");
+			const TexInfo& t = textures[id];
+			/* the kernels read texels (x, y) < (w, h) of level 0, and for a
+			 * trilinear map of all 5 levels: reject maps they would read past */
+			const uint64_t need = mipmapped ? mip_texels(t.width, t.height, 5) : (uint64_t)t.width * t.height;
+			if (t.width == 0 || t.height == 0 || t.width > 32767 || t.height > 32767 || (mipmapped && ((t.width >> 4) == 0 || (t.height >> 4) == 0)))
+				throw CoreError(LH2HIP_ERR_INVALID, "SetMaterials: texture size unsupported (1..32767; a diffuse map needs at least 16x16 for 5 MIP levels)");
+			if (t.pixelCount < need) throw CoreError(LH2HIP_ERR_INVALID, "SetMaterials: texture holds fewer texels than its size (and MIP levels) need");
+			if (t.storage != storage) throw CoreError(LH2HIP_ERR_INVALID, "SetMaterials: normal maps need NRM32 texels, other maps ARGB32");
+			q.x = (t.width & 0xffffu) | ((t.height & 0xffffu) << 16);
+			q.y = (uint32_t)to_half(sc.x) | ((uint32_t)to_half(sc.y) << 16);
+			q.z = (uint32_t)to_half(of.x) | ((uint32_t)to_half(of.y) << 16);
+			q.w = t.firstPixel;
+		};
+		map(d.t0, m.color.textureID, m.color.uvscale, m.color.uvoffset, true);
+		map(d.t1, m.detailColor.textureID, m.detailColor.uvscale, m.detailColor.uvoffset, false);
+		map(d.n0, m.normals.textureID, m.normals.uvscale, m.normals.uvoffset, false, LH2_NRM32);
+		map(d.n1, m.detailNormals.textureID, m.detailNormals.uvscale, m.detailNormals.uvoffset, false, LH2_NRM32);
+		map(d.s, m.specular.textureID, m.specular.uvscale, m.specular.uvoffset, false);
+		map(d.r, m.roughness.textureID, m.roughness.uvscale, m.roughness.uvoffset, false);
+		const bool hdr = m.color.textureID >= 0 && (textures[m.color.textureID].flags & 8u);
+		d.flags = (m.eta.value < 1 ? MF_ISDIELECTRIC : 0) + (hdr ? MF_DIFFUSEMAPISHDR : 0) + (m.color.textureID != -1 ? MF_HASDIFFUSEMAP : 0) +
 			(m.normals.textureID != -1 ? MF_HASNORMALMAP : 0) + (m.specular.textureID != -1 ? MF_HASSPECULARITYMAP : 0) +
 			(m.roughness.textureID != -1 ? MF_HASROUGHNESSMAP : 0) + (m.detailNormals.textureID != -1 ? MF_HAS2NDNORMALMAP : 0) +
 			(m.detailColor.textureID != -1 ? MF_HAS2NDDIFFUSEMAP : 0) + ((m.flags & 1) ? MF_HASSMOOTHNORMALS : 0) + ((m.flags & 2) ? MF_HASALPHA : 0);
@@ -583,14 +616,57 @@ void RenderCore::SetLights(const CoreLightTri* tl, const int nt, const CorePoint
 void RenderCore::SetTextures(const CoreTexDesc* tex, const int textureCount)
 {
 	if (textureCount < 0 || (textureCount > 0 && !tex)) throw CoreError(LH2HIP_ERR_INVALID, "SetTextures: bad arguments");
-	(void)tex; /* texture maps are not sampled by the reference estimator on the parity scenes */
-	stats.argb32TexelCount = 0;
-	for (int i = 0; i < textureCount; i++) if (tex[i].storage == LH2_ARGB32) stats.argb32TexelCount += tex[i].pixelCount;
+	/* SyncStorageType (rendercore.cpp:310-355): one continuous array per storage
+	 * kind, textures in order, firstPixel = start in that array. ARGB128 texels
+	 * are copied whole (float4 each). */
+	std::vector<uint32_t> a32, n32;
+	std::vector<float4> a128;
+	textures.assign(textureCount, TexInfo());
+	for (int i = 0; i < textureCount; i++)
+	{
+		const CoreTexDesc& t = tex[i];
+		if (t.pixelCount > 0 && !t.idata) throw CoreError(LH2HIP_ERR_INVALID, "SetTextures: texture without texels");
+		if (t.storage < LH2_ARGB32 || t.storage > LH2_NRM32) throw CoreError(LH2HIP_ERR_INVALID, "SetTextures: unknown texel storage");
+		TexInfo& d = textures[i];
+		d.width = t.width, d.height = t.height, d.flags = t.flags, d.storage = t.storage, d.pixelCount = t.pixelCount;
+		if (t.storage == LH2_ARGB128)
+		{
+			d.firstPixel = (uint32_t)a128.size();
+			const float4* src = (const float4*)t.fdata;
+			a128.insert(a128.end(), src, src + t.pixelCount);
+		}
+		else
+		{
+			std::vector<uint32_t>& dst = t.storage == LH2_NRM32 ? n32 : a32;
+			d.firstPixel = (uint32_t)dst.size();
+			dst.insert(dst.end(), t.idata, t.idata + t.pixelCount);
+		}
+	}
+	a32.resize(std::max<size_t>(a32.size(), 16)), n32.resize(std::max<size_t>(n32.size(), 16)), a128.resize(std::max<size_t>(a128.size(), 16));
+	texARGB32.upload(a32.data(), a32.size() * 4, stream);
+	texNRM32.upload(n32.data(), n32.size() * 4, stream);
+	texARGB128.upload(a128.data(), a128.size() * 16, stream);
+	hipCheck(hipStreamSynchronize(stream), "sync");
+	stats.argb32TexelCount = 0, stats.argb128TexelCount = 0, stats.nrm32TexelCount = 0;
+	for (int i = 0; i < textureCount; i++)
+	{
+		if (tex[i].storage == LH2_ARGB32) stats.argb32TexelCount += tex[i].pixelCount;
+		else if (tex[i].storage == LH2_ARGB128) stats.argb128TexelCount += tex[i].pixelCount;
+		else stats.nrm32TexelCount += tex[i].pixelCount;
+	}
 }
 
 void RenderCore::SetSkyData(const lighthouse2::float3* pixels, const uint32_t width, const uint32_t height, const mat4& worldToLight)
 {
-	(void)pixels, (void)width, (void)height, (void)worldToLight; /* black sky on a miss */
+	/* rendercore.cpp:466-478: float3 texels stored as float4 (w = 0). */
+	const size_t n = (size_t)width * height;
+	if (n > 0 && !pixels) throw CoreError(LH2HIP_ERR_INVALID, "SetSkyData: no pixels");
+	std::vector<float4> px(std::max<size_t>(n, 1), make_float4(0, 0, 0, 0));
+	for (size_t i = 0; i < n; i++) px[i] = make_float4(pixels[i].x, pixels[i].y, pixels[i].z, 0.0f);
+	skyPixels.upload(px.data(), px.size() * sizeof(float4), stream);
+	hipCheck(hipStreamSynchronize(stream), "sync");
+	skyW = (int)width, skyH = (int)height;
+	for (int r = 0; r < 3; r++) for (int c = 0; c < 4; c++) worldToSky[4 * r + c] = worldToLight.cell[4 * r + c];
 }
 
 void RenderCore::Render(const ViewPyramid& view, const Convergence converge, bool async)
@@ -627,6 +703,12 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	P.nTri = lightCounts[0], P.nPoint = lightCounts[1], P.nSpot = lightCounts[2], P.nDir = lightCounts[3];
 	P.w = scrw, P.h = scrh, P.maxDepth = maxDepth, P.pass = samplesTaken, P.eps = geometryEpsilon;
 	P.probePixel = probePos.x + scrw * probePos.y;
+	P.argb32 = texARGB32.as<uint32_t>(), P.argb128 = texARGB128.as<float4>(), P.nrm32 = texNRM32.as<uint32_t>();
+	P.spreadAngle = view.spreadAngle;
+	P.sky = skyPixels.as<float4>(), P.skyW = skyW, P.skyH = skyH;
+	P.w2s0 = make_float4(worldToSky[0], worldToSky[1], worldToSky[2], 0.0f);
+	P.w2s1 = make_float4(worldToSky[4], worldToSky[5], worldToSky[6], 0.0f);
+	P.w2s2 = make_float4(worldToSky[8], worldToSky[9], worldToSky[10], 0.0f);
 	const int pixels = scrw * scrh;
 	const int passSamples = std::max(1, std::min(scrspp, (int)(Wavefront::PASS_CAP / pixels)));
 	wf.resizePaths((size_t)passSamples * pixels, stream);

@@ -146,6 +146,12 @@ private:
 	lighthouse2::int2 probePos = { 0, 0 };
 	DevBuf materials, triLights, pointLights, spotLights, dirLights;
 	int lightCounts[4] = { 0, 0, 0, 0 };
+	/* textures (continuous texel arrays per storage kind) and skydome */
+	struct TexInfo { uint32_t width, height, flags, firstPixel, pixelCount; int storage; };
+	std::vector<TexInfo> textures;
+	DevBuf texARGB32, texARGB128, texNRM32, skyPixels;
+	int skyW = 0, skyH = 0;
+	float worldToSky[12] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0 };
 	int materialCount = 0;
 	Wavefront wf;
 	hipEvent_t ev[8];

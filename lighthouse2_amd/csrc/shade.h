@@ -44,6 +44,15 @@ struct ShadeParams
 	float eps;
 	int probePixel;
 	int* probe;                   /* {inst, prim, t bits} */
+	/* texel arrays (SyncStorageType) and the ray cone spread for texture LOD */
+	const uint32_t* argb32;
+	const float4* argb128;
+	const uint32_t* nrm32;
+	float spreadAngle;
+	/* skydome: float4 texels, size, rows of the world-to-sky 3x3 */
+	const float4* sky;
+	int skyW, skyH;
+	float4 w2s0, w2s1, w2s2;
 };
 
 /* Path-state buffers of one pass (SoA). */

@@ -3,7 +3,9 @@
  * Follows RenderCore_PrimeRef ("REFERENCE, for image validation"):
  *   k_generate  generateEyeRaysKernel    lib/RenderCore_PrimeRef/kernels/camera.h:39-68
  *   k_shade     shadeKernel              lib/RenderCore_PrimeRef/kernels/pathtracer.h:45-168
- *               GetShadingData/SetupFrame lib/CUDA/shared_kernel_code/material_shared.h:42-128
+ *               GetShadingData/SetupFrame lib/CUDA/shared_kernel_code/material_shared.h:42-206
+ *               FetchTexel(Trilinear)    lib/CUDA/shared_kernel_code/sampling_shared.h:35-104
+ *               SampleSkydome            lib/CUDA/shared_kernel_code/tools_shared.h:184-202
  *               RandomPointOnLight       lib/CUDA/shared_kernel_code/lights_shared.h:225-313
  *               Lambert+specular BSDF    lib/RenderCore_PrimeRef/kernels/bsdf.h:19-101
  *   k_connect   finalizeConnectionKernel lib/RenderCore_PrimeRef/kernels/connections.h:22-34
@@ -44,8 +46,74 @@ __device__ __forceinline__ v3 consistent_normal(v3 D, v3 iN, float alpha)
 	return normalize(D + Rc);
 }
 
-/* CoreTri as 13 float4 (common_classes.h:133-163). */
-__device__ void shading_data(const ShadeParams& P, v3 D, float u, float v, const float4* tri, int instIdx,
+/* __uchar4_to_float4 (sampling_shared.h:29-33): byte k of the texel word is
+ * component k, scaled by 1/256. */
+__device__ __forceinline__ float4 unpack_texel(uint32_t t)
+{
+	const float r = 1.0f / 256.0f;
+	return make_float4((float)(t & 255u) * r, (float)((t >> 8) & 255u) * r, (float)((t >> 16) & 255u) * r, (float)(t >> 24) * r);
+}
+
+__device__ __forceinline__ float4 texel_at(const ShadeParams& P, int storage, int idx)
+{
+	if (storage == 1) return P.argb128[idx];
+	return unpack_texel(storage == 2 ? P.nrm32[idx] : P.argb32[idx]);
+}
+
+/* FetchTexel, BILINEAR on (lib/CUDA/shared_kernel_code/sampling_shared.h:35-79). */
+__device__ float4 fetch_texel(const ShadeParams& P, float tu, float tv, int o, int w, int h, int storage)
+{
+	const float tcx = (fmaxf(tu + 1000.0f, 0.0f) * (float)w) - 0.5f;
+	const float tcy = (fmaxf(tv + 1000.0f, 0.0f) * (float)h) - 0.5f;
+	const int iu = f2i_sat(tcx) % w, iv = f2i_sat(tcy) % h;
+	const float fu = tcx - floorf(tcx), fv = tcy - floorf(tcy);
+	const float w0 = (1 - fu) * (1 - fv), w1 = fu * (1 - fv), w2 = (1 - fu) * fv, w3 = 1 - (w0 + w1 + w2);
+	const int iu1 = (iu + 1) % w, iv1 = (iv + 1) % h;
+	const float4 p0 = texel_at(P, storage, o + iu + iv * w), p1 = texel_at(P, storage, o + iu1 + iv * w);
+	const float4 p2 = texel_at(P, storage, o + iu + iv1 * w), p3 = texel_at(P, storage, o + iu1 + iv1 * w);
+	return make_float4(p0.x * w0 + p1.x * w1 + p2.x * w2 + p3.x * w3, p0.y * w0 + p1.y * w1 + p2.y * w2 + p3.y * w3,
+		p0.z * w0 + p1.z * w1 + p2.z * w2 + p3.z * w3, p0.w * w0 + p1.w * w1 + p2.w * w2 + p3.w * w3);
+}
+
+/* FetchTexelTrilinear (sampling_shared.h:81-104), 5 MIP levels. */
+__device__ float4 fetch_texel_trilinear(const ShadeParams& P, float lambda, float tu, float tv, int offset, int width, int height)
+{
+	int level0 = 0, level1 = 0;
+	float f = 0;
+	if (lambda >= 0) level0 = min(4, f2i_sat(lambda)), level1 = min(4, level0 + 1), f = lambda - floorf(lambda);
+	const float scale = (float)(width * height) * 1.3333333333f;
+	const int o0 = offset + f2i_sat(scale * (1 - bitsf((uint32_t)(127 - 2 * level0) << 23)));
+	const int o1 = offset + f2i_sat(scale * (1 - bitsf((uint32_t)(127 - 2 * level1) << 23)));
+	const float4 p0 = fetch_texel(P, tu, tv, o0, width >> level0, height >> level0, 0);
+	const float4 p1 = fetch_texel(P, tu, tv, o1, width >> level1, height >> level1, 0);
+	return make_float4((1 - f) * p0.x + f * p1.x, (1 - f) * p0.y + f * p1.y, (1 - f) * p0.z + f * p1.z, (1 - f) * p0.w + f * p1.w);
+}
+
+/* SampleSkydome(-worldToSky.TransformVector(D)) (tools_shared.h:184-202;
+ * common_types.h:608-613; kernels/pathtracer.h:73-78). */
+__device__ v3 sample_sky(const ShadeParams& P, v3 D)
+{
+	if (P.skyW <= 0 || P.skyH <= 0) return mk3(0, 0, 0);
+	const float4 a = P.w2s0, b = P.w2s1, c = P.w2s2;
+	const v3 d = mk3(-(a.x * D.x + a.y * D.y + a.z * D.z), -(b.x * D.x + b.y * D.y + b.z * D.z), -(c.x * D.x + c.y * D.y + c.z * D.z));
+	const float p = lh_atan2f(d.y, d.x);
+	const float phi = p < 0 ? (p + 2 * PI_F) : p;
+	const float theta = lh_acosf(fminf(fmaxf(d.z, -1.0f), 1.0f));
+	const uint32_t u = f2u_sat((float)P.skyW * phi * 0.15915494309189533577f - 0.5f);
+	const uint32_t v = f2u_sat((float)P.skyH * theta * 0.31830988618379067154f - 0.5f);
+	const uint32_t idx = u + v * (uint32_t)P.skyW;
+	if (idx >= (uint32_t)P.skyW * (uint32_t)P.skyH) return mk3(0, 0, 0);
+	return xyz(P.sky[idx]);
+}
+
+__device__ __forceinline__ float h2f(uint32_t bits16) { return half_to_float(bits16 & 0xffffu); }
+
+/* CoreTri as 13 float4 (common_classes.h:133-163); SetupFrame + GetShadingData
+ * (material_shared.h:42-206) with the texture maps: diffuse (trilinear, LOD
+ * TRI_LOD + log2(coneWidth / |D.N|); alpha below 0.5 ends the lookup with the
+ * untextured colour), detail diffuse, normal maps (tangent frame T, B of the
+ * triangle) and roughness; the normal-map scale's __expf is cw_expf. */
+__device__ void shading_data(const ShadeParams& P, v3 D, float u, float v, float coneWidth, const float4* tri, int instIdx,
 	Shading& sd, v3& N, v3& iN, v3& fN, v3& T)
 {
 	const float4 t1 = tri[1], t2 = tri[2], t3 = tri[3], t4 = tri[4], t5 = tri[5], t7 = tri[7];
@@ -66,6 +134,60 @@ __device__ void shading_data(const ShadeParams& P, v3 D, float u, float v, const
 	const float alpha = u * t7.x + v * t7.y + w * t7.z;
 	iN = (backSide ? -1.0f : 1.0f) * consistent_normal(D * -1.0f, backSide ? (iN * -1.0f) : iN, alpha);
 	fN = iN;
+	if (!(flags & (MF_HASDIFFUSEMAP | MF_HAS2NDDIFFUSEMAP | MF_HASSPECULARITYMAP | MF_HASNORMALMAP | MF_HAS2NDNORMALMAP | MF_HASROUGHNESSMAP))) return;
+	const float4 t0 = tri[0];
+	const float ww = 1 - (u + v);
+	const float tu = u * t0.x + v * t0.y + ww * t0.z;
+	const float tv = u * t1.x + v * t1.y + ww * t1.z;
+	if (flags & MF_HASDIFFUSEMAP)
+	{
+		const float lambda = t7.w + lh_log2f(coneWidth * (1.0f / fabsf(dot(D, N))));
+		const uint4 m = mat.t0;
+		const float x = h2f(m.y) * (h2f(m.z) + tu), y = h2f(m.y >> 16) * (h2f(m.z >> 16) + tv);
+		const float4 texel = fetch_texel_trilinear(P, lambda, x, y, (int)m.w, (int)(m.x & 0xffffu), (int)(m.x >> 16));
+		if (texel.w < 0.5f) return;
+		sd.color = sd.color * xyz(texel);
+		if (flags & MF_HAS2NDDIFFUSEMAP)
+		{
+			const uint4 m1 = mat.t1;
+			const float x1 = h2f(m1.y) * (h2f(m1.z) + tu), y1 = h2f(m1.y >> 16) * (h2f(m1.z >> 16) + tv);
+			const float4 tx = fetch_texel(P, x1, y1, (int)m1.w, (int)(m1.x & 0xffffu), (int)(m1.x >> 16), 0);
+			sd.color = sd.color + (xyz(tx) - mk3(0.5f, 0.5f, 0.5f));
+		}
+	}
+	if (flags & MF_HASNORMALMAP)
+	{
+		const v3 Bt = xyz(tri[6]);
+		const uint32_t part3 = mat.medium_gb;
+		const float c0 = (float)((part3 >> 8) & 255u) - 128.0f;
+		const float n0scale = copysignf(-0.0001f + 0.0001f * cw_expf(0.1f * fabsf(c0)), c0);
+		const uint4 m = mat.n0;
+		const float x = h2f(m.y) * (h2f(m.z) + tu), y = h2f(m.y >> 16) * (h2f(m.z >> 16) + tv);
+		const float4 tx = fetch_texel(P, x, y, (int)m.w, (int)(m.x & 0xffffu), (int)(m.x >> 16), 2);
+		v3 sn = (xyz(tx) - mk3(0.5f, 0.5f, 0.5f)) * 2.0f;
+		sn.x = sn.x * n0scale, sn.y = sn.y * n0scale;
+		if (flags & MF_HAS2NDNORMALMAP)
+		{
+			const float c1 = (float)((part3 >> 16) & 255u) - 128.0f;
+			const float n1scale = copysignf(-0.0001f + 0.0001f * cw_expf(0.1f * c1), c1);
+			const uint4 m1 = mat.n1;
+			const float x1 = h2f(m1.y) * (h2f(m1.z) + tu), y1 = h2f(m1.y >> 16) * (h2f(m1.z >> 16) + tv);
+			const float4 t1x = fetch_texel(P, x1, y1, (int)m1.w, (int)(m1.x & 0xffffu), (int)(m1.x >> 16), 2);
+			v3 l1 = (xyz(t1x) - mk3(0.5f, 0.5f, 0.5f)) * 2.0f;
+			l1.x = l1.x * n1scale, l1.y = l1.y * n1scale;
+			sn = sn + l1;
+		}
+		sn = normalize(sn);
+		fN = normalize(sn.x * T + sn.y * Bt + sn.z * iN);
+	}
+	if (flags & MF_HASROUGHNESSMAP)
+	{
+		const uint4 m = mat.r;
+		const float x = h2f(m.y) * (h2f(m.z) + tu), y = h2f(m.y >> 16) * (h2f(m.z >> 16) + tv);
+		const float4 tx = fetch_texel(P, x, y, (int)m.w, (int)(m.x & 0xffffu), (int)(m.x >> 16), 0);
+		sd.params.x = (sd.params.x & 0x00ffffffu) + ((uint32_t)f2i_sat(tx.y * 255.0f) << 24);
+		sd.params.x = (sd.params.x & 0xffffff00u) + (uint32_t)f2i_sat(tx.x * 255.0f);
+	}
 }
 
 __device__ v3 random_barycentrics(float r0)
@@ -319,8 +441,8 @@ __global__ void __launch_bounds__(128) k_shade(const ShadeParams P, const float4
 	const float t = hd.w;
 	if (P.pathLength == 1) acc->w = acc->w + (prim == 0xffffffffu ? 10000.0f : t);
 	if (prim == 0xffffffffu)
-	{	/* no sky data: black */
-		acc_add(acc, throughput * mk3(0, 0, 0));
+	{
+		acc_add(acc, throughput * sample_sky(P, D));
 		return;
 	}
 	if (P.probe && (int)pixelIdx == P.probePixel && P.pathLength == 1 && sampleIdx == 0)
@@ -332,7 +454,7 @@ __global__ void __launch_bounds__(128) k_shade(const ShadeParams P, const float4
 	Shading sd;
 	v3 N, iN, fN, T;
 	const v3 I = xyz(O4) + t * D;
-	shading_data(P, D, u, v, tri, instIdx, sd, N, iN, fN, T);
+	shading_data(P, D, u, v, P.spreadAngle * t, tri, instIdx, sd, N, iN, fN, T);
 	if (sd.color.x > 1.0f || sd.color.y > 1.0f || sd.color.z > 1.0f)
 	{
 		const float DdotNL = -dot(D, N);

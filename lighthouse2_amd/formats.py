@@ -88,6 +88,33 @@ CoreStats = np.dtype({"names": ["deviceName", "SMcount", "ccMajor", "ccMinor", "
                                   84, 88, 92, 96, 100, 104, 108],
                       "itemsize": 120})
 
+# Host texture descriptor (common_classes.h:336-359, host variant): texel
+# pointer, width, height, flags (8 = HDR), pixelCount (all MIP levels),
+# firstPixel, MIPlevels, storage (0 ARGB32, 1 ARGB128, 2 NRM32).
+CoreTexDesc = np.dtype([("data", "<u8"), ("width", "<u4"), ("height", "<u4"), ("flags", "<u4"), ("pixelCount", "<u4"),
+                        ("firstPixel", "<u4"), ("MIPlevels", "<u4"), ("storage", "<i4")], align=True)
+ARGB32, ARGB128, NRM32 = 0, 1, 2
+MIPLEVELCOUNT = 5   # common_settings.h:57
+
+
+def texture_descs(textures):
+    """CoreTexDesc array for a list of textures (dicts with width, height,
+    flags, storage and `pixels`: uint32 words, or (n,4) float32 for ARGB128,
+    all MIP levels included). The texel arrays must outlive the call that
+    receives the descriptors; they are returned alongside."""
+    d = np.zeros(len(textures), CoreTexDesc)
+    keep = []
+    for i, t in enumerate(textures):
+        px = np.ascontiguousarray(t["pixels"], np.float32 if t["storage"] == ARGB128 else np.uint32)
+        keep.append(px)
+        d[i]["data"] = px.ctypes.data
+        d[i]["width"], d[i]["height"], d[i]["flags"] = t["width"], t["height"], t.get("flags", 0)
+        d[i]["pixelCount"] = len(px) if px.ndim == 1 else px.shape[0]
+        d[i]["MIPlevels"] = t.get("mips", MIPLEVELCOUNT)
+        d[i]["storage"] = t["storage"]
+    return d, keep
+
+
 # Build-tree node of the BVH download (lighthouse2_amd/csrc/bvh.h BuildNode).
 BuildNode = np.dtype([("lo", F3), ("leftFirst", "<i4"), ("hi", F3), ("count", "<i4")])
 

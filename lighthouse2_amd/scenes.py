@@ -8,6 +8,8 @@ order:
   light_triangle    HostTriLight + ConvertToCoreLightTri  lib/RenderSystem/host_light.cpp:25-59
   camera_view       Camera::LookAt + GetView  lib/RenderSystem/camera.cpp:68-71,107-128,
                                               mat4::LookAt common_types.h:519-544
+  mip_chain         HostTexture::ConstructMIPmaps lib/RenderSystem/host_texture.cpp:131-153
+  triangle_lod      triangle LOD of HostMesh  lib/RenderSystem/host_mesh.cpp:308-317
 and generate the seeded synthetic scenes the benchmarks and parity tests use.
 """
 import numpy as np
@@ -52,6 +54,12 @@ class Scene:
         self.point_lights = np.zeros(0, F.CorePointLight)
         self.spot_lights = np.zeros(0, F.CoreSpotLight)
         self.dir_lights = np.zeros(0, F.CoreDirectionalLight)
+        self.textures = []        # dicts: width, height, flags, storage, pixels (all MIP levels)
+        self.sky = None           # ((h, w, 3) float32 texels, (16,) float32 world-to-sky transform)
+
+    def add_texture(self, tex):
+        self.textures.append(tex)
+        return len(self.textures) - 1
 
     def add_material(self, mat):
         self.materials = concat([self.materials, mat], F.CoreMaterial)
@@ -245,3 +253,141 @@ def random_triangle_scene(n, seed=1, with_tris=False):
 def microbench_view(width=1920, height=1080):
     """Camera of the incoherent-traversal workload: (0.5,0.5,-1.2) -> (0.5,0.5,0.5), FOV 40."""
     return camera_view((0.5, 0.5, -1.2), (0.5, 0.5, 0.5), 40.0, width, height)
+
+
+def mip_chain(base, levels=F.MIPLEVELCOUNT):
+    """HostTexture::ConstructMIPmaps (host_texture.cpp:131-153): level i+1
+    averages 2x2 texels of level i (r, g, b summed, shifted right by 2; alpha
+    = min of the four). base: (h, w) uint32 words 0xAARRGGBB. Returns all
+    levels concatenated (HostTexture::PixelsNeeded texels)."""
+    src = np.asarray(base, np.uint32)
+    out = [src.reshape(-1)]
+    for _ in range(1, levels):
+        h, w = src.shape[0] >> 1, src.shape[1] >> 1
+        q = src[:2 * h, :2 * w].reshape(h, 2, w, 2)
+        c = [q[:, 0, :, 0], q[:, 0, :, 1], q[:, 1, :, 0], q[:, 1, :, 1]]
+        a = np.minimum(np.minimum(c[0] >> 24, c[1] >> 24), np.minimum(c[2] >> 24, c[3] >> 24))
+        ch = lambda sh: sum((x >> sh) & 255 for x in c) >> 2
+        src = ((a << 24) + (ch(16) << 16) + (ch(8) << 8) + ch(0)).astype(np.uint32)
+        out.append(src.reshape(-1))
+    return np.concatenate(out)
+
+
+def pack_texels(r, g, b, a):
+    """Bytes -> texel words: byte 0 (x) = b, 1 (y) = g, 2 (z) = r, 3 (w) = a."""
+    c = lambda x: np.clip(np.asarray(x), 0, 255).astype(np.uint32)
+    return (c(a) << 24) | (c(r) << 16) | (c(g) << 8) | c(b)
+
+
+def checker_texture(size=64, cells=8, seed=4, alpha_hole=False):
+    """ARGB32 checkerboard with seeded cell colours (and, optionally, a
+    transparent square of alpha 0 in one corner), 5 MIP levels."""
+    rng = np.random.default_rng(seed)
+    col = rng.integers(40, 256, (cells, cells, 3))
+    y, x = np.mgrid[0:size, 0:size]
+    cy, cx = y * cells // size, x * cells // size
+    c = col[cy, cx]
+    a = np.full((size, size), 255)
+    if alpha_hole:
+        a[: size // 4, : size // 4] = 0
+    px = pack_texels(c[..., 0], c[..., 1], c[..., 2], a)
+    return {"width": size, "height": size, "flags": 0, "storage": F.ARGB32, "pixels": mip_chain(px)}
+
+
+def normal_texture(size=64, seed=5):
+    """NRM32 normal map: smooth bumps, byte k = (n_k * 0.5 + 0.5) * 255."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:size, 0:size] / size
+    fx, fy = rng.uniform(1, 4, 2)
+    nx, ny = 0.5 * np.sin(2 * np.pi * fx * x), 0.5 * np.cos(2 * np.pi * fy * y)
+    nz = np.sqrt(np.maximum(0.0, 1 - nx * nx - ny * ny))
+    enc = lambda v: np.round((v * 0.5 + 0.5) * 255)
+    px = pack_texels(enc(nz), enc(ny), enc(nx), np.full((size, size), 255))
+    return {"width": size, "height": size, "flags": 0, "storage": F.NRM32, "pixels": mip_chain(px)}
+
+
+def roughness_texture(size=64, seed=6):
+    """ARGB32 roughness/metalness map (glTF layout): byte 1 (y) roughness in
+    [0.3, 1], byte 0 (x) metalness."""
+    rng = np.random.default_rng(seed)
+    rough = rng.integers(77, 256, (size, size))
+    metal = rng.integers(0, 256, (size, size))
+    px = pack_texels(0, rough, metal, np.full((size, size), 255))
+    return {"width": size, "height": size, "flags": 0, "storage": F.ARGB32, "pixels": mip_chain(px)}
+
+
+def triangle_lod(tris, tex_w, tex_h):
+    """tri.LOD = 0.5 log2(Ta / Pa), Ta = texel area of the uv triangle, Pa =
+    twice the world-space area (host_mesh.cpp:308-317), in float32."""
+    ta = f32(tex_w * tex_h) * np.abs((tris["u1"] - tris["u0"]) * (tris["v2"] - tris["v0"]) -
+                                     (tris["u2"] - tris["u0"]) * (tris["v1"] - tris["v0"])).astype(f32)
+    e1 = (tris["vertex1"] - tris["vertex0"]).astype(f32)
+    e2 = (tris["vertex2"] - tris["vertex0"]).astype(f32)
+    cr = np.cross(e1, e2).astype(f32)
+    pa = np.sqrt((cr * cr).sum(1)).astype(f32)
+    return (f32(0.5) * np.log2(ta / pa)).astype(f32)
+
+
+def sky_dome(width=64, height=32, seed=8, rotate_deg=30.0):
+    """Procedural equirect sky: a vertical gradient plus seeded per-texel
+    noise, and a world-to-sky rotation about the y axis."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(height)[:, None] + 0.5) / height
+    base = np.stack([0.2 + 0.8 * v, 0.3 + 0.5 * v, 1.0 - 0.6 * v], -1) * np.ones((1, width, 1))
+    px = (base + rng.uniform(0, 0.3, (height, width, 3))).astype(f32)
+    a = np.radians(rotate_deg)
+    T = np.eye(4, dtype=f32)
+    T[0, 0], T[0, 2], T[2, 0], T[2, 2] = np.cos(a), np.sin(a), -np.sin(a), np.cos(a)
+    return px, T.reshape(16).astype(f32)
+
+
+def _planar_uv(tris, axes, scale):
+    for k in range(3):
+        p = tris[f"vertex{k}"]
+        tris[f"u{k}"] = (p[:, axes[0]] * scale).astype(f32)
+        tris[f"v{k}"] = (p[:, axes[1]] * scale).astype(f32)
+
+
+def textured_cornell_box(seed=1234, sphere_tris=4000, tex_size=64):
+    """The Cornell box with texture maps and a skydome: floor with a diffuse
+    checkerboard holding a transparent corner, back wall diffuse + detail
+    diffuse, ceiling a roughness map, left wall a normal map, right wall two
+    normal maps; planar uv (2 repeats per wall) and the reference's triangle
+    LOD; the open front shows the sky."""
+    s = cornell_box(seed=seed, sphere_tris=sphere_tris)
+    checker = s.add_texture(checker_texture(tex_size, 8, seed=4, alpha_hole=True))
+    detail = s.add_texture(checker_texture(tex_size, 4, seed=9))
+    rough = s.add_texture(roughness_texture(tex_size, seed=6))
+    nrm = s.add_texture(normal_texture(tex_size, seed=5))
+    nrm2 = s.add_texture(normal_texture(tex_size, seed=7))
+    axes = [(0, 2), (0, 2), (0, 1), (2, 1), (2, 1)]   # floor, ceiling, back, left, right
+    for wall, ax in enumerate(axes):
+        v, t = s.meshes[wall]
+        _planar_uv(t, ax, 2.0)
+        t["LOD"] = triangle_lod(t, tex_size, tex_size)
+        m = t["material"][0]
+        mat = s.materials[m:m + 1]
+        use = {0: [("color", checker)], 1: [("roughness", rough)], 2: [("color", checker), ("detailColor", detail)],
+               3: [("normals", nrm)], 4: [("normals", nrm), ("detailNormals", nrm2)]}[wall]
+        for field, tid in use:
+            mat[field]["textureID"] = tid
+            mat[field]["uvscale"] = 1.0
+            mat[field]["uvoffset"] = 0.0
+        s.materials[m] = mat[0]
+    s.sky = sky_dome()
+    return s
+
+
+def sky_scene(n=64, seed=3):
+    """A few triangles in a mostly empty view, so most primary rays see the
+    skydome."""
+    s = Scene()
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.3, 0.7, (n, 1, 3)).astype(f32)
+    e = rng.normal(0, 0.02, (n, 3, 3)).astype(f32)
+    v = (c + e).astype(f32)
+    m = s.add_material(F.lambert_material((0.6, 0.5, 0.4)))
+    verts, tris = add_triangles(v[:, 0], v[:, 1], v[:, 2], m)
+    s.add_instance(s.add_mesh(verts, tris))
+    s.sky = sky_dome(96, 48, seed=11, rotate_deg=75.0)
+    return s

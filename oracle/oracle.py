@@ -35,6 +35,9 @@ def lib():
             "orc_float_to_half": (ctypes.c_uint16, [ctypes.c_float]), "orc_half_to_float": (ctypes.c_float, [ctypes.c_uint16]),
             "orc_mat4_inverse": (None, [P, P]), "orc_convert_material": (None, [P, P]), "orc_sizeof_omat": (I, []),
             "orc_tri_intersect": (I, [P, P, P, P]),
+            "orc_set_textures": (None, [P, P, I]), "orc_set_sky": (None, [P, P, I, I, P]),
+            "orc_sample_sky": (None, [P, P, I, P]), "orc_fetch_texels": (None, [P, I, P, I, I, ctypes.c_float, P]),
+            "orc_math": (None, [I, P, P, I, P]),
         }
         for n, (r, a) in sigs.items():
             f = getattr(L, n)
@@ -78,11 +81,38 @@ class OracleScene:
         for m, T in scene.instances:
             self.add_instance(m, T)
         self.finalize()
+        if getattr(scene, "sky", None) is not None:
+            self.set_sky(*scene.sky)
+        if getattr(scene, "textures", None):
+            self.set_textures(scene.textures)
         mats = np.ascontiguousarray(scene.materials)
         lib().orc_set_materials(self.h, _p(mats), len(mats))
         a = [np.ascontiguousarray(x) for x in (scene.tri_lights, scene.point_lights, scene.spot_lights, scene.dir_lights)]
         self._keep += [mats] + a
         lib().orc_set_lights(self.h, _p(a[0]), len(a[0]), _p(a[1]), len(a[1]), _p(a[2]), len(a[2]), _p(a[3]), len(a[3]))
+
+    def set_textures(self, textures):
+        from lighthouse2_amd import formats as F
+        d, keep = F.texture_descs(textures)
+        lib().orc_set_textures(self.h, _p(d), len(d))
+        del keep
+
+    def set_sky(self, pixels, T=None):
+        px = np.ascontiguousarray(pixels, np.float32)
+        t = None if T is None else np.ascontiguousarray(T, np.float32).reshape(16)
+        lib().orc_set_sky(self.h, _p(px), px.shape[1], px.shape[0], _p(t))
+
+    def sample_sky(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros_like(d)
+        lib().orc_sample_sky(self.h, _p(d), len(d), _p(out))
+        return out
+
+    def fetch_texels(self, tex_id, uv, lam=None):
+        u = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((len(u), 4), np.float32)
+        lib().orc_fetch_texels(self.h, tex_id, _p(u), len(u), 0 if lam is None else 1, 0.0 if lam is None else lam, _p(out))
+        return out
 
     def trace(self, rays, any_hit=False, brute=False, threads=8, counters=False):
         """rays (n, 8) float32 {O, tmin, D, tmax}. Closest: (n, 4) records with
@@ -161,6 +191,16 @@ def render(scene_or_oracle, view, w, h, spp=1, frames=1, max_depth=3, eps=1e-3, 
             acc64 += blk.astype(np.float64)
         taken += spp
     return acc32.reshape(h, w, 4), acc64.reshape(h, w, 4), taken
+
+
+def math(which, a, b=None):
+    """The oracle's own transcendental functions: 'atan2' (a = y, b = x), 'acos', 'log2', 'exp'."""
+    k = {"atan2": 0, "acos": 1, "log2": 2, "exp": 3}[which]
+    a = np.ascontiguousarray(a, np.float32)
+    bb = np.ascontiguousarray(a if b is None else b, np.float32)
+    out = np.zeros_like(a)
+    lib().orc_math(k, _p(a), _p(bb), len(a), _p(out))
+    return out
 
 
 def sincos(x):

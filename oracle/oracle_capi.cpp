@@ -43,7 +43,75 @@ OAPI void orc_set_materials(void* sp, const CoreMaterial* m, int n)
 {
 	RenderScene& s = *(RenderScene*)sp;
 	s.mats.resize(n);
-	for (int i = 0; i < n; i++) s.mats[i] = convert_material(m[i]);
+	for (int i = 0; i < n; i++) s.mats[i] = convert_material(m[i], s.tex);
+}
+
+/* RenderCore::SetTextures + SyncStorageType (lib/RenderCore_PrimeRef/
+ * rendercore.cpp:283-355): texels of each storage kind go to one continuous
+ * array in texture order; firstPixel is the texture's start in its array.
+ * ARGB128 texels are copied whole (float4 each). */
+OAPI void orc_set_textures(void* sp, const lh2hip::CoreTexDesc* t, int n)
+{
+	RenderScene& s = *(RenderScene*)sp;
+	s.tex.assign(n, TexInfo());
+	s.argb32.clear(), s.nrm32.clear(), s.argb128.clear();
+	for (int i = 0; i < n; i++)
+	{
+		TexInfo& d = s.tex[i];
+		d.width = t[i].width, d.height = t[i].height, d.flags = t[i].flags, d.storage = t[i].storage;
+		if (t[i].storage == 1)
+		{
+			d.firstPixel = (uint)s.argb128.size();
+			const f4* src = (const f4*)t[i].fdata;
+			s.argb128.insert(s.argb128.end(), src, src + t[i].pixelCount);
+		}
+		else
+		{
+			std::vector<uint>& dst = t[i].storage == 2 ? s.nrm32 : s.argb32;
+			d.firstPixel = (uint)dst.size();
+			dst.insert(dst.end(), t[i].idata, t[i].idata + t[i].pixelCount);
+		}
+	}
+}
+
+/* RenderCore::SetSkyData (rendercore.cpp:466-478): float3 texels stored as float4, w = 0. */
+OAPI void orc_set_sky(void* sp, const float* pixels3, int w, int h, const float* worldToSky16)
+{
+	RenderScene& s = *(RenderScene*)sp;
+	s.skyW = w, s.skyH = h;
+	s.sky.resize((size_t)w * h);
+	for (size_t i = 0; i < s.sky.size(); i++) s.sky[i] = { pixels3[3 * i], pixels3[3 * i + 1], pixels3[3 * i + 2], 0.0f };
+	for (int i = 0; i < 16; i++) s.worldToSky[i] = worldToSky16 ? worldToSky16[i] : (i % 5 == 0 ? 1.0f : 0.0f);
+}
+
+OAPI void orc_sample_sky(void* sp, const float* dirs3, int n, float* out3)
+{
+	const RenderScene& s = *(const RenderScene*)sp;
+	for (int i = 0; i < n; i++)
+	{
+		const f3 c = sample_sky(s, mk3(dirs3[3 * i], dirs3[3 * i + 1], dirs3[3 * i + 2]));
+		out3[3 * i] = c.x, out3[3 * i + 1] = c.y, out3[3 * i + 2] = c.z;
+	}
+}
+
+/* uv (n, 2) -> float4 texels of texture `id` (bilinear, or trilinear at `lambda` when trilinear != 0). */
+OAPI void orc_fetch_texels(void* sp, int id, const float* uv, int n, int trilinear, float lambda, float* out4)
+{
+	const RenderScene& s = *(const RenderScene*)sp;
+	const TexInfo& t = s.tex[id];
+	for (int i = 0; i < n; i++)
+	{
+		const f4 r = trilinear ? fetch_texel_trilinear(s, lambda, uv[2 * i], uv[2 * i + 1], (int)t.firstPixel, (int)t.width, (int)t.height)
+			: fetch_texel(s, uv[2 * i], uv[2 * i + 1], (int)t.firstPixel, (int)t.width, (int)t.height, t.storage);
+		out4[4 * i] = r.x, out4[4 * i + 1] = r.y, out4[4 * i + 2] = r.z, out4[4 * i + 3] = r.w;
+	}
+}
+
+/* The self-written transcendental functions, elementwise (which: 0 atan2(a, b), 1 acos(a), 2 log2(a), 3 exp(a)). */
+OAPI void orc_math(int which, const float* a, const float* b, int n, float* out)
+{
+	for (int i = 0; i < n; i++)
+		out[i] = which == 0 ? lh_atan2f(a[i], b[i]) : which == 1 ? lh_acosf(a[i]) : which == 2 ? lh_log2f(a[i]) : cw_expf(a[i]);
 }
 
 OAPI void orc_set_lights(void* sp, const CoreLightTri* t, int nt, const CorePointLight* p, int np,
@@ -158,7 +226,7 @@ OAPI void orc_sincos(float x, float* s, float* c) { cw_sincos(x, s, c); }
 OAPI uint16_t orc_float_to_half(float f) { return float_to_half(f); }
 OAPI float orc_half_to_float(uint16_t h) { return half_to_float(h); }
 OAPI void orc_mat4_inverse(const float* m, float* out) { mat4_inverse(m, out); }
-OAPI void orc_convert_material(const CoreMaterial* m, OMat* out) { *out = convert_material(*m); }
+OAPI void orc_convert_material(const CoreMaterial* m, OMat* out) { *out = convert_material(*m, std::vector<TexInfo>()); }
 OAPI int orc_sizeof_omat() { return (int)sizeof(OMat); }
 OAPI int orc_tri_intersect(const float* O3, const float* D3, const float* v9, float* out4)
 {

@@ -32,7 +32,7 @@ static inline uint tochar(float a)
 static inline uint touint4(float a, float b, float c, float d) { return tochar(a) + (tochar(b) << 8) + (tochar(c) << 16) + (tochar(d) << 24); }
 
 /* RenderCore::SetMaterials (lib/RenderCore_PrimeRef/rendercore.cpp:360-402). */
-OMat convert_material(const CoreMaterial& m)
+OMat convert_material(const CoreMaterial& m, const std::vector<TexInfo>& tex)
 {
 	OMat g;
 	memset(&g, 0, sizeof(g));
@@ -46,7 +46,22 @@ OMat convert_material(const CoreMaterial& m)
 	g.params[1] = touint4(m.specularTint.value, m.anisotropic.value, m.sheen.value, m.sheenTint.value);
 	g.params[2] = touint4(m.clearcoat.value, m.clearcoatGloss.value, m.transmission.value, 0.0f);
 	g.params[3] = f2u(m.eta.value);
-	g.flags = (m.eta.value < 1 ? M_ISDIELECTRIC : 0) +
+	const auto map = [&](uint* d, int id, decltype(m.color.uvscale) scale, decltype(m.color.uvoffset) offs) {
+		if (id < 0 || id >= (int)tex.size()) return;
+		const TexInfo& t = tex[id];
+		d[0] = (t.width & 0xffffu) | ((t.height & 0xffffu) << 16);
+		d[1] = (uint)float_to_half(scale.x) | ((uint)float_to_half(scale.y) << 16);
+		d[2] = (uint)float_to_half(offs.x) | ((uint)float_to_half(offs.y) << 16);
+		d[3] = t.firstPixel;
+	};
+	map(g.maps[0], m.color.textureID, m.color.uvscale, m.color.uvoffset);
+	map(g.maps[1], m.detailColor.textureID, m.detailColor.uvscale, m.detailColor.uvoffset);
+	map(g.maps[2], m.normals.textureID, m.normals.uvscale, m.normals.uvoffset);
+	map(g.maps[3], m.detailNormals.textureID, m.detailNormals.uvscale, m.detailNormals.uvoffset);
+	map(g.maps[4], m.specular.textureID, m.specular.uvscale, m.specular.uvoffset);
+	map(g.maps[5], m.roughness.textureID, m.roughness.uvscale, m.roughness.uvoffset);
+	const bool hdr = m.color.textureID >= 0 && m.color.textureID < (int)tex.size() && (tex[m.color.textureID].flags & 8u);
+	g.flags = (m.eta.value < 1 ? M_ISDIELECTRIC : 0) + (hdr ? M_DIFFUSEMAPISHDR : 0) +
 		(m.color.textureID != -1 ? M_HASDIFFUSEMAP : 0) + (m.normals.textureID != -1 ? M_HASNORMALMAP : 0) +
 		(m.specular.textureID != -1 ? M_HASSPECULARITYMAP : 0) + (m.roughness.textureID != -1 ? M_HASROUGHNESSMAP : 0) +
 		(m.detailNormals.textureID != -1 ? M_HAS2NDNORMALMAP : 0) + (m.detailColor.textureID != -1 ? M_HAS2NDDIFFUSEMAP : 0) +
@@ -76,10 +91,90 @@ static f3 consistent_normal(f3 D, f3 iN, float alpha)
 	return normalize(D + Rc);
 }
 
+/* __uchar4_to_float4 (sampling_shared.h:29-33): byte k of the texel word is
+ * component k, scaled by 1/256. */
+static inline f4 unpack_texel(uint t)
+{
+	const float r = 1.0f / 256.0f;
+	f4 o = { (float)(t & 255u) * r, (float)((t >> 8) & 255u) * r, (float)((t >> 16) & 255u) * r, (float)(t >> 24) * r };
+	return o;
+}
+
+static inline f4 texel_at(const RenderScene& s, int storage, int idx)
+{
+	if (storage == 1) return s.argb128[idx];
+	return unpack_texel(storage == 2 ? s.nrm32[idx] : s.argb32[idx]);
+}
+
+/* FetchTexel, BILINEAR on (lib/CUDA/shared_kernel_code/sampling_shared.h:35-79;
+ * lib/RenderCore_PrimeRef/core_settings.h:31): wrap by +1000, bilinear
+ * weights with w3 = 1 - (w0 + w1 + w2). */
+f4 fetch_texel(const RenderScene& s, float tu, float tv, int o, int w, int h, int storage)
+{
+	const float tcx = (std::max(tu + 1000.0f, 0.0f) * (float)w) - 0.5f;
+	const float tcy = (std::max(tv + 1000.0f, 0.0f) * (float)h) - 0.5f;
+	const int iu = f2i_sat(tcx) % w, iv = f2i_sat(tcy) % h;
+	const float fu = tcx - floorf(tcx), fv = tcy - floorf(tcy);
+	const float w0 = (1 - fu) * (1 - fv), w1 = fu * (1 - fv), w2 = (1 - fu) * fv, w3 = 1 - (w0 + w1 + w2);
+	const int iu1 = (iu + 1) % w, iv1 = (iv + 1) % h;
+	const f4 p0 = texel_at(s, storage, o + iu + iv * w), p1 = texel_at(s, storage, o + iu1 + iv * w);
+	const f4 p2 = texel_at(s, storage, o + iu + iv1 * w), p3 = texel_at(s, storage, o + iu1 + iv1 * w);
+	f4 r;
+	r.x = p0.x * w0 + p1.x * w1 + p2.x * w2 + p3.x * w3;
+	r.y = p0.y * w0 + p1.y * w1 + p2.y * w2 + p3.y * w3;
+	r.z = p0.z * w0 + p1.z * w1 + p2.z * w2 + p3.z * w3;
+	r.w = p0.w * w0 + p1.w * w1 + p2.w * w2 + p3.w * w3;
+	return r;
+}
+
+/* FetchTexelTrilinear (sampling_shared.h:81-104), MIPLEVELCOUNT 5
+ * (common_settings.h:57); level offsets by the closed form of the reference,
+ * scale * (1 - 4^-level). */
+f4 fetch_texel_trilinear(const RenderScene& s, float lambda, float tu, float tv, int offset, int width, int height)
+{
+	int level0 = 0, level1 = 0;
+	float f = 0;
+	if (lambda >= 0) level0 = std::min(4, f2i_sat(lambda)), level1 = std::min(4, level0 + 1), f = lambda - floorf(lambda);
+	const float scale = (float)(width * height) * 1.3333333333f;
+	const int o0 = offset + f2i_sat(scale * (1 - u2f((uint)(127 - 2 * level0) << 23)));
+	const int o1 = offset + f2i_sat(scale * (1 - u2f((uint)(127 - 2 * level1) << 23)));
+	const f4 p0 = fetch_texel(s, tu, tv, o0, width >> level0, height >> level0, 0);
+	const f4 p1 = fetch_texel(s, tu, tv, o1, width >> level1, height >> level1, 0);
+	f4 r;
+	r.x = (1 - f) * p0.x + f * p1.x, r.y = (1 - f) * p0.y + f * p1.y;
+	r.z = (1 - f) * p0.z + f * p1.z, r.w = (1 - f) * p0.w + f * p1.w;
+	return r;
+}
+
+/* SampleSkydome(-worldToSky.TransformVector(D)) (lib/CUDA/shared_kernel_code/
+ * tools_shared.h:184-202; common_types.h:608-613; kernels/pathtracer.h:73-78):
+ * equirect texel (uint)(w * phi / 2pi - 0.5), (uint)(h * theta / pi - 0.5). */
+f3 sample_sky(const RenderScene& s, f3 D)
+{
+	if (s.skyW <= 0 || s.skyH <= 0) return mk3(0, 0, 0);
+	const float* m = s.worldToSky;
+	const f3 d = mk3(-(m[0] * D.x + m[1] * D.y + m[2] * D.z), -(m[4] * D.x + m[5] * D.y + m[6] * D.z), -(m[8] * D.x + m[9] * D.y + m[10] * D.z));
+	const float p = lh_atan2f(d.y, d.x);
+	const float phi = p < 0 ? (p + 2 * PI) : p;
+	const float theta = lh_acosf(std::min(std::max(d.z, -1.0f), 1.0f));
+	const uint u = f2u_sat((float)s.skyW * phi * 0.15915494309189533577f - 0.5f);
+	const uint v = f2u_sat((float)s.skyH * theta * 0.31830988618379067154f - 0.5f);
+	const uint idx = u + v * (uint)s.skyW;
+	if (idx >= (uint)s.skyW * (uint)s.skyH) return mk3(0, 0, 0);
+	const f4 t = s.sky[idx];
+	return mk3(t.x, t.y, t.z);
+}
+
+static inline float h2f(uint bits16) { return half_to_float((uint16_t)(bits16 & 0xffffu)); }
+
 /* SetupFrame + GetShadingData, OPTIXPRIMEBUILD barycentric convention
- * (lib/CUDA/shared_kernel_code/material_shared.h:42-94,96-128), CONSISTENTNORMALS
- * on (lib/RenderCore_PrimeRef/core_settings.h:26); no texture maps. */
-static void shading_data(const RenderScene& s, f3 D, float u, float v, const CoreTri& tri, int instIdx,
+ * (lib/CUDA/shared_kernel_code/material_shared.h:42-94,96-206), CONSISTENTNORMALS
+ * on (lib/RenderCore_PrimeRef/core_settings.h:26). Texture maps: diffuse
+ * (trilinear, LOD TRI_LOD + log2(coneWidth / |D.N|); alpha below 0.5 ends the
+ * lookup with the untextured colour), detail diffuse, normal maps (tangent
+ * frame T, B of the triangle, not instance-transformed, as the reference) and
+ * roughness. __expf of the normal-map scale is cw_expf. Returns the ALPHA flag. */
+static bool shading_data(const RenderScene& s, f3 D, float u, float v, float coneWidth, const CoreTri& tri, int instIdx,
 	Shading& sd, f3& N, f3& iN, f3& fN, f3& T)
 {
 	const OMat& mat = s.mats[tri.material];
@@ -98,6 +193,71 @@ static void shading_data(const RenderScene& s, f3 D, float u, float v, const Cor
 	const float alpha = u * tri.alpha.x + v * tri.alpha.y + w * tri.alpha.z;
 	iN = (backSide ? -1.0f : 1.0f) * consistent_normal(D * -1.0f, backSide ? (iN * -1.0f) : iN, alpha);
 	fN = iN;
+	const uint fl = mat.flags;
+	float tu = 0, tv = 0;
+	if (fl & (M_HASDIFFUSEMAP | M_HAS2NDDIFFUSEMAP | M_HASSPECULARITYMAP | M_HASNORMALMAP | M_HAS2NDNORMALMAP | M_HASROUGHNESSMAP))
+	{
+		const float ww = 1 - (u + v);
+		tu = u * tri.u0 + v * tri.u1 + ww * tri.u2;
+		tv = u * tri.v0 + v * tri.v1 + ww * tri.v2;
+	}
+	const auto uvmap = [&](const uint* m, float& x, float& y) {
+		x = h2f(m[1]) * (h2f(m[2]) + tu);
+		y = h2f(m[1] >> 16) * (h2f(m[2] >> 16) + tv);
+	};
+	if (fl & M_HASDIFFUSEMAP)
+	{
+		const float lambda = tri.LOD + lh_log2f(coneWidth * (1.0f / fabsf(dot(D, N))));
+		const uint* m = mat.maps[0];
+		float x, y;
+		uvmap(m, x, y);
+		const f4 texel = fetch_texel_trilinear(s, lambda, x, y, (int)m[3], (int)(m[0] & 0xffffu), (int)(m[0] >> 16));
+		if (texel.w < 0.5f) return true;
+		sd.color = sd.color * mk3(texel.x, texel.y, texel.z);
+		if (fl & M_HAS2NDDIFFUSEMAP)
+		{
+			const uint* m1 = mat.maps[1];
+			uvmap(m1, x, y);
+			const f4 t1 = fetch_texel(s, x, y, (int)m1[3], (int)(m1[0] & 0xffffu), (int)(m1[0] >> 16), 0);
+			sd.color = sd.color + (mk3(t1.x, t1.y, t1.z) - mk3(0.5f, 0.5f, 0.5f));
+		}
+	}
+	if (fl & M_HASNORMALMAP)
+	{
+		const f3 Bt = f3of(tri.B);
+		const uint part3 = (uint)mat.transm[1] | ((uint)mat.transm[2] << 16);
+		const float c0 = (float)((part3 >> 8) & 255) - 128.0f;
+		const float n0scale = copysignf(-0.0001f + 0.0001f * cw_expf(0.1f * fabsf(c0)), c0);
+		const uint* m = mat.maps[2];
+		float x, y;
+		uvmap(m, x, y);
+		const f4 t = fetch_texel(s, x, y, (int)m[3], (int)(m[0] & 0xffffu), (int)(m[0] >> 16), 2);
+		f3 sn = (mk3(t.x, t.y, t.z) - mk3(0.5f, 0.5f, 0.5f)) * 2.0f;
+		sn.x = sn.x * n0scale, sn.y = sn.y * n0scale;
+		if (fl & M_HAS2NDNORMALMAP)
+		{
+			const float c1 = (float)((part3 >> 16) & 255) - 128.0f;
+			const float n1scale = copysignf(-0.0001f + 0.0001f * cw_expf(0.1f * c1), c1);
+			const uint* m1 = mat.maps[3];
+			uvmap(m1, x, y);
+			const f4 t1 = fetch_texel(s, x, y, (int)m1[3], (int)(m1[0] & 0xffffu), (int)(m1[0] >> 16), 2);
+			f3 l1 = (mk3(t1.x, t1.y, t1.z) - mk3(0.5f, 0.5f, 0.5f)) * 2.0f;
+			l1.x = l1.x * n1scale, l1.y = l1.y * n1scale;
+			sn = sn + l1;
+		}
+		sn = normalize(sn);
+		fN = normalize(sn.x * T + sn.y * Bt + sn.z * iN);
+	}
+	if (fl & M_HASROUGHNESSMAP)
+	{
+		const uint* m = mat.maps[5];
+		float x, y;
+		uvmap(m, x, y);
+		const f4 t = fetch_texel(s, x, y, (int)m[3], (int)(m[0] & 0xffffu), (int)(m[0] >> 16), 0);
+		sd.params[0] = (sd.params[0] & 0x00ffffffu) + ((uint)f2i_sat(t.y * 255.0f) << 24);
+		sd.params[0] = (sd.params[0] & 0xffffff00u) + (uint)f2i_sat(t.x * 255.0f);
+	}
+	return false;
 }
 
 /* RandomBarycentrics (lib/RenderSystem/common_functions.h:52-71); the 0:32
@@ -278,17 +438,6 @@ static bool refract_l(f3 wi, f3 n, float eta, f3& wt)
 	wt = eta * (wi * -1.0f) + (eta * cosThetaI - cosThetaT) * n;
 	return true;
 }
-/* expf of Beer's law: exp(x) = 2^k * exp(r), k = round(x/ln2), Cody-Waite ln2
- * split, Taylor series of exp(r) to r^7 on |r| <= ln2/2. */
-static float cw_expf(float x)
-{
-	if (x < -87.0f) return 0.0f;
-	if (x > 88.0f) return INFINITY;
-	const float k = floorf(x * 1.44269504f + 0.5f);
-	const float r = (x - k * 0.693359375f) - k * -2.12194440e-4f;
-	const float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.16666667f + r * (4.1666668e-2f + r * (8.3333333e-3f + r * (1.3888889e-3f + r * 1.9841270e-4f))))));
-	return p * u2f((uint)((int)k + 127) << 23);
-}
 static f3 evaluate_bsdf(const Shading& sd, f3 iN, f3 wo, f3 wi, float& pdf)
 {
 	(void)wo;
@@ -399,8 +548,8 @@ static void trace_path(const RenderScene& s, const RenderParams& p, int job, uin
 		const uint R0 = (uint)pass * 7907u + (uint)pathLength * 91771u;
 		if (pathLength == 1) aw += h.prim < 0 ? 10000 : h.t;
 		if (h.prim < 0)
-		{	/* sky: black (no sky data in the parity scenes) */
-			const f3 c = throughput * mk3(0, 0, 0);
+		{
+			const f3 c = throughput * sample_sky(s, D);
 			ax += c.x, ay += c.y, az += c.z;
 			break;
 		}
@@ -410,7 +559,7 @@ static void trace_path(const RenderScene& s, const RenderParams& p, int job, uin
 		Shading sd;
 		f3 N, iN, fN, T;
 		const f3 I = O + t * D;
-		shading_data(s, D, u, v, tri, h.inst, sd, N, iN, fN, T);
+		shading_data(s, D, u, v, p.view.spreadAngle * t, tri, h.inst, sd, N, iN, fN, T);
 		if (sd.emissive())
 		{
 			const float DdotNL = -dot(D, N);

@@ -21,10 +21,17 @@ struct OMat
 	uint16_t diffuse[3], transm[3];
 	uint flags;
 	uint params[4];
+	/* texture maps t0 (diffuse), t1 (detail diffuse), n0, n1 (normal maps), s, r
+	 * (specularity, roughness): x = width | height << 16, y = half uscale |
+	 * half vscale << 16, z = half uoffs | half voffs << 16, w = first texel. */
+	uint maps[6][4];
 };
 
+/* What SetMaterials needs of a texture (CoreTexDesc, host variant). */
+struct TexInfo { uint width, height, flags, firstPixel; int storage; };
+
 enum { S_SPECULAR = 1 };
-enum { M_ISDIELECTRIC = 1, M_HASDIFFUSEMAP = 4, M_HASNORMALMAP = 8, M_HASSPECULARITYMAP = 16,
+enum { M_ISDIELECTRIC = 1, M_DIFFUSEMAPISHDR = 2, M_HASDIFFUSEMAP = 4, M_HASNORMALMAP = 8, M_HASSPECULARITYMAP = 16,
 	M_HASROUGHNESSMAP = 32, M_HAS2NDNORMALMAP = 128, M_HAS2NDDIFFUSEMAP = 512, M_HASSMOOTHNORMALS = 2048, M_HASALPHA = 4096 };
 
 struct RenderScene : Scene
@@ -35,13 +42,26 @@ struct RenderScene : Scene
 	std::vector<CorePointLight> pointLights;
 	std::vector<CoreSpotLight> spotLights;
 	std::vector<CoreDirectionalLight> dirLights;
+	/* textures: continuous texel arrays per storage kind (SyncStorageType) */
+	std::vector<TexInfo> tex;
+	std::vector<uint> argb32, nrm32;
+	std::vector<f4> argb128;
+	/* skydome: float4 texels (w = 0) and the world-to-sky transform */
+	std::vector<f4> sky;
+	int skyW = 0, skyH = 0;
+	float worldToSky[16] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 };
 };
 
-OMat convert_material(const CoreMaterial& m);
+OMat convert_material(const CoreMaterial& m, const std::vector<TexInfo>& tex);
+
+/* Texel lookups of the shading code (exposed for the tests). */
+f4 fetch_texel(const RenderScene& s, float tu, float tv, int o, int w, int h, int storage);
+f4 fetch_texel_trilinear(const RenderScene& s, float lambda, float tu, float tv, int o, int w, int h);
+f3 sample_sky(const RenderScene& s, f3 D);
 
 struct RenderParams
 {
-	ViewPyramid view;
+	ViewPyramid view;   /* spreadAngle sets the texture LOD cone */
 	int w, h, spp, maxDepth;
 	float eps;
 };

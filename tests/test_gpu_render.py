@@ -93,3 +93,29 @@ def test_no_lights_renders_only_emitters(torch_cuda):
     assert np.array_equal(acc, r64)
     vals = set(np.unique(img[..., :3]).tolist())
     assert vals <= {0.0, 7.5, 15.0} and 15.0 in vals  # only directly seen emitters (1 or 2 of 2 frames)
+
+
+def test_textured_cornell_box_with_sky(torch_cuda):
+    """Diffuse (trilinear, alpha-masked corner), detail diffuse, normal and
+    roughness maps and the skydome through the open front: accumulators
+    identical to the oracle's."""
+    s = scenes.textured_cornell_box(sphere_tris=4000)
+    view = scenes.cornell_view(96, 96)
+    img, acc, _ = render_gpu(s, view, 96, 96, 2, 3, 4, seed=77)
+    r32, r64, n = orc.render(s, view, 96, 96, 2, 3, 4, 1e-3, seed=77)
+    assert np.array_equal(acc, r64)
+    assert errors(img, r64 / n)[1] <= REL_TOL
+    plain = scenes.cornell_box(seed=1234, sphere_tris=4000)       # same box without maps and sky
+    _, p64, _ = orc.render(plain, view, 96, 96, 2, 3, 4, 1e-3, seed=77)
+    assert not np.array_equal(p64, r64)
+
+
+def test_sky_seen_by_primary_rays(torch_cuda):
+    s = scenes.sky_scene()
+    view = scenes.camera_view((0.5, 0.5, -1.2), (0.5, 0.5, 0.5), 70.0, 128, 96)
+    img, acc, _ = render_gpu(s, view, 128, 96, 1, 2, 2, seed=5)
+    r32, r64, n = orc.render(s, view, 128, 96, 1, 2, 2, 1e-3, seed=5)
+    assert np.array_equal(acc, r64)
+    sky_px = s.sky[0].reshape(-1, 3)
+    seen = np.isin(img[..., 0], sky_px[:, 0])
+    assert seen.mean() > 0.5                 # most pixels show a sky texel exactly

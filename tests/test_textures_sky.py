@@ -1,0 +1,138 @@
+"""Texture maps and skydome of the shading path: the
+oracle's texel fetch, MIP chain, sky lookup and self-written transcendental
+functions, pinned against closed forms and float64 numpy. CPU only."""
+import numpy as np
+import pytest
+
+from lighthouse2_amd import scenes
+from lighthouse2_amd import formats as F
+import oracle as orc
+
+
+def test_atan2_acos_log2_exp_accuracy():
+    rng = np.random.default_rng(0)
+    y = rng.standard_normal(20000).astype(np.float32)
+    x = rng.standard_normal(20000).astype(np.float32)
+    x[:8], y[:8] = [1, -1, 0, 0, 1, -1, 2, -2], [0, 0, 1, -1, 1, -1, -2, 2]
+    got = orc.math("atan2", y, x).astype(np.float64)
+    ref = np.arctan2(y.astype(np.float64), x.astype(np.float64))
+    assert np.max(np.abs(got - ref)) < 5e-7
+    z = rng.uniform(-1, 1, 20000).astype(np.float32)
+    z[:3] = [-1, 0, 1]
+    assert np.max(np.abs(orc.math("acos", z) - np.arccos(z.astype(np.float64)))) < 1e-6
+    v = np.exp(rng.uniform(-60, 60, 20000)).astype(np.float32)
+    v[:4] = [1.0, 2.0, 0.5, 1e-40]              # exact powers of two, a subnormal
+    lg = orc.math("log2", v)
+    ref = np.log2(v.astype(np.float64))
+    assert np.all(np.abs(lg - ref) <= 3e-7 + 1e-7 * np.abs(ref))   # about one ulp of the result
+    assert lg[0] == 0.0 and lg[1] == 1.0 and lg[2] == -1.0
+    assert np.isneginf(orc.math("log2", np.zeros(1, np.float32)))[0]
+    e = rng.uniform(-80, 80, 20000).astype(np.float32)
+    rel = np.abs(orc.math("exp", e) - np.exp(e.astype(np.float64))) / np.exp(e.astype(np.float64))
+    assert np.max(rel) < 2e-6
+
+
+def _brute_mip(base, levels):
+    out, src = [base.reshape(-1)], base
+    for _ in range(1, levels):
+        h, w = src.shape[0] // 2, src.shape[1] // 2
+        dst = np.zeros((h, w), np.uint32)
+        for yy in range(h):
+            for xx in range(w):
+                q = [int(src[2 * yy, 2 * xx]), int(src[2 * yy, 2 * xx + 1]), int(src[2 * yy + 1, 2 * xx]), int(src[2 * yy + 1, 2 * xx + 1])]
+                a = min(t >> 24 for t in q)
+                r = sum((t >> 16) & 255 for t in q) >> 2
+                g = sum((t >> 8) & 255 for t in q) >> 2
+                b = sum(t & 255 for t in q) >> 2
+                dst[yy, xx] = (a << 24) + (r << 16) + (g << 8) + b
+        out.append(dst.reshape(-1))
+        src = dst
+    return np.concatenate(out)
+
+
+def test_mip_chain_matches_brute_force_and_keeps_constants():
+    rng = np.random.default_rng(1)
+    base = rng.integers(0, 2**32, (16, 16), dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(scenes.mip_chain(base), _brute_mip(base, 5))
+    const = np.full((32, 32), 0x80402010, np.uint32)
+    ch = scenes.mip_chain(const)
+    assert len(ch) == 32 * 32 + 16 * 16 + 8 * 8 + 4 * 4 + 2 * 2
+    assert np.all(ch == 0x80402010)
+
+
+def _oracle_with(textures):
+    s = scenes.Scene()
+    v, t = scenes.add_quad((0, 0, -1), (0.5, 0.5, 1.0), 1.0, 1.0, s.add_material(F.lambert_material((0.5, 0.5, 0.5))))
+    s.add_instance(s.add_mesh(v, t))
+    s.textures = textures
+    return orc.OracleScene(s)
+
+
+def _bilinear_ref(words, w, h, uv, level_off=0):
+    """float64 restatement: texel coordinates computed in float32 exactly as
+    FetchTexel (wrap by +1000), weights and the four taps in float64."""
+    tu, tv = uv[:, 0].astype(np.float32), uv[:, 1].astype(np.float32)
+    tcx = (np.maximum(tu + np.float32(1000), np.float32(0)) * np.float32(w)) - np.float32(0.5)
+    tcy = (np.maximum(tv + np.float32(1000), np.float32(0)) * np.float32(h)) - np.float32(0.5)
+    iu, iv = tcx.astype(np.int64) % w, tcy.astype(np.int64) % h
+    fu, fv = (tcx - np.floor(tcx)).astype(np.float64), (tcy - np.floor(tcy)).astype(np.float64)
+    def tap(x, y):
+        t = words[level_off + x + y * w]
+        return np.stack([(t >> (8 * k)) & 255 for k in range(4)], 1).astype(np.float64) / 256.0
+    iu1, iv1 = (iu + 1) % w, (iv + 1) % h
+    return (tap(iu, iv) * ((1 - fu) * (1 - fv))[:, None] + tap(iu1, iv) * (fu * (1 - fv))[:, None] +
+            tap(iu, iv1) * ((1 - fu) * fv)[:, None] + tap(iu1, iv1) * (fu * fv)[:, None])
+
+
+def test_fetch_texel_bilinear_and_trilinear():
+    rng = np.random.default_rng(2)
+    tex = scenes.checker_texture(32, 8, seed=3)
+    tex2 = scenes.normal_texture(16, seed=4)
+    o = _oracle_with([tex, tex2])
+    uv = rng.uniform(-3, 3, (5000, 2)).astype(np.float32)
+    got = o.fetch_texels(0, uv)
+    assert np.max(np.abs(got - _bilinear_ref(tex["pixels"], 32, 32, uv))) < 2e-6
+    got2 = o.fetch_texels(1, uv)                                   # NRM32 storage, second texture
+    assert np.max(np.abs(got2 - _bilinear_ref(tex2["pixels"], 16, 16, uv))) < 2e-6
+    offs = np.cumsum([0] + [(32 >> k) * (32 >> k) for k in range(5)])
+    for lam in (0.0, 1.0, 1.25, 3.5, 4.0, 9.0, -2.0):
+        l0 = 0 if lam < 0 else min(4, int(lam))
+        l1 = 0 if lam < 0 else min(4, l0 + 1)
+        f = 0.0 if lam < 0 else lam - np.floor(lam)
+        ref = (1 - f) * _bilinear_ref(tex["pixels"], 32 >> l0, 32 >> l0, uv, offs[l0]) + \
+            f * _bilinear_ref(tex["pixels"], 32 >> l1, 32 >> l1, uv, offs[l1])
+        assert np.max(np.abs(o.fetch_texels(0, uv, lam) - ref)) < 4e-6, lam
+
+
+def test_sky_lookup_matches_equirect_closed_form():
+    rng = np.random.default_rng(5)
+    px, T = scenes.sky_dome(64, 32, seed=8, rotate_deg=30.0)
+    s = scenes.Scene()
+    s.sky = (px, T)
+    o = orc.OracleScene(s)
+    d = rng.standard_normal((20000, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    got = o.sample_sky(d)
+    M = T.reshape(4, 4)[:3, :3].astype(np.float64)
+    e = -(d.astype(np.float64) @ M.T)
+    phi = np.mod(np.arctan2(e[:, 1], e[:, 0]), 2 * np.pi)
+    theta = np.arccos(np.clip(e[:, 2], -1, 1))
+    u = np.floor(64 * phi / (2 * np.pi) - 0.5).clip(0).astype(np.int64)
+    v = np.floor(32 * theta / np.pi - 0.5).clip(0).astype(np.int64)
+    idx = u + v * 64
+    ok = idx < 64 * 32
+    ref = np.zeros_like(got)
+    ref[ok] = px.reshape(-1, 3)[idx[ok]]
+    match = np.all(got == ref, axis=1)
+    assert match.mean() > 0.999           # texel boundaries may round either way
+    up = o.sample_sky(np.array([[0, 0, 1], [0, 0, -1]], np.float32))    # poles: theta = pi and 0
+    assert np.all(np.isfinite(up))
+
+
+def test_material_maps_need_textures_first():
+    s = scenes.textured_cornell_box(sphere_tris=400)
+    o = orc.OracleScene(s)                 # loads sky, textures, then materials
+    assert len(s.textures) == 5 and s.sky is not None
+    m = s.materials
+    assert (m["color"]["textureID"] >= 0).sum() == 2 and (m["normals"]["textureID"] >= 0).sum() == 2
+    del o
