@@ -135,6 +135,32 @@ LH2HIP_API int lh2hip_bvh_info(lh2hip_core* core, int meshIdx, int64_t* out8);
 LH2HIP_API int lh2hip_bvh_download(lh2hip_core* core, int meshIdx, void* nodesOut, int maxNodes,
 	uint32_t* primsOut, int maxPrims, int* nodeCount);
 
+/* ---- render job server ---------------------------------------------------
+ * No counterpart in the reference (its nearest is the asynchronous Render +
+ * RenderThread of lib/rendercore_optix7/rendercore.cpp:803-847). A queue of
+ * render jobs over the core's current scene: mixed resolutions, spp and
+ * depths, admitted while their buffers (64 B per pixel) fit budgetBytes of
+ * device memory, run by a worker thread in quanta of at most passPaths paths,
+ * least remaining work first. A job's image equals RenderCore::Render(view,
+ * Restart) with the same size, spp, depth and seed, bit for bit. Submit from
+ * any thread; the scene must not change while jobs are queued. */
+typedef struct lh2hip_server lh2hip_server;
+LH2HIP_API int lh2hip_server_create(lh2hip_core* core, uint64_t budgetBytes, uint64_t passPaths, lh2hip_server** out);
+LH2HIP_API int lh2hip_server_destroy(lh2hip_server* server);
+LH2HIP_API int lh2hip_server_submit(lh2hip_server* server, const ViewPyramid* view, int width, int height, int spp,
+	int maxDepth, uint32_t seed, uint64_t* jobId);
+/* *done = 1 once the job finished (or failed: lh2hip_server_read reports it). */
+LH2HIP_API int lh2hip_server_done(lh2hip_server* server, uint64_t jobId, int* done);
+/* Blocks until the job is done; timeoutSeconds < 0 waits without limit.
+ * Returns LH2HIP_ERR_STATE on timeout. */
+LH2HIP_API int lh2hip_server_wait(lh2hip_server* server, uint64_t jobId, double timeoutSeconds);
+/* Copies the finished job's image (w*h float4, host) and optionally its fp64
+ * accumulator (w*h*4 doubles, may be NULL), then releases the job. */
+LH2HIP_API int lh2hip_server_read(lh2hip_server* server, uint64_t jobId, lh2_float4* image, double* acc64);
+/* Seconds since the server started at which the job was queued, admitted
+ * and finished (out3). */
+LH2HIP_API int lh2hip_server_times(lh2hip_server* server, uint64_t jobId, double* out3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,13 @@ _SIGS = {
     "lh2hip_bounce_diffuse": (I, [P, P, P, P, I, ctypes.c_uint32, ctypes.c_float, P, P, P, P]),
     "lh2hip_bvh_info": (I, [P, I, P]),
     "lh2hip_bvh_download": (I, [P, I, P, I, P, I, P]),
+    "lh2hip_server_create": (I, [P, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(P)]),
+    "lh2hip_server_destroy": (I, [P]),
+    "lh2hip_server_submit": (I, [P, P, I, I, I, I, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
+    "lh2hip_server_done": (I, [P, ctypes.c_uint64, ctypes.POINTER(I)]),
+    "lh2hip_server_wait": (I, [P, ctypes.c_uint64, ctypes.c_double]),
+    "lh2hip_server_read": (I, [P, ctypes.c_uint64, P, P]),
+    "lh2hip_server_times": (I, [P, ctypes.c_uint64, P]),
 }
 EXPORTED = sorted(_SIGS) + ["CreateCore"]
 
@@ -216,6 +223,52 @@ class RenderCore:
         _check(library().lh2hip_bvh_download(self._h, mesh, _ptr(nodes), len(nodes), _ptr(prims), len(prims),
                                              ctypes.byref(cnt)))
         return nodes, prims
+
+
+class RenderServer:
+    """Render job server over a core's current scene (lh2hip_server_*): submit
+    jobs of any size/spp/depth/seed, wait, read images. A job's image equals
+    RenderCore.Render(view, LH2_RESTART) with the same settings."""
+
+    def __init__(self, core, budget_bytes=8 << 30, pass_paths=1 << 21):
+        h = P()
+        _check(library().lh2hip_server_create(core._h, budget_bytes, pass_paths, ctypes.byref(h)))
+        self._h, self._sizes = h, {}
+
+    def submit(self, view, width, height, spp=1, max_depth=3, seed=0x12345678):
+        v = np.ascontiguousarray(view, F.ViewPyramid)
+        jid = ctypes.c_uint64()
+        _check(library().lh2hip_server_submit(self._h, _ptr(v), width, height, spp, max_depth, seed, ctypes.byref(jid)))
+        self._sizes[jid.value] = (height, width)
+        return jid.value
+
+    def done(self, jid):
+        d = ctypes.c_int()
+        _check(library().lh2hip_server_done(self._h, jid, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, jid, timeout=-1.0):
+        _check(library().lh2hip_server_wait(self._h, jid, timeout))
+
+    def times(self, jid):
+        t = np.zeros(3, np.float64)
+        _check(library().lh2hip_server_times(self._h, jid, _ptr(t)))
+        return t
+
+    def read(self, jid, with_acc=False):
+        h, w = self._sizes.pop(jid)
+        img = np.zeros((h, w, 4), np.float32)
+        acc = np.zeros((h, w, 4), np.float64) if with_acc else None
+        _check(library().lh2hip_server_read(self._h, jid, _ptr(img), _ptr(acc) if with_acc else None))
+        return (img, acc) if with_acc else img
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().lh2hip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def generate_primary(view, width, height, R0, tmin, rays_ptr, path_ptr=None, stream=None):

@@ -3,6 +3,7 @@
  * the message is kept per thread for lh2hip_last_error(). */
 #include "../../include/lh2hip.h"
 #include "rendercore.h"
+#include "server.h"
 #include <cstring>
 #include <new>
 
@@ -10,6 +11,7 @@ using lh::RenderCore;
 using lh::CoreError;
 
 struct lh2hip_core { RenderCore core; };
+struct lh2hip_server { lh::RenderServer* srv; };
 
 static thread_local std::string g_lastError;
 
@@ -26,6 +28,20 @@ static int guarded(lh2hip_core* c, F f)
 	try
 	{
 		f(c->core);
+		return LH2HIP_OK;
+	}
+	catch (const CoreError& e) { return fail(e.code, e.what()); }
+	catch (const std::bad_alloc&) { return fail(LH2HIP_ERR_OOM, "host allocation failed"); }
+	catch (const std::exception& e) { return fail(LH2HIP_ERR_DEVICE, e.what()); }
+}
+
+template <typename F>
+static int guarded_server(lh2hip_server* s, F f)
+{
+	if (!s || !s->srv) return fail(LH2HIP_ERR_INVALID, "null server handle");
+	try
+	{
+		f(*s->srv);
 		return LH2HIP_OK;
 	}
 	catch (const CoreError& e) { return fail(e.code, e.what()); }
@@ -204,6 +220,67 @@ LH2HIP_API int lh2hip_bvh_download(lh2hip_core* core, int meshIdx, void* nodesOu
 	return guarded(core, [&](RenderCore& c) {
 		const int n = c.BvhDownload(meshIdx, nodesOut, maxNodes, primsOut, maxPrims);
 		if (nodeCount) *nodeCount = n;
+	});
+}
+
+
+LH2HIP_API int lh2hip_server_create(lh2hip_core* core, uint64_t budgetBytes, uint64_t passPaths, lh2hip_server** out)
+{
+	if (!out) return fail(LH2HIP_ERR_INVALID, "null output");
+	*out = nullptr;
+	return guarded(core, [&](RenderCore& c) {
+		lh2hip_server* s = new lh2hip_server;
+		try { s->srv = new lh::RenderServer(c, budgetBytes, (size_t)passPaths); }
+		catch (...) { delete s; throw; }
+		*out = s;
+	});
+}
+
+LH2HIP_API int lh2hip_server_destroy(lh2hip_server* server)
+{
+	if (!server) return LH2HIP_OK;
+	delete server->srv;
+	delete server;
+	return LH2HIP_OK;
+}
+
+LH2HIP_API int lh2hip_server_submit(lh2hip_server* server, const ViewPyramid* view, int width, int height, int spp,
+	int maxDepth, uint32_t seed, uint64_t* jobId)
+{
+	return guarded_server(server, [&](lh::RenderServer& s) {
+		if (!view || !jobId) throw CoreError(LH2HIP_ERR_INVALID, "server submit: null argument");
+		lighthouse2::ViewPyramid v;
+		memcpy(&v, view, sizeof(v));
+		*jobId = s.Submit(v, width, height, spp, maxDepth, seed);
+	});
+}
+
+LH2HIP_API int lh2hip_server_done(lh2hip_server* server, uint64_t jobId, int* done)
+{
+	return guarded_server(server, [&](lh::RenderServer& s) {
+		if (!done) throw CoreError(LH2HIP_ERR_INVALID, "null output");
+		*done = s.Done(jobId) ? 1 : 0;
+	});
+}
+
+LH2HIP_API int lh2hip_server_wait(lh2hip_server* server, uint64_t jobId, double timeoutSeconds)
+{
+	return guarded_server(server, [&](lh::RenderServer& s) {
+		if (!s.Wait(jobId, timeoutSeconds)) throw CoreError(LH2HIP_ERR_STATE, "server wait: timed out");
+	});
+}
+
+LH2HIP_API int lh2hip_server_read(lh2hip_server* server, uint64_t jobId, lh2_float4* image, double* acc64)
+{
+	return guarded_server(server, [&](lh::RenderServer& s) { s.Read(jobId, image, acc64); });
+}
+
+LH2HIP_API int lh2hip_server_times(lh2hip_server* server, uint64_t jobId, double* out3)
+{
+	return guarded_server(server, [&](lh::RenderServer& s) {
+		if (!out3) throw CoreError(LH2HIP_ERR_INVALID, "null output");
+		const lh::JobTimes t = s.Times(jobId);
+		out3[0] = t.queued, out3[1] = t.admitted, out3[2] = t.done;
 	});
 }
 

@@ -669,11 +669,8 @@ void RenderCore::SetSkyData(const lighthouse2::float3* pixels, const uint32_t wi
 	for (int r = 0; r < 3; r++) for (int c = 0; c < 4; c++) worldToSky[4 * r + c] = worldToLight.cell[4 * r + c];
 }
 
-void RenderCore::Render(const ViewPyramid& view, const Convergence converge, bool async)
+void RenderCore::checkRenderable()
 {
-	if (scrw == 0) throw CoreError(LH2HIP_ERR_STATE, "Render: SetTarget first");
-	updateTopLevel();
-	if (!sceneReady) return; /* no geometry yet: nothing to render (rendercore_optix7/rendercore.cpp:821) */
 	for (const InstanceData& I : instances)
 	{
 		const MeshData& m = meshes[I.mesh];
@@ -681,6 +678,97 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 		if (m.maxMaterial < 0) throw CoreError(LH2HIP_ERR_STATE, "Render: a mesh was set without CoreTri data (traversal-only geometry)");
 		if (m.maxMaterial >= materialCount) throw CoreError(LH2HIP_ERR_STATE, "Render: a triangle references a material beyond SetMaterials");
 	}
+}
+
+/* Shade and camera constants of one Render call (the reference's staged
+ * __constant__ globals, lib/RenderCore_PrimeRef/rendercore.cpp:535-584). */
+FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int spp, int depth, uint32_t R0gen, int samplesTakenAtStart)
+{
+	FrameSetup f;
+	memcpy(f.cam.pos, &view.pos, 12), memcpy(f.cam.p1, &view.p1, 12), memcpy(f.cam.p2, &view.p2, 12), memcpy(f.cam.p3, &view.p3, 12);
+	f.cam.aperture = view.aperture, f.cam.spreadAngle = view.spreadAngle, f.cam.w = w, f.cam.h = h;
+	ShadeParams& P = f.P;
+	memset(&P, 0, sizeof(P));
+	P.inst = shadeInsts.as<ShadeInst>();
+	P.mats = materials.as<GpuMaterial>();
+	P.triLights = triLights.as<float4>(), P.pointLights = pointLights.as<float4>();
+	P.spotLights = spotLights.as<float4>(), P.dirLights = dirLights.as<float4>();
+	P.nTri = lightCounts[0], P.nPoint = lightCounts[1], P.nSpot = lightCounts[2], P.nDir = lightCounts[3];
+	P.w = w, P.h = h, P.maxDepth = depth, P.pass = samplesTakenAtStart, P.eps = geometryEpsilon;
+	P.probePixel = -1;
+	P.argb32 = texARGB32.as<uint32_t>(), P.argb128 = texARGB128.as<float4>(), P.nrm32 = texNRM32.as<uint32_t>();
+	P.spreadAngle = view.spreadAngle;
+	P.sky = skyPixels.as<float4>(), P.skyW = skyW, P.skyH = skyH;
+	P.w2s0 = make_float4(worldToSky[0], worldToSky[1], worldToSky[2], 0.0f);
+	P.w2s1 = make_float4(worldToSky[4], worldToSky[5], worldToSky[6], 0.0f);
+	P.w2s2 = make_float4(worldToSky[8], worldToSky[9], worldToSky[10], 0.0f);
+	f.R0gen = R0gen, f.samplesTaken = samplesTakenAtStart, f.spp = spp, f.maxDepth = depth, f.pixels = w * h;
+	return f;
+}
+
+/* Samples [s0, s1) of a frame: the wavefront loop of RenderCore::Render
+ * (rendercore.cpp:564-611) over passes of at most passCap paths, each pass
+ * resolved into acc32/acc64 in sample order. The image depends on neither
+ * the pass size nor on how the samples are split between calls: a path's
+ * random numbers are keyed by its index within the frame. */
+void RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
+	uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing)
+{
+	FrameSetup f = f0;
+	const TraceScene& sc = scene;
+	const int pixels = f.pixels;
+	const int passSamples = std::max(1, std::min(s1 - s0, (int)(std::min(passCap, Wavefront::PASS_CAP) / (size_t)pixels)));
+	w.resizePaths((size_t)passSamples * pixels, st);
+	for (int a = s0; a < s1; a += passSamples)
+	{
+		const int k = std::min(passSamples, s1 - a);
+		const int n = k * pixels;
+		PathBuffers B = w.buffers();
+		f.P.jobBase = (uint32_t)a * (uint32_t)pixels;
+		hipCheck(launch_generate(f.cam, f.R0gen, f.P.jobBase, n, geometryEpsilon, B, 0, st), "generate");
+		int in = 0;
+		for (int pathLength = 1; pathLength <= f.maxDepth; pathLength++)
+		{
+			if (evs) hipCheck(hipEventRecord(evs[0], st), "event");
+			hipCheck(launch_trace_dev(sc, MODE_CLOSEST, B.rays[in], B.hits, n, B.counters + 0, queue, ovf, persistentBlocks, st), "extend");
+			if (evs) hipCheck(hipEventRecord(evs[1], st), "event");
+			f.P.pathLength = pathLength;
+			f.P.R0 = (uint32_t)f.samplesTaken * 7907u + (uint32_t)pathLength * 91771u;
+			hipCheck(launch_shade(f.P, B, in, n, st), "shade");
+			if (evs) hipCheck(hipEventRecord(evs[2], st), "event");
+			if (pathLength < f.maxDepth)
+			{
+				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, B.counters + 2, queue, ovf, persistentBlocks, st), "connect");
+				hipCheck(launch_connect(B, n, st), "connect");
+			}
+			if (evs) hipCheck(hipEventRecord(evs[3], st), "event");
+			hipCheck(launch_advance(B.counters, w.countLog() + pathLength, st), "advance");
+			in ^= 1;
+			/* stop early once every path has terminated (checked every 4 waves) */
+			if ((pathLength & 3) == 0 || pathLength == f.maxDepth)
+			{
+				uint32_t live = 0;
+				hipCheck(hipMemcpyAsync(&live, B.counters + 0, 4, hipMemcpyDeviceToHost, st), "D2H");
+				hipCheck(hipStreamSynchronize(st), "sync");
+				if (evs && timing)
+				{
+					float a1, b2, c;
+					(void)hipEventElapsedTime(&a1, evs[0], evs[1]), (void)hipEventElapsedTime(&b2, evs[1], evs[2]), (void)hipEventElapsedTime(&c, evs[2], evs[3]);
+					timing[std::min(pathLength - 1, 2)] += a1, timing[3] += b2, timing[4] += c;
+				}
+				if (live == 0) break;
+			}
+		}
+		hipCheck(launch_resolve(B.pathAcc, pixels, k, acc32, acc64, st), "resolve");
+	}
+}
+
+void RenderCore::Render(const ViewPyramid& view, const Convergence converge, bool async)
+{
+	if (scrw == 0) throw CoreError(LH2HIP_ERR_STATE, "Render: SetTarget first");
+	updateTopLevel();
+	if (!sceneReady) return; /* no geometry yet: nothing to render (rendercore_optix7/rendercore.cpp:821) */
+	checkRenderable();
 	const auto t0 = std::chrono::high_resolution_clock::now();
 	if (converge == Restart || firstConvergingFrame)
 	{
@@ -691,70 +779,13 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	}
 	if (converge == Converge) firstConvergingFrame = false;
 	const uint32_t R0gen = xor32_host(camSeed);
-	CamParams cam;
-	memcpy(cam.pos, &view.pos, 12), memcpy(cam.p1, &view.p1, 12), memcpy(cam.p2, &view.p2, 12), memcpy(cam.p3, &view.p3, 12);
-	cam.aperture = view.aperture, cam.spreadAngle = view.spreadAngle, cam.w = scrw, cam.h = scrh;
-	ShadeParams P;
-	memset(&P, 0, sizeof(P));
-	P.inst = shadeInsts.as<ShadeInst>();
-	P.mats = materials.as<GpuMaterial>();
-	P.triLights = triLights.as<float4>(), P.pointLights = pointLights.as<float4>();
-	P.spotLights = spotLights.as<float4>(), P.dirLights = dirLights.as<float4>();
-	P.nTri = lightCounts[0], P.nPoint = lightCounts[1], P.nSpot = lightCounts[2], P.nDir = lightCounts[3];
-	P.w = scrw, P.h = scrh, P.maxDepth = maxDepth, P.pass = samplesTaken, P.eps = geometryEpsilon;
-	P.probePixel = probePos.x + scrw * probePos.y;
-	P.argb32 = texARGB32.as<uint32_t>(), P.argb128 = texARGB128.as<float4>(), P.nrm32 = texNRM32.as<uint32_t>();
-	P.spreadAngle = view.spreadAngle;
-	P.sky = skyPixels.as<float4>(), P.skyW = skyW, P.skyH = skyH;
-	P.w2s0 = make_float4(worldToSky[0], worldToSky[1], worldToSky[2], 0.0f);
-	P.w2s1 = make_float4(worldToSky[4], worldToSky[5], worldToSky[6], 0.0f);
-	P.w2s2 = make_float4(worldToSky[8], worldToSky[9], worldToSky[10], 0.0f);
+	FrameSetup f = frameParams(view, scrw, scrh, scrspp, maxDepth, R0gen, samplesTaken);
+	f.P.probePixel = probePos.x + scrw * probePos.y;
+	f.P.probe = wf.probe();
+	float timing[5] = { 0, 0, 0, 0, 0 };
+	runSamples(f, 0, scrspp, Wavefront::PASS_CAP, wf, wf.acc32(), wf.acc64(), counters.as<uint32_t>() + TRACE_QUEUE_WORDS,
+		overflow.as<uint32_t>(), stream, ev, timing);
 	const int pixels = scrw * scrh;
-	const int passSamples = std::max(1, std::min(scrspp, (int)(Wavefront::PASS_CAP / pixels)));
-	wf.resizePaths((size_t)passSamples * pixels, stream);
-	P.probe = wf.probe();
-	const TraceScene& sc = Scene();
-	float traceMs[3] = { 0, 0, 0 }, shadowMs = 0, shadeMs = 0;
-	uint32_t rayCounts[3] = { 0, 0, 0 };
-	for (int s0 = 0; s0 < scrspp; s0 += passSamples)
-	{
-		const int k = std::min(passSamples, scrspp - s0);
-		const int n = k * pixels;
-		PathBuffers B = wf.buffers();
-		P.jobBase = (uint32_t)s0 * (uint32_t)pixels;
-		hipCheck(launch_generate(cam, R0gen, P.jobBase, n, geometryEpsilon, B, 0, stream), "generate");
-		int in = 0;
-		for (int pathLength = 1; pathLength <= maxDepth; pathLength++)
-		{
-			hipCheck(hipEventRecord(ev[0], stream), "event");
-			hipCheck(launch_trace_dev(sc, MODE_CLOSEST, B.rays[in], B.hits, n, B.counters + 0, counters.as<uint32_t>() + TRACE_QUEUE_WORDS, overflow.as<uint32_t>(), persistentBlocks, stream), "extend");
-			hipCheck(hipEventRecord(ev[1], stream), "event");
-			P.pathLength = pathLength;
-			P.R0 = (uint32_t)samplesTaken * 7907u + (uint32_t)pathLength * 91771u;
-			hipCheck(launch_shade(P, B, in, n, stream), "shade");
-			hipCheck(hipEventRecord(ev[2], stream), "event");
-			if (pathLength < maxDepth)
-			{
-				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, B.counters + 2, counters.as<uint32_t>() + TRACE_QUEUE_WORDS, overflow.as<uint32_t>(), persistentBlocks, stream), "connect");
-				hipCheck(launch_connect(B, n, stream), "connect");
-			}
-			hipCheck(hipEventRecord(ev[3], stream), "event");
-			hipCheck(launch_advance(B.counters, wf.countLog() + pathLength, stream), "advance");
-			in ^= 1;
-			/* stop early once every path has terminated (checked every 4 waves) */
-			if ((pathLength & 3) == 0 || pathLength == maxDepth)
-			{
-				uint32_t live = 0;
-				hipCheck(hipMemcpyAsync(&live, B.counters + 0, 4, hipMemcpyDeviceToHost, stream), "D2H");
-				hipCheck(hipStreamSynchronize(stream), "sync");
-				float a, b2, c;
-				(void)hipEventElapsedTime(&a, ev[0], ev[1]), (void)hipEventElapsedTime(&b2, ev[1], ev[2]), (void)hipEventElapsedTime(&c, ev[2], ev[3]);
-				traceMs[std::min(pathLength - 1, 2)] += a, shadeMs += b2, shadowMs += c;
-				if (live == 0) break;
-			}
-		}
-		hipCheck(launch_resolve(B.pathAcc, pixels, k, wf.acc32(), wf.acc64(), stream), "resolve");
-	}
 	samplesTaken += scrspp;
 	hipCheck(launch_finalize(wf.acc32(), pixels, 1.0f / (float)samplesTaken, wf.image(), stream), "finalize");
 	if (!async)
@@ -767,10 +798,9 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	}
 	const auto t1 = std::chrono::high_resolution_clock::now();
 	stats.renderTime = std::chrono::duration<float>(t1 - t0).count();
-	stats.traceTime0 = traceMs[0] * 1e-3f, stats.traceTime1 = traceMs[1] * 1e-3f, stats.traceTimeX = traceMs[2] * 1e-3f;
-	stats.shadeTime = shadeMs * 1e-3f, stats.shadowTraceTime = shadowMs * 1e-3f;
+	stats.traceTime0 = timing[0] * 1e-3f, stats.traceTime1 = timing[1] * 1e-3f, stats.traceTimeX = timing[2] * 1e-3f;
+	stats.shadeTime = timing[3] * 1e-3f, stats.shadowTraceTime = timing[4] * 1e-3f;
 	stats.primaryRayCount = (uint32_t)(pixels * scrspp);
-	(void)rayCounts;
 }
 
 void RenderCore::WaitForRender()

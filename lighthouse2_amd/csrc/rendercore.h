@@ -76,6 +76,15 @@ struct MeshData
 
 struct InstanceData { int mesh; float T[16]; };
 
+/* Per-frame constants of the wavefront kernels. */
+struct FrameSetup
+{
+	CamParams cam;
+	ShadeParams P;
+	uint32_t R0gen = 0;
+	int samplesTaken = 0, spp = 1, maxDepth = 1, pixels = 0;
+};
+
 class RenderCore : public lighthouse2::CoreAPI_Base
 {
 public:
@@ -113,6 +122,14 @@ public:
 	void BvhInfo(int mesh, int64_t* out8) const;
 	int BvhDownload(int mesh, void* nodesOut, int maxNodes, uint32_t* primsOut, int maxPrims) const;
 	const TraceScene& Scene();
+	/* Render building blocks, shared with the job server (server.cpp). */
+	void checkRenderable();
+	FrameSetup frameParams(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int depth, uint32_t R0gen, int samplesTakenAtStart);
+	void runSamples(const FrameSetup& f, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
+		uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing);
+	bool sceneIsReady() const { return sceneReady; }
+	int traceBlocks() const { return persistentBlocks; }
+	float epsilon() const { return geometryEpsilon; }
 
 	hipStream_t stream = nullptr;
 	int deviceCUs = 0;
