@@ -135,6 +135,34 @@ LH2HIP_API int lh2hip_bvh_info(lh2hip_core* core, int meshIdx, int64_t* out8);
 LH2HIP_API int lh2hip_bvh_download(lh2hip_core* core, int meshIdx, void* nodesOut, int maxNodes,
 	uint32_t* primsOut, int maxPrims, int* nodeCount);
 
+/* ---- learned denoiser kernels --------------------------------------------
+ * No counterpart in the reference. A 5-level 3x3 convolutional U-Net is
+ * composed from these calls (lighthouse2_amd/denoiser.py); tensors are NHWC
+ * device arrays, activations and weights bf16 (uint16 bits), accumulation
+ * fp32 on the matrix cores. Channel counts are multiples of 16. */
+enum { LH2HIP_DN_RELU = 1, LH2HIP_DN_OUT_F32 = 2 };
+/* y = act(conv3x3(x, w) + bias): x [H][W][Cin] bf16, w [9][Cout][Cin] bf16
+ * (tap = 3 dy + dx), bias [Cout] fp32 or NULL, y bf16 (fp32 with
+ * LH2HIP_DN_OUT_F32); Cout <= 64. With the flipped weights of lh2hip_dn_adam
+ * this is also the input gradient of a convolution. */
+LH2HIP_API int lh2hip_dn_conv3x3(const void* x, int Cin, const void* w, const float* bias, void* y, int Cout, int H, int W, int flags, void* stream);
+LH2HIP_API uint64_t lh2hip_dn_dw_workspace_bytes(int Cin, int Cout);
+/* dw [9][Cout][Cin] and db [Cout] (fp32, overwritten) of a 3x3 convolution
+ * from its input x and output gradient dy; ws: workspace bytes above. */
+LH2HIP_API int lh2hip_dn_conv3x3_dw(const void* x, int Cin, const void* dy, int Cout, int H, int W, float* dw, float* db, void* ws, void* stream);
+LH2HIP_API int lh2hip_dn_relu_bwd(const void* y, const void* dy, void* dyPre, uint64_t n, void* stream);
+LH2HIP_API int lh2hip_dn_pool2(const void* x, int C, int H, int W, void* y, void* stream);
+LH2HIP_API int lh2hip_dn_pool2_bwd(const void* dy, int C, int H, int W, void* dx, int accumulate, void* stream);
+LH2HIP_API int lh2hip_dn_upcat(const void* coarse, int C0, const void* skip, int C1, int H, int W, void* out, void* stream);
+LH2HIP_API int lh2hip_dn_upcat_bwd(const void* dcat, int C0, int C1, int H, int W, void* dcoarse, void* dskip, void* stream);
+LH2HIP_API int lh2hip_dn_pack_input(const lh2_float4* radiance, const lh2_float4* albedo, const lh2_float4* normal, const float* depth,
+	int H, int W, void* out, void* stream);
+LH2HIP_API int lh2hip_dn_relmse(const float* out, int Cout, const lh2_float4* target, int H, int W, void* dout, float* loss, void* stream);
+/* Adam step (g NULL: no step, only the bf16 copies); wfwd / wbwd (NULL to
+ * skip): bf16 weights in the forward layout and flipped for the input gradient. */
+LH2HIP_API int lh2hip_dn_adam(float* p, float* m, float* v, const float* g, uint64_t n, float lr, float b1, float b2, float eps, int t,
+	void* wfwd, void* wbwd, int Cout, int Cin, void* stream);
+
 /* ---- render job server ---------------------------------------------------
  * No counterpart in the reference (its nearest is the asynchronous Render +
  * RenderThread of lib/rendercore_optix7/rendercore.cpp:803-847). A queue of

@@ -56,6 +56,18 @@ _SIGS = {
     "lh2hip_bounce_diffuse": (I, [P, P, P, P, I, ctypes.c_uint32, ctypes.c_float, P, P, P, P]),
     "lh2hip_bvh_info": (I, [P, I, P]),
     "lh2hip_bvh_download": (I, [P, I, P, I, P, I, P]),
+    "lh2hip_dn_conv3x3": (I, [P, I, P, P, P, I, I, I, I, P]),
+    "lh2hip_dn_dw_workspace_bytes": (ctypes.c_uint64, [I, I]),
+    "lh2hip_dn_conv3x3_dw": (I, [P, I, P, I, I, I, P, P, P, P]),
+    "lh2hip_dn_relu_bwd": (I, [P, P, P, ctypes.c_uint64, P]),
+    "lh2hip_dn_pool2": (I, [P, I, I, I, P, P]),
+    "lh2hip_dn_pool2_bwd": (I, [P, I, I, I, P, I, P]),
+    "lh2hip_dn_upcat": (I, [P, I, P, I, I, I, P, P]),
+    "lh2hip_dn_upcat_bwd": (I, [P, I, I, I, I, P, P, P]),
+    "lh2hip_dn_pack_input": (I, [P, P, P, P, I, I, P, P]),
+    "lh2hip_dn_relmse": (I, [P, I, P, I, I, P, P, P]),
+    "lh2hip_dn_adam": (I, [P, P, P, P, ctypes.c_uint64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, I,
+                           P, P, I, I, P]),
     "lh2hip_server_create": (I, [P, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(P)]),
     "lh2hip_server_destroy": (I, [P]),
     "lh2hip_server_submit": (I, [P, P, I, I, I, I, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),

@@ -4,6 +4,7 @@
 #include "../../include/lh2hip.h"
 #include "rendercore.h"
 #include "server.h"
+#include "denoise.h"
 #include <cstring>
 #include <new>
 
@@ -282,6 +283,76 @@ LH2HIP_API int lh2hip_server_times(lh2hip_server* server, uint64_t jobId, double
 		const lh::JobTimes t = s.Times(jobId);
 		out3[0] = t.queued, out3[1] = t.admitted, out3[2] = t.done;
 	});
+}
+
+
+static int dn_status(hipError_t e, const char* what)
+{
+	if (e == hipSuccess) return LH2HIP_OK;
+	if (e == hipErrorInvalidValue) return fail(LH2HIP_ERR_INVALID, what);
+	g_lastError = std::string(what) + ": " + hipGetErrorString(e);
+	return LH2HIP_ERR_DEVICE;
+}
+
+LH2HIP_API int lh2hip_dn_conv3x3(const void* x, int Cin, const void* w, const float* bias, void* y, int Cout, int H, int W, int flags, void* stream)
+{
+	if (!x || !w || !y) return fail(LH2HIP_ERR_INVALID, "dn_conv3x3: null pointer");
+	return dn_status(lh::dn_conv3x3((const lh::bf16_t*)x, Cin, (const lh::bf16_t*)w, bias, y, Cout, H, W, flags, (hipStream_t)stream),
+		"dn_conv3x3 (Cin, Cout multiples of 16, Cout <= 64)");
+}
+
+LH2HIP_API uint64_t lh2hip_dn_dw_workspace_bytes(int Cin, int Cout) { return lh::dn_dw_workspace_bytes(Cin, Cout); }
+
+LH2HIP_API int lh2hip_dn_conv3x3_dw(const void* x, int Cin, const void* dy, int Cout, int H, int W, float* dw, float* db, void* ws, void* stream)
+{
+	if (!x || !dy || !dw || !ws) return fail(LH2HIP_ERR_INVALID, "dn_conv3x3_dw: null pointer");
+	return dn_status(lh::dn_conv3x3_dw((const lh::bf16_t*)x, Cin, (const lh::bf16_t*)dy, Cout, H, W, dw, db, ws, (hipStream_t)stream),
+		"dn_conv3x3_dw (Cin x Cout in 16x32, 32x32, 64x32, 32x16, 16x16)");
+}
+
+LH2HIP_API int lh2hip_dn_relu_bwd(const void* y, const void* dy, void* dyPre, uint64_t n, void* stream)
+{
+	return dn_status(lh::dn_relu_bwd((const lh::bf16_t*)y, (const lh::bf16_t*)dy, (lh::bf16_t*)dyPre, (size_t)n, (hipStream_t)stream), "dn_relu_bwd");
+}
+
+LH2HIP_API int lh2hip_dn_pool2(const void* x, int C, int H, int W, void* y, void* stream)
+{
+	return dn_status(lh::dn_pool2((const lh::bf16_t*)x, C, H, W, (lh::bf16_t*)y, (hipStream_t)stream), "dn_pool2 (even H, W)");
+}
+
+LH2HIP_API int lh2hip_dn_pool2_bwd(const void* dy, int C, int H, int W, void* dx, int accumulate, void* stream)
+{
+	return dn_status(lh::dn_pool2_bwd((const lh::bf16_t*)dy, C, H, W, (lh::bf16_t*)dx, accumulate, (hipStream_t)stream), "dn_pool2_bwd (even H, W)");
+}
+
+LH2HIP_API int lh2hip_dn_upcat(const void* coarse, int C0, const void* skip, int C1, int H, int W, void* out, void* stream)
+{
+	return dn_status(lh::dn_upcat((const lh::bf16_t*)coarse, C0, (const lh::bf16_t*)skip, C1, H, W, (lh::bf16_t*)out, (hipStream_t)stream), "dn_upcat (even H, W)");
+}
+
+LH2HIP_API int lh2hip_dn_upcat_bwd(const void* dcat, int C0, int C1, int H, int W, void* dcoarse, void* dskip, void* stream)
+{
+	return dn_status(lh::dn_upcat_bwd((const lh::bf16_t*)dcat, C0, C1, H, W, (lh::bf16_t*)dcoarse, (lh::bf16_t*)dskip, (hipStream_t)stream),
+		"dn_upcat_bwd (even H, W)");
+}
+
+LH2HIP_API int lh2hip_dn_pack_input(const lh2_float4* radiance, const lh2_float4* albedo, const lh2_float4* normal, const float* depth,
+	int H, int W, void* out, void* stream)
+{
+	return dn_status(lh::dn_pack_input((const float4*)radiance, (const float4*)albedo, (const float4*)normal, depth, H, W, (lh::bf16_t*)out,
+		(hipStream_t)stream), "dn_pack_input");
+}
+
+LH2HIP_API int lh2hip_dn_relmse(const float* out, int Cout, const lh2_float4* target, int H, int W, void* dout, float* loss, void* stream)
+{
+	return dn_status(lh::dn_relmse(out, Cout, (const float4*)target, H, W, (lh::bf16_t*)dout, loss, (hipStream_t)stream), "dn_relmse");
+}
+
+LH2HIP_API int lh2hip_dn_adam(float* p, float* m, float* v, const float* g, uint64_t n, float lr, float b1, float b2, float eps, int t,
+	void* wfwd, void* wbwd, int Cout, int Cin, void* stream)
+{
+	return dn_status(lh::dn_adam(p, m, v, g, (size_t)n, lr, b1, b2, eps, t, (lh::bf16_t*)wfwd, (lh::bf16_t*)wbwd, Cout, Cin, (hipStream_t)stream),
+		"dn_adam (a flipped copy needs n = 9 Cout Cin)");
 }
 
 } /* extern "C" */

@@ -1,0 +1,445 @@
+/* denoise.hip - learned-denoiser kernels for gfx950 (see denoise.h).
+ *
+ * Convolutions are implicit GEMMs on the matrix cores,
+ * v_mfma_f32_16x16x16_bf16 (bf16 operands, fp32 accumulation). Operand
+ * layout of one 16x16x16 product in a wave of 64 lanes: A[m][k] with
+ * m = lane % 16, k = 4 (lane / 16) + i; B[k][n] with n = lane % 16,
+ * k = 4 (lane / 16) + i; D[m][n] with n = lane % 16, m = 4 (lane / 16) + i
+ * (i = 0..3 the element of the lane's 4-vector).
+ *
+ * Forward / input gradient: M = 64 output pixels of one row per wave, N =
+ * output channels, K = 9 taps x input channels; activations are NHWC, so the
+ * 4 k-values a lane needs are 4 consecutive channels of one pixel (one 8-byte
+ * load) and the weights [tap][n][k] give the same for B.
+ * Weight gradient: M = output channels, N = input channels, K = pixels;
+ * a 64-pixel tile of dy and the 3 x 66 pixel input halo are staged in LDS
+ * channel-major so that both operands read consecutive pixels; each block
+ * keeps partial sums for its tiles and a second kernel adds the partials in
+ * block order (deterministic).
+ */
+#include "denoise.h"
+
+namespace lh {
+
+typedef short s4v __attribute__((__vector_size__(4 * sizeof(short))));
+typedef float f4v __attribute__((__vector_size__(4 * sizeof(float))));
+
+__device__ __forceinline__ float bf2f(bf16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
+/* round to nearest even; NaN stays a quiet NaN */
+__device__ __forceinline__ bf16_t f2bf(float f)
+{
+	const uint32_t u = __builtin_bit_cast(uint32_t, f);
+	if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)((u >> 16) | 0x40u);
+	return (bf16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+static const int DW_BLOCKS = 256;
+
+/* ---- 3x3 convolution, forward and input gradient ------------------------ */
+template <int NB, bool RELU, bool F32OUT>
+__global__ void __launch_bounds__(256) k_conv3x3(const bf16_t* __restrict__ x, int Cin, const bf16_t* __restrict__ w,
+	const float* __restrict__ bias, void* __restrict__ y, int H, int W)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int row = blockIdx.y * 4 + wave;
+	const int x0 = blockIdx.x * 64;
+	if (row >= H) return;
+	const int Cout = NB * 16;
+	const int ml = lane & 15, kq = (lane >> 4) * 4;
+	f4v acc[4][NB];
+#pragma unroll
+	for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+		for (int nb = 0; nb < NB; nb++) acc[mb][nb] = (f4v){ 0.0f, 0.0f, 0.0f, 0.0f };
+	for (int tap = 0; tap < 9; tap++)
+	{
+		const int yy = row + tap / 3 - 1, dx = tap % 3 - 1;
+		const bool rowOk = yy >= 0 && yy < H;
+		const bf16_t* xrow = x + (size_t)(rowOk ? yy : 0) * W * Cin;
+		for (int c = 0; c < Cin; c += 16)
+		{
+			s4v b[NB];
+#pragma unroll
+			for (int nb = 0; nb < NB; nb++) b[nb] = *(const s4v*)(w + ((size_t)(tap * Cout + nb * 16 + ml) * Cin + c + kq));
+#pragma unroll
+			for (int mb = 0; mb < 4; mb++)
+			{
+				const int xx = x0 + mb * 16 + ml + dx;
+				s4v a = (s4v){ 0, 0, 0, 0 };
+				if (rowOk && xx >= 0 && xx < W) a = *(const s4v*)(xrow + (size_t)xx * Cin + c + kq);
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++) acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nb], acc[mb][nb], 0, 0, 0);
+			}
+		}
+	}
+#pragma unroll
+	for (int nb = 0; nb < NB; nb++)
+	{
+		const int ch = nb * 16 + ml;
+		const float bv = bias ? bias[ch] : 0.0f;
+#pragma unroll
+		for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+			for (int i = 0; i < 4; i++)
+			{
+				const int px = x0 + mb * 16 + kq + i;
+				if (px >= W) continue;
+				float v = acc[mb][nb][i] + bv;
+				if (RELU) v = v > 0.0f ? v : 0.0f;
+				const size_t o = ((size_t)row * W + px) * Cout + ch;
+				if (F32OUT) ((float*)y)[o] = v;
+				else ((bf16_t*)y)[o] = f2bf(v);
+			}
+	}
+}
+
+template <int NB>
+static hipError_t conv_nb(const bf16_t* x, int Cin, const bf16_t* w, const float* bias, void* y, int H, int W, int flags, hipStream_t s)
+{
+	const dim3 grid((W + 63) / 64, (H + 3) / 4), block(256);
+	const bool relu = flags & DN_RELU, f32 = flags & DN_OUT_F32;
+	if (relu && f32) k_conv3x3<NB, true, true><<<grid, block, 0, s>>>(x, Cin, w, bias, y, H, W);
+	else if (relu) k_conv3x3<NB, true, false><<<grid, block, 0, s>>>(x, Cin, w, bias, y, H, W);
+	else if (f32) k_conv3x3<NB, false, true><<<grid, block, 0, s>>>(x, Cin, w, bias, y, H, W);
+	else k_conv3x3<NB, false, false><<<grid, block, 0, s>>>(x, Cin, w, bias, y, H, W);
+	return hipGetLastError();
+}
+
+hipError_t dn_conv3x3(const bf16_t* x, int Cin, const bf16_t* w, const float* bias, void* y, int Cout, int H, int W, int flags, hipStream_t s)
+{
+	if (Cin <= 0 || Cin % 16 || Cout <= 0 || Cout % 16 || Cout > 64 || H <= 0 || W <= 0) return hipErrorInvalidValue;
+	switch (Cout / 16)
+	{
+	case 1: return conv_nb<1>(x, Cin, w, bias, y, H, W, flags, s);
+	case 2: return conv_nb<2>(x, Cin, w, bias, y, H, W, flags, s);
+	case 3: return conv_nb<3>(x, Cin, w, bias, y, H, W, flags, s);
+	default: return conv_nb<4>(x, Cin, w, bias, y, H, W, flags, s);
+	}
+}
+
+/* ---- weight gradient --------------------------------------------------- */
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_conv3x3_dw(const bf16_t* __restrict__ x, const bf16_t* __restrict__ dy, int H, int W,
+	float* __restrict__ partial)
+{
+	constexpr int MB = COUT / 16, NBK = CIN / 16, XP = 68;
+	__shared__ __attribute__((aligned(16))) bf16_t dys[COUT][64];
+	__shared__ __attribute__((aligned(16))) bf16_t xs[3][CIN][XP];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int ml = lane & 15, kq = (lane >> 4) * 4;
+	/* taps of this wave: wave, wave + 4, wave + 8 */
+	f4v acc[3][MB][NBK];
+#pragma unroll
+	for (int t = 0; t < 3; t++)
+#pragma unroll
+		for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+			for (int nb = 0; nb < NBK; nb++) acc[t][mb][nb] = (f4v){ 0.0f, 0.0f, 0.0f, 0.0f };
+	const int tilesX = (W + 63) / 64, tiles = tilesX * H;
+	for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+	{
+		const int ty = tile / tilesX, x0 = (tile % tilesX) * 64;
+		__syncthreads();
+		for (int i = threadIdx.x; i < 64 * COUT; i += 256)
+		{
+			const int px = i / COUT, c = i % COUT;
+			dys[c][px] = (x0 + px < W) ? dy[((size_t)ty * W + x0 + px) * COUT + c] : (bf16_t)0;
+		}
+		for (int i = threadIdx.x; i < 3 * 66 * CIN; i += 256)
+		{
+			const int r = i / (66 * CIN), rem = i % (66 * CIN), px = rem / CIN, c = rem % CIN;
+			const int yy = ty + r - 1, xx = x0 + px - 1;
+			xs[r][c][px] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? x[((size_t)yy * W + xx) * CIN + c] : (bf16_t)0;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int t = 0; t < 3; t++)
+		{
+			const int tap = wave + 4 * t;
+			if (tap >= 9) break;
+			const int r = tap / 3, dxo = tap % 3;
+#pragma unroll
+			for (int kk = 0; kk < 4; kk++)
+			{
+				s4v a[MB];
+#pragma unroll
+				for (int mb = 0; mb < MB; mb++) a[mb] = *(const s4v*)&dys[mb * 16 + ml][kk * 16 + kq];
+#pragma unroll
+				for (int nb = 0; nb < NBK; nb++)
+				{
+					const bf16_t* src = &xs[r][nb * 16 + ml][kk * 16 + kq + dxo];
+					const s4v b = (s4v){ (short)src[0], (short)src[1], (short)src[2], (short)src[3] };
+#pragma unroll
+					for (int mb = 0; mb < MB; mb++) acc[t][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mb], b, acc[t][mb][nb], 0, 0, 0);
+				}
+			}
+		}
+	}
+	float* out = partial + (size_t)blockIdx.x * 9 * COUT * CIN;
+#pragma unroll
+	for (int t = 0; t < 3; t++)
+	{
+		const int tap = wave + 4 * t;
+		if (tap >= 9) break;
+#pragma unroll
+		for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+			for (int nb = 0; nb < NBK; nb++)
+#pragma unroll
+				for (int i = 0; i < 4; i++)
+					out[((size_t)tap * COUT + mb * 16 + kq + i) * CIN + nb * 16 + ml] = acc[t][mb][nb][i];
+	}
+}
+
+/* per-block partial sums of dy over channels (bias gradient) */
+__global__ void k_bias_partial(const bf16_t* __restrict__ dy, int C, size_t pixels, float* __restrict__ partial)
+{
+	for (int c = threadIdx.x; c < C; c += blockDim.x)
+	{
+		float s = 0.0f;
+		for (size_t p = blockIdx.x; p < pixels; p += gridDim.x) s += bf2f(dy[p * C + c]);
+		partial[(size_t)blockIdx.x * C + c] = s;
+	}
+}
+
+__global__ void k_reduce_partials(const float* __restrict__ partial, int blocks, int n, float* __restrict__ out)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	float s = 0.0f;
+	for (int b = 0; b < blocks; b++) s += partial[(size_t)b * n + i];
+	out[i] = s;
+}
+
+size_t dn_dw_workspace_bytes(int Cin, int Cout) { return (size_t)DW_BLOCKS * (9 * Cout * Cin + Cout) * sizeof(float); }
+
+template <int CIN, int COUT>
+static void launch_dw(const bf16_t* x, const bf16_t* dy, int H, int W, float* partial, int blocks, hipStream_t s)
+{
+	k_conv3x3_dw<CIN, COUT><<<blocks, 256, 0, s>>>(x, dy, H, W, partial);
+}
+
+hipError_t dn_conv3x3_dw(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, int H, int W, float* dw, float* db, void* ws, hipStream_t s)
+{
+	if (H <= 0 || W <= 0 || !ws) return hipErrorInvalidValue;
+	const int tiles = ((W + 63) / 64) * H;
+	const int blocks = tiles < DW_BLOCKS ? tiles : DW_BLOCKS;
+	float* partial = (float*)ws;
+	if (Cin == 16 && Cout == 32) launch_dw<16, 32>(x, dy, H, W, partial, blocks, s);
+	else if (Cin == 32 && Cout == 32) launch_dw<32, 32>(x, dy, H, W, partial, blocks, s);
+	else if (Cin == 64 && Cout == 32) launch_dw<64, 32>(x, dy, H, W, partial, blocks, s);
+	else if (Cin == 32 && Cout == 16) launch_dw<32, 16>(x, dy, H, W, partial, blocks, s);
+	else if (Cin == 16 && Cout == 16) launch_dw<16, 16>(x, dy, H, W, partial, blocks, s);
+	else return hipErrorInvalidValue;
+	const int n = 9 * Cout * Cin;
+	k_reduce_partials<<<(n + 255) / 256, 256, 0, s>>>(partial, blocks, n, dw);
+	if (db)
+	{
+		float* bp = partial + (size_t)DW_BLOCKS * n;
+		k_bias_partial<<<DW_BLOCKS, 64, 0, s>>>(dy, Cout, (size_t)H * W, bp);
+		k_reduce_partials<<<1, 64, 0, s>>>(bp, DW_BLOCKS, Cout, db);
+	}
+	return hipGetLastError();
+}
+
+/* ---- elementwise and layout kernels ------------------------------------ */
+__global__ void k_relu_bwd(const bf16_t* __restrict__ y, const bf16_t* __restrict__ dy, bf16_t* __restrict__ out, size_t n)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		out[i] = bf2f(y[i]) > 0.0f ? dy[i] : (bf16_t)0;
+}
+
+hipError_t dn_relu_bwd(const bf16_t* y, const bf16_t* dy, bf16_t* dyPre, size_t n, hipStream_t s)
+{
+	const int blocks = (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+	if (n) k_relu_bwd<<<blocks, 256, 0, s>>>(y, dy, dyPre, n);
+	return hipGetLastError();
+}
+
+__global__ void k_pool2(const bf16_t* __restrict__ x, int C, int H, int W, bf16_t* __restrict__ y)
+{
+	const int h2 = H / 2, w2 = W / 2;
+	const size_t n = (size_t)h2 * w2 * C;
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+	{
+		const int c = (int)(i % C);
+		const size_t p = i / C;
+		const int px = (int)(p % w2), py = (int)(p / w2);
+		const size_t b = ((size_t)(2 * py) * W + 2 * px) * C + c;
+		const float v = ((bf2f(x[b]) + bf2f(x[b + C])) + (bf2f(x[b + (size_t)W * C]) + bf2f(x[b + (size_t)W * C + C]))) * 0.25f;
+		y[i] = f2bf(v);
+	}
+}
+
+__global__ void k_pool2_bwd(const bf16_t* __restrict__ dy, int C, int H, int W, bf16_t* __restrict__ dx, int acc)
+{
+	const size_t n = (size_t)H * W * C;
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+	{
+		const int c = (int)(i % C);
+		const size_t p = i / C;
+		const int px = (int)(p % W), py = (int)(p / W);
+		float v = bf2f(dy[((size_t)(py / 2) * (W / 2) + px / 2) * C + c]) * 0.25f;
+		if (acc) v += bf2f(dx[i]);
+		dx[i] = f2bf(v);
+	}
+}
+
+__global__ void k_upcat(const bf16_t* __restrict__ coarse, int C0, const bf16_t* __restrict__ skip, int C1, int H, int W, bf16_t* __restrict__ out)
+{
+	const int C = C0 + C1;
+	const size_t n = (size_t)H * W * C;
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+	{
+		const int c = (int)(i % C);
+		const size_t p = i / C;
+		const int px = (int)(p % W), py = (int)(p / W);
+		out[i] = c < C0 ? coarse[((size_t)(py / 2) * (W / 2) + px / 2) * C0 + c] : skip[p * C1 + (c - C0)];
+	}
+}
+
+__global__ void k_upcat_bwd_coarse(const bf16_t* __restrict__ dcat, int C0, int C1, int H, int W, bf16_t* __restrict__ dcoarse)
+{
+	const int C = C0 + C1, h2 = H / 2, w2 = W / 2;
+	const size_t n = (size_t)h2 * w2 * C0;
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+	{
+		const int c = (int)(i % C0);
+		const size_t p = i / C0;
+		const int px = (int)(p % w2), py = (int)(p / w2);
+		const size_t b = ((size_t)(2 * py) * W + 2 * px) * C + c;
+		dcoarse[i] = f2bf((bf2f(dcat[b]) + bf2f(dcat[b + C])) + (bf2f(dcat[b + (size_t)W * C]) + bf2f(dcat[b + (size_t)W * C + C])));
+	}
+}
+
+__global__ void k_upcat_bwd_skip(const bf16_t* __restrict__ dcat, int C0, int C1, size_t pixels, bf16_t* __restrict__ dskip)
+{
+	const int C = C0 + C1;
+	const size_t n = pixels * C1;
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		dskip[i] = dcat[(i / C1) * C + C0 + (i % C1)];
+}
+
+static int grid_for(size_t n) { const size_t b = (n + 255) / 256; return (int)(b < 65536 ? (b ? b : 1) : 65536); }
+
+hipError_t dn_pool2(const bf16_t* x, int C, int H, int W, bf16_t* y, hipStream_t s)
+{
+	if (H % 2 || W % 2) return hipErrorInvalidValue;
+	k_pool2<<<grid_for((size_t)H * W * C / 4), 256, 0, s>>>(x, C, H, W, y);
+	return hipGetLastError();
+}
+
+hipError_t dn_pool2_bwd(const bf16_t* dy, int C, int H, int W, bf16_t* dx, int accumulate, hipStream_t s)
+{
+	if (H % 2 || W % 2) return hipErrorInvalidValue;
+	k_pool2_bwd<<<grid_for((size_t)H * W * C), 256, 0, s>>>(dy, C, H, W, dx, accumulate);
+	return hipGetLastError();
+}
+
+hipError_t dn_upcat(const bf16_t* coarse, int C0, const bf16_t* skip, int C1, int H, int W, bf16_t* out, hipStream_t s)
+{
+	if (H % 2 || W % 2) return hipErrorInvalidValue;
+	k_upcat<<<grid_for((size_t)H * W * (C0 + C1)), 256, 0, s>>>(coarse, C0, skip, C1, H, W, out);
+	return hipGetLastError();
+}
+
+hipError_t dn_upcat_bwd(const bf16_t* dcat, int C0, int C1, int H, int W, bf16_t* dcoarse, bf16_t* dskip, hipStream_t s)
+{
+	if (H % 2 || W % 2) return hipErrorInvalidValue;
+	k_upcat_bwd_coarse<<<grid_for((size_t)H * W * C0 / 4), 256, 0, s>>>(dcat, C0, C1, H, W, dcoarse);
+	k_upcat_bwd_skip<<<grid_for((size_t)H * W * C1), 256, 0, s>>>(dcat, C0, C1, (size_t)H * W, dskip);
+	return hipGetLastError();
+}
+
+__global__ void k_pack_input(const float4* __restrict__ rad, const float4* __restrict__ alb, const float4* __restrict__ nrm,
+	const float* __restrict__ depth, size_t pixels, bf16_t* __restrict__ out)
+{
+	for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < pixels; p += (size_t)gridDim.x * blockDim.x)
+	{
+		const float4 r = rad[p], a = alb[p], n = nrm[p];
+		bf16_t* o = out + p * 16;
+		o[0] = f2bf(r.x), o[1] = f2bf(r.y), o[2] = f2bf(r.z);
+		o[3] = f2bf(a.x), o[4] = f2bf(a.y), o[5] = f2bf(a.z);
+		o[6] = f2bf(n.x), o[7] = f2bf(n.y), o[8] = f2bf(n.z);
+		o[9] = f2bf(depth[p]);
+		for (int c = 10; c < 16; c++) o[c] = 0;
+	}
+}
+
+hipError_t dn_pack_input(const float4* radiance, const float4* albedo, const float4* normal, const float* depth, int H, int W, bf16_t* out, hipStream_t s)
+{
+	k_pack_input<<<grid_for((size_t)H * W), 256, 0, s>>>(radiance, albedo, normal, depth, (size_t)H * W, out);
+	return hipGetLastError();
+}
+
+__global__ void k_relmse(const float* __restrict__ out, int Cout, const float4* __restrict__ target, size_t pixels, float invN,
+	bf16_t* __restrict__ dout, float* __restrict__ loss)
+{
+	__shared__ float red[256];
+	float l = 0.0f;
+	for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < pixels; p += (size_t)gridDim.x * blockDim.x)
+	{
+		const float4 t4 = target[p];
+		const float t[3] = { t4.x, t4.y, t4.z };
+		for (int c = 0; c < Cout; c++)
+		{
+			if (c < 3)
+			{
+				const float d = out[p * Cout + c] - t[c], den = t[c] * t[c] + 0.01f;
+				l += d * d / den;
+				dout[p * Cout + c] = f2bf(2.0f * d / den * invN);
+			}
+			else dout[p * Cout + c] = 0;
+		}
+	}
+	red[threadIdx.x] = l;
+	__syncthreads();
+	for (int k = 128; k > 0; k >>= 1)
+	{
+		if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) atomicAdd(loss, red[0] * invN);
+}
+
+hipError_t dn_relmse(const float* out, int Cout, const float4* target, int H, int W, bf16_t* dout, float* loss, hipStream_t s)
+{
+	const size_t pixels = (size_t)H * W;
+	const int blocks = grid_for(pixels) < 1024 ? grid_for(pixels) : 1024;
+	k_relmse<<<blocks, 256, 0, s>>>(out, Cout, target, pixels, 1.0f / (3.0f * (float)pixels), dout, loss);
+	return hipGetLastError();
+}
+
+__global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g, size_t n,
+	float lr, float b1, float b2, float eps, float c1, float c2, bf16_t* __restrict__ wfwd, bf16_t* __restrict__ wbwd, int Cout, int Cin)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+	{
+		float w = p[i];
+		if (g)
+		{
+			const float gi = g[i];
+			const float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+			m[i] = mi, v[i] = vi;
+			w -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+			p[i] = w;
+		}
+		if (wfwd) wfwd[i] = f2bf(w);
+		if (wbwd)
+		{
+			const int ci = (int)(i % Cin), co = (int)((i / Cin) % Cout), tap = (int)(i / ((size_t)Cin * Cout));
+			wbwd[((size_t)(8 - tap) * Cin + ci) * Cout + co] = f2bf(w);
+		}
+	}
+}
+
+hipError_t dn_adam(float* p, float* m, float* v, const float* g, size_t n, float lr, float b1, float b2, float eps, int t,
+	bf16_t* wfwd, bf16_t* wbwd, int Cout, int Cin, hipStream_t s)
+{
+	if (wbwd && (Cout <= 0 || Cin <= 0 || n != (size_t)9 * Cout * Cin)) return hipErrorInvalidValue;
+	const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+	k_adam<<<grid_for(n), 256, 0, s>>>(p, m, v, g, n, lr, b1, b2, eps, c1, c2, wfwd, wbwd, Cout, Cin);
+	return hipGetLastError();
+}
+
+} // namespace lh

@@ -1,0 +1,123 @@
+"""Learned denoiser on the gfx950 matrix cores against the
+FP64 reference (oracle/denoiser_ref.py) with bf16 rounding emulated where
+the kernels store bf16. Tolerances: fp32-accumulated outputs within 1e-5 of
+the output scale; bf16-stored outputs within 1 bf16 ulp (2^-7 relative);
+whole-network gradients within 3e-2 relative L2 per layer (bf16 rounding
+flips of intermediate activations propagate)."""
+import numpy as np
+import pytest
+
+import lighthouse2_amd as lh
+from lighthouse2_amd import denoiser as D
+import denoiser_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def dev_bf16(torch, a):
+    return torch.from_numpy(D.bf16_bits(a).reshape(-1).copy()).cuda()
+
+
+def host_f32(t):
+    return t.cpu().numpy().astype(np.float32)
+
+
+@pytest.mark.parametrize("cin,cout", [(16, 32), (32, 32), (64, 32), (32, 16), (16, 16)])
+def test_conv_forward_dx_dw_match_fp64(torch_cuda, cin, cout):
+    torch = torch_cuda
+    L = lh.library()
+    rng = np.random.default_rng(cin * 7 + cout)
+    H, W = 20, 70                                   # ragged: W not a multiple of the 64-pixel tile
+    x = R.bf16(rng.standard_normal((H, W, cin)))
+    w = R.bf16(rng.standard_normal((9, cout, cin)) * 0.2)
+    b = rng.standard_normal(cout).astype(np.float32)
+    gy = R.bf16(rng.standard_normal((H, W, cout)))
+    dx_, dw_ = dev_bf16(torch, x), dev_bf16(torch, w)
+    db_ = torch.from_numpy(b).cuda()
+    sh = torch.cuda.current_stream().cuda_stream
+    y32 = torch.zeros(H * W * cout, dtype=torch.float32, device="cuda")
+    assert L.lh2hip_dn_conv3x3(dx_.data_ptr(), cin, dw_.data_ptr(), db_.data_ptr(), y32.data_ptr(), cout, H, W, 2, sh) == 0
+    yb = torch.zeros(H * W * cout, dtype=torch.int16, device="cuda")
+    assert L.lh2hip_dn_conv3x3(dx_.data_ptr(), cin, dw_.data_ptr(), db_.data_ptr(), yb.data_ptr(), cout, H, W, 1, sh) == 0
+    torch.cuda.synchronize()
+    ref = R.conv(x, w, b)
+    got = host_f32(y32).reshape(H, W, cout)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+    gotb = D.bf16_to_f32(yb.cpu().numpy()).reshape(H, W, cout)
+    refb = R.bf16(np.maximum(ref, 0))
+    assert np.all(np.abs(gotb - refb) <= np.abs(refb) * 2 ** -7 + 1e-30)
+    # input gradient through the flipped weight layout written by the Adam kernel
+    wf = torch.from_numpy(w.astype(np.float32).reshape(-1).copy()).cuda()
+    z = torch.zeros_like(wf)
+    wfwd = torch.zeros(wf.numel(), dtype=torch.int16, device="cuda")
+    wbwd = torch.zeros(wf.numel(), dtype=torch.int16, device="cuda")
+    assert L.lh2hip_dn_adam(wf.data_ptr(), z.data_ptr(), z.data_ptr(), None, wf.numel(), 0.0, 0.9, 0.999, 1e-8, 1,
+                            wfwd.data_ptr(), wbwd.data_ptr(), cout, cin, sh) == 0
+    if cin <= 64:
+        gyb = dev_bf16(torch, gy)
+        gx = torch.zeros(H * W * cin, dtype=torch.float32, device="cuda")
+        assert L.lh2hip_dn_conv3x3(gyb.data_ptr(), cout, wbwd.data_ptr(), None, gx.data_ptr(), cin, H, W, 2, sh) == 0
+        torch.cuda.synchronize()
+        rgx = R.conv_dx(gy, w)
+        assert np.max(np.abs(host_f32(gx).reshape(H, W, cin) - rgx)) <= 1e-5 * np.max(np.abs(rgx))
+    ws = torch.zeros(int(L.lh2hip_dn_dw_workspace_bytes(cin, cout)) // 4, dtype=torch.float32, device="cuda")
+    gw = torch.zeros(9 * cout * cin, dtype=torch.float32, device="cuda")
+    gb = torch.zeros(cout, dtype=torch.float32, device="cuda")
+    gyb = dev_bf16(torch, gy)
+    assert L.lh2hip_dn_conv3x3_dw(dx_.data_ptr(), cin, gyb.data_ptr(), cout, H, W, gw.data_ptr(), gb.data_ptr(), ws.data_ptr(), sh) == 0
+    torch.cuda.synchronize()
+    rgw, rgb = R.conv_dw(x, gy)
+    assert np.max(np.abs(host_f32(gw).reshape(9, cout, cin) - rgw)) <= 1e-5 * np.max(np.abs(rgw))
+    assert np.max(np.abs(host_f32(gb) - rgb)) <= 1e-5 * np.max(np.abs(rgb))
+
+
+def test_network_forward_and_gradients_match_fp64(torch_cuda):
+    torch = torch_cuda
+    H, W = 32, 48
+    params = D.init_params(seed=3)
+    rng = np.random.default_rng(4)
+    for k, (w, b) in params.items():               # non-zero biases exercise the bias paths
+        params[k] = (w, (rng.standard_normal(len(b)) * 0.05).astype(np.float32) * (0 if k == "out" else 1))
+    net = D.UNetDenoiser(H, W, params=params)
+    x = np.zeros((H, W, 16))
+    x[..., :10] = rng.uniform(0, 1, (H, W, 10))
+    t = rng.uniform(0.05, 1.0, (H, W, 4)).astype(np.float32)
+    net.set_input_packed(dev_bf16(torch, x))
+    out = net.forward()
+    loss = net.loss_backward(torch.from_numpy(t).cuda())
+    torch.cuda.synchronize()
+    rp = {k: (R.bf16(w.astype(np.float64)), b.astype(np.float64)) for k, (w, b) in params.items()}
+    rloss, rgrads, rout = R.loss_and_grads(rp, x, t.astype(np.float64), quant=R.bf16)
+    o = out.cpu().numpy()[..., :3]
+    assert np.max(np.abs(o - rout[..., :3])) <= 2e-2 * np.max(np.abs(rout[..., :3]))
+    assert abs(float(loss.item()) - rloss) <= 1e-2 * rloss
+    g = net.grads_numpy()
+    for name, (rgw, rgb) in rgrads.items():
+        gw = g[name][0].reshape(rgw.shape)
+        err = np.linalg.norm(gw - rgw) / max(np.linalg.norm(rgw), 1e-30)
+        assert err < 3e-2, (name, err)
+
+
+def test_training_reduces_relmse(torch_cuda):
+    """Denoise a noisy synthetic image (smooth radiance x albedo, features
+    clean): 150 Adam steps must cut the relMSE loss by more than half."""
+    torch = torch_cuda
+    H, W = 64, 64
+    rng = np.random.default_rng(5)
+    yy, xx = np.mgrid[0:H, 0:W] / 64.0
+    clean = np.stack([0.5 + 0.4 * np.sin(4 * xx), 0.5 + 0.4 * np.cos(3 * yy), 0.5 + 0.3 * np.sin(2 * (xx + yy)), np.ones_like(xx)], -1)
+    noisy = clean * rng.gamma(2.0, 0.5, (H, W, 1))
+    albedo = np.concatenate([clean[..., :3], np.ones((H, W, 1))], -1)
+    normal = np.zeros((H, W, 4))
+    depth = np.ones((H, W))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+    net = D.UNetDenoiser(H, W, seed=1)
+    net.set_input(dev(noisy), dev(albedo), dev(normal), dev(depth))
+    tgt = dev(clean)
+    losses = []
+    for i in range(150):
+        net.forward()
+        losses.append(float(net.loss_backward(tgt).item()))
+        net.step(lr=2e-3)
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
