@@ -135,6 +135,28 @@ LH2HIP_API int lh2hip_bvh_info(lh2hip_core* core, int meshIdx, int64_t* out8);
 LH2HIP_API int lh2hip_bvh_download(lh2hip_core* core, int meshIdx, void* nodesOut, int maxNodes,
 	uint32_t* primsOut, int maxPrims, int* nodeCount);
 
+/* ---- SVGF ----------------------------------------------------------------
+ * prepareFilter + applyFilter x 3 + finalizeNoTAA of RenderCore_Optix7Filter
+ * (lib/CUDA/shared_kernel_code/finalize_shared.h:169-484,587-600; driver
+ * lib/RenderCore_Optix7Filter/rendercore.cpp:897-935). The handle keeps the
+ * history (previous world positions, luminance moments, phase-1 output,
+ * history counts). Frame inputs are device arrays of w*h pixels:
+ * accumulator float4[2wh] (direct sums, then indirect sums), features uint4
+ * {HDRtoRGB32 albedo, PackNormal2 normal, depth bits, specular << 4 | matid << 6},
+ * worldPos float4 {xyz, packed normal | specular bits}, deltaDepth float4
+ * (z, w used); out float4[wh]: sqrt of the gamma-2 filtered image (border
+ * pixels left untouched, as the reference). */
+typedef struct lh2hip_svgf lh2hip_svgf;
+LH2HIP_API int lh2hip_svgf_create(int width, int height, lh2hip_svgf** out);
+LH2HIP_API int lh2hip_svgf_destroy(lh2hip_svgf* f);
+LH2HIP_API int lh2hip_svgf_frame(lh2hip_svgf* f, const lh2_float4* accumulator, const uint32_t* features, const lh2_float4* worldPos,
+	const lh2_float4* deltaDepth, const ViewPyramid* prevView, float j0, float j1, float prevj0, float prevj1, int samplesTaken,
+	float clampDirect, float clampIndirect, int camIsStationary, lh2_float4* out, void* stream);
+/* Copies internal state to host memory: which 0 features (uint4), 1 luminance
+ * moments (float4), 2 motion (float2), 3 phase-1 filtered history (float4),
+ * 4 last-pass image (float4), 5 world positions of the last frame (float4). */
+LH2HIP_API int lh2hip_svgf_read(lh2hip_svgf* f, int which, void* hostOut);
+
 /* ---- learned denoiser kernels --------------------------------------------
  * No counterpart in the reference. A 5-level 3x3 convolutional U-Net is
  * composed from these calls (lighthouse2_amd/denoiser.py); tensors are NHWC

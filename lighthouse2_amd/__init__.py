@@ -5,4 +5,4 @@ include/lh2hip.h); this package holds its Python binding (`core`), the boundary
 struct layouts (`formats`) and the host-side scene producers (`scenes`).
 """
 from . import formats, scenes  # noqa: F401
-from .core import RenderCore, RenderServer, LH2HIPError, load_scene, library, LH2_CONVERGE, LH2_RESTART  # noqa: F401
+from .core import RenderCore, RenderServer, Svgf, LH2HIPError, load_scene, library, LH2_CONVERGE, LH2_RESTART  # noqa: F401

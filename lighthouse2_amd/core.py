@@ -68,6 +68,11 @@ _SIGS = {
     "lh2hip_dn_relmse": (I, [P, I, P, I, I, P, P, P]),
     "lh2hip_dn_adam": (I, [P, P, P, P, ctypes.c_uint64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, I,
                            P, P, I, I, P]),
+    "lh2hip_svgf_create": (I, [I, I, ctypes.POINTER(P)]),
+    "lh2hip_svgf_destroy": (I, [P]),
+    "lh2hip_svgf_frame": (I, [P, P, P, P, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, I,
+                              ctypes.c_float, ctypes.c_float, I, P, P]),
+    "lh2hip_svgf_read": (I, [P, I, P]),
     "lh2hip_server_create": (I, [P, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(P)]),
     "lh2hip_server_destroy": (I, [P]),
     "lh2hip_server_submit": (I, [P, P, I, I, I, I, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
@@ -277,6 +282,36 @@ class RenderServer:
     def close(self):
         if getattr(self, "_h", None):
             library().lh2hip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Svgf:
+    """SVGF filter state on the GPU (lh2hip_svgf_*); frame() takes device
+    pointers, read() returns internal buffers as numpy arrays."""
+
+    def __init__(self, width, height):
+        h = P()
+        _check(library().lh2hip_svgf_create(width, height, ctypes.byref(h)))
+        self._h, self.w, self.h = h, width, height
+
+    def frame(self, acc_ptr, features_ptr, world_pos_ptr, delta_depth_ptr, prev_view, out_ptr, j=(0.0, 0.0), prev_j=(0.0, 0.0),
+              samples=1, clamp_direct=2.5, clamp_indirect=15.0, stationary=0, stream=None):
+        v = np.ascontiguousarray(prev_view, F.ViewPyramid)
+        _check(library().lh2hip_svgf_frame(self._h, acc_ptr, features_ptr, world_pos_ptr, delta_depth_ptr, _ptr(v), j[0], j[1],
+                                           prev_j[0], prev_j[1], samples, clamp_direct, clamp_indirect, stationary, out_ptr, stream))
+
+    def read(self, which):
+        shape = {0: ((self.h, self.w, 4), np.uint32), 2: ((self.h, self.w, 2), np.float32)}.get(which, ((self.h, self.w, 4), np.float32))
+        out = np.zeros(shape[0], shape[1])
+        _check(library().lh2hip_svgf_read(self._h, which, _ptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().lh2hip_svgf_destroy(self._h)
             self._h = None
 
     def __del__(self):

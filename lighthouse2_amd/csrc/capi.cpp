@@ -5,6 +5,7 @@
 #include "rendercore.h"
 #include "server.h"
 #include "denoise.h"
+#include "svgf.h"
 #include <cstring>
 #include <new>
 
@@ -13,6 +14,7 @@ using lh::CoreError;
 
 struct lh2hip_core { RenderCore core; };
 struct lh2hip_server { lh::RenderServer* srv; };
+struct lh2hip_svgf { lh::Svgf* f; };
 
 static thread_local std::string g_lastError;
 
@@ -353,6 +355,59 @@ LH2HIP_API int lh2hip_dn_adam(float* p, float* m, float* v, const float* g, uint
 {
 	return dn_status(lh::dn_adam(p, m, v, g, (size_t)n, lr, b1, b2, eps, t, (lh::bf16_t*)wfwd, (lh::bf16_t*)wbwd, Cout, Cin, (hipStream_t)stream),
 		"dn_adam (a flipped copy needs n = 9 Cout Cin)");
+}
+
+
+LH2HIP_API int lh2hip_svgf_create(int width, int height, lh2hip_svgf** out)
+{
+	if (!out) return fail(LH2HIP_ERR_INVALID, "null output");
+	*out = nullptr;
+	try
+	{
+		lh2hip_svgf* h = new lh2hip_svgf;
+		try { h->f = new lh::Svgf(width, height); }
+		catch (...) { delete h; throw; }
+		*out = h;
+		return LH2HIP_OK;
+	}
+	catch (const CoreError& e) { return fail(e.code, e.what()); }
+	catch (const std::exception& e) { return fail(LH2HIP_ERR_DEVICE, e.what()); }
+}
+
+LH2HIP_API int lh2hip_svgf_destroy(lh2hip_svgf* f)
+{
+	if (f) delete f->f, delete f;
+	return LH2HIP_OK;
+}
+
+LH2HIP_API int lh2hip_svgf_frame(lh2hip_svgf* f, const lh2_float4* accumulator, const uint32_t* features, const lh2_float4* worldPos,
+	const lh2_float4* deltaDepth, const ViewPyramid* prevView, float j0, float j1, float prevj0, float prevj1, int samplesTaken,
+	float clampDirect, float clampIndirect, int camIsStationary, lh2_float4* out, void* stream)
+{
+	if (!f || !f->f) return fail(LH2HIP_ERR_INVALID, "null svgf handle");
+	if (!accumulator || !features || !worldPos || !deltaDepth || !prevView || !out) return fail(LH2HIP_ERR_INVALID, "svgf frame: null pointer");
+	try
+	{
+		lighthouse2::ViewPyramid v;
+		memcpy(&v, prevView, sizeof(v));
+		f->f->Frame((const float4*)accumulator, (const uint4*)features, (const float4*)worldPos, (const float4*)deltaDepth, v, j0, j1, prevj0, prevj1,
+			samplesTaken, clampDirect, clampIndirect, camIsStationary, (float4*)out, (hipStream_t)stream);
+		return LH2HIP_OK;
+	}
+	catch (const CoreError& e) { return fail(e.code, e.what()); }
+	catch (const std::exception& e) { return fail(LH2HIP_ERR_DEVICE, e.what()); }
+}
+
+LH2HIP_API int lh2hip_svgf_read(lh2hip_svgf* f, int which, void* hostOut)
+{
+	if (!f || !f->f || !hostOut) return fail(LH2HIP_ERR_INVALID, "svgf read: null argument");
+	lh::Svgf& s = *f->f;
+	lh::DevBuf* b = which == 0 ? &s.features : which == 1 ? &s.prevMoments : which == 2 ? &s.motion : which == 3 ? &s.filteredIn
+		: which == 4 ? &s.lastPass : which == 5 ? &s.prevWorldPos : nullptr;
+	if (!b) return fail(LH2HIP_ERR_INVALID, "svgf read: which 0..5");
+	if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(hostOut, b->p, b->bytes, hipMemcpyDeviceToHost) != hipSuccess)
+		return fail(LH2HIP_ERR_DEVICE, "svgf read: copy failed");
+	return LH2HIP_OK;
 }
 
 } /* extern "C" */

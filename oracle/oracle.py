@@ -38,6 +38,10 @@ def lib():
             "orc_set_textures": (None, [P, P, I]), "orc_set_sky": (None, [P, P, I, I, P]),
             "orc_sample_sky": (None, [P, P, I, P]), "orc_fetch_texels": (None, [P, I, P, I, I, ctypes.c_float, P]),
             "orc_math": (None, [I, P, P, I, P]),
+            "orc_svgf_create": (P, [I, I]), "orc_svgf_destroy": (None, [P]),
+            "orc_svgf_frame": (None, [P, P, P, P, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, I,
+                                      ctypes.c_float, ctypes.c_float, I, P]),
+            "orc_svgf_read": (None, [P, I, P]),
         }
         for n, (r, a) in sigs.items():
             f = getattr(L, n)
@@ -191,6 +195,37 @@ def render(scene_or_oracle, view, w, h, spp=1, frames=1, max_depth=3, eps=1e-3, 
             acc64 += blk.astype(np.float64)
         taken += spp
     return acc32.reshape(h, w, 4), acc64.reshape(h, w, 4), taken
+
+
+class OracleSvgf:
+    """CPU restatement of SVGF (oracle_svgf.cpp) with history across frames."""
+
+    def __init__(self, w, h):
+        self.w, self.h = w, h
+        self.hd = lib().orc_svgf_create(w, h)
+
+    def __del__(self):
+        if getattr(self, "hd", None):
+            lib().orc_svgf_destroy(self.hd)
+            self.hd = None
+
+    def frame(self, acc, features, world_pos, delta_depth, prev_view, j=(0.0, 0.0), prev_j=(0.0, 0.0), samples=1,
+              clamp_direct=2.5, clamp_indirect=15.0, stationary=0, out=None):
+        a = np.ascontiguousarray(acc, np.float32)
+        f = np.ascontiguousarray(features, np.uint32)
+        wp = np.ascontiguousarray(world_pos, np.float32)
+        dd = np.ascontiguousarray(delta_depth, np.float32)
+        v = np.ascontiguousarray(prev_view)
+        o = np.zeros((self.h, self.w, 4), np.float32) if out is None else out
+        lib().orc_svgf_frame(self.hd, _p(a), _p(f), _p(wp), _p(dd), _p(v), j[0], j[1], prev_j[0], prev_j[1], samples,
+                             clamp_direct, clamp_indirect, stationary, _p(o))
+        return o
+
+    def read(self, which):
+        shape = {0: ((self.h, self.w, 4), np.uint32), 2: ((self.h, self.w, 2), np.float32)}.get(which, ((self.h, self.w, 4), np.float32))
+        out = np.zeros(shape[0], shape[1])
+        lib().orc_svgf_read(self.hd, which, _p(out))
+        return out
 
 
 def math(which, a, b=None):
