@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent traversal blocks per CU (0: core default)")
     ap.add_argument("--min-rays-per-lane", type=float, default=0.0, help="shrink the grid of small waves (0: off)")
+    ap.add_argument("--workload", choices=["extend", "denoise", "svgf", "serve"], default="extend",
+                    help="extend: the headline traversal metric; denoise: U-Net forward/train at 1088x1920; "
+                         "svgf: filter ms/frame at 1920x1080; serve: job-server latency under Poisson arrivals")
+    ap.add_argument("--serve-jobs", type=int, default=40)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (FETCH_SIZE/WRITE_SIZE calibration)")
     return ap.parse_args()
@@ -89,6 +93,126 @@ class Waves:
         return counts
 
 
+def _events(torch, stream, fn, reps):
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record(stream)
+    for _ in range(reps):
+        fn()
+    e.record(stream)
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def bench_denoise(a, np, torch, lh, scenes):
+    """U-Net denoiser at 1088x1920 (1080p padded to the 16-pixel grid of 4
+    pooling steps): forward ms and TFLOP/s (multiply-add FLOPs of the 19
+    convolutions), and one training step (forward + backward + Adam)."""
+    from lighthouse2_amd import denoiser as D
+    H, W = 1088, 1920
+    net = D.UNetDenoiser(H, W, seed=0)
+    rng = np.random.default_rng(0)
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x, np.float32)).cuda()
+    rad, alb, nrm = (dev(rng.uniform(0, 1, (H, W, 4))) for _ in range(3))
+    dep = dev(rng.uniform(0, 10, (H, W)))
+    tgt = dev(rng.uniform(0.1, 1, (H, W, 4)))
+    net.set_input(rad, alb, nrm, dep)
+    st = net.stream
+    for _ in range(a.warmup):
+        net.forward()
+    fwd_ms = _events(torch, st, net.forward, a.steps)
+
+    def train():
+        net.forward()
+        net.loss_backward(tgt)
+        net.step(lr=1e-4)
+    for _ in range(2):
+        train()
+    train_ms = _events(torch, st, train, max(2, a.steps // 2))
+    flops = net.flops_forward()
+    tf = flops / (fwd_ms * 1e-3) / 1e12
+    print(json.dumps({"metric": "ms/frame learned denoiser forward 1080p", "value": round(fwd_ms, 3), "unit": "ms",
+                      "higher_is_better": False, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "dtype": "bf16 (fp32 accumulate)",
+                      "data": "synthetic", "vs_baseline": None,
+                      "config": {"workload": "5-level 3x3 conv U-Net, 32 channels, 1088x1920", "layers": len(D.layer_specs())},
+                      "train_step_ms": round(train_ms, 3), "forward_gflop": round(flops / 1e9, 1),
+                      "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tf / 2500.0, 4),
+                                   "traffic": None}}))
+
+
+def bench_svgf(a, np, torch, lh, scenes):
+    """SVGF filter (prepare + 3 a-trous passes + display) on a 1920x1080
+    synthetic G-buffer, moving camera: ms per frame."""
+    W, H = 1920, 1080
+    v0 = scenes.camera_view((0.5, 0.6, -1.0), (0.5, 0.5, 1.0), 50.0, W, H)
+    v1 = scenes.camera_view((0.505, 0.6, -0.999), (0.5, 0.5, 1.0), 50.0, W, H)
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    g0 = [dev(x) for x in scenes.gbuffer(v0, W, H, seed=1)]
+    g0[1] = dev(scenes.gbuffer(v0, W, H, seed=1)[1].view(np.int32))
+    out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    f = lh.Svgf(W, H)
+    st = torch.cuda.current_stream()
+
+    def frame():
+        f.frame(g0[0].data_ptr(), g0[1].data_ptr(), g0[2].data_ptr(), g0[3].data_ptr(), v1, out.data_ptr(), j=(0.1, -0.1),
+                samples=1, stationary=0, stream=st.cuda_stream)
+    for _ in range(a.warmup):
+        frame()
+    ms = _events(torch, st, frame, a.steps)
+    # compulsory bytes per pixel over all passes: prepare reads 2x16 acc + 16 feat + 16 pos + 16 prevpos + 16 moments (history),
+    # writes 16 shading + 8 motion + 16 moments + 16 feat; each a-trous pass reads ~16 A + 16 feat + 16 moments + 16 depth;
+    # phase 1 adds 16 motion-history; plus 16 out: about 240 B/px
+    bpp = 240
+    gbs = W * H * bpp / (ms * 1e-3) / 1e9
+    print(json.dumps({"metric": "ms/frame SVGF 1080p", "value": round(ms, 3), "unit": "ms", "higher_is_better": False, "n_gpus": 1,
+                      "steps": a.steps, "warmup": a.warmup, "dtype": "f32", "data": "synthetic G-buffer", "vs_baseline": None,
+                      "config": {"workload": "SVGF prepare + 3 a-trous passes + display, 1920x1080, moving camera"},
+                      "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s", "frac": round(gbs / 8000.0, 4),
+                                   "traffic": None, "bytes_per_pixel": bpp}}))
+
+
+def bench_serve(a, np, torch, lh, scenes):
+    """Job server under Poisson arrivals (lambda 20 jobs/s, seed 7):
+    resolutions {256^2, 512^2, 1920x1080, 3840x2160}, spp {1, 4, 16, 64},
+    depth U{4..8}, on the Cornell box; latency = completion - arrival."""
+    import threading
+    s = scenes.cornell_box(sphere_tris=20000)
+    core = lh.RenderCore()
+    lh.load_scene(core, s)
+    srv = lh.RenderServer(core, budget_bytes=64 << 30, pass_paths=1 << 21)
+    rng = np.random.default_rng(7)
+    res = [(256, 256), (512, 512), (1920, 1080), (3840, 2160)]
+    jobs = [(res[rng.integers(4)], int(rng.choice([1, 4, 16, 64])), int(rng.integers(4, 9))) for _ in range(a.serve_jobs)]
+    gaps = rng.exponential(1 / 20.0, a.serve_jobs)
+    ids = []
+    t_start = time.time()
+
+    def submitter():
+        t = time.time()
+        for (wh, spp, d), gap in zip(jobs, gaps):
+            t += gap
+            time.sleep(max(0.0, t - time.time()))
+            ids.append(srv.submit(scenes.cornell_view(*wh), wh[0], wh[1], spp, d, 1234))
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    lat, paths = [], 0
+    for j, ((w, h), spp, d) in zip(ids, jobs):
+        srv.wait(j)
+        t = srv.times(j)
+        lat.append(t[2] - t[0])
+        paths += w * h * spp
+        srv.read(j)
+    makespan = time.time() - t_start
+    lat = np.array(lat)
+    print(json.dumps({"metric": "p50/p99 job latency", "value": round(float(np.percentile(lat, 50)), 3), "unit": "s",
+                      "p99_s": round(float(np.percentile(lat, 99)), 3), "higher_is_better": False, "n_gpus": 1,
+                      "jobs": len(lat), "lambda_jobs_per_s": 20, "makespan_s": round(makespan, 2),
+                      "mpaths_per_s": round(paths / makespan / 1e6, 1), "dtype": "f32", "data": "synthetic", "vs_baseline": None,
+                      "config": {"workload": "Poisson job stream on the Cornell box (20k-triangle sphere), mixed resolution/spp/depth",
+                                 "budget_gb": 64, "pass_paths": 1 << 21}}))
+    srv.close()
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -101,6 +225,9 @@ def main():
     from lighthouse2_amd import scenes
 
     torch.cuda.set_device(0)
+    if a.workload != "extend":
+        {"denoise": bench_denoise, "svgf": bench_svgf, "serve": bench_serve}[a.workload](a, np, torch, lh, scenes)
+        return
     if a.copy_bytes:
         src = torch.ones(a.copy_bytes // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

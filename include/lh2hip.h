@@ -54,7 +54,10 @@ LH2HIP_API int lh2hip_set_target(lh2hip_core* core, int width, int height, int s
  * greedy loop weight of waiting leaf lanes in 1/16, default 16;
  * "traceRefillLanes": idle lanes of a wave that trigger fetching new rays,
  * 1..64, default 16; "traceChunk": rays a wave reserves per atomic, 1..1024,
- * default 32). */
+ * default 32; "estimator": 0 the shade kernel of RenderCore_PrimeRef
+ * (kernels/pathtracer.h, default), 1 that of rendercore_optix7 with its
+ * Lambert BSDF (kernels/pathtracer.h:54-238, sharedBSDFs/lambert.h): blue
+ * noise, MIS, firefly clamp "clampValue", at most 3 path vertices). */
 LH2HIP_API int lh2hip_setting(lh2hip_core* core, const char* name, float value);
 /* CoreAPI_Base::SetGeometry (rendercore.cpp:214-222; core_mesh.cpp:36-64).
  * vertexData: 3 float4 per triangle (w ignored); builds the mesh BVH on the GPU. */
@@ -77,6 +80,11 @@ LH2HIP_API int lh2hip_set_textures(lh2hip_core* core, const CoreTexDesc* tex, in
 /* CoreAPI_Base::SetSkyData (rendercore.cpp:466-478). */
 LH2HIP_API int lh2hip_set_sky(lh2hip_core* core, const lh2_float3* pixels, int width, int height,
 	const lh2_mat4* worldToLight);
+/* Blue-noise tables of the optix7 estimator (rendercore_optix7/rendercore.cpp:246-255
+ * converts its 8-bit sob256_64 / scr256_64 / rnk256_64 tables the same way):
+ * 65536 sequence, 131072 scrambling and 131072 ranking entries. Optional; a
+ * seeded generator fills the tables otherwise. */
+LH2HIP_API int lh2hip_set_blue_noise(lh2hip_core* core, const uint8_t* sequence, const uint8_t* scrambling, const uint8_t* ranking);
 /* CoreAPI_Base::SetProbePos (rendercore.cpp:91-94). */
 LH2HIP_API int lh2hip_set_probe_pos(lh2hip_core* core, int x, int y);
 /* CoreAPI_Base::Render (rendercore.cpp:535-632): one frame of spp samples per

@@ -43,6 +43,7 @@ _SIGS = {
     "lh2hip_set_lights": (I, [P, P, I, P, I, P, I, P, I]),
     "lh2hip_set_textures": (I, [P, P, I]),
     "lh2hip_set_sky": (I, [P, P, I, I, P]),
+    "lh2hip_set_blue_noise": (I, [P, P, P, P]),
     "lh2hip_set_probe_pos": (I, [P, I, I]),
     "lh2hip_render": (I, [P, P, I, I]),
     "lh2hip_wait": (I, [P]),
@@ -179,6 +180,14 @@ class RenderCore:
         m = None if worldToLight is None else np.ascontiguousarray(worldToLight, np.float32).reshape(16)
         _check(library().lh2hip_set_sky(self._h, _ptr(p), width, height, _ptr(m)))
 
+    def SetBlueNoise(self, sequence, scrambling, ranking):
+        """Blue-noise tables of the optix7 estimator (8-bit entries: 65536
+        sequence, 131072 scrambling, 131072 ranking)."""
+        t = [np.ascontiguousarray(x, np.uint8).reshape(-1) for x in (sequence, scrambling, ranking)]
+        if t[0].size != 65536 or t[1].size != 131072 or t[2].size != 131072:
+            raise ValueError("SetBlueNoise: sizes 65536, 131072, 131072")
+        _check(library().lh2hip_set_blue_noise(self._h, _ptr(t[0]), _ptr(t[1]), _ptr(t[2])))
+
     def SetProbePos(self, x, y):
         _check(library().lh2hip_set_probe_pos(self._h, x, y))
 
@@ -273,10 +282,13 @@ class RenderServer:
         return t
 
     def read(self, jid, with_acc=False):
-        h, w = self._sizes.pop(jid)
+        if jid not in self._sizes:
+            raise LH2HIPError(-1, "server: unknown job (never submitted or already read)")
+        h, w = self._sizes[jid]
         img = np.zeros((h, w, 4), np.float32)
         acc = np.zeros((h, w, 4), np.float64) if with_acc else None
         _check(library().lh2hip_server_read(self._h, jid, _ptr(img), _ptr(acc) if with_acc else None))
+        del self._sizes[jid]
         return (img, acc) if with_acc else img
 
     def close(self):

@@ -134,6 +134,11 @@ LH2HIP_API int lh2hip_set_sky(lh2hip_core* core, const lh2_float3* pixels, int w
 	});
 }
 
+LH2HIP_API int lh2hip_set_blue_noise(lh2hip_core* core, const uint8_t* sequence, const uint8_t* scrambling, const uint8_t* ranking)
+{
+	return guarded(core, [&](RenderCore& c) { c.SetBlueNoise(sequence, scrambling, ranking); });
+}
+
 LH2HIP_API int lh2hip_set_probe_pos(lh2hip_core* core, int x, int y)
 {
 	return guarded(core, [&](RenderCore& c) { lh2_int2 p; p.x = x, p.y = y; c.SetProbePos(p); });

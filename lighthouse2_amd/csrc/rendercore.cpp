@@ -120,15 +120,16 @@ void Wavefront::resizePaths(size_t n, hipStream_t s)
 	if (n <= pathCap) return;
 	pathCap = n;
 	for (int i = 0; i < 2; i++)
-		rays[i].ensure(32 * n), state[i].ensure(16 * n), pathIdB[i].ensure(4 * n);
+		rays[i].ensure(32 * n), state[i].ensure(16 * n), pathIdB[i].ensure(4 * n), auxB[i].ensure(8 * n);
 	hitsB.ensure(16 * n), shadowRaysB.ensure(32 * n), shadowPotB.ensure(16 * n), occludedB.ensure(4 * n), pathAccB.ensure(16 * n);
 	countersB.ensure(64);
+	countLogB.ensure(sizeof(uint32_t) * 1100);   /* one slot per wave, maxDepth <= 1024 */
 	hipCheck(hipMemsetAsync(countersB.p, 0, 64, s), "memset");
 }
 
 void Wavefront::release()
 {
-	for (int i = 0; i < 2; i++) rays[i].release(), state[i].release(), pathIdB[i].release();
+	for (int i = 0; i < 2; i++) rays[i].release(), state[i].release(), pathIdB[i].release(), auxB[i].release();
 	hitsB.release(), shadowRaysB.release(), shadowPotB.release(), occludedB.release(), pathAccB.release();
 	countersB.release(), countLogB.release(), probeB.release(), acc32B.release(), acc64B.release(), imageB.release();
 	pathCap = 0, pixelCap = 0;
@@ -137,7 +138,8 @@ void Wavefront::release()
 PathBuffers Wavefront::buffers()
 {
 	PathBuffers b;
-	for (int i = 0; i < 2; i++) b.rays[i] = rays[i].as<float4>(), b.state[i] = state[i].as<float4>(), b.pathId[i] = pathIdB[i].as<uint32_t>();
+	for (int i = 0; i < 2; i++)
+		b.rays[i] = rays[i].as<float4>(), b.state[i] = state[i].as<float4>(), b.pathId[i] = pathIdB[i].as<uint32_t>(), b.aux[i] = auxB[i].as<float2>();
 	b.hits = hitsB.as<float4>(), b.shadowRays = shadowRaysB.as<float4>(), b.shadowPot = shadowPotB.as<float4>();
 	b.occluded = occludedB.as<uint32_t>(), b.pathAcc = pathAccB.as<float4>(), b.counters = countersB.as<uint32_t>();
 	return b;
@@ -244,6 +246,11 @@ void RenderCore::Setting(const char* name, float value)
 		scene.triBias = triBias;
 	}
 	else if (!strcmp(name, "seed")) seedSetting = camSeed = (uint32_t)value; /* replaces camRNGseed's 0x12345678 */
+	else if (!strcmp(name, "estimator"))
+	{	/* 0: RenderCore_PrimeRef's shadeKernel, 1: rendercore_optix7's (Lambert BSDF) */
+		if (value != 0 && value != 1) throw CoreError(LH2HIP_ERR_INVALID, "Setting estimator: 0 (reference) or 1 (optix7)");
+		estimator = (int)value;
+	}
 	else if (!strcmp(name, "traceMinRaysPerLane"))
 	{
 		if (!(value >= 0 && value <= 64)) throw CoreError(LH2HIP_ERR_INVALID, "Setting traceMinRaysPerLane: 0..64");
@@ -658,15 +665,41 @@ void RenderCore::SetTextures(const CoreTexDesc* tex, const int textureCount)
 
 void RenderCore::SetSkyData(const lighthouse2::float3* pixels, const uint32_t width, const uint32_t height, const mat4& worldToLight)
 {
-	/* rendercore.cpp:466-478: float3 texels stored as float4 (w = 0). */
+	/* rendercore.cpp:466-478: float3 texels stored as float4 (w = 0), followed
+	 * by the 64x64-box downsampled dome the optix7 estimator samples after a
+	 * diffuse bounce (rendercore_optix7/rendercore.cpp:716-740; float sums, rows
+	 * outer, columns inner, then times 1/4096). */
 	const size_t n = (size_t)width * height;
 	if (n > 0 && !pixels) throw CoreError(LH2HIP_ERR_INVALID, "SetSkyData: no pixels");
-	std::vector<float4> px(std::max<size_t>(n, 1), make_float4(0, 0, 0, 0));
+	const uint32_t sw = width >> 6, sh = height >> 6;
+	std::vector<float4> px(std::max<size_t>(n + (size_t)sw * sh, 1), make_float4(0, 0, 0, 0));
 	for (size_t i = 0; i < n; i++) px[i] = make_float4(pixels[i].x, pixels[i].y, pixels[i].z, 0.0f);
+	for (uint32_t y = 0; y < sh; y++) for (uint32_t x = 0; x < sw; x++)
+	{
+		float t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+		const float4* tile = px.data() + (size_t)x * 64 + (size_t)y * 64 * width;
+		for (int v = 0; v < 64; v++) for (int u = 0; u < 64; u++)
+		{
+			const float4 q = tile[u + (size_t)v * width];
+			t0 += q.x, t1 += q.y, t2 += q.z, t3 += q.w;
+		}
+		const float k = 1.0f / (64 * 64);
+		px[n + x + (size_t)y * sw] = make_float4(t0 * k, t1 * k, t2 * k, t3 * k);
+	}
 	skyPixels.upload(px.data(), px.size() * sizeof(float4), stream);
 	hipCheck(hipStreamSynchronize(stream), "sync");
 	skyW = (int)width, skyH = (int)height;
 	for (int r = 0; r < 3; r++) for (int c = 0; c < 4; c++) worldToSky[4 * r + c] = worldToLight.cell[4 * r + c];
+}
+
+void RenderCore::SetBlueNoise(const uint8_t* sequence, const uint8_t* scrambling, const uint8_t* ranking)
+{
+	if (!sequence || !scrambling || !ranking) throw CoreError(LH2HIP_ERR_INVALID, "SetBlueNoise: null table");
+	std::vector<uint32_t> bn(BLUE_NOISE_WORDS, 0u);
+	for (int i = 0; i < 65536; i++) bn[i] = sequence[i];
+	for (int i = 0; i < 128 * 128 * 8; i++) bn[65536 + i] = scrambling[i], bn[3 * 65536 + i] = ranking[i];
+	blueNoise.upload(bn.data(), bn.size() * 4, stream);
+	hipCheck(hipStreamSynchronize(stream), "sync");
 }
 
 void RenderCore::checkRenderable()
@@ -686,7 +719,7 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
 {
 	FrameSetup f;
 	memcpy(f.cam.pos, &view.pos, 12), memcpy(f.cam.p1, &view.p1, 12), memcpy(f.cam.p2, &view.p2, 12), memcpy(f.cam.p3, &view.p3, 12);
-	f.cam.aperture = view.aperture, f.cam.spreadAngle = view.spreadAngle, f.cam.w = w, f.cam.h = h;
+	f.cam.aperture = view.aperture, f.cam.spreadAngle = view.spreadAngle, f.cam.distortion = view.distortion, f.cam.w = w, f.cam.h = h;
 	ShadeParams& P = f.P;
 	memset(&P, 0, sizeof(P));
 	P.inst = shadeInsts.as<ShadeInst>();
@@ -702,7 +735,24 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
 	P.w2s0 = make_float4(worldToSky[0], worldToSky[1], worldToSky[2], 0.0f);
 	P.w2s1 = make_float4(worldToSky[4], worldToSky[5], worldToSky[6], 0.0f);
 	P.w2s2 = make_float4(worldToSky[8], worldToSky[9], worldToSky[10], 0.0f);
-	f.R0gen = R0gen, f.samplesTaken = samplesTakenAtStart, f.spp = spp, f.maxDepth = depth, f.pixels = w * h;
+	P.clampValue = clampValue;
+	P.connections = (P.nTri + P.nPoint + P.nSpot + P.nDir) > 0;   /* noDirectLightsInScene, rendercore_optix7/rendercore.cpp:708 */
+	f.estimator = estimator;
+	if (estimator == ESTIMATOR_OPTIX7)
+	{
+		if (!blueNoise.p)
+		{
+			std::vector<uint32_t> bn(BLUE_NOISE_WORDS);
+			blue_noise_tables(bn.data(), BLUE_NOISE_SEED);
+			blueNoise.upload(bn.data(), bn.size() * 4, stream);
+			hipCheck(hipStreamSynchronize(stream), "sync");
+		}
+		P.blueNoise = blueNoise.as<uint32_t>();
+		P.shift = shiftSeed;
+		f.maxDepth = MAXPATHLENGTH_OPTIX7;
+	}
+	f.R0gen = R0gen, f.samplesTaken = samplesTakenAtStart, f.spp = spp, f.pixels = w * h;
+	if (estimator != ESTIMATOR_OPTIX7) f.maxDepth = depth;
 	return f;
 }
 
@@ -711,12 +761,22 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
  * resolved into acc32/acc64 in sample order. The image depends on neither
  * the pass size nor on how the samples are split between calls: a path's
  * random numbers are keyed by its index within the frame. */
-void RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
+int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
 	uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing)
 {
 	FrameSetup f = f0;
 	const TraceScene& sc = scene;
 	const int pixels = f.pixels;
+	const bool o7 = f.estimator == ESTIMATOR_OPTIX7;
+	/* optix7: R0 of shade wave k is RandomUInt(camRNGseed) + k * 91771, the
+	 * seed advancing once per wave the frame runs (rendercore.cpp:881-905) */
+	uint32_t R0wave[MAXPATHLENGTH_OPTIX7 + 1] = { 0 };
+	if (o7)
+	{
+		uint32_t cs = f.camSeed;
+		for (int k = 1; k <= MAXPATHLENGTH_OPTIX7; k++) R0wave[k] = xor32_host(cs) + (uint32_t)k * 91771u;
+	}
+	int deepest = 0;
 	const int passSamples = std::max(1, std::min(s1 - s0, (int)(std::min(passCap, Wavefront::PASS_CAP) / (size_t)pixels)));
 	w.resizePaths((size_t)passSamples * pixels, st);
 	for (int a = s0; a < s1; a += passSamples)
@@ -725,7 +785,8 @@ void RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap
 		const int n = k * pixels;
 		PathBuffers B = w.buffers();
 		f.P.jobBase = (uint32_t)a * (uint32_t)pixels;
-		hipCheck(launch_generate(f.cam, f.R0gen, f.P.jobBase, n, geometryEpsilon, B, 0, st), "generate");
+		if (o7) hipCheck(launch_generate7(f.cam, f.P, f.P.jobBase, n, geometryEpsilon, B, 0, st), "generate");
+		else hipCheck(launch_generate(f.cam, f.R0gen, f.P.jobBase, n, geometryEpsilon, B, 0, st), "generate");
 		int in = 0;
 		for (int pathLength = 1; pathLength <= f.maxDepth; pathLength++)
 		{
@@ -733,19 +794,31 @@ void RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap
 			hipCheck(launch_trace_dev(sc, MODE_CLOSEST, B.rays[in], B.hits, n, B.counters + 0, queue, ovf, persistentBlocks, st), "extend");
 			if (evs) hipCheck(hipEventRecord(evs[1], st), "event");
 			f.P.pathLength = pathLength;
-			f.P.R0 = (uint32_t)f.samplesTaken * 7907u + (uint32_t)pathLength * 91771u;
-			hipCheck(launch_shade(f.P, B, in, n, st), "shade");
+			if (o7)
+			{
+				f.P.R0 = R0wave[pathLength];
+				hipCheck(launch_shade7(f.P, B, in, n, st), "shade");
+			}
+			else
+			{
+				f.P.R0 = (uint32_t)f.samplesTaken * 7907u + (uint32_t)pathLength * 91771u;
+				hipCheck(launch_shade(f.P, B, in, n, st), "shade");
+			}
 			if (evs) hipCheck(hipEventRecord(evs[2], st), "event");
-			if (pathLength < f.maxDepth)
+			/* the reference estimator connects from every vertex but the last;
+			 * optix7's from every vertex (its NEE precedes the depth cut) */
+			if (o7 || pathLength < f.maxDepth)
 			{
 				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, B.counters + 2, queue, ovf, persistentBlocks, st), "connect");
-				hipCheck(launch_connect(B, n, st), "connect");
+				hipCheck(launch_connect(B, n, st, o7 ? 1.0f : 0.0f), "connect");
 			}
 			if (evs) hipCheck(hipEventRecord(evs[3], st), "event");
-			hipCheck(launch_advance(B.counters, w.countLog() + pathLength, st), "advance");
+			hipCheck(launch_advance(B.counters, w.countLog() ? w.countLog() + pathLength : nullptr, st), "advance");
 			in ^= 1;
-			/* stop early once every path has terminated (checked every 4 waves) */
-			if ((pathLength & 3) == 0 || pathLength == f.maxDepth)
+			deepest = std::max(deepest, pathLength);
+			/* stop early once every path has terminated (checked every 4 waves;
+			 * after every wave for optix7, whose loop reads the count back each time) */
+			if (o7 || (pathLength & 3) == 0 || pathLength == f.maxDepth)
 			{
 				uint32_t live = 0;
 				hipCheck(hipMemcpyAsync(&live, B.counters + 0, 4, hipMemcpyDeviceToHost, st), "D2H");
@@ -761,6 +834,7 @@ void RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap
 		}
 		hipCheck(launch_resolve(B.pathAcc, pixels, k, acc32, acc64, st), "resolve");
 	}
+	return deepest;
 }
 
 void RenderCore::Render(const ViewPyramid& view, const Convergence converge, bool async)
@@ -778,13 +852,17 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 		camSeed = seedSetting;
 	}
 	if (converge == Converge) firstConvergingFrame = false;
-	const uint32_t R0gen = xor32_host(camSeed);
+	uint32_t R0gen = 0;
+	if (estimator == ESTIMATOR_OPTIX7) xor32_host(shiftSeed);   /* once per frame, rendercore_optix7/rendercore.cpp:855 */
+	else R0gen = xor32_host(camSeed);
 	FrameSetup f = frameParams(view, scrw, scrh, scrspp, maxDepth, R0gen, samplesTaken);
+	f.camSeed = camSeed;
 	f.P.probePixel = probePos.x + scrw * probePos.y;
 	f.P.probe = wf.probe();
 	float timing[5] = { 0, 0, 0, 0, 0 };
-	runSamples(f, 0, scrspp, Wavefront::PASS_CAP, wf, wf.acc32(), wf.acc64(), counters.as<uint32_t>() + TRACE_QUEUE_WORDS,
+	const int waves = runSamples(f, 0, scrspp, Wavefront::PASS_CAP, wf, wf.acc32(), wf.acc64(), counters.as<uint32_t>() + TRACE_QUEUE_WORDS,
 		overflow.as<uint32_t>(), stream, ev, timing);
+	if (estimator == ESTIMATOR_OPTIX7) for (int k = 0; k < waves; k++) xor32_host(camSeed);
 	const int pixels = scrw * scrh;
 	samplesTaken += scrspp;
 	hipCheck(launch_finalize(wf.acc32(), pixels, 1.0f / (float)samplesTaken, wf.image(), stream), "finalize");

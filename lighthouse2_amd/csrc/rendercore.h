@@ -44,7 +44,7 @@ struct DevBuf
 struct Wavefront
 {
 	static constexpr size_t PASS_CAP = (size_t)1 << 24;
-	DevBuf rays[2], state[2], pathIdB[2], hitsB, shadowRaysB, shadowPotB, occludedB, pathAccB, countersB, countLogB, probeB;
+	DevBuf rays[2], state[2], pathIdB[2], auxB[2], hitsB, shadowRaysB, shadowPotB, occludedB, pathAccB, countersB, countLogB, probeB;
 	DevBuf acc32B, acc64B, imageB;
 	size_t pathCap = 0;
 	int pixelCap = 0;
@@ -83,7 +83,12 @@ struct FrameSetup
 	ShadeParams P;
 	uint32_t R0gen = 0;
 	int samplesTaken = 0, spp = 1, maxDepth = 1, pixels = 0;
+	/* ESTIMATOR_OPTIX7: per-bounce shade seeds R0 = xor32^k(camSeed) + k * 91771 */
+	int estimator = 0;
+	uint32_t camSeed = 0;
 };
+
+enum { ESTIMATOR_REFERENCE = 0, ESTIMATOR_OPTIX7 = 1 };
 
 class RenderCore : public lighthouse2::CoreAPI_Base
 {
@@ -106,6 +111,11 @@ public:
 		const lighthouse2::CoreSpotLight* spotLights, const int spotLightCount,
 		const lighthouse2::CoreDirectionalLight* directionalLights, const int directionalLightCount) override;
 	void SetSkyData(const lighthouse2::float3* pixels, const uint32_t width, const uint32_t height, const lighthouse2::mat4& worldToLight) override;
+	/* Blue-noise tables of the optix7 estimator, 8 bits per entry: 65536
+	 * sequence values, 131072 scrambling and 131072 ranking entries (the
+	 * conversion of rendercore_optix7/rendercore.cpp:246-255). Without this
+	 * call a seeded generator fills them. */
+	void SetBlueNoise(const uint8_t* sequence, const uint8_t* scrambling, const uint8_t* ranking);
 	void SetGeometry(const int meshIdx, const lighthouse2::float4* vertexData, const int vertexCount, const int triangleCount, const lighthouse2::CoreTri* triangles) override;
 	void SetInstance(const int instanceIdx, const int modelIdx, const lighthouse2::mat4& transform) override;
 	void FinalizeInstances() override;
@@ -125,8 +135,10 @@ public:
 	/* Render building blocks, shared with the job server (server.cpp). */
 	void checkRenderable();
 	FrameSetup frameParams(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int depth, uint32_t R0gen, int samplesTakenAtStart);
-	void runSamples(const FrameSetup& f, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
+	/* Returns the number of shade waves the deepest pass ran. */
+	int runSamples(const FrameSetup& f, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
 		uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing);
+	int estimatorSetting() const { return estimator; }
 	bool sceneIsReady() const { return sceneReady; }
 	int traceBlocks() const { return persistentBlocks; }
 	float epsilon() const { return geometryEpsilon; }
@@ -156,6 +168,9 @@ private:
 	float geometryEpsilon = 1.0e-3f, clampValue = 10.0f;
 	int maxDepth = 64, nodeFormat = FMT_Q8, traceLoop = LOOP_MERGED, triBias = 16, refillLanes = 16, chunk = 32;
 	uint32_t camSeed = 0x12345678u, seedSetting = 0x12345678u;
+	int estimator = ESTIMATOR_REFERENCE;
+	uint32_t shiftSeed = 0x11331445u;  /* blue-noise shift of the optix7 estimator (rendercore_optix7/rendercore.h:123) */
+	DevBuf blueNoise;
 	/* render state */
 	int scrw = 0, scrh = 0, scrspp = 1;
 	int samplesTaken = 0;

@@ -53,6 +53,11 @@ struct ShadeParams
 	const float4* sky;
 	int skyW, skyH;
 	float4 w2s0, w2s1, w2s2;
+	/* optix7 estimator: blue-noise tables and shift, firefly clamp, NEE on/off */
+	const uint32_t* blueNoise;
+	uint32_t shift;
+	float clampValue;
+	int connections;
 };
 
 /* Path-state buffers of one pass (SoA). */
@@ -67,14 +72,31 @@ struct PathBuffers
 	uint32_t* occluded;
 	float4* pathAcc;       /* per path slot, summed in path order */
 	uint32_t* counters;    /* [0] extension count in, [1] extension out, [2] shadow, [3] trace work counter, [4] second trace counter */
+	float2* aux[2];        /* optix7 estimator: postponed bsdf pdf, packed normal of the last vertex */
 };
 
 hipError_t launch_generate(const CamParams& c, uint32_t R0, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s);
-hipError_t launch_connect(const PathBuffers& b, int maxCount, hipStream_t s);
+/* Adds each unoccluded connection to its path slot; countW is added to w
+ * (optix7's connections count 1 there, rendercore_optix7/optix/.optix.cu:153). */
+hipError_t launch_connect(const PathBuffers& b, int maxCount, hipStream_t s, float countW = 0.0f);
 hipError_t launch_resolve(const float4* pathAcc, int pixels, int samples, float4* acc32, double* acc64, hipStream_t s);
 hipError_t launch_finalize(const float4* acc32, int pixels, float scale, float4* image, hipStream_t s);
 hipError_t launch_set_u32(uint32_t* p, int n, uint32_t v, hipStream_t s);
+/* optix7 estimator (lib/rendercore_optix7): eye rays with blue-noise pixel and
+ * lens samples (optix/.optix.cu:85-129) and the MIS shade kernel
+ * (kernels/pathtracer.h:54-238) with the Lambert BSDF (sharedBSDFs/lambert.h). */
+hipError_t launch_generate7(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s);
+hipError_t launch_shade7(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s);
+/* Blue-noise tables in the layout of Heitz et al.'s sampler: [0, 65536)
+ * sequence values dimension + 256 * index, [65536, 196608) scrambling tile
+ * and [196608, 327680) ranking tile, 8 entries per texel of a 128 x 128 tile,
+ * plus 16 zero entries of padding. Filled by a seeded generator. */
+void blue_noise_tables(uint32_t* out, uint32_t seed);
+static const int BLUE_NOISE_WORDS = 65536 * 5 + 16;
+static const uint32_t BLUE_NOISE_SEED = 0x2545f491u;
+/* MAXPATHLENGTH of rendercore_optix7 (core_settings.h:25) */
+static const int MAXPATHLENGTH_OPTIX7 = 3;
 /* counters[0] = counters[1] (next wave size, also logged), counters[1] = counters[2] = 0. */
 hipError_t launch_advance(uint32_t* counters, uint32_t* logSlot, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t s);

@@ -113,7 +113,8 @@ __device__ __forceinline__ float h2f(uint32_t bits16) { return half_to_float(bit
  * TRI_LOD + log2(coneWidth / |D.N|); alpha below 0.5 ends the lookup with the
  * untextured colour), detail diffuse, normal maps (tangent frame T, B of the
  * triangle) and roughness; the normal-map scale's __expf is cw_expf. */
-__device__ void shading_data(const ShadeParams& P, v3 D, float u, float v, float coneWidth, const float4* tri, int instIdx,
+template <bool PRIME>
+__device__ bool shading_data(const ShadeParams& P, v3 D, float u, float v, float coneWidth, const float4* tri, int instIdx,
 	Shading& sd, v3& N, v3& iN, v3& fN, v3& T)
 {
 	const float4 t1 = tri[1], t2 = tri[2], t3 = tri[3], t4 = tri[4], t5 = tri[5], t7 = tri[7];
@@ -125,27 +126,31 @@ __device__ void shading_data(const ShadeParams& P, v3 D, float u, float v, float
 	N = iN = fN = mk3(t2.w, t3.w, t4.w);
 	T = xyz(t5);
 	const float w = 1 - (u + v);
-	if (flags & MF_HASSMOOTHNORMALS) iN = normalize(u * xyz(t2) + v * xyz(t3) + w * xyz(t4));
+	if (flags & MF_HASSMOOTHNORMALS)
+		iN = PRIME ? normalize(u * xyz(t2) + v * xyz(t3) + w * xyz(t4)) : normalize(w * xyz(t2) + u * xyz(t3) + v * xyz(t4));
 	const ShadeInst& I = P.inst[instIdx];
 	const v3 A = xyz(I.inv0), B = xyz(I.inv1), C = xyz(I.inv2);
 	N = normalize(N.x * A + N.y * B + N.z * C);
 	iN = normalize(iN.x * A + iN.y * B + iN.z * C);
-	const bool backSide = dot(D, N) > 0;
-	const float alpha = u * t7.x + v * t7.y + w * t7.z;
-	iN = (backSide ? -1.0f : 1.0f) * consistent_normal(D * -1.0f, backSide ? (iN * -1.0f) : iN, alpha);
+	if (PRIME)
+	{	/* CONSISTENTNORMALS, on in RenderCore_PrimeRef only */
+		const bool backSide = dot(D, N) > 0;
+		const float alpha = u * t7.x + v * t7.y + w * t7.z;
+		iN = (backSide ? -1.0f : 1.0f) * consistent_normal(D * -1.0f, backSide ? (iN * -1.0f) : iN, alpha);
+	}
 	fN = iN;
-	if (!(flags & (MF_HASDIFFUSEMAP | MF_HAS2NDDIFFUSEMAP | MF_HASSPECULARITYMAP | MF_HASNORMALMAP | MF_HAS2NDNORMALMAP | MF_HASROUGHNESSMAP))) return;
+	if (!(flags & (MF_HASDIFFUSEMAP | MF_HAS2NDDIFFUSEMAP | MF_HASSPECULARITYMAP | MF_HASNORMALMAP | MF_HAS2NDNORMALMAP | MF_HASROUGHNESSMAP))) return false;
 	const float4 t0 = tri[0];
 	const float ww = 1 - (u + v);
-	const float tu = u * t0.x + v * t0.y + ww * t0.z;
-	const float tv = u * t1.x + v * t1.y + ww * t1.z;
+	const float tu = PRIME ? u * t0.x + v * t0.y + ww * t0.z : ww * t0.x + u * t0.y + v * t0.z;
+	const float tv = PRIME ? u * t1.x + v * t1.y + ww * t1.z : ww * t1.x + u * t1.y + v * t1.z;
 	if (flags & MF_HASDIFFUSEMAP)
 	{
 		const float lambda = t7.w + lh_log2f(coneWidth * (1.0f / fabsf(dot(D, N))));
 		const uint4 m = mat.t0;
 		const float x = h2f(m.y) * (h2f(m.z) + tu), y = h2f(m.y >> 16) * (h2f(m.z >> 16) + tv);
 		const float4 texel = fetch_texel_trilinear(P, lambda, x, y, (int)m.w, (int)(m.x & 0xffffu), (int)(m.x >> 16));
-		if (texel.w < 0.5f) return;
+		if (texel.w < 0.5f) return true;
 		sd.color = sd.color * xyz(texel);
 		if (flags & MF_HAS2NDDIFFUSEMAP)
 		{
@@ -188,6 +193,7 @@ __device__ void shading_data(const ShadeParams& P, v3 D, float u, float v, float
 		sd.params.x = (sd.params.x & 0x00ffffffu) + ((uint32_t)f2i_sat(tx.y * 255.0f) << 24);
 		sd.params.x = (sd.params.x & 0xffffff00u) + (uint32_t)f2i_sat(tx.x * 255.0f);
 	}
+	return false;
 }
 
 __device__ v3 random_barycentrics(float r0)
@@ -318,8 +324,12 @@ __device__ v3 random_point_on_light(const ShadeParams& P, float r0, float r1, v3
 
 __device__ __forceinline__ v3 reflect(v3 i, v3 n) { return i - 2.0f * n * dot(n, i); }
 
+/* Fr_L; the shared Lambert BSDF of the optix7 core (sharedBSDFs/lambert.h:32-45)
+ * also takes a view vector below the surface (inverted eta, |VDotN|). */
+template <bool BELOW = false>
 __device__ float fr_l(float VDotN, float eio)
 {
+	if (BELOW && VDotN < 0.0f) eio = 1.0f / eio, VDotN = fabsf(VDotN);
 	const float SinThetaT2 = (eio * eio) * (1.0f - VDotN * VDotN);
 	if (SinThetaT2 > 1.0f) return 1.0f;
 	const float LDotN = sqrtf(1.0f - SinThetaT2);
@@ -346,6 +356,7 @@ __device__ __forceinline__ v3 evaluate_bsdf(const Shading& sd, v3 iN, v3 wi, flo
 	return sd.color * INVPI_F * roughness(sd);
 }
 
+template <bool SHARED = false>
 __device__ v3 sample_bsdf(const Shading& sd, v3 iN, v3 N, v3 wo, float distance, float r3, float r4, v3& wi, float& pdf, bool& specular)
 {
 	const float flip = (dot(wo, N) < 0) ? -1 : 1;
@@ -355,7 +366,7 @@ __device__ v3 sample_bsdf(const Shading& sd, v3 iN, v3 N, v3 wo, float distance,
 	const float TR = transmission(sd), RO = roughness(sd);
 	if (r4 < TR)
 	{
-		const float eio = flip < 0 ? (1.0f / eta(sd)) : eta(sd), F = fr_l(dot(iN, wo), eio);
+		const float eio = flip < 0 ? (1.0f / eta(sd)) : eta(sd), F = fr_l<SHARED>(dot(iN, wo), eio);
 		v3 beer = mk3(1, 1, 1);
 		beer.x = cw_expf(-sd.transmittance.x * distance * 2.0f);
 		beer.y = cw_expf(-sd.transmittance.y * distance * 2.0f);
@@ -454,7 +465,7 @@ __global__ void __launch_bounds__(128) k_shade(const ShadeParams P, const float4
 	Shading sd;
 	v3 N, iN, fN, T;
 	const v3 I = xyz(O4) + t * D;
-	shading_data(P, D, u, v, P.spreadAngle * t, tri, instIdx, sd, N, iN, fN, T);
+	shading_data<true>(P, D, u, v, P.spreadAngle * t, tri, instIdx, sd, N, iN, fN, T);
 	if (sd.color.x > 1.0f || sd.color.y > 1.0f || sd.color.z > 1.0f)
 	{
 		const float DdotNL = -dot(D, N);
@@ -509,13 +520,16 @@ __global__ void __launch_bounds__(128) k_shade(const ShadeParams P, const float4
 }
 
 __global__ void k_connect(const float4* __restrict__ shadowPot, const uint32_t* __restrict__ occluded, float4* __restrict__ pathAcc,
-	const uint32_t* __restrict__ counters)
+	const uint32_t* __restrict__ counters, float countW)
 {
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= (int)counters[2]) return;
 	if (occluded[j]) return;
 	const float4 p = shadowPot[j];
-	acc_add(pathAcc + fbits(p.w), xyz(p));
+	float4* a = pathAcc + fbits(p.w);
+	float4 v = *a;
+	v.x = v.x + p.x, v.y = v.y + p.y, v.z = v.z + p.z, v.w = v.w + countW;
+	*a = v;
 }
 
 /* Sample-ordered reduction of the path slots into the pixel accumulators. */
@@ -564,6 +578,319 @@ hipError_t launch_advance(uint32_t* counters, uint32_t* logSlot, hipStream_t s)
 	return hipGetLastError();
 }
 
+/* ==== optix7 estimator ======================================================
+ * lib/rendercore_optix7: eye rays (optix/.optix.cu:85-129), shadeKernel
+ * (kernels/pathtracer.h:54-238) with the Lambert BSDF variant of the core
+ * (kernels/bsdf.h selects sharedBSDFs/lambert.h; BSDF_HAS_PURE_SPECULARS),
+ * MAXPATHLENGTH 3, ENOUGH_BOUNCES = S_BOUNCED, CLAMPFIREFLIES and BLUENOISE on
+ * (core_settings.h:24-33,185-197). Path state: state = (throughput, flags),
+ * aux = (postponed bsdf pdf, packed normal of the last vertex). */
+#define S7_SPECULAR 1u
+#define S7_BOUNCED 2u
+#define S7_VIASPECULAR 4u
+#define S7_BOUNCEDTWICE 8u
+
+/* blueNoiseSampler4 (tools_shared.h:323-337) */
+__device__ __forceinline__ float4 blue_noise4(const uint32_t* bn, int x, int y, int sampleIndex, int dim)
+{
+	const uint32_t* rk = bn + dim + (x + y * 128) * 8 + 65536 * 3;
+	const int rsi1 = (int)((sampleIndex ^ rk[0]) & 255u), rsi2 = (int)((sampleIndex ^ rk[1]) & 255u);
+	const int rsi3 = (int)((sampleIndex ^ rk[2]) & 255u), rsi4 = (int)((sampleIndex ^ rk[3]) & 255u);
+	const uint32_t v1 = bn[dim + 0 + rsi1 * 256], v2 = bn[dim + 1 + rsi2 * 256], v3 = bn[dim + 2 + rsi3 * 256], v4 = bn[dim + 3 + rsi4 * 256];
+	const uint32_t* sc = bn + (dim & 7) + (x + y * 128) * 8 + 65536;
+	return make_float4((0.5f + (float)(v1 ^ sc[0])) * (1.0f / 256.0f), (0.5f + (float)(v2 ^ sc[1])) * (1.0f / 256.0f),
+		(0.5f + (float)(v3 ^ sc[2])) * (1.0f / 256.0f), (0.5f + (float)(v4 ^ sc[3])) * (1.0f / 256.0f));
+}
+
+/* SampleSmallSkydome (tools_shared.h:204-211): the 64x downsampled dome stored after the full one */
+__device__ v3 sample_small_sky(const ShadeParams& P, v3 D)
+{
+	if (P.skyW <= 0 || P.skyH <= 0) return mk3(0, 0, 0);
+	const float4 a = P.w2s0, b = P.w2s1, c = P.w2s2;
+	const v3 d = mk3(-(a.x * D.x + a.y * D.y + a.z * D.z), -(b.x * D.x + b.y * D.y + b.z * D.z), -(c.x * D.x + c.y * D.y + c.z * D.z));
+	const uint32_t w = (uint32_t)P.skyW >> 6, h = (uint32_t)P.skyH >> 6;
+	const float p = lh_atan2f(d.y, d.x);
+	const float phi = p < 0 ? (p + 2 * PI_F) : p;
+	const float theta = lh_acosf(fminf(fmaxf(d.z, -1.0f), 1.0f));
+	const uint32_t u = f2u_sat((float)w * phi * 0.15915494309189533577f - 0.5f);
+	const uint32_t v = f2u_sat((float)h * theta * 0.31830988618379067154f - 0.5f);
+	const uint32_t idx = u + v * w;
+	if (idx >= w * h) return mk3(0, 0, 0);
+	return xyz(P.sky[idx + (uint32_t)P.skyW * (uint32_t)P.skyH]);
+}
+
+/* LightPickProb with ISLIGHTS (lights_shared.h:174-190): potential of light idx
+ * seen from the previous vertex O with normal N, the tri light's point being I. */
+__device__ float light_pick_prob(const ShadeParams& P, int idx, v3 O, v3 N, v3 I)
+{
+	const int count = P.nTri + P.nPoint + P.nSpot + P.nDir;
+	float sum = 0, mine = 0;
+	for (int i = 0; i < count; i++)
+	{
+		float c;
+		if (i < P.nTri)
+		{
+			const float4* L = P.triLights + 6 * i;
+			v3 Lv = I - O;
+			const float att = 1.0f / dot(Lv, Lv);
+			Lv = normalize(Lv);
+			const float LNdotL = fmaxf(0.0f, -dot(xyz(L[1]), Lv));
+			const float NdotL = fmaxf(0.0f, dot(N, Lv));
+			c = L[0].w * LNdotL * NdotL * att;
+		}
+		else c = potential(P, i, O, N, mk3(-1, -1, -1));
+		sum += c;
+		if (i == idx) mine = c;
+	}
+	if (sum <= 0) return 0;
+	return mine / sum;
+}
+
+/* UnpackNormal / PackNormal (tools_shared.h:101-120) */
+__device__ __forceinline__ v3 unpack_normal(uint32_t p)
+{
+	float nx = (float)(p & 65535u) * (2.0f / 65535.0f), ny = (float)(p >> 16) * (2.0f / 65535.0f), nz = 0, nw = 0;
+	nx += -1, ny += -1, nz += 1, nw += -1;
+	float l = nx * -nx + ny * -ny + nz * -nw;
+	nz = l, l = sqrtf(l), nx *= l, ny *= l;
+	return mk3(nx * 2.0f + 0, ny * 2.0f + 0, nz * 2.0f + -1);
+}
+__device__ __forceinline__ uint32_t pack_normal(v3 N)
+{
+	const float f = 65535.0f / fmaxf(sqrtf(8.0f * N.z + 8.0f), 0.0001f);
+	return f2u_sat(N.x * f + 32767.0f) + (f2u_sat(N.y * f + 32767.0f) << 16);
+}
+
+__device__ __forceinline__ v3 clamp_intensity(v3 c, float clampValue)
+{
+	const float v = fmaxf(c.x, fmaxf(c.y, c.z));
+	if (v > clampValue) { const float m = clampValue / v; c.x *= m, c.y *= m, c.z *= m; }
+	return c;
+}
+__device__ __forceinline__ v3 fixnan(v3 c)
+{	/* FIXNAN_FLOAT3: a non-finite sum zeroes the vector */
+	return __builtin_isfinite(c.x + c.y + c.z) ? c : mk3(0, 0, 0);
+}
+
+__global__ void k_generate7(CamParams c, ShadeParams P, uint32_t jobBase, int n, float eps, float4* __restrict__ rays,
+	float4* __restrict__ state, uint32_t* __restrict__ pathId, float2* __restrict__ aux, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j == 0) counters[0] = (uint32_t)n, counters[1] = 0, counters[2] = 0;
+	if (j >= n) return;
+	const uint32_t pathIdx = jobBase + (uint32_t)j;
+	const uint32_t wh = (uint32_t)(c.w * c.h);
+	const uint32_t pixelIdx = pathIdx % wh, sampleIdx = pathIdx / wh + (uint32_t)P.pass;
+	uint32_t seed = wang_hash(pathIdx * 16789u + (uint32_t)P.pass * 1791u);
+	const int sx = (int)(pixelIdx % (uint32_t)c.w), sy = (int)(pixelIdx / (uint32_t)c.w);
+	float4 r4;
+	if (sampleIdx < 64)
+		r4 = blue_noise4(P.blueNoise, (sx + (int)(P.shift & 127u)) & 127, (sy + (int)(P.shift >> 24)) & 127, (int)sampleIdx, 0);
+	else
+	{
+		r4.x = rand_float(seed), r4.y = rand_float(seed);
+		r4.z = rand_float(seed), r4.w = rand_float(seed);
+	}
+	const v3 pos = mk3(c.pos[0], c.pos[1], c.pos[2]), p1 = mk3(c.p1[0], c.p1[1], c.p1[2]);
+	const v3 right = mk3(c.p2[0], c.p2[1], c.p2[2]) - p1, up = mk3(c.p3[0], c.p3[1], c.p3[2]) - p1;
+	const v3 O = point_on_lens(r4.x, r4.z, pos, c.aperture, right, up);
+	v3 target;
+	if (c.distortion == 0)
+	{	/* RayTarget (common_functions.h:29-50) */
+		const float u = ((float)sx + r4.y) * (1.0f / (float)c.w), v = ((float)sy + r4.w) * (1.0f / (float)c.h);
+		target = p1 + u * right + v * up;
+	}
+	else
+	{
+		const float tx = (float)sx / (float)c.w - 0.5f, ty = (float)sy / (float)c.h - 0.5f;
+		const float rr = tx * tx + ty * ty;
+		const float rq = sqrtf(rr) * (1.0f + c.distortion * rr + c.distortion * rr * rr);
+		const float theta = lh_atan2f(tx, ty);
+		float st, ct;
+		cw_sincos(theta, &st, &ct);
+		const float bx = (st * rq + 0.5f) * (float)c.w, by = (ct * rq + 0.5f) * (float)c.h;
+		target = p1 + (bx + r4.y) * (right / (float)c.w) + (by + r4.w) * (up / (float)c.h);
+	}
+	const v3 D = normalize(target - O);
+	rays[2 * j] = make_float4(O.x, O.y, O.z, eps);
+	rays[2 * j + 1] = make_float4(D.x, D.y, D.z, 1e34f);
+	state[j] = make_float4(1.0f, 1.0f, 1.0f, bitsf(S7_SPECULAR));
+	aux[j] = make_float2(1.0f, bitsf(0u));
+	pathId[j] = pathIdx;
+	pathAcc[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ void __launch_bounds__(128) k_shade7(const ShadeParams P, const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
+	const float2* __restrict__ auxIn, const uint32_t* __restrict__ pathIn, const float4* __restrict__ hits, float4* __restrict__ raysOut,
+	float4* __restrict__ stateOut, float2* __restrict__ auxOut, uint32_t* __restrict__ pathOut, float4* __restrict__ shadowRays,
+	float4* __restrict__ shadowPot, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= (int)counters[0]) return;
+	const float4 O4 = raysIn[2 * j], D4 = raysIn[2 * j + 1], T4 = stateIn[j];
+	const float2 A2 = auxIn[j];
+	const float4 hd = hits[j];
+	const uint32_t pathIdx = pathIn[j];
+	const uint32_t slot = pathIdx - P.jobBase;
+	float4* acc = pathAcc + slot;
+	const uint32_t wh = (uint32_t)(P.w * P.h);
+	const uint32_t pixelIdx = pathIdx % wh;
+	const uint32_t sampleIdx = pathIdx / wh + (uint32_t)P.pass;
+	uint32_t flags = fbits(T4.w);
+	const v3 D = xyz(D4);
+	v3 throughput = xyz(T4);
+	const float bsdfPdf = A2.x;
+	const uint32_t prim = fbits(hd.z);
+	const int instIdx = (int)fbits(hd.y);
+	const float t = hd.w;
+	if (prim == 0xffffffffu)
+	{
+		const v3 sky = (flags & S7_BOUNCED) ? sample_small_sky(P, D) : sample_sky(P, D);
+		v3 c = throughput * sky * (1.0f / bsdfPdf);
+		c = fixnan(clamp_intensity(c, P.clampValue));
+		acc_add(acc, c);
+		return;
+	}
+	if (P.probe && (int)pixelIdx == P.probePixel && P.pathLength == 1)
+		P.probe[0] = instIdx, P.probe[1] = (int)prim, P.probe[2] = (int)fbits(t);
+	const uint32_t uvb = fbits(hd.x);
+	const float u = (float)(uvb & 65535u) * (1.0f / 65535.0f), v = (float)(uvb >> 16) * (1.0f / 65535.0f);
+	const ShadeInst& SI = P.inst[instIdx];
+	const float4* tri = SI.coreTris + 13 * (size_t)prim;
+	Shading sd;
+	v3 N, iN, fN, T;
+	const v3 I = xyz(O4) + t * D;
+	/* the hit record holds the weights of vertices 0 and 1; optix7's holds those of 1 and 2 */
+	const bool alphaHit = shading_data<false>(P, D, v, 1.0f - (u + v), P.spreadAngle * t, tri, instIdx, sd, N, iN, fN, T);
+	uint32_t seed = wang_hash(pathIdx * 17u + P.R0);
+	if (alphaHit)
+	{	/* continue through the cut-out, same path state */
+		if (P.pathLength < MAXPATHLENGTH_OPTIX7)
+		{
+			const uint32_t k = atomicAdd(&counters[1], 1u);
+			const v3 o = I + D * P.eps;
+			raysOut[2 * k] = make_float4(o.x, o.y, o.z, P.eps);
+			raysOut[2 * k + 1] = make_float4(D.x, D.y, D.z, 1e34f);
+			float4 st = T4;
+			if (!__builtin_isfinite(T4.x + T4.y + T4.z)) st = make_float4(0, 0, 0, T4.w);
+			stateOut[k] = st;
+			auxOut[k] = A2;
+			pathOut[k] = pathIdx;
+		}
+		return;
+	}
+	if (sd.color.x > 1.0f || sd.color.y > 1.0f || sd.color.z > 1.0f)
+	{
+		const float DdotNL = -dot(D, N);
+		v3 c = mk3(0, 0, 0);
+		if (DdotNL > 0)
+		{
+			if (P.pathLength == 1 || (flags & S7_SPECULAR) || !P.connections) c = sd.color;
+			else
+			{	/* MIS with the light sample of the previous vertex */
+				const v3 lastN = unpack_normal(fbits(A2.y));
+				const float area = tri[5].w;
+				const int ltri = (int)fbits(tri[0].w);
+				const float lightPdf = (t * t) / (fabsf(dot(D, N)) * area);
+				const float pickProb = light_pick_prob(P, ltri, xyz(O4), lastN, I);
+				if ((bsdfPdf + lightPdf * pickProb) > 0) c = throughput * sd.color * (1.0f / (bsdfPdf + lightPdf * pickProb));
+			}
+			c = fixnan(clamp_intensity(c, P.clampValue));
+			acc_add(acc, c);
+		}
+		return;
+	}
+	if (flags & S7_BOUNCED) sd.params.x |= 255u << 24;   /* path regularisation: roughness 1 after a bounce */
+	if (roughness(sd) <= 0.001f || transmission(sd) > 0.5f) flags |= S7_SPECULAR; else flags &= ~S7_SPECULAR;
+	const float faceDir = (dot(D, N) > 0) ? -1 : 1;
+	if (faceDir == 1) sd.transmittance = mk3(0, 0, 0);
+	throughput = throughput * (1.0f / bsdfPdf);
+	float4 r4;
+	if (sampleIdx < 64)
+	{
+		const int x = (int)(((pathIdx % (uint32_t)P.w) + (P.shift & 127u)) & 127u);
+		const int y = (int)(((pathIdx / (uint32_t)P.w) + (P.shift >> 24)) & 127u);
+		r4 = blue_noise4(P.blueNoise, x, y, (int)sampleIdx, 4 * P.pathLength - 4);
+	}
+	else
+	{
+		r4.x = rand_float(seed), r4.y = rand_float(seed);
+		r4.z = rand_float(seed), r4.w = rand_float(seed);
+	}
+	if ((flags & S7_SPECULAR) == 0 && P.connections)
+	{
+		float pickProb = 0, lightPdf = 0;
+		v3 lightColor = mk3(0, 0, 0);
+		v3 L = random_point_on_light(P, r4.x, r4.y, I, fN * faceDir, pickProb, lightPdf, lightColor) - I;
+		const float dist = length(L);
+		L = L * (1.0f / dist);
+		const float NdotL = dot(L, fN * faceDir);
+		if (NdotL > 0 && lightPdf > 0)
+		{
+			float pdf;
+			const v3 sampled = evaluate_bsdf(sd, fN, L, pdf) ;   /* Lambert: colour / pi, times ROUGHNESS (pure speculars) */
+			if (pdf > 0)
+			{
+				v3 c = throughput * sampled * lightColor * (NdotL / (pickProb * lightPdf + pdf));
+				c = clamp_intensity(fixnan(c), P.clampValue);
+				const uint32_t k = atomicAdd(&counters[2], 1u);
+				const v3 so = safe_origin(I, L, N, P.eps);
+				shadowRays[2 * k] = make_float4(so.x, so.y, so.z, P.eps);
+				shadowRays[2 * k + 1] = make_float4(L.x, L.y, L.z, dist - 2 * P.eps);
+				shadowPot[k] = make_float4(c.x, c.y, c.z, bitsf(slot));
+			}
+		}
+	}
+	if ((flags & S7_BOUNCED) || P.pathLength == MAXPATHLENGTH_OPTIX7) return;
+	v3 R;
+	float newBsdfPdf;
+	bool specular = false;
+	const v3 bsdf = sample_bsdf<true>(sd, fN, N, D * -1.0f, t, r4.z, r4.w, R, newBsdfPdf, specular);
+	(void)rand_float(seed);                         /* SampleBSDF's third random number (unused by Lambert) */
+	if (newBsdfPdf < EPSILON_F || __builtin_isnan(newBsdfPdf)) return;
+	if (specular) flags |= S7_SPECULAR;
+	const float p = ((flags & S7_SPECULAR) || ((flags & S7_BOUNCED) == 0)) ? 1 : fminf(1.0f, fmaxf(fmaxf(bsdf.x, bsdf.y), bsdf.z));
+	if (p < rand_float(seed)) return;
+	throughput = throughput * (1 / p);
+	const uint32_t k = atomicAdd(&counters[1], 1u);
+	const uint32_t packedNormal = pack_normal(fN * faceDir);
+	if (!(flags & S7_SPECULAR)) flags |= (flags & S7_BOUNCED) ? S7_BOUNCEDTWICE : S7_BOUNCED; else flags |= S7_VIASPECULAR;
+	const v3 no = safe_origin(I, R, N, P.eps);
+	raysOut[2 * k] = make_float4(no.x, no.y, no.z, P.eps);
+	raysOut[2 * k + 1] = make_float4(R.x, R.y, R.z, 1e34f);
+	const v3 tp = fixnan(throughput) * bsdf * fabsf(dot(fN, R));
+	stateOut[k] = make_float4(tp.x, tp.y, tp.z, bitsf(flags));
+	auxOut[k] = make_float2(newBsdfPdf, bitsf(packedNormal));
+	pathOut[k] = pathIdx;
+}
+
+hipError_t launch_generate7(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s)
+{
+	if (n <= 0) return hipSuccess;
+	k_generate7<<<(n + 255) / 256, 256, 0, s>>>(c, p, jobBase, n, eps, b.rays[out], b.state[out], b.pathId[out], b.aux[out], b.pathAcc, b.counters);
+	return hipGetLastError();
+}
+
+hipError_t launch_shade7(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s)
+{
+	if (maxCount <= 0) return hipSuccess;
+	k_shade7<<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.aux[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
+		b.aux[in ^ 1], b.pathId[in ^ 1], b.shadowRays, b.shadowPot, b.pathAcc, b.counters);
+	return hipGetLastError();
+}
+
+/* Seeded tables in the sampler's layout (xorshift32 bytes); the sampler's
+ * structure is the reference's, the table values are not its optimised ones. */
+void blue_noise_tables(uint32_t* out, uint32_t seed)
+{
+	uint32_t s = seed ? seed : 1u;
+	for (int i = 0; i < BLUE_NOISE_WORDS; i++) out[i] = 0;
+	const auto next = [&]() { s ^= s << 13; s ^= s >> 17; s ^= s << 5; return s; };
+	for (int i = 0; i < 65536; i++) out[i] = next() & 255u;
+	for (int i = 0; i < 128 * 128 * 8; i++) out[65536 + i] = next() & 255u;
+	for (int i = 0; i < 128 * 128 * 8; i++) out[3 * 65536 + i] = next() & 255u;
+}
+
 hipError_t launch_generate(const CamParams& c, uint32_t R0, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s)
 {
 	k_generate<<<(n + 255) / 256, 256, 0, s>>>(c, R0, jobBase, n, eps, b.rays[out], b.state[out], b.pathId[out], b.pathAcc, b.counters);
@@ -578,10 +905,10 @@ hipError_t launch_shade(const ShadeParams& p, const PathBuffers& b, int in, int 
 	return hipGetLastError();
 }
 
-hipError_t launch_connect(const PathBuffers& b, int maxCount, hipStream_t s)
+hipError_t launch_connect(const PathBuffers& b, int maxCount, hipStream_t s, float countW)
 {
 	if (maxCount <= 0) return hipSuccess;
-	k_connect<<<(maxCount + 255) / 256, 256, 0, s>>>(b.shadowPot, b.occluded, b.pathAcc, b.counters);
+	k_connect<<<(maxCount + 255) / 256, 256, 0, s>>>(b.shadowPot, b.occluded, b.pathAcc, b.counters, countW);
 	return hipGetLastError();
 }
 

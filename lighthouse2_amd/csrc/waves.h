@@ -9,7 +9,7 @@ namespace lh {
 struct CamParams
 {
 	float pos[3], p1[3], p2[3], p3[3];
-	float aperture, spreadAngle;
+	float aperture, spreadAngle, distortion;
 	int w, h;
 };
 

@@ -36,12 +36,15 @@ def lib():
             "orc_mat4_inverse": (None, [P, P]), "orc_convert_material": (None, [P, P]), "orc_sizeof_omat": (I, []),
             "orc_tri_intersect": (I, [P, P, P, P]),
             "orc_set_textures": (None, [P, P, I]), "orc_set_sky": (None, [P, P, I, I, P]),
-            "orc_sample_sky": (None, [P, P, I, P]), "orc_fetch_texels": (None, [P, I, P, I, I, ctypes.c_float, P]),
+            "orc_sample_sky": (None, [P, P, I, P, I]), "orc_fetch_texels": (None, [P, I, P, I, I, ctypes.c_float, P]),
             "orc_math": (None, [I, P, P, I, P]),
             "orc_svgf_create": (P, [I, I]), "orc_svgf_destroy": (None, [P]),
             "orc_svgf_frame": (None, [P, P, P, P, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, I,
                                       ctypes.c_float, ctypes.c_float, I, P]),
             "orc_svgf_read": (None, [P, I, P]),
+            "orc_set_blue_noise": (None, [P, P, P, P, U]),
+            "orc_blue_noise4": (None, [P, P, I, P]), "orc_normals": (None, [I, P, P, I]),
+            "orc_render_frame7": (I, [P, P, I, I, I, I, ctypes.c_float, ctypes.c_float, U, U, P, I]),
         }
         for n, (r, a) in sigs.items():
             f = getattr(L, n)
@@ -106,10 +109,11 @@ class OracleScene:
         t = None if T is None else np.ascontiguousarray(T, np.float32).reshape(16)
         lib().orc_set_sky(self.h, _p(px), px.shape[1], px.shape[0], _p(t))
 
-    def sample_sky(self, dirs):
+    def sample_sky(self, dirs, small=False):
+        """SampleSkydome, or SampleSmallSkydome (the 64x64-averaged dome) with small=True."""
         d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
         out = np.zeros_like(d)
-        lib().orc_sample_sky(self.h, _p(d), len(d), _p(out))
+        lib().orc_sample_sky(self.h, _p(d), len(d), _p(out), 1 if small else 0)
         return out
 
     def fetch_texels(self, tex_id, uv, lam=None):
@@ -143,6 +147,33 @@ class OracleScene:
         v = np.ascontiguousarray(view)
         lib().orc_render_frame(self.h, _p(v), w, h, spp, max_depth, eps, R0gen, pass_, _p(acc), threads)
         return acc
+
+    def set_blue_noise(self, tables=None, seed=0x2545F491):
+        """tables: (sequence[65536], scrambling[131072], ranking[131072]) uint8,
+        or None for the core's seeded default tables."""
+        if tables is None:
+            lib().orc_set_blue_noise(self.h, None, None, None, seed)
+            return
+        t = [np.ascontiguousarray(x, np.uint8) for x in tables]
+        assert t[0].size == 65536 and t[1].size == 131072 and t[2].size == 131072
+        lib().orc_set_blue_noise(self.h, _p(t[0]), _p(t[1]), _p(t[2]), 0)
+
+    def blue_noise4(self, xysd):
+        """blueNoiseSampler4 at rows (x, y, sampleIndex, dimension)."""
+        q = np.ascontiguousarray(xysd, np.int32).reshape(-1, 4)
+        out = np.zeros((len(q), 4), np.float32)
+        lib().orc_blue_noise4(self.h, _p(q), len(q), _p(out))
+        return out
+
+    def render_frame7(self, view, w, h, spp, pass_, eps, clamp_value, shift, cam_seed, threads=8):
+        """One optix7-estimator frame: per-path (xyz radiance, w connections) and
+        the number of shade waves it ran."""
+        acc = np.zeros((spp * w * h, 4), np.float32)
+        v = np.ascontiguousarray(view)
+        waves = lib().orc_render_frame7(self.h, _p(v), w, h, spp, pass_, eps, clamp_value, shift, cam_seed, _p(acc), threads)
+        if waves < 0:
+            raise RuntimeError("set_blue_noise first")
+        return acc, waves
 
     def bounce(self, rays, hits, path_ids, R0, tmin):
         r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
@@ -236,6 +267,55 @@ def math(which, a, b=None):
     out = np.zeros_like(a)
     lib().orc_math(k, _p(a), _p(bb), len(a), _p(out))
     return out
+
+
+def pack_normals(n3):
+    """PackNormal of (n, 3) unit vectors -> uint32."""
+    n = np.ascontiguousarray(n3, np.float32).reshape(-1, 3)
+    out = np.zeros(len(n), np.uint32)
+    lib().orc_normals(1, _p(n), _p(out), len(n))
+    return out
+
+
+def unpack_normals(packed):
+    p = np.ascontiguousarray(packed, np.uint32)
+    out = np.zeros((len(p), 3), np.float32)
+    lib().orc_normals(0, _p(out), _p(p), len(p))
+    return out
+
+
+def render7(scene_or_oracle, view, w, h, spp=1, frames=1, eps=1e-3, seed=0x12345678, clamp_value=10.0, threads=8,
+            modes=None, blue_noise=None, shift_seed=0x11331445):
+    """Oracle of a sequence of Render calls with the optix7 estimator
+    (rendercore_optix7/rendercore.cpp:822-905): the blue-noise shift seed
+    advances once per frame and is never reset; a Restart (or the first
+    Converge call after one) clears the accumulator and resets the camera seed;
+    the camera seed advances once per shade wave the frame ran. Returns
+    (acc32 (h,w,4), acc64, samplesTaken, camSeed, shiftSeed)."""
+    o = scene_or_oracle if isinstance(scene_or_oracle, OracleScene) else OracleScene(scene_or_oracle)
+    o.set_blue_noise(blue_noise)
+    if modes is None:
+        modes = [RESTART] + [CONVERGE] * frames
+    acc32 = np.zeros((w * h, 4), np.float32)
+    acc64 = np.zeros((w * h, 4), np.float64)
+    cam, shift, taken, first = seed, shift_seed, 0, False
+    for m in modes:
+        if m == RESTART or first:
+            acc32[:] = 0
+            acc64[:] = 0
+            taken, first, cam = 0, True, seed
+        if m == CONVERGE:
+            first = False
+        shift, _ = xor32_host(shift)
+        pa, waves = o.render_frame7(view, w, h, spp, taken, eps, clamp_value, shift, cam, threads)
+        for _ in range(waves):
+            cam, _ = xor32_host(cam)
+        for s in range(spp):
+            blk = pa[s * w * h:(s + 1) * w * h]
+            acc32 += blk
+            acc64 += blk.astype(np.float64)
+        taken += spp
+    return acc32.reshape(h, w, 4), acc64.reshape(h, w, 4), taken, cam, shift
 
 
 def sincos(x):

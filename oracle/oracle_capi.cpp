@@ -79,17 +79,33 @@ OAPI void orc_set_sky(void* sp, const float* pixels3, int w, int h, const float*
 {
 	RenderScene& s = *(RenderScene*)sp;
 	s.skyW = w, s.skyH = h;
-	s.sky.resize((size_t)w * h);
-	for (size_t i = 0; i < s.sky.size(); i++) s.sky[i] = { pixels3[3 * i], pixels3[3 * i + 1], pixels3[3 * i + 2], 0.0f };
+	const size_t n = (size_t)w * h;
+	const int sw = w >> 6, sh = h >> 6;
+	s.sky.resize(n + (size_t)sw * sh);
+	for (size_t i = 0; i < n; i++) s.sky[i] = { pixels3[3 * i], pixels3[3 * i + 1], pixels3[3 * i + 2], 0.0f };
+	/* 64x64 box average (rendercore_optix7/rendercore.cpp:726-738) */
+	for (int y = 0; y < sh; y++) for (int x = 0; x < sw; x++)
+	{
+		float t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+		const f4* tile = s.sky.data() + (size_t)x * 64 + (size_t)y * 64 * w;
+		for (int v = 0; v < 64; v++) for (int u = 0; u < 64; u++)
+		{
+			const f4 q = tile[u + (size_t)v * w];
+			t0 += q.x, t1 += q.y, t2 += q.z, t3 += q.w;
+		}
+		const float k = 1.0f / (64 * 64);
+		s.sky[n + x + (size_t)y * sw] = { t0 * k, t1 * k, t2 * k, t3 * k };
+	}
 	for (int i = 0; i < 16; i++) s.worldToSky[i] = worldToSky16 ? worldToSky16[i] : (i % 5 == 0 ? 1.0f : 0.0f);
 }
 
-OAPI void orc_sample_sky(void* sp, const float* dirs3, int n, float* out3)
+OAPI void orc_sample_sky(void* sp, const float* dirs3, int n, float* out3, int small)
 {
 	const RenderScene& s = *(const RenderScene*)sp;
 	for (int i = 0; i < n; i++)
 	{
-		const f3 c = sample_sky(s, mk3(dirs3[3 * i], dirs3[3 * i + 1], dirs3[3 * i + 2]));
+		const f3 d = mk3(dirs3[3 * i], dirs3[3 * i + 1], dirs3[3 * i + 2]);
+		const f3 c = small ? sample_sky_small(s, d) : sample_sky(s, d);
 		out3[3 * i] = c.x, out3[3 * i + 1] = c.y, out3[3 * i + 2] = c.z;
 	}
 }
@@ -195,6 +211,39 @@ OAPI void orc_render_frame(void* sp, const ViewPyramid* view, int w, int h, int 
 	RenderParams p;
 	p.view = *view, p.w = w, p.h = h, p.spp = spp, p.maxDepth = maxDepth, p.eps = eps;
 	render_frame(*(RenderScene*)sp, p, R0gen, pass, pathAcc, threads);
+}
+
+/* Blue-noise tables from 8-bit arrays (rendercore_optix7/rendercore.cpp:246-255);
+ * null arrays select the core's seeded default tables. */
+OAPI void orc_set_blue_noise(void* sp, const uint8_t* seq, const uint8_t* scr, const uint8_t* rnk, uint seed)
+{
+	RenderScene& s = *(RenderScene*)sp;
+	s.blueNoise.assign(65536 * 5 + 16, 0u);
+	if (!seq || !scr || !rnk) { blue_noise_default(s.blueNoise.data(), seed); return; }
+	for (int i = 0; i < 65536; i++) s.blueNoise[i] = seq[i];
+	for (int i = 0; i < 128 * 128 * 8; i++) s.blueNoise[65536 + i] = scr[i], s.blueNoise[3 * 65536 + i] = rnk[i];
+}
+
+OAPI void orc_blue_noise4(void* sp, const int* xysd, int n, float* out4)
+{
+	const RenderScene& s = *(const RenderScene*)sp;
+	for (int i = 0; i < n; i++) blue_noise_sample4(s, xysd[4 * i], xysd[4 * i + 1], xysd[4 * i + 2], xysd[4 * i + 3], out4 + 4 * i);
+}
+
+OAPI void orc_normals(int pack, float* n3, uint* packed, int n)
+{
+	for (int i = 0; i < n; i++)
+		if (pack) packed[i] = normal_pack(n3 + 3 * i); else normal_unpack(packed[i], n3 + 3 * i);
+}
+
+OAPI int orc_render_frame7(void* sp, const ViewPyramid* view, int w, int h, int spp, int pass, float eps, float clampValue,
+	uint shift, uint camSeed, float* pathAcc, int threads)
+{
+	RenderScene& s = *(RenderScene*)sp;
+	if (s.blueNoise.empty()) return -1;
+	Render7Params p;
+	p.view = *view, p.w = w, p.h = h, p.spp = spp, p.pass = pass, p.eps = eps, p.clampValue = clampValue, p.shift = shift;
+	return render_frame7(s, p, camSeed, pathAcc, threads);
 }
 
 OAPI void orc_primary_rays(const ViewPyramid* view, int w, int h, uint R0, float tmin, float* rays8)

@@ -175,7 +175,7 @@ static inline float h2f(uint bits16) { return half_to_float((uint16_t)(bits16 & 
  * frame T, B of the triangle, not instance-transformed, as the reference) and
  * roughness. __expf of the normal-map scale is cw_expf. Returns the ALPHA flag. */
 static bool shading_data(const RenderScene& s, f3 D, float u, float v, float coneWidth, const CoreTri& tri, int instIdx,
-	Shading& sd, f3& N, f3& iN, f3& fN, f3& T)
+	Shading& sd, f3& N, f3& iN, f3& fN, f3& T, bool prime = true)
 {
 	const OMat& mat = s.mats[tri.material];
 	sd.color = mk3(half_to_float(mat.diffuse[0]), half_to_float(mat.diffuse[1]), half_to_float(mat.diffuse[2]));
@@ -184,22 +184,28 @@ static bool shading_data(const RenderScene& s, f3 D, float u, float v, float con
 	N = iN = fN = mk3(tri.Nx, tri.Ny, tri.Nz);
 	T = f3of(tri.T);
 	const float w = 1 - (u + v);
-	if (mat.flags & M_HASSMOOTHNORMALS) iN = normalize(u * f3of(tri.vN0) + v * f3of(tri.vN1) + w * f3of(tri.vN2));
+	/* !prime: the non-OPTIXPRIMEBUILD convention (w, u, v) and CONSISTENTNORMALS off,
+	 * as rendercore_optix7 compiles material_shared.h (core_settings.h:26) */
+	if (mat.flags & M_HASSMOOTHNORMALS)
+		iN = prime ? normalize(u * f3of(tri.vN0) + v * f3of(tri.vN1) + w * f3of(tri.vN2)) : normalize(w * f3of(tri.vN0) + u * f3of(tri.vN1) + v * f3of(tri.vN2));
 	const float* inv = s.inst[instIdx].inv;
 	const f3 A = mk3(inv[0], inv[1], inv[2]), B = mk3(inv[4], inv[5], inv[6]), C = mk3(inv[8], inv[9], inv[10]);
 	N = normalize(N.x * A + N.y * B + N.z * C);
 	iN = normalize(iN.x * A + iN.y * B + iN.z * C);
-	const bool backSide = dot(D, N) > 0;
-	const float alpha = u * tri.alpha.x + v * tri.alpha.y + w * tri.alpha.z;
-	iN = (backSide ? -1.0f : 1.0f) * consistent_normal(D * -1.0f, backSide ? (iN * -1.0f) : iN, alpha);
+	if (prime)
+	{
+		const bool backSide = dot(D, N) > 0;
+		const float alpha = u * tri.alpha.x + v * tri.alpha.y + w * tri.alpha.z;
+		iN = (backSide ? -1.0f : 1.0f) * consistent_normal(D * -1.0f, backSide ? (iN * -1.0f) : iN, alpha);
+	}
 	fN = iN;
 	const uint fl = mat.flags;
 	float tu = 0, tv = 0;
 	if (fl & (M_HASDIFFUSEMAP | M_HAS2NDDIFFUSEMAP | M_HASSPECULARITYMAP | M_HASNORMALMAP | M_HAS2NDNORMALMAP | M_HASROUGHNESSMAP))
 	{
 		const float ww = 1 - (u + v);
-		tu = u * tri.u0 + v * tri.u1 + ww * tri.u2;
-		tv = u * tri.v0 + v * tri.v1 + ww * tri.v2;
+		tu = prime ? u * tri.u0 + v * tri.u1 + ww * tri.u2 : ww * tri.u0 + u * tri.u1 + v * tri.u2;
+		tv = prime ? u * tri.v0 + v * tri.v1 + ww * tri.v2 : ww * tri.v0 + u * tri.v1 + v * tri.v2;
 	}
 	const auto uvmap = [&](const uint* m, float& x, float& y) {
 		x = h2f(m[1]) * (h2f(m[2]) + tu);
@@ -418,9 +424,12 @@ static f3 cosine_sample(float r0, float r1)
 
 static inline f3 reflect(f3 i, f3 n) { return i - 2.0f * n * dot(n, i); }
 
-/* Lambert + pure specular BSDF (lib/RenderCore_PrimeRef/kernels/bsdf.h:19-101). */
-static float fr_l(float VDotN, float eio)
+/* Lambert + pure specular BSDF (lib/RenderCore_PrimeRef/kernels/bsdf.h:19-101).
+ * below: the shared Lambert BSDF's Fr_L (sharedBSDFs/lambert.h:32-45), which
+ * inverts eta for a view vector below the surface. */
+static float fr_l(float VDotN, float eio, bool below = false)
 {
+	if (below && VDotN < 0.0f) eio = 1.0f / eio, VDotN = fabsf(VDotN);
 	const float SinThetaT2 = (eio * eio) * (1.0f - VDotN * VDotN);
 	if (SinThetaT2 > 1.0f) return 1.0f;
 	const float LDotN = sqrtf(1.0f - SinThetaT2);
@@ -445,7 +454,8 @@ static f3 evaluate_bsdf(const Shading& sd, f3 iN, f3 wo, f3 wi, float& pdf)
 	pdf = fabsf(dot(wi, iN)) * INVPI;
 	return sd.color * INVPI * sd.roughness();
 }
-static f3 sample_bsdf(const Shading& sd, f3 iN, f3 N, f3 wo, float distance, float r3, float r4, f3& wi, float& pdf, bool& specular)
+static f3 sample_bsdf(const Shading& sd, f3 iN, f3 N, f3 wo, float distance, float r3, float r4, f3& wi, float& pdf, bool& specular,
+	bool shared = false)
 {
 	const float flip = (dot(wo, N) < 0) ? -1 : 1;
 	iN = iN * flip;
@@ -454,7 +464,7 @@ static f3 sample_bsdf(const Shading& sd, f3 iN, f3 N, f3 wo, float distance, flo
 	const float TR = sd.transmission(), RO = sd.roughness();
 	if (r4 < TR)
 	{
-		const float eio = flip < 0 ? (1.0f / sd.eta()) : sd.eta(), F = fr_l(dot(iN, wo), eio);
+		const float eio = flip < 0 ? (1.0f / sd.eta()) : sd.eta(), F = fr_l(dot(iN, wo), eio, shared);
 		f3 beer = mk3(1, 1, 1);
 		beer.x = cw_expf(-sd.transmittance.x * distance * 2.0f);
 		beer.y = cw_expf(-sd.transmittance.y * distance * 2.0f);
@@ -622,6 +632,285 @@ void render_frame(const RenderScene& s, const RenderParams& p, uint R0gen, int p
 			for (int j = t; j < n; j += threads) trace_path(s, p, j, R0gen, pass, pathAcc + (size_t)j * 4);
 		});
 	for (auto& th : pool) th.join();
+}
+
+/* ==== optix7 estimator ======================================================
+ * rendercore_optix7: setupPrimaryRay/generateEyeRay (optix/.optix.cu:85-129),
+ * shadeKernel (kernels/pathtracer.h:54-238) with the shared Lambert BSDF
+ * (sharedBSDFs/lambert.h; BSDF_HAS_PURE_SPECULARS), connections traced after
+ * each shade wave and added in that order (generateShadowRay, .optix.cu:142-154).
+ * MAXPATHLENGTH 3, ENOUGH_BOUNCES S_BOUNCED, CLAMPFIREFLIES and BLUENOISE on
+ * (core_settings.h:24-33,185-197). */
+enum { S7_SPECULAR = 1, S7_BOUNCED = 2, S7_VIASPECULAR = 4, S7_BOUNCEDTWICE = 8 };
+static const int MAXPATHLENGTH7 = 3;
+
+/* blueNoiseSampler4 (optix/.optix.cu:56-69) */
+static f4 blue_noise4(const std::vector<uint>& bn, int x, int y, int sampleIndex, int dim)
+{
+	const uint* rk = bn.data() + dim + (x + y * 128) * 8 + 65536 * 3;
+	const int rsi1 = (int)((sampleIndex ^ rk[0]) & 255u), rsi2 = (int)((sampleIndex ^ rk[1]) & 255u);
+	const int rsi3 = (int)((sampleIndex ^ rk[2]) & 255u), rsi4 = (int)((sampleIndex ^ rk[3]) & 255u);
+	const uint v1 = bn[dim + 0 + rsi1 * 256], v2 = bn[dim + 1 + rsi2 * 256], v3 = bn[dim + 2 + rsi3 * 256], v4 = bn[dim + 3 + rsi4 * 256];
+	const uint* sc = bn.data() + (dim & 7) + (x + y * 128) * 8 + 65536;
+	return { (0.5f + (float)(v1 ^ sc[0])) * (1.0f / 256.0f), (0.5f + (float)(v2 ^ sc[1])) * (1.0f / 256.0f),
+		(0.5f + (float)(v3 ^ sc[2])) * (1.0f / 256.0f), (0.5f + (float)(v4 ^ sc[3])) * (1.0f / 256.0f) };
+}
+
+/* SampleSmallSkydome (tools_shared.h:204-211) */
+static f3 sample_small_sky(const RenderScene& s, f3 D)
+{
+	if (s.skyW <= 0 || s.skyH <= 0) return mk3(0, 0, 0);
+	const float* m = s.worldToSky;
+	const f3 d = mk3(-(m[0] * D.x + m[1] * D.y + m[2] * D.z), -(m[4] * D.x + m[5] * D.y + m[6] * D.z), -(m[8] * D.x + m[9] * D.y + m[10] * D.z));
+	const uint w = (uint)s.skyW >> 6, h = (uint)s.skyH >> 6;
+	const float p = lh_atan2f(d.y, d.x);
+	const float phi = p < 0 ? (p + 2 * PI) : p;
+	const float theta = lh_acosf(std::min(std::max(d.z, -1.0f), 1.0f));
+	const uint u = f2u_sat((float)w * phi * 0.15915494309189533577f - 0.5f);
+	const uint v = f2u_sat((float)h * theta * 0.31830988618379067154f - 0.5f);
+	const uint idx = u + v * w;
+	if (idx >= w * h) return mk3(0, 0, 0);
+	const f4 t = s.sky[idx + (size_t)s.skyW * s.skyH];
+	return mk3(t.x, t.y, t.z);
+}
+
+/* LightPickProb, ISLIGHTS (lights_shared.h:174-190) */
+static float light_pick_prob(const RenderScene& s, int idx, f3 O, f3 N, f3 I)
+{
+	const int nt = (int)s.triLights.size(), np = (int)s.pointLights.size(), ns = (int)s.spotLights.size(), nd = (int)s.dirLights.size();
+	float sum = 0, mine = 0;
+	for (int i = 0; i < nt + np + ns + nd; i++)
+	{
+		float c;
+		if (i < nt) c = potential_tri(s.triLights[i], O, N, I, mk3(-1, -1, -1));
+		else if (i < nt + np) c = potential_point(s.pointLights[i - nt], O, N);
+		else if (i < nt + np + ns) c = potential_spot(s.spotLights[i - nt - np], O, N);
+		else c = potential_dir(s.dirLights[i - nt - np - ns], N);
+		sum += c;
+		if (i == idx) mine = c;
+	}
+	if (sum <= 0) return 0;
+	return mine / sum;
+}
+
+/* UnpackNormal / PackNormal (tools_shared.h:101-120) */
+static f3 unpack_normal(uint p)
+{
+	float nx = (float)(p & 65535u) * (2.0f / 65535.0f), ny = (float)(p >> 16) * (2.0f / 65535.0f), nz = 0, nw = 0;
+	nx += -1, ny += -1, nz += 1, nw += -1;
+	float l = nx * -nx + ny * -ny + nz * -nw;
+	nz = l, l = sqrtf(l), nx *= l, ny *= l;
+	return mk3(nx * 2.0f + 0, ny * 2.0f + 0, nz * 2.0f + -1);
+}
+static uint pack_normal(f3 N)
+{
+	const float f = 65535.0f / std::max(sqrtf(8.0f * N.z + 8.0f), 0.0001f);
+	return f2u_sat(N.x * f + 32767.0f) + (f2u_sat(N.y * f + 32767.0f) << 16);
+}
+
+static f3 clamp_intensity(f3 c, float clampValue)
+{
+	const float v = std::max(c.x, std::max(c.y, c.z));
+	if (v > clampValue) { const float m = clampValue / v; c.x *= m, c.y *= m, c.z *= m; }
+	return c;
+}
+static f3 fixnan(f3 c) { return std::isfinite(c.x + c.y + c.z) ? c : mk3(0, 0, 0); }
+
+/* generateEyeRay + RayTarget (optix/.optix.cu:85-104; common_functions.h:29-50) */
+static void eye_ray7(const RenderScene& s, const Render7Params& p, uint pathIdx, f3& O, f3& D)
+{
+	const uint wh = (uint)(p.w * p.h);
+	const uint pixelIdx = pathIdx % wh, sampleIdx = pathIdx / wh + (uint)p.pass;
+	uint seed = wang_hash(pathIdx * 16789u + (uint)p.pass * 1791u);
+	const int sx = (int)(pixelIdx % (uint)p.w), sy = (int)(pixelIdx / (uint)p.w);
+	f4 r4;
+	if (sampleIdx < 64) r4 = blue_noise4(s.blueNoise, (sx + (int)(p.shift & 127u)) & 127, (sy + (int)(p.shift >> 24)) & 127, (int)sampleIdx, 0);
+	else
+	{
+		r4.x = rand_float(seed), r4.y = rand_float(seed);
+		r4.z = rand_float(seed), r4.w = rand_float(seed);
+	}
+	const ViewPyramid& v = p.view;
+	const f3 pos = f3of(v.pos), p1 = f3of(v.p1);
+	const f3 right = f3of(v.p2) - p1, up = f3of(v.p3) - p1;
+	O = point_on_lens(r4.x, r4.z, pos, v.aperture, right, up);
+	f3 target;
+	if (v.distortion == 0)
+	{
+		const float u = ((float)sx + r4.y) * (1.0f / (float)p.w), vv = ((float)sy + r4.w) * (1.0f / (float)p.h);
+		target = p1 + u * right + vv * up;
+	}
+	else
+	{
+		const float tx = (float)sx / (float)p.w - 0.5f, ty = (float)sy / (float)p.h - 0.5f;
+		const float rr = tx * tx + ty * ty;
+		const float rq = sqrtf(rr) * (1.0f + v.distortion * rr + v.distortion * rr * rr);
+		const float theta = lh_atan2f(tx, ty);
+		float st, ct;
+		cw_sincos(theta, &st, &ct);
+		const float bx = (st * rq + 0.5f) * (float)p.w, by = (ct * rq + 0.5f) * (float)p.h;
+		target = p1 + (bx + r4.y) * (right / (float)p.w) + (by + r4.w) * (up / (float)p.h);
+	}
+	D = normalize(target - O);
+}
+
+/* One optix7 path, depth first; returns the number of shade waves it took part in. */
+static int trace_path7(const RenderScene& s, const Render7Params& p, const uint* R0wave, uint pathIdx, float* acc4)
+{
+	const uint wh = (uint)(p.w * p.h);
+	const uint sampleIdx = pathIdx / wh + (uint)p.pass;
+	const bool connections = (s.triLights.size() + s.pointLights.size() + s.spotLights.size() + s.dirLights.size()) > 0;
+	f3 O, D;
+	eye_ray7(s, p, pathIdx, O, D);
+	f3 throughput = mk3(1, 1, 1);
+	float bsdfPdf = 1;
+	uint flags = S7_SPECULAR, lastNormal = 0;
+	float ax = 0, ay = 0, az = 0, aw = 0;
+	int waves = 0;
+	for (int pathLength = 1; pathLength <= MAXPATHLENGTH7; pathLength++)
+	{
+		waves = pathLength;
+		const Hit h = trace_closest(s, O, p.eps, D, 1e34f, nullptr);
+		if (h.prim < 0)
+		{
+			const f3 sky = (flags & S7_BOUNCED) ? sample_small_sky(s, D) : sample_sky(s, D);
+			f3 c = throughput * sky * (1.0f / bsdfPdf);
+			c = fixnan(clamp_intensity(c, p.clampValue));
+			ax += c.x, ay += c.y, az += c.z;
+			break;
+		}
+		const float b0 = (float)(h.uv & 65535u) * (1.0f / 65535.0f), b1 = (float)(h.uv >> 16) * (1.0f / 65535.0f);
+		const float t = h.t;
+		const CoreTri& tri = s.tris[s.inst[h.inst].mesh][h.prim];
+		Shading sd;
+		f3 N, iN, fN, T;
+		const f3 I = O + t * D;
+		/* the hit record holds the weights of vertices 0 and 1 */
+		const bool alphaHit = shading_data(s, D, b1, 1.0f - (b0 + b1), p.view.spreadAngle * t, tri, h.inst, sd, N, iN, fN, T, false);
+		uint seed = wang_hash(pathIdx * 17u + R0wave[pathLength]);
+		if (alphaHit)
+		{
+			if (pathLength == MAXPATHLENGTH7) break;
+			if (!std::isfinite(throughput.x + throughput.y + throughput.z)) throughput = mk3(0, 0, 0);
+			O = I + D * p.eps;
+			continue;
+		}
+		if (sd.emissive())
+		{
+			const float DdotNL = -dot(D, N);
+			if (DdotNL > 0)
+			{
+				f3 c = mk3(0, 0, 0);
+				if (pathLength == 1 || (flags & S7_SPECULAR) || !connections) c = sd.color;
+				else
+				{
+					const f3 lastN = unpack_normal(lastNormal);
+					const float lightPdf = (t * t) / (fabsf(dot(D, N)) * tri.area);
+					const float pickProb = light_pick_prob(s, tri.ltriIdx, O, lastN, I);
+					if ((bsdfPdf + lightPdf * pickProb) > 0) c = throughput * sd.color * (1.0f / (bsdfPdf + lightPdf * pickProb));
+				}
+				c = fixnan(clamp_intensity(c, p.clampValue));
+				ax += c.x, ay += c.y, az += c.z;
+			}
+			break;
+		}
+		if (flags & S7_BOUNCED) sd.params[0] |= 255u << 24;
+		if (sd.roughness() <= 0.001f || sd.transmission() > 0.5f) flags |= S7_SPECULAR; else flags &= ~(uint)S7_SPECULAR;
+		const float faceDir = (dot(D, N) > 0) ? -1 : 1;
+		if (faceDir == 1) sd.transmittance = mk3(0, 0, 0);
+		throughput = throughput * (1.0f / bsdfPdf);
+		f4 r4;
+		if (sampleIdx < 64)
+		{
+			const int x = (int)(((pathIdx % (uint)p.w) + (p.shift & 127u)) & 127u);
+			const int y = (int)(((pathIdx / (uint)p.w) + (p.shift >> 24)) & 127u);
+			r4 = blue_noise4(s.blueNoise, x, y, (int)sampleIdx, 4 * pathLength - 4);
+		}
+		else
+		{
+			r4.x = rand_float(seed), r4.y = rand_float(seed);
+			r4.z = rand_float(seed), r4.w = rand_float(seed);
+		}
+		if ((flags & S7_SPECULAR) == 0 && connections)
+		{
+			float pickProb = 0, lightPdf = 0;
+			f3 lightColor = mk3(0, 0, 0);
+			f3 L = random_point_on_light(s, r4.x, r4.y, I, fN * faceDir, pickProb, lightPdf, lightColor) - I;
+			const float dist = length(L);
+			L = L * (1.0f / dist);
+			const float NdotL = dot(L, fN * faceDir);
+			if (NdotL > 0 && lightPdf > 0)
+			{
+				float pdf;
+				const f3 sampled = evaluate_bsdf(sd, fN, D * -1.0f, L, pdf);   /* colour / pi times ROUGHNESS */
+				if (pdf > 0)
+				{
+					f3 c = throughput * sampled * lightColor * (NdotL / (pickProb * lightPdf + pdf));
+					c = clamp_intensity(fixnan(c), p.clampValue);
+					const f3 so = safe_origin(I, L, N, p.eps);
+					if (!trace_any(s, so, p.eps, L, dist - 2 * p.eps, nullptr)) ax += c.x, ay += c.y, az += c.z, aw += 1.0f;
+				}
+			}
+		}
+		if ((flags & S7_BOUNCED) || pathLength == MAXPATHLENGTH7) break;
+		f3 R;
+		float newBsdfPdf;
+		bool specular = false;
+		const float r5 = rand_float(seed);
+		(void)r5;   /* SampleBSDF's third random number; the Lambert BSDF renormalises r3 instead */
+		const f3 bsdf = sample_bsdf(sd, fN, N, D * -1.0f, t, r4.z, r4.w, R, newBsdfPdf, specular, true);
+		if (newBsdfPdf < EPSILON || std::isnan(newBsdfPdf)) break;
+		if (specular) flags |= S7_SPECULAR;
+		const float pr = ((flags & S7_SPECULAR) || ((flags & S7_BOUNCED) == 0)) ? 1 : std::min(1.0f, std::max(std::max(bsdf.x, bsdf.y), bsdf.z));
+		if (pr < rand_float(seed)) break;
+		throughput = throughput * (1 / pr);
+		lastNormal = pack_normal(fN * faceDir);
+		if (!(flags & S7_SPECULAR)) flags |= (flags & S7_BOUNCED) ? S7_BOUNCEDTWICE : S7_BOUNCED; else flags |= S7_VIASPECULAR;
+		O = safe_origin(I, R, N, p.eps);
+		D = R;
+		throughput = fixnan(throughput) * bsdf * fabsf(dot(fN, R));
+		bsdfPdf = newBsdfPdf;
+	}
+	acc4[0] = ax, acc4[1] = ay, acc4[2] = az, acc4[3] = aw;
+	return waves;
+}
+
+int render_frame7(const RenderScene& s, const Render7Params& p, uint camSeed, float* pathAcc, int threads)
+{
+	uint R0wave[MAXPATHLENGTH7 + 1] = { 0 };
+	uint cs = camSeed;
+	for (int k = 1; k <= MAXPATHLENGTH7; k++) R0wave[k] = xor32(cs) + (uint)k * 91771u;
+	const int n = p.w * p.h * p.spp;
+	if (threads < 1) threads = 1;
+	std::vector<int> deepest(threads, 0);
+	std::vector<std::thread> pool;
+	for (int t = 0; t < threads; t++)
+		pool.emplace_back([&, t]() {
+			for (int j = t; j < n; j += threads)
+				deepest[t] = std::max(deepest[t], trace_path7(s, p, R0wave, (uint)j, pathAcc + (size_t)j * 4));
+		});
+	for (auto& th : pool) th.join();
+	return *std::max_element(deepest.begin(), deepest.end());
+}
+
+void blue_noise_sample4(const RenderScene& s, int x, int y, int sampleIndex, int dim, float* out4)
+{
+	const f4 r = blue_noise4(s.blueNoise, x, y, sampleIndex, dim);
+	out4[0] = r.x, out4[1] = r.y, out4[2] = r.z, out4[3] = r.w;
+}
+f3 sample_sky_small(const RenderScene& s, f3 D) { return sample_small_sky(s, D); }
+uint normal_pack(const float* n3) { return pack_normal(mk3(n3[0], n3[1], n3[2])); }
+void normal_unpack(uint p, float* n3) { const f3 n = unpack_normal(p); n3[0] = n.x, n3[1] = n.y, n3[2] = n.z; }
+
+/* Seeded blue-noise tables of the core (xorshift32 bytes in the sampler's layout). */
+void blue_noise_default(uint* out, uint seed)
+{
+	uint x = seed ? seed : 1u;
+	const int words = 65536 * 5 + 16;
+	for (int i = 0; i < words; i++) out[i] = 0;
+	for (int i = 0; i < 65536; i++) out[i] = xor32(x) & 255u;
+	for (int i = 0; i < 128 * 128 * 8; i++) out[65536 + i] = xor32(x) & 255u;
+	for (int i = 0; i < 128 * 128 * 8; i++) out[3 * 65536 + i] = xor32(x) & 255u;
 }
 
 void primary_ray(const ViewPyramid& v, int w, int h, uint R0, int job, float tmin, float* ray8)

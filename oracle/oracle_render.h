@@ -50,6 +50,8 @@ struct RenderScene : Scene
 	std::vector<f4> sky;
 	int skyW = 0, skyH = 0;
 	float worldToSky[16] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1 };
+	/* blue-noise tables of the optix7 estimator: 5 * 65536 + 16 words */
+	std::vector<uint> blueNoise;
 };
 
 OMat convert_material(const CoreMaterial& m, const std::vector<TexInfo>& tex);
@@ -69,6 +71,25 @@ struct RenderParams
 /* One frame (spp samples per pixel) of the PrimeRef estimator; per-path
  * contributions are summed in fp32 per path (pathAcc: w*h*spp float4). */
 void render_frame(const RenderScene& s, const RenderParams& p, uint R0gen, int pass, float* pathAcc, int threads);
+
+struct Render7Params
+{
+	ViewPyramid view;
+	int w, h, spp, pass;
+	float eps, clampValue;
+	uint shift;
+};
+
+/* One frame of the optix7 estimator (Lambert BSDF). camSeed is camRNGseed at
+ * the frame's start; the per-wave shade seeds are RandomUInt(camSeed) + k *
+ * 91771. pathAcc: w*h*spp float4 (xyz radiance, w unoccluded connections).
+ * Returns the number of shade waves the frame ran. */
+int render_frame7(const RenderScene& s, const Render7Params& p, uint camSeed, float* pathAcc, int threads);
+void blue_noise_default(uint* out, uint seed);
+void blue_noise_sample4(const RenderScene& s, int x, int y, int sampleIndex, int dim, float* out4);
+uint normal_pack(const float* n3);
+f3 sample_sky_small(const RenderScene& s, f3 D);
+void normal_unpack(uint p, float* n3);
 
 /* Microbenchmark rays. */
 void primary_ray(const ViewPyramid& v, int w, int h, uint R0, int job, float tmin, float* ray8);
