@@ -251,6 +251,11 @@ void RenderCore::Setting(const char* name, float value)
 		if (value != 0 && value != 1) throw CoreError(LH2HIP_ERR_INVALID, "Setting estimator: 0 (reference) or 1 (optix7)");
 		estimator = (int)value;
 	}
+	else if (!strcmp(name, "bsdf"))
+	{	/* optix7 estimator's BSDF: the compile-time choice of kernels/bsdf.h:7-22 */
+		if (value != 0 && value != 1) throw CoreError(LH2HIP_ERR_INVALID, "Setting bsdf: 0 (Lambert) or 1 (Disney)");
+		bsdfSetting = (int)value;
+	}
 	else if (!strcmp(name, "traceMinRaysPerLane"))
 	{
 		if (!(value >= 0 && value <= 64)) throw CoreError(LH2HIP_ERR_INVALID, "Setting traceMinRaysPerLane: 0..64");
@@ -736,6 +741,7 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
 	P.w2s1 = make_float4(worldToSky[4], worldToSky[5], worldToSky[6], 0.0f);
 	P.w2s2 = make_float4(worldToSky[8], worldToSky[9], worldToSky[10], 0.0f);
 	P.clampValue = clampValue;
+	P.bsdf = bsdfSetting;
 	P.connections = (P.nTri + P.nPoint + P.nSpot + P.nDir) > 0;   /* noDirectLightsInScene, rendercore_optix7/rendercore.cpp:708 */
 	f.estimator = estimator;
 	if (estimator == ESTIMATOR_OPTIX7)

@@ -169,6 +169,7 @@ private:
 	int maxDepth = 64, nodeFormat = FMT_Q8, traceLoop = LOOP_MERGED, triBias = 16, refillLanes = 16, chunk = 32;
 	uint32_t camSeed = 0x12345678u, seedSetting = 0x12345678u;
 	int estimator = ESTIMATOR_REFERENCE;
+	int bsdfSetting = 1;               /* Disney, as rendercore_optix7 ships (kernels/bsdf.h) */
 	uint32_t shiftSeed = 0x11331445u;  /* blue-noise shift of the optix7 estimator (rendercore_optix7/rendercore.h:123) */
 	DevBuf blueNoise;
 	/* render state */

@@ -58,6 +58,7 @@ struct ShadeParams
 	uint32_t shift;
 	float clampValue;
 	int connections;
+	int bsdf;                     /* optix7 estimator: 0 Lambert (sharedBSDFs/lambert.h), 1 Disney (sharedBSDFs/disney.h) */
 };
 
 /* Path-state buffers of one pass (SoA). */

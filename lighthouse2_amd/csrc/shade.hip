@@ -32,6 +32,7 @@ struct Shading
 {
 	v3 color, transmittance;
 	uint4 params;
+	float4 tint;   /* tint rgb and luminance of the untextured base colour (Disney BSDF) */
 };
 __device__ __forceinline__ float char2flt(uint32_t a, int s) { return ((float)((a >> s) & 255u)) * (1.0f / 255.0f); }
 __device__ __forceinline__ float roughness(const Shading& sd) { const float r = char2flt(sd.params.x, 24); return r > 0.001f ? r : 0.001f; }
@@ -121,6 +122,20 @@ __device__ bool shading_data(const ShadeParams& P, v3 D, float u, float v, float
 	const GpuMaterial& mat = P.mats[fbits(t1.w)];
 	const uint32_t flags = mat.flags;
 	sd.color = mk3(half_to_float(mat.diffuse_rg & 0xffffu), half_to_float(mat.diffuse_rg >> 16), half_to_float(mat.diffuse_b_medium_r & 0xffffu));
+	if (!PRIME)
+	{	/* tint4 (material_shared.h:19-33,118-119) */
+		const v3 c = sd.color;
+		const v3 X = mk3(fmaxf(0.0f, 0.412453f * c.x + 0.357580f * c.y + 0.180423f * c.z), fmaxf(0.0f, 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z),
+			fmaxf(0.0f, 0.019334f * c.x + 0.119193f * c.y + 0.950227f * c.z));
+		v3 t = mk3(1, 1, 1);
+		if (X.y > 0)
+		{
+			const v3 n = X * (1.0f / X.y);
+			t = mk3(fmaxf(0.0f, 3.240479f * n.x - 1.537150f * n.y - 0.498535f * n.z), fmaxf(0.0f, -0.969256f * n.x + 1.875992f * n.y + 0.041556f * n.z),
+				fmaxf(0.0f, 0.055648f * n.x - 0.204043f * n.y + 1.057311f * n.z));
+		}
+		sd.tint = make_float4(t.x, t.y, t.z, X.y);
+	}
 	sd.transmittance = mk3(half_to_float(mat.diffuse_b_medium_r >> 16), half_to_float(mat.medium_gb & 0xffffu), half_to_float(mat.medium_gb >> 16));
 	sd.params = mat.params;
 	N = iN = fN = mk3(t2.w, t3.w, t4.w);
@@ -578,6 +593,8 @@ hipError_t launch_advance(uint32_t* counters, uint32_t* logSlot, hipStream_t s)
 	return hipGetLastError();
 }
 
+#include "disney.h"
+
 /* ==== optix7 estimator ======================================================
  * lib/rendercore_optix7: eye rays (optix/.optix.cu:85-129), shadeKernel
  * (kernels/pathtracer.h:54-238) with the Lambert BSDF variant of the core
@@ -828,7 +845,8 @@ __global__ void __launch_bounds__(128) k_shade7(const ShadeParams P, const float
 		if (NdotL > 0 && lightPdf > 0)
 		{
 			float pdf;
-			const v3 sampled = evaluate_bsdf(sd, fN, L, pdf) ;   /* Lambert: colour / pi, times ROUGHNESS (pure speculars) */
+			/* Lambert: colour / pi times ROUGHNESS (BSDF_HAS_PURE_SPECULARS); Disney as evaluated */
+			const v3 sampled = P.bsdf ? disney_evaluate(sd, fN, T, D * -1.0f, L, pdf) : evaluate_bsdf(sd, fN, L, pdf);
 			if (pdf > 0)
 			{
 				v3 c = throughput * sampled * lightColor * (NdotL / (pickProb * lightPdf + pdf));
@@ -845,8 +863,9 @@ __global__ void __launch_bounds__(128) k_shade7(const ShadeParams P, const float
 	v3 R;
 	float newBsdfPdf;
 	bool specular = false;
-	const v3 bsdf = sample_bsdf<true>(sd, fN, N, D * -1.0f, t, r4.z, r4.w, R, newBsdfPdf, specular);
-	(void)rand_float(seed);                         /* SampleBSDF's third random number (unused by Lambert) */
+	const float r5 = rand_float(seed);              /* SampleBSDF's third random number, drawn at the call */
+	const v3 bsdf = P.bsdf ? disney_sample(sd, fN, N, T, D * -1.0f, t, r4.z, r4.w, r5, R, newBsdfPdf, specular)
+		: sample_bsdf<true>(sd, fN, N, D * -1.0f, t, r4.z, r4.w, R, newBsdfPdf, specular);
 	if (newBsdfPdf < EPSILON_F || __builtin_isnan(newBsdfPdf)) return;
 	if (specular) flags |= S7_SPECULAR;
 	const float p = ((flags & S7_SPECULAR) || ((flags & S7_BOUNCED) == 0)) ? 1 : fminf(1.0f, fmaxf(fmaxf(bsdf.x, bsdf.y), bsdf.z));

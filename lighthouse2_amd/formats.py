@@ -155,4 +155,19 @@ def lambert_material(color, roughness=1.0, transmission=0.0):
     return m
 
 
+def disney_material(color, metallic=0.0, subsurface=0.0, specular=0.5, roughness=0.5, specularTint=0.0, anisotropic=0.0,
+                    sheen=0.0, sheenTint=0.0, clearcoat=0.0, clearcoatGloss=1.0, transmission=0.0, eta=1.5,
+                    absorption=(0.0, 0.0, 0.0)):
+    """One CoreMaterial with the Disney BSDF's parameters (HostMaterial fields
+    of host_material.h, packed by SetMaterials into the 8-bit parameter words)."""
+    m = new_materials(1)
+    m["color"]["value"][0] = np.asarray(color, np.float32)
+    m["absorption"]["value"][0] = np.asarray(absorption, np.float32)
+    for name, v in (("metallic", metallic), ("subsurface", subsurface), ("specular", specular), ("roughness", roughness),
+                    ("specularTint", specularTint), ("anisotropic", anisotropic), ("sheen", sheen), ("sheenTint", sheenTint),
+                    ("clearcoat", clearcoat), ("clearcoatGloss", clearcoatGloss), ("transmission", transmission), ("eta", eta)):
+        m[name]["value"] = v
+    return m
+
+
 IDENTITY = np.eye(4, dtype=np.float32).reshape(16)

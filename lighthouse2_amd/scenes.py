@@ -122,6 +122,15 @@ def add_triangles(v0, v1, v2, mat):
     tris["vN0"], tris["vN1"], tris["vN2"] = N, N, N
     tris["Nx"], tris["Ny"], tris["Nz"] = N[:, 0], N[:, 1], N[:, 2]
     tris["vertex0"], tris["vertex1"], tris["vertex2"] = v0, v1, v2
+    # tangent frame from the first edge, as HostMesh builds it for triangles without uv
+    # (host_mesh.cpp:634-637)
+    d = (e1[:, 0] * e1[:, 0] + e1[:, 1] * e1[:, 1]) + e1[:, 2] * e1[:, 2]
+    Tn = (e1 * (f32(1.0) / np.sqrt(d))[:, None]).astype(f32)
+    c = np.stack([N[:, 1] * Tn[:, 2] - N[:, 2] * Tn[:, 1], N[:, 2] * Tn[:, 0] - N[:, 0] * Tn[:, 2],
+                  N[:, 0] * Tn[:, 1] - N[:, 1] * Tn[:, 0]], 1).astype(f32)
+    dc = (c[:, 0] * c[:, 0] + c[:, 1] * c[:, 1]) + c[:, 2] * c[:, 2]
+    tris["T"] = Tn
+    tris["B"] = (c * (f32(1.0) / np.sqrt(dc))[:, None]).astype(f32)
     return verts, tris
 
 
@@ -238,6 +247,24 @@ def cornell_box(seed=1234, sphere_tris=20000):
     v, t = add_quad((0, -1, 0), (0.5, 0.999, 0.5), 0.2, 0.2, lm)
     li = s.add_instance(s.add_mesh(v, t))
     add_emitters(s, t, (15.0, 15.0, 15.0), li)
+    return s
+
+
+def disney_cornell_box(seed=1234, sphere_tris=4000):
+    """The Cornell box with Disney materials: plastic floor, brushed-metal back
+    wall (anisotropic), clear-coated left wall, sheen + subsurface right wall,
+    diffuse ceiling, and a rough glass sphere with absorption."""
+    s = cornell_box(seed=seed, sphere_tris=sphere_tris)
+    rng = np.random.default_rng(seed + 1)
+    c = lambda: rng.uniform(0.25, 0.85, 3).astype(f32)
+    mats = [F.disney_material(c(), specular=0.5, roughness=0.35),
+            F.disney_material(c(), specular=0.0, roughness=0.9),
+            F.disney_material(c(), metallic=1.0, roughness=0.45, anisotropic=0.5),
+            F.disney_material(c(), specular=0.3, roughness=0.6, clearcoat=1.0, clearcoatGloss=0.7),
+            F.disney_material(c(), specular=0.2, roughness=0.8, sheen=0.8, sheenTint=0.5, subsurface=0.5),
+            F.disney_material((0.9, 0.95, 1.0), roughness=0.15, transmission=1.0, eta=1.5, absorption=(0.3, 0.1, 0.05))]
+    for k, m in enumerate(mats):
+        s.materials[k] = m[0]
     return s
 
 

@@ -44,7 +44,9 @@ def lib():
             "orc_svgf_read": (None, [P, I, P]),
             "orc_set_blue_noise": (None, [P, P, P, P, U]),
             "orc_blue_noise4": (None, [P, P, I, P]), "orc_normals": (None, [I, P, P, I]),
-            "orc_render_frame7": (I, [P, P, I, I, I, I, ctypes.c_float, ctypes.c_float, U, U, P, I]),
+            "orc_disney_eval": (None, [P, P, P, P, P, P, P, I, P]), "orc_disney_sample": (None, [P, P, P, P, P, P, P, I, P]),
+            "orc_ggx": (None, [I, P, P, P, I, ctypes.c_float, ctypes.c_float, P]),
+            "orc_render_frame7": (I, [P, P, I, I, I, I, ctypes.c_float, ctypes.c_float, U, U, P, I, I]),
         }
         for n, (r, a) in sigs.items():
             f = getattr(L, n)
@@ -165,12 +167,12 @@ class OracleScene:
         lib().orc_blue_noise4(self.h, _p(q), len(q), _p(out))
         return out
 
-    def render_frame7(self, view, w, h, spp, pass_, eps, clamp_value, shift, cam_seed, threads=8):
+    def render_frame7(self, view, w, h, spp, pass_, eps, clamp_value, shift, cam_seed, threads=8, bsdf=1):
         """One optix7-estimator frame: per-path (xyz radiance, w connections) and
         the number of shade waves it ran."""
         acc = np.zeros((spp * w * h, 4), np.float32)
         v = np.ascontiguousarray(view)
-        waves = lib().orc_render_frame7(self.h, _p(v), w, h, spp, pass_, eps, clamp_value, shift, cam_seed, _p(acc), threads)
+        waves = lib().orc_render_frame7(self.h, _p(v), w, h, spp, pass_, eps, clamp_value, shift, cam_seed, _p(acc), threads, bsdf)
         if waves < 0:
             raise RuntimeError("set_blue_noise first")
         return acc, waves
@@ -269,6 +271,49 @@ def math(which, a, b=None):
     return out
 
 
+def disney_params(metallic=0.0, subsurface=0.0, specular=0.5, roughness=0.5, spectint=0.0, anisotropic=0.0, sheen=0.0,
+                  sheentint=0.0, clearcoat=0.0, clearcoatgloss=0.0, transmission=0.0, eta=1.5):
+    """Packed ShadingData.parameters words (tools_shared.h:33-36) from values in [0, 1]."""
+    b = lambda v: int(round(min(max(v, 0.0), 1.0) * 255))
+    x = b(metallic) | b(subsurface) << 8 | b(specular) << 16 | b(roughness) << 24
+    y = b(spectint) | b(anisotropic) << 8 | b(sheen) << 16 | b(sheentint) << 24
+    z = b(clearcoat) | b(clearcoatgloss) << 8 | b(transmission) << 16
+    w = int(np.float32(eta).view(np.uint32))
+    return np.array([x, y, z, w], np.uint32)
+
+
+def disney_eval(color, params, wo, wi, N=(0, 0, 1), T=(1, 0, 0), transmittance=(0, 0, 0)):
+    """EvaluateBSDF rows: (n, 4) value rgb and pdf."""
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    out = np.zeros((len(wo), 4), np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (color, transmittance, N, T)]
+    lib().orc_disney_eval(_p(a[0]), _p(np.ascontiguousarray(params, np.uint32)), _p(a[1]), _p(a[2]), _p(a[3]), _p(wo), _p(wi), len(wo), _p(out))
+    return out
+
+
+def disney_sample(color, params, wo, r3, N=(0, 0, 1), T=(1, 0, 0), transmittance=(0, 0, 0)):
+    """SampleBSDF rows: (n, 8) value rgb, pdf, wi xyz, specular flag."""
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    r = np.ascontiguousarray(r3, np.float32).reshape(-1, 3)
+    out = np.zeros((len(wo), 8), np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (color, transmittance, N, T)]
+    lib().orc_disney_sample(_p(a[0]), _p(np.ascontiguousarray(params, np.uint32)), _p(a[1]), _p(a[2]), _p(a[3]), _p(wo), _p(r), len(wo), _p(out))
+    return out
+
+
+def ggx(which, v, m, r2, ax, ay):
+    """0 D(m), 1 visible-normal pdf(v, m), 2 sample(v, r), 3 GTR1 D(m), 4 GTR1 sample(r), 5 G1(v)."""
+    k = {"D": 0, "pdf": 1, "sample": 2, "gtr1_D": 3, "gtr1_sample": 4, "G1": 5}[which]
+    v = np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+    m = np.ascontiguousarray(m, np.float32).reshape(-1, 3)
+    r = np.ascontiguousarray(r2, np.float32).reshape(-1, 2)
+    n = max(len(v), len(m), len(r))
+    out = np.zeros((n, 3), np.float32)
+    lib().orc_ggx(k, _p(v), _p(m), _p(r), n, ax, ay, _p(out))
+    return out
+
+
 def pack_normals(n3):
     """PackNormal of (n, 3) unit vectors -> uint32."""
     n = np.ascontiguousarray(n3, np.float32).reshape(-1, 3)
@@ -285,7 +330,7 @@ def unpack_normals(packed):
 
 
 def render7(scene_or_oracle, view, w, h, spp=1, frames=1, eps=1e-3, seed=0x12345678, clamp_value=10.0, threads=8,
-            modes=None, blue_noise=None, shift_seed=0x11331445):
+            modes=None, blue_noise=None, shift_seed=0x11331445, bsdf=1):
     """Oracle of a sequence of Render calls with the optix7 estimator
     (rendercore_optix7/rendercore.cpp:822-905): the blue-noise shift seed
     advances once per frame and is never reset; a Restart (or the first
@@ -307,7 +352,7 @@ def render7(scene_or_oracle, view, w, h, spp=1, frames=1, eps=1e-3, seed=0x12345
         if m == CONVERGE:
             first = False
         shift, _ = xor32_host(shift)
-        pa, waves = o.render_frame7(view, w, h, spp, taken, eps, clamp_value, shift, cam, threads)
+        pa, waves = o.render_frame7(view, w, h, spp, taken, eps, clamp_value, shift, cam, threads, bsdf)
         for _ in range(waves):
             cam, _ = xor32_host(cam)
         for s in range(spp):

@@ -236,13 +236,29 @@ OAPI void orc_normals(int pack, float* n3, uint* packed, int n)
 		if (pack) packed[i] = normal_pack(n3 + 3 * i); else normal_unpack(packed[i], n3 + 3 * i);
 }
 
+/* Batched Disney / GGX probes (rows of inputs). */
+OAPI void orc_disney_eval(const float* color3, const uint* params4, const float* transm3, const float* N3, const float* T3,
+	const float* wo3, const float* wi3, int n, float* out4)
+{
+	for (int i = 0; i < n; i++) disney_probe_eval(color3, params4, transm3, N3, T3, wo3 + 3 * i, wi3 + 3 * i, out4 + 4 * i);
+}
+OAPI void orc_disney_sample(const float* color3, const uint* params4, const float* transm3, const float* N3, const float* T3,
+	const float* wo3, const float* r3, int n, float* out8)
+{
+	for (int i = 0; i < n; i++) disney_probe_sample(color3, params4, transm3, N3, T3, wo3 + 3 * i, r3 + 3 * i, out8 + 8 * i);
+}
+OAPI void orc_ggx(int which, const float* v3, const float* m3, const float* r2, int n, float ax, float ay, float* out3)
+{
+	for (int i = 0; i < n; i++) ggx_probe(which, v3 + 3 * i, m3 + 3 * i, ax, ay, r2[2 * i], r2[2 * i + 1], out3 + 3 * i);
+}
+
 OAPI int orc_render_frame7(void* sp, const ViewPyramid* view, int w, int h, int spp, int pass, float eps, float clampValue,
-	uint shift, uint camSeed, float* pathAcc, int threads)
+	uint shift, uint camSeed, float* pathAcc, int threads, int bsdf)
 {
 	RenderScene& s = *(RenderScene*)sp;
 	if (s.blueNoise.empty()) return -1;
 	Render7Params p;
-	p.view = *view, p.w = w, p.h = h, p.spp = spp, p.pass = pass, p.eps = eps, p.clampValue = clampValue, p.shift = shift;
+	p.view = *view, p.w = w, p.h = h, p.spp = spp, p.pass = pass, p.eps = eps, p.clampValue = clampValue, p.shift = shift, p.bsdf = bsdf;
 	return render_frame7(s, p, camSeed, pathAcc, threads);
 }
 

@@ -79,7 +79,10 @@ struct Shading
 	float transmission() const { return char2flt(params[2], 16); }
 	float eta() const { return u2f(params[3]); }
 	bool emissive() const { return color.x > 1.0f || color.y > 1.0f || color.z > 1.0f; }
+	f4 tint;   /* tint rgb and luminance of the untextured base colour (Disney BSDF) */
 };
+
+#include "oracle_disney.h"
 
 /* ConsistentNormal, Reshetov et al. 2010, Eq. 1 approximation
  * (lib/CUDA/shared_kernel_code/tools_shared.h:266-280). */
@@ -179,6 +182,21 @@ static bool shading_data(const RenderScene& s, f3 D, float u, float v, float con
 {
 	const OMat& mat = s.mats[tri.material];
 	sd.color = mk3(half_to_float(mat.diffuse[0]), half_to_float(mat.diffuse[1]), half_to_float(mat.diffuse[2]));
+	sd.tint = { 1, 1, 1, 0 };
+	if (!prime)
+	{	/* tint4 = (ciexyz_to_linear_rgb(xyz / xyz.y), xyz.y) of the base colour (material_shared.h:19-33,118-119) */
+		const f3 c = sd.color;
+		const f3 X = mk3(fmaxf(0.0f, 0.412453f * c.x + 0.357580f * c.y + 0.180423f * c.z), fmaxf(0.0f, 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z),
+			fmaxf(0.0f, 0.019334f * c.x + 0.119193f * c.y + 0.950227f * c.z));
+		f3 t = mk3(1, 1, 1);
+		if (X.y > 0)
+		{
+			const f3 n = X * (1.0f / X.y);
+			t = mk3(fmaxf(0.0f, 3.240479f * n.x - 1.537150f * n.y - 0.498535f * n.z), fmaxf(0.0f, -0.969256f * n.x + 1.875992f * n.y + 0.041556f * n.z),
+				fmaxf(0.0f, 0.055648f * n.x - 0.204043f * n.y + 1.057311f * n.z));
+		}
+		sd.tint = { t.x, t.y, t.z, X.y };
+	}
 	sd.transmittance = mk3(half_to_float(mat.transm[0]), half_to_float(mat.transm[1]), half_to_float(mat.transm[2]));
 	for (int i = 0; i < 4; i++) sd.params[i] = mat.params[i];
 	N = iN = fN = mk3(tri.Nx, tri.Ny, tri.Nz);
@@ -842,7 +860,8 @@ static int trace_path7(const RenderScene& s, const Render7Params& p, const uint*
 			if (NdotL > 0 && lightPdf > 0)
 			{
 				float pdf;
-				const f3 sampled = evaluate_bsdf(sd, fN, D * -1.0f, L, pdf);   /* colour / pi times ROUGHNESS */
+				/* Lambert: colour / pi times ROUGHNESS (BSDF_HAS_PURE_SPECULARS); Disney as evaluated */
+				const f3 sampled = p.bsdf ? disney_evaluate(sd, fN, T, D * -1.0f, L, pdf) : evaluate_bsdf(sd, fN, D * -1.0f, L, pdf);
 				if (pdf > 0)
 				{
 					f3 c = throughput * sampled * lightColor * (NdotL / (pickProb * lightPdf + pdf));
@@ -856,9 +875,9 @@ static int trace_path7(const RenderScene& s, const Render7Params& p, const uint*
 		f3 R;
 		float newBsdfPdf;
 		bool specular = false;
-		const float r5 = rand_float(seed);
-		(void)r5;   /* SampleBSDF's third random number; the Lambert BSDF renormalises r3 instead */
-		const f3 bsdf = sample_bsdf(sd, fN, N, D * -1.0f, t, r4.z, r4.w, R, newBsdfPdf, specular, true);
+		const float r5 = rand_float(seed);   /* SampleBSDF's third random number, drawn at the call */
+		const f3 bsdf = p.bsdf ? disney_sample(sd, fN, N, T, D * -1.0f, t, r4.z, r4.w, r5, R, newBsdfPdf, specular)
+			: sample_bsdf(sd, fN, N, D * -1.0f, t, r4.z, r4.w, R, newBsdfPdf, specular, true);
 		if (newBsdfPdf < EPSILON || std::isnan(newBsdfPdf)) break;
 		if (specular) flags |= S7_SPECULAR;
 		const float pr = ((flags & S7_SPECULAR) || ((flags & S7_BOUNCED) == 0)) ? 1 : std::min(1.0f, std::max(std::max(bsdf.x, bsdf.y), bsdf.z));
@@ -899,6 +918,60 @@ void blue_noise_sample4(const RenderScene& s, int x, int y, int sampleIndex, int
 	out4[0] = r.x, out4[1] = r.y, out4[2] = r.z, out4[3] = r.w;
 }
 f3 sample_sky_small(const RenderScene& s, f3 D) { return sample_small_sky(s, D); }
+
+/* Disney BSDF probes for the closed-form tests: shading data from a colour,
+ * the packed parameter words and a transmittance; tint as shading_data computes it. */
+static Shading probe_shading(const float* color3, const uint* params4, const float* transm3)
+{
+	Shading sd;
+	sd.color = mk3(color3[0], color3[1], color3[2]);
+	sd.transmittance = mk3(transm3[0], transm3[1], transm3[2]);
+	for (int i = 0; i < 4; i++) sd.params[i] = params4[i];
+	const f3 c = sd.color;
+	const f3 X = mk3(fmaxf(0.0f, 0.412453f * c.x + 0.357580f * c.y + 0.180423f * c.z), fmaxf(0.0f, 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z),
+		fmaxf(0.0f, 0.019334f * c.x + 0.119193f * c.y + 0.950227f * c.z));
+	f3 t = mk3(1, 1, 1);
+	if (X.y > 0)
+	{
+		const f3 n = X * (1.0f / X.y);
+		t = mk3(fmaxf(0.0f, 3.240479f * n.x - 1.537150f * n.y - 0.498535f * n.z), fmaxf(0.0f, -0.969256f * n.x + 1.875992f * n.y + 0.041556f * n.z),
+			fmaxf(0.0f, 0.055648f * n.x - 0.204043f * n.y + 1.057311f * n.z));
+	}
+	sd.tint = { t.x, t.y, t.z, X.y };
+	return sd;
+}
+
+void disney_probe_eval(const float* color3, const uint* params4, const float* transm3, const float* N3, const float* T3,
+	const float* wo3, const float* wi3, float* out4)
+{
+	const Shading sd = probe_shading(color3, params4, transm3);
+	float pdf = 0;
+	const f3 v = disney_evaluate(sd, mk3(N3[0], N3[1], N3[2]), mk3(T3[0], T3[1], T3[2]), mk3(wo3[0], wo3[1], wo3[2]), mk3(wi3[0], wi3[1], wi3[2]), pdf);
+	out4[0] = v.x, out4[1] = v.y, out4[2] = v.z, out4[3] = pdf;
+}
+
+void disney_probe_sample(const float* color3, const uint* params4, const float* transm3, const float* N3, const float* T3,
+	const float* wo3, const float* r3, float* out8)
+{
+	const Shading sd = probe_shading(color3, params4, transm3);
+	const f3 N = mk3(N3[0], N3[1], N3[2]);
+	f3 wi = mk3(0, 0, 0);
+	float pdf = 0;
+	bool specular = false;
+	const f3 v = disney_sample(sd, N, N, mk3(T3[0], T3[1], T3[2]), mk3(wo3[0], wo3[1], wo3[2]), 1.0f, r3[0], r3[1], r3[2], wi, pdf, specular);
+	out8[0] = v.x, out8[1] = v.y, out8[2] = v.z, out8[3] = pdf, out8[4] = wi.x, out8[5] = wi.y, out8[6] = wi.z, out8[7] = specular ? 1.0f : 0.0f;
+}
+
+void ggx_probe(int which, const float* v3in, const float* m3, float ax, float ay, float r0, float r1, float* out3)
+{
+	const f3 v = mk3(v3in[0], v3in[1], v3in[2]), m = mk3(m3[0], m3[1], m3[2]);
+	if (which == 0) out3[0] = ggx_D(m, ax, ay);
+	else if (which == 1) out3[0] = ggx_pdf(v, m, ax, ay);
+	else if (which == 2) { const f3 h = ggx_sample(v, r0, r1, ax, ay); out3[0] = h.x, out3[1] = h.y, out3[2] = h.z; }
+	else if (which == 3) out3[0] = gtr1_D(m, ax);
+	else if (which == 4) { const f3 h = gtr1_sample(r0, r1, ax); out3[0] = h.x, out3[1] = h.y, out3[2] = h.z; }
+	else if (which == 5) out3[0] = ggx_G1(v, ax, ay);
+}
 uint normal_pack(const float* n3) { return pack_normal(mk3(n3[0], n3[1], n3[2])); }
 void normal_unpack(uint p, float* n3) { const f3 n = unpack_normal(p); n3[0] = n.x, n3[1] = n.y, n3[2] = n.z; }
 

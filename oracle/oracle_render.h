@@ -78,6 +78,7 @@ struct Render7Params
 	int w, h, spp, pass;
 	float eps, clampValue;
 	uint shift;
+	int bsdf;   /* 0 Lambert, 1 Disney */
 };
 
 /* One frame of the optix7 estimator (Lambert BSDF). camSeed is camRNGseed at
@@ -89,6 +90,11 @@ void blue_noise_default(uint* out, uint seed);
 void blue_noise_sample4(const RenderScene& s, int x, int y, int sampleIndex, int dim, float* out4);
 uint normal_pack(const float* n3);
 f3 sample_sky_small(const RenderScene& s, f3 D);
+void disney_probe_eval(const float* color3, const uint* params4, const float* transm3, const float* N3, const float* T3,
+	const float* wo3, const float* wi3, float* out4);
+void disney_probe_sample(const float* color3, const uint* params4, const float* transm3, const float* N3, const float* T3,
+	const float* wo3, const float* r3, float* out8);
+void ggx_probe(int which, const float* v3, const float* m3, float ax, float ay, float r0, float r1, float* out3);
 void normal_unpack(uint p, float* n3);
 
 /* Microbenchmark rays. */

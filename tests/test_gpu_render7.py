@@ -4,7 +4,8 @@ the oracle through the C ABI: fp64 accumulators identical, fp32 image within
 1e-4 relative of the fp64 oracle. Covers the frame state machine (blue-noise
 shift per frame, camera seed per executed wave), the random-number fallback
 past 64 samples, alpha cut-outs, the small skydome after a bounce, a distorted
-thin-lens camera and a scene without lights."""
+thin-lens camera, a scene without lights, and both BSDFs of the core (Lambert,
+and Disney: diffuse, sheen, GGX specular, GTR1 clear coat, rough dielectric)."""
 import numpy as np
 import pytest
 
@@ -17,11 +18,12 @@ pytestmark = pytest.mark.gpu
 REL_TOL = 1e-4
 
 
-def render_gpu7(scene, view, w, h, spp, frames, eps=1e-3, seed=0x12345678, tables=None, clamp=None, modes=None):
+def render_gpu7(scene, view, w, h, spp, frames, eps=1e-3, seed=0x12345678, tables=None, clamp=None, modes=None, bsdf=1):
     core = lh.RenderCore()
     lh.load_scene(core, scene)
     core.SetTarget(w, h, spp)
     core.Setting("estimator", 1)
+    core.Setting("bsdf", bsdf)
     core.Setting("epsilon", eps)
     if clamp is not None:
         core.Setting("clampValue", clamp)
@@ -50,11 +52,13 @@ def _tables(seed):
             rng.integers(0, 256, 131072, dtype=np.uint8))
 
 
-def test_cornell_box_default_tables(torch_cuda):
+@pytest.mark.parametrize("bsdf", [0, 1])
+def test_cornell_box_default_tables(torch_cuda, bsdf):
+    """Lambert (0) and Disney (1) BSDFs on the Lambert-parameterised box."""
     s = scenes.cornell_box(seed=1234, sphere_tris=8000)
     view = scenes.cornell_view(96, 96)
-    img, acc = render_gpu7(s, view, 96, 96, 1, 6)
-    r32, r64, n, _, _ = orc.render7(s, view, 96, 96, 1, 6, threads=16)
+    img, acc = render_gpu7(s, view, 96, 96, 1, 6, bsdf=bsdf)
+    r32, r64, n, _, _ = orc.render7(s, view, 96, 96, 1, 6, threads=16, bsdf=bsdf)
     print(f"radiance mean {(r64[..., :3] / n).mean():.5f}; connections per pixel {(r64[..., 3] / n).mean():.3f}")
     check(img, acc, r32, r64, n)
     assert (r64[..., 3] > 0).mean() > 0.5
@@ -71,6 +75,18 @@ def test_textured_box_past_64_samples(torch_cuda):
     r32, r64, n, _, _ = orc.render7(s, view, 40, 32, 8, 9, seed=4321, blue_noise=tables, threads=16)
     assert n == 72
     check(img, acc, r32, r64, n)
+
+
+def test_disney_materials(torch_cuda):
+    """Plastic, anisotropic metal, clear coat, sheen + subsurface and a rough
+    absorbing glass sphere through the Disney BSDF."""
+    s = scenes.disney_cornell_box(sphere_tris=6000)
+    view = scenes.cornell_view(80, 64)
+    img, acc = render_gpu7(s, view, 80, 64, 2, 4, seed=777)
+    r32, r64, n, _, _ = orc.render7(s, view, 80, 64, 2, 4, seed=777, threads=16)
+    print(f"radiance mean {(r64[..., :3] / n).mean():.5f}")
+    check(img, acc, r32, r64, n)
+    assert np.isfinite(r64).all() and (r64[..., :3] > 0).mean() > 0.5
 
 
 def test_state_machine_and_clamp(torch_cuda):
@@ -116,5 +132,5 @@ def test_uniform_sky_over_plane(torch_cuda):
     s.add_instance(s.add_mesh(v, t))
     s.sky = (np.full((128, 256, 3), 0.75, np.float32), F.IDENTITY.copy())
     view = scenes.camera_view((0.0, 1.0, -2.0), (0.0, 0.0, 0.0), 40.0, 32, 32)
-    img, acc = render_gpu7(s, view, 32, 32, 2, 2)
+    img, acc = render_gpu7(s, view, 32, 32, 2, 2, bsdf=0)
     assert np.allclose(acc[..., :3] / 4, a * 0.75, rtol=2e-6, atol=0)

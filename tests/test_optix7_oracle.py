@@ -1,5 +1,5 @@
 """The oracle's optix7 estimator (rendercore_optix7 shadeKernel with the
-shared Lambert BSDF) pinned by what the mathematics fixes: normal packing
+shared Lambert BSDF, setting bsdf 0) pinned by what the mathematics fixes: normal packing
 inverts, the blue-noise sampler reduces to its sequence table and ranks by
 permutation, the small skydome is a 64x64 box average, a white-furnace-style
 scene (uniform sky over a plane) returns albedo x sky radiance for every
@@ -109,7 +109,7 @@ def test_uniform_sky_over_plane(roughness):
     L = 0.75
     s = _floor_scene(a, roughness, L)
     view = scenes.camera_view((0.0, 1.0, -2.0), (0.0, 0.0, 0.0), 40.0, 32, 32)
-    r32, r64, n, _, _ = orc.render7(s, view, 32, 32, spp=2, frames=2, threads=8)
+    r32, r64, n, _, _ = orc.render7(s, view, 32, 32, spp=2, frames=2, threads=8, bsdf=0)
     img = r64[..., :3] / n
     assert np.allclose(img, a * L, rtol=2e-6, atol=0)
     assert np.all(r64[..., 3] == 0)
@@ -158,7 +158,7 @@ def test_triangle_emitter_form_factor():
     scenes.add_emitters(s, t, (Le, Le, Le), li)
     # narrow view from below the emitter straight down at the origin
     view = scenes.camera_view((0.0, 0.5, 0.0001), (0.0, 0.0, 0.0), 1.0, 32, 32)
-    r32, r64, n, _, _ = orc.render7(s, view, 32, 32, spp=16, frames=1, threads=8)
+    r32, r64, n, _, _ = orc.render7(s, view, 32, 32, spp=16, frames=1, threads=8, bsdf=0)
     est = r64[..., 0] / n
     want = a * Le * _form_factor_polygon(np.zeros(3), tri.astype(float), np.array([0, 1.0, 0]))
     err = abs(est.mean() - want) / want
