@@ -98,6 +98,13 @@ LH2HIP_API int lh2hip_wait(lh2hip_core* core);
 LH2HIP_API int lh2hip_read_image(lh2hip_core* core, lh2_float4* hostOut);
 /* Reads the FP64 accumulator (sum of all samples, 4 doubles per pixel). */
 LH2HIP_API int lh2hip_read_accumulator_f64(lh2hip_core* core, double* hostOut);
+/* Internal frame buffers (no reference counterpart; the reference keeps them
+ * on the device): 0 fp64 accumulators, both halves (2 * w * h * 4 doubles:
+ * direct, then indirect radiance of the filter estimator), 1 SVGF features
+ * (w * h uint4, RenderCore_Optix7Filter/kernels/pathtracer.h:44-48), 2 world
+ * positions (w * h float4), 3 depth derivatives (w * h float4), 4 fp32
+ * accumulators (2 * w * h float4). bytes must equal the buffer size. */
+LH2HIP_API int lh2hip_read_buffer(lh2hip_core* core, int which, void* hostOut, size_t bytes);
 /* CoreAPI_Base::GetCoreStats (core_api_base.h:87). */
 LH2HIP_API int lh2hip_get_stats(lh2hip_core* core, CoreStats* out);
 

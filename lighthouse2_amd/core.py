@@ -44,6 +44,7 @@ _SIGS = {
     "lh2hip_set_textures": (I, [P, P, I]),
     "lh2hip_set_sky": (I, [P, P, I, I, P]),
     "lh2hip_set_blue_noise": (I, [P, P, P, P]),
+    "lh2hip_read_buffer": (I, [P, I, P, ctypes.c_size_t]),
     "lh2hip_set_probe_pos": (I, [P, I, I]),
     "lh2hip_render": (I, [P, P, I, I]),
     "lh2hip_wait": (I, [P]),
@@ -212,6 +213,19 @@ class RenderCore:
     def read_accumulator_f64(self):
         out = np.zeros((self.height, self.width, 4), np.float64)
         _check(library().lh2hip_read_accumulator_f64(self._h, _ptr(out)))
+        return out
+
+    def read_buffer(self, which):
+        """Internal frame buffers: 'acc64' (2, h, w, 4) float64 (direct, indirect halves),
+        'features' (h, w, 4) uint32, 'world_pos' / 'delta_depth' (h, w, 4) float32,
+        'acc32' (2, h, w, 4) float32."""
+        k, shape, dt = {"acc64": (0, (2, self.height, self.width, 4), np.float64),
+                        "features": (1, (self.height, self.width, 4), np.uint32),
+                        "world_pos": (2, (self.height, self.width, 4), np.float32),
+                        "delta_depth": (3, (self.height, self.width, 4), np.float32),
+                        "acc32": (4, (2, self.height, self.width, 4), np.float32)}[which]
+        out = np.zeros(shape, dt)
+        _check(library().lh2hip_read_buffer(self._h, k, _ptr(out), out.nbytes))
         return out
 
     def trace_closest(self, rays_ptr, hits_ptr, count, stream=None):

@@ -167,6 +167,11 @@ LH2HIP_API int lh2hip_read_accumulator_f64(lh2hip_core* core, double* out)
 	return guarded(core, [&](RenderCore& c) { c.ReadAccumulatorF64(out); });
 }
 
+LH2HIP_API int lh2hip_read_buffer(lh2hip_core* core, int which, void* out, size_t bytes)
+{
+	return guarded(core, [&](RenderCore& c) { c.ReadBuffer(which, out, bytes); });
+}
+
 LH2HIP_API int lh2hip_get_stats(lh2hip_core* core, CoreStats* out)
 {
 	return guarded(core, [&](RenderCore& c) {

@@ -10,6 +10,7 @@
  * copies the counters to the host after every bounce, rendercore.cpp:592).
  */
 #include "rendercore.h"
+#include "svgf.h"
 #include "../../include/lh2hip.h"
 #include <chrono>
 #include <cstring>
@@ -100,8 +101,11 @@ void Wavefront::resizeTarget(int pixels, hipStream_t s)
 {
 	if (pixels <= pixelCap) return;
 	pixelCap = pixels + (pixels >> 4);
-	acc32B.ensure(sizeof(float4) * pixelCap);
-	acc64B.ensure(sizeof(double) * 4 * pixelCap);
+	/* two halves: the filter estimator keeps direct and indirect radiance apart */
+	acc32B.ensure(sizeof(float4) * 2 * pixelCap);
+	acc64B.ensure(sizeof(double) * 8 * pixelCap);
+	gFeaturesB.ensure(16 * (size_t)pixelCap), gWorldPosB.ensure(16 * (size_t)pixelCap), gDeltaDepthB.ensure(16 * (size_t)pixelCap);
+	hipCheck(hipMemsetAsync(gFeaturesB.p, 0, gFeaturesB.bytes, s), "memset");
 	imageB.ensure(sizeof(float4) * pixelCap);
 	probeB.ensure(16);
 	countLogB.ensure(sizeof(uint32_t) * 1100);
@@ -121,7 +125,7 @@ void Wavefront::resizePaths(size_t n, hipStream_t s)
 	pathCap = n;
 	for (int i = 0; i < 2; i++)
 		rays[i].ensure(32 * n), state[i].ensure(16 * n), pathIdB[i].ensure(4 * n), auxB[i].ensure(8 * n);
-	hitsB.ensure(16 * n), shadowRaysB.ensure(32 * n), shadowPotB.ensure(16 * n), occludedB.ensure(4 * n), pathAccB.ensure(16 * n);
+	hitsB.ensure(16 * n), shadowRaysB.ensure(32 * n), shadowPotB.ensure(16 * n), occludedB.ensure(4 * n), pathAccB.ensure(32 * n);
 	countersB.ensure(64);
 	countLogB.ensure(sizeof(uint32_t) * 1100);   /* one slot per wave, maxDepth <= 1024 */
 	hipCheck(hipMemsetAsync(countersB.p, 0, 64, s), "memset");
@@ -132,6 +136,7 @@ void Wavefront::release()
 	for (int i = 0; i < 2; i++) rays[i].release(), state[i].release(), pathIdB[i].release(), auxB[i].release();
 	hitsB.release(), shadowRaysB.release(), shadowPotB.release(), occludedB.release(), pathAccB.release();
 	countersB.release(), countLogB.release(), probeB.release(), acc32B.release(), acc64B.release(), imageB.release();
+	gFeaturesB.release(), gWorldPosB.release(), gDeltaDepthB.release();
 	pathCap = 0, pixelCap = 0;
 }
 
@@ -182,6 +187,7 @@ void RenderCore::Shutdown()
 	tlasNodes.release(), instRecs.release(), shadeInsts.release(), overflow.release(), counters.release();
 	materials.release(), triLights.release(), pointLights.release(), spotLights.release(), dirLights.release();
 	wf.release();
+	svgf.reset();
 	for (int i = 0; i < 8; i++) (void)hipEventDestroy(ev[i]);
 	(void)hipStreamDestroy(stream);
 	stream = nullptr;
@@ -248,8 +254,15 @@ void RenderCore::Setting(const char* name, float value)
 	else if (!strcmp(name, "seed")) seedSetting = camSeed = (uint32_t)value; /* replaces camRNGseed's 0x12345678 */
 	else if (!strcmp(name, "estimator"))
 	{	/* 0: RenderCore_PrimeRef's shadeKernel, 1: rendercore_optix7's (Lambert BSDF) */
-		if (value != 0 && value != 1) throw CoreError(LH2HIP_ERR_INVALID, "Setting estimator: 0 (reference) or 1 (optix7)");
+		if (value != 0 && value != 1 && value != 2) throw CoreError(LH2HIP_ERR_INVALID, "Setting estimator: 0 (reference), 1 (optix7) or 2 (optix7 filter)");
 		estimator = (int)value;
+	}
+	else if (!strcmp(name, "filter")) filterOn = value != 0;          /* RenderCore_Optix7Filter/rendercore.cpp:676 */
+	else if (!strcmp(name, "clampDirect")) clampDirect = value;      /* :674 */
+	else if (!strcmp(name, "clampIndirect")) clampIndirect = value;  /* :675 */
+	else if (!strcmp(name, "TAA"))
+	{	/* :677; the TAA pass is not built: only 0 is accepted */
+		if (value != 0) throw CoreError(LH2HIP_ERR_INVALID, "Setting TAA: only 0 (TAA pass not available)");
 	}
 	else if (!strcmp(name, "bsdf"))
 	{	/* optix7 estimator's BSDF: the compile-time choice of kernels/bsdf.h:7-22 */
@@ -744,7 +757,7 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
 	P.bsdf = bsdfSetting;
 	P.connections = (P.nTri + P.nPoint + P.nSpot + P.nDir) > 0;   /* noDirectLightsInScene, rendercore_optix7/rendercore.cpp:708 */
 	f.estimator = estimator;
-	if (estimator == ESTIMATOR_OPTIX7)
+	if (estimator == ESTIMATOR_OPTIX7 || estimator == ESTIMATOR_FILTER)
 	{
 		if (!blueNoise.p)
 		{
@@ -755,10 +768,24 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
 		}
 		P.blueNoise = blueNoise.as<uint32_t>();
 		P.shift = shiftSeed;
-		f.maxDepth = MAXPATHLENGTH_OPTIX7;
+		f.maxDepth = estimator == ESTIMATOR_OPTIX7 ? MAXPATHLENGTH_OPTIX7 : MAXPATHLENGTH_FILTER;
+	}
+	if (estimator == ESTIMATOR_FILTER)
+	{	/* rendercore.cpp:780-797: with the filter on, aperture 0 and no distortion */
+		P.blueSlot = blueSlot;
+		if (filterOn && w == scrw && h == scrh)
+		{
+			P.gFeatures = wf.gFeatures(), P.gWorldPos = wf.gWorldPos(), P.gDeltaDepth = wf.gDeltaDepth();
+			f.cam.aperture = 0;
+		}
+		f.cam.distortion = 0;
+		P.camPos = make_float4(view.pos.x, view.pos.y, view.pos.z, 0);
+		P.camP1 = make_float4(view.p1.x, view.p1.y, view.p1.z, 0);
+		P.camRight = make_float4(view.p2.x - view.p1.x, view.p2.y - view.p1.y, view.p2.z - view.p1.z, 0);
+		P.camUp = make_float4(view.p3.x - view.p1.x, view.p3.y - view.p1.y, view.p3.z - view.p1.z, 0);
 	}
 	f.R0gen = R0gen, f.samplesTaken = samplesTakenAtStart, f.spp = spp, f.pixels = w * h;
-	if (estimator != ESTIMATOR_OPTIX7) f.maxDepth = depth;
+	if (estimator == ESTIMATOR_REFERENCE) f.maxDepth = depth;
 	return f;
 }
 
@@ -773,14 +800,15 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 	FrameSetup f = f0;
 	const TraceScene& sc = scene;
 	const int pixels = f.pixels;
-	const bool o7 = f.estimator == ESTIMATOR_OPTIX7;
-	/* optix7: R0 of shade wave k is RandomUInt(camRNGseed) + k * 91771, the
-	 * seed advancing once per wave the frame runs (rendercore.cpp:881-905) */
-	uint32_t R0wave[MAXPATHLENGTH_OPTIX7 + 1] = { 0 };
-	if (o7)
+	const bool o7 = f.estimator == ESTIMATOR_OPTIX7, fl = f.estimator == ESTIMATOR_FILTER, mis = o7 || fl;
+	const bool gbuf = f.P.gFeatures != nullptr;
+	/* optix7 cores: R0 of shade wave k is RandomUInt(camRNGseed) + k * 91771,
+	 * the seed advancing once per wave the frame runs (rendercore.cpp:881-905) */
+	uint32_t R0wave[MAXPATHLENGTH_FILTER + 1] = { 0 };
+	if (mis)
 	{
 		uint32_t cs = f.camSeed;
-		for (int k = 1; k <= MAXPATHLENGTH_OPTIX7; k++) R0wave[k] = xor32_host(cs) + (uint32_t)k * 91771u;
+		for (int k = 1; k <= f.maxDepth; k++) R0wave[k] = xor32_host(cs) + (uint32_t)k * 91771u;
 	}
 	int deepest = 0;
 	const int passSamples = std::max(1, std::min(s1 - s0, (int)(std::min(passCap, Wavefront::PASS_CAP) / (size_t)pixels)));
@@ -791,7 +819,9 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 		const int n = k * pixels;
 		PathBuffers B = w.buffers();
 		f.P.jobBase = (uint32_t)a * (uint32_t)pixels;
+		f.P.passPaths = (uint32_t)n;
 		if (o7) hipCheck(launch_generate7(f.cam, f.P, f.P.jobBase, n, geometryEpsilon, B, 0, st), "generate");
+		else if (fl) hipCheck(launch_generateF(f.cam, f.P, f.P.jobBase, n, geometryEpsilon, B, 0, st), "generate");
 		else hipCheck(launch_generate(f.cam, f.R0gen, f.P.jobBase, n, geometryEpsilon, B, 0, st), "generate");
 		int in = 0;
 		for (int pathLength = 1; pathLength <= f.maxDepth; pathLength++)
@@ -800,10 +830,11 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 			hipCheck(launch_trace_dev(sc, MODE_CLOSEST, B.rays[in], B.hits, n, B.counters + 0, queue, ovf, persistentBlocks, st), "extend");
 			if (evs) hipCheck(hipEventRecord(evs[1], st), "event");
 			f.P.pathLength = pathLength;
-			if (o7)
+			if (mis)
 			{
 				f.P.R0 = R0wave[pathLength];
-				hipCheck(launch_shade7(f.P, B, in, n, st), "shade");
+				if (o7) hipCheck(launch_shade7(f.P, B, in, n, st), "shade");
+				else hipCheck(launch_shadeF(f.P, B, in, n, st), "shade");
 			}
 			else
 			{
@@ -813,10 +844,10 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 			if (evs) hipCheck(hipEventRecord(evs[2], st), "event");
 			/* the reference estimator connects from every vertex but the last;
 			 * optix7's from every vertex (its NEE precedes the depth cut) */
-			if (o7 || pathLength < f.maxDepth)
+			if (mis || pathLength < f.maxDepth)
 			{
 				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, B.counters + 2, queue, ovf, persistentBlocks, st), "connect");
-				hipCheck(launch_connect(B, n, st, o7 ? 1.0f : 0.0f), "connect");
+				hipCheck(launch_connect(B, n, st, mis ? 1.0f : 0.0f), "connect");
 			}
 			if (evs) hipCheck(hipEventRecord(evs[3], st), "event");
 			hipCheck(launch_advance(B.counters, w.countLog() ? w.countLog() + pathLength : nullptr, st), "advance");
@@ -824,7 +855,7 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 			deepest = std::max(deepest, pathLength);
 			/* stop early once every path has terminated (checked every 4 waves;
 			 * after every wave for optix7, whose loop reads the count back each time) */
-			if (o7 || (pathLength & 3) == 0 || pathLength == f.maxDepth)
+			if (mis || (pathLength & 3) == 0 || pathLength == f.maxDepth)
 			{
 				uint32_t live = 0;
 				hipCheck(hipMemcpyAsync(&live, B.counters + 0, 4, hipMemcpyDeviceToHost, st), "D2H");
@@ -839,6 +870,7 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 			}
 		}
 		hipCheck(launch_resolve(B.pathAcc, pixels, k, acc32, acc64, st), "resolve");
+		if (gbuf) hipCheck(launch_resolve(B.pathAcc + n, pixels, k, acc32 + pixels, acc64 + 4 * (size_t)pixels, st), "resolve");
 	}
 	return deepest;
 }
@@ -855,12 +887,12 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 		wf.clearAccumulators(stream);
 		samplesTaken = 0;
 		firstConvergingFrame = true;
-		camSeed = seedSetting;
+		if (estimator != ESTIMATOR_FILTER) camSeed = seedSetting;   /* the filter core keeps its seed (RenderCore_Optix7Filter/rendercore.cpp:749) */
 	}
 	if (converge == Converge) firstConvergingFrame = false;
 	uint32_t R0gen = 0;
 	if (estimator == ESTIMATOR_OPTIX7) xor32_host(shiftSeed);   /* once per frame, rendercore_optix7/rendercore.cpp:855 */
-	else R0gen = xor32_host(camSeed);
+	else if (estimator == ESTIMATOR_REFERENCE) R0gen = xor32_host(camSeed);   /* RenderCore_PrimeRef/rendercore.cpp:556 */
 	FrameSetup f = frameParams(view, scrw, scrh, scrspp, maxDepth, R0gen, samplesTaken);
 	f.camSeed = camSeed;
 	f.P.probePixel = probePos.x + scrw * probePos.y;
@@ -868,10 +900,26 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	float timing[5] = { 0, 0, 0, 0, 0 };
 	const int waves = runSamples(f, 0, scrspp, Wavefront::PASS_CAP, wf, wf.acc32(), wf.acc64(), counters.as<uint32_t>() + TRACE_QUEUE_WORDS,
 		overflow.as<uint32_t>(), stream, ev, timing);
-	if (estimator == ESTIMATOR_OPTIX7) for (int k = 0; k < waves; k++) xor32_host(camSeed);
+	if (estimator != ESTIMATOR_REFERENCE) for (int k = 0; k < waves; k++) xor32_host(camSeed);
 	const int pixels = scrw * scrh;
 	samplesTaken += scrspp;
-	hipCheck(launch_finalize(wf.acc32(), pixels, 1.0f / (float)samplesTaken, wf.image(), stream), "finalize");
+	bool filtered = false;
+	if (estimator == ESTIMATOR_FILTER && f.P.gFeatures)
+	{	/* FinalizeRender (RenderCore_Optix7Filter/rendercore.cpp:897-935), TAA off */
+		blueSlot = (blueSlot + 1) & 255;
+		if (!svgf || svgf->w != scrw || svgf->h != scrh) svgf.reset(new Svgf(scrw, scrh));
+		hipCheck(hipEventRecord(ev[4], stream), "event");
+		svgf->Frame(wf.acc32(), wf.gFeatures(), wf.gWorldPos(), wf.gDeltaDepth(), prevView, 0, 0, 0, 0, samplesTaken, clampDirect, clampIndirect,
+			samplesTaken == scrspp ? 0 : 1, wf.image(), stream);
+		hipCheck(hipEventRecord(ev[5], stream), "event");
+		filtered = true;
+	}
+	else
+	{
+		if (estimator == ESTIMATOR_FILTER) blueSlot = (blueSlot + 1) & 255;
+		hipCheck(launch_finalize(wf.acc32(), pixels, 1.0f / (float)samplesTaken, wf.image(), stream), "finalize");
+	}
+	prevView = view;
 	if (!async)
 	{
 		hipCheck(hipStreamSynchronize(stream), "sync");
@@ -879,6 +927,8 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 		hipCheck(hipMemcpy(probe, wf.probe(), 12, hipMemcpyDeviceToHost), "probe");
 		stats.probedInstid = probe[0], stats.probedTriid = probe[1];
 		memcpy(&stats.probedDist, &probe[2], 4);
+		float ms = 0;
+		if (filtered && hipEventElapsedTime(&ms, ev[4], ev[5]) == hipSuccess) stats.filterTime = ms * 1e-3f;
 	}
 	const auto t1 = std::chrono::high_resolution_clock::now();
 	stats.renderTime = std::chrono::duration<float>(t1 - t0).count();
@@ -897,6 +947,24 @@ void RenderCore::ReadImage(void* hostOut)
 	if (!hostOut) throw CoreError(LH2HIP_ERR_INVALID, "read_image: null");
 	if (scrw == 0) throw CoreError(LH2HIP_ERR_STATE, "read_image: no target");
 	hipCheck(hipMemcpyAsync(hostOut, wf.image(), sizeof(float4) * scrw * scrh, hipMemcpyDeviceToHost, stream), "D2H");
+	hipCheck(hipStreamSynchronize(stream), "sync");
+}
+
+void RenderCore::ReadBuffer(int which, void* hostOut, size_t bytes)
+{
+	if (!hostOut) throw CoreError(LH2HIP_ERR_INVALID, "read_buffer: null");
+	if (scrw == 0) throw CoreError(LH2HIP_ERR_STATE, "read_buffer: no target");
+	const size_t px = (size_t)scrw * scrh;
+	const void* src = nullptr;
+	size_t n = 0;
+	if (which == 0) src = wf.acc64(), n = px * 8 * sizeof(double);
+	else if (which == 1) src = wf.gFeatures(), n = px * 16;
+	else if (which == 2) src = wf.gWorldPos(), n = px * 16;
+	else if (which == 3) src = wf.gDeltaDepth(), n = px * 16;
+	else if (which == 4) src = wf.acc32(), n = px * 2 * sizeof(float4);
+	else throw CoreError(LH2HIP_ERR_INVALID, "read_buffer: which 0..4");
+	if (bytes != n) throw CoreError(LH2HIP_ERR_INVALID, "read_buffer: size mismatch (expected " + std::to_string(n) + " bytes)");
+	hipCheck(hipMemcpyAsync(hostOut, src, n, hipMemcpyDeviceToHost, stream), "D2H");
 	hipCheck(hipStreamSynchronize(stream), "sync");
 }
 

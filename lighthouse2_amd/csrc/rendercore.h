@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 #include <stdexcept>
+#include <memory>
 
 namespace lh {
 
@@ -46,6 +47,7 @@ struct Wavefront
 	static constexpr size_t PASS_CAP = (size_t)1 << 24;
 	DevBuf rays[2], state[2], pathIdB[2], auxB[2], hitsB, shadowRaysB, shadowPotB, occludedB, pathAccB, countersB, countLogB, probeB;
 	DevBuf acc32B, acc64B, imageB;
+	DevBuf gFeaturesB, gWorldPosB, gDeltaDepthB;   /* SVGF G-buffer of the filter estimator */
 	size_t pathCap = 0;
 	int pixelCap = 0;
 	void resizeTarget(int pixels, hipStream_t s);
@@ -58,6 +60,9 @@ struct Wavefront
 	double* acc64() { return acc64B.as<double>(); }
 	float4* image() { return imageB.as<float4>(); }
 	uint32_t* countLog() { return countLogB.as<uint32_t>(); }
+	uint4* gFeatures() { return gFeaturesB.as<uint4>(); }
+	float4* gWorldPos() { return gWorldPosB.as<float4>(); }
+	float4* gDeltaDepth() { return gDeltaDepthB.as<float4>(); }
 };
 
 struct MeshData
@@ -88,7 +93,9 @@ struct FrameSetup
 	uint32_t camSeed = 0;
 };
 
-enum { ESTIMATOR_REFERENCE = 0, ESTIMATOR_OPTIX7 = 1 };
+enum { ESTIMATOR_REFERENCE = 0, ESTIMATOR_OPTIX7 = 1, ESTIMATOR_FILTER = 2 };
+
+class Svgf;
 
 class RenderCore : public lighthouse2::CoreAPI_Base
 {
@@ -129,6 +136,7 @@ public:
 		void* raysOut, uint32_t* pathOut, uint32_t* countOut, hipStream_t s);
 	void ReadImage(void* hostOut);
 	void ReadAccumulatorF64(double* hostOut);
+	void ReadBuffer(int which, void* hostOut, size_t bytes);
 	void BvhInfo(int mesh, int64_t* out8) const;
 	int BvhDownload(int mesh, void* nodesOut, int maxNodes, uint32_t* primsOut, int maxPrims) const;
 	const TraceScene& Scene();
@@ -170,6 +178,12 @@ private:
 	uint32_t camSeed = 0x12345678u, seedSetting = 0x12345678u;
 	int estimator = ESTIMATOR_REFERENCE;
 	int bsdfSetting = 1;               /* Disney, as rendercore_optix7 ships (kernels/bsdf.h) */
+	/* filter estimator (RenderCore_Optix7Filter/rendercore.h:32-35,124) */
+	bool filterOn = true;
+	float clampDirect = 2.5f, clampIndirect = 15.0f;
+	int blueSlot = 0;
+	lighthouse2::ViewPyramid prevView = {};
+	std::unique_ptr<Svgf> svgf;
 	uint32_t shiftSeed = 0x11331445u;  /* blue-noise shift of the optix7 estimator (rendercore_optix7/rendercore.h:123) */
 	DevBuf blueNoise;
 	/* render state */

@@ -59,6 +59,15 @@ struct ShadeParams
 	float clampValue;
 	int connections;
 	int bsdf;                     /* optix7 estimator: 0 Lambert (sharedBSDFs/lambert.h), 1 Disney (sharedBSDFs/disney.h) */
+	/* filter estimator (RenderCore_Optix7Filter): G-buffer of the first diffuse
+	 * vertex (null: filter off), blue-noise slot, path slots per pass (indirect
+	 * radiance goes to slot + passPaths), camera for the depth derivatives */
+	uint4* gFeatures;
+	float4* gWorldPos;
+	float4* gDeltaDepth;
+	int blueSlot;
+	uint32_t passPaths;
+	float4 camPos, camP1, camRight, camUp;
 };
 
 /* Path-state buffers of one pass (SoA). */
@@ -92,12 +101,21 @@ hipError_t launch_shade7(const ShadeParams& p, const PathBuffers& b, int in, int
 /* Blue-noise tables in the layout of Heitz et al.'s sampler: [0, 65536)
  * sequence values dimension + 256 * index, [65536, 196608) scrambling tile
  * and [196608, 327680) ranking tile, 8 entries per texel of a 128 x 128 tile,
- * plus 16 zero entries of padding. Filled by a seeded generator. */
+ * plus 64 zero entries of padding (the ranking reads of dimensions above 7
+ * at the last texel run past the tables). Filled by a seeded generator. */
 void blue_noise_tables(uint32_t* out, uint32_t seed);
-static const int BLUE_NOISE_WORDS = 65536 * 5 + 16;
+static const int BLUE_NOISE_WORDS = 65536 * 5 + 64;
 static const uint32_t BLUE_NOISE_SEED = 0x2545f491u;
 /* MAXPATHLENGTH of rendercore_optix7 (core_settings.h:25) */
 static const int MAXPATHLENGTH_OPTIX7 = 3;
+/* filter estimator (lib/RenderCore_Optix7Filter): pinhole pixel-corner eye
+ * rays with the filter on, blue-noise jittered thin-lens rays with it off
+ * (optix/.optix.cu:66-129), and the shade kernel that also writes the SVGF
+ * G-buffer and splits direct and indirect radiance (kernels/pathtracer.h:76-306). */
+hipError_t launch_generateF(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s);
+hipError_t launch_shadeF(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s);
+/* MAXPATHLENGTH of RenderCore_Optix7Filter (core_settings.h:25) */
+static const int MAXPATHLENGTH_FILTER = 7;
 /* counters[0] = counters[1] (next wave size, also logged), counters[1] = counters[2] = 0. */
 hipError_t launch_advance(uint32_t* counters, uint32_t* logSlot, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t s);

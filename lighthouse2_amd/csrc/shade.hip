@@ -898,6 +898,324 @@ hipError_t launch_shade7(const ShadeParams& p, const PathBuffers& b, int in, int
 	return hipGetLastError();
 }
 
+/* ==== filter estimator ======================================================
+ * lib/RenderCore_Optix7Filter: generateEyeRay / setupPrimaryRay
+ * (optix/.optix.cu:66-129), shadeKernel (kernels/pathtracer.h:76-306) with
+ * PackFeatures / calculateDepthDerivatives (:44-58), the BSDF of the core
+ * (Disney by default, kernels/bsdf.h), MAXPATHLENGTH 7, one diffuse bounce.
+ * Radiance of paths that bounced off a diffuse vertex goes to the indirect
+ * half (path slot + passPaths) when the G-buffer is on. */
+#define SF_SPECULAR 1u
+#define SF_BOUNCED 2u
+#define SF_VIASPECULAR 4u
+
+/* blueNoiseSampler (optix/.optix.cu:40-54) */
+__device__ __forceinline__ float blue_noise1(const uint32_t* bn, int x, int y, int sampleIndex, int dim)
+{
+	x &= 127, y &= 127, sampleIndex &= 255, dim &= 255;
+	const int ranked = (int)((uint32_t)(sampleIndex ^ (int)bn[dim + (x + y * 128) * 8 + 65536 * 3]) & 255u);
+	uint32_t value = bn[dim + ranked * 256];
+	value ^= bn[(dim & 7) + (x + y * 128) * 8 + 65536];
+	return (0.5f + (float)value) * (1.0f / 256.0f);
+}
+
+/* PackNormal2, HDRtoRGB32, RGB32toHDR (tools_shared.h:122-129,162-175) */
+__device__ __forceinline__ uint32_t pack_normal2(v3 N)
+{
+	const uint32_t x = min(f2u_sat((N.x + 1) * 511), 1023u), y = min(f2u_sat((N.y + 1) * 511), 1023u), z = min(f2u_sat((N.z + 1) * 511), 1023u);
+	return (x << 2u) + (y << 12u) + (z << 22u);
+}
+__device__ __forceinline__ uint32_t hdr_to_rgb32(v3 c)
+{
+	const uint32_t r = f2u_sat(1023.0f * fminf(1.0f, c.x)), g = f2u_sat(2047.0f * fminf(1.0f, c.y)), b = f2u_sat(2047.0f * fminf(1.0f, c.z));
+	return (r << 22) + (g << 11) + b;
+}
+__device__ __forceinline__ v3 rgb32_to_hdr(uint32_t c)
+{
+	return mk3((float)(c >> 22) * (1.0f / 1023.0f), (float)((c >> 11) & 2047u) * (1.0f / 2047.0f), (float)(c & 2047u) * (1.0f / 2047.0f));
+}
+__device__ __forceinline__ void pack_features(uint4* f, uint32_t albedo, uint32_t packedNormal, float t, uint32_t isSpecular, uint32_t matid)
+{
+	*f = make_uint4(albedo, packedNormal, fbits(t), (isSpecular << 4) + (matid << 6) + (f->w & 15u));
+}
+/* calculateDepthDerivatives: rays through (x+1, y) and (x, y+1) against the
+ * object-space plane of the triangle */
+__device__ void depth_derivatives(const ShadeParams& P, int x, int y, float depth, const float4* tri, float4* dd)
+{
+	const v3 triN = mk3(tri[2].w, tri[3].w, tri[4].w);
+	const v3 p1 = xyz(P.camP1), p2 = xyz(P.camRight), p3 = xyz(P.camUp), pos = xyz(P.camPos), v0 = xyz(tri[8]);
+	const float iw = 1.0f / (float)P.w, ih = 1.0f / (float)P.h;
+	const v3 dxDir = normalize((p1 + ((float)x + 0.5f + 1) * iw * p2 + ((float)y + 0.5f) * ih * p3) - pos);
+	const float dxDepth = dot(v0 - pos, triN) / dot(triN, dxDir);
+	const v3 dyDir = normalize((p1 + ((float)x + 0.5f) * iw * p2 + ((float)y + 0.5f + 1) * ih * p3) - pos);
+	const float dyDepth = dot(v0 - pos, triN) / dot(triN, dyDir);
+	dd[x + y * P.w] = make_float4(0, 0, dxDepth - depth, dyDepth - depth);
+}
+
+__global__ void k_generateF(CamParams c, ShadeParams P, uint32_t jobBase, int n, float eps, float4* __restrict__ rays,
+	float4* __restrict__ state, uint32_t* __restrict__ pathId, float2* __restrict__ aux, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j == 0) counters[0] = (uint32_t)n, counters[1] = 0, counters[2] = 0;
+	if (j >= n) return;
+	const uint32_t pathIdx = jobBase + (uint32_t)j;
+	const uint32_t wh = (uint32_t)(c.w * c.h);
+	const uint32_t pixelIdx = pathIdx % wh, sampleIdx = pathIdx / wh + (uint32_t)P.pass;
+	const int sx = (int)(pixelIdx % (uint32_t)c.w), sy = (int)(pixelIdx / (uint32_t)c.w);
+	const v3 pos = mk3(c.pos[0], c.pos[1], c.pos[2]), p1 = mk3(c.p1[0], c.p1[1], c.p1[2]);
+	const v3 right = mk3(c.p2[0], c.p2[1], c.p2[2]) - p1, up = mk3(c.p3[0], c.p3[1], c.p3[2]) - p1;
+	float r0 = 0, r1 = 0;
+	v3 O = pos;
+	if (!P.gFeatures)
+	{	/* filter off: jittered pixel and lens */
+		uint32_t seed = wang_hash(pathIdx * 16789u + (uint32_t)P.pass * 1791u);
+		float r2, r3;
+		if (sampleIdx < 256)
+			r0 = blue_noise1(P.blueNoise, sx, sy, (int)sampleIdx, 0), r1 = blue_noise1(P.blueNoise, sx, sy, (int)sampleIdx, 1),
+			r2 = blue_noise1(P.blueNoise, sx, sy, (int)sampleIdx, 2), r3 = blue_noise1(P.blueNoise, sx, sy, (int)sampleIdx, 3);
+		else
+			r0 = rand_float(seed), r1 = rand_float(seed), r2 = rand_float(seed), r3 = rand_float(seed);
+		O = point_on_lens(r2, r3, pos, c.aperture, right, up);
+	}
+	const float u = ((float)sx + r0) * (1.0f / (float)c.w), v = ((float)sy + r1) * (1.0f / (float)c.h);
+	const v3 target = p1 + u * right + v * up;
+	const v3 D = normalize(target - O);
+	rays[2 * j] = make_float4(O.x, O.y, O.z, eps);
+	rays[2 * j + 1] = make_float4(D.x, D.y, D.z, 1e34f);
+	state[j] = make_float4(1.0f, 1.0f, 1.0f, bitsf(SF_SPECULAR));
+	aux[j] = make_float2(1.0f, bitsf(0u));
+	pathId[j] = pathIdx;
+	pathAcc[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	pathAcc[j + n] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ void __launch_bounds__(128) k_shadeF(const ShadeParams P, const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
+	const float2* __restrict__ auxIn, const uint32_t* __restrict__ pathIn, const float4* __restrict__ hits, float4* __restrict__ raysOut,
+	float4* __restrict__ stateOut, float2* __restrict__ auxOut, uint32_t* __restrict__ pathOut, float4* __restrict__ shadowRays,
+	float4* __restrict__ shadowPot, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= (int)counters[0]) return;
+	const float4 O4 = raysIn[2 * j], D4 = raysIn[2 * j + 1], T4 = stateIn[j];
+	const float2 A2 = auxIn[j];
+	const float4 hd = hits[j];
+	const uint32_t pathIdx = pathIn[j];
+	uint32_t flags = fbits(T4.w);
+	const bool gbuf = P.gFeatures != nullptr;
+	const uint32_t wh = (uint32_t)(P.w * P.h);
+	const uint32_t pixelIdx = (pathIdx % wh) + ((gbuf && (flags & SF_BOUNCED)) ? wh : 0u);
+	const uint32_t sampleIdx = pathIdx / wh + (uint32_t)P.pass;
+	const uint32_t slot = (pathIdx - P.jobBase) + ((gbuf && (flags & SF_BOUNCED)) ? P.passPaths : 0u);
+	float4* acc = pathAcc + slot;
+	const v3 D = xyz(D4);
+	v3 throughput = xyz(T4);
+	const float bsdfPdf = A2.x;
+	const uint32_t prim = fbits(hd.z);
+	const int instIdx = (int)fbits(hd.y);
+	const float t = hd.w;
+	const bool firstHit = ((flags & SF_BOUNCED) == 0) && sampleIdx == 0 && gbuf;
+	uint4* feat = gbuf ? P.gFeatures + pathIdx : nullptr;
+	if (P.pathLength == 1 && firstHit)
+	{
+		pack_features(feat, 0, 0, 1e34f, 0, 0);
+		P.gWorldPos[pathIdx] = make_float4(0, 0, 0, bitsf(0u));
+		P.gDeltaDepth[pathIdx] = make_float4(0, 0, 0, 0);
+	}
+	if (P.pathLength == 1)
+	{	/* depth for the DOF shader, in w of the (direct) accumulator */
+		float4 a = *acc;
+		a.w = a.w + (prim == 0xffffffffu ? 10000.0f : t);
+		*acc = a;
+	}
+	if (prim == 0xffffffffu)
+	{
+		v3 c = throughput * sample_sky(P, D) * (1.0f / bsdfPdf);
+		c = fixnan(clamp_intensity(c, P.clampValue));
+		acc_add(acc, c);
+		if (firstHit)
+		{
+			const uint32_t isSpec = (flags & SF_VIASPECULAR) ? 1u : 0u;
+			const uint32_t pn = pack_normal2(D * -1.0f) + isSpec;
+			pack_features(feat, hdr_to_rgb32(c), pn, t, isSpec, 0);
+			const v3 far = xyz(O4) + 50000.0f * D;
+			P.gWorldPos[pathIdx] = make_float4(far.x, far.y, far.z, bitsf(pn));
+			P.gDeltaDepth[pathIdx] = make_float4(0, 0, 0, 0);
+		}
+		return;
+	}
+	if (P.probe && (int)pixelIdx == P.probePixel && P.pathLength == 1 && sampleIdx == 0)
+		P.probe[0] = instIdx, P.probe[1] = (int)prim, P.probe[2] = (int)fbits(t);
+	const uint32_t uvb = fbits(hd.x);
+	const float u = (float)(uvb & 65535u) * (1.0f / 65535.0f), v = (float)(uvb >> 16) * (1.0f / 65535.0f);
+	const ShadeInst& SI = P.inst[instIdx];
+	const float4* tri = SI.coreTris + 13 * (size_t)prim;
+	Shading sd;
+	v3 N, iN, fN, T;
+	const v3 I = xyz(O4) + t * D;
+	/* the hit record holds the weights of vertices 0 and 1; the core's holds those of 1 and 2 */
+	const bool alphaHit = shading_data<false>(P, D, v, 1.0f - (u + v), P.spreadAngle * t, tri, instIdx, sd, N, iN, fN, T);
+	if (alphaHit)
+	{
+		if (P.pathLength == MAXPATHLENGTH_FILTER)
+		{
+			if (gbuf && sampleIdx == 0 && ((flags & SF_BOUNCED) == 0))
+			{
+				const uint32_t isSpec = (flags & SF_VIASPECULAR) ? 1u : 0u;
+				const uint32_t pn = pack_normal2(N) + isSpec;
+				pack_features(feat, 0, pn, t, isSpec, 0);
+				P.gWorldPos[pathIdx] = make_float4(I.x, I.y, I.z, bitsf(pn));
+			}
+		}
+		else
+		{
+			const uint32_t k = atomicAdd(&counters[1], 1u);
+			const v3 o = I + D * P.eps;
+			raysOut[2 * k] = make_float4(o.x, o.y, o.z, P.eps);
+			raysOut[2 * k + 1] = make_float4(D.x, D.y, D.z, 1e34f);
+			float4 st = T4;
+			if (!__builtin_isfinite(T4.x + T4.y + T4.z)) st = make_float4(0, 0, 0, T4.w);
+			stateOut[k] = st;
+			auxOut[k] = A2;
+			pathOut[k] = pathIdx;
+		}
+		return;
+	}
+	if (sd.color.x > 1.0f || sd.color.y > 1.0f || sd.color.z > 1.0f)
+	{
+		const float DdotNL = -dot(D, N);
+		if (DdotNL > 0)
+		{
+			v3 c = mk3(0, 0, 0);
+			if (P.pathLength == 1 || (flags & SF_SPECULAR) || !P.connections) c = sd.color;
+			else
+			{
+				const v3 lastN = unpack_normal(fbits(A2.y));
+				const float area = tri[5].w;
+				const int ltri = (int)fbits(tri[0].w);
+				const float lightPdf = (t * t) / (fabsf(dot(D, N)) * area);
+				const float pickProb = light_pick_prob(P, ltri, xyz(O4), lastN, I);
+				if ((bsdfPdf + lightPdf * pickProb) > 0) c = throughput * sd.color * (1.0f / (bsdfPdf + lightPdf * pickProb));
+			}
+			c = fixnan(clamp_intensity(c, P.clampValue));
+			acc_add(acc, c);
+		}
+		if (firstHit)
+		{
+			const uint32_t isSpec = (flags & SF_VIASPECULAR) ? 1u : 0u;
+			const uint32_t pn = pack_normal2(N) + isSpec;
+			pack_features(feat, hdr_to_rgb32(sd.color), pn, t, isSpec, 0);
+			P.gWorldPos[pathIdx] = make_float4(I.x, I.y, I.z, bitsf(pn));
+			depth_derivatives(P, (int)(pathIdx % (uint32_t)P.w), (int)(pathIdx / (uint32_t)P.w), t, tri, P.gDeltaDepth);
+		}
+		return;
+	}
+	if (roughness(sd) <= 0.001f || transmission(sd) > 0.999f) flags |= SF_SPECULAR; else flags &= ~SF_SPECULAR;
+	uint32_t seed = wang_hash(pathIdx * 17u + P.R0);
+	if (firstHit)
+	{
+		if (flags & SF_SPECULAR) feat->x = hdr_to_rgb32(sd.color);
+		else
+		{
+			v3 albedo = sd.color;
+			if (flags & SF_VIASPECULAR) albedo = albedo * rgb32_to_hdr(feat->x);
+			const uint32_t isSpec = (flags & SF_VIASPECULAR) ? 1u : 0u;
+			/* PackNormal2(flip ? -fN : fN) with flip = +-1 is always -fN */
+			const uint32_t pn = pack_normal2(fN * -1.0f) + isSpec;
+			pack_features(feat, hdr_to_rgb32(albedo), pn, t, isSpec, 0);
+			P.gWorldPos[pathIdx] = make_float4(I.x, I.y, I.z, bitsf(pn));
+			depth_derivatives(P, (int)(pathIdx % (uint32_t)P.w), (int)(pathIdx / (uint32_t)P.w), t, tri, P.gDeltaDepth);
+		}
+	}
+	const float faceDir = (dot(D, N) > 0) ? -1 : 1;
+	if (faceDir == 1) sd.transmittance = mk3(0, 0, 0);
+	throughput = throughput * (1.0f / bsdfPdf);
+	if ((flags & SF_SPECULAR) == 0 && P.connections)
+	{
+		float r0, r1;
+		if (sampleIdx < 2)
+		{
+			const int x = (int)((pixelIdx % (uint32_t)P.w) & 127u), y = (int)((pixelIdx / (uint32_t)P.w) & 127u);
+			r0 = blue_noise1(P.blueNoise, x, y, (int)sampleIdx + P.blueSlot, 4 + 4 * P.pathLength);
+			r1 = blue_noise1(P.blueNoise, x, y, (int)sampleIdx + P.blueSlot, 5 + 4 * P.pathLength);
+		}
+		else r0 = rand_float(seed), r1 = rand_float(seed);
+		float pickProb = 0, lightPdf = 0;
+		v3 lightColor = mk3(0, 0, 0);
+		v3 L = random_point_on_light(P, r0, r1, I, fN * faceDir, pickProb, lightPdf, lightColor) - I;
+		const float dist = length(L);
+		L = L * (1.0f / dist);
+		const float NdotL = dot(L, fN * faceDir);
+		if (NdotL > 0 && lightPdf > 0)
+		{
+			float pdf;
+			const v3 sampled = P.bsdf ? disney_evaluate(sd, fN, T, D * -1.0f, L, pdf) : evaluate_bsdf(sd, fN, L, pdf);
+			if (pdf > 0)
+			{
+				v3 c = throughput * sampled * lightColor * (NdotL / (pickProb * lightPdf + pdf));
+				c = clamp_intensity(fixnan(c), P.clampValue);
+				const uint32_t k = atomicAdd(&counters[2], 1u);
+				const v3 so = safe_origin(I, L, N, P.eps);
+				shadowRays[2 * k] = make_float4(so.x, so.y, so.z, P.eps);
+				shadowRays[2 * k + 1] = make_float4(L.x, L.y, L.z, dist - 2 * P.eps);
+				shadowPot[k] = make_float4(c.x, c.y, c.z, bitsf(slot));
+			}
+		}
+	}
+	if (flags & SF_BOUNCED) return;
+	if (P.pathLength == MAXPATHLENGTH_FILTER)
+	{
+		if (firstHit)
+		{
+			const uint32_t isSpec = (flags & SF_VIASPECULAR) ? 1u : 0u;
+			const uint32_t pn = pack_normal2(N) + isSpec;
+			pack_features(feat, 0, pn, t, isSpec, 0);
+			P.gWorldPos[pathIdx] = make_float4(I.x, I.y, I.z, bitsf(pn));
+		}
+		return;
+	}
+	float r3, r4;
+	if (sampleIdx < 256)
+	{
+		const int x = (int)((pixelIdx % (uint32_t)P.w) & 127u), y = (int)((pixelIdx / (uint32_t)P.w) & 127u);
+		r3 = blue_noise1(P.blueNoise, x, y, (int)sampleIdx + P.blueSlot, 6 + 4 * P.pathLength);
+		r4 = blue_noise1(P.blueNoise, x, y, (int)sampleIdx + P.blueSlot, 7 + 4 * P.pathLength);
+	}
+	else r3 = rand_float(seed), r4 = rand_float(seed);
+	v3 R;
+	float newBsdfPdf;
+	bool specular = false;
+	const float r5 = rand_float(seed);
+	const v3 bsdf = P.bsdf ? disney_sample(sd, fN, N, T, D * -1.0f, t, r3, r4, r5, R, newBsdfPdf, specular)
+		: sample_bsdf<true>(sd, fN, N, D * -1.0f, t, r3, r4, R, newBsdfPdf, specular);
+	if (newBsdfPdf < EPSILON_F || __builtin_isnan(newBsdfPdf)) return;
+	if (specular) flags |= SF_SPECULAR;
+	const uint32_t k = atomicAdd(&counters[1], 1u);
+	const uint32_t packedNormal = pack_normal(fN * faceDir);
+	if (!(flags & SF_SPECULAR)) flags |= SF_BOUNCED; else flags |= SF_VIASPECULAR;
+	const v3 no = safe_origin(I, R, N, P.eps);
+	raysOut[2 * k] = make_float4(no.x, no.y, no.z, P.eps);
+	raysOut[2 * k + 1] = make_float4(R.x, R.y, R.z, 1e34f);
+	const v3 tp = fixnan(throughput) * bsdf * fabsf(dot(fN, R));
+	stateOut[k] = make_float4(tp.x, tp.y, tp.z, bitsf(flags));
+	auxOut[k] = make_float2(newBsdfPdf, bitsf(packedNormal));
+	pathOut[k] = pathIdx;
+}
+
+hipError_t launch_generateF(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s)
+{
+	if (n <= 0) return hipSuccess;
+	k_generateF<<<(n + 255) / 256, 256, 0, s>>>(c, p, jobBase, n, eps, b.rays[out], b.state[out], b.pathId[out], b.aux[out], b.pathAcc, b.counters);
+	return hipGetLastError();
+}
+
+hipError_t launch_shadeF(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s)
+{
+	if (maxCount <= 0) return hipSuccess;
+	k_shadeF<<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.aux[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
+		b.aux[in ^ 1], b.pathId[in ^ 1], b.shadowRays, b.shadowPot, b.pathAcc, b.counters);
+	return hipGetLastError();
+}
+
 /* Seeded tables in the sampler's layout (xorshift32 bytes); the sampler's
  * structure is the reference's, the table values are not its optimised ones. */
 void blue_noise_tables(uint32_t* out, uint32_t seed)

@@ -46,6 +46,8 @@ def lib():
             "orc_blue_noise4": (None, [P, P, I, P]), "orc_normals": (None, [I, P, P, I]),
             "orc_disney_eval": (None, [P, P, P, P, P, P, P, I, P]), "orc_disney_sample": (None, [P, P, P, P, P, P, P, I, P]),
             "orc_ggx": (None, [I, P, P, P, I, ctypes.c_float, ctypes.c_float, P]),
+            "orc_set_brute": (None, [P, I]),
+            "orc_render_frameF": (I, [P, P, I, I, I, I, I, ctypes.c_float, ctypes.c_float, U, P, P, P, P, I, I]),
             "orc_render_frame7": (I, [P, P, I, I, I, I, ctypes.c_float, ctypes.c_float, U, U, P, I, I]),
         }
         for n, (r, a) in sigs.items():
@@ -150,6 +152,10 @@ class OracleScene:
         lib().orc_render_frame(self.h, _p(v), w, h, spp, max_depth, eps, R0gen, pass_, _p(acc), threads)
         return acc
 
+    def set_brute_force(self, on=True):
+        """Render with brute-force traversal (every triangle) instead of the BVH."""
+        lib().orc_set_brute(self.h, 1 if on else 0)
+
     def set_blue_noise(self, tables=None, seed=0x2545F491):
         """tables: (sequence[65536], scrambling[131072], ranking[131072]) uint8,
         or None for the core's seeded default tables."""
@@ -173,6 +179,19 @@ class OracleScene:
         acc = np.zeros((spp * w * h, 4), np.float32)
         v = np.ascontiguousarray(view)
         waves = lib().orc_render_frame7(self.h, _p(v), w, h, spp, pass_, eps, clamp_value, shift, cam_seed, _p(acc), threads, bsdf)
+        if waves < 0:
+            raise RuntimeError("set_blue_noise first")
+        return acc, waves
+
+    def render_frameF(self, view, w, h, spp, pass_, blue_slot, eps, clamp_value, cam_seed, gbuffer=None, threads=8, bsdf=1):
+        """One filter-estimator frame: per-path direct and indirect sums (2, spp*w*h, 4)
+        and the waves it ran; gbuffer = (features (h,w,4) uint32, world_pos, delta_depth)
+        arrays updated in place (None: filter off)."""
+        acc = np.zeros((2, spp * w * h, 4), np.float32)
+        v = np.ascontiguousarray(view)
+        f, wp, dd = gbuffer if gbuffer is not None else (None, None, None)
+        waves = lib().orc_render_frameF(self.h, _p(v), w, h, spp, pass_, blue_slot, eps, clamp_value, cam_seed, _p(acc), _p(f), _p(wp),
+                                        _p(dd), threads, bsdf)
         if waves < 0:
             raise RuntimeError("set_blue_noise first")
         return acc, waves
@@ -361,6 +380,60 @@ def render7(scene_or_oracle, view, w, h, spp=1, frames=1, eps=1e-3, seed=0x12345
             acc64 += blk.astype(np.float64)
         taken += spp
     return acc32.reshape(h, w, 4), acc64.reshape(h, w, 4), taken, cam, shift
+
+
+class RenderF:
+    """Oracle of a sequence of Render calls with the filter estimator
+    (RenderCore_Optix7Filter/rendercore.cpp:740-935, TAA off): the camera seed
+    is never reset and advances once per shade wave; the blue-noise slot
+    advances once per frame; with the filter on, the G-buffer of sample 0 and
+    SVGF (prepare, three a-trous passes, display) give the image, the history
+    carried from frame to frame."""
+
+    def __init__(self, scene_or_oracle, w, h, spp=1, eps=1e-3, seed=0x12345678, clamp_value=10.0, filter_on=True,
+                 clamp_direct=2.5, clamp_indirect=15.0, bsdf=1, blue_noise=None, threads=8):
+        from lighthouse2_amd import formats as F
+        self.o = scene_or_oracle if isinstance(scene_or_oracle, OracleScene) else OracleScene(scene_or_oracle)
+        self.o.set_blue_noise(blue_noise)
+        self.w, self.h, self.spp, self.eps, self.clamp, self.filter = w, h, spp, eps, clamp_value, filter_on
+        self.cd, self.ci, self.bsdf, self.threads = clamp_direct, clamp_indirect, bsdf, threads
+        self.cam, self.slot, self.taken, self.first = seed, 0, 0, False
+        self.acc32 = np.zeros((2, h * w, 4), np.float32)
+        self.acc64 = np.zeros((2, h * w, 4), np.float64)
+        self.features = np.zeros((h, w, 4), np.uint32)
+        self.world_pos = np.zeros((h, w, 4), np.float32)
+        self.delta_depth = np.zeros((h, w, 4), np.float32)
+        self.prev_view = np.zeros(1, F.ViewPyramid)
+        self.svgf = OracleSvgf(w, h) if filter_on else None
+        self.image = np.zeros((h, w, 4), np.float32)
+
+    def frame(self, view, mode=RESTART):
+        w, h, spp = self.w, self.h, self.spp
+        if mode == RESTART or self.first:
+            self.acc32[:] = 0
+            self.acc64[:] = 0
+            self.taken, self.first = 0, True
+        if mode == CONVERGE:
+            self.first = False
+        gb = (self.features, self.world_pos, self.delta_depth) if self.filter else None
+        pa, waves = self.o.render_frameF(view, w, h, spp, self.taken, self.slot, self.eps, self.clamp, self.cam, gb, self.threads, self.bsdf)
+        for _ in range(waves):
+            self.cam, _ = xor32_host(self.cam)
+        for half in range(2):
+            for s in range(spp):
+                blk = pa[half, s * w * h:(s + 1) * w * h]
+                self.acc32[half] += blk
+                self.acc64[half] += blk.astype(np.float64)
+        self.taken += spp
+        self.slot = (self.slot + 1) & 255
+        if self.filter:
+            self.image = self.svgf.frame(self.acc32.reshape(2, h, w, 4), self.features, self.world_pos, self.delta_depth, self.prev_view,
+                                         j=(0.0, 0.0), prev_j=(0.0, 0.0), samples=self.taken, clamp_direct=self.cd,
+                                         clamp_indirect=self.ci, stationary=0 if self.taken == spp else 1)
+        else:
+            self.image = (self.acc32[0] * np.float32(1.0 / self.taken)).reshape(h, w, 4)
+        self.prev_view = np.ascontiguousarray(view).copy()
+        return self.image
 
 
 def sincos(x):

@@ -218,7 +218,7 @@ OAPI void orc_render_frame(void* sp, const ViewPyramid* view, int w, int h, int 
 OAPI void orc_set_blue_noise(void* sp, const uint8_t* seq, const uint8_t* scr, const uint8_t* rnk, uint seed)
 {
 	RenderScene& s = *(RenderScene*)sp;
-	s.blueNoise.assign(65536 * 5 + 16, 0u);
+	s.blueNoise.assign(65536 * 5 + 64, 0u);
 	if (!seq || !scr || !rnk) { blue_noise_default(s.blueNoise.data(), seed); return; }
 	for (int i = 0; i < 65536; i++) s.blueNoise[i] = seq[i];
 	for (int i = 0; i < 128 * 128 * 8; i++) s.blueNoise[65536 + i] = scr[i], s.blueNoise[3 * 65536 + i] = rnk[i];
@@ -260,6 +260,18 @@ OAPI int orc_render_frame7(void* sp, const ViewPyramid* view, int w, int h, int 
 	Render7Params p;
 	p.view = *view, p.w = w, p.h = h, p.spp = spp, p.pass = pass, p.eps = eps, p.clampValue = clampValue, p.shift = shift, p.bsdf = bsdf;
 	return render_frame7(s, p, camSeed, pathAcc, threads);
+}
+
+OAPI void orc_set_brute(void* sp, int on) { ((RenderScene*)sp)->bruteForce = on != 0; }
+
+OAPI int orc_render_frameF(void* sp, const ViewPyramid* view, int w, int h, int spp, int pass, int blueSlot, float eps, float clampValue,
+	uint camSeed, float* pathAcc, uint* features, float* worldPos, float* deltaDepth, int threads, int bsdf)
+{
+	RenderScene& s = *(RenderScene*)sp;
+	if (s.blueNoise.empty()) return -1;
+	RenderFParams p;
+	p.view = *view, p.w = w, p.h = h, p.spp = spp, p.pass = pass, p.blueSlot = blueSlot, p.bsdf = bsdf, p.eps = eps, p.clampValue = clampValue;
+	return render_frameF(s, p, camSeed, pathAcc, features, worldPos, deltaDepth, threads);
 }
 
 OAPI void orc_primary_rays(const ViewPyramid* view, int w, int h, uint R0, float tmin, float* rays8)

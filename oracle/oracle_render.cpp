@@ -975,11 +975,288 @@ void ggx_probe(int which, const float* v3in, const float* m3, float ax, float ay
 uint normal_pack(const float* n3) { return pack_normal(mk3(n3[0], n3[1], n3[2])); }
 void normal_unpack(uint p, float* n3) { const f3 n = unpack_normal(p); n3[0] = n.x, n3[1] = n.y, n3[2] = n.z; }
 
+/* ==== filter estimator ======================================================
+ * RenderCore_Optix7Filter: generateEyeRay / setupPrimaryRay
+ * (optix/.optix.cu:66-129), shadeKernel (kernels/pathtracer.h:76-306) with
+ * PackFeatures / calculateDepthDerivatives (:44-58); MAXPATHLENGTH 7, one
+ * diffuse bounce; radiance after a diffuse bounce goes to the indirect sum
+ * when the G-buffer is written. Connections are traced after each shade wave
+ * and added in that order. */
+enum { SF_SPECULAR = 1, SF_BOUNCED = 2, SF_VIASPECULAR = 4 };
+static const int MAXPATHLENGTH_F = 7;
+
+/* blueNoiseSampler (optix/.optix.cu:40-54) */
+static float blue_noise1(const std::vector<uint>& bn, int x, int y, int sampleIndex, int dim)
+{
+	x &= 127, y &= 127, sampleIndex &= 255, dim &= 255;
+	const int ranked = (int)((uint)(sampleIndex ^ (int)bn[dim + (x + y * 128) * 8 + 65536 * 3]) & 255u);
+	uint value = bn[dim + ranked * 256];
+	value ^= bn[(dim & 7) + (x + y * 128) * 8 + 65536];
+	return (0.5f + (float)value) * (1.0f / 256.0f);
+}
+/* PackNormal2, HDRtoRGB32, RGB32toHDR (tools_shared.h:122-129,162-175) */
+static uint pack_normal2(f3 N)
+{
+	const uint x = std::min(f2u_sat((N.x + 1) * 511), 1023u), y = std::min(f2u_sat((N.y + 1) * 511), 1023u), z = std::min(f2u_sat((N.z + 1) * 511), 1023u);
+	return (x << 2u) + (y << 12u) + (z << 22u);
+}
+static uint hdr_to_rgb32(f3 c)
+{
+	const uint r = f2u_sat(1023.0f * fminf(1.0f, c.x)), g = f2u_sat(2047.0f * fminf(1.0f, c.y)), b = f2u_sat(2047.0f * fminf(1.0f, c.z));
+	return (r << 22) + (g << 11) + b;
+}
+static f3 rgb32_to_hdr(uint c)
+{
+	return mk3((float)(c >> 22) * (1.0f / 1023.0f), (float)((c >> 11) & 2047u) * (1.0f / 2047.0f), (float)(c & 2047u) * (1.0f / 2047.0f));
+}
+static void pack_features(uint* f, uint albedo, uint packedNormal, float t, uint isSpecular, uint matid)
+{
+	f[0] = albedo, f[1] = packedNormal, f[2] = f2u(t), f[3] = (isSpecular << 4) + (matid << 6) + (f[3] & 15u);
+}
+static void depth_derivatives(const RenderFParams& p, int x, int y, float depth, const CoreTri& tri, float* dd)
+{
+	const f3 triN = mk3(tri.Nx, tri.Ny, tri.Nz);
+	const ViewPyramid& v = p.view;
+	const f3 p1 = f3of(v.p1), p2 = f3of(v.p2) - f3of(v.p1), p3 = f3of(v.p3) - f3of(v.p1), pos = f3of(v.pos), v0 = f3of(tri.vertex0);
+	const float iw = 1.0f / (float)p.w, ih = 1.0f / (float)p.h;
+	const f3 dxDir = normalize((p1 + ((float)x + 0.5f + 1) * iw * p2 + ((float)y + 0.5f) * ih * p3) - pos);
+	const float dxDepth = dot(v0 - pos, triN) / dot(triN, dxDir);
+	const f3 dyDir = normalize((p1 + ((float)x + 0.5f) * iw * p2 + ((float)y + 0.5f + 1) * ih * p3) - pos);
+	const float dyDepth = dot(v0 - pos, triN) / dot(triN, dyDir);
+	float* o = dd + 4 * (size_t)(x + y * p.w);
+	o[0] = 0, o[1] = 0, o[2] = dxDepth - depth, o[3] = dyDepth - depth;
+}
+
+/* One path of the filter estimator, depth first; returns the number of shade waves it took part in. */
+static int trace_pathF(const RenderScene& s, const RenderFParams& p, const uint* R0wave, uint pathIdx, float* accD, float* accI,
+	uint* features, float* worldPos, float* deltaDepth)
+{
+	const uint wh = (uint)(p.w * p.h);
+	const uint sampleIdx = pathIdx / wh + (uint)p.pass;
+	const bool gbuf = features != nullptr;
+	const bool connections = (s.triLights.size() + s.pointLights.size() + s.spotLights.size() + s.dirLights.size()) > 0;
+	const ViewPyramid& v = p.view;
+	/* eye ray */
+	const uint pix0 = pathIdx % wh;
+	const int sx = (int)(pix0 % (uint)p.w), sy = (int)(pix0 / (uint)p.w);
+	const f3 pos = f3of(v.pos), p1 = f3of(v.p1), right = f3of(v.p2) - p1, up = f3of(v.p3) - p1;
+	float r0 = 0, r1 = 0;
+	f3 O = pos;
+	if (!gbuf)
+	{
+		uint seed = wang_hash(pathIdx * 16789u + (uint)p.pass * 1791u);
+		float r2, r3;
+		if (sampleIdx < 256)
+			r0 = blue_noise1(s.blueNoise, sx, sy, (int)sampleIdx, 0), r1 = blue_noise1(s.blueNoise, sx, sy, (int)sampleIdx, 1),
+			r2 = blue_noise1(s.blueNoise, sx, sy, (int)sampleIdx, 2), r3 = blue_noise1(s.blueNoise, sx, sy, (int)sampleIdx, 3);
+		else
+			r0 = rand_float(seed), r1 = rand_float(seed), r2 = rand_float(seed), r3 = rand_float(seed);
+		O = point_on_lens(r2, r3, pos, v.aperture, right, up);
+	}
+	const float uu = ((float)sx + r0) * (1.0f / (float)p.w), vv = ((float)sy + r1) * (1.0f / (float)p.h);
+	f3 D = normalize((p1 + uu * right + vv * up) - O);
+	f3 throughput = mk3(1, 1, 1);
+	float bsdfPdf = 1;
+	uint flags = SF_SPECULAR, lastNormal = 0;
+	float d4[4] = { 0, 0, 0, 0 }, i4[4] = { 0, 0, 0, 0 };
+	int waves = 0;
+	uint* feat = gbuf ? features + 4 * (size_t)pathIdx : nullptr;
+	auto acc = [&]() -> float* { return (gbuf && (flags & SF_BOUNCED)) ? i4 : d4; };
+	auto add = [&](f3 c) { float* a = acc(); a[0] += c.x, a[1] += c.y, a[2] += c.z; };
+	auto setwp = [&](f3 P, uint pn) { float* o = worldPos + 4 * (size_t)pathIdx; o[0] = P.x, o[1] = P.y, o[2] = P.z, o[3] = u2f(pn); };
+	for (int pathLength = 1; pathLength <= MAXPATHLENGTH_F; pathLength++)
+	{
+		waves = pathLength;
+		const Hit h = trace_closest(s, O, p.eps, D, 1e34f, nullptr);
+		const uint pixelIdx = (pathIdx % wh) + ((gbuf && (flags & SF_BOUNCED)) ? wh : 0u);
+		const bool firstHit = ((flags & SF_BOUNCED) == 0) && sampleIdx == 0 && gbuf;
+		if (pathLength == 1 && firstHit)
+		{
+			pack_features(feat, 0, 0, 1e34f, 0, 0);
+			setwp(mk3(0, 0, 0), 0);
+			float* dd = deltaDepth + 4 * (size_t)pathIdx;
+			dd[0] = dd[1] = dd[2] = dd[3] = 0;
+		}
+		if (pathLength == 1) acc()[3] += h.prim < 0 ? 10000.0f : h.t;
+		if (h.prim < 0)
+		{
+			f3 c = throughput * sample_sky(s, D) * (1.0f / bsdfPdf);
+			c = fixnan(clamp_intensity(c, p.clampValue));
+			add(c);
+			if (firstHit)
+			{
+				const uint isSpec = (flags & SF_VIASPECULAR) ? 1u : 0u;
+				const uint pn = pack_normal2(D * -1.0f) + isSpec;
+				pack_features(feat, hdr_to_rgb32(c), pn, h.t, isSpec, 0);
+				setwp(O + 50000.0f * D, pn);
+				float* dd = deltaDepth + 4 * (size_t)pathIdx;
+				dd[0] = dd[1] = dd[2] = dd[3] = 0;
+			}
+			break;
+		}
+		const float b0 = (float)(h.uv & 65535u) * (1.0f / 65535.0f), b1 = (float)(h.uv >> 16) * (1.0f / 65535.0f);
+		const float t = h.t;
+		const CoreTri& tri = s.tris[s.inst[h.inst].mesh][h.prim];
+		Shading sd;
+		f3 N, iN, fN, T;
+		const f3 I = O + t * D;
+		const bool alphaHit = shading_data(s, D, b1, 1.0f - (b0 + b1), v.spreadAngle * t, tri, h.inst, sd, N, iN, fN, T, false);
+		if (alphaHit)
+		{
+			if (pathLength == MAXPATHLENGTH_F)
+			{
+				if (gbuf && sampleIdx == 0 && ((flags & SF_BOUNCED) == 0))
+				{
+					const uint isSpec = (flags & SF_VIASPECULAR) ? 1u : 0u;
+					const uint pn = pack_normal2(N) + isSpec;
+					pack_features(feat, 0, pn, t, isSpec, 0);
+					setwp(I, pn);
+				}
+				break;
+			}
+			if (!std::isfinite(throughput.x + throughput.y + throughput.z)) throughput = mk3(0, 0, 0);
+			O = I + D * p.eps;
+			continue;
+		}
+		if (sd.emissive())
+		{
+			const float DdotNL = -dot(D, N);
+			if (DdotNL > 0)
+			{
+				f3 c = mk3(0, 0, 0);
+				if (pathLength == 1 || (flags & SF_SPECULAR) || !connections) c = sd.color;
+				else
+				{
+					const f3 lastN = unpack_normal(lastNormal);
+					const float lightPdf = (t * t) / (fabsf(dot(D, N)) * tri.area);
+					const float pickProb = light_pick_prob(s, tri.ltriIdx, O, lastN, I);
+					if ((bsdfPdf + lightPdf * pickProb) > 0) c = throughput * sd.color * (1.0f / (bsdfPdf + lightPdf * pickProb));
+				}
+				c = fixnan(clamp_intensity(c, p.clampValue));
+				add(c);
+			}
+			if (firstHit)
+			{
+				const uint isSpec = (flags & SF_VIASPECULAR) ? 1u : 0u;
+				const uint pn = pack_normal2(N) + isSpec;
+				pack_features(feat, hdr_to_rgb32(sd.color), pn, t, isSpec, 0);
+				setwp(I, pn);
+				depth_derivatives(p, (int)(pathIdx % (uint)p.w), (int)(pathIdx / (uint)p.w), t, tri, deltaDepth);
+			}
+			break;
+		}
+		if (sd.roughness() <= 0.001f || sd.transmission() > 0.999f) flags |= SF_SPECULAR; else flags &= ~(uint)SF_SPECULAR;
+		uint seed = wang_hash(pathIdx * 17u + R0wave[pathLength]);
+		if (firstHit)
+		{
+			if (flags & SF_SPECULAR) feat[0] = hdr_to_rgb32(sd.color);
+			else
+			{
+				f3 albedo = sd.color;
+				if (flags & SF_VIASPECULAR) albedo = albedo * rgb32_to_hdr(feat[0]);
+				const uint isSpec = (flags & SF_VIASPECULAR) ? 1u : 0u;
+				const uint pn = pack_normal2(fN * -1.0f) + isSpec;   /* PackNormal2(flip ? -fN : fN), flip = +-1 */
+				pack_features(feat, hdr_to_rgb32(albedo), pn, t, isSpec, 0);
+				setwp(I, pn);
+				depth_derivatives(p, (int)(pathIdx % (uint)p.w), (int)(pathIdx / (uint)p.w), t, tri, deltaDepth);
+			}
+		}
+		const float faceDir = (dot(D, N) > 0) ? -1 : 1;
+		if (faceDir == 1) sd.transmittance = mk3(0, 0, 0);
+		throughput = throughput * (1.0f / bsdfPdf);
+		if ((flags & SF_SPECULAR) == 0 && connections)
+		{
+			float q0, q1;
+			if (sampleIdx < 2)
+			{
+				const int x = (int)((pixelIdx % (uint)p.w) & 127u), y = (int)((pixelIdx / (uint)p.w) & 127u);
+				q0 = blue_noise1(s.blueNoise, x, y, (int)sampleIdx + p.blueSlot, 4 + 4 * pathLength);
+				q1 = blue_noise1(s.blueNoise, x, y, (int)sampleIdx + p.blueSlot, 5 + 4 * pathLength);
+			}
+			else q0 = rand_float(seed), q1 = rand_float(seed);
+			float pickProb = 0, lightPdf = 0;
+			f3 lightColor = mk3(0, 0, 0);
+			f3 L = random_point_on_light(s, q0, q1, I, fN * faceDir, pickProb, lightPdf, lightColor) - I;
+			const float dist = length(L);
+			L = L * (1.0f / dist);
+			const float NdotL = dot(L, fN * faceDir);
+			if (NdotL > 0 && lightPdf > 0)
+			{
+				float pdf;
+				const f3 sampled = p.bsdf ? disney_evaluate(sd, fN, T, D * -1.0f, L, pdf) : evaluate_bsdf(sd, fN, D * -1.0f, L, pdf);
+				if (pdf > 0)
+				{
+					f3 c = throughput * sampled * lightColor * (NdotL / (pickProb * lightPdf + pdf));
+					c = clamp_intensity(fixnan(c), p.clampValue);
+					const f3 so = safe_origin(I, L, N, p.eps);
+					if (!trace_any(s, so, p.eps, L, dist - 2 * p.eps, nullptr)) { float* a = acc(); a[0] += c.x, a[1] += c.y, a[2] += c.z, a[3] += 1.0f; }
+				}
+			}
+		}
+		if (flags & SF_BOUNCED) break;
+		if (pathLength == MAXPATHLENGTH_F)
+		{
+			if (firstHit)
+			{
+				const uint isSpec = (flags & SF_VIASPECULAR) ? 1u : 0u;
+				const uint pn = pack_normal2(N) + isSpec;
+				pack_features(feat, 0, pn, t, isSpec, 0);
+				setwp(I, pn);
+			}
+			break;
+		}
+		float q3, q4;
+		if (sampleIdx < 256)
+		{
+			const int x = (int)((pixelIdx % (uint)p.w) & 127u), y = (int)((pixelIdx / (uint)p.w) & 127u);
+			q3 = blue_noise1(s.blueNoise, x, y, (int)sampleIdx + p.blueSlot, 6 + 4 * pathLength);
+			q4 = blue_noise1(s.blueNoise, x, y, (int)sampleIdx + p.blueSlot, 7 + 4 * pathLength);
+		}
+		else q3 = rand_float(seed), q4 = rand_float(seed);
+		f3 R;
+		float newBsdfPdf;
+		bool specular = false;
+		const float q5 = rand_float(seed);
+		const f3 bsdf = p.bsdf ? disney_sample(sd, fN, N, T, D * -1.0f, t, q3, q4, q5, R, newBsdfPdf, specular)
+			: sample_bsdf(sd, fN, N, D * -1.0f, t, q3, q4, R, newBsdfPdf, specular, true);
+		if (newBsdfPdf < EPSILON || std::isnan(newBsdfPdf)) break;
+		if (specular) flags |= SF_SPECULAR;
+		lastNormal = pack_normal(fN * faceDir);
+		if (!(flags & SF_SPECULAR)) flags |= SF_BOUNCED; else flags |= SF_VIASPECULAR;
+		O = safe_origin(I, R, N, p.eps);
+		D = R;
+		throughput = fixnan(throughput) * bsdf * fabsf(dot(fN, R));
+		bsdfPdf = newBsdfPdf;
+	}
+	for (int k = 0; k < 4; k++) accD[k] = d4[k], accI[k] = i4[k];
+	return waves;
+}
+
+int render_frameF(const RenderScene& s, const RenderFParams& p, uint camSeed, float* pathAcc, uint* features, float* worldPos,
+	float* deltaDepth, int threads)
+{
+	uint R0wave[MAXPATHLENGTH_F + 1] = { 0 };
+	uint cs = camSeed;
+	for (int k = 1; k <= MAXPATHLENGTH_F; k++) R0wave[k] = xor32(cs) + (uint)k * 91771u;
+	const int n = p.w * p.h * p.spp;
+	if (threads < 1) threads = 1;
+	std::vector<int> deepest(threads, 0);
+	std::vector<std::thread> pool;
+	for (int t = 0; t < threads; t++)
+		pool.emplace_back([&, t]() {
+			for (int j = t; j < n; j += threads)
+				deepest[t] = std::max(deepest[t], trace_pathF(s, p, R0wave, (uint)j, pathAcc + (size_t)j * 4, pathAcc + ((size_t)n + j) * 4,
+					features, worldPos, deltaDepth));
+		});
+	for (auto& th : pool) th.join();
+	return *std::max_element(deepest.begin(), deepest.end());
+}
+
 /* Seeded blue-noise tables of the core (xorshift32 bytes in the sampler's layout). */
 void blue_noise_default(uint* out, uint seed)
 {
 	uint x = seed ? seed : 1u;
-	const int words = 65536 * 5 + 16;
+	const int words = 65536 * 5 + 64;
 	for (int i = 0; i < words; i++) out[i] = 0;
 	for (int i = 0; i < 65536; i++) out[i] = xor32(x) & 255u;
 	for (int i = 0; i < 128 * 128 * 8; i++) out[65536 + i] = xor32(x) & 255u;

@@ -86,6 +86,21 @@ struct Render7Params
  * 91771. pathAcc: w*h*spp float4 (xyz radiance, w unoccluded connections).
  * Returns the number of shade waves the frame ran. */
 int render_frame7(const RenderScene& s, const Render7Params& p, uint camSeed, float* pathAcc, int threads);
+
+struct RenderFParams
+{
+	ViewPyramid view;   /* aperture and distortion as the core uses them */
+	int w, h, spp, pass, blueSlot, bsdf;
+	float eps, clampValue;
+};
+/* One frame of the filter estimator. pathAcc: 2 * w*h*spp float4, direct
+ * radiance (w: depth at the first hit plus unoccluded connections) then
+ * indirect; features (w*h uint4), worldPos and deltaDepth (w*h float4) are
+ * the G-buffer, written for sample 0 only; null features: filter off (no
+ * G-buffer, jittered thin-lens eye rays, everything in the direct sum).
+ * Returns the number of shade waves the frame ran. */
+int render_frameF(const RenderScene& s, const RenderFParams& p, uint camSeed, float* pathAcc, uint* features, float* worldPos,
+	float* deltaDepth, int threads);
 void blue_noise_default(uint* out, uint seed);
 void blue_noise_sample4(const RenderScene& s, int x, int y, int sampleIndex, int dim, float* out4);
 uint normal_pack(const float* n3);

@@ -226,11 +226,22 @@ void finalize_instances(Scene& s)
 /* Conservative slab test: the far distance is widened by 2^-16 relative so a
  * box whose exact interval meets [tmin, tmax] is never rejected by rounding.
  * Box tests only prune; they never decide a hit. */
+/* A direction component of zero gives an infinite reciprocal; a ray lying in
+ * a face plane of the box then gives 0 * inf = NaN for that plane. Such a ray
+ * stays inside the slab for all t, so the NaN bound becomes the opposite
+ * infinity (the test stays conservative). */
+static inline void slab_axis(float& t1, float& t2)
+{
+	if (t1 != t1 && t2 != t2) { t1 = -INFINITY, t2 = INFINITY; }
+	else if (t1 != t1) t1 = -t2;
+	else if (t2 != t2) t2 = -t1;
+}
 static inline bool slab(f3 lo, f3 hi, f3 O, f3 inv, float tmin, float tmax, float& tnear)
 {
-	const float tx1 = (lo.x - O.x) * inv.x, tx2 = (hi.x - O.x) * inv.x;
-	const float ty1 = (lo.y - O.y) * inv.y, ty2 = (hi.y - O.y) * inv.y;
-	const float tz1 = (lo.z - O.z) * inv.z, tz2 = (hi.z - O.z) * inv.z;
+	float tx1 = (lo.x - O.x) * inv.x, tx2 = (hi.x - O.x) * inv.x;
+	float ty1 = (lo.y - O.y) * inv.y, ty2 = (hi.y - O.y) * inv.y;
+	float tz1 = (lo.z - O.z) * inv.z, tz2 = (hi.z - O.z) * inv.z;
+	slab_axis(tx1, tx2), slab_axis(ty1, ty2), slab_axis(tz1, tz2);
 	const float tn = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fmaxf(fminf(tz1, tz2), tmin));
 	float tf = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fminf(fmaxf(tz1, tz2), tmax));
 	tf = tf + fabsf(tf) * 1.52587890625e-5f;
@@ -338,8 +349,12 @@ static Hit make_hit(const Best& b)
 	return h;
 }
 
+Hit brute_closest(const Scene& s, f3 O, float tmin, f3 D, float tmax);
+bool brute_any(const Scene& s, f3 O, float tmin, f3 D, float tmax);
+
 Hit trace_closest(const Scene& s, f3 O, float tmin, f3 D, float tmax, Counters* cnt)
 {
+	if (s.bruteForce) return brute_closest(s, O, tmin, D, tmax);
 	Best b;
 	scene_trace<false>(s, O, tmin, D, tmax, b, cnt);
 	return make_hit(b);
@@ -347,6 +362,7 @@ Hit trace_closest(const Scene& s, f3 O, float tmin, f3 D, float tmax, Counters* 
 
 bool trace_any(const Scene& s, f3 O, float tmin, f3 D, float tmax, Counters* cnt)
 {
+	if (s.bruteForce) return brute_any(s, O, tmin, D, tmax);
 	Best b;
 	return scene_trace<true>(s, O, tmin, D, tmax, b, cnt);
 }

@@ -126,6 +126,7 @@ struct Scene
 	std::vector<Instance> inst;
 	std::vector<BuildNode> tlas;
 	std::vector<uint> tlasPrims;
+	bool bruteForce = false;   /* trace_closest / trace_any test every triangle (checks the traversal) */
 };
 
 void mat4_inverse(const float* c, float* out);

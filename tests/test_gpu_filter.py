@@ -1,0 +1,91 @@
+"""GPU parity of the filter estimator ("estimator" setting 2: RenderCore_Optix7Filter's
+eye rays and shade kernel, which writes the SVGF G-buffer and splits direct and
+indirect radiance, followed by SVGF in Render) against the oracle through the
+C ABI: fp64 accumulators of both halves, the G-buffer and the displayed image
+(interior; the display pass leaves the border alone) identical frame by frame,
+over restarts, converging frames and a moving camera."""
+import numpy as np
+import pytest
+
+import lighthouse2_amd as lh
+from lighthouse2_amd import scenes
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+np.set_printoptions(threshold=20, edgeitems=2)
+
+
+def diff(a, b):
+    """Short report of where two arrays differ (first entries, counts per channel)."""
+    a, b = np.asarray(a), np.asarray(b)
+    bad = a != b
+    if a.dtype.kind == "f":
+        bad &= ~(np.isnan(a) & np.isnan(b))
+    idx = np.argwhere(bad)
+    per_ch = bad.reshape(-1, a.shape[-1]).sum(0).tolist()
+    rows = "; ".join(f"{tuple(i)}: {a[tuple(i)]!r} vs {b[tuple(i)]!r}" for i in idx[:6])
+    return f"{len(idx)} of {a.size} differ, per channel {per_ch}; {rows}"
+
+
+def run(scene, views_modes, w, h, spp, bsdf=1, filter_on=True, seed=0x12345678, tables=None):
+    core = lh.RenderCore()
+    lh.load_scene(core, scene)
+    core.SetTarget(w, h, spp)
+    core.Setting("estimator", 2)
+    core.Setting("bsdf", bsdf)
+    core.Setting("filter", 1 if filter_on else 0)
+    if seed != 0x12345678:
+        core.Setting("seed", seed)
+    if tables is not None:
+        core.SetBlueNoise(*tables)
+    ref = orc.RenderF(scene, w, h, spp=spp, seed=seed, filter_on=filter_on, bsdf=bsdf, blue_noise=tables, threads=16)
+    for k, (view, mode) in enumerate(views_modes):
+        core.Render(view, lh.LH2_RESTART if mode == orc.RESTART else lh.LH2_CONVERGE)
+        want = ref.frame(view, mode)
+        img = core.read_image()
+        acc = core.read_buffer("acc64")
+        if filter_on:
+            g = core.read_buffer("features")
+            assert np.array_equal(g, ref.features), f"frame {k}: features: {diff(g, ref.features)}"
+            g = core.read_buffer("world_pos").view(np.uint32)
+            assert np.array_equal(g, ref.world_pos.view(np.uint32)), f"frame {k}: world pos: {diff(g, ref.world_pos.view(np.uint32))}"
+            g = core.read_buffer("delta_depth").view(np.uint32)
+            assert np.array_equal(g, ref.delta_depth.view(np.uint32)), f"frame {k}: depth derivatives: {diff(g, ref.delta_depth.view(np.uint32))}"
+        r64 = ref.acc64.reshape(2, h, w, 4)
+        assert np.array_equal(acc[0], r64[0]), f"frame {k}: direct: {diff(acc[0], r64[0])}"
+        assert np.array_equal(acc[1], r64[1]), f"frame {k}: indirect: {diff(acc[1], r64[1])}"
+        if filter_on:
+            assert np.array_equal(img[1:-1, 1:-1].view(np.uint32), want[1:-1, 1:-1].view(np.uint32)), \
+                f"frame {k}: display: {diff(img[1:-1, 1:-1], want[1:-1, 1:-1])}"
+        else:
+            assert np.array_equal(img, want), f"frame {k}: image"
+    st = core.GetCoreStats()
+    core.Shutdown()
+    return img, st
+
+
+@pytest.mark.parametrize("bsdf", [0, 1])
+def test_box_sequence(torch_cuda, bsdf):
+    s = scenes.disney_cornell_box(sphere_tris=5000) if bsdf else scenes.cornell_box(sphere_tris=5000)
+    v0 = scenes.cornell_view(72, 56)
+    v1 = scenes.camera_view((0.52, 0.49, -1.19), (0.5, 0.5, 0.5), 40.0, 72, 56)
+    R, C = orc.RESTART, orc.CONVERGE
+    img, st = run(s, [(v0, R), (v0, C), (v0, C), (v1, R), (v1, R)], 72, 56, 1, bsdf=bsdf)
+    assert np.isfinite(img).all() and img[1:-1, 1:-1, :3].mean() > 0
+    assert st["filterTime"] > 0
+
+
+def test_lambert_textured_sky_two_spp(torch_cuda):
+    s = scenes.textured_cornell_box(sphere_tris=3000)
+    s.sky = scenes.sky_dome(128, 64, seed=3)
+    v = scenes.cornell_view(48, 40)
+    rng = np.random.default_rng(9)
+    tables = (rng.integers(0, 256, 65536, dtype=np.uint8), rng.integers(0, 256, 131072, dtype=np.uint8),
+              rng.integers(0, 256, 131072, dtype=np.uint8))
+    run(s, [(v, orc.RESTART), (v, orc.CONVERGE), (v, orc.CONVERGE)], 48, 40, 2, bsdf=0, seed=321, tables=tables)
+
+
+def test_filter_off_is_plain_accumulation(torch_cuda):
+    s = scenes.disney_cornell_box(sphere_tris=2000)
+    v = scenes.camera_view((0.5, 0.5, -1.2), (0.5, 0.5, 0.5), 40.0, 40, 32, aperture=0.01)
+    run(s, [(v, orc.RESTART), (v, orc.CONVERGE), (v, orc.CONVERGE)], 40, 32, 3, filter_on=False)

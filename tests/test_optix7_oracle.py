@@ -165,3 +165,43 @@ def test_triangle_emitter_form_factor():
     print(f"mean {est.mean():.5f} closed form {want:.5f} rel {err:.2e}")
     assert err < 0.005
     assert np.all(r64[..., 3] > 0)          # every pixel had unoccluded connections (w counts them)
+
+
+def test_filter_gbuffer_closed_form():
+    """Filter estimator G-buffer on the Lambert Cornell box: eye rays through
+    pixel corners hit the walls at t = plane distance / D.n; the world
+    position is O + t D; the depth derivatives are those of the neighbouring
+    pixel corners' rays against the same plane; albedo is the wall colour."""
+    s = scenes.cornell_box(seed=5, sphere_tris=200)
+    w, h = 48, 40
+    view = scenes.cornell_view(w, h)
+    r = orc.RenderF(s, w, h, spp=1, bsdf=0)
+    r.frame(view, orc.RESTART)
+    v = view[0]
+    pos, p1 = v["pos"].astype(np.float64), v["p1"].astype(np.float64)
+    right, up = (v["p2"] - v["p1"]).astype(np.float64), (v["p3"] - v["p1"]).astype(np.float64)
+    back = s.materials[2]["color"]["value"].astype(np.float64)
+    checked = 0
+    for y in range(h):
+        for x in range(w):
+            D = p1 + (x / w) * right + (y / h) * up - pos
+            D /= np.linalg.norm(D)
+            t = (1.0 - pos[2]) / D[2]
+            P = pos + t * D
+            if not (0.02 < P[0] < 0.98 and 0.02 < P[1] < 0.98) or np.hypot(P[0] - 0.5, P[1] - 0.5) < 0.33:
+                continue                                  # back wall only, away from edges and the sphere
+            f = r.features[y, x]
+            assert abs(f[2:3].view(np.float32)[0] - t) < 1e-5 * t
+            assert np.allclose(r.world_pos[y, x, :3], P, atol=2e-6)
+            alb = np.array([f[0] >> 22, (f[0] >> 11) & 2047, f[0] & 2047], np.float64) / [1023, 2047, 2047]
+            assert np.allclose(alb, back, atol=1.5e-3)
+            def depth(px, py):
+                d = p1 + px / w * right + py / h * up - pos
+                d /= np.linalg.norm(d)
+                return (1.0 - pos[2]) / d[2]
+            dd = r.delta_depth[y, x]
+            assert abs(dd[2] - (depth(x + 1.5, y + 0.5) - t)) < 2e-5 and abs(dd[3] - (depth(x + 0.5, y + 1.5) - t)) < 2e-5
+            checked += 1
+    assert checked > 200
+    # direct and indirect halves: the indirect sum holds radiance of paths after a diffuse bounce
+    assert r.acc64[0, :, :3].sum() > 0 and r.acc64[1, :, :3].sum() > 0

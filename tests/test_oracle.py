@@ -247,3 +247,26 @@ def test_oracle_render_is_deterministic_and_accumulates():
     assert n == 4 and np.array_equal(a32, b32) and np.array_equal(a64, b64)
     rel = np.abs(a32[..., :3] - a64[..., :3]) / np.maximum(a64[..., :3], 1e-6)
     assert rel.max() < 1e-5 and a64[..., :3].mean() > 0.01
+
+
+def test_rays_in_box_planes_match_brute_force():
+    """Rays with zero direction components lying in the face planes of BVH
+    boxes (0 * inf in the slab test) must not be culled: pixel-corner rays of
+    the centred camera through the Cornell sphere, and axis-parallel rays."""
+    s = scenes.cornell_box(sphere_tris=5000)
+    o = orc.OracleScene(s)
+    w, h = 72, 56
+    v = scenes.cornell_view(w, h)[0]
+    pos, p1 = v["pos"].astype(np.float32), v["p1"].astype(np.float32)
+    right, up = (v["p2"] - v["p1"]).astype(np.float32), (v["p3"] - v["p1"]).astype(np.float32)
+    rays = []
+    for y in range(h):
+        D = p1 + np.float32(0.5) * right + np.float32(y / h) * up - pos      # the centre column: D.x == 0
+        rays.append(np.concatenate([pos, [1e-3], D / np.linalg.norm(D), [1e34]]))
+    for k in range(64):
+        c = np.float32(0.2 + 0.6 * k / 63)
+        rays.append([0.5, c, -1.0, 1e-3, 0.0, 0.0, 1.0, 1e34])            # along z in the plane x = 0.5
+        rays.append([c, 0.5, -1.0, 1e-3, 0.0, 0.0, 1.0, 1e34])
+    r = np.array(rays, np.float32)
+    assert np.array_equal(o.trace(r).view(np.uint32), o.trace(r, brute=True).view(np.uint32))
+    assert (o.trace(r).view(np.int32)[:, 2] >= 0).all()
