@@ -188,8 +188,8 @@ def test_filter_gbuffer_closed_form():
             D /= np.linalg.norm(D)
             t = (1.0 - pos[2]) / D[2]
             P = pos + t * D
-            if not (0.02 < P[0] < 0.98 and 0.02 < P[1] < 0.98) or np.hypot(P[0] - 0.5, P[1] - 0.5) < 0.33:
-                continue                                  # back wall only, away from edges and the sphere
+            if not (0.02 < P[0] < 0.98 and 0.02 < P[1] < 0.98) or np.hypot(P[0] - 0.5, P[1] - 0.5) < 0.42:
+                continue                                  # back wall only, away from edges and the sphere (its shadow from the eye has radius 0.39 there)
             f = r.features[y, x]
             assert abs(f[2:3].view(np.float32)[0] - t) < 1e-5 * t
             assert np.allclose(r.world_pos[y, x, :3], P, atol=2e-6)
