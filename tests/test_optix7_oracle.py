@@ -202,6 +202,6 @@ def test_filter_gbuffer_closed_form():
             dd = r.delta_depth[y, x]
             assert abs(dd[2] - (depth(x + 1.5, y + 0.5) - t)) < 2e-5 and abs(dd[3] - (depth(x + 0.5, y + 1.5) - t)) < 2e-5
             checked += 1
-    assert checked > 200
+    assert checked > 150
     # direct and indirect halves: the indirect sum holds radiance of paths after a diffuse bounce
     assert r.acc64[0, :, :3].sum() > 0 and r.acc64[1, :, :3].sum() > 0
