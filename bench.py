@@ -44,9 +44,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent traversal blocks per CU (0: core default)")
     ap.add_argument("--min-rays-per-lane", type=float, default=0.0, help="shrink the grid of small waves (0: off)")
-    ap.add_argument("--workload", choices=["extend", "denoise", "svgf", "serve"], default="extend",
+    ap.add_argument("--workload", choices=["extend", "denoise", "svgf", "serve", "frame"], default="extend",
                     help="extend: the headline traversal metric; denoise: U-Net forward/train at 1088x1920; "
-                         "svgf: filter ms/frame at 1920x1080; serve: job-server latency under Poisson arrivals")
+                         "svgf: filter ms/frame at 1920x1080; serve: job-server latency under Poisson arrivals; "
+                         "frame: instanced scene, 1080p 1 spp + SVGF")
     ap.add_argument("--serve-jobs", type=int, default=40)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (FETCH_SIZE/WRITE_SIZE calibration)")
@@ -170,6 +171,65 @@ def bench_svgf(a, np, torch, lh, scenes):
                                    "traffic": None, "bytes_per_pixel": bpp}}))
 
 
+def bench_frame(a, np, torch, lh, scenes):
+    """One real-time frame of the instanced workload at 1920x1080, 1 spp:
+    10 noise-displaced spheres (50k-200k triangles), 1000 instances with
+    random affine transforms (two-level BVH), ten 2048^2 textures, Disney
+    materials (Lambert diffuse + GGX, roughness U(0.05, 1)), 64 area lights;
+    the filter estimator (eye rays, shade with G-buffer, NEE, one diffuse
+    bounce) followed by SVGF. Camera fixed; each frame restarts accumulation,
+    as a moving camera would. ms/frame is wall time around Render."""
+    t0 = time.time()
+    s = scenes.instanced_scene()
+    t_scene = time.time() - t0
+    core = lh.RenderCore()
+    t0 = time.time()
+    lh.load_scene(core, s)
+    W, H = 1920, 1080
+    core.SetTarget(W, H, 1)
+    core.Setting("estimator", 2)
+    view = scenes.instanced_view(W, H)
+    core.Render(view, lh.LH2_RESTART)
+    core.WaitForRender()
+    t_load = time.time() - t0
+    for _ in range(a.warmup):
+        core.Render(view, lh.LH2_RESTART)
+    parts = {k: 0.0 for k in ("traceTime0", "traceTime1", "traceTimeX", "shadeTime", "shadowTraceTime", "filterTime")}
+    rays = 0
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(a.steps):
+        core.Render(view, lh.LH2_RESTART)
+        st = core.GetCoreStats()
+        for k in parts:
+            parts[k] += float(st[k])
+    core.WaitForRender()
+    ms = (time.time() - t0) * 1e3 / a.steps
+    parts = {k: round(v * 1e3 / a.steps, 3) for k, v in parts.items()}
+    img = core.read_image()
+    tris = sum(len(v) // 3 for v, _ in s.meshes)
+    line = {"metric": "ms/frame 1080p 1spp+denoise", "value": round(ms, 3), "unit": "ms", "higher_is_better": False, "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "dtype": "f32", "data": "synthetic scene", "vs_baseline": None,
+            "config": {"workload": "instanced scene: 10 displaced spheres 50k-200k tris, 1000 instances, 2048^2 textures, "
+                                   "Disney (diffuse + GGX), 64 area lights; 1920x1080, 1 spp, filter estimator + SVGF",
+                       "mesh_triangles": int(tris), "instances": len(s.instances)},
+            "breakdown_ms": parts, "image_mean": float(img[1:-1, 1:-1, :3].mean()),
+            "scene_build_s": round(t_scene, 1), "load_and_first_frame_s": round(t_load, 1)}
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        threads = min(16, os.cpu_count() or 1)
+        w, h = W // 8, H // 8                        # 1/64 of the pixels, same view frustum
+        ref = orc.RenderF(s, w, h, spp=1, threads=threads)
+        t0 = time.time()
+        ref.frame(scenes.instanced_view(w, h), orc.RESTART)
+        cpu_s = time.time() - t0
+        line["cpu_baseline"] = {"value": round(cpu_s * 64 * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+                                "sample": f"oracle filter estimator + SVGF at {w}x{h} (1/64 of the pixels), scaled by 64"}
+    print(json.dumps(line))
+    core.Shutdown()
+
+
 def bench_serve(a, np, torch, lh, scenes):
     """Job server under Poisson arrivals (lambda 20 jobs/s, seed 7):
     resolutions {256^2, 512^2, 1920x1080, 3840x2160}, spp {1, 4, 16, 64},
@@ -226,7 +286,7 @@ def main():
 
     torch.cuda.set_device(0)
     if a.workload != "extend":
-        {"denoise": bench_denoise, "svgf": bench_svgf, "serve": bench_serve}[a.workload](a, np, torch, lh, scenes)
+        {"denoise": bench_denoise, "svgf": bench_svgf, "serve": bench_serve, "frame": bench_frame}[a.workload](a, np, torch, lh, scenes)
         return
     if a.copy_bytes:
         src = torch.ones(a.copy_bytes // 4, dtype=torch.float32, device="cuda")

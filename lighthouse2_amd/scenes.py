@@ -268,6 +268,119 @@ def disney_cornell_box(seed=1234, sphere_tris=4000):
     return s
 
 
+def displaced_sphere(n_tris, radius=1.0, seed=0, bumps=12, amplitude=0.15):
+    """Noise-displaced sphere of about n_tris triangles: a latitude/longitude
+    sphere whose radius is modulated by a seeded sum of smooth plane waves;
+    spherical uv (4 x 2 repeats). Returns v0, v1, v2 (T,3) and uv (T,3,2), float32."""
+    rng = np.random.default_rng(seed)
+    stacks = max(4, int(round(np.sqrt(n_tris / 4.0))))
+    slices = max(4, int(round(n_tris / (2.0 * (stacks - 1)))))
+    th = np.linspace(0.0, np.pi, stacks + 1)
+    ph = np.linspace(0.0, 2.0 * np.pi, slices + 1)
+    d = np.stack([np.sin(th)[:, None] * np.cos(ph)[None, :], np.cos(th)[:, None] * np.ones_like(ph)[None, :],
+                  np.sin(th)[:, None] * np.sin(ph)[None, :]], -1)
+    k = rng.normal(size=(bumps, 3))
+    k *= rng.uniform(1.5, 6.0, (bumps, 1)) / np.linalg.norm(k, axis=1, keepdims=True)
+    phase = rng.uniform(0, 2 * np.pi, bumps)
+    wgt = rng.uniform(0.3, 1.0, bumps)
+    disp = (np.sin(d @ k.T + phase) * wgt).sum(-1) / wgt.sum()
+    P = d * (radius * (1.0 + amplitude * disp))[..., None]
+    uvg = np.stack([np.broadcast_to(ph[None, :] / (2 * np.pi) * 4.0, th[:, None].shape[:1] + ph.shape),
+                    np.broadcast_to(th[:, None] / np.pi * 2.0, th.shape + ph.shape[:1])], -1)
+    i, j = np.meshgrid(np.arange(stacks), np.arange(slices), indexing="ij")
+    i, j = i.reshape(-1), j.reshape(-1)
+    top = i > 0                                   # triangle (00, 01, 10) exists below the north pole
+    bot = i < stacks - 1                          # triangle (01, 11, 10) exists above the south pole
+    a = np.concatenate([P[i[top], j[top]], P[i[bot], j[bot] + 1]])
+    b = np.concatenate([P[i[top], j[top] + 1], P[i[bot] + 1, j[bot] + 1]])
+    c = np.concatenate([P[i[top] + 1, j[top]], P[i[bot] + 1, j[bot]]])
+    ua = np.concatenate([uvg[i[top], j[top]], uvg[i[bot], j[bot] + 1]])
+    ub = np.concatenate([uvg[i[top], j[top] + 1], uvg[i[bot] + 1, j[bot] + 1]])
+    uc = np.concatenate([uvg[i[top] + 1, j[top]], uvg[i[bot] + 1, j[bot]]])
+    return a.astype(f32), b.astype(f32), c.astype(f32), np.stack([ua, ub, uc], 1).astype(f32)
+
+
+def noise_texture(size=2048, seed=4):
+    """Procedural ARGB32 texture (value noise at three octaves times seeded
+    stripes), opaque, 5 MIP levels."""
+    rng = np.random.default_rng(seed)
+    n = min(size, 512)                            # noise resolution, replicated up to size
+    img = np.zeros((n, n, 3), f32)
+    t = (np.arange(n) / n).astype(f32)
+    for octave, amp in ((8, 0.5), (32, 0.3), (128, 0.2)):
+        g = rng.random((octave + 1, octave + 1, 3)).astype(f32)
+        f = t * octave
+        i = np.floor(f).astype(int)
+        w = (f - i).astype(f32)
+        gx = g[:, i] * (1 - w)[None, :, None] + g[:, i + 1] * w[None, :, None]          # (octave+1, n, 3)
+        v = gx[i] * (1 - w)[:, None, None] + gx[i + 1] * w[:, None, None]              # (n, n, 3)
+        img += f32(amp) * v
+    r = size // n
+    ang, freq = rng.uniform(0, np.pi), rng.uniform(6, 24)
+    t = (np.arange(size) / size).astype(f32)
+    stripes = f32(0.75) + f32(0.25) * np.sin((t[None, :] * f32(np.cos(ang)) + t[:, None] * f32(np.sin(ang))) * f32(freq * 2 * np.pi))
+    ch = [np.clip(np.kron(img[..., k], np.ones((r, r), f32)) * stripes * f32(255), 0, 255) for k in range(3)]
+    px = pack_texels(ch[0], ch[1], ch[2], np.full((size, size), 255))
+    return {"width": size, "height": size, "flags": 0, "storage": F.ARGB32, "pixels": mip_chain(px)}
+
+
+def _random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def instanced_scene(meshes=10, tris=(50000, 200000), instances=1000, tex_size=2048, lights=64, extent=50.0,
+                    seeds=(2, 3, 4, 5)):
+    """Instanced scene: `meshes` noise-displaced spheres (seeds 0..meshes-1;
+    triangle counts uniform in `tris`, seed seeds[0]), `instances` instances
+    with random affine transforms (uniform rotation, uniform scale U(0.5, 2),
+    translation U[-extent, extent]^3; seed seeds[1]), one procedural ARGB32
+    texture of tex_size^2 per mesh (seed seeds[2] + k), Disney materials
+    (diffuse texture, GGX specular 0.5, roughness U(0.05, 1); seed seeds[3])
+    and `lights` single-triangle area lights facing down above the cloud."""
+    s = Scene()
+    rng_t = np.random.default_rng(seeds[0])
+    counts = rng_t.integers(tris[0], tris[1] + 1, meshes)
+    rng_m = np.random.default_rng(seeds[3])
+    for k in range(meshes):
+        tex = s.add_texture(noise_texture(tex_size, seed=seeds[2] + k))
+        mat = F.disney_material((1.0, 1.0, 1.0), specular=0.5, roughness=float(rng_m.uniform(0.05, 1.0)))
+        mat["color"]["textureID"] = tex
+        mat["color"]["uvscale"] = 1.0
+        mat["color"]["uvoffset"] = 0.0
+        m = s.add_material(mat)
+        a, b, c, uv = displaced_sphere(int(counts[k]), 1.0, seed=k)
+        v, t = add_triangles(a, b, c, m)
+        for q in range(3):
+            t[f"u{q}"], t[f"v{q}"] = uv[:, q, 0], uv[:, q, 1]
+        t["LOD"] = triangle_lod(t, tex_size, tex_size)
+        s.add_mesh(v, t)
+    rng_i = np.random.default_rng(seeds[1])
+    for _ in range(instances):
+        M = np.eye(4)
+        M[:3, :3] = _random_rotation(rng_i) * rng_i.uniform(0.5, 2.0)
+        M[:3, 3] = rng_i.uniform(-extent, extent, 3)
+        s.add_instance(int(rng_i.integers(0, meshes)), M.reshape(16).astype(f32))
+    if lights:
+        rng_l = np.random.default_rng(seeds[3] + 100)
+        lm = s.add_material(F.lambert_material((20.0, 18.0, 15.0)))
+        c = np.column_stack([rng_l.uniform(-extent, extent, lights), np.full(lights, 1.2 * extent), rng_l.uniform(-extent, extent, lights)])
+        size = 0.08 * extent
+        v0, v1, v2 = c + [-size, 0, -size], c + [size, 0, -size], c + [0, 0, size]   # wound to face -y
+        v, t = add_triangles(v0, v1, v2, lm)
+        li = s.add_instance(s.add_mesh(v, t))
+        add_emitters(s, t, (20.0, 18.0, 15.0), li)
+    return s
+
+
+def instanced_view(width=1920, height=1080, extent=50.0):
+    return camera_view((0.0, 0.25 * extent, -2.4 * extent), (0.0, 0.0, 0.0), 45.0, width, height)
+
+
 def cornell_view(width=256, height=256):
     return camera_view((0.5, 0.5, -1.2), (0.5, 0.5, 0.5), 40.0, width, height)
 

@@ -89,3 +89,13 @@ def test_filter_off_is_plain_accumulation(torch_cuda):
     s = scenes.disney_cornell_box(sphere_tris=2000)
     v = scenes.camera_view((0.5, 0.5, -1.2), (0.5, 0.5, 0.5), 40.0, 40, 32, aperture=0.01)
     run(s, [(v, orc.RESTART), (v, orc.CONVERGE), (v, orc.CONVERGE)], 40, 32, 3, filter_on=False)
+
+
+def test_instanced_textured_scene(torch_cuda):
+    """A small instance of the instanced workload: displaced-sphere meshes
+    with textures and Disney materials, randomly transformed instances,
+    single-triangle area lights."""
+    s = scenes.instanced_scene(meshes=3, tris=(2000, 4000), instances=40, tex_size=64, lights=8, extent=10.0)
+    v = scenes.instanced_view(80, 48, extent=10.0)
+    img, _ = run(s, [(v, orc.RESTART), (v, orc.RESTART), (v, orc.CONVERGE)], 80, 48, 1)
+    assert img[1:-1, 1:-1, :3].mean() > 0
