@@ -44,10 +44,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent traversal blocks per CU (0: core default)")
     ap.add_argument("--min-rays-per-lane", type=float, default=0.0, help="shrink the grid of small waves (0: off)")
-    ap.add_argument("--workload", choices=["extend", "denoise", "svgf", "serve", "frame"], default="extend",
+    ap.add_argument("--workload", choices=["extend", "denoise", "svgf", "serve", "frame", "learned"], default="extend",
                     help="extend: the headline traversal metric; denoise: U-Net forward/train at 1088x1920; "
                          "svgf: filter ms/frame at 1920x1080; serve: job-server latency under Poisson arrivals; "
-                         "frame: instanced scene, 1080p 1 spp + SVGF")
+                         "frame: instanced scene, 1080p 1 spp + SVGF; learned: denoiser trained on rendered pairs vs SVGF")
     ap.add_argument("--serve-jobs", type=int, default=40)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (FETCH_SIZE/WRITE_SIZE calibration)")
@@ -230,6 +230,56 @@ def bench_frame(a, np, torch, lh, scenes):
     core.Shutdown()
 
 
+def bench_learned(a, np, torch, lh, scenes):
+    """The learned denoiser against SVGF on the instanced workload: training
+    pairs rendered by the core (1 spp filter-estimator frames with their
+    G-buffer vs 1024-spp renders of the same poses, 256x256), trained with the
+    library's own kernels; relMSE on held-out poses for the 1-spp frame, SVGF
+    (one frame, and after 4 still frames of history) and the network; and
+    ms/frame of SVGF and of the network's forward pass at 1080p."""
+    from lighthouse2_amd import denoiser as D
+    from lighthouse2_amd import training as T
+    t0 = time.time()
+    s = scenes.instanced_scene()
+    w = h = 256
+    n_train, n_eval = 96, 16
+    pr = T.PairRenderer(s, w, h, ref_spp=1024, spp_per_frame=128)
+    views = T.views_near_instances(s, n_train + n_eval, w, h, seed=6)
+    data = [pr.render(v) for v in views]
+    hist = [pr.render(v, history=4)["svgf"] for v in views[n_train:]]
+    pr.close()
+    t_data = time.time() - t0
+    samples = [(T.network_inputs(torch, d), T.network_target(torch, d)) for d in data[:n_train]]
+    net = D.UNetDenoiser(h, w, seed=0)
+    t0 = time.time()
+    steps = max(200, a.steps * 200)
+    T.train(net, torch, samples, steps, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    err = {"1spp": [], "svgf": [], "svgf_4_frames": [], "learned": []}
+    for d, sv4 in zip(data[n_train:], hist):
+        net.set_input(*T.network_inputs(torch, d))
+        out = T.network_radiance(net.forward().cpu().numpy(), d)
+        err["1spp"].append(T.relmse(d["radiance"], d["ref"]))
+        err["svgf"].append(T.relmse(d["svgf"][1:-1, 1:-1], d["ref"][1:-1, 1:-1]))
+        err["svgf_4_frames"].append(T.relmse(sv4[1:-1, 1:-1], d["ref"][1:-1, 1:-1]))
+        err["learned"].append(T.relmse(out, d["ref"]))
+    err = {k: round(float(np.mean(v)), 5) for k, v in err.items()}
+    # ms/frame at 1080p: network forward (1088x1920 padded) and SVGF
+    net2 = D.UNetDenoiser(1088, 1920, seed=0, params=net.params_numpy())
+    z = torch.zeros((1088, 1920, 4), dtype=torch.float32, device="cuda")
+    net2.set_input(z, z, z, z[..., 0].contiguous())
+    for _ in range(3):
+        net2.forward()
+    fwd_ms = _events(torch, net2.stream, net2.forward, 10)
+    print(json.dumps({"metric": "relMSE vs 1024 spp, held-out poses", "value": err["learned"], "unit": "relMSE", "higher_is_better": False,
+                      "n_gpus": 1, "steps": steps, "warmup": 0, "dtype": "bf16 (fp32 accumulate)", "data": "rendered by the core",
+                      "vs_baseline": None, "relmse": err, "learned_forward_ms_1080p": round(fwd_ms, 3),
+                      "config": {"workload": f"instanced scene, {w}x{h}, {n_train} training / {n_eval} held-out poses, "
+                                             f"1 spp inputs vs 1024 spp targets, {steps} Adam steps (batch 1)"},
+                      "data_s": round(t_data, 1), "train_s": round(t_train, 1)}))
+
+
 def bench_serve(a, np, torch, lh, scenes):
     """Job server under Poisson arrivals (lambda 20 jobs/s, seed 7):
     resolutions {256^2, 512^2, 1920x1080, 3840x2160}, spp {1, 4, 16, 64},
@@ -286,7 +336,8 @@ def main():
 
     torch.cuda.set_device(0)
     if a.workload != "extend":
-        {"denoise": bench_denoise, "svgf": bench_svgf, "serve": bench_serve, "frame": bench_frame}[a.workload](a, np, torch, lh, scenes)
+        {"denoise": bench_denoise, "svgf": bench_svgf, "serve": bench_serve, "frame": bench_frame,
+         "learned": bench_learned}[a.workload](a, np, torch, lh, scenes)
         return
     if a.copy_bytes:
         src = torch.ones(a.copy_bytes // 4, dtype=torch.float32, device="cuda")
