@@ -183,6 +183,11 @@ enum { LH2HIP_DN_RELU = 1, LH2HIP_DN_OUT_F32 = 2 };
  * LH2HIP_DN_OUT_F32); Cout <= 64. With the flipped weights of lh2hip_dn_adam
  * this is also the input gradient of a convolution. */
 LH2HIP_API int lh2hip_dn_conv3x3(const void* x, int Cin, const void* w, const float* bias, void* y, int Cout, int H, int W, int flags, void* stream);
+/* The convolution above (Cout 32) of the 64-channel concatenation
+ * [nearest 2x upsample of coarse [H/2][W/2][32], skip [H][W][32]] without
+ * materialising it (H, W even): lh2hip_dn_upcat then lh2hip_dn_conv3x3. */
+LH2HIP_API int lh2hip_dn_conv3x3_cat(const void* coarse, const void* skip, const void* w, const float* bias, void* y, int Cout, int H, int W,
+	int flags, void* stream);
 LH2HIP_API uint64_t lh2hip_dn_dw_workspace_bytes(int Cin, int Cout);
 /* dw [9][Cout][Cin] and db [Cout] (fp32, overwritten) of a 3x3 convolution
  * from its input x and output gradient dy; ws: workspace bytes above. */

@@ -313,6 +313,14 @@ LH2HIP_API int lh2hip_dn_conv3x3(const void* x, int Cin, const void* w, const fl
 		"dn_conv3x3 (Cin, Cout multiples of 16, Cout <= 64)");
 }
 
+LH2HIP_API int lh2hip_dn_conv3x3_cat(const void* coarse, const void* skip, const void* w, const float* bias, void* y, int Cout, int H, int W,
+	int flags, void* stream)
+{
+	if (!coarse || !skip || !w || !y) return fail(LH2HIP_ERR_INVALID, "dn_conv3x3_cat: null pointer");
+	return dn_status(lh::dn_conv3x3_cat((const lh::bf16_t*)coarse, (const lh::bf16_t*)skip, (const lh::bf16_t*)w, bias, y, Cout, H, W, flags,
+		(hipStream_t)stream), "dn_conv3x3_cat (32 + 32 channels in, Cout 32, even H, W)");
+}
+
 LH2HIP_API uint64_t lh2hip_dn_dw_workspace_bytes(int Cin, int Cout) { return lh::dn_dw_workspace_bytes(Cin, Cout); }
 
 LH2HIP_API int lh2hip_dn_conv3x3_dw(const void* x, int Cin, const void* dy, int Cout, int H, int W, float* dw, float* db, void* ws, void* stream)

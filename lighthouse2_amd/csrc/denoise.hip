@@ -93,6 +93,166 @@ __global__ void __launch_bounds__(256) k_conv3x3(const bf16_t* __restrict__ x, i
 	}
 }
 
+/* Tiled variant for the network's shapes: a block of 4 waves computes 4 rows
+ * x 64 pixels (one row per wave). The 6 x 66 pixel input halo is staged once
+ * in LDS with 16-byte loads, all of a thread's loads in flight together, CP
+ * channels per pixel (8 of padding against bank conflicts). Each wave then
+ * runs the 9 taps as v_mfma_f32_16x16x32_bf16 products with
+ * A = 16 output channels x 32 input channels of the [tap][Cout][Cin] weights
+ * (read from memory one tap ahead of their use) and B = 32 input channels x
+ * 16 pixels from LDS (lane: pixel lane % 16, channels 8 (lane / 16) + 0..7);
+ * D then gives each lane 4 consecutive output channels of one pixel, stored
+ * as one 8-byte (bf16) or 16-byte (fp32) write.
+ * CIN = 16 is padded to K = 32 with zero channels. CAT: the input is the
+ * 64-channel concatenation [nearest 2x upsample of coarse (32 channels),
+ * x (32 channels)]; the 4 x 34 coarse pixels under the tile are staged
+ * separately and upsampled by the fragment addressing. */
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+template <int CIN, int NB, bool RELU, bool F32OUT, bool CAT>
+__global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x, const bf16_t* __restrict__ coarse,
+	const bf16_t* __restrict__ w, const float* __restrict__ bias, void* __restrict__ y, int H, int W)
+{
+	constexpr int CX = CAT ? 32 : CIN;                  /* channels of x */
+	constexpr int CKX = CX < 32 ? 32 : CX;              /* channels per pixel of the x tile (K, padded) */
+	constexpr int CP = CKX + 8, TW = 66, TR = 6, CH8 = CX / 8, CHP = CKX / 8;
+	constexpr int KS = CAT ? 2 : CKX / 32;              /* 32-channel steps of K per tap */
+	constexpr int WCIN = CAT ? 64 : CIN;                /* weight row length */
+	constexpr int CPC = 40, TWC = 34, TRC = 4;          /* coarse tile */
+	constexpr int Cout = NB * 16;
+	__shared__ __attribute__((aligned(16))) bf16_t xs[TR * TW * CP];
+	__shared__ __attribute__((aligned(16))) bf16_t cs[CAT ? TRC * TWC * CPC : 8];
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, ml = lane & 15, kg = lane >> 4;
+	const int x0 = blockIdx.x * 64, r0 = blockIdx.y * 4;
+	{
+		constexpr int N = TR * TW * CHP, IT = (N + 255) / 256, BATCH = 8;
+#pragma unroll
+		for (int it0 = 0; it0 < IT; it0 += BATCH)
+		{
+			uint4 v[BATCH];
+#pragma unroll
+			for (int j = 0; j < BATCH; j++)
+			{
+				const int i = tid + (it0 + j) * 256;
+				const int c8 = i % CHP, p = (i / CHP) % TW, gy = r0 - 1 + i / (CHP * TW), gx = x0 - 1 + p;
+				v[j] = it0 + j < IT && i < N && c8 < CH8 && gy >= 0 && gy < H && gx >= 0 && gx < W ?
+					*(const uint4*)(x + ((size_t)gy * W + gx) * CX + c8 * 8) : make_uint4(0, 0, 0, 0);
+			}
+#pragma unroll
+			for (int j = 0; j < BATCH; j++)
+			{
+				const int i = tid + (it0 + j) * 256;
+				if (it0 + j < IT && i < N) *(uint4*)(xs + (i / CHP) * CP + (i % CHP) * 8) = v[j];
+			}
+		}
+		if (CAT)
+		{
+			constexpr int NC = TRC * TWC * 4, ITC = (NC + 255) / 256;
+			const int Hc = H >> 1, Wc = W >> 1;
+			uint4 v[ITC];
+#pragma unroll
+			for (int j = 0; j < ITC; j++)
+			{
+				const int i = tid + j * 256;
+				const int c8 = i % 4, p = (i / 4) % TWC, cy = (r0 >> 1) - 1 + i / (4 * TWC), cx = (x0 >> 1) - 1 + p;
+				v[j] = i < NC && cy >= 0 && cy < Hc && cx >= 0 && cx < Wc ? *(const uint4*)(coarse + ((size_t)cy * Wc + cx) * 32 + c8 * 8) : make_uint4(0, 0, 0, 0);
+			}
+#pragma unroll
+			for (int j = 0; j < ITC; j++)
+			{
+				const int i = tid + j * 256;
+				if (i < NC) *(uint4*)(cs + (i / 4) * CPC + (i % 4) * 8) = v[j];
+			}
+		}
+	}
+	auto wfrag = [&](int tap, int ks, int nb) -> bf8v {
+		const int k = ks * 32 + kg * 8;
+		return k < WCIN ? *(const bf8v*)(w + ((size_t)(tap * Cout + nb * 16 + ml) * WCIN + k)) : (bf8v){ 0, 0, 0, 0, 0, 0, 0, 0 };
+	};
+	bf8v wc[KS][NB], wn[KS][NB];
+#pragma unroll
+	for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+		for (int nb = 0; nb < NB; nb++) wc[ks][nb] = wfrag(0, ks, nb);
+	f4v acc[4][NB];
+#pragma unroll
+	for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+		for (int nb = 0; nb < NB; nb++) acc[mb][nb] = (f4v){ 0.0f, 0.0f, 0.0f, 0.0f };
+	__syncthreads();
+#pragma unroll 1
+	for (int tap = 0; tap < 9; tap++)
+	{
+		const int dy = tap / 3, dx = tap % 3;
+		if (tap < 8)
+#pragma unroll
+			for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++) wn[ks][nb] = wfrag(tap + 1, ks, nb);
+#pragma unroll
+		for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+			for (int mb = 0; mb < 4; mb++)
+			{
+				const int m = mb * 16 + ml;
+				bf8v a;
+				if (CAT && ks == 0) a = *(const bf8v*)(cs + ((((wave + dy - 1) >> 1) + 1) * TWC + ((m + dx - 1) >> 1) + 1) * CPC + kg * 8);
+				else a = *(const bf8v*)(xs + ((wave + dy) * TW + m + dx) * CP + (CAT ? 0 : ks * 32) + kg * 8);
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++) acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wc[ks][nb], a, acc[mb][nb], 0, 0, 0);
+			}
+#pragma unroll
+		for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+			for (int nb = 0; nb < NB; nb++) wc[ks][nb] = wn[ks][nb];
+	}
+	const int ry = r0 + wave;
+	if (ry >= H) return;
+#pragma unroll
+	for (int nb = 0; nb < NB; nb++)
+	{
+		const int ch = nb * 16 + kg * 4;
+		float bv[4];
+#pragma unroll
+		for (int i = 0; i < 4; i++) bv[i] = bias ? bias[ch + i] : 0.0f;
+#pragma unroll
+		for (int mb = 0; mb < 4; mb++)
+		{
+			const int px = x0 + mb * 16 + ml;
+			if (px >= W) continue;
+			float v[4];
+#pragma unroll
+			for (int i = 0; i < 4; i++)
+			{
+				v[i] = acc[mb][nb][i] + bv[i];
+				if (RELU) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+			}
+			const size_t o = ((size_t)ry * W + px) * Cout + ch;
+			if (F32OUT) *(float4*)((float*)y + o) = make_float4(v[0], v[1], v[2], v[3]);
+			else *(uint2*)((bf16_t*)y + o) = make_uint2((uint32_t)f2bf(v[0]) | (uint32_t)f2bf(v[1]) << 16,
+				(uint32_t)f2bf(v[2]) | (uint32_t)f2bf(v[3]) << 16);
+		}
+	}
+}
+
+template <int CIN, int NB, bool CAT>
+static hipError_t conv_t(const bf16_t* x, const bf16_t* coarse, const bf16_t* w, const float* bias, void* y, int H, int W, int flags, hipStream_t s)
+{
+	const dim3 grid((W + 63) / 64, (H + 3) / 4), block(256);
+	const bool relu = flags & DN_RELU, f32 = flags & DN_OUT_F32;
+	if (relu && f32) k_conv3x3_t<CIN, NB, true, true, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W);
+	else if (relu) k_conv3x3_t<CIN, NB, true, false, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W);
+	else if (f32) k_conv3x3_t<CIN, NB, false, true, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W);
+	else k_conv3x3_t<CIN, NB, false, false, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W);
+	return hipGetLastError();
+}
+
+hipError_t dn_conv3x3_cat(const bf16_t* coarse, const bf16_t* skip, const bf16_t* w, const float* bias, void* y, int Cout, int H, int W,
+	int flags, hipStream_t s)
+{
+	if (Cout != 32 || H <= 0 || W <= 0 || H % 2 || W % 2) return hipErrorInvalidValue;
+	return conv_t<64, 2, true>(skip, coarse, w, bias, y, H, W, flags, s);
+}
+
 template <int NB>
 static hipError_t conv_nb(const bf16_t* x, int Cin, const bf16_t* w, const float* bias, void* y, int H, int W, int flags, hipStream_t s)
 {
@@ -108,6 +268,15 @@ static hipError_t conv_nb(const bf16_t* x, int Cin, const bf16_t* w, const float
 hipError_t dn_conv3x3(const bf16_t* x, int Cin, const bf16_t* w, const float* bias, void* y, int Cout, int H, int W, int flags, hipStream_t s)
 {
 	if (Cin <= 0 || Cin % 16 || Cout <= 0 || Cout % 16 || Cout > 64 || H <= 0 || W <= 0) return hipErrorInvalidValue;
+	if (Cin == 16 && Cout == 32) return conv_t<16, 2, false>(x, nullptr, w, bias, y, H, W, flags, s);
+	if (Cin == 16 && Cout == 16) return conv_t<16, 1, false>(x, nullptr, w, bias, y, H, W, flags, s);
+	if (Cin == 32)
+	{
+		if (Cout == 16) return conv_t<32, 1, false>(x, nullptr, w, bias, y, H, W, flags, s);
+		if (Cout == 32) return conv_t<32, 2, false>(x, nullptr, w, bias, y, H, W, flags, s);
+		if (Cout == 64) return conv_t<32, 4, false>(x, nullptr, w, bias, y, H, W, flags, s);
+	}
+	if (Cin == 64 && Cout == 32) return conv_t<64, 2, false>(x, nullptr, w, bias, y, H, W, flags, s);
 	switch (Cout / 16)
 	{
 	case 1: return conv_nb<1>(x, Cin, w, bias, y, H, W, flags, s);
@@ -192,13 +361,25 @@ __global__ void __launch_bounds__(256) k_conv3x3_dw(const bf16_t* __restrict__ x
 }
 
 /* per-block partial sums of dy over channels (bias gradient) */
-__global__ void k_bias_partial(const bf16_t* __restrict__ dy, int C, size_t pixels, float* __restrict__ partial)
+/* Bias gradient partial sums: block b sums a contiguous range of pixels;
+ * 256 threads = (256 / C) pixel lanes x C channels read whole pixels
+ * (coalesced); the pixel lanes are then added in lane order. */
+__global__ void __launch_bounds__(256) k_bias_partial(const bf16_t* __restrict__ dy, int C, size_t pixels, float* __restrict__ partial)
 {
-	for (int c = threadIdx.x; c < C; c += blockDim.x)
+	__shared__ float red[256];
+	const int P = 256 / C, c = threadIdx.x % C, pl = threadIdx.x / C;
+	const size_t chunk = (pixels + gridDim.x - 1) / gridDim.x;
+	const size_t p0 = blockIdx.x * chunk, p1 = p0 + chunk < pixels ? p0 + chunk : pixels;
+	float s = 0.0f;
+	if (pl < P)
+		for (size_t p = p0 + pl; p < p1; p += P) s += bf2f(dy[p * C + c]);
+	red[threadIdx.x] = s;
+	__syncthreads();
+	if (threadIdx.x < C)
 	{
-		float s = 0.0f;
-		for (size_t p = blockIdx.x; p < pixels; p += gridDim.x) s += bf2f(dy[p * C + c]);
-		partial[(size_t)blockIdx.x * C + c] = s;
+		float t = 0.0f;
+		for (int k = 0; k < P; k++) t += red[k * C + threadIdx.x];
+		partial[(size_t)blockIdx.x * C + threadIdx.x] = t;
 	}
 }
 
@@ -236,7 +417,7 @@ hipError_t dn_conv3x3_dw(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, i
 	if (db)
 	{
 		float* bp = partial + (size_t)DW_BLOCKS * n;
-		k_bias_partial<<<DW_BLOCKS, 64, 0, s>>>(dy, Cout, (size_t)H * W, bp);
+		k_bias_partial<<<DW_BLOCKS, 256, 0, s>>>(dy, Cout, (size_t)H * W, bp);
 		k_reduce_partials<<<1, 64, 0, s>>>(bp, DW_BLOCKS, Cout, db);
 	}
 	return hipGetLastError();

@@ -7,6 +7,8 @@ Network (channels): input 16 = [radiance 3, albedo 3, normal 3, depth 1, 0 x 6]
   encoder level l = 0..3: a = relu(conv(in)), b = relu(conv(a)), in_{l+1} = avgpool2(b)
   bottleneck (level 4):   a = relu(conv(in)), b = relu(conv(a))
   decoder level l = 3..0: cat = [upsample2(prev), b_l], c = relu(conv(cat)), d = relu(conv(c))
+                          (the forward reads cat from its two halves; the backward
+                          pass materialises it for the weight gradient)
   output: conv(d_0) -> 16 channels fp32, the first 3 are the denoised radiance.
 Loss: relMSE over the 3 output channels. Optimiser: Adam on fp32 master weights.
 Torch tensors serve as device buffers only; every operation is a kernel of
@@ -143,8 +145,9 @@ class UNetDenoiser:
         prev = self.a[f"b_{LEVELS - 1}"]
         for lvl in range(LEVELS - 2, -1, -1):
             h, w = H >> lvl, W >> lvl
-            _check(L.lh2hip_dn_upcat(prev.data_ptr(), CH, self.a[f"b_{lvl}"].data_ptr(), CH, h, w, self.a[f"cat_{lvl}"].data_ptr(), self.sh))
-            self._conv(self.a[f"cat_{lvl}"], f"d1_{lvl}", self.a[f"c_{lvl}"], h, w, 1)
+            d = self.p[f"d1_{lvl}"]
+            _check(L.lh2hip_dn_conv3x3_cat(prev.data_ptr(), self.a[f"b_{lvl}"].data_ptr(), d["wf"].data_ptr(), d["b"].data_ptr(),
+                                           self.a[f"c_{lvl}"].data_ptr(), CH, h, w, 1, self.sh))
             self._conv(self.a[f"c_{lvl}"], f"d2_{lvl}", self.a[f"d_{lvl}"], h, w, 1)
             prev = self.a[f"d_{lvl}"]
         self._conv(self.a["d_0"], "out", self.out, H, W, 2)
@@ -166,6 +169,8 @@ class UNetDenoiser:
             self._conv_dw(a[f"c_{lvl}"], f"d2_{lvl}", tmp, h, w)
             self._conv_dx(tmp, f"d2_{lvl}", g[f"c_{lvl}"], h, w)
             self._relu_bwd(a[f"c_{lvl}"], g[f"c_{lvl}"], tmp[:h * w * CH])
+            prev = a[f"d_{lvl + 1}"] if lvl + 1 < LEVELS - 1 else a[f"b_{LEVELS - 1}"]
+            _check(L.lh2hip_dn_upcat(prev.data_ptr(), CH, a[f"b_{lvl}"].data_ptr(), CH, h, w, a[f"cat_{lvl}"].data_ptr(), self.sh))
             self._conv_dw(a[f"cat_{lvl}"], f"d1_{lvl}", tmp, h, w)
             self._conv_dx(tmp, f"d1_{lvl}", g[f"cat_{lvl}"], h, w)
             coarse = g[f"d_{lvl + 1}"] if lvl + 1 < LEVELS - 1 else g[f"b_{LEVELS - 1}"]

@@ -121,3 +121,75 @@ def test_training_reduces_relmse(torch_cuda):
         net.step(lr=2e-3)
     assert np.isfinite(losses).all()
     assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
+
+
+def test_conv_of_upsampled_concatenation_matches_fp64(torch_cuda):
+    """The decoder's first convolution read straight from the coarse level and
+    the skip connection equals the convolution of their concatenation."""
+    torch = torch_cuda
+    L = lh.library()
+    rng = np.random.default_rng(21)
+    H, W = 22, 70                                   # even, ragged against the 4 x 64 tile
+    coarse = R.bf16(rng.standard_normal((H // 2, W // 2, 32)))
+    skip = R.bf16(rng.standard_normal((H, W, 32)))
+    w = R.bf16(rng.standard_normal((9, 32, 64)) * 0.2)
+    b = rng.standard_normal(32).astype(np.float32)
+    sh = torch.cuda.current_stream().cuda_stream
+    dc, ds, dw_ = dev_bf16(torch, coarse), dev_bf16(torch, skip), dev_bf16(torch, w)
+    db_ = torch.from_numpy(b).cuda()
+    y = torch.zeros(H * W * 32, dtype=torch.float32, device="cuda")
+    assert L.lh2hip_dn_conv3x3_cat(dc.data_ptr(), ds.data_ptr(), dw_.data_ptr(), db_.data_ptr(), y.data_ptr(), 32, H, W, 2, sh) == 0
+    cat = torch.zeros(H * W * 64, dtype=torch.int16, device="cuda")
+    assert L.lh2hip_dn_upcat(dc.data_ptr(), 32, ds.data_ptr(), 32, H, W, cat.data_ptr(), sh) == 0
+    y2 = torch.zeros_like(y)
+    assert L.lh2hip_dn_conv3x3(cat.data_ptr(), 64, dw_.data_ptr(), db_.data_ptr(), y2.data_ptr(), 32, H, W, 2, sh) == 0
+    torch.cuda.synchronize()
+    ref = R.conv(R.upcat(coarse, skip), w, b)
+    got = host_f32(y).reshape(H, W, 32)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+    assert np.array_equal(host_f32(y), host_f32(y2))
+    assert L.lh2hip_dn_conv3x3_cat(dc.data_ptr(), ds.data_ptr(), dw_.data_ptr(), None, y.data_ptr(), 32, H + 1, W, 2, sh) != 0
+
+
+def _conv_rows(x, w, b, rows=256):
+    """FP64 conv3x3 (as denoiser_ref.conv) computed in row blocks through matmul."""
+    H, W, C = x.shape
+    wm = w.transpose(0, 2, 1).reshape(9 * C, -1)
+    p = np.zeros((H + 2, W + 2, C))
+    p[1:-1, 1:-1] = x
+    out = np.empty((H, W, wm.shape[1]))
+    for r in range(0, H, rows):
+        h = min(rows, H - r)
+        t = np.stack([p[r + dy:r + dy + h, dx:dx + W] for dy in range(3) for dx in range(3)], 2)
+        out[r:r + h] = (t.reshape(h * W, 9 * C) @ wm).reshape(h, W, -1)
+    return out + b
+
+
+@pytest.mark.parametrize("cin,cout,cat", [(16, 32, False), (32, 32, False), (32, 16, False), (32, 64, False), (64, 32, True)])
+def test_tall_image_walks_several_bands_per_block(torch_cuda, cin, cout, cat):
+    """An image tall enough that each block walks several 4-row bands down its
+    strip (rows fetched ahead into a ring), with ragged last strip and band."""
+    torch = torch_cuda
+    L = lh.library()
+    rng = np.random.default_rng(cin + cout)
+    H, W = 4098, 70
+    w = R.bf16(rng.standard_normal((9, cout, cin)) * 0.2)
+    b = rng.standard_normal(cout).astype(np.float32)
+    sh = torch.cuda.current_stream().cuda_stream
+    dw_, db_ = dev_bf16(torch, w), torch.from_numpy(b).cuda()
+    y = torch.zeros(H * W * cout, dtype=torch.float32, device="cuda")
+    if cat:
+        coarse = R.bf16(rng.standard_normal((H // 2, W // 2, 32)))
+        skip = R.bf16(rng.standard_normal((H, W, 32)))
+        dc, ds = dev_bf16(torch, coarse), dev_bf16(torch, skip)
+        assert L.lh2hip_dn_conv3x3_cat(dc.data_ptr(), ds.data_ptr(), dw_.data_ptr(), db_.data_ptr(), y.data_ptr(), cout, H, W, 2, sh) == 0
+        x = R.upcat(coarse, skip)
+    else:
+        x = R.bf16(rng.standard_normal((H, W, cin)))
+        dx_ = dev_bf16(torch, x)
+        assert L.lh2hip_dn_conv3x3(dx_.data_ptr(), cin, dw_.data_ptr(), db_.data_ptr(), y.data_ptr(), cout, H, W, 2, sh) == 0
+    torch.cuda.synchronize()
+    ref = _conv_rows(x, w, b)
+    got = host_f32(y).reshape(H, W, cout)
+    err = np.abs(got - ref)
+    assert err.max() <= 1e-5 * np.abs(ref).max(), np.unravel_index(err.argmax(), err.shape)
