@@ -183,6 +183,10 @@ enum { LH2HIP_DN_RELU = 1, LH2HIP_DN_OUT_F32 = 2 };
  * LH2HIP_DN_OUT_F32); Cout <= 64. With the flipped weights of lh2hip_dn_adam
  * this is also the input gradient of a convolution. */
 LH2HIP_API int lh2hip_dn_conv3x3(const void* x, int Cin, const void* w, const float* bias, void* y, int Cout, int H, int W, int flags, void* stream);
+/* y = relu(conv3x3(x, w) + bias) as above (Cin 16 or 32, Cout 32, bf16) and,
+ * written from the same tile, ypool = lh2hip_dn_pool2(y) (H, W even). */
+LH2HIP_API int lh2hip_dn_conv3x3_pool(const void* x, int Cin, const void* w, const float* bias, void* y, int Cout, int H, int W, void* ypool,
+	void* stream);
 /* The convolution above (Cout 32) of the 64-channel concatenation
  * [nearest 2x upsample of coarse [H/2][W/2][32], skip [H][W][32]] without
  * materialising it (H, W even): lh2hip_dn_upcat then lh2hip_dn_conv3x3. */

@@ -66,6 +66,7 @@ _SIGS = {
     "lh2hip_dn_pool2_bwd": (I, [P, I, I, I, P, I, P]),
     "lh2hip_dn_upcat": (I, [P, I, P, I, I, I, P, P]),
     "lh2hip_dn_conv3x3_cat": (I, [P, P, P, P, P, I, I, I, I, P]),
+    "lh2hip_dn_conv3x3_pool": (I, [P, I, P, P, P, I, I, I, P, P]),
     "lh2hip_dn_upcat_bwd": (I, [P, I, I, I, I, P, P, P]),
     "lh2hip_dn_pack_input": (I, [P, P, P, P, I, I, P, P]),
     "lh2hip_dn_relmse": (I, [P, I, P, I, I, P, P, P]),

@@ -313,6 +313,14 @@ LH2HIP_API int lh2hip_dn_conv3x3(const void* x, int Cin, const void* w, const fl
 		"dn_conv3x3 (Cin, Cout multiples of 16, Cout <= 64)");
 }
 
+LH2HIP_API int lh2hip_dn_conv3x3_pool(const void* x, int Cin, const void* w, const float* bias, void* y, int Cout, int H, int W, void* ypool,
+	void* stream)
+{
+	if (!x || !w || !y || !ypool) return fail(LH2HIP_ERR_INVALID, "dn_conv3x3_pool: null pointer");
+	return dn_status(lh::dn_conv3x3_pool((const lh::bf16_t*)x, Cin, (const lh::bf16_t*)w, bias, (lh::bf16_t*)y, Cout, H, W, (lh::bf16_t*)ypool,
+		(hipStream_t)stream), "dn_conv3x3_pool (Cin 16 or 32, Cout 32, even H, W)");
+}
+
 LH2HIP_API int lh2hip_dn_conv3x3_cat(const void* coarse, const void* skip, const void* w, const float* bias, void* y, int Cout, int H, int W,
 	int flags, void* stream)
 {

@@ -17,6 +17,10 @@ enum { DN_RELU = 1, DN_OUT_F32 = 2 };
  * w: [9][Cout][Cin] bf16 (tap = 3 * dy + dx); Cin, Cout multiples of 16,
  * Cout <= 64. flags: DN_RELU, DN_OUT_F32 (y is fp32 instead of bf16). */
 hipError_t dn_conv3x3(const bf16_t* x, int Cin, const bf16_t* w, const float* bias, void* y, int Cout, int H, int W, int flags, hipStream_t s);
+/* y = relu(conv3x3(x) + bias) (Cin 16 or 32, Cout 32, bf16) and, from the same
+ * tile, ypool = dn_pool2(y) ([H/2][W/2][32]; H, W even). */
+hipError_t dn_conv3x3_pool(const bf16_t* x, int Cin, const bf16_t* w, const float* bias, bf16_t* y, int Cout, int H, int W,
+	bf16_t* ypool, hipStream_t s);
 /* The same convolution (Cin 64, Cout 32) of the concatenation
  * [nearest 2x upsample of coarse[H/2][W/2][32], skip[H][W][32]], read from
  * the two tensors (H, W even); equal to dn_upcat followed by dn_conv3x3. */

@@ -108,9 +108,9 @@ __global__ void __launch_bounds__(256) k_conv3x3(const bf16_t* __restrict__ x, i
  * x (32 channels)]; the 4 x 34 coarse pixels under the tile are staged
  * separately and upsampled by the fragment addressing. */
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
-template <int CIN, int NB, bool RELU, bool F32OUT, bool CAT>
+template <int CIN, int NB, bool RELU, bool F32OUT, bool CAT, bool POOL = false>
 __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x, const bf16_t* __restrict__ coarse,
-	const bf16_t* __restrict__ w, const float* __restrict__ bias, void* __restrict__ y, int H, int W)
+	const bf16_t* __restrict__ w, const float* __restrict__ bias, void* __restrict__ y, int H, int W, bf16_t* __restrict__ ypool)
 {
 	constexpr int CX = CAT ? 32 : CIN;                  /* channels of x */
 	constexpr int CKX = CX < 32 ? 32 : CX;              /* channels per pixel of the x tile (K, padded) */
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 			for (int nb = 0; nb < NB; nb++) wc[ks][nb] = wn[ks][nb];
 	}
 	const int ry = r0 + wave;
-	if (ry >= H) return;
+	float hs[POOL ? 4 : 1][NB][4];                       /* POOL: sums of horizontal pixel pairs */
 #pragma unroll
 	for (int nb = 0; nb < NB; nb++)
 	{
@@ -218,7 +218,6 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 		for (int mb = 0; mb < 4; mb++)
 		{
 			const int px = x0 + mb * 16 + ml;
-			if (px >= W) continue;
 			float v[4];
 #pragma unroll
 			for (int i = 0; i < 4; i++)
@@ -227,10 +226,53 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 				if (RELU) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
 			}
 			const size_t o = ((size_t)ry * W + px) * Cout + ch;
-			if (F32OUT) *(float4*)((float*)y + o) = make_float4(v[0], v[1], v[2], v[3]);
-			else *(uint2*)((bf16_t*)y + o) = make_uint2((uint32_t)f2bf(v[0]) | (uint32_t)f2bf(v[1]) << 16,
-				(uint32_t)f2bf(v[2]) | (uint32_t)f2bf(v[3]) << 16);
+			if (ry < H && px < W)
+			{
+				if (F32OUT) *(float4*)((float*)y + o) = make_float4(v[0], v[1], v[2], v[3]);
+				else *(uint2*)((bf16_t*)y + o) = make_uint2((uint32_t)f2bf(v[0]) | (uint32_t)f2bf(v[1]) << 16,
+					(uint32_t)f2bf(v[2]) | (uint32_t)f2bf(v[3]) << 16);
+			}
+			if (POOL)                                     /* the stored (bf16) values, left + right */
+#pragma unroll
+				for (int i = 0; i < 4; i++)
+				{
+					const float r = bf2f(f2bf(v[i]));
+					hs[POOL ? mb : 0][nb][i] = r + __shfl_xor(r, 1);
+				}
 		}
+	}
+	if (POOL)
+	{
+		/* 2x2 average: (top pair) + (bottom pair), as dn_pool2 on the stored tensor.
+		 * Odd rows pass their pair sums to the even rows through LDS. */
+		float* ex = (float*)xs;
+		__syncthreads();                                  /* every wave is done with the staged tile */
+		if (wave & 1)
+#pragma unroll
+			for (int mb = 0; mb < 4; mb++)
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+					for (int i = 0; i < 4; i++) ex[(((wave >> 1) * 4 + mb) * NB * 4 + nb * 4 + i) * 64 + lane] = hs[POOL ? mb : 0][nb][i];
+		__syncthreads();
+		const int Wp = W >> 1, pr = (r0 >> 1) + (wave >> 1);
+		if (!(wave & 1) && ry + 1 < H && !(ml & 1))
+#pragma unroll
+			for (int mb = 0; mb < 4; mb++)
+			{
+				const int px = x0 + mb * 16 + ml;
+				if (px >= W) continue;
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++)
+				{
+					bf16_t q[4];
+#pragma unroll
+					for (int i = 0; i < 4; i++)
+						q[i] = f2bf((hs[POOL ? mb : 0][nb][i] + ex[(((wave >> 1) * 4 + mb) * NB * 4 + nb * 4 + i) * 64 + lane]) * 0.25f);
+					*(uint2*)(ypool + ((size_t)pr * Wp + (px >> 1)) * Cout + nb * 16 + kg * 4) =
+						make_uint2((uint32_t)q[0] | (uint32_t)q[1] << 16, (uint32_t)q[2] | (uint32_t)q[3] << 16);
+				}
+			}
 	}
 }
 
@@ -239,10 +281,20 @@ static hipError_t conv_t(const bf16_t* x, const bf16_t* coarse, const bf16_t* w,
 {
 	const dim3 grid((W + 63) / 64, (H + 3) / 4), block(256);
 	const bool relu = flags & DN_RELU, f32 = flags & DN_OUT_F32;
-	if (relu && f32) k_conv3x3_t<CIN, NB, true, true, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W);
-	else if (relu) k_conv3x3_t<CIN, NB, true, false, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W);
-	else if (f32) k_conv3x3_t<CIN, NB, false, true, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W);
-	else k_conv3x3_t<CIN, NB, false, false, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W);
+	if (relu && f32) k_conv3x3_t<CIN, NB, true, true, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
+	else if (relu) k_conv3x3_t<CIN, NB, true, false, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
+	else if (f32) k_conv3x3_t<CIN, NB, false, true, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
+	else k_conv3x3_t<CIN, NB, false, false, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
+	return hipGetLastError();
+}
+
+hipError_t dn_conv3x3_pool(const bf16_t* x, int Cin, const bf16_t* w, const float* bias, bf16_t* y, int Cout, int H, int W,
+	bf16_t* ypool, hipStream_t s)
+{
+	if (Cout != 32 || (Cin != 16 && Cin != 32) || H <= 0 || W <= 0 || H % 2 || W % 2) return hipErrorInvalidValue;
+	const dim3 grid((W + 63) / 64, (H + 3) / 4), block(256);
+	if (Cin == 16) k_conv3x3_t<16, 2, true, false, false, true><<<grid, block, 0, s>>>(x, nullptr, w, bias, y, H, W, ypool);
+	else k_conv3x3_t<32, 2, true, false, false, true><<<grid, block, 0, s>>>(x, nullptr, w, bias, y, H, W, ypool);
 	return hipGetLastError();
 }
 
@@ -538,12 +590,11 @@ __global__ void k_pack_input(const float4* __restrict__ rad, const float4* __res
 	for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < pixels; p += (size_t)gridDim.x * blockDim.x)
 	{
 		const float4 r = rad[p], a = alb[p], n = nrm[p];
-		bf16_t* o = out + p * 16;
-		o[0] = f2bf(r.x), o[1] = f2bf(r.y), o[2] = f2bf(r.z);
-		o[3] = f2bf(a.x), o[4] = f2bf(a.y), o[5] = f2bf(a.z);
-		o[6] = f2bf(n.x), o[7] = f2bf(n.y), o[8] = f2bf(n.z);
-		o[9] = f2bf(depth[p]);
-		for (int c = 10; c < 16; c++) o[c] = 0;
+		const float d = depth[p];
+		auto pk = [](float lo, float hi) { return (uint32_t)f2bf(lo) | (uint32_t)f2bf(hi) << 16; };
+		uint4* o = (uint4*)(out + p * 16);
+		o[0] = make_uint4(pk(r.x, r.y), pk(r.z, a.x), pk(a.y, a.z), pk(n.x, n.y));
+		o[1] = make_uint4(pk(n.z, d), 0, 0, 0);
 	}
 }
 

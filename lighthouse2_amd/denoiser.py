@@ -139,9 +139,12 @@ class UNetDenoiser:
         for lvl in range(LEVELS):
             h, w = H >> lvl, W >> lvl
             self._conv(self.a[f"in_{lvl}"], f"e1_{lvl}", self.a[f"a_{lvl}"], h, w, 1)
-            self._conv(self.a[f"a_{lvl}"], f"e2_{lvl}", self.a[f"b_{lvl}"], h, w, 1)
-            if lvl < LEVELS - 1:
-                _check(L.lh2hip_dn_pool2(self.a[f"b_{lvl}"].data_ptr(), CH, h, w, self.a[f"in_{lvl + 1}"].data_ptr(), self.sh))
+            if lvl < LEVELS - 1:                   # b_l and its 2x2 average in_{l+1} from one kernel
+                d = self.p[f"e2_{lvl}"]
+                _check(L.lh2hip_dn_conv3x3_pool(self.a[f"a_{lvl}"].data_ptr(), CH, d["wf"].data_ptr(), d["b"].data_ptr(),
+                                                self.a[f"b_{lvl}"].data_ptr(), CH, h, w, self.a[f"in_{lvl + 1}"].data_ptr(), self.sh))
+            else:
+                self._conv(self.a[f"a_{lvl}"], f"e2_{lvl}", self.a[f"b_{lvl}"], h, w, 1)
         prev = self.a[f"b_{LEVELS - 1}"]
         for lvl in range(LEVELS - 2, -1, -1):
             h, w = H >> lvl, W >> lvl

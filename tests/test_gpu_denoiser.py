@@ -193,3 +193,32 @@ def test_tall_image_walks_several_bands_per_block(torch_cuda, cin, cout, cat):
     got = host_f32(y).reshape(H, W, cout)
     err = np.abs(got - ref)
     assert err.max() <= 1e-5 * np.abs(ref).max(), np.unravel_index(err.argmax(), err.shape)
+
+
+@pytest.mark.parametrize("cin", [16, 32])
+def test_conv_with_fused_pool_equals_conv_then_pool(torch_cuda, cin):
+    """The encoder convolution that also writes the 2x2 average of its output
+    gives exactly dn_conv3x3 (relu, bf16) followed by dn_pool2."""
+    torch = torch_cuda
+    L = lh.library()
+    rng = np.random.default_rng(40 + cin)
+    H, W = 22, 70                                   # the last 4-row tile has only 2 rows; ragged strip
+    x = R.bf16(rng.standard_normal((H, W, cin)))
+    w = R.bf16(rng.standard_normal((9, 32, cin)) * 0.3)
+    b = (rng.standard_normal(32) * 0.1).astype(np.float32)
+    sh = torch.cuda.current_stream().cuda_stream
+    dx_, dw_, db_ = dev_bf16(torch, x), dev_bf16(torch, w), torch.from_numpy(b).cuda()
+    y = torch.zeros(H * W * 32, dtype=torch.int16, device="cuda")
+    yp = torch.zeros(H * W * 8, dtype=torch.int16, device="cuda")
+    assert L.lh2hip_dn_conv3x3_pool(dx_.data_ptr(), cin, dw_.data_ptr(), db_.data_ptr(), y.data_ptr(), 32, H, W, yp.data_ptr(), sh) == 0
+    y2 = torch.zeros_like(y)
+    yp2 = torch.zeros_like(yp)
+    assert L.lh2hip_dn_conv3x3(dx_.data_ptr(), cin, dw_.data_ptr(), db_.data_ptr(), y2.data_ptr(), 32, H, W, 1, sh) == 0
+    assert L.lh2hip_dn_pool2(y2.data_ptr(), 32, H, W, yp2.data_ptr(), sh) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(y.cpu().numpy(), y2.cpu().numpy())
+    assert np.array_equal(yp.cpu().numpy(), yp2.cpu().numpy())
+    ref = R.pool2(R.bf16(np.maximum(R.conv(x, w, b), 0)))
+    got = D.bf16_to_f32(yp.cpu().numpy()).reshape(H // 2, W // 2, 32)
+    assert np.all(np.abs(got - R.bf16(ref)) <= np.abs(R.bf16(ref)) * 2 ** -7 + 1e-30)
+    assert L.lh2hip_dn_conv3x3_pool(dx_.data_ptr(), cin, dw_.data_ptr(), None, y.data_ptr(), 32, H - 1, W, yp.data_ptr(), sh) != 0
