@@ -211,7 +211,7 @@ def bench_frame(a, np, torch, lh, scenes):
     line = {"metric": "ms/frame 1080p 1spp+denoise", "value": round(ms, 3), "unit": "ms", "higher_is_better": False, "n_gpus": 1,
             "steps": a.steps, "warmup": a.warmup, "dtype": "f32", "data": "synthetic scene", "vs_baseline": None,
             "config": {"workload": "instanced scene: 10 displaced spheres 50k-200k tris, 1000 instances, 2048^2 textures, "
-                                   "Disney (diffuse + GGX), 64 area lights; 1920x1080, 1 spp, filter estimator + SVGF",
+                                   "Disney (diffuse + GGX), 64 area lights; 1920x1080, 1 spp, filter estimator + SVGF + TAA",
                        "mesh_triangles": int(tris), "instances": len(s.instances)},
             "breakdown_ms": parts, "image_mean": float(img[1:-1, 1:-1, :3].mean()),
             "scene_build_s": round(t_scene, 1), "load_and_first_frame_s": round(t_load, 1)}
@@ -225,7 +225,7 @@ def bench_frame(a, np, torch, lh, scenes):
         ref.frame(scenes.instanced_view(w, h), orc.RESTART)
         cpu_s = time.time() - t0
         line["cpu_baseline"] = {"value": round(cpu_s * 64 * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
-                                "sample": f"oracle filter estimator + SVGF at {w}x{h} (1/64 of the pixels), scaled by 64"}
+                                "sample": f"oracle filter estimator + SVGF + TAA at {w}x{h} (1/64 of the pixels), scaled by 64"}
     print(json.dumps(line))
     core.Shutdown()
 

@@ -167,9 +167,14 @@ LH2HIP_API int lh2hip_svgf_destroy(lh2hip_svgf* f);
 LH2HIP_API int lh2hip_svgf_frame(lh2hip_svgf* f, const lh2_float4* accumulator, const uint32_t* features, const lh2_float4* worldPos,
 	const lh2_float4* deltaDepth, const ViewPyramid* prevView, float j0, float j1, float prevj0, float prevj1, int samplesTaken,
 	float clampDirect, float clampIndirect, int camIsStationary, lh2_float4* out, void* stream);
+/* TAA after the last pass (RenderCore_Optix7Filter "TAA" setting,
+ * rendercore.cpp:677,922-926): 1 = TAApass + unsharpenTAA give `out`
+ * (squared, border untouched), 0 (the default of this object) = finalizeNoTAA. */
+LH2HIP_API int lh2hip_svgf_set_taa(lh2hip_svgf* f, int enabled);
 /* Copies internal state to host memory: which 0 features (uint4), 1 luminance
  * moments (float4), 2 motion (float2), 3 phase-1 filtered history (float4),
- * 4 last-pass image (float4), 5 world positions of the last frame (float4). */
+ * 4 last-pass image (float4), 5 world positions of the last frame (float4),
+ * 6 TAA history: the last TAA output (float4). */
 LH2HIP_API int lh2hip_svgf_read(lh2hip_svgf* f, int which, void* hostOut);
 
 /* ---- learned denoiser kernels --------------------------------------------

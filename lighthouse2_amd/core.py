@@ -77,6 +77,7 @@ _SIGS = {
     "lh2hip_svgf_frame": (I, [P, P, P, P, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, I,
                               ctypes.c_float, ctypes.c_float, I, P, P]),
     "lh2hip_svgf_read": (I, [P, I, P]),
+    "lh2hip_svgf_set_taa": (I, [P, I]),
     "lh2hip_server_create": (I, [P, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(P)]),
     "lh2hip_server_destroy": (I, [P]),
     "lh2hip_server_submit": (I, [P, P, I, I, I, I, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
@@ -320,10 +321,12 @@ class Svgf:
     """SVGF filter state on the GPU (lh2hip_svgf_*); frame() takes device
     pointers, read() returns internal buffers as numpy arrays."""
 
-    def __init__(self, width, height):
+    def __init__(self, width, height, taa=False):
         h = P()
         _check(library().lh2hip_svgf_create(width, height, ctypes.byref(h)))
         self._h, self.w, self.h = h, width, height
+        if taa:
+            _check(library().lh2hip_svgf_set_taa(self._h, 1))
 
     def frame(self, acc_ptr, features_ptr, world_pos_ptr, delta_depth_ptr, prev_view, out_ptr, j=(0.0, 0.0), prev_j=(0.0, 0.0),
               samples=1, clamp_direct=2.5, clamp_indirect=15.0, stationary=0, stream=None):

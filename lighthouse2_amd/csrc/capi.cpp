@@ -424,13 +424,21 @@ LH2HIP_API int lh2hip_svgf_frame(lh2hip_svgf* f, const lh2_float4* accumulator, 
 	catch (const std::exception& e) { return fail(LH2HIP_ERR_DEVICE, e.what()); }
 }
 
+LH2HIP_API int lh2hip_svgf_set_taa(lh2hip_svgf* f, int enabled)
+{
+	if (!f || !f->f) return fail(LH2HIP_ERR_INVALID, "null svgf handle");
+	if (enabled != 0 && enabled != 1) return fail(LH2HIP_ERR_INVALID, "svgf set_taa: 0 or 1");
+	f->f->taa = enabled;
+	return LH2HIP_OK;
+}
+
 LH2HIP_API int lh2hip_svgf_read(lh2hip_svgf* f, int which, void* hostOut)
 {
 	if (!f || !f->f || !hostOut) return fail(LH2HIP_ERR_INVALID, "svgf read: null argument");
 	lh::Svgf& s = *f->f;
 	lh::DevBuf* b = which == 0 ? &s.features : which == 1 ? &s.prevMoments : which == 2 ? &s.motion : which == 3 ? &s.filteredIn
-		: which == 4 ? &s.lastPass : which == 5 ? &s.prevWorldPos : nullptr;
-	if (!b) return fail(LH2HIP_ERR_INVALID, "svgf read: which 0..5");
+		: which == 4 ? &s.lastPass : which == 5 ? &s.prevWorldPos : which == 6 ? &s.prevPixels : nullptr;
+	if (!b) return fail(LH2HIP_ERR_INVALID, "svgf read: which 0..6");
 	if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(hostOut, b->p, b->bytes, hipMemcpyDeviceToHost) != hipSuccess)
 		return fail(LH2HIP_ERR_DEVICE, "svgf read: copy failed");
 	return LH2HIP_OK;

@@ -261,8 +261,9 @@ void RenderCore::Setting(const char* name, float value)
 	else if (!strcmp(name, "clampDirect")) clampDirect = value;      /* :674 */
 	else if (!strcmp(name, "clampIndirect")) clampIndirect = value;  /* :675 */
 	else if (!strcmp(name, "TAA"))
-	{	/* :677; the TAA pass is not built: only 0 is accepted */
-		if (value != 0) throw CoreError(LH2HIP_ERR_INVALID, "Setting TAA: only 0 (TAA pass not available)");
+	{	/* :677 */
+		if (value != 0 && value != 1) throw CoreError(LH2HIP_ERR_INVALID, "Setting TAA: 0 or 1");
+		taaOn = value != 0;
 	}
 	else if (!strcmp(name, "bsdf"))
 	{	/* optix7 estimator's BSDF: the compile-time choice of kernels/bsdf.h:7-22 */
@@ -733,7 +734,8 @@ void RenderCore::checkRenderable()
 
 /* Shade and camera constants of one Render call (the reference's staged
  * __constant__ globals, lib/RenderCore_PrimeRef/rendercore.cpp:535-584). */
-FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int spp, int depth, uint32_t R0gen, int samplesTakenAtStart)
+FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int spp, int depth, uint32_t R0gen, int samplesTakenAtStart,
+	float jitterX, float jitterY)
 {
 	FrameSetup f;
 	memcpy(f.cam.pos, &view.pos, 12), memcpy(f.cam.p1, &view.p1, 12), memcpy(f.cam.p2, &view.p2, 12), memcpy(f.cam.p3, &view.p3, 12);
@@ -779,10 +781,22 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
 			f.cam.aperture = 0;
 		}
 		f.cam.distortion = 0;
+		/* TAA jitter (rendercore.cpp:781-786): p1..p3 move by (j0 / w) right + (j1 / h) up.
+		 * Eye rays span the unjittered right/up from the jittered p1 (camP1); the
+		 * shade kernel's depth derivatives use the jittered corners throughout. */
+		lighthouse2::float3 p1 = view.p1, p2 = view.p2, p3 = view.p3;
+		if (jitterX != 0 || jitterY != 0)
+		{
+			const float rx = view.p2.x - view.p1.x, ry = view.p2.y - view.p1.y, rz = view.p2.z - view.p1.z;
+			const float ux = view.p3.x - view.p1.x, uy = view.p3.y - view.p1.y, uz = view.p3.z - view.p1.z;
+			const float sx = jitterX / (float)w, sy = jitterY / (float)h;
+			const float jx = sx * rx + sy * ux, jy = sx * ry + sy * uy, jz = sx * rz + sy * uz;
+			p1.x += jx, p1.y += jy, p1.z += jz, p2.x += jx, p2.y += jy, p2.z += jz, p3.x += jx, p3.y += jy, p3.z += jz;
+		}
 		P.camPos = make_float4(view.pos.x, view.pos.y, view.pos.z, 0);
-		P.camP1 = make_float4(view.p1.x, view.p1.y, view.p1.z, 0);
-		P.camRight = make_float4(view.p2.x - view.p1.x, view.p2.y - view.p1.y, view.p2.z - view.p1.z, 0);
-		P.camUp = make_float4(view.p3.x - view.p1.x, view.p3.y - view.p1.y, view.p3.z - view.p1.z, 0);
+		P.camP1 = make_float4(p1.x, p1.y, p1.z, 0);
+		P.camRight = make_float4(p2.x - p1.x, p2.y - p1.y, p2.z - p1.z, 0);
+		P.camUp = make_float4(p3.x - p1.x, p3.y - p1.y, p3.z - p1.z, 0);
 	}
 	f.R0gen = R0gen, f.samplesTaken = samplesTakenAtStart, f.spp = spp, f.pixels = w * h;
 	if (estimator == ESTIMATOR_REFERENCE) f.maxDepth = depth;
@@ -893,7 +907,15 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	uint32_t R0gen = 0;
 	if (estimator == ESTIMATOR_OPTIX7) xor32_host(shiftSeed);   /* once per frame, rendercore_optix7/rendercore.cpp:855 */
 	else if (estimator == ESTIMATOR_REFERENCE) R0gen = xor32_host(camSeed);   /* RenderCore_PrimeRef/rendercore.cpp:556 */
-	FrameSetup f = frameParams(view, scrw, scrh, scrspp, maxDepth, R0gen, samplesTaken);
+	if (estimator == ESTIMATOR_FILTER)
+	{	/* jitter the view for TAA (RenderCore_Optix7Filter/rendercore.cpp:754-758) */
+		static const float haltonX[4] = { 0.3f, 0.7f, 0.2f, 0.8f }, haltonY[4] = { 0.2f, 0.8f, 0.7f, 0.3f };
+		j0 = taaOn && filterOn ? haltonX[frameCycle] : 0.0f;
+		j1 = taaOn && filterOn ? haltonY[frameCycle] : 0.0f;
+		frameCycle = (frameCycle + 1) & 3;
+	}
+	FrameSetup f = frameParams(view, scrw, scrh, scrspp, maxDepth, R0gen, samplesTaken, estimator == ESTIMATOR_FILTER ? j0 : 0.0f,
+		estimator == ESTIMATOR_FILTER ? j1 : 0.0f);
 	f.camSeed = camSeed;
 	f.P.probePixel = probePos.x + scrw * probePos.y;
 	f.P.probe = wf.probe();
@@ -905,12 +927,14 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	samplesTaken += scrspp;
 	bool filtered = false;
 	if (estimator == ESTIMATOR_FILTER && f.P.gFeatures)
-	{	/* FinalizeRender (RenderCore_Optix7Filter/rendercore.cpp:897-935), TAA off */
+	{	/* FinalizeRender (RenderCore_Optix7Filter/rendercore.cpp:897-935) */
 		blueSlot = (blueSlot + 1) & 255;
 		if (!svgf || svgf->w != scrw || svgf->h != scrh) svgf.reset(new Svgf(scrw, scrh));
+		svgf->taa = taaOn;
 		hipCheck(hipEventRecord(ev[4], stream), "event");
-		svgf->Frame(wf.acc32(), wf.gFeatures(), wf.gWorldPos(), wf.gDeltaDepth(), prevView, 0, 0, 0, 0, samplesTaken, clampDirect, clampIndirect,
-			samplesTaken == scrspp ? 0 : 1, wf.image(), stream);
+		svgf->Frame(wf.acc32(), wf.gFeatures(), wf.gWorldPos(), wf.gDeltaDepth(), prevView, j0, j1, prevj0, prevj1, samplesTaken, clampDirect,
+			clampIndirect, samplesTaken == scrspp ? 0 : 1, wf.image(), stream);
+		prevj0 = j0, prevj1 = j1;
 		hipCheck(hipEventRecord(ev[5], stream), "event");
 		filtered = true;
 	}

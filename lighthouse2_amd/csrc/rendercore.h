@@ -142,7 +142,8 @@ public:
 	const TraceScene& Scene();
 	/* Render building blocks, shared with the job server (server.cpp). */
 	void checkRenderable();
-	FrameSetup frameParams(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int depth, uint32_t R0gen, int samplesTakenAtStart);
+	FrameSetup frameParams(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int depth, uint32_t R0gen, int samplesTakenAtStart,
+		float jitterX = 0, float jitterY = 0);
 	/* Returns the number of shade waves the deepest pass ran. */
 	int runSamples(const FrameSetup& f, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
 		uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing);
@@ -182,6 +183,11 @@ private:
 	bool filterOn = true;
 	float clampDirect = 2.5f, clampIndirect = 15.0f;
 	int blueSlot = 0;
+	/* TAA (rendercore.h:35,126; rendercore.cpp:754-758): on by default; the
+	 * view is jittered by a 4-frame Halton cycle while TAA and the filter are on */
+	bool taaOn = true;
+	int frameCycle = 0;
+	float j0 = 0, j1 = 0, prevj0 = 0, prevj1 = 0;
 	lighthouse2::ViewPyramid prevView = {};
 	std::unique_ptr<Svgf> svgf;
 	uint32_t shiftSeed = 0x11331445u;  /* blue-noise shift of the optix7 estimator (rendercore_optix7/rendercore.h:123) */

@@ -962,8 +962,9 @@ __global__ void k_generateF(CamParams c, ShadeParams P, uint32_t jobBase, int n,
 	const uint32_t wh = (uint32_t)(c.w * c.h);
 	const uint32_t pixelIdx = pathIdx % wh, sampleIdx = pathIdx / wh + (uint32_t)P.pass;
 	const int sx = (int)(pixelIdx % (uint32_t)c.w), sy = (int)(pixelIdx / (uint32_t)c.w);
-	const v3 pos = mk3(c.pos[0], c.pos[1], c.pos[2]), p1 = mk3(c.p1[0], c.p1[1], c.p1[2]);
-	const v3 right = mk3(c.p2[0], c.p2[1], c.p2[2]) - p1, up = mk3(c.p3[0], c.p3[1], c.p3[2]) - p1;
+	const v3 pos = mk3(c.pos[0], c.pos[1], c.pos[2]), p1u = mk3(c.p1[0], c.p1[1], c.p1[2]);
+	const v3 right = mk3(c.p2[0], c.p2[1], c.p2[2]) - p1u, up = mk3(c.p3[0], c.p3[1], c.p3[2]) - p1u;
+	const v3 p1 = xyz(P.camP1);                     /* p1 moved by the TAA jitter (.optix.cu:85-129 with params.p1) */
 	float r0 = 0, r1 = 0;
 	v3 O = pos;
 	if (!P.gFeatures)

@@ -27,7 +27,12 @@ public:
 	void Frame(const float4* accumulator, const uint4* features, const float4* worldPos, const float4* deltaDepth,
 		const lighthouse2::ViewPyramid& prevView, float j0, float j1, float prevj0, float prevj1, int samplesTaken,
 		float clampDirect, float clampIndirect, int camIsStationary, float4* out, hipStream_t s);
+	/* taa: after the last pass, TAApass and unsharpenTAA produce the display
+	 * image (finalize_shared.h:490-575); otherwise finalizeNoTAA. The TAA
+	 * history (prevPixels) is kept only while taa is on. */
+	int taa = 0;
 	DevBuf features, worldPos, prevWorldPos, deltaDepth, motion, moments, prevMoments, shading, filteredIn, filteredOut, lastPass;
+	DevBuf prevPixels, taaOut;
 };
 
 } // namespace lh

@@ -350,6 +350,104 @@ __global__ void k_apply(const uint4* __restrict__ features, const float4* __rest
 	else C[p] = combine(df, inf);
 }
 
+/* mitchellNetravali (sampling_shared.h:22-27), B = C = 1/3 */
+__device__ __forceinline__ float mitchell_netravali(float v)
+{
+	const float B = 1.0f / 3.0f, C = 1.0f / 3.0f, x = fabsf(v), x2 = x * x, x3 = x2 * x;
+	if (x < 1) return (1.0f / 6.0f) * ((12 - 9 * B - 6 * C) * x3 + (-18 + 12 * B + 6 * C) * x2 + (6 - 2 * B));
+	if (x < 2) return 1.0f / 6.0f * ((-B - 6 * C) * x3 + (6 * B + 30 * C) * x2 + (-12 * B - 48 * C) * x + (8 * B + 24 * C));
+	return 0.0f;
+}
+
+/* ReadTexelBmitchellNetravali (sampling_shared.h:117-133): 4x4 taps from
+ * (int)(u - 2), (int)(v - 2); row 0 is excluded (y > 0), as in the reference. */
+__device__ v3 read_texel_mitchell(const float4* buf, float u, float v, int w, int h)
+{
+	const int x1 = f2i_sat(u - 2.0f), y1 = f2i_sat(v - 2.0f);
+	float tw = 0;
+	float4 t = make_float4(0, 0, 0, 0);
+	for (int y = y1; y < y1 + 4; y++)
+		for (int x = x1; x < x1 + 4; x++)
+			if (x >= 0 && y > 0 && x < w && y < h)
+			{
+				const float dx = (float)x - u, dy = (float)y - v;
+				const float wt = mitchell_netravali(dx) * mitchell_netravali(dy);
+				const float4 b = buf[x + y * w];
+				t = make_float4(t.x + b.x * wt, t.y + b.y * wt, t.z + b.z * wt, t.w + b.w * wt);
+				tw += wt;
+			}
+	const float r = 1.0f / tw;
+	return mk3(t.x * r, t.y * r, t.z * r);
+}
+
+/* TAApassKernel (finalize_shared.h:490-538): Salvi's variance clip of the
+ * reprojected history (Mitchell-Netravali read of the previous output),
+ * blend 0.1 new / 0.9 history in YCoCg, clamp to 10. The reference updates
+ * its input in place while neighbours read it; here the neighbourhood is read
+ * from the unmodified input `px` and the result goes to `out`. */
+__global__ void k_taa(const float4* __restrict__ px, const float4* __restrict__ prevPixels, const float2* __restrict__ motion, int w, int h,
+	float4* __restrict__ out)
+{
+	const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+	if (x >= w || y >= h) return;
+	const int p = x + y * w;
+	v3 pixel = mk3(px[p].x, px[p].y, px[p].z);
+	const float2 mv = motion[p];
+	const float ux = mv.x - 0.5f, uy = mv.y - 0.5f;
+	if (ux >= 0 && ux < (float)w && uy >= 0 && uy < (float)h)
+	{
+		const v3 np = rgb_to_ycocg(pixel);
+		v3 hist = rgb_to_ycocg(read_texel_mitchell(prevPixels, ux, uy, w, h));
+		v3 avg = np, var = np * np;
+		const auto add = [&](int q) {
+			const v3 f = rgb_to_ycocg(mk3(px[q].x, px[q].y, px[q].z));
+			avg = avg + f, var = var + f * f;
+		};
+		if (x > 1)
+		{
+			if (y > 1) add(p - w - 1);
+			add(p - 1);
+			if (y < h - 1) add(p + w - 1);
+		}
+		if (y > 1) add(p - w);
+		if (y < h - 1) add(p + w);
+		if (x < w - 1)
+		{
+			if (y > 1) add(p + 1 - w);
+			add(p + 1);
+			if (y < h - 1) add(p + 1 + w);
+		}
+		avg = avg * (1.0f / 9.0f), var = var * (1.0f / 9.0f);
+		const v3 s0 = var - avg * avg;
+		const v3 sg = mk3(sqrtf(fmaxf(0.0f, s0.x)), sqrtf(fmaxf(0.0f, s0.y)), sqrtf(fmaxf(0.0f, s0.z)));
+		const v3 mn = avg - 1.25f * sg, mx = avg + 1.25f * sg;
+		hist = mk3(clampf(hist.x, mn.x, mx.x), clampf(hist.y, mn.y, mx.y), clampf(hist.z, mn.z, mx.z));
+		pixel = ycocg_to_rgb(np * 0.1f + hist * 0.9f);
+		if (__builtin_isnan(pixel.x + pixel.y + pixel.z)) pixel = ycocg_to_rgb(np);
+	}
+	out[p] = make_float4(fminf(10.0f, pixel.x), fminf(10.0f, pixel.y), fminf(10.0f, pixel.z), 0);
+}
+
+/* unsharpenTAAKernel (finalize_shared.h:553-575): sharpen against the 8
+ * neighbours, then square (the display path applies gamma again); border
+ * pixels are not written. */
+__global__ void k_unsharpen(const float4* __restrict__ px, int w, int h, float4* __restrict__ out)
+{
+	const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+	if (x == 0 || y == 0 || x >= w - 1 || y >= h - 1) return;
+	const float4 q[8] = { px[x - 1 + (y - 1) * w], px[x + (y - 1) * w], px[x + 1 + (y - 1) * w], px[x + 1 + y * w],
+		px[x + 1 + (y + 1) * w], px[x + (y + 1) * w], px[x - 1 + (y + 1) * w], px[x - 1 + y * w] };
+	const float4 c = px[x + y * w];
+	float4 s = make_float4(q[0].x * 0.35f, q[0].y * 0.35f, q[0].z * 0.35f, q[0].w * 0.35f);
+	for (int k = 1; k < 8; k++)
+	{
+		const float wk = (k & 1) ? 0.5f : 0.35f;
+		s = make_float4(s.x + q[k].x * wk, s.y + q[k].y * wk, s.z + q[k].z * wk, s.w + q[k].w * wk);
+	}
+	const float4 v = make_float4(fmaxf(c.x, c.x * 2.7f - s.x * 0.5f), fmaxf(c.y, c.y * 2.7f - s.y * 0.5f), fmaxf(c.z, c.z * 2.7f - s.z * 0.5f), 0);
+	out[x + y * w] = make_float4(v.x * v.x, v.y * v.y, v.z * v.z, 0);
+}
+
 /* finalizeNoTAAKernel (finalize_shared.h:587-595): display value = sqrt of the last pass; border pixels are not written */
 __global__ void k_finalize_no_taa(const float4* __restrict__ px, int w, int h, float4* __restrict__ out)
 {
@@ -396,8 +494,9 @@ Svgf::Svgf(int w_, int h_) : w(w_), h(h_)
 	const size_t n = (size_t)w * h;
 	features.ensure(n * 16), worldPos.ensure(n * 16), prevWorldPos.ensure(n * 16), deltaDepth.ensure(n * 16);
 	motion.ensure(n * 8), moments.ensure(n * 16), prevMoments.ensure(n * 16), shading.ensure(n * 16);
-	filteredIn.ensure(n * 16), filteredOut.ensure(n * 16), lastPass.ensure(n * 16);
-	for (DevBuf* b : { &features, &worldPos, &prevWorldPos, &deltaDepth, &motion, &moments, &prevMoments, &shading, &filteredIn, &filteredOut, &lastPass })
+	filteredIn.ensure(n * 16), filteredOut.ensure(n * 16), lastPass.ensure(n * 16), prevPixels.ensure(n * 16), taaOut.ensure(n * 16);
+	for (DevBuf* b : { &features, &worldPos, &prevWorldPos, &deltaDepth, &motion, &moments, &prevMoments, &shading, &filteredIn, &filteredOut, &lastPass,
+		&prevPixels, &taaOut })
 		chk(hipMemset(b->p, 0, b->bytes), "memset");
 }
 
@@ -420,7 +519,13 @@ void Svgf::Frame(const float4* acc, const uint4* feat, const float4* wp, const f
 	apply(1, shading, filteredIn.as<float4>(), filteredOut, 0);
 	apply(2, filteredOut, nullptr, filteredIn, 0);
 	apply(3, filteredIn, nullptr, lastPass, 1);
-	k_finalize_no_taa<<<g, b, 0, s>>>(lastPass.as<float4>(), w, h, out);
+	if (taa)
+	{	/* TAApass + unsharpenTAA (rendercore.cpp:922-926); the TAA output is the next frame's history */
+		k_taa<<<g, b, 0, s>>>(lastPass.as<float4>(), prevPixels.as<float4>(), motion.as<float2>(), w, h, taaOut.as<float4>());
+		k_unsharpen<<<g, b, 0, s>>>(taaOut.as<float4>(), w, h, out);
+		std::swap(taaOut, prevPixels);
+	}
+	else k_finalize_no_taa<<<g, b, 0, s>>>(lastPass.as<float4>(), w, h, out);
 	chk(hipGetLastError(), "svgf kernels");
 	std::swap(filteredIn, filteredOut);
 	std::swap(prevWorldPos, worldPos);

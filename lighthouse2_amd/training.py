@@ -58,7 +58,7 @@ def relmse(x, ref):
 
 class PairRenderer:
     """Two cores on one scene: a 1-spp filter-estimator core (its Render runs
-    SVGF) and a converging core for the targets."""
+    SVGF, without TAA) and a converging core for the targets."""
 
     def __init__(self, scene, w, h, ref_spp=1024, spp_per_frame=64, bsdf=1):
         self.w, self.h, self.ref_spp, self.per = w, h, ref_spp, spp_per_frame
@@ -67,11 +67,13 @@ class PairRenderer:
         self.fast.SetTarget(w, h, 1)
         self.fast.Setting("estimator", 2)
         self.fast.Setting("bsdf", bsdf)
+        self.fast.Setting("TAA", 0)                  # SVGF alone; no view jitter
         self.slow = C.RenderCore()
         C.load_scene(self.slow, scene)
         self.slow.SetTarget(w, h, spp_per_frame)
         self.slow.Setting("estimator", 2)
         self.slow.Setting("bsdf", bsdf)                # same eye rays as the fast core (pixel corners)
+        self.slow.Setting("TAA", 0)                    # no jitter between the converging frames
 
     def render(self, view, history=1):
         """Returns dict: radiance (h,w,3) 1 spp, albedo, normal, depth, svgf (h,w,3)

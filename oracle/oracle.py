@@ -40,14 +40,15 @@ def lib():
             "orc_math": (None, [I, P, P, I, P]),
             "orc_svgf_create": (P, [I, I]), "orc_svgf_destroy": (None, [P]),
             "orc_svgf_frame": (None, [P, P, P, P, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, I,
-                                      ctypes.c_float, ctypes.c_float, I, P]),
+                                      ctypes.c_float, ctypes.c_float, I, P, I]),
             "orc_svgf_read": (None, [P, I, P]),
             "orc_set_blue_noise": (None, [P, P, P, P, U]),
             "orc_blue_noise4": (None, [P, P, I, P]), "orc_normals": (None, [I, P, P, I]),
             "orc_disney_eval": (None, [P, P, P, P, P, P, P, I, P]), "orc_disney_sample": (None, [P, P, P, P, P, P, P, I, P]),
             "orc_ggx": (None, [I, P, P, P, I, ctypes.c_float, ctypes.c_float, P]),
             "orc_set_brute": (None, [P, I]),
-            "orc_render_frameF": (I, [P, P, I, I, I, I, I, ctypes.c_float, ctypes.c_float, U, P, P, P, P, I, I]),
+            "orc_render_frameF": (I, [P, P, I, I, I, I, I, ctypes.c_float, ctypes.c_float, U, P, P, P, P, I, I, ctypes.c_float,
+                                      ctypes.c_float]),
             "orc_render_frame7": (I, [P, P, I, I, I, I, ctypes.c_float, ctypes.c_float, U, U, P, I, I]),
         }
         for n, (r, a) in sigs.items():
@@ -183,15 +184,15 @@ class OracleScene:
             raise RuntimeError("set_blue_noise first")
         return acc, waves
 
-    def render_frameF(self, view, w, h, spp, pass_, blue_slot, eps, clamp_value, cam_seed, gbuffer=None, threads=8, bsdf=1):
+    def render_frameF(self, view, w, h, spp, pass_, blue_slot, eps, clamp_value, cam_seed, gbuffer=None, threads=8, bsdf=1, jitter=(0.0, 0.0)):
         """One filter-estimator frame: per-path direct and indirect sums (2, spp*w*h, 4)
         and the waves it ran; gbuffer = (features (h,w,4) uint32, world_pos, delta_depth)
-        arrays updated in place (None: filter off)."""
+        arrays updated in place (None: filter off); jitter: the TAA view jitter in pixels."""
         acc = np.zeros((2, spp * w * h, 4), np.float32)
         v = np.ascontiguousarray(view)
         f, wp, dd = gbuffer if gbuffer is not None else (None, None, None)
         waves = lib().orc_render_frameF(self.h, _p(v), w, h, spp, pass_, blue_slot, eps, clamp_value, cam_seed, _p(acc), _p(f), _p(wp),
-                                        _p(dd), threads, bsdf)
+                                        _p(dd), threads, bsdf, jitter[0], jitter[1])
         if waves < 0:
             raise RuntimeError("set_blue_noise first")
         return acc, waves
@@ -262,7 +263,7 @@ class OracleSvgf:
             self.hd = None
 
     def frame(self, acc, features, world_pos, delta_depth, prev_view, j=(0.0, 0.0), prev_j=(0.0, 0.0), samples=1,
-              clamp_direct=2.5, clamp_indirect=15.0, stationary=0, out=None):
+              clamp_direct=2.5, clamp_indirect=15.0, stationary=0, out=None, taa=False):
         a = np.ascontiguousarray(acc, np.float32)
         f = np.ascontiguousarray(features, np.uint32)
         wp = np.ascontiguousarray(world_pos, np.float32)
@@ -270,10 +271,11 @@ class OracleSvgf:
         v = np.ascontiguousarray(prev_view)
         o = np.zeros((self.h, self.w, 4), np.float32) if out is None else out
         lib().orc_svgf_frame(self.hd, _p(a), _p(f), _p(wp), _p(dd), _p(v), j[0], j[1], prev_j[0], prev_j[1], samples,
-                             clamp_direct, clamp_indirect, stationary, _p(o))
+                             clamp_direct, clamp_indirect, stationary, _p(o), 1 if taa else 0)
         return o
 
     def read(self, which):
+        """which: 0 features, 1 moments, 2 motion, 3 phase-1 history, 4 last pass, 5 world positions, 6 TAA history."""
         shape = {0: ((self.h, self.w, 4), np.uint32), 2: ((self.h, self.w, 2), np.float32)}.get(which, ((self.h, self.w, 4), np.float32))
         out = np.zeros(shape[0], shape[1])
         lib().orc_svgf_read(self.hd, which, _p(out))
@@ -382,16 +384,20 @@ def render7(scene_or_oracle, view, w, h, spp=1, frames=1, eps=1e-3, seed=0x12345
     return acc32.reshape(h, w, 4), acc64.reshape(h, w, 4), taken, cam, shift
 
 
+HALTON_X, HALTON_Y = (0.3, 0.7, 0.2, 0.8), (0.2, 0.8, 0.7, 0.3)   # RenderCore_Optix7Filter/rendercore.cpp:755
+
+
 class RenderF:
     """Oracle of a sequence of Render calls with the filter estimator
-    (RenderCore_Optix7Filter/rendercore.cpp:740-935, TAA off): the camera seed
-    is never reset and advances once per shade wave; the blue-noise slot
-    advances once per frame; with the filter on, the G-buffer of sample 0 and
-    SVGF (prepare, three a-trous passes, display) give the image, the history
-    carried from frame to frame."""
+    (RenderCore_Optix7Filter/rendercore.cpp:740-935): the camera seed is never
+    reset and advances once per shade wave; the blue-noise slot advances once
+    per frame; with the filter on, the G-buffer of sample 0 and SVGF (prepare,
+    three a-trous passes, then TAA + unsharpen or the plain display) give the
+    image, the history carried from frame to frame. With TAA and the filter on
+    (the reference's default) the view is jittered by a 4-frame Halton cycle."""
 
     def __init__(self, scene_or_oracle, w, h, spp=1, eps=1e-3, seed=0x12345678, clamp_value=10.0, filter_on=True,
-                 clamp_direct=2.5, clamp_indirect=15.0, bsdf=1, blue_noise=None, threads=8):
+                 clamp_direct=2.5, clamp_indirect=15.0, bsdf=1, blue_noise=None, threads=8, taa=True):
         from lighthouse2_amd import formats as F
         self.o = scene_or_oracle if isinstance(scene_or_oracle, OracleScene) else OracleScene(scene_or_oracle)
         self.o.set_blue_noise(blue_noise)
@@ -406,6 +412,7 @@ class RenderF:
         self.prev_view = np.zeros(1, F.ViewPyramid)
         self.svgf = OracleSvgf(w, h) if filter_on else None
         self.image = np.zeros((h, w, 4), np.float32)
+        self.taa, self.cycle, self.j, self.prev_j = taa, 0, (0.0, 0.0), (0.0, 0.0)
 
     def frame(self, view, mode=RESTART):
         w, h, spp = self.w, self.h, self.spp
@@ -415,8 +422,12 @@ class RenderF:
             self.taken, self.first = 0, True
         if mode == CONVERGE:
             self.first = False
+        on = self.taa and self.filter
+        self.j = (HALTON_X[self.cycle], HALTON_Y[self.cycle]) if on else (0.0, 0.0)
+        self.cycle = (self.cycle + 1) & 3
         gb = (self.features, self.world_pos, self.delta_depth) if self.filter else None
-        pa, waves = self.o.render_frameF(view, w, h, spp, self.taken, self.slot, self.eps, self.clamp, self.cam, gb, self.threads, self.bsdf)
+        pa, waves = self.o.render_frameF(view, w, h, spp, self.taken, self.slot, self.eps, self.clamp, self.cam, gb, self.threads, self.bsdf,
+                                         jitter=self.j)
         for _ in range(waves):
             self.cam, _ = xor32_host(self.cam)
         for half in range(2):
@@ -428,8 +439,9 @@ class RenderF:
         self.slot = (self.slot + 1) & 255
         if self.filter:
             self.image = self.svgf.frame(self.acc32.reshape(2, h, w, 4), self.features, self.world_pos, self.delta_depth, self.prev_view,
-                                         j=(0.0, 0.0), prev_j=(0.0, 0.0), samples=self.taken, clamp_direct=self.cd,
-                                         clamp_indirect=self.ci, stationary=0 if self.taken == spp else 1)
+                                         j=self.j, prev_j=self.prev_j, samples=self.taken, clamp_direct=self.cd,
+                                         clamp_indirect=self.ci, stationary=0 if self.taken == spp else 1, taa=self.taa)
+            self.prev_j = self.j
         else:
             self.image = (self.acc32[0] * np.float32(1.0 / self.taken)).reshape(h, w, 4)
         self.prev_view = np.ascontiguousarray(view).copy()

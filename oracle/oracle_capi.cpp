@@ -265,12 +265,13 @@ OAPI int orc_render_frame7(void* sp, const ViewPyramid* view, int w, int h, int 
 OAPI void orc_set_brute(void* sp, int on) { ((RenderScene*)sp)->bruteForce = on != 0; }
 
 OAPI int orc_render_frameF(void* sp, const ViewPyramid* view, int w, int h, int spp, int pass, int blueSlot, float eps, float clampValue,
-	uint camSeed, float* pathAcc, uint* features, float* worldPos, float* deltaDepth, int threads, int bsdf)
+	uint camSeed, float* pathAcc, uint* features, float* worldPos, float* deltaDepth, int threads, int bsdf, float j0, float j1)
 {
 	RenderScene& s = *(RenderScene*)sp;
 	if (s.blueNoise.empty()) return -1;
 	RenderFParams p;
 	p.view = *view, p.w = w, p.h = h, p.spp = spp, p.pass = pass, p.blueSlot = blueSlot, p.bsdf = bsdf, p.eps = eps, p.clampValue = clampValue;
+	p.j0 = j0, p.j1 = j1;
 	return render_frameF(s, p, camSeed, pathAcc, features, worldPos, deltaDepth, threads);
 }
 

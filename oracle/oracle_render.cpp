@@ -1013,11 +1013,26 @@ static void pack_features(uint* f, uint albedo, uint packedNormal, float t, uint
 {
 	f[0] = albedo, f[1] = packedNormal, f[2] = f2u(t), f[3] = (isSpecular << 4) + (matid << 6) + (f[3] & 15u);
 }
+/* The view moved by the TAA jitter (RenderCore_Optix7Filter/rendercore.cpp:781-786):
+ * p1, p2, p3 += (j0 / w) (p2 - p1) + (j1 / h) (p3 - p1). */
+static void jittered_corners(const RenderFParams& p, f3& p1, f3& p2, f3& p3)
+{
+	const ViewPyramid& v = p.view;
+	p1 = f3of(v.p1), p2 = f3of(v.p2), p3 = f3of(v.p3);
+	if (p.j0 == 0 && p.j1 == 0) return;
+	const f3 right = p2 - p1, up = p3 - p1;
+	const float sx = p.j0 / (float)p.w, sy = p.j1 / (float)p.h;
+	const f3 j = mk3(sx * right.x + sy * up.x, sx * right.y + sy * up.y, sx * right.z + sy * up.z);
+	p1 = p1 + j, p2 = p2 + j, p3 = p3 + j;
+}
+
 static void depth_derivatives(const RenderFParams& p, int x, int y, float depth, const CoreTri& tri, float* dd)
 {
 	const f3 triN = mk3(tri.Nx, tri.Ny, tri.Nz);
 	const ViewPyramid& v = p.view;
-	const f3 p1 = f3of(v.p1), p2 = f3of(v.p2) - f3of(v.p1), p3 = f3of(v.p3) - f3of(v.p1), pos = f3of(v.pos), v0 = f3of(tri.vertex0);
+	f3 jp1, jp2, jp3;
+	jittered_corners(p, jp1, jp2, jp3);   /* the shade kernel receives the jittered view (rendercore.cpp:836) */
+	const f3 p1 = jp1, p2 = jp2 - jp1, p3 = jp3 - jp1, pos = f3of(v.pos), v0 = f3of(tri.vertex0);
 	const float iw = 1.0f / (float)p.w, ih = 1.0f / (float)p.h;
 	const f3 dxDir = normalize((p1 + ((float)x + 0.5f + 1) * iw * p2 + ((float)y + 0.5f) * ih * p3) - pos);
 	const float dxDepth = dot(v0 - pos, triN) / dot(triN, dxDir);
@@ -1039,7 +1054,9 @@ static int trace_pathF(const RenderScene& s, const RenderFParams& p, const uint*
 	/* eye ray */
 	const uint pix0 = pathIdx % wh;
 	const int sx = (int)(pix0 % (uint)p.w), sy = (int)(pix0 / (uint)p.w);
-	const f3 pos = f3of(v.pos), p1 = f3of(v.p1), right = f3of(v.p2) - p1, up = f3of(v.p3) - p1;
+	const f3 pos = f3of(v.pos), right = f3of(v.p2) - f3of(v.p1), up = f3of(v.p3) - f3of(v.p1);
+	f3 p1, jp2, jp3;
+	jittered_corners(p, p1, jp2, jp3);      /* eye rays: jittered p1, unjittered right/up (rendercore.cpp:782,791-793) */
 	float r0 = 0, r1 = 0;
 	f3 O = pos;
 	if (!gbuf)

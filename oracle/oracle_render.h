@@ -92,6 +92,7 @@ struct RenderFParams
 	ViewPyramid view;   /* aperture and distortion as the core uses them */
 	int w, h, spp, pass, blueSlot, bsdf;
 	float eps, clampValue;
+	float j0, j1;       /* TAA jitter in pixels (0, 0: none) */
 };
 /* One frame of the filter estimator. pathAcc: 2 * w*h*spp float4, direct
  * radiance (w: depth at the first hit plus unoccluded connections) then

@@ -1,7 +1,8 @@
 /* oracle_svgf.cpp - CPU oracle of SVGF. TEST INFRASTRUCTURE (see oracle_math.h).
  *
- * prepareFilterKernel, applyFilterKernel x 3 and finalizeNoTAAKernel of
- * lib/CUDA/shared_kernel_code/finalize_shared.h:169-484,587-600, driven as
+ * prepareFilterKernel, applyFilterKernel x 3, TAApassKernel + unsharpenTAAKernel
+ * or finalizeNoTAAKernel of lib/CUDA/shared_kernel_code/finalize_shared.h:169-600
+ * (with ReadTexelBmitchellNetravali, sampling_shared.h:22-27,117-133), driven as
  * RenderCore_Optix7Filter::FinalizeRender (rendercore.cpp:897-935), with the
  * packers of tools_shared.h:113-136,162-183,282-304 and the history reads of
  * sampling_shared.h:111-115,136-220. powf(x, 128) is seven squarings and
@@ -62,13 +63,104 @@ struct Svgf
 	int w, h;
 	std::vector<u4> features;
 	std::vector<f4> worldPos, prevWorldPos, deltaDepth, moments, prevMoments, shading, filteredIn, filteredOut, lastPass;
+	std::vector<f4> prevPixels, taaOut;   /* TAA history (the previous TAA output) and this frame's */
 	std::vector<fl2> motion;
 	Svgf(int w_, int h_) : w(w_), h(h_)
 	{
 		const size_t n = (size_t)w * h;
 		const f4 z = { 0, 0, 0, 0 };
 		features.assign(n, u4{ 0, 0, 0, 0 }), motion.assign(n, fl2{ 0, 0 });
-		for (auto* v : { &worldPos, &prevWorldPos, &deltaDepth, &moments, &prevMoments, &shading, &filteredIn, &filteredOut, &lastPass }) v->assign(n, z);
+		for (auto* v : { &worldPos, &prevWorldPos, &deltaDepth, &moments, &prevMoments, &shading, &filteredIn, &filteredOut, &lastPass,
+			&prevPixels, &taaOut }) v->assign(n, z);
+	}
+
+	/* mitchellNetravali (sampling_shared.h:22-27), B = C = 1/3 */
+	static float mitchell_netravali(float v)
+	{
+		const float B = 1.0f / 3.0f, C = 1.0f / 3.0f, x = fabsf(v), x2 = x * x, x3 = x2 * x;
+		if (x < 1) return (1.0f / 6.0f) * ((12 - 9 * B - 6 * C) * x3 + (-18 + 12 * B + 6 * C) * x2 + (6 - 2 * B));
+		if (x < 2) return 1.0f / 6.0f * ((-B - 6 * C) * x3 + (6 * B + 30 * C) * x2 + (-12 * B - 48 * C) * x + (8 * B + 24 * C));
+		return 0.0f;
+	}
+	/* ReadTexelBmitchellNetravali (sampling_shared.h:117-133); row 0 excluded (y > 0) as in the reference */
+	f3 read_texel_mitchell(const std::vector<f4>& buf, float u, float v) const
+	{
+		const int x1 = f2i_sat(u - 2.0f), y1 = f2i_sat(v - 2.0f);
+		float tw = 0;
+		f4 t = { 0, 0, 0, 0 };
+		for (int y = y1; y < y1 + 4; y++)
+			for (int x = x1; x < x1 + 4; x++)
+				if (x >= 0 && y > 0 && x < w && y < h)
+				{
+					const float dx = (float)x - u, dy = (float)y - v;
+					const float wt = mitchell_netravali(dx) * mitchell_netravali(dy);
+					const f4 b = buf[x + y * w];
+					t = f4{ t.x + b.x * wt, t.y + b.y * wt, t.z + b.z * wt, t.w + b.w * wt };
+					tw += wt;
+				}
+		const float r = 1.0f / tw;
+		return mk3(t.x * r, t.y * r, t.z * r);
+	}
+	/* TAApassKernel (finalize_shared.h:490-538), reading the neighbourhood from the
+	 * unmodified last pass; clamps with fminf/fmaxf (IEEE: a NaN operand loses). */
+	void taa_pass()
+	{
+		for (int y = 0; y < h; y++) for (int x = 0; x < w; x++)
+		{
+			const int p = x + y * w;
+			const std::vector<f4>& px = lastPass;
+			f3 pixel = xyz(px[p]);
+			const float ux = motion[p].x - 0.5f, uy = motion[p].y - 0.5f;
+			if (ux >= 0 && ux < (float)w && uy >= 0 && uy < (float)h)
+			{
+				const f3 np = rgb_to_ycocg(pixel);
+				f3 hist = rgb_to_ycocg(read_texel_mitchell(prevPixels, ux, uy));
+				f3 avg = np, var = np * np;
+				const auto add = [&](int q) { const f3 f = rgb_to_ycocg(xyz(px[q])); avg = avg + f, var = var + f * f; };
+				if (x > 1)
+				{
+					if (y > 1) add(p - w - 1);
+					add(p - 1);
+					if (y < h - 1) add(p + w - 1);
+				}
+				if (y > 1) add(p - w);
+				if (y < h - 1) add(p + w);
+				if (x < w - 1)
+				{
+					if (y > 1) add(p + 1 - w);
+					add(p + 1);
+					if (y < h - 1) add(p + 1 + w);
+				}
+				avg = avg * (1.0f / 9.0f), var = var * (1.0f / 9.0f);
+				const f3 s0 = var - avg * avg;
+				const f3 sg = mk3(sqrtf(fmaxf(0.0f, s0.x)), sqrtf(fmaxf(0.0f, s0.y)), sqrtf(fmaxf(0.0f, s0.z)));
+				const f3 mn = avg - 1.25f * sg, mx = avg + 1.25f * sg;
+				hist = mk3(fmaxf(mn.x, fminf(hist.x, mx.x)), fmaxf(mn.y, fminf(hist.y, mx.y)), fmaxf(mn.z, fminf(hist.z, mx.z)));
+				pixel = ycocg_to_rgb(np * 0.1f + hist * 0.9f);
+				if (std::isnan(pixel.x + pixel.y + pixel.z)) pixel = ycocg_to_rgb(np);
+			}
+			taaOut[p] = f4{ fminf(10.0f, pixel.x), fminf(10.0f, pixel.y), fminf(10.0f, pixel.z), 0 };
+		}
+	}
+	/* unsharpenTAAKernel (finalize_shared.h:553-575): interior pixels, squared */
+	void unsharpen(float* out) const
+	{
+		for (int y = 1; y < h - 1; y++) for (int x = 1; x < w - 1; x++)
+		{
+			const std::vector<f4>& px = taaOut;
+			const f4 q[8] = { px[x - 1 + (y - 1) * w], px[x + (y - 1) * w], px[x + 1 + (y - 1) * w], px[x + 1 + y * w],
+				px[x + 1 + (y + 1) * w], px[x + (y + 1) * w], px[x - 1 + (y + 1) * w], px[x - 1 + y * w] };
+			const f4 c = px[x + y * w];
+			f4 s = { q[0].x * 0.35f, q[0].y * 0.35f, q[0].z * 0.35f, q[0].w * 0.35f };
+			for (int k = 1; k < 8; k++)
+			{
+				const float wk = (k & 1) ? 0.5f : 0.35f;
+				s = f4{ s.x + q[k].x * wk, s.y + q[k].y * wk, s.z + q[k].z * wk, s.w + q[k].w * wk };
+			}
+			const float vx = fmaxf(c.x, c.x * 2.7f - s.x * 0.5f), vy = fmaxf(c.y, c.y * 2.7f - s.y * 0.5f), vz = fmaxf(c.z, c.z * 2.7f - s.z * 0.5f);
+			float* o = out + 4 * (size_t)(x + y * w);
+			o[0] = vx * vx, o[1] = vy * vy, o[2] = vz * vz, o[3] = 0;
+		}
 	}
 
 	f4 read_world_pos(int x, int y) const
@@ -337,7 +429,7 @@ OAPI void orc_svgf_destroy(void* p) { delete (Svgf*)p; }
 
 /* One frame with host arrays (layouts as lh2hip_svgf_frame); out: w*h float4, border untouched. */
 OAPI void orc_svgf_frame(void* sp, const float* acc, const uint* feat, const float* wp, const float* dd, const lh2hip::ViewPyramid* prevView,
-	float j0, float j1, float pj0, float pj1, int samplesTaken, float cD, float cI, int stationary, float* out)
+	float j0, float j1, float pj0, float pj1, int samplesTaken, float cD, float cI, int stationary, float* out, int taa)
 {
 	Svgf& s = *(Svgf*)sp;
 	const int n = s.w * s.h;
@@ -351,7 +443,13 @@ OAPI void orc_svgf_frame(void* sp, const float* acc, const uint* feat, const flo
 	s.apply(1, s.shading, &s.filteredIn, s.filteredOut, 0);
 	s.apply(2, s.filteredOut, nullptr, s.filteredIn, 0);
 	s.apply(3, s.filteredIn, nullptr, s.lastPass, 1);
-	for (int y = 1; y < s.h - 1; y++) for (int x = 1; x < s.w - 1; x++)
+	if (taa)
+	{	/* TAApass + unsharpenTAA (RenderCore_Optix7Filter/rendercore.cpp:922-926) */
+		s.taa_pass();
+		s.unsharpen(out);
+		std::swap(s.taaOut, s.prevPixels);
+	}
+	else for (int y = 1; y < s.h - 1; y++) for (int x = 1; x < s.w - 1; x++)
 	{
 		const f4 v = s.lastPass[x + y * s.w];
 		float* o = out + 4 * (size_t)(x + y * s.w);
@@ -372,4 +470,5 @@ OAPI void orc_svgf_read(void* sp, int which, void* out)
 	else if (which == 3) memcpy(out, s.filteredIn.data(), n * 16);
 	else if (which == 4) memcpy(out, s.lastPass.data(), n * 16);
 	else if (which == 5) memcpy(out, s.prevWorldPos.data(), n * 16);
+	else if (which == 6) memcpy(out, s.prevPixels.data(), n * 16);
 }

@@ -1,9 +1,10 @@
 """GPU parity of the filter estimator ("estimator" setting 2: RenderCore_Optix7Filter's
 eye rays and shade kernel, which writes the SVGF G-buffer and splits direct and
-indirect radiance, followed by SVGF in Render) against the oracle through the
-C ABI: fp64 accumulators of both halves, the G-buffer and the displayed image
-(interior; the display pass leaves the border alone) identical frame by frame,
-over restarts, converging frames and a moving camera."""
+indirect radiance, followed by SVGF and, by default, TAA in Render) against the
+oracle through the C ABI: fp64 accumulators of both halves, the G-buffer and the
+displayed image (interior; the display pass leaves the border alone) identical
+frame by frame, over restarts, converging frames, the TAA jitter cycle and a
+moving camera."""
 import numpy as np
 import pytest
 
@@ -27,18 +28,20 @@ def diff(a, b):
     return f"{len(idx)} of {a.size} differ, per channel {per_ch}; {rows}"
 
 
-def run(scene, views_modes, w, h, spp, bsdf=1, filter_on=True, seed=0x12345678, tables=None):
+def run(scene, views_modes, w, h, spp, bsdf=1, filter_on=True, seed=0x12345678, tables=None, taa=True):
     core = lh.RenderCore()
     lh.load_scene(core, scene)
     core.SetTarget(w, h, spp)
     core.Setting("estimator", 2)
     core.Setting("bsdf", bsdf)
     core.Setting("filter", 1 if filter_on else 0)
+    if not taa:
+        core.Setting("TAA", 0)
     if seed != 0x12345678:
         core.Setting("seed", seed)
     if tables is not None:
         core.SetBlueNoise(*tables)
-    ref = orc.RenderF(scene, w, h, spp=spp, seed=seed, filter_on=filter_on, bsdf=bsdf, blue_noise=tables, threads=16)
+    ref = orc.RenderF(scene, w, h, spp=spp, seed=seed, filter_on=filter_on, bsdf=bsdf, blue_noise=tables, threads=16, taa=taa)
     for k, (view, mode) in enumerate(views_modes):
         core.Render(view, lh.LH2_RESTART if mode == orc.RESTART else lh.LH2_CONVERGE)
         want = ref.frame(view, mode)
@@ -64,13 +67,15 @@ def run(scene, views_modes, w, h, spp, bsdf=1, filter_on=True, seed=0x12345678, 
     return img, st
 
 
-@pytest.mark.parametrize("bsdf", [0, 1])
-def test_box_sequence(torch_cuda, bsdf):
+@pytest.mark.parametrize("bsdf,taa", [(0, True), (1, True), (1, False)])
+def test_box_sequence(torch_cuda, bsdf, taa):
+    """Five frames: converging on a still camera, then a moved camera; with TAA
+    (the default) the view cycles through the 4 Halton jitters."""
     s = scenes.disney_cornell_box(sphere_tris=5000) if bsdf else scenes.cornell_box(sphere_tris=5000)
     v0 = scenes.cornell_view(72, 56)
     v1 = scenes.camera_view((0.52, 0.49, -1.19), (0.5, 0.5, 0.5), 40.0, 72, 56)
     R, C = orc.RESTART, orc.CONVERGE
-    img, st = run(s, [(v0, R), (v0, C), (v0, C), (v1, R), (v1, R)], 72, 56, 1, bsdf=bsdf)
+    img, st = run(s, [(v0, R), (v0, C), (v0, C), (v1, R), (v1, R)], 72, 56, 1, bsdf=bsdf, taa=taa)
     assert np.isfinite(img).all() and img[1:-1, 1:-1, :3].mean() > 0
     assert st["filterTime"] > 0
 

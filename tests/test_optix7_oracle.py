@@ -167,16 +167,20 @@ def test_triangle_emitter_form_factor():
     assert np.all(r64[..., 3] > 0)          # every pixel had unoccluded connections (w counts them)
 
 
-def test_filter_gbuffer_closed_form():
+@pytest.mark.parametrize("taa", [False, True])
+def test_filter_gbuffer_closed_form(taa):
     """Filter estimator G-buffer on the Lambert Cornell box: eye rays through
     pixel corners hit the walls at t = plane distance / D.n; the world
     position is O + t D; the depth derivatives are those of the neighbouring
-    pixel corners' rays against the same plane; albedo is the wall colour."""
+    pixel corners' rays against the same plane; albedo is the wall colour.
+    With TAA the first frame's view is shifted by (0.3, 0.2) pixels
+    (RenderCore_Optix7Filter/rendercore.cpp:755-757,781-786)."""
     s = scenes.cornell_box(seed=5, sphere_tris=200)
     w, h = 48, 40
     view = scenes.cornell_view(w, h)
-    r = orc.RenderF(s, w, h, spp=1, bsdf=0)
+    r = orc.RenderF(s, w, h, spp=1, bsdf=0, taa=taa)
     r.frame(view, orc.RESTART)
+    jx, jy = (0.3, 0.2) if taa else (0.0, 0.0)
     v = view[0]
     pos, p1 = v["pos"].astype(np.float64), v["p1"].astype(np.float64)
     right, up = (v["p2"] - v["p1"]).astype(np.float64), (v["p3"] - v["p1"]).astype(np.float64)
@@ -184,7 +188,7 @@ def test_filter_gbuffer_closed_form():
     checked = 0
     for y in range(h):
         for x in range(w):
-            D = p1 + (x / w) * right + (y / h) * up - pos
+            D = p1 + ((x + jx) / w) * right + ((y + jy) / h) * up - pos
             D /= np.linalg.norm(D)
             t = (1.0 - pos[2]) / D[2]
             P = pos + t * D
@@ -200,7 +204,8 @@ def test_filter_gbuffer_closed_form():
                 d /= np.linalg.norm(d)
                 return (1.0 - pos[2]) / d[2]
             dd = r.delta_depth[y, x]
-            assert abs(dd[2] - (depth(x + 1.5, y + 0.5) - t)) < 2e-5 and abs(dd[3] - (depth(x + 0.5, y + 1.5) - t)) < 2e-5
+            assert abs(dd[2] - (depth(x + 1.5 + jx, y + 0.5 + jy) - t)) < 2e-5
+            assert abs(dd[3] - (depth(x + 0.5 + jx, y + 1.5 + jy) - t)) < 2e-5
             checked += 1
     assert checked > 150
     # direct and indirect halves: the indirect sum holds radiance of paths after a diffuse bounce

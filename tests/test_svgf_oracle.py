@@ -26,6 +26,24 @@ def test_constant_image_stays_constant():
     assert np.max(np.abs(inner - expect)) < 2e-3
 
 
+def test_taa_keeps_constants_and_drops_stale_history():
+    """TAA + unsharpen (finalize_shared.h:490-575): a constant image stays
+    constant (the 3x3 variance is 0, so the history is clamped onto the new
+    value, and the unsharpen weights sum to 2.7 - 0.5 * 3.4 = 1); when the
+    constant changes, the clamp drops the old history at once (no ghosting)."""
+    w, h = 64, 48
+    view = scenes.camera_view((0.5, 0.6, -1.0), (0.5, 0.5, 1.0), 50.0, w, h)
+    s = orc.OracleSvgf(w, h)
+    for k, (d, i) in enumerate([(0.8, 0.2), (0.8, 0.2), (0.3, 0.1)]):
+        acc, feat, wp, dd = gbuffer(view, w, h, const=(0.5, d, i), spec_strip=False)
+        feat[..., 3] = (0 << 4) + (1 << 6)
+        out = s.frame(acc, feat, wp, dd, view, samples=1, stationary=1 if k else 0, taa=True)
+        inner = out[16:-16, 16:-16, :3]
+        assert np.max(np.abs(inner - (d + i))) < 4e-3, (k, np.max(np.abs(inner - (d + i))))
+        hist = s.read(6)[16:-16, 16:-16, :3]             # the TAA output kept for the next frame: sqrt of the radiance
+        assert np.max(np.abs(hist - np.sqrt(d + i))) < 2e-3
+
+
 def test_static_camera_reprojects_onto_itself_and_counts_history():
     w, h = 32, 24
     view = scenes.camera_view((0.5, 0.6, -1.0), (0.5, 0.5, 1.0), 50.0, w, h)
