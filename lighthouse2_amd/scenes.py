@@ -154,11 +154,12 @@ def light_triangle(tri, radiance, tri_idx, inst_idx):
     return L
 
 
-def add_emitters(scene, tris, radiance, inst_idx):
+def add_emitters(scene, tris, radiance, inst_idx, tri_ids=None):
     """HostNode::PrepareLights (host_node.cpp:203-233) for one emissive mesh:
     each triangle gets its object-space area (CoreTri::UpdateArea, Heron's
     formula in fp32, common_classes.h:89-96) and its index in the light list
-    (ltriIdx), and a CoreLightTri is appended."""
+    (ltriIdx), and a CoreLightTri is appended. tri_ids: the triangles' indices
+    in their mesh (default 0, 1, ...)."""
     ln = lambda a: np.sqrt(f32(a[0] * a[0] + a[1] * a[1]) + f32(a[2] * a[2])).astype(f32)
     lights = []
     for k in range(len(tris)):
@@ -167,7 +168,7 @@ def add_emitters(scene, tris, radiance, inst_idx):
         sp = f32(f32(a + b) + c) * f32(0.5)
         tris[k]["area"] = np.sqrt(f32(f32(f32(sp * f32(sp - a)) * f32(sp - b)) * f32(sp - c))).astype(f32)
         tris[k]["ltriIdx"] = len(scene.tri_lights) + k
-        lights.append(light_triangle(tris[k], radiance, k, inst_idx))
+        lights.append(light_triangle(tris[k], radiance, k if tri_ids is None else int(tri_ids[k]), inst_idx))
     scene.tri_lights = concat([scene.tri_lights] + lights, F.CoreLightTri)
 
 
