@@ -221,6 +221,11 @@ void RenderCore::Setting(const char* name, float value)
 		if (!(value >= 1 && value <= 1024)) throw CoreError(LH2HIP_ERR_INVALID, "Setting maxDepth: 1..1024");
 		maxDepth = (int)value;
 	}
+	else if (!strcmp(name, "refit"))
+	{	/* not in the reference (its OptiX BLAS is rebuilt on every SetGeometry) */
+		if (value != 0 && value != 1) throw CoreError(LH2HIP_ERR_INVALID, "Setting refit: 0 (rebuild) or 1 (refit)");
+		refitSetting = value != 0;
+	}
 	else if (!strcmp(name, "nodeFormat"))
 	{
 		if (value != 0 && value != 1) throw CoreError(LH2HIP_ERR_INVALID, "Setting nodeFormat: 0 (fp32) or 1 (q8)");
@@ -301,6 +306,52 @@ void RenderCore::buildMesh(MeshData& m)
 	hipCheck(hipStreamSynchronize(stream), "sync");
 	m.buildDepth = b.depth;
 	free_build(b);
+	layoutMesh(m);
+	const auto t1 = std::chrono::high_resolution_clock::now();
+	m.buildMs = std::chrono::duration<float, std::milli>(t1 - t0).count();
+	stats.bvhBuildTime = m.buildMs * 0.001f;
+}
+
+/* The build tree's nodes are stored level by level (parents before children),
+ * so one reverse sweep recomputes every box from its leaves up: a leaf box
+ * is the min / max of its triangles' vertices (as tri_bounds computes them),
+ * an interior box the union of its two children. */
+void RenderCore::refitMesh(MeshData& m, const lighthouse2::float4* v)
+{
+	const auto t0 = std::chrono::high_resolution_clock::now();
+	std::vector<BuildNode>& nodes = m.buildNodes;
+	for (int i = (int)nodes.size() - 1; i >= 0; i--)
+	{
+		BuildNode& b = nodes[i];
+		float lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+		if (b.count > 0)
+			for (int k = 0; k < b.count; k++)
+			{
+				const lighthouse2::float4* t = v + 3 * (size_t)m.buildPrims[b.leftFirst + k];
+				for (int c = 0; c < 3; c++)
+				{
+					const float x0 = (&t[0].x)[c], x1 = (&t[1].x)[c], x2 = (&t[2].x)[c];
+					lo[c] = std::min(lo[c], std::min(std::min(x0, x1), x2));
+					hi[c] = std::max(hi[c], std::max(std::max(x0, x1), x2));
+				}
+			}
+		else
+			for (int k = 0; k < 2; k++)
+			{
+				const BuildNode& ch = nodes[b.leftFirst + k];
+				for (int c = 0; c < 3; c++) lo[c] = std::min(lo[c], ch.lo[c]), hi[c] = std::max(hi[c], ch.hi[c]);
+			}
+		for (int c = 0; c < 3; c++) b.lo[c] = lo[c], b.hi[c] = hi[c];
+	}
+	layoutMesh(m);
+	const auto t1 = std::chrono::high_resolution_clock::now();
+	m.buildMs = std::chrono::duration<float, std::milli>(t1 - t0).count();
+	stats.bvhBuildTime = m.buildMs * 0.001f;
+}
+
+void RenderCore::layoutMesh(MeshData& m)
+{
+	const int n = m.triCount;
 	Layout L;
 	layout_bvh(m.buildNodes, m.buildPrims, true, L);
 	m.rootF32 = L.rootF32, m.rootQ8 = L.rootQ8;
@@ -317,9 +368,6 @@ void RenderCore::buildMesh(MeshData& m)
 	m.tris.ensure(sizeof(float4) * 3 * (size_t)n);
 	hipCheck(gather_tris(m.verts.as<float4>(), order.as<uint32_t>(), last.as<uint8_t>(), n, m.tris.as<float4>(), stream), "gather_tris");
 	hipCheck(hipStreamSynchronize(stream), "sync");
-	const auto t1 = std::chrono::high_resolution_clock::now();
-	m.buildMs = std::chrono::duration<float, std::milli>(t1 - t0).count();
-	stats.bvhBuildTime = m.buildMs * 0.001f;
 }
 
 void RenderCore::SetGeometry(const int meshIdx, const lighthouse2::float4* vertexData, const int vertexCount, const int triangleCount, const CoreTri* triangles)
@@ -330,6 +378,9 @@ void RenderCore::SetGeometry(const int meshIdx, const lighthouse2::float4* verte
 	if (triangleCount >= (1 << 29)) throw CoreError(LH2HIP_ERR_INVALID, "SetGeometry: too many triangles");
 	if (meshIdx == (int)meshes.size()) meshes.emplace_back();
 	MeshData& m = meshes[meshIdx];
+	/* animation (rendercore_optix7/core_mesh.cpp:34-61 rebuilds): with the refit setting and an
+	 * unchanged triangle count the existing tree is refitted */
+	const bool refit = refitSetting && triangleCount > 0 && m.triCount == triangleCount && !m.buildNodes.empty();
 	m.triCount = triangleCount;
 	m.verts.upload(vertexData, sizeof(float4) * 3 * (size_t)triangleCount, stream);
 	m.maxMaterial = -1;
@@ -339,7 +390,8 @@ void RenderCore::SetGeometry(const int meshIdx, const lighthouse2::float4* verte
 		for (int i = 0; i < triangleCount; i++) m.maxMaterial = std::max(m.maxMaterial, (int)std::min<uint32_t>(triangles[i].material, 0x7fffffffu));
 	}
 	else m.coreTris.release();
-	if (triangleCount > 0) buildMesh(m);
+	if (refit) refitMesh(m, vertexData);
+	else if (triangleCount > 0) buildMesh(m);
 	else m.buildNodes.clear(), m.buildPrims.clear(), m.nodeCountF32 = m.nodeCountQ8 = 0;
 	instancesDirty = true;
 }

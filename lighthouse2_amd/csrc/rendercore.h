@@ -157,6 +157,10 @@ public:
 
 private:
 	void buildMesh(MeshData& m);
+	/* Refit: the build tree keeps its topology, node boxes are recomputed
+	 * bottom-up from the new vertices (host copy), then laid out again. */
+	void refitMesh(MeshData& m, const lighthouse2::float4* vertexData);
+	void layoutMesh(MeshData& m);
 	void ensureTraceBuffers();
 	void updateTopLevel();
 
@@ -178,6 +182,7 @@ private:
 	int maxDepth = 64, nodeFormat = FMT_Q8, traceLoop = LOOP_MERGED, triBias = 16, refillLanes = 16, chunk = 32;
 	uint32_t camSeed = 0x12345678u, seedSetting = 0x12345678u;
 	int estimator = ESTIMATOR_REFERENCE;
+	bool refitSetting = false;         /* SetGeometry on a mesh of unchanged size: refit instead of rebuild */
 	int bsdfSetting = 1;               /* Disney, as rendercore_optix7 ships (kernels/bsdf.h) */
 	/* filter estimator (RenderCore_Optix7Filter/rendercore.h:32-35,124) */
 	bool filterOn = true;
