@@ -7,12 +7,16 @@ calls a core:
 * PNG textures: HostTexture::Load (host_texture.cpp:200-241) for 8-bit images,
   FLIPPED / LINEARIZED as the OBJ path requests them, MIP chain of
   ConstructMIPmaps;
-* lights of emissive triangles: HostNode::PrepareLights (host_node.cpp:203-233),
-  a material being emissive when a colour component exceeds 1
-  (HostMaterial::IsEmissive, host_material.h:82).
+* glTF 2.0 (.glb, .gltf with .bin files or data: URIs): HostScene::AddScene
+  (host_scene.cpp:249-392) with HostMesh::ConvertFromGTLFMesh /
+  BuildFromIndexedData (host_mesh.cpp:327-681), HostMaterial::ConvertFrom
+  (host_material.cpp:76-170) and the HostNode hierarchy (host_node.cpp:77-199);
+* lights of emissive triangles: HostNode::PrepareLights / UpdateLights
+  (host_node.cpp:203-255) in world space, a material being emissive when a
+  colour component exceeds 1 (HostMaterial::IsEmissive, host_material.h:82).
 
-The file parsers are this project's (the reference uses tinyobjloader and
-FreeImage): polygons are split as a fan, which is what tinyobjloader's ear
+The file parsers are this project's (the reference uses tinyobjloader,
+tinygltf and FreeImage): polygons are split as a fan, which is what tinyobjloader's ear
 clipping gives for convex polygons; PNG is decoded with zlib (8-bit grey,
 grey + alpha, RGB, RGBA and palette images, not interlaced). All float
 arithmetic of the producers is float32 in the reference's operation order.
@@ -106,11 +110,18 @@ def decode_png(data):
     return rgba
 
 
+TEX_NORMALMAP, TEX_LDR = 2, 4                         # HostTexture flags (host_texture.h:39-44)
+
+
 def texture_from_rgba(rgba, flipped=False, linearized=False, normal_map=False):
-    """HostTexture::Load for an 8-bit image (host_texture.cpp:225-241): the
-    image library delivers rows bottom-up, so FLIPPED keeps the bottom row
-    first and no flag stores the top row first; LINEARIZED squares r, g, b
-    ((c * c) >> 8, sRGBtoLinear :73-81); 5 MIP levels."""
+    """HostTexture::Load for an 8-bit image (host_texture.cpp:225-241): each
+    texel is the bytes r, g, b, a in that order (make_uchar4 of the red,
+    green, blue and alpha channels; the core reads byte k as component k,
+    sampling_shared.h:29-33). The image library delivers rows bottom-up, so
+    FLIPPED keeps the bottom row first and no flag stores the top row first;
+    LINEARIZED squares r, g, b ((c * c) >> 8, sRGBtoLinear :73-81); 5 MIP
+    levels (ConstructMIPmaps); flags LDR (| NORMALMAP, storage NRM32, for a
+    normal map: ConvertToCoreTexDesc :43-67)."""
     a = np.asarray(rgba, np.uint8)
     h, w = a.shape[:2]
     rows = a[::-1] if flipped else a
@@ -118,9 +129,18 @@ def texture_from_rgba(rgba, flipped=False, linearized=False, normal_map=False):
     r, g, b, al = c[..., 0], c[..., 1], c[..., 2], c[..., 3]
     if linearized:
         r, g, b = (r * r) >> 8, (g * g) >> 8, (b * b) >> 8
-    px = scenes.pack_texels(r, g, b, al)
-    return {"width": w, "height": h, "flags": 0, "storage": F.NRM32 if normal_map else F.ARGB32,
-            "pixels": scenes.mip_chain(px)}
+    px = (al << 24) | (b << 16) | (g << 8) | r
+    t = {"width": w, "height": h, "flags": TEX_LDR, "storage": F.ARGB32, "pixels": scenes.mip_chain(px)}
+    if normal_map:
+        mark_normal_map(t)
+    return t
+
+
+def mark_normal_map(tex):
+    """HostMaterial sets NORMALMAP on a texture it uses as a normal map; the
+    core then receives it as NRM32 texels (host_texture.cpp:61)."""
+    tex["flags"] |= TEX_NORMALMAP
+    tex["storage"] = F.NRM32
 
 
 def load_texture(path, flipped=False, linearized=False, normal_map=False):
@@ -355,19 +375,449 @@ def load_obj(scene, path, transform=None, flat_shaded=False):
     return scene.add_mesh(verts, tris)
 
 
-def add_instance_with_lights(scene, mesh, transform=None):
-    """SetInstance plus HostNode::PrepareLights for an identity-transformed
-    instance: every triangle whose material is emissive becomes a
-    CoreLightTri of radiance = material colour. Returns the instance index."""
-    if transform is not None and not np.array_equal(np.asarray(transform, f32).reshape(16), F.IDENTITY):
-        raise ValueError("lights of transformed instances are not produced here")
-    inst = scene.add_instance(mesh, transform)
+# ---- glTF 2.0 -------------------------------------------------------------------
+_GLTF_COMPONENT = {5120: "<i1", 5121: "<u1", 5122: "<i2", 5123: "<u2", 5124: "<i4", 5125: "<u4", 5126: "<f4"}
+_GLTF_WIDTH = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT2": 4, "MAT3": 9, "MAT4": 16}
+MODE_TRIANGLES, MODE_TRIANGLE_STRIP, MODE_TRIANGLE_FAN = 4, 5, 6
+MAT_FROM_MTL, MAT_SINGLE_COLOR_COPY = 4, 16          # HostMaterial flags (host_material.h:68-74)
+
+
+def read_gltf(path):
+    """A .gltf (JSON; buffers and images from files beside it or data: URIs)
+    or .glb (binary container, JSON chunk + BIN chunk) -> (document dict, list
+    of buffer bytes, directory)."""
+    import base64
+    import json
+    directory = os.path.dirname(os.path.abspath(path))
+    with open(path, "rb") as fh:
+        data = fh.read()
+    glb_bin = None
+    if data[:4] == b"glTF":
+        version, length = struct.unpack("<II", data[4:12])
+        if version != 2:
+            raise ValueError(f"{path}: glb version {version}")
+        pos, doc = 12, None
+        while pos < min(length, len(data)):
+            n, kind = struct.unpack("<II", data[pos:pos + 8])
+            body = data[pos + 8:pos + 8 + n]
+            if kind == 0x4E4F534A:
+                doc = json.loads(body.decode("utf-8"))
+            elif kind == 0x004E4942 and glb_bin is None:
+                glb_bin = body
+            pos += 8 + n
+        if doc is None:
+            raise ValueError(f"{path}: glb without a JSON chunk")
+    else:
+        doc = json.loads(data.decode("utf-8"))
+
+    def fetch(uri):
+        if uri.startswith("data:"):
+            return base64.b64decode(uri.split(",", 1)[1])
+        from urllib.parse import unquote
+        with open(os.path.join(directory, unquote(uri)), "rb") as fh:
+            return fh.read()
+    bufs = []
+    for b in doc.get("buffers", []):
+        if "uri" in b:
+            bufs.append(fetch(b["uri"]))
+        elif glb_bin is not None:
+            bufs.append(glb_bin)
+        else:
+            raise ValueError(f"{path}: buffer without data")
+    doc["_fetch"] = fetch
+    return doc, bufs, directory
+
+
+def gltf_accessor(doc, bufs, index):
+    """An accessor's elements as an (count, components) array, reading
+    bufferView.byteOffset + accessor.byteOffset and stepping byteStride (the
+    element size when the view gives none: Accessor::ByteStride)."""
+    acc = doc["accessors"][index]
+    if "bufferView" not in acc or "sparse" in acc:
+        raise ValueError(f"accessor {index}: accessors without a buffer view, or sparse, are not read")
+    view = doc["bufferViews"][acc["bufferView"]]
+    dt = np.dtype(_GLTF_COMPONENT[acc["componentType"]])
+    width = _GLTF_WIDTH[acc["type"]]
+    stride = view.get("byteStride", 0) or dt.itemsize * width
+    count = acc["count"]
+    base = view.get("byteOffset", 0) + acc.get("byteOffset", 0)
+    buf = bufs[view["buffer"]]
+    if count and base + stride * (count - 1) + dt.itemsize * width > len(buf):
+        raise ValueError(f"accessor {index} runs past the end of buffer {view['buffer']}")
+    if count == 0:
+        return np.zeros((0, width), dt)
+    return np.ndarray((count, width), dt, buffer=buf, offset=base, strides=(stride, dt.itemsize)).copy()
+
+
+def _gltf_float_attribute(doc, bufs, index, acc_type, what):
+    acc = doc["accessors"][index]
+    if acc["type"] != acc_type:
+        raise ValueError(f"{what}: expected {acc_type} in the glTF file, got {acc['type']}")
+    if acc["componentType"] != 5126:
+        raise ValueError(f"{what}: only float components are read")
+    return gltf_accessor(doc, bufs, index).astype(f32)
+
+
+def gltf_primitive_indices(doc, bufs, prim):
+    """Triangle corner indices of one primitive (host_mesh.cpp:333-374): the
+    index accessor widened to int (signed for BYTE / SHORT), a fan expanded to
+    (f0, f[i-1], f[i]) and a strip to (s[i-2], s[i-1], s[i]) (no winding
+    alternation); None for a non-triangle mode. A primitive without indices
+    uses 0 .. count-1 of its positions, as the glTF specification defines."""
+    mode = prim.get("mode", MODE_TRIANGLES)
+    if "indices" in prim:
+        idx = gltf_accessor(doc, bufs, prim["indices"]).reshape(-1).astype(np.int64)
+    else:
+        idx = np.arange(doc["accessors"][prim["attributes"]["POSITION"]]["count"], dtype=np.int64)
+    if mode == MODE_TRIANGLE_FAN:
+        k = np.arange(2, len(idx))
+        idx = np.stack([np.full(len(k), idx[0] if len(idx) else 0), idx[k - 1], idx[k]], 1).reshape(-1)
+    elif mode == MODE_TRIANGLE_STRIP:
+        k = np.arange(2, len(idx))
+        idx = np.stack([idx[k - 2], idx[k - 1], idx[k]], 1).reshape(-1)
+    elif mode != MODE_TRIANGLES:
+        return None
+    return idx[:len(idx) // 3 * 3].reshape(-1, 3)
+
+
+def indexed_triangles(scene, idx, P, Nv=None, UV=None, UV2=None, material=0):
+    """HostMesh::BuildFromIndexedData (host_mesh.cpp:520-681): the vertex soup
+    and CoreTri array of indexed triangles. Per unique vertex the
+    consistent-normal alpha is min over its triangles of dot(vN, N) (N flipped
+    only when all three vertex normals face away; no 0.7 clamp), then
+    acos(a) (1 + 0.03632 (1 - a)^2); the stored face normal is not flipped;
+    tangents from the uvs, PBRT's frame for a degenerate uv mapping
+    (pbrt-v3 triangle.cpp), the edge frame when they come out NaN or when
+    there are no uvs; a triangle whose three uvs are one point of a textured
+    material gets a single-colour copy of the material
+    (HostScene::FindOrCreateMaterialCopy, host_scene.cpp:570-595). LOD and
+    area stay 0 (the glTF path sets neither)."""
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return _indexed_triangles(scene, np.asarray(idx, np.int64).reshape(-1, 3), np.asarray(P, f32), Nv, UV, UV2, material)
+
+
+def _indexed_triangles(scene, idx, P, Nv, UV, UV2, material):
+    T = len(idx)
+    q = P[idx]                                        # (T, 3, 3)
+    v0, v1, v2 = q[:, 0], q[:, 1], q[:, 2]
+    e1, e2 = (v1 - v0).astype(f32), (v2 - v0).astype(f32)
+    N = _normalize(_cross(e1, e2))
+    alphas = np.ones(len(P), f32)
+    if Nv is not None:
+        vN = np.asarray(Nv, f32)[idx]
+        flip = (_dot(N, vN[:, 0]) < 0) & (_dot(N, vN[:, 1]) < 0) & (_dot(N, vN[:, 2]) < 0)
+        Na = np.where(flip[:, None], -N, N).astype(f32)
+    else:
+        vN = np.repeat(N[:, None, :], 3, 1)
+        Na = N
+    for c in range(3):                                # std::min(alpha, d) keeps alpha when d is NaN
+        np.fmin.at(alphas, idx[:, c], _dot(vN[:, c], Na))
+    w = f32(0.03632)
+    alphas = (np.arccos(alphas).astype(f32) * (f32(1) + w * (f32(1) - alphas) * (f32(1) - alphas))).astype(f32)
+    verts = np.ones((3 * T, 4), f32)
+    verts[:, :3] = q.reshape(-1, 3)
+    tris = F.new_tris(T)
+    tris["material"] = material
+    tris["Nx"], tris["Ny"], tris["Nz"] = N[:, 0], N[:, 1], N[:, 2]
+    tris["vertex0"], tris["vertex1"], tris["vertex2"] = v0, v1, v2
+    tris["alpha"] = alphas[idx]
+    tris["vN0"], tris["vN1"], tris["vN2"] = vN[:, 0], vN[:, 1], vN[:, 2]
+    Te = _normalize(e1)
+    Be = _normalize(_cross(N, Te))
+    if UV is not None:
+        uv = np.asarray(UV, f32)[idx]                 # (T, 3, 2)
+        for c in range(3):
+            tris[f"u{c}"], tris[f"v{c}"] = uv[:, c, 0], uv[:, c, 1]
+        _single_texel_materials(scene, tris, uv, material)
+        uv01, uv02 = (uv[:, 1] - uv[:, 0]).astype(f32), (uv[:, 2] - uv[:, 0]).astype(f32)
+        d01 = (uv01[:, 0] * uv01[:, 0] + uv01[:, 1] * uv01[:, 1]).astype(f32)
+        d02 = (uv02[:, 0] * uv02[:, 0] + uv02[:, 1] * uv02[:, 1]).astype(f32)
+        degenerate = (d01 == 0) | (d02 == 0)
+        ax = np.abs(N[:, 0]) > np.abs(N[:, 1])
+        sx = np.sqrt((N[:, 0] * N[:, 0] + N[:, 2] * N[:, 2]).astype(f32)).astype(f32)
+        sy = np.sqrt((N[:, 1] * N[:, 1] + N[:, 2] * N[:, 2]).astype(f32)).astype(f32)
+        zero = np.zeros(T, f32)
+        Tp = np.where(ax[:, None], np.stack([-N[:, 2] / sx, zero / sx, N[:, 0] / sx], 1),
+                      np.stack([zero / sy, N[:, 2] / sy, -N[:, 1] / sy], 1)).astype(f32)
+        Bp = _normalize(_cross(N, Tp))
+        Tu = _normalize((e1 * uv02[:, 1:2] - e2 * uv01[:, 1:2]).astype(f32))
+        Bu = _normalize((e2 * uv01[:, 0:1] - e1 * uv02[:, 0:1]).astype(f32))
+        Tn = np.where(degenerate[:, None], Tp, Tu)
+        Bn = np.where(degenerate[:, None], Bp, Bu)
+        ssum = Tn[:, 0] + Tn[:, 1]
+        for v in (Tn[:, 2], Bn[:, 0], Bn[:, 1], Bn[:, 2]):
+            ssum = (ssum + v).astype(f32)
+        bad = np.isnan(ssum)
+        Tn = np.where(bad[:, None], Te, Tn).astype(f32)
+        Bn = np.where(bad[:, None], Be, Bn).astype(f32)
+    else:
+        Tn, Bn = Te, Be
+    tris["T"], tris["B"] = Tn, Bn
+    if UV2 is not None:
+        uv2 = np.asarray(UV2, f32)[idx]
+        for c in range(3):
+            tris[f"u1_{c}"], tris[f"v1_{c}"] = uv2[:, c, 0], uv2[:, c, 1]
+    return verts, tris
+
+
+def _single_texel_materials(scene, tris, uv, material):
+    """Triangles that read one texel of a textured material get an untextured
+    copy of it whose colour is that texel (level 0, texel & 0xffffff, bytes
+    2, 1, 0 as b, g, r scaled by 1/255 into x, y, z); an existing copy of
+    the same colour is reused, whatever it was copied from."""
+    tid = int(scene.materials["color"]["textureID"][material])
+    if tid < 0:
+        return
+    one = (uv[:, 0, 0] == uv[:, 1, 0]) & (uv[:, 1, 0] == uv[:, 2, 0]) & (uv[:, 0, 1] == uv[:, 1, 1]) & (uv[:, 1, 1] == uv[:, 2, 1])
+    if not one.any():
+        return
+    tex = scene.textures[tid]
+    w, h = tex["width"], tex["height"]
+    for t in np.nonzero(one)[0]:
+        u = (int(np.trunc(f32(uv[t, 0, 0]) * f32(w))) & 0xffffffff) % w
+        v = (int(np.trunc(f32(uv[t, 0, 1]) * f32(h))) & 0xffffffff) % h
+        texel = int(tex["pixels"][u + v * w]) & 0xffffff
+        r, g, b = (texel >> 16) & 255, (texel >> 8) & 255, texel & 255
+        c = np.array([f32(b) * f32(1.0 / 255.0), f32(g) * f32(1.0 / 255.0), f32(r) * f32(1.0 / 255.0)], f32)
+        mats = scene.materials
+        hit = np.nonzero(((mats["flags"] & MAT_SINGLE_COLOR_COPY) != 0) & (mats["color"]["value"] == c).all(1))[0]
+        if len(hit):
+            tris["material"][t] = hit[0]
+            continue
+        m = mats[material:material + 1].copy()
+        m["color"]["textureID"] = -1
+        m["color"]["value"][0] = c
+        m["flags"] |= MAT_SINGLE_COLOR_COPY
+        tris["material"][t] = scene.add_material(m)
+
+
+def _gltf_material(mat, tex_ids, scene):
+    """HostMaterial::ConvertFrom(tinygltf material) (host_material.cpp:76-170):
+    only the factors present in the file are set (absent ones stay "not
+    set"): baseColorFactor -> colour, metallicFactor, roughnessFactor;
+    baseColorTexture -> colour map; metallicRoughnessTexture -> roughness and
+    metallic maps; normalTexture (index, scale) -> normal map, its texture
+    marked as one. Emissive, occlusion and alpha-mode entries are ignored."""
+    m = F.new_materials(1)
+    m["flags"] = 1 | MAT_FROM_MTL                     # SMOOTH (the default) | FROM_MTL
+    nt = mat.get("normalTexture")
+    if nt is not None and nt.get("index", -1) > -1:
+        m["normals"]["textureID"] = tex_ids[nt["index"]]
+        m["normals"]["scale"] = nt.get("scale", 1.0)
+        mark_normal_map(scene.textures[tex_ids[nt["index"]]])
+    pbr = mat.get("pbrMetallicRoughness", {})
+    if "baseColorFactor" in pbr:
+        m["color"]["value"][0] = pbr["baseColorFactor"][:3]
+    if "metallicFactor" in pbr:
+        m["metallic"]["value"] = pbr["metallicFactor"]
+    if "roughnessFactor" in pbr:
+        m["roughness"]["value"] = pbr["roughnessFactor"]
+    if "baseColorTexture" in pbr and "index" in pbr["baseColorTexture"]:
+        m["color"]["textureID"] = tex_ids[int(pbr["baseColorTexture"]["index"])]
+    if "metallicRoughnessTexture" in pbr and "index" in pbr["metallicRoughnessTexture"]:
+        t = tex_ids[int(pbr["metallicRoughnessTexture"]["index"])]
+        m["roughness"]["textureID"] = t
+        m["metallic"]["textureID"] = t
+    return m
+
+
+def _gltf_image_rgba(doc, bufs, image):
+    if "bufferView" in image:
+        view = doc["bufferViews"][image["bufferView"]]
+        off = view.get("byteOffset", 0)
+        data = bytes(bufs[view["buffer"]][off:off + view["byteLength"]])
+    elif "uri" in image:
+        data = doc["_fetch"](image["uri"])
+    else:
+        raise ValueError("glTF image without data")
+    return decode_png(data)
+
+
+def gltf_node_transform(node):
+    """HostNode::ConvertFromGLTFNode + UpdateTransformFromTRS
+    (host_node.cpp:77-135): a column-major matrix is transposed into the
+    row-major cells; T * R * S * matrix in fp32 when any of matrix,
+    translation, rotation (x, y, z, w; quat::toMatrix common_types.h:692-705)
+    or scale is given, identity otherwise."""
+    M = F.IDENTITY.copy()
+    given = False
+    if len(node.get("matrix", [])) == 16:
+        M = np.asarray(node["matrix"], np.float64).astype(f32).reshape(4, 4).T.reshape(16).copy()
+        given = True
+    t = np.zeros(3, f32)
+    if len(node.get("translation", [])) == 3:
+        t = np.asarray(node["translation"], np.float64).astype(f32)
+        given = True
+    x = y = z = f32(0)
+    w = f32(1)
+    if len(node.get("rotation", [])) == 4:
+        x, y, z, w = (f32(c) for c in node["rotation"])
+        given = True
+    sc = np.ones(3, f32)
+    if len(node.get("scale", [])) == 3:
+        sc = np.asarray(node["scale"], np.float64).astype(f32)
+        given = True
+    if not given:
+        return M
+    Tm = F.IDENTITY.copy()
+    Tm[3], Tm[7], Tm[11] = t
+    two = f32(2)
+    R = F.IDENTITY.copy()
+    R[0] = (f32(1) - two * y * y) - two * z * z
+    R[1] = two * x * y - two * w * z
+    R[2] = two * x * z + two * w * y
+    R[4] = two * x * y + two * w * z
+    R[5] = (f32(1) - two * x * x) - two * z * z
+    R[6] = two * y * z - two * w * x
+    R[8] = two * x * z - two * w * y
+    R[9] = two * y * z + two * w * x
+    R[10] = (f32(1) - two * x * x) - two * y * y
+    S = F.IDENTITY.copy()
+    S[0], S[5], S[10] = sc
+    return mat4_mul(mat4_mul(mat4_mul(Tm, R), S), M)
+
+
+def load_gltf(scene, path, transform=None):
+    """HostScene::AddScene for a glTF file (host_scene.cpp:249-392): adds the
+    file's textures (8-bit PNG images, top row first, neither flipped nor
+    linearised), materials, one mesh per glTF mesh (its triangle primitives
+    concatenated, BuildFromIndexedData each), and the node hierarchy of scene
+    0 under a root node holding `transform`: one instance per node with a
+    mesh, in the order HostNode::Update visits them (children before their
+    parent, host_node.cpp:141-199), at the combined transform, plus the
+    lights of emissive triangles in world space in node order. Returns the
+    list of instance indices. Without materials in the file every triangle
+    uses scene material 0 (a grey Lambert material is added when the scene
+    has none); a primitive without a material gets a grey Lambert one.
+    Skins, morph targets and animations are not read (the base pose is)."""
+    doc, bufs, directory = read_gltf(path)
+    tex_ids = []
+    for t in doc.get("textures", []):
+        if "source" not in t:
+            raise ValueError("glTF texture without a source image")
+        tex_ids.append(scene.add_texture(texture_from_rgba(_gltf_image_rgba(doc, bufs, doc["images"][t["source"]]))))
+    mat_ids = [scene.add_material(_gltf_material(m, tex_ids, scene)) for m in doc.get("materials", [])]
+    override = None
+    if not mat_ids:
+        if len(scene.materials) == 0:
+            scene.add_material(F.lambert_material((0.5, 0.5, 0.5)))
+        override = 0
+    default = [None]
+
+    def prim_material(prim):
+        if override is not None:
+            return override
+        if prim.get("material", -1) < 0:
+            if default[0] is None:
+                default[0] = scene.add_material(F.lambert_material((0.5, 0.5, 0.5)))
+            return default[0]
+        return mat_ids[prim["material"]]
+    mesh_ids = []
+    for mesh in doc.get("meshes", []):
+        parts = []
+        for prim in mesh.get("primitives", []):
+            idx = gltf_primitive_indices(doc, bufs, prim)
+            if idx is None:
+                continue
+            at = prim["attributes"]
+            P = _gltf_float_attribute(doc, bufs, at["POSITION"], "VEC3", "POSITION")
+            Nv = _gltf_float_attribute(doc, bufs, at["NORMAL"], "VEC3", "NORMAL") if "NORMAL" in at else None
+            UV = _gltf_float_attribute(doc, bufs, at["TEXCOORD_0"], "VEC2", "TEXCOORD_0") if "TEXCOORD_0" in at else None
+            UV2 = _gltf_float_attribute(doc, bufs, at["TEXCOORD_1"], "VEC2", "TEXCOORD_1") if "TEXCOORD_1" in at else None
+            if "TANGENT" in at:
+                _gltf_float_attribute(doc, bufs, at["TANGENT"], "VEC4", "TANGENT")    # read and unused
+            if len(idx) and (idx.min() < 0 or idx.max() >= len(P)):
+                raise ValueError("glTF indices outside the vertex range")
+            parts.append(indexed_triangles(scene, idx, P, Nv, UV, UV2, prim_material(prim)))
+        if parts:
+            verts = np.concatenate([p[0] for p in parts])
+            tris = np.concatenate([p[1] for p in parts])
+        else:
+            verts, tris = np.zeros((0, 4), f32), F.new_tris(0)
+        mesh_ids.append(scene.add_mesh(verts, tris))
+    nodes = doc.get("nodes", [])
+    node_base = getattr(scene, "node_count", len(scene.instances)) + 1
+    scene.node_count = node_base + len(nodes)
+    root = F.IDENTITY.copy() if transform is None else np.asarray(transform, f32).reshape(16)
+    local = [gltf_node_transform(n) for n in nodes]
+    combined = [None] * len(nodes)
+    order = []
+
+    def visit(i, parent):
+        combined[i] = mat4_mul(parent, local[i])
+        for c in nodes[i].get("children", []):
+            visit(c, combined[i])
+        if "mesh" in nodes[i]:
+            order.append(i)
+    scenes_ = doc.get("scenes", [])
+    root_combined = mat4_mul(F.IDENTITY, root)
+    for r in (scenes_[0].get("nodes", []) if scenes_ else []):
+        visit(r, root_combined)
+    instances = [scene.add_instance(mesh_ids[nodes[i]["mesh"]], combined[i]) for i in order]
+    for i in sorted(order):                          # lights in node order (PrepareLights at construction)
+        instance_lights(scene, mesh_ids[nodes[i]["mesh"]], combined[i], node_base + i)
+    return instances
+
+
+def mat4_mul(a, b):
+    """operator*(mat4, mat4) (lib/platform/system.cpp:57-69), fp32, cells
+    row-major: r[i][j] = ((a[i][0] b[0][j] + a[i][1] b[1][j]) + a[i][2] b[2][j]) + a[i][3] b[3][j]."""
+    a = np.asarray(a, f32).reshape(4, 4)
+    b = np.asarray(b, f32).reshape(4, 4)
+    r = ((a[:, 0:1] * b[0:1, :] + a[:, 1:2] * b[1:2, :]) + a[:, 2:3] * b[2:3, :]) + a[:, 3:4] * b[3:4, :]
+    return r.astype(f32).reshape(16)
+
+
+def _transform_points(M, p, w):
+    """float4(p, w) * mat4 (system.cpp:97-103), rows 0..3."""
+    M = np.asarray(M, f32).reshape(4, 4)
+    return np.stack([((M[r, 0] * p[..., 0] + M[r, 1] * p[..., 1]) + M[r, 2] * p[..., 2]) + M[r, 3] * f32(w)
+                     for r in range(4)], -1).astype(f32)
+
+
+def _transformed_tris(tris, M):
+    """TransformedHostTri (host_node.cpp:19-30): vertices (v, 1) * T, the
+    face normal normalize((N, 0) * T) as a float4."""
+    out = tris.copy()
+    for k in ("vertex0", "vertex1", "vertex2"):
+        out[k] = _transform_points(M, tris[k], 1.0)[:, :3]
+    N4 = _transform_points(M, np.stack([tris["Nx"], tris["Ny"], tris["Nz"]], -1), 0.0)
+    d = (((N4[:, 0] * N4[:, 0] + N4[:, 1] * N4[:, 1]) + N4[:, 2] * N4[:, 2]) + N4[:, 3] * N4[:, 3]).astype(f32)
+    N4 = (N4 * (f32(1) / np.sqrt(d))[:, None]).astype(f32)
+    out["Nx"], out["Ny"], out["Nz"] = N4[:, 0], N4[:, 1], N4[:, 2]
+    return out
+
+
+def instance_lights(scene, mesh, transform, node_id):
+    """HostNode::PrepareLights / UpdateLights (host_node.cpp:203-255) for one
+    instance: every triangle whose material is emissive (a colour component
+    above 1, HostMaterial::IsEmissive host_material.h:82) gets its
+    object-space area (CoreTri::UpdateArea) and its index in the light list
+    (ltriIdx), and a CoreLightTri of the triangle transformed by the
+    instance's transform is appended, radiance = material colour, instIdx =
+    the node's id."""
     verts, tris = scene.meshes[mesh]
+    if len(tris) == 0:
+        return
     col = scene.materials["color"]["value"][tris["material"]]
-    emissive = (col > 1).any(1)
-    for m in np.unique(tris["material"][emissive]):
-        sel = np.nonzero(tris["material"] == m)[0]
-        sub = tris[sel]
-        scenes.add_emitters(scene, sub, scene.materials["color"]["value"][m], inst, tri_ids=sel)
-        tris[sel] = sub
+    sel = np.nonzero((col > 1).any(1))[0]
+    if len(sel) == 0:
+        return
+    sub = tris[sel]
+    scenes.add_emitters(scene, sub, np.zeros(3, f32), node_id, tri_ids=sel)   # areas, ltriIdx, light slots
+    world = _transformed_tris(sub, transform)
+    first = len(scene.tri_lights) - len(sel)
+    for k in range(len(sel)):
+        rad = scene.materials["color"]["value"][sub["material"][k]]
+        scene.tri_lights[first + k] = scenes.light_triangle(world[k], rad, int(sel[k]), node_id)[0]
+    tris[sel] = sub
+
+
+def add_instance_with_lights(scene, mesh, transform=None):
+    """SetInstance plus the lights of HostNode(meshIdx, transform)
+    (host_node.cpp:40-47): the instance's emissive triangles become
+    CoreLightTris in world space. Returns the instance index (which is also
+    the node id when every node of the scene is an instance)."""
+    inst = scene.add_instance(mesh, transform)
+    instance_lights(scene, mesh, scene.instances[inst][1], inst)
     return inst

@@ -80,3 +80,70 @@ def test_obj_scene_matches_oracle(torch_cuda, tmp_path, estimator):
         _, r64, n, _, _ = orc.render7(s, view, w, h, 2, 2, threads=16, bsdf=0)
     assert np.array_equal(acc, r64)
     assert r64[..., :3].sum() > 0
+
+
+def write_gltf_scene(d, slices=20, stacks=10):
+    """A glTF scene: one sphere mesh (u16 indices, normals, uvs) instanced by a
+    rotated, scaled root node and by its child (column-major matrix), a
+    textured floor (u8 indices) and an emissive quad under a translated node."""
+    from gltf_build import GltfBuilder
+    g = GltfBuilder()
+    y, x = np.mgrid[0:16, 0:16]
+    img = np.zeros((16, 16, 4), np.uint8)
+    img[..., 0] = np.where(((x // 2) + (y // 2)) % 2, 220, 50)
+    img[..., 1] = 60 + 8 * x
+    img[..., 2] = 190 - 6 * y
+    img[..., 3] = 255
+    tex = g.image_png(encode_png(img, 6, [1, 2, 3, 4, 0]))
+    plastic = g.material(pbrMetallicRoughness={"baseColorFactor": [0.7, 0.3, 0.2, 1], "roughnessFactor": 1.0, "metallicFactor": 0.0})
+    floor = g.material(pbrMetallicRoughness={"baseColorFactor": [1, 1, 1, 1], "roughnessFactor": 1.0, "metallicFactor": 0.0,
+                                             "baseColorTexture": {"index": tex}})
+    lamp = g.material(pbrMetallicRoughness={"baseColorFactor": [14, 12, 10, 1], "roughnessFactor": 1.0})
+    P, N, UV, I = [], [], [], []
+    for i in range(stacks + 1):
+        th = np.pi * i / stacks
+        for j in range(slices + 1):
+            ph = 2 * np.pi * j / slices
+            n = (np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph))
+            P.append(n), N.append(n), UV.append((j / slices, i / stacks))
+    for i in range(stacks):
+        for j in range(slices):
+            a, b = i * (slices + 1) + j, (i + 1) * (slices + 1) + j
+            I += [a, a + 1, b + 1, a, b + 1, b]
+    sphere = g.mesh([g.primitive(np.float32(P), I, np.float32(N), np.float32(UV), plastic)])
+    quad = lambda y0, s: np.float32([[-s, y0, -s], [s, y0, -s], [s, y0, s], [-s, y0, s]])
+    ground = g.mesh([g.primitive(quad(0, 1.5) + [0.5, 0, 0.5], [0, 3, 2, 0, 2, 1], None,
+                                 np.float32([[0, 0], [0, 3], [3, 3], [3, 0]]), floor, index_dtype="u1")])
+    light = g.mesh([g.primitive(quad(0, 0.2), [0, 2, 1, 0, 3, 2], None, None, lamp)])     # faces +y
+    child = g.node(mesh=sphere, matrix=[0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0, 2.2, 0.1, 0, 1])
+    g.node(root=True, mesh=sphere, children=[child], translation=[0.35, 0.22, 0.5],
+           rotation=[0, np.sin(0.3), 0, np.cos(0.3)], scale=[0.22, 0.22, 0.22])
+    g.node(root=True, mesh=ground)
+    g.node(root=True, mesh=light, translation=[0.5, 1.1, 0.5], rotation=[1, 0, 0, 0])   # flipped to face down
+    return g.write_glb(d / "scene.glb")
+
+
+@pytest.mark.parametrize("estimator", [0, 1])
+def test_gltf_scene_matches_oracle(torch_cuda, tmp_path, estimator):
+    s = scenes.Scene()
+    L.load_gltf(s, str(write_gltf_scene(tmp_path)))
+    assert len(s.instances) == 4 and len(s.tri_lights) == 2
+    w, h = 48, 40
+    view = scenes.camera_view((0.5, 0.6, -1.0), (0.5, 0.3, 0.5), 55.0, w, h)
+    core = lh.RenderCore()
+    lh.load_scene(core, s)
+    core.SetTarget(w, h, 2)
+    core.Setting("estimator", estimator)
+    core.Setting("maxDepth", 3)
+    if estimator == 1:
+        core.Setting("bsdf", 0)
+    for m in [lh.LH2_RESTART, lh.LH2_CONVERGE]:
+        core.Render(view, m)
+    acc = core.read_accumulator_f64()
+    core.Shutdown()
+    if estimator == 0:
+        r64 = orc.render(s, view, w, h, 2, 1, max_depth=3, threads=16)[1]
+    else:
+        r64 = orc.render7(s, view, w, h, 2, 1, threads=16, bsdf=0)[1]
+    assert np.array_equal(acc, r64)
+    assert r64[..., :3].sum() > 0

@@ -71,9 +71,12 @@ def test_texture_flip_and_linearize():
     t = L.texture_from_rgba(img, flipped=True, linearized=True)
     px = t["pixels"][:8].reshape(2, 4)
     assert np.all((px[0] >> 8) & 255 == (100 * 100) >> 8)   # FLIPPED: the bottom row comes first
-    assert np.all((px[1] >> 16) & 255 == (200 * 200) >> 8)
+    assert np.all(px[1] & 255 == (200 * 200) >> 8)           # byte 0 is red (make_uchar4(r, g, b, a))
+    assert np.all(px >> 24 == 255)
     t2 = L.texture_from_rgba(img)
-    assert np.all((t2["pixels"][:4] >> 16) & 255 == 200)
+    assert np.all(t2["pixels"][:4] & 255 == 200) and t2["storage"] == F.ARGB32 and t2["flags"] == L.TEX_LDR
+    t3 = L.texture_from_rgba(img, normal_map=True)
+    assert t3["storage"] == F.NRM32 and t3["flags"] == L.TEX_LDR | L.TEX_NORMALMAP
 
 
 OBJ = """mtllib box.mtl
@@ -151,7 +154,7 @@ def test_obj_closed_forms(tmp_path):
     tex = s.textures[0]
     px = tex["pixels"][:64].reshape(8, 8)
     # row 0 is the image's bottom row: pixel (x 0, y 7) is 220 red; row 7 the top row (40 red)
-    assert ((px[0, 0] >> 16) & 255) == (220 * 220) >> 8 and ((px[7, 0] >> 16) & 255) == (40 * 40) >> 8
+    assert (px[0, 0] & 255) == (220 * 220) >> 8 and (px[7, 0] & 255) == (40 * 40) >> 8
     assert ((px[0, 0] >> 8) & 255) == (120 * 120) >> 8
     # lights: the two glow triangles, areas by Heron, ltriIdx in list order
     inst = L.add_instance_with_lights(s, mesh)
