@@ -33,9 +33,51 @@ from . import scenes
 f32 = np.float32
 
 
-# ---- PNG -------------------------------------------------------------------
+# ---- PNG / TGA -------------------------------------------------------------------
+def _unfilter(rows, bpp):
+    """PNG row filters (None, Sub, Up, Average, Paeth) over bytes; bpp is the
+    filter unit in bytes (at least 1)."""
+    h, stride = rows.shape[0], rows.shape[1] - 1
+    out = np.zeros((h, stride), np.uint8)
+    prev = [0] * stride
+    for y in range(h):
+        f = int(rows[y, 0])
+        line = rows[y, 1:]
+        if f == 0:
+            cur = line.tolist()
+        elif f == 1:                                # Sub: a running sum per byte lane, mod 256
+            pad = (-stride) % bpp
+            lanes = np.concatenate([line, np.zeros(pad, np.uint8)]).reshape(-1, bpp).astype(np.int64)
+            cur = (np.cumsum(lanes, 0) & 255).reshape(-1)[:stride].tolist()
+        elif f == 2:
+            cur = ((line.astype(np.int32) + np.asarray(prev, np.int32)) & 255).tolist()
+        elif f in (3, 4):
+            ln = line.tolist()
+            cur = [0] * stride
+            for x in range(stride):                 # depends on the reconstructed left neighbour
+                a = cur[x - bpp] if x >= bpp else 0
+                b = prev[x]
+                if f == 3:
+                    p = (a + b) >> 1
+                else:
+                    c = prev[x - bpp] if x >= bpp else 0
+                    pa, pb, pc = abs(b - c), abs(a - c), abs(a + b - 2 * c)
+                    p = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+                cur[x] = (ln[x] + p) & 255
+        else:
+            raise ValueError(f"PNG filter type {f}")
+        out[y] = cur
+        prev = cur
+    return out
+
+
 def decode_png(data):
-    """PNG bytes -> (h, w, 4) uint8 RGBA, rows top to bottom."""
+    """PNG bytes -> (h, w, 4) uint8 RGBA, rows top to bottom. Grey, grey +
+    alpha, RGB, RGBA and palette images at every bit depth, not interlaced:
+    16-bit samples keep their high byte, grey below 8 bits is scaled to
+    0..255 and palette indices below 8 bits index the palette (what 8-bit
+    loads of stb_image and FreeImage return); a tRNS chunk gives palette
+    alphas or, for 8-bit grey and RGB, alpha 0 on the key colour."""
     if data[:8] != b"\x89PNG\r\n\x1a\n":
         raise ValueError("not a PNG file")
     pos, idat, plte, trns = 8, [], None, None
@@ -56,58 +98,97 @@ def decode_png(data):
             break
     if w is None:
         raise ValueError("PNG without IHDR")
-    if depth != 8 or interlace != 0:
-        raise ValueError(f"PNG bit depth {depth} / interlace {interlace}: only 8-bit, non-interlaced images are read")
+    if interlace != 0:
+        raise ValueError("interlaced PNG images are not read")
     chans = {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype]
+    bits = depth * chans
+    stride = (w * bits + 7) // 8
     raw = np.frombuffer(zlib.decompress(b"".join(idat)), np.uint8)
-    stride = w * chans
-    rows = raw.reshape(h, stride + 1)
-    out = np.zeros((h, stride), np.uint8)
-    prev = np.zeros(stride, np.int32)
-    for y in range(h):
-        f, line = rows[y, 0], rows[y, 1:].astype(np.int32)
-        if f == 0:
-            cur = line
-        elif f == 2:
-            cur = (line + prev) & 255
-        elif f in (1, 3, 4):
-            cur = np.zeros(stride, np.int32)
-            for x in range(stride):                 # depends on the reconstructed left neighbour
-                a = cur[x - chans] if x >= chans else 0
-                b = prev[x]
-                c = prev[x - chans] if x >= chans else 0
-                if f == 1:
-                    p = a
-                elif f == 3:
-                    p = (a + b) >> 1
-                else:
-                    pa, pb, pc = abs(b - c), abs(a - c), abs(a + b - 2 * c)
-                    p = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
-                cur[x] = (line[x] + p) & 255
-        else:
-            raise ValueError(f"PNG filter type {f}")
-        out[y] = cur
-        prev = cur
-    px = out.reshape(h, w, chans)
+    out = _unfilter(raw[:h * (stride + 1)].reshape(h, stride + 1), max(1, bits // 8))
+    if depth == 8:
+        px = out.reshape(h, w, chans)
+    elif depth == 16:
+        px = out.reshape(h, w, chans, 2)[..., 0]
+    else:
+        per = 8 // depth
+        shifts = np.array([8 - depth * (k + 1) for k in range(per)], np.uint8)
+        v = ((out[:, :, None] >> shifts) & ((1 << depth) - 1)).reshape(h, -1)[:, :w]
+        if ctype == 0:
+            v = v * {1: 255, 2: 85, 4: 17}[depth]
+        px = v.astype(np.uint8)[..., None]
     rgba = np.full((h, w, 4), 255, np.uint8)
     if ctype == 0:
         rgba[..., :3] = px
+        if trns is not None and depth == 8 and len(trns) >= 2:
+            rgba[..., 3] = np.where(px[..., 0] == trns[1], 0, 255)
     elif ctype == 4:
         rgba[..., :3], rgba[..., 3] = px[..., :1], px[..., 1]
     elif ctype == 2:
         rgba[..., :3] = px
+        if trns is not None and depth == 8 and len(trns) >= 6:
+            key = trns[1::2][:3]
+            rgba[..., 3] = np.where((px == key).all(-1), 0, 255)
     elif ctype == 6:
         rgba[:] = px
     else:
         if plte is None:
             raise ValueError("palette PNG without PLTE")
         idx = px[..., 0]
+        if idx.max(initial=0) >= len(plte):
+            raise ValueError("PNG palette index out of range")
         rgba[..., :3] = plte[idx]
         if trns is not None:
             alpha = np.full(256, 255, np.uint8)
             alpha[:len(trns)] = trns
             rgba[..., 3] = alpha[idx]
     return rgba
+
+
+def decode_tga(data):
+    """TGA bytes -> (h, w, 4) uint8 RGBA, rows top to bottom: true-colour
+    (24/32-bit BGR(A)) and grey (8-bit) images, uncompressed or run-length
+    encoded, either origin."""
+    if len(data) < 18:
+        raise ValueError("not a TGA file")
+    idlen, cmtype, itype = data[0], data[1], data[2]
+    cmlen, cmdepth = struct.unpack("<H", data[5:7])[0], data[7]
+    w, h = struct.unpack("<HH", data[12:16])
+    depth, desc = data[16], data[17]
+    if itype not in (2, 3, 10, 11) or (itype in (2, 10) and depth not in (24, 32)) or (itype in (3, 11) and depth != 8):
+        raise ValueError(f"TGA image type {itype} / {depth}-bit: only true-colour 24/32-bit and 8-bit grey are read")
+    bpp = depth // 8
+    pos = 18 + idlen + (cmlen * ((cmdepth + 7) // 8) if cmtype else 0)
+    n = w * h * bpp
+    if itype in (2, 3):
+        pix = np.frombuffer(data[pos:pos + n], np.uint8)
+    else:
+        buf = bytearray()
+        while len(buf) < n:
+            head = data[pos]
+            pos += 1
+            count = (head & 127) + 1
+            if head & 128:
+                buf += data[pos:pos + bpp] * count
+                pos += bpp
+            else:
+                buf += data[pos:pos + count * bpp]
+                pos += count * bpp
+        pix = np.frombuffer(bytes(buf[:n]), np.uint8)
+    if len(pix) < n:
+        raise ValueError("TGA pixel data is truncated")
+    pix = pix.reshape(h, w, bpp)
+    rgba = np.full((h, w, 4), 255, np.uint8)
+    if bpp == 1:
+        rgba[..., :3] = pix
+    else:
+        rgba[..., 0], rgba[..., 1], rgba[..., 2] = pix[..., 2], pix[..., 1], pix[..., 0]
+        if bpp == 4:
+            rgba[..., 3] = pix[..., 3]
+    if not desc & 0x20:                             # bottom-left origin: first row stored is the bottom one
+        rgba = rgba[::-1]
+    if desc & 0x10:
+        rgba = rgba[:, ::-1]
+    return np.ascontiguousarray(rgba)
 
 
 TEX_NORMALMAP, TEX_LDR = 2, 4                         # HostTexture flags (host_texture.h:39-44)
@@ -143,11 +224,38 @@ def mark_normal_map(tex):
     tex["storage"] = F.NRM32
 
 
+def decode_image(data, name=""):
+    """PNG or TGA bytes (by signature, then by file name) -> RGBA, top row first."""
+    if data[:8] == b"\x89PNG\r\n\x1a\n":
+        return decode_png(data)
+    if name.lower().endswith(".tga"):
+        return decode_tga(data)
+    raise ValueError(f"{name or 'image'}: only PNG and TGA images are read")
+
+
+def resolve_path(path):
+    """The file at `path`, with backslashes read as separators and, when the
+    exact name is absent, a case-insensitive match per path component (scene
+    files written on Windows name their textures that way)."""
+    path = path.replace("\\", "/")
+    if os.path.exists(path):
+        return path
+    parts = os.path.abspath(path).split("/")
+    cur = "/"
+    for part in parts[1:]:
+        nxt = os.path.join(cur, part)
+        if not os.path.exists(nxt) and os.path.isdir(cur):
+            match = [e for e in os.listdir(cur) if e.lower() == part.lower()]
+            if match:
+                nxt = os.path.join(cur, match[0])
+        cur = nxt
+    return cur
+
+
 def load_texture(path, flipped=False, linearized=False, normal_map=False):
-    if not path.lower().endswith(".png"):
-        raise ValueError(f"{path}: only PNG textures are read")
+    path = resolve_path(path)
     with open(path, "rb") as fh:
-        return texture_from_rgba(decode_png(fh.read()), flipped, linearized, normal_map)
+        return texture_from_rgba(decode_image(fh.read(), path), flipped, linearized, normal_map)
 
 
 # ---- OBJ / MTL ----------------------------------------------------------------
@@ -352,7 +460,7 @@ def load_obj(scene, path, transform=None, flat_shaded=False):
         obj = parse_obj(fh.read())
     mtls = []
     for lib in obj["mtllib"]:
-        with open(os.path.join(directory, lib), "r") as fh:
+        with open(resolve_path(os.path.join(directory, lib)), "r") as fh:
             mtls += parse_mtl(fh.read())
     offset = len(scene.materials)
     by_name, sizes, cache = {}, {}, {}
@@ -414,7 +522,7 @@ def read_gltf(path):
         if uri.startswith("data:"):
             return base64.b64decode(uri.split(",", 1)[1])
         from urllib.parse import unquote
-        with open(os.path.join(directory, unquote(uri)), "rb") as fh:
+        with open(resolve_path(os.path.join(directory, unquote(uri))), "rb") as fh:
             return fh.read()
     bufs = []
     for b in doc.get("buffers", []):
@@ -630,7 +738,7 @@ def _gltf_image_rgba(doc, bufs, image):
         data = doc["_fetch"](image["uri"])
     else:
         raise ValueError("glTF image without data")
-    return decode_png(data)
+    return decode_image(data, image.get("uri", image.get("name", "")))
 
 
 def gltf_node_transform(node):

@@ -175,3 +175,85 @@ def test_loaded_scene_renders_in_the_oracle(tmp_path):
     view = scenes.camera_view((0.5, 0.5, 2.0), (0.5, 0.5, 0.0), 60.0, 16, 16)
     r32, r64, n, _, _ = orc.render7(s, view, 16, 16, 1, 1, threads=4, bsdf=0)
     assert np.isfinite(r64).all() and r64[..., :3].sum() > 0
+
+
+def png_raw(w, h, depth, ctype, rows, plte=None, trns=None):
+    """PNG of already packed rows (filter type 0 each)."""
+    chunk = lambda k, d: struct.pack(">I", len(d)) + k + d + struct.pack(">I", zlib.crc32(k + d) & 0xffffffff)
+    body = b"".join(b"\x00" + r for r in rows)
+    out = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 0))
+    if plte is not None:
+        out += chunk(b"PLTE", bytes(plte))
+    if trns is not None:
+        out += chunk(b"tRNS", bytes(trns))
+    return out + chunk(b"IDAT", zlib.compress(body)) + chunk(b"IEND", b"")
+
+
+@pytest.mark.parametrize("depth", [1, 2, 4])
+def test_png_low_bit_depths(depth):
+    rng = np.random.default_rng(depth)
+    w, h = 13, 5
+    v = rng.integers(0, 1 << depth, (h, w))
+    per = 8 // depth
+
+    def pack(row):
+        padded = np.concatenate([row, np.zeros((-len(row)) % per, np.int64)]).reshape(-1, per)
+        return bytes(int(sum(int(x) << (8 - depth * (k + 1)) for k, x in enumerate(g))) for g in padded)
+    rows = [pack(v[y]) for y in range(h)]
+    grey = L.decode_png(png_raw(w, h, depth, 0, rows))
+    scale = {1: 255, 2: 85, 4: 17}[depth]
+    assert np.array_equal(grey[..., 0], v * scale) and np.all(grey[..., 3] == 255)
+    plte = rng.integers(0, 256, (1 << depth, 3)).astype(np.uint8)
+    trns = [7, 200]
+    pal = L.decode_png(png_raw(w, h, depth, 3, rows, plte.tobytes(), trns))
+    assert np.array_equal(pal[..., :3], plte[v])
+    assert np.array_equal(pal[..., 3], np.where(v == 0, 7, np.where(v == 1, 200, 255)))
+
+
+def test_png_16_bit_keeps_high_byte_and_trns_key():
+    rng = np.random.default_rng(16)
+    w, h = 6, 4
+    v = rng.integers(0, 65536, (h, w, 3)).astype(">u2")
+    rgb = L.decode_png(png_raw(w, h, 16, 2, [v[y].tobytes() for y in range(h)]))
+    assert np.array_equal(rgb[..., :3], (v.astype(np.int64) >> 8).astype(np.uint8))
+    img = rng.integers(0, 256, (h, w, 3)).astype(np.uint8)
+    img[1, 2] = (10, 20, 30)
+    keyed = L.decode_png(png_raw(w, h, 8, 2, [img[y].tobytes() for y in range(h)], trns=[0, 10, 0, 20, 0, 30]))
+    assert keyed[1, 2, 3] == 0 and (keyed[..., 3] == 0).sum() == (img == (10, 20, 30)).all(-1).sum()
+
+
+def tga(img, rle=False, top_left=False):
+    """Test-side TGA encoder: (h, w, c) uint8 RGB(A) or grey (c = 1)."""
+    h, w, c = img.shape
+    px = np.concatenate([img[..., 2::-1], img[..., 3:]], -1) if c >= 3 else img     # BGR(A)
+    rows = px if top_left else px[::-1]
+    flat = rows.reshape(-1, c)
+    if rle:
+        data, i = bytearray(), 0
+        while i < len(flat):
+            j = i
+            while j + 1 < len(flat) and j - i < 127 and np.array_equal(flat[j + 1], flat[i]):
+                j += 1
+            if j > i:
+                data += bytes([128 | (j - i)]) + flat[i].tobytes()
+            else:
+                data += bytes([0]) + flat[i].tobytes()
+            i = j + 1
+    else:
+        data = flat.tobytes()
+    itype = (3 if c == 1 else 2) + (8 if rle else 0)
+    head = struct.pack("<BBBHHBHHHHBB", 0, 0, itype, 0, 0, 0, 0, 0, w, h, 8 * c, (0x20 if top_left else 0) | (8 if c == 4 else 0))
+    return head + bytes(data)
+
+
+@pytest.mark.parametrize("c,rle,top", [(3, False, False), (4, True, True), (1, True, False), (4, False, True)])
+def test_tga_decoder(c, rle, top):
+    rng = np.random.default_rng(c + 2 * rle)
+    img = rng.integers(0, 4, (7, 9, c)).astype(np.uint8) * 60          # few values: RLE runs occur
+    rgba = L.decode_tga(tga(img, rle, top))
+    want = np.full((7, 9, 4), 255, np.uint8)
+    if c == 1:
+        want[..., :3] = img
+    else:
+        want[..., :c] = img
+    assert np.array_equal(rgba, want)
