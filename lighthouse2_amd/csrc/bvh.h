@@ -71,6 +71,14 @@ hipError_t gpu_build(const float4* dLo, const float4* dHi, int n, int maxLeaf, i
 	DeviceBuild& out, hipStream_t stream);
 void free_build(DeviceBuild& b);
 
+/* Refit on the device, topology unchanged. Boxes of the build nodes listed in
+ * ids[first .. first+count) (one tree level: leaves take the min / max of their
+ * triangles' vertices, interior nodes the union of their two children, which
+ * must already be refitted); then the laid-out f32 / q8 nodes are rewritten
+ * from the refitted build boxes through the layout maps (refs unchanged). */
+hipError_t refit_level(BuildNode* dNodes, const uint32_t* dPrims, const float4* dVerts, const int32_t* dIds, int first, int count, hipStream_t stream);
+hipError_t refit_layout(const BuildNode* dNodes, NodeF32* dF32, const int32_t* dF32Kids, int nF32, NodeQ8* dQ8, const int32_t* dQ8Map, int nQ8, hipStream_t stream);
+
 /* Triangle soup (3 float4 per triangle) -> primitive boxes. */
 hipError_t tri_bounds(const float4* dVerts, int n, float4* dLo, float4* dHi, hipStream_t stream);
 
@@ -83,6 +91,10 @@ struct Layout
 	std::vector<NodeQ8> q8;
 	std::vector<uint32_t> order;
 	std::vector<uint8_t> last;
+	/* build-tree nodes behind each laid-out node, for refits that keep the
+	 * topology: f32Kids[2i..2i+1] = children of f32 node i; q8Map[5i] = the
+	 * node itself, q8Map[5i+1..5i+4] = its children (-1 past the last) */
+	std::vector<int32_t> f32Kids, q8Map;
 	uint32_t rootF32 = 0, rootQ8 = 0;
 	int depthF32 = 0, depthQ8 = 0, stackF32 = 0, stackQ8 = 0;
 };

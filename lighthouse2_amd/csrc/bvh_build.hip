@@ -440,6 +440,113 @@ void free_build(DeviceBuild& b)
 	b.nodes = nullptr, b.prims = nullptr, b.nodeCount = 0;
 }
 
+/* std::min / std::max of the host refit and layout: (b < a) ? b : a and
+ * (a < b) ? b : a, so a NaN operand resolves the same way on both sides. */
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+
+__global__ void k_refit_level(BuildNode* __restrict__ nodes, const uint32_t* __restrict__ prims, const float4* __restrict__ v,
+	const int32_t* __restrict__ ids, int first, int count)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= count) return;
+	const int x = ids[first + i];
+	BuildNode b = nodes[x];
+	float lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+	if (b.count > 0)
+		for (int k = 0; k < b.count; k++)
+		{
+			const float4* t = v + 3 * (size_t)prims[b.leftFirst + k];
+			const float4 p0 = t[0], p1 = t[1], p2 = t[2];
+			const float a0[3] = { p0.x, p0.y, p0.z }, a1[3] = { p1.x, p1.y, p1.z }, a2[3] = { p2.x, p2.y, p2.z };
+			for (int c = 0; c < 3; c++)
+				lo[c] = smin(lo[c], smin(smin(a0[c], a1[c]), a2[c])), hi[c] = smax(hi[c], smax(smax(a0[c], a1[c]), a2[c]));
+		}
+	else
+		for (int k = 0; k < 2; k++)
+		{
+			const BuildNode ch = nodes[b.leftFirst + k];
+			for (int c = 0; c < 3; c++) lo[c] = smin(lo[c], ch.lo[c]), hi[c] = smax(hi[c], ch.hi[c]);
+		}
+	for (int c = 0; c < 3; c++) nodes[x].lo[c] = lo[c], nodes[x].hi[c] = hi[c];
+}
+
+hipError_t refit_level(BuildNode* dNodes, const uint32_t* dPrims, const float4* dVerts, const int32_t* dIds, int first, int count, hipStream_t stream)
+{
+	if (count <= 0) return hipSuccess;
+	k_refit_level<<<(count + 255) / 256, 256, 0, stream>>>(dNodes, dPrims, dVerts, dIds, first, count);
+	return hipGetLastError();
+}
+
+/* The host layout's quantise_axis: the smallest power-of-two step from
+ * ceil(log2(extent / 255)) up for which every child's lo / hi rounds outward
+ * to a byte offset from the parent's lo. The step search ends by exponent 128
+ * (an infinite step makes every test false, i.e. accepted). */
+__device__ uint32_t quantise_axis_dev(float lo, float hi, const float* clo, const float* chi, int cnt, uint32_t* qlo, uint32_t* qhi)
+{
+	const float ext = hi - lo;
+	int e = -126;
+	if (ext > 0) e = max(-126, (int)ceil(log2((double)ext / 255.0)));
+	for (; e <= 128; e++)
+	{
+		const float s = __uint_as_float((uint32_t)(e + 127) << 23);
+		bool ok = true;
+		for (int c = 0; c < cnt && ok; c++)
+		{
+			const double ql = floor(((double)clo[c] - (double)lo) / (double)s);
+			const double qlc = (ql < 255.0) ? ql : 255.0;
+			int a = (int)((0.0 < qlc) ? qlc : 0.0);
+			while (a > 0 && __fadd_rn(lo, __fmul_rn((float)a, s)) > clo[c]) a--;
+			const double qh = ceil(((double)chi[c] - (double)lo) / (double)s);
+			const double qhc = (qh < 255.0) ? qh : 255.0;
+			int b = (int)((0.0 < qhc) ? qhc : 0.0);
+			while (b < 255 && __fadd_rn(lo, __fmul_rn((float)b, s)) < chi[c]) b++;
+			if (__fadd_rn(lo, __fmul_rn((float)b, s)) < chi[c] || __fadd_rn(lo, __fmul_rn((float)a, s)) > clo[c]) { ok = false; break; }
+			qlo[c] = (uint32_t)a, qhi[c] = (uint32_t)b;
+		}
+		if (ok) return (uint32_t)(e + 127);
+	}
+	return 255u;
+}
+
+__global__ void k_refit_f32(const BuildNode* __restrict__ nodes, NodeF32* __restrict__ out, const int32_t* __restrict__ kids, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const BuildNode L = nodes[kids[2 * i]], R = nodes[kids[2 * i + 1]];
+	out[i].lxy = make_float4(L.lo[0], L.hi[0], L.lo[1], L.hi[1]);
+	out[i].rxy = make_float4(R.lo[0], R.hi[0], R.lo[1], R.hi[1]);
+	out[i].lrz = make_float4(L.lo[2], L.hi[2], R.lo[2], R.hi[2]);
+}
+
+__global__ void k_refit_q8(const BuildNode* __restrict__ nodes, NodeQ8* __restrict__ out, const int32_t* __restrict__ map, int n)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const BuildNode B = nodes[map[5 * i]];
+	int cnt = 0;
+	int kid[4];
+	for (int c = 0; c < 4; c++) { kid[c] = map[5 * i + 1 + c]; if (kid[c] >= 0) cnt = c + 1; }
+	uint32_t ex[3], q[6][4] = {};
+	for (int a = 0; a < 3; a++)
+	{
+		float clo[4], chi[4];
+		for (int c = 0; c < cnt; c++) { const BuildNode& K = nodes[kid[c]]; clo[c] = K.lo[a], chi[c] = K.hi[a]; }
+		ex[a] = quantise_axis_dev(B.lo[a], B.hi[a], clo, chi, cnt, q[a], q[3 + a]);
+	}
+	auto pack = [](const uint32_t* v) { return v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24); };
+	out[i].origin = make_float4(B.lo[0], B.lo[1], B.lo[2], __uint_as_float(ex[0] | (ex[1] << 8) | (ex[2] << 16) | ((uint32_t)cnt << 24)));
+	out[i].qa = make_uint4(pack(q[0]), pack(q[1]), pack(q[2]), pack(q[3]));
+	out[i].qb = make_uint4(pack(q[4]), pack(q[5]), 0, 0);
+}
+
+hipError_t refit_layout(const BuildNode* dNodes, NodeF32* dF32, const int32_t* dF32Kids, int nF32, NodeQ8* dQ8, const int32_t* dQ8Map, int nQ8, hipStream_t stream)
+{
+	if (nF32 > 0) k_refit_f32<<<(nF32 + 255) / 256, 256, 0, stream>>>(dNodes, dF32, dF32Kids, nF32);
+	if (nQ8 > 0) k_refit_q8<<<(nQ8 + 127) / 128, 128, 0, stream>>>(dNodes, dQ8, dQ8Map, nQ8);
+	return hipGetLastError();
+}
+
 __global__ void k_gather_tris(const float4* __restrict__ v, const uint32_t* __restrict__ order, const uint8_t* __restrict__ last, int n, float4* __restrict__ tris)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;

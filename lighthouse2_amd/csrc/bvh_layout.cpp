@@ -76,6 +76,7 @@ struct Walker
 		std::vector<Item> st;
 		const uint32_t root = (uint32_t)out.f32.size();
 		out.f32.push_back(NodeF32());
+		out.f32Kids.push_back(0), out.f32Kids.push_back(0);
 		st.push_back({ x, root });
 		while (!st.empty())
 		{
@@ -96,11 +97,13 @@ struct Walker
 				if (n[kids[c]].count > 0) { refs[c] = ref_leaf(kids[c]); continue; }
 				const uint32_t s = (uint32_t)out.f32.size();
 				out.f32.push_back(NodeF32());
+				out.f32Kids.push_back(0), out.f32Kids.push_back(0);
 				refs[c] = s;
 				pend[np++] = { kids[c], s };
 			}
 			nd.refs = make_uint4(refs[0], refs[1], 0, 0);
 			out.f32[it.slot] = nd;
+			out.f32Kids[2 * it.slot] = kids[0], out.f32Kids[2 * it.slot + 1] = kids[1];
 			for (int k = np - 1; k >= 0; k--) st.push_back(pend[k]);
 		}
 		return root;
@@ -161,6 +164,7 @@ struct Walker
 		std::vector<Item> st;
 		const uint32_t root = (uint32_t)out.q8.size();
 		out.q8.push_back(NodeQ8());
+		out.q8Map.resize(out.q8Map.size() + 5, -1);
 		st.push_back({ x, root });
 		std::vector<int> kids;
 		while (!st.empty())
@@ -190,11 +194,14 @@ struct Walker
 				if (n[kids[c]].count > 0) { refs[c] = ref_leaf(kids[c]); continue; }
 				const uint32_t s = (uint32_t)out.q8.size();
 				out.q8.push_back(NodeQ8());
+				out.q8Map.resize(out.q8Map.size() + 5, -1);
 				refs[c] = s;
 				pend.push_back({ kids[c], s });
 			}
 			nd.refs = make_uint4(refs[0], refs[1], refs[2], refs[3]);
 			out.q8[it.slot] = nd;
+			out.q8Map[5 * it.slot] = it.node;
+			for (int c = 0; c < cnt; c++) out.q8Map[5 * it.slot + 1 + c] = kids[c];
 			for (int k = (int)pend.size() - 1; k >= 0; k--) st.push_back(pend[k]);
 		}
 		return root;

@@ -312,41 +312,98 @@ void RenderCore::buildMesh(MeshData& m)
 	stats.bvhBuildTime = m.buildMs * 0.001f;
 }
 
-/* The build tree's nodes are stored level by level (parents before children),
- * so one reverse sweep recomputes every box from its leaves up: a leaf box
- * is the min / max of its triangles' vertices (as tri_bounds computes them),
- * an interior box the union of its two children. */
+/* Refit keeps the build tree's topology and recomputes its boxes from the new
+ * vertices. On the device (once the mesh has been laid out with the refit
+ * setting on): one launch per tree level from the deepest up (a leaf box is
+ * the min / max of its triangles' vertices, as tri_bounds computes them, an
+ * interior box the union of its two children), then the f32 and q8 nodes are
+ * rewritten from the refitted boxes through the layout maps and the leaf
+ * triangles gathered again. Otherwise the same sweep runs on the host copy
+ * (children follow parents in the build array) and the mesh is laid out anew. */
 void RenderCore::refitMesh(MeshData& m, const lighthouse2::float4* v)
 {
 	const auto t0 = std::chrono::high_resolution_clock::now();
-	std::vector<BuildNode>& nodes = m.buildNodes;
-	for (int i = (int)nodes.size() - 1; i >= 0; i--)
+	if (m.deviceRefit)
 	{
-		BuildNode& b = nodes[i];
-		float lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
-		if (b.count > 0)
-			for (int k = 0; k < b.count; k++)
-			{
-				const lighthouse2::float4* t = v + 3 * (size_t)m.buildPrims[b.leftFirst + k];
-				for (int c = 0; c < 3; c++)
-				{
-					const float x0 = (&t[0].x)[c], x1 = (&t[1].x)[c], x2 = (&t[2].x)[c];
-					lo[c] = std::min(lo[c], std::min(std::min(x0, x1), x2));
-					hi[c] = std::max(hi[c], std::max(std::max(x0, x1), x2));
-				}
-			}
-		else
-			for (int k = 0; k < 2; k++)
-			{
-				const BuildNode& ch = nodes[b.leftFirst + k];
-				for (int c = 0; c < 3; c++) lo[c] = std::min(lo[c], ch.lo[c]), hi[c] = std::max(hi[c], ch.hi[c]);
-			}
-		for (int c = 0; c < 3; c++) b.lo[c] = lo[c], b.hi[c] = hi[c];
+		for (int d = (int)m.levelStart.size() - 2; d >= 0; d--)
+			hipCheck(refit_level(m.dBuildNodes.as<BuildNode>(), m.dBuildPrims.as<uint32_t>(), m.verts.as<float4>(), m.dLevelIds.as<int32_t>(),
+				m.levelStart[d], m.levelStart[d + 1] - m.levelStart[d], stream), "refit_level");
+		hipCheck(refit_layout(m.dBuildNodes.as<BuildNode>(), m.nodesF32.as<NodeF32>(), m.dF32Kids.as<int32_t>(), (int)m.nodeCountF32,
+			m.nodesQ8.as<NodeQ8>(), m.dQ8Map.as<int32_t>(), (int)m.nodeCountQ8, stream), "refit_layout");
+		hipCheck(gather_tris(m.verts.as<float4>(), m.dOrder.as<uint32_t>(), m.dLast.as<uint8_t>(), m.triCount, m.tris.as<float4>(), stream), "gather_tris");
+		BuildNode root;
+		hipCheck(hipMemcpyAsync(&root, m.dBuildNodes.p, sizeof(BuildNode), hipMemcpyDeviceToHost, stream), "D2H root");
+		hipCheck(hipStreamSynchronize(stream), "sync");
+		for (int c = 0; c < 3; c++) m.lo[c] = root.lo[c], m.hi[c] = root.hi[c];
 	}
-	layoutMesh(m);
+	else
+	{
+		std::vector<BuildNode>& nodes = m.buildNodes;
+		for (int i = (int)nodes.size() - 1; i >= 0; i--)
+		{
+			BuildNode& b = nodes[i];
+			float lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+			if (b.count > 0)
+				for (int k = 0; k < b.count; k++)
+				{
+					const lighthouse2::float4* t = v + 3 * (size_t)m.buildPrims[b.leftFirst + k];
+					for (int c = 0; c < 3; c++)
+					{
+						const float x0 = (&t[0].x)[c], x1 = (&t[1].x)[c], x2 = (&t[2].x)[c];
+						lo[c] = std::min(lo[c], std::min(std::min(x0, x1), x2));
+						hi[c] = std::max(hi[c], std::max(std::max(x0, x1), x2));
+					}
+				}
+			else
+				for (int k = 0; k < 2; k++)
+				{
+					const BuildNode& ch = nodes[b.leftFirst + k];
+					for (int c = 0; c < 3; c++) lo[c] = std::min(lo[c], ch.lo[c]), hi[c] = std::max(hi[c], ch.hi[c]);
+				}
+			for (int c = 0; c < 3; c++) b.lo[c] = lo[c], b.hi[c] = hi[c];
+		}
+		layoutMesh(m);
+	}
 	const auto t1 = std::chrono::high_resolution_clock::now();
 	m.buildMs = std::chrono::duration<float, std::milli>(t1 - t0).count();
 	stats.bvhBuildTime = m.buildMs * 0.001f;
+}
+
+/* Device copies of the build tree, its nodes grouped by depth (children lie
+ * one level below their parent) and the layout maps, for refit. */
+void RenderCore::prepareDeviceRefit(MeshData& m, const Layout& L)
+{
+	const std::vector<BuildNode>& nodes = m.buildNodes;
+	const int n = (int)nodes.size();
+	std::vector<int> depth(n, -1);
+	depth[0] = 0;
+	int maxDepth = 0;
+	for (int i = 0; i < n; i++)
+	{
+		if (depth[i] < 0) throw CoreError(LH2HIP_ERR_STATE, "refit: build node before its parent");
+		maxDepth = std::max(maxDepth, depth[i]);
+		if (nodes[i].count == 0)
+			for (int k = 0; k < 2; k++)
+			{
+				const int c = nodes[i].leftFirst + k;
+				if (c <= i || c >= n) throw CoreError(LH2HIP_ERR_STATE, "refit: child index out of order");
+				depth[c] = depth[i] + 1;
+			}
+	}
+	m.levelStart.assign(maxDepth + 2, 0);
+	for (int i = 0; i < n; i++) m.levelStart[depth[i] + 1]++;
+	for (int d = 0; d <= maxDepth; d++) m.levelStart[d + 1] += m.levelStart[d];
+	std::vector<int32_t> ids(n);
+	std::vector<int> fill(m.levelStart.begin(), m.levelStart.end() - 1);
+	for (int i = 0; i < n; i++) ids[fill[depth[i]]++] = i;
+	m.dBuildNodes.upload(nodes.data(), sizeof(BuildNode) * n, stream);
+	m.dBuildPrims.upload(m.buildPrims.data(), sizeof(uint32_t) * m.buildPrims.size(), stream);
+	m.dLevelIds.upload(ids.data(), sizeof(int32_t) * n, stream);
+	m.dF32Kids.upload(L.f32Kids.data(), sizeof(int32_t) * L.f32Kids.size(), stream);
+	m.dQ8Map.upload(L.q8Map.data(), sizeof(int32_t) * L.q8Map.size(), stream);
+	m.dOrder.upload(L.order.data(), sizeof(uint32_t) * L.order.size(), stream);
+	m.dLast.upload(L.last.data(), L.last.size(), stream);
+	m.deviceRefit = true;
 }
 
 void RenderCore::layoutMesh(MeshData& m)
@@ -367,6 +424,10 @@ void RenderCore::layoutMesh(MeshData& m)
 	last.upload(L.last.data(), L.last.size(), stream);
 	m.tris.ensure(sizeof(float4) * 3 * (size_t)n);
 	hipCheck(gather_tris(m.verts.as<float4>(), order.as<uint32_t>(), last.as<uint8_t>(), n, m.tris.as<float4>(), stream), "gather_tris");
+	m.deviceRefit = false;
+	m.dBuildNodes.release(), m.dBuildPrims.release(), m.dLevelIds.release(), m.dF32Kids.release(), m.dQ8Map.release();
+	m.dOrder.release(), m.dLast.release();
+	if (refitSetting) prepareDeviceRefit(m, L);
 	hipCheck(hipStreamSynchronize(stream), "sync");
 }
 

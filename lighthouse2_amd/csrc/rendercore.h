@@ -73,8 +73,13 @@ struct MeshData
 	int depthF32 = 0, depthQ8 = 0, stackF32 = 0, stackQ8 = 0, buildDepth = 0;
 	size_t nodeCountF32 = 0, nodeCountQ8 = 0, leafCount = 0;
 	float lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
-	std::vector<BuildNode> buildNodes;   /* canonical build tree (host copy) */
+	std::vector<BuildNode> buildNodes;   /* canonical build tree (host copy; stale after a device refit) */
 	std::vector<uint32_t> buildPrims;
+	/* device refit state (kept while the refit setting is on): the build tree,
+	 * its nodes grouped by depth, the layout maps and the leaf slot order */
+	DevBuf dBuildNodes, dBuildPrims, dLevelIds, dF32Kids, dQ8Map, dOrder, dLast;
+	std::vector<int> levelStart;         /* ids of depth d: [levelStart[d], levelStart[d + 1]) */
+	bool deviceRefit = false;
 	float buildMs = 0;
 	int maxMaterial = -1;                /* largest CoreTri::material, -1 without CoreTri data */
 };
@@ -161,6 +166,7 @@ private:
 	 * bottom-up from the new vertices (host copy), then laid out again. */
 	void refitMesh(MeshData& m, const lighthouse2::float4* vertexData);
 	void layoutMesh(MeshData& m);
+	void prepareDeviceRefit(MeshData& m, const Layout& L);
 	void ensureTraceBuffers();
 	void updateTopLevel();
 

@@ -1,5 +1,7 @@
 """Animation refit ("refit" setting): SetGeometry with the same triangle count
-keeps the mesh's build tree and recomputes its boxes from the new vertices.
+keeps the mesh's build tree and recomputes its boxes from the new vertices
+(on the device, level by level, then the laid-out nodes from the refitted
+boxes; on the host copy when the setting was turned on after the build).
 Traversal must still return exactly the oracle's hits on the moved geometry
 (boxes only prune; the watertight test decides), for both node formats, and
 a render of the moved scene must equal the oracle's render of it. Refitting
@@ -87,3 +89,29 @@ def test_refit_render_matches_oracle(torch_cuda):
     s.meshes[sphere] = (v2, t2)
     r32, r64, n = orc.render(s, view, 48, 48, 1, 0, max_depth=3, threads=16)
     assert np.array_equal(acc, r64)
+
+
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_refit_enabled_after_the_build(torch_cuda, fmt):
+    """The refit setting turned on after the mesh was built: the first refit
+    runs on the host copy of the build tree and lays the mesh out again, the
+    following ones on the device; hits equal the oracle's after each."""
+    torch = torch_cuda
+    s = scenes.random_triangle_scene(6000, seed=5)
+    core = lh.RenderCore()
+    core.Setting("nodeFormat", fmt)
+    lh.load_scene(core, s)
+    core.Setting("refit", 1)
+    rays = workload_rays(orc.OracleScene(s), 3)
+    for k, amount in enumerate([0.01, 0.03, 0.005]):
+        s2 = moved(s, 20 + k, amount)
+        core.SetGeometry(0, *s2.meshes[0])
+        core.FinalizeInstances()
+        ms = float(core.GetCoreStats()["bvhBuildTime"]) * 1e3
+        o = orc.OracleScene(s2)
+        got = gpu_closest(torch, core, rays)
+        assert np.array_equal(got.view(np.uint32), o.trace(rays).view(np.uint32)), k
+        sub = rays[::4]
+        assert np.array_equal(gpu_any(torch, core, sub), o.trace(sub, any_hit=True)), k
+        print(f"fmt {fmt}, refit {k} ({'host' if k == 0 else 'device'}): {ms:.3f} ms")
+    core.Shutdown()
