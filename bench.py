@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--chunk", type=int, default=32, help="rays a wave reserves per atomic")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--shadow-lane", type=int, default=1, choices=[0, 1],
+                    help="frame: trace shadow rays on a second stream beside the next extension wave")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent traversal blocks per CU (0: core default)")
     ap.add_argument("--min-rays-per-lane", type=float, default=0.0, help="shrink the grid of small waves (0: off)")
     ap.add_argument("--workload", choices=["extend", "denoise", "svgf", "serve", "frame", "learned"], default="extend",
@@ -188,6 +190,7 @@ def bench_frame(a, np, torch, lh, scenes):
     W, H = 1920, 1080
     core.SetTarget(W, H, 1)
     core.Setting("estimator", 2)
+    core.Setting("shadowLane", a.shadow_lane)
     view = scenes.instanced_view(W, H)
     core.Render(view, lh.LH2_RESTART)
     core.WaitForRender()

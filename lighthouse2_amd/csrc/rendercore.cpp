@@ -177,20 +177,29 @@ void RenderCore::Init()
 	int clockKHz = 0;
 	if (hipDeviceGetAttribute(&clockKHz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && clockKHz > 0) wallClockNs = 1e6 / clockKHz;
 	overflow.ensure(trace_overflow_bytes(persistentBlocks));
+	hipCheck(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking), "hipStreamCreate");
+	for (int i = 0; i < 2 + 2 * SIDE_TIMED; i++)
+		hipCheck(i < 2 ? hipEventCreateWithFlags(&sideEv[i], hipEventDisableTiming) : hipEventCreate(&sideEv[i]), "hipEventCreate");
+	sideCounters.ensure((4 + TRACE_QUEUE_WORDS) * sizeof(uint32_t));
+	overflow2.ensure(trace_overflow_bytes(persistentBlocks));
 }
 
 void RenderCore::Shutdown()
 {
 	if (!stream) return;
 	(void)hipStreamSynchronize(stream);
+	(void)hipStreamSynchronize(stream2);
 	meshes.clear();
 	tlasNodes.release(), instRecs.release(), shadeInsts.release(), overflow.release(), counters.release();
+	overflow2.release(), sideCounters.release();
 	materials.release(), triLights.release(), pointLights.release(), spotLights.release(), dirLights.release();
 	wf.release();
 	svgf.reset();
 	for (int i = 0; i < 8; i++) (void)hipEventDestroy(ev[i]);
+	for (int i = 0; i < 2 + 2 * SIDE_TIMED; i++) (void)hipEventDestroy(sideEv[i]);
+	(void)hipStreamDestroy(stream2);
 	(void)hipStreamDestroy(stream);
-	stream = nullptr;
+	stream = stream2 = nullptr;
 }
 
 void RenderCore::SetProbePos(const int2 pos) { probePos = pos; }
@@ -256,6 +265,11 @@ void RenderCore::Setting(const char* name, float value)
 		triBias = (int)value;
 		scene.triBias = triBias;
 	}
+	else if (!strcmp(name, "shadowLane"))
+	{
+		if (value != 0 && value != 1) throw CoreError(LH2HIP_ERR_INVALID, "Setting shadowLane: 0 or 1");
+		shadowLane = value != 0;
+	}
 	else if (!strcmp(name, "seed")) seedSetting = camSeed = (uint32_t)value; /* replaces camRNGseed's 0x12345678 */
 	else if (!strcmp(name, "estimator"))
 	{	/* 0: RenderCore_PrimeRef's shadeKernel, 1: rendercore_optix7's (Lambert BSDF) */
@@ -284,8 +298,10 @@ void RenderCore::Setting(const char* name, float value)
 	{
 		if (!(value >= 1 && value <= 64)) throw CoreError(LH2HIP_ERR_INVALID, "Setting traceBlocksPerCU: 1..64");
 		hipCheck(hipStreamSynchronize(stream), "sync");
+		hipCheck(hipStreamSynchronize(stream2), "sync");
 		persistentBlocks = deviceCUs * (int)value;
 		overflow.ensure(trace_overflow_bytes(persistentBlocks));
+		overflow2.ensure(trace_overflow_bytes(persistentBlocks));
 	}
 	/* unknown keys are ignored, as in the reference cores */
 }
@@ -922,7 +938,7 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
  * the pass size nor on how the samples are split between calls: a path's
  * random numbers are keyed by its index within the frame. */
 int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
-	uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing)
+	uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing, const SideLane* side)
 {
 	FrameSetup f = f0;
 	const TraceScene& sc = scene;
@@ -938,6 +954,14 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 		for (int k = 1; k <= f.maxDepth; k++) R0wave[k] = xor32_host(cs) + (uint32_t)k * 91771u;
 	}
 	int deepest = 0;
+	bool sidePending = false;
+	int sideTimed = 0;
+	/* the main stream waits for the shadow lane before anything that writes
+	 * what the lane reads (shade: shadow rays, potentials, path slots) or
+	 * reads what it writes (resolve: path slots) */
+	const auto joinSide = [&]() {
+		if (sidePending) hipCheck(hipStreamWaitEvent(st, side->done, 0), "wait"), sidePending = false;
+	};
 	const int passSamples = std::max(1, std::min(s1 - s0, (int)(std::min(passCap, Wavefront::PASS_CAP) / (size_t)pixels)));
 	w.resizePaths((size_t)passSamples * pixels, st);
 	for (int a = s0; a < s1; a += passSamples)
@@ -957,6 +981,7 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 			hipCheck(launch_trace_dev(sc, MODE_CLOSEST, B.rays[in], B.hits, n, B.counters + 0, queue, ovf, persistentBlocks, st), "extend");
 			if (evs) hipCheck(hipEventRecord(evs[1], st), "event");
 			f.P.pathLength = pathLength;
+			joinSide();
 			if (mis)
 			{
 				f.P.R0 = R0wave[pathLength];
@@ -971,7 +996,24 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 			if (evs) hipCheck(hipEventRecord(evs[2], st), "event");
 			/* the reference estimator connects from every vertex but the last;
 			 * optix7's from every vertex (its NEE precedes the depth cut) */
-			if (mis || pathLength < f.maxDepth)
+			if ((mis || pathLength < f.maxDepth) && side)
+			{	/* snapshot the counters (advance clears the shadow count), then
+				 * trace and connect on the side stream */
+				hipCheck(hipMemcpyAsync(side->counters, B.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "D2D");
+				hipCheck(hipEventRecord(side->shaded, st), "event");
+				hipCheck(hipStreamWaitEvent(side->st, side->shaded, 0), "wait");
+				const bool timed = side->timing && sideTimed < SIDE_TIMED;
+				if (timed) hipCheck(hipEventRecord(side->timing[2 * sideTimed], side->st), "event");
+				PathBuffers Bs = B;
+				Bs.counters = side->counters;
+				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, side->counters + 2, side->queue, side->ovf, persistentBlocks,
+					side->st), "connect");
+				hipCheck(launch_connect(Bs, n, side->st, mis ? 1.0f : 0.0f), "connect");
+				if (timed) hipCheck(hipEventRecord(side->timing[2 * sideTimed + 1], side->st), "event"), sideTimed++;
+				hipCheck(hipEventRecord(side->done, side->st), "event");
+				sidePending = true;
+			}
+			else if (mis || pathLength < f.maxDepth)
 			{
 				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, B.counters + 2, queue, ovf, persistentBlocks, st), "connect");
 				hipCheck(launch_connect(B, n, st, mis ? 1.0f : 0.0f), "connect");
@@ -991,13 +1033,24 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 				{
 					float a1, b2, c;
 					(void)hipEventElapsedTime(&a1, evs[0], evs[1]), (void)hipEventElapsedTime(&b2, evs[1], evs[2]), (void)hipEventElapsedTime(&c, evs[2], evs[3]);
-					timing[std::min(pathLength - 1, 2)] += a1, timing[3] += b2, timing[4] += c;
+					timing[std::min(pathLength - 1, 2)] += a1, timing[3] += b2;
+					if (!side) timing[4] += c;
 				}
 				if (live == 0) break;
 			}
 		}
+		joinSide();
 		hipCheck(launch_resolve(B.pathAcc, pixels, k, acc32, acc64, st), "resolve");
 		if (gbuf) hipCheck(launch_resolve(B.pathAcc + n, pixels, k, acc32 + pixels, acc64 + 4 * (size_t)pixels, st), "resolve");
+	}
+	if (side && side->timing && timing && sideTimed > 0)
+	{	/* shadow-lane durations, read once the lane has finished */
+		hipCheck(hipEventSynchronize(side->timing[2 * sideTimed - 1]), "sync");
+		for (int i = 0; i < sideTimed; i++)
+		{
+			float c = 0;
+			if (hipEventElapsedTime(&c, side->timing[2 * i], side->timing[2 * i + 1]) == hipSuccess) timing[4] += c;
+		}
 	}
 	return deepest;
 }
@@ -1033,8 +1086,11 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	f.P.probePixel = probePos.x + scrw * probePos.y;
 	f.P.probe = wf.probe();
 	float timing[5] = { 0, 0, 0, 0, 0 };
+	SideLane lane;
+	lane.st = stream2, lane.counters = sideCounters.as<uint32_t>(), lane.queue = sideCounters.as<uint32_t>() + 4, lane.ovf = overflow2.as<uint32_t>();
+	lane.shaded = sideEv[0], lane.done = sideEv[1], lane.timing = sideEv + 2;
 	const int waves = runSamples(f, 0, scrspp, Wavefront::PASS_CAP, wf, wf.acc32(), wf.acc64(), counters.as<uint32_t>() + TRACE_QUEUE_WORDS,
-		overflow.as<uint32_t>(), stream, ev, timing);
+		overflow.as<uint32_t>(), stream, ev, timing, shadowLane ? &lane : nullptr);
 	if (estimator != ESTIMATOR_REFERENCE) for (int k = 0; k < waves; k++) xor32_host(camSeed);
 	const int pixels = scrw * scrh;
 	samplesTaken += scrspp;

@@ -149,9 +149,22 @@ public:
 	void checkRenderable();
 	FrameSetup frameParams(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int depth, uint32_t R0gen, int samplesTakenAtStart,
 		float jitterX = 0, float jitterY = 0);
+	/* A second stream on which the shadow rays of wave k are traced and
+	 * connected while wave k + 1's extension rays are traced (own ray feed,
+	 * overflow stack and counter snapshot; timing pairs per wave). */
+	struct SideLane
+	{
+		hipStream_t st = nullptr;
+		uint32_t* counters = nullptr;   /* 4 words: snapshot of the wave's counters */
+		uint32_t* queue = nullptr;
+		uint32_t* ovf = nullptr;
+		hipEvent_t shaded = nullptr, done = nullptr;
+		hipEvent_t* timing = nullptr;   /* 2 per wave, SIDE_TIMED waves */
+	};
+	static const int SIDE_TIMED = 16;
 	/* Returns the number of shade waves the deepest pass ran. */
 	int runSamples(const FrameSetup& f, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
-		uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing);
+		uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing, const SideLane* side = nullptr);
 	int estimatorSetting() const { return estimator; }
 	bool sceneIsReady() const { return sceneReady; }
 	int traceBlocks() const { return persistentBlocks; }
@@ -177,8 +190,10 @@ private:
 	/* top level */
 	DevBuf tlasNodes, instRecs, shadeInsts;
 	TraceScene scene;
-	/* trace scratch */
-	DevBuf overflow, counters;
+	/* trace scratch (overflow2 / sideCounters: the shadow-ray lane) */
+	DevBuf overflow, counters, overflow2, sideCounters;
+	hipStream_t stream2 = nullptr;
+	hipEvent_t sideEv[2 + 2 * SIDE_TIMED];
 	double wallClockNs = 10.0;         /* period of the device wall clock */
 	int persistentBlocks = 0;
 	float minRaysPerLane = 0.0f;       /* 0: always launch persistentBlocks */
@@ -189,6 +204,7 @@ private:
 	uint32_t camSeed = 0x12345678u, seedSetting = 0x12345678u;
 	int estimator = ESTIMATOR_REFERENCE;
 	bool refitSetting = false;         /* SetGeometry on a mesh of unchanged size: refit instead of rebuild */
+	bool shadowLane = true;            /* shadow rays of wave k on a second stream, beside wave k + 1's extension */
 	int bsdfSetting = 1;               /* Disney, as rendercore_optix7 ships (kernels/bsdf.h) */
 	/* filter estimator (RenderCore_Optix7Filter/rendercore.h:32-35,124) */
 	bool filterOn = true;
