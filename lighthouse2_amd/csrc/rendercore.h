@@ -168,6 +168,7 @@ public:
 	int estimatorSetting() const { return estimator; }
 	bool sceneIsReady() const { return sceneReady; }
 	int traceBlocks() const { return persistentBlocks; }
+	bool shadowLaneOn() const { return shadowLane; }
 	float epsilon() const { return geometryEpsilon; }
 
 	hipStream_t stream = nullptr;

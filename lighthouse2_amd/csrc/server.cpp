@@ -1,6 +1,7 @@
 /* server.cpp - render job server (see server.h).
  *
- * One worker thread owns a HIP stream, its own path buffers and traversal
+ * One worker thread owns a HIP stream (plus a second one for the shadow
+ * lane, as in RenderCore::Render), its own path buffers and traversal
  * scratch. Jobs wait in arrival order until their per-job buffers fit the
  * HBM budget; among admitted jobs the one with the least remaining work
  * (samples x pixels) runs next, one quantum of at most passPaths paths at a
@@ -32,6 +33,10 @@ RenderServer::RenderServer(RenderCore& c, uint64_t budgetBytes, size_t pp) : cor
 	check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
 	queue.ensure(TRACE_QUEUE_WORDS * sizeof(uint32_t));
 	ovf.ensure(trace_overflow_bytes(core.traceBlocks()));
+	check(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking), "hipStreamCreate");
+	for (int i = 0; i < 2; i++) check(hipEventCreateWithFlags(&laneEv[i], hipEventDisableTiming), "hipEventCreate");
+	laneScratch.ensure((4 + TRACE_QUEUE_WORDS) * sizeof(uint32_t));
+	laneOvf.ensure(trace_overflow_bytes(core.traceBlocks()));
 	t0 = std::chrono::steady_clock::now();
 	worker = std::thread([this] { loop(); });
 }
@@ -45,7 +50,9 @@ RenderServer::~RenderServer()
 	wake.notify_all();
 	if (worker.joinable()) worker.join();
 	jobs.clear(), waiting.clear();
-	wf.release(), queue.release(), ovf.release();
+	wf.release(), queue.release(), ovf.release(), laneScratch.release(), laneOvf.release();
+	for (int i = 0; i < 2; i++) if (laneEv[i]) (void)hipEventDestroy(laneEv[i]);
+	if (stream2) (void)hipStreamDestroy(stream2);
 	if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -115,8 +122,11 @@ void RenderServer::loop()
 			}
 			const int quantum = std::max(1, (int)std::min<size_t>(passPaths / (size_t)pixels, (size_t)(j->spp - j->next)));
 			const int s1 = std::min(j->spp, j->next + quantum);
+			RenderCore::SideLane lane;
+			lane.st = stream2, lane.counters = laneScratch.as<uint32_t>(), lane.queue = laneScratch.as<uint32_t>() + 4;
+			lane.ovf = laneOvf.as<uint32_t>(), lane.shaded = laneEv[0], lane.done = laneEv[1];
 			core.runSamples(j->frame, j->next, s1, passPaths, wf, j->acc32.as<float4>(), j->acc64.as<double>(),
-				queue.as<uint32_t>(), ovf.as<uint32_t>(), stream, nullptr, nullptr);
+				queue.as<uint32_t>(), ovf.as<uint32_t>(), stream, nullptr, nullptr, core.shadowLaneOn() ? &lane : nullptr);
 			j->next = s1;
 			if (j->next == j->spp)
 			{

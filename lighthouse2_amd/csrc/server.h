@@ -63,10 +63,11 @@ private:
 	std::map<uint64_t, std::shared_ptr<Job>> jobs;       /* every job not yet read */
 	uint64_t nextId = 1, inUse = 0;
 	bool stopping = false;
-	hipStream_t stream = nullptr;
+	hipStream_t stream = nullptr, stream2 = nullptr;   /* stream2: the shadow lane */
+	hipEvent_t laneEv[2] = { nullptr, nullptr };
 	int device = 0;
 	Wavefront wf;
-	DevBuf queue, ovf;
+	DevBuf queue, ovf, laneScratch, laneOvf;
 	std::chrono::steady_clock::time_point t0;
 	std::thread worker;
 };
