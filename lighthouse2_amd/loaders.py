@@ -741,34 +741,13 @@ def _gltf_image_rgba(doc, bufs, image):
     return decode_image(data, image.get("uri", image.get("name", "")))
 
 
-def gltf_node_transform(node):
-    """HostNode::ConvertFromGLTFNode + UpdateTransformFromTRS
-    (host_node.cpp:77-135): a column-major matrix is transposed into the
-    row-major cells; T * R * S * matrix in fp32 when any of matrix,
-    translation, rotation (x, y, z, w; quat::toMatrix common_types.h:692-705)
-    or scale is given, identity otherwise."""
-    M = F.IDENTITY.copy()
-    given = False
-    if len(node.get("matrix", [])) == 16:
-        M = np.asarray(node["matrix"], np.float64).astype(f32).reshape(4, 4).T.reshape(16).copy()
-        given = True
-    t = np.zeros(3, f32)
-    if len(node.get("translation", [])) == 3:
-        t = np.asarray(node["translation"], np.float64).astype(f32)
-        given = True
-    x = y = z = f32(0)
-    w = f32(1)
-    if len(node.get("rotation", [])) == 4:
-        x, y, z, w = (f32(c) for c in node["rotation"])
-        given = True
-    sc = np.ones(3, f32)
-    if len(node.get("scale", [])) == 3:
-        sc = np.asarray(node["scale"], np.float64).astype(f32)
-        given = True
-    if not given:
-        return M
+def trs_matrix(t, q, sc, M):
+    """HostNode::UpdateTransformFromTRS (host_node.cpp:131-138): T * R * S *
+    matrix in fp32; q = (x, y, z, w), rotated by quat::toMatrix
+    (common_types.h:692-705)."""
     Tm = F.IDENTITY.copy()
-    Tm[3], Tm[7], Tm[11] = t
+    Tm[3], Tm[7], Tm[11] = np.asarray(t, f32)
+    x, y, z, w = (f32(c) for c in q)
     two = f32(2)
     R = F.IDENTITY.copy()
     R[0] = (f32(1) - two * y * y) - two * z * z
@@ -781,11 +760,201 @@ def gltf_node_transform(node):
     R[9] = two * y * z + two * w * x
     R[10] = (f32(1) - two * x * x) - two * y * y
     S = F.IDENTITY.copy()
-    S[0], S[5], S[10] = sc
+    S[0], S[5], S[10] = np.asarray(sc, f32)
     return mat4_mul(mat4_mul(mat4_mul(Tm, R), S), M)
 
 
-def load_gltf(scene, path, transform=None):
+def _node_state(node):
+    """A glTF node's matrix (column-major in the file, transposed into
+    row-major cells), translation, rotation (x, y, z, w) and scale, and
+    whether any was given (HostNode::ConvertFromGLTFNode, host_node.cpp:77-125)."""
+    st = {"M": F.IDENTITY.copy(), "t": np.zeros(3, f32), "q": np.array([0, 0, 0, 1], f32), "s": np.ones(3, f32), "given": False}
+    if len(node.get("matrix", [])) == 16:
+        st["M"] = np.asarray(node["matrix"], np.float64).astype(f32).reshape(4, 4).T.reshape(16).copy()
+        st["given"] = True
+    for key, name, n in (("t", "translation", 3), ("q", "rotation", 4), ("s", "scale", 3)):
+        if len(node.get(name, [])) == n:
+            st[key] = np.asarray(node[name], np.float64).astype(f32)
+            st["given"] = True
+    return st
+
+
+def gltf_node_transform(node):
+    """A node's local transform: T * R * S * matrix when any of matrix,
+    translation, rotation or scale is given, identity otherwise."""
+    st = _node_state(node)
+    return trs_matrix(st["t"], st["q"], st["s"], st["M"]) if st["given"] else st["M"]
+
+
+# ---- glTF animation ------------------------------------------------------------------
+def _quat_normalize(q):
+    """quat::normalize: scale by 1 / sqrtf(((w w + x x) + y y) + z z); q = (x, y, z, w)."""
+    x, y, z, w = (f32(c) for c in q)
+    m = np.sqrt(f32(f32(f32(w * w + x * x) + y * y) + z * z)).astype(f32)
+    r = f32(1) / m
+    return np.array([x * r, y * r, z * r, w * r], f32)
+
+
+def quat_slerp(a, b, t):
+    """quat::slerp (common_types.h:738-762), q = (x, y, z, w): b flipped to
+    a's hemisphere, a linear blend above cos 0.99, otherwise
+    (sin(1 - t) a + sin(t angle) b) / sin(angle) (the reference's first
+    weight is sin(1 - t), not sin((1 - t) angle))."""
+    a = np.asarray(a, f32)
+    r = np.asarray(b, f32).copy()
+    t = f32(t)
+    ct = f32(f32(f32(a[3] * r[3] + a[0] * r[0]) + a[1] * r[1]) + a[2] * r[2])
+    if ct < 0:
+        r, ct = (r * f32(-1)).astype(f32), -ct
+    if ct > f32(0.99):
+        return ((f32(1) - t) * a + t * r).astype(f32)
+    ang = np.arccos(ct).astype(f32)
+    s1, s2, s3 = np.sin(f32(1) - t).astype(f32), np.sin(t * ang).astype(f32), np.sin(ang).astype(f32)
+    return ((s1 * a + s2 * r) / s3).astype(f32)
+
+
+class AnimationSampler:
+    """HostAnimation::Sampler (host_anim.cpp:31-181): key times, keys (vec3,
+    quaternion (x, y, z, w) or scalar) and the interpolation (STEP,
+    CUBICSPLINE with in-tangent, value, out-tangent per key, else LINEAR).
+    Integer rotation / weight keys are scaled as the reference reads them
+    (byte / 127 clamped at -1; unsigned byte read as a signed byte / 255;
+    short and unsigned short read as their first byte, signed, / 32767
+    clamped at -1 and / 65535)."""
+
+    def __init__(self, doc, bufs, sampler):
+        self.t = gltf_accessor(doc, bufs, sampler["input"]).reshape(-1).astype(f32)
+        self.kind = {"STEP": "step", "CUBICSPLINE": "spline"}.get(sampler.get("interpolation", "LINEAR"), "linear")
+        acc = doc["accessors"][sampler["output"]]
+        raw = gltf_accessor(doc, bufs, sampler["output"])
+        if acc["type"] == "VEC3":
+            self.keys = raw.astype(f32)
+        else:
+            ct = acc["componentType"]
+            if ct == 5126:
+                v = raw.astype(f32)
+            else:
+                first = raw.astype(np.int64) & 255
+                sb = np.where(first > 127, first - 256, first).astype(f32)       # the first byte, as a signed char
+                v = {5120: np.maximum(sb / f32(127), f32(-1)), 5121: sb / f32(255),
+                     5122: np.maximum(sb / f32(32767), f32(-1)), 5123: sb / f32(65535)}[ct].astype(f32)
+            self.keys = v if acc["type"] == "VEC4" else v.reshape(-1)
+
+    def sample(self, time, k, quat):
+        """SampleVec3 / SampleQuat (:120-181)."""
+        K, spline = self.keys, self.kind == "spline"
+        if k == 0 and time < self.t[0]:
+            return K[1] if spline else K[0]
+        t0, t1 = self.t[k], self.t[k + 1]
+        f = f32(f32(f32(time) - t0) / f32(t1 - t0))
+        if f <= 0:
+            return K[0]
+        if spline:
+            t2 = f * f
+            t3 = t2 * f
+            dt = f32(t1 - t0)
+            p0, m0 = K[k * 3 + 1], (K[k * 3 + 2] * dt).astype(f32)
+            p1, m1 = K[(k + 1) * 3 + 1], (K[(k + 1) * 3] * dt).astype(f32)
+            out = (((m0 * f32((t3 - f32(2) * t2) + f) + p0 * f32((f32(2) * t3 - f32(3) * t2) + f32(1)))
+                    + p1 * f32(f32(-2) * t3 + f32(3) * t2)) + m1 * f32(t3 - t2)).astype(f32)
+            return _quat_normalize(_quat_normalize(out)) if quat else out
+        if self.kind == "step":
+            return _quat_normalize(K[k]) if quat else K[k]
+        if quat:
+            return _quat_normalize(_quat_normalize(quat_slerp(K[k], K[k + 1], f)))
+        return ((f32(1) - f) * K[k] + f * K[k + 1]).astype(f32)
+
+
+class AnimationChannel:
+    """HostAnimation::Channel (host_anim.cpp:183-276): its own clock and key
+    index; times wrap at the last key; a single key, or a last key at time
+    0, holds the first key. Targets: translation, rotation, scale (morph
+    weights are not applied: meshes are read in their base pose)."""
+
+    def __init__(self, ch):
+        self.sampler = ch["sampler"]
+        self.node = ch["target"].get("node", -1)
+        self.path = ch["target"]["path"]
+        self.t, self.k = f32(0), 0
+
+    def update(self, dt, smp, state):
+        self.t = f32(self.t + f32(dt))
+        if self.path not in ("translation", "rotation", "scale") or self.node < 0:
+            return False
+        key = {"translation": "t", "rotation": "q", "scale": "s"}[self.path]
+        n = len(smp.t)
+        dur = smp.t[n - 1]
+        if dur == 0 or n == 1:
+            v = smp.keys[0]
+        else:
+            while self.t >= dur:
+                self.t, self.k = f32(self.t - dur), 0
+            while self.t >= smp.t[(self.k + 1) % n]:
+                self.k += 1
+            v = smp.sample(self.t, self.k, key == "q")
+        state[self.node][key] = np.asarray(v, f32).copy()
+        state[self.node]["given"] = True
+        return True
+
+
+class GltfScene:
+    """The node hierarchy and animations of one glTF file added to a Scene:
+    `instances` (scene instance indices, in HostNode::Update's children-first
+    order), `update(dt)` advancing every animation channel by dt seconds
+    (HostAnimation::Update) and re-placing the instances and the world-space
+    lights of emissive meshes (HostNode::Update / UpdateLights)."""
+
+    def __init__(self, scene, doc, bufs, mesh_ids, node_base, root):
+        self.scene, self.mesh_ids, self.node_base = scene, mesh_ids, node_base
+        self.docnodes = doc.get("nodes", [])
+        self.state = [_node_state(n) for n in self.docnodes]
+        self.root = mat4_mul(F.IDENTITY, root)
+        sc = doc.get("scenes", [])
+        self.roots = sc[0].get("nodes", []) if sc else []
+        self.samplers, self.channels = [], []
+        for a in doc.get("animations", []):
+            smp = [AnimationSampler(doc, bufs, x) for x in a.get("samplers", [])]
+            self.channels.append([(AnimationChannel(c), smp) for c in a.get("channels", [])])
+        combined, order = self._walk()
+        self.order = order
+        self.instances = [scene.add_instance(mesh_ids[self.docnodes[i]["mesh"]], combined[i]) for i in order]
+        self.lights = {}
+        for i in sorted(order):                      # lights in node order (PrepareLights at construction)
+            self.lights[i] = instance_lights(scene, mesh_ids[self.docnodes[i]["mesh"]], combined[i], node_base + i)
+
+    def _local(self, i):
+        st = self.state[i]
+        return trs_matrix(st["t"], st["q"], st["s"], st["M"]) if st["given"] else st["M"]
+
+    def _walk(self):
+        combined, order = {}, []
+
+        def visit(i, parent):
+            combined[i] = mat4_mul(parent, self._local(i))
+            for c in self.docnodes[i].get("children", []):
+                visit(c, combined[i])
+            if "mesh" in self.docnodes[i]:
+                order.append(i)
+        for r in self.roots:
+            visit(r, self.root)
+        return combined, order
+
+    def update(self, dt):
+        moved = False
+        for chans in self.channels:
+            for ch, smp in chans:
+                moved |= ch.update(dt, smp[ch.sampler], self.state)
+        if not moved:
+            return
+        combined, _ = self._walk()
+        for inst, i in zip(self.instances, self.order):
+            self.scene.instances[inst] = (self.scene.instances[inst][0], combined[i])
+        for i, slot in self.lights.items():
+            if slot is not None:
+                write_instance_lights(self.scene, self.mesh_ids[self.docnodes[i]["mesh"]], combined[i], self.node_base + i, *slot)
+
+
+def load_gltf(scene, path, transform=None, animated=False):
     """HostScene::AddScene for a glTF file (host_scene.cpp:249-392): adds the
     file's textures (8-bit PNG images, top row first, neither flipped nor
     linearised), materials, one mesh per glTF mesh (its triangle primitives
@@ -797,7 +966,9 @@ def load_gltf(scene, path, transform=None):
     list of instance indices. Without materials in the file every triangle
     uses scene material 0 (a grey Lambert material is added when the scene
     has none); a primitive without a material gets a grey Lambert one.
-    Skins, morph targets and animations are not read (the base pose is)."""
+    Skins and morph targets are not applied (the base pose is read); with
+    animated=True the GltfScene is returned, whose update(dt) plays the
+    file's translation / rotation / scale animations."""
     doc, bufs, directory = read_gltf(path)
     tex_ids = []
     for t in doc.get("textures", []):
@@ -843,28 +1014,11 @@ def load_gltf(scene, path, transform=None):
         else:
             verts, tris = np.zeros((0, 4), f32), F.new_tris(0)
         mesh_ids.append(scene.add_mesh(verts, tris))
-    nodes = doc.get("nodes", [])
     node_base = getattr(scene, "node_count", len(scene.instances)) + 1
-    scene.node_count = node_base + len(nodes)
+    scene.node_count = node_base + len(doc.get("nodes", []))
     root = F.IDENTITY.copy() if transform is None else np.asarray(transform, f32).reshape(16)
-    local = [gltf_node_transform(n) for n in nodes]
-    combined = [None] * len(nodes)
-    order = []
-
-    def visit(i, parent):
-        combined[i] = mat4_mul(parent, local[i])
-        for c in nodes[i].get("children", []):
-            visit(c, combined[i])
-        if "mesh" in nodes[i]:
-            order.append(i)
-    scenes_ = doc.get("scenes", [])
-    root_combined = mat4_mul(F.IDENTITY, root)
-    for r in (scenes_[0].get("nodes", []) if scenes_ else []):
-        visit(r, root_combined)
-    instances = [scene.add_instance(mesh_ids[nodes[i]["mesh"]], combined[i]) for i in order]
-    for i in sorted(order):                          # lights in node order (PrepareLights at construction)
-        instance_lights(scene, mesh_ids[nodes[i]["mesh"]], combined[i], node_base + i)
-    return instances
+    g = GltfScene(scene, doc, bufs, mesh_ids, node_base, root)
+    return g.instances if not animated else g
 
 
 def mat4_mul(a, b):
@@ -903,22 +1057,29 @@ def instance_lights(scene, mesh, transform, node_id):
     object-space area (CoreTri::UpdateArea) and its index in the light list
     (ltriIdx), and a CoreLightTri of the triangle transformed by the
     instance's transform is appended, radiance = material colour, instIdx =
-    the node's id."""
+    the node's id. Returns (first light index, triangle ids) or None."""
     verts, tris = scene.meshes[mesh]
     if len(tris) == 0:
-        return
+        return None
     col = scene.materials["color"]["value"][tris["material"]]
     sel = np.nonzero((col > 1).any(1))[0]
     if len(sel) == 0:
-        return
+        return None
     sub = tris[sel]
     scenes.add_emitters(scene, sub, np.zeros(3, f32), node_id, tri_ids=sel)   # areas, ltriIdx, light slots
-    world = _transformed_tris(sub, transform)
-    first = len(scene.tri_lights) - len(sel)
-    for k in range(len(sel)):
-        rad = scene.materials["color"]["value"][sub["material"][k]]
-        scene.tri_lights[first + k] = scenes.light_triangle(world[k], rad, int(sel[k]), node_id)[0]
     tris[sel] = sub
+    first = len(scene.tri_lights) - len(sel)
+    write_instance_lights(scene, mesh, transform, node_id, first, sel)
+    return first, sel
+
+
+def write_instance_lights(scene, mesh, transform, node_id, first, sel):
+    """Rewrites an instance's light slots from its (new) transform."""
+    _, tris = scene.meshes[mesh]
+    world = _transformed_tris(tris[sel], transform)
+    for k in range(len(sel)):
+        rad = scene.materials["color"]["value"][tris["material"][sel[k]]]
+        scene.tri_lights[first + k] = scenes.light_triangle(world[k], rad, int(sel[k]), node_id)[0]
 
 
 def add_instance_with_lights(scene, mesh, transform=None):

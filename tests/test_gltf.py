@@ -225,3 +225,73 @@ def test_unsupported_inputs_raise(tmp_path):
     g2.mesh([p])
     with pytest.raises(ValueError):
         L.load_gltf(scenes.Scene(), str(g2.write_glb(tmp_path / "bad2.glb")))
+
+
+def animated_file(tmp_path, interp, times, keys, path="translation", emissive=False):
+    g = GltfBuilder()
+    m = g.material(pbrMetallicRoughness={"baseColorFactor": [5, 5, 5, 1] if emissive else [0.5, 0.5, 0.5, 1]})
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], f32)
+    g.mesh([g.primitive(tri, [0, 1, 2], material=m)])
+    g.node(root=True, mesh=0)
+    tin = g.accessor(np.asarray(times, f32))
+    tout = g.accessor(np.asarray(keys, f32))
+    g.doc["animations"] = [{"samplers": [{"input": tin, "output": tout, "interpolation": interp}],
+                            "channels": [{"sampler": 0, "target": {"node": 0, "path": path}}]}]
+    return g.write_glb(tmp_path / f"anim_{interp}_{path}.glb")
+
+
+def translation(s):
+    return s.instances[0][1].reshape(4, 4)[:3, 3]
+
+
+def test_linear_and_step_translation_keys_wrap_in_time(tmp_path):
+    keys = [[0, 0, 0], [2, 0, 0], [2, 4, 0]]
+    s = scenes.Scene()
+    g = L.load_gltf(s, str(animated_file(tmp_path, "LINEAR", [0, 1, 2], keys)), animated=True)
+    assert np.array_equal(translation(s), [0, 0, 0])
+    for dt, want in [(0.5, [1, 0, 0]), (1.0, [2, 2, 0]), (1.0, [1, 0, 0])]:   # t = 0.5, 1.5, 2.5 -> 0.5
+        g.update(dt)
+        assert np.allclose(translation(s), want, atol=1e-6), dt
+    s2 = scenes.Scene()
+    g2 = L.load_gltf(s2, str(animated_file(tmp_path, "STEP", [0, 1, 2], keys)), animated=True)
+    g2.update(1.2)
+    assert np.array_equal(translation(s2), [2, 0, 0])
+
+
+def test_cubic_spline_translation_is_hermite(tmp_path):
+    # per key: in-tangent, value, out-tangent; tangents scaled by the key interval
+    keys = [[0, 0, 0], [0, 0, 0], [1, 0, 0], [0, 0, 0], [4, 0, 0], [0, 0, 0]]
+    s = scenes.Scene()
+    g = L.load_gltf(s, str(animated_file(tmp_path, "CUBICSPLINE", [0, 2], keys)), animated=True)
+    g.update(1.0)    # f = 0.5: h00 p0 + h01 p1 + h10 m0 = 0.5 * 0 + 0.5 * 4 + 0.125 * (out-tangent 1 * interval 2)
+    assert np.allclose(translation(s), [2.25, 0, 0], atol=1e-6)
+
+
+def test_rotation_slerp_stays_a_rotation_about_the_key_axis(tmp_path):
+    h = np.sqrt(0.5)
+    keys = [[0, 0, 0, 1], [0, h, 0, h]]                                     # identity -> 90 degrees about y
+    s = scenes.Scene()
+    g = L.load_gltf(s, str(animated_file(tmp_path, "LINEAR", [0, 1], keys, path="rotation")), animated=True)
+    angles = []
+    for _ in range(3):
+        g.update(0.25)
+        R = s.instances[0][1].reshape(4, 4)[:3, :3].astype(np.float64)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-5) and np.allclose(R[1], [0, 1, 0], atol=1e-6)
+        angles.append(np.arctan2(R[0, 2], R[0, 0]))
+    assert 0 < angles[0] < angles[1] < angles[2] < np.pi / 2
+    # near-parallel keys take the linear branch (cos > 0.99): a small rotation stays small
+    q = L.quat_slerp([0, 0, 0, 1], [0, 0.05, 0, np.sqrt(1 - 0.0025)], 0.5)
+    assert abs(q[1] - 0.025) < 1e-6
+    # the reference's first weight is sin(1 - t): at t = 0.5 between keys 90 degrees apart in quaternion angle
+    q = L.quat_slerp([0, 0, 0, 1], [0, 1, 0, 0], 0.5)
+    assert np.allclose(q, [0, np.sin(np.pi / 4), 0, np.sin(0.5)], atol=1e-6)
+
+
+def test_animated_lamp_moves_its_lights(tmp_path):
+    keys = [[0, 0, 0], [0, 3, 0]]
+    s = scenes.Scene()
+    g = L.load_gltf(s, str(animated_file(tmp_path, "LINEAR", [0, 1], keys, emissive=True)), animated=True)
+    assert np.array_equal(s.tri_lights["vertex0"][0], [0, 0, 0])
+    g.update(0.5)
+    assert np.allclose(s.tri_lights["vertex0"][0], [0, 1.5, 0]) and np.allclose(s.tri_lights["vertex2"][0], [0, 2.5, 0])
+    assert abs(s.tri_lights["area"][0] - 0.5) < 1e-6

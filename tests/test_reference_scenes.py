@@ -94,3 +94,20 @@ def test_legocar_texture_is_read_from_tga():
     assert tex["width"] > 0 and tex["height"] > 0
     textured = s.materials["color"]["textureID"] == 0
     assert textured.sum() == 1 and np.all(s.materials["color"]["value"][textured] == 1)
+
+
+def test_reference_animations_play():
+    """AnimatedCube's rotation channel and the strip's joint rotations advance
+    without disturbing unit rotations; instance transforms move."""
+    s = scenes.Scene()
+    g = L.load_gltf(s, os.path.join(SHARED, "animatedCube/AnimatedCube.gltf"), animated=True)
+    T0 = s.instances[0][1].copy()
+    for _ in range(5):
+        g.update(0.37)
+        R = s.instances[0][1].reshape(4, 4)[:3, :3].astype(np.float64)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-5)
+    assert not np.array_equal(s.instances[0][1], T0)
+    s2 = scenes.Scene()
+    g2 = L.load_gltf(s2, os.path.join(SHARED, "strip.gltf"), animated=True)
+    g2.update(0.5)
+    assert np.array_equal(s2.instances[0][1], np.eye(4, dtype=np.float32).reshape(16))   # only joint nodes animate
