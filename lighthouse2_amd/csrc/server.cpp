@@ -67,6 +67,8 @@ uint64_t RenderServer::Submit(const lighthouse2::ViewPyramid& view, int w, int h
 	if (j->bytes() > budget) throw CoreError(LH2HIP_ERR_INVALID, "server submit: job needs more device memory than the budget");
 	/* the frame constants only read scene state, which must stay fixed */
 	j->frame = core.frameParams(view, w, h, spp, maxDepth, xor32_seed(seed), 0);
+	/* jobs write no G-buffer: the core's buffers belong to Render's target */
+	j->frame.P.gFeatures = nullptr, j->frame.P.gWorldPos = nullptr, j->frame.P.gDeltaDepth = nullptr;
 	std::lock_guard<std::mutex> g(lock);
 	j->id = nextId++;
 	j->t.queued = now();
