@@ -286,9 +286,11 @@ def bench_learned(a, np, torch, lh, scenes):
 def bench_serve(a, np, torch, lh, scenes):
     """Job server under Poisson arrivals (lambda 20 jobs/s, seed 7):
     resolutions {256^2, 512^2, 1920x1080, 3840x2160}, spp {1, 4, 16, 64},
-    depth U{4..8}, on the instanced scene (10 displaced spheres, 1,000
-    instances, 2048^2 textures, 64 area lights), reference estimator;
-    latency = completion - arrival. Jobs are not denoised."""
+    on the instanced scene (10 displaced spheres, 1,000 instances, 2048^2
+    textures, 64 area lights); every job is rendered by the filter estimator
+    with its own G-buffer and denoised by SVGF (a first frame, TAA off);
+    latency = completion - arrival (the jobs' depth draws, U{4..8}, are kept
+    in the stream but the filter estimator's path length is its own)."""
     import threading
     s = scenes.instanced_scene()
     core = lh.RenderCore()
@@ -306,7 +308,7 @@ def bench_serve(a, np, torch, lh, scenes):
         for (wh, spp, d), gap in zip(jobs, gaps):
             t += gap
             time.sleep(max(0.0, t - time.time()))
-            ids.append(srv.submit(scenes.instanced_view(*wh), wh[0], wh[1], spp, d, 1234))
+            ids.append(srv.submit(scenes.instanced_view(*wh), wh[0], wh[1], spp, d, 1234, denoise=1))
     th = threading.Thread(target=submitter)
     th.start()
     th.join()
@@ -324,7 +326,7 @@ def bench_serve(a, np, torch, lh, scenes):
                       "jobs": len(lat), "lambda_jobs_per_s": 20, "makespan_s": round(makespan, 2),
                       "mpaths_per_s": round(paths / makespan / 1e6, 1), "dtype": "f32", "data": "synthetic", "vs_baseline": None,
                       "config": {"workload": "Poisson job stream on the instanced scene (10 displaced spheres 50k-200k tris, 1000 instances, "
-                                             "2048^2 textures, 64 area lights), mixed resolution/spp/depth, reference estimator, no denoising",
+                                             "2048^2 textures, 64 area lights), mixed resolution/spp, filter estimator + SVGF per job",
                                  "budget_gb": 64, "pass_paths": 1 << 21}}))
     srv.close()
 
@@ -333,7 +335,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {a.gpus}); "
-              "the traversal shards by image tile with the scene replicated, see DESIGN.md", file=sys.stderr)
+              "the traversal would shard by image tile with the scene replicated", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

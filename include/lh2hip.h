@@ -228,6 +228,15 @@ LH2HIP_API int lh2hip_server_create(lh2hip_core* core, uint64_t budgetBytes, uin
 LH2HIP_API int lh2hip_server_destroy(lh2hip_server* server);
 LH2HIP_API int lh2hip_server_submit(lh2hip_server* server, const ViewPyramid* view, int width, int height, int spp,
 	int maxDepth, uint32_t seed, uint64_t* jobId);
+/* As lh2hip_server_submit, with denoise 1: the job is rendered by the filter
+ * estimator with its own G-buffer and filtered by SVGF, its image equal to
+ * RenderCore::Render(view, Restart) with the filter estimator, TAA off and
+ * camera seed `seed` on a fresh core (RenderCore_Optix7Filter's
+ * FinalizeRender for a first frame); budget 160 B per pixel. denoise 0 is
+ * lh2hip_server_submit. No counterpart in the reference (its cores have no
+ * job server). */
+LH2HIP_API int lh2hip_server_submit_denoised(lh2hip_server* server, const ViewPyramid* view, int width, int height, int spp,
+	int maxDepth, uint32_t seed, int denoise, uint64_t* jobId);
 /* *done = 1 once the job finished (or failed: lh2hip_server_read reports it). */
 LH2HIP_API int lh2hip_server_done(lh2hip_server* server, uint64_t jobId, int* done);
 /* Blocks until the job is done; timeoutSeconds < 0 waits without limit.

@@ -81,6 +81,7 @@ _SIGS = {
     "lh2hip_server_create": (I, [P, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(P)]),
     "lh2hip_server_destroy": (I, [P]),
     "lh2hip_server_submit": (I, [P, P, I, I, I, I, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
+    "lh2hip_server_submit_denoised": (I, [P, P, I, I, I, I, ctypes.c_uint32, I, ctypes.POINTER(ctypes.c_uint64)]),
     "lh2hip_server_done": (I, [P, ctypes.c_uint64, ctypes.POINTER(I)]),
     "lh2hip_server_wait": (I, [P, ctypes.c_uint64, ctypes.c_double]),
     "lh2hip_server_read": (I, [P, ctypes.c_uint64, P, P]),
@@ -278,10 +279,14 @@ class RenderServer:
         _check(library().lh2hip_server_create(core._h, budget_bytes, pass_paths, ctypes.byref(h)))
         self._h, self._sizes = h, {}
 
-    def submit(self, view, width, height, spp=1, max_depth=3, seed=0x12345678):
+    def submit(self, view, width, height, spp=1, max_depth=3, seed=0x12345678, denoise=0):
+        """denoise 1: filter estimator + SVGF for this job."""
         v = np.ascontiguousarray(view, F.ViewPyramid)
         jid = ctypes.c_uint64()
-        _check(library().lh2hip_server_submit(self._h, _ptr(v), width, height, spp, max_depth, seed, ctypes.byref(jid)))
+        if denoise:
+            _check(library().lh2hip_server_submit_denoised(self._h, _ptr(v), width, height, spp, max_depth, seed, denoise, ctypes.byref(jid)))
+        else:
+            _check(library().lh2hip_server_submit(self._h, _ptr(v), width, height, spp, max_depth, seed, ctypes.byref(jid)))
         self._sizes[jid.value] = (height, width)
         return jid.value
 

@@ -268,6 +268,17 @@ LH2HIP_API int lh2hip_server_submit(lh2hip_server* server, const ViewPyramid* vi
 	});
 }
 
+LH2HIP_API int lh2hip_server_submit_denoised(lh2hip_server* server, const ViewPyramid* view, int width, int height, int spp,
+	int maxDepth, uint32_t seed, int denoise, uint64_t* jobId)
+{
+	return guarded_server(server, [&](lh::RenderServer& s) {
+		if (!view || !jobId) throw CoreError(LH2HIP_ERR_INVALID, "server submit: null argument");
+		lighthouse2::ViewPyramid v;
+		memcpy(&v, view, sizeof(v));
+		*jobId = s.Submit(v, width, height, spp, maxDepth, seed, denoise);
+	});
+}
+
 LH2HIP_API int lh2hip_server_done(lh2hip_server* server, uint64_t jobId, int* done)
 {
 	return guarded_server(server, [&](lh::RenderServer& s) {

@@ -864,8 +864,9 @@ void RenderCore::checkRenderable()
 /* Shade and camera constants of one Render call (the reference's staged
  * __constant__ globals, lib/RenderCore_PrimeRef/rendercore.cpp:535-584). */
 FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int spp, int depth, uint32_t R0gen, int samplesTakenAtStart,
-	float jitterX, float jitterY)
+	float jitterX, float jitterY, int estimatorOverride)
 {
+	const int estimator = estimatorOverride >= 0 ? estimatorOverride : this->estimator;
 	FrameSetup f;
 	memcpy(f.cam.pos, &view.pos, 12), memcpy(f.cam.p1, &view.p1, 12), memcpy(f.cam.p2, &view.p2, 12), memcpy(f.cam.p3, &view.p3, 12);
 	f.cam.aperture = view.aperture, f.cam.spreadAngle = view.spreadAngle, f.cam.distortion = view.distortion, f.cam.w = w, f.cam.h = h;

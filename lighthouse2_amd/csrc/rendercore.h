@@ -148,7 +148,9 @@ public:
 	/* Render building blocks, shared with the job server (server.cpp). */
 	void checkRenderable();
 	FrameSetup frameParams(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int depth, uint32_t R0gen, int samplesTakenAtStart,
-		float jitterX = 0, float jitterY = 0);
+		float jitterX = 0, float jitterY = 0, int estimatorOverride = -1);
+	float clampDirectSetting() const { return clampDirect; }
+	float clampIndirectSetting() const { return clampIndirect; }
 	/* A second stream on which the shadow rays of wave k are traced and
 	 * connected while wave k + 1's extension rays are traced (own ray feed,
 	 * overflow stack and counter snapshot; timing pairs per wave). */

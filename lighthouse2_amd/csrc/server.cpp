@@ -11,6 +11,7 @@
  * renders: the quanta split its samples, and a path's random numbers depend on
  * its index in the frame only. The scene must not change while jobs run. */
 #include "server.h"
+#include "svgf.h"
 #include "../../include/lh2hip.h"
 #include <algorithm>
 #include <cstring>
@@ -58,17 +59,21 @@ RenderServer::~RenderServer()
 
 double RenderServer::now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 
-uint64_t RenderServer::Submit(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int maxDepth, uint32_t seed)
+uint64_t RenderServer::Submit(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int maxDepth, uint32_t seed, int denoise)
 {
 	if (w <= 0 || h <= 0 || spp <= 0 || maxDepth < 1 || maxDepth > 1024 || (int64_t)w * h > (1 << 24))
 		throw CoreError(LH2HIP_ERR_INVALID, "server submit: bad job size (w*h <= 2^24, spp >= 1, depth 1..1024)");
+	if (denoise != 0 && denoise != 1) throw CoreError(LH2HIP_ERR_INVALID, "server submit: denoise 0 (none) or 1 (SVGF)");
+	if (denoise && (w < 3 || h < 3)) throw CoreError(LH2HIP_ERR_INVALID, "server submit: SVGF needs at least 3 x 3 pixels");
 	auto j = std::make_shared<Job>();
-	j->view = view, j->w = w, j->h = h, j->spp = spp, j->depth = maxDepth, j->seed = seed;
+	j->view = view, j->w = w, j->h = h, j->spp = spp, j->depth = maxDepth, j->seed = seed, j->denoise = denoise;
 	if (j->bytes() > budget) throw CoreError(LH2HIP_ERR_INVALID, "server submit: job needs more device memory than the budget");
 	/* the frame constants only read scene state, which must stay fixed */
-	j->frame = core.frameParams(view, w, h, spp, maxDepth, xor32_seed(seed), 0);
-	/* jobs write no G-buffer: the core's buffers belong to Render's target */
+	j->frame = core.frameParams(view, w, h, spp, maxDepth, xor32_seed(seed), 0, 0.0f, 0.0f, denoise ? ESTIMATOR_FILTER : -1);
+	/* the core's G-buffer belongs to Render's target; an SVGF job gets its own
+	 * when admitted (rendercore.cpp:780-797: aperture 0 while filtering) */
 	j->frame.P.gFeatures = nullptr, j->frame.P.gWorldPos = nullptr, j->frame.P.gDeltaDepth = nullptr;
+	if (denoise) j->frame.cam.aperture = 0, j->frame.camSeed = seed;
 	std::lock_guard<std::mutex> g(lock);
 	j->id = nextId++;
 	j->t.queued = now();
@@ -115,12 +120,21 @@ void RenderServer::loop()
 		try
 		{
 			const int pixels = j->w * j->h;
+			const int halves = j->denoise ? 2 : 1;
 			if (j->next == 0)
 			{	/* first quantum: per-job buffers, cleared */
-				j->acc32.ensure((size_t)pixels * sizeof(float4)), j->acc64.ensure((size_t)pixels * 4 * sizeof(double));
+				j->acc32.ensure((size_t)halves * pixels * sizeof(float4)), j->acc64.ensure((size_t)halves * pixels * 4 * sizeof(double));
 				j->image.ensure((size_t)pixels * sizeof(float4));
 				check(hipMemsetAsync(j->acc32.p, 0, j->acc32.bytes, stream), "memset");
 				check(hipMemsetAsync(j->acc64.p, 0, j->acc64.bytes, stream), "memset");
+				if (j->denoise)
+				{	/* SVGF writes the interior pixels only: the border stays 0, as in a fresh render target */
+					check(hipMemsetAsync(j->image.p, 0, j->image.bytes, stream), "memset");
+					j->gFeatures.ensure((size_t)pixels * 16), j->gWorldPos.ensure((size_t)pixels * 16), j->gDeltaDepth.ensure((size_t)pixels * 16);
+					check(hipMemsetAsync(j->gFeatures.p, 0, j->gFeatures.bytes, stream), "memset");
+					j->frame.P.gFeatures = j->gFeatures.as<uint4>(), j->frame.P.gWorldPos = j->gWorldPos.as<float4>();
+					j->frame.P.gDeltaDepth = j->gDeltaDepth.as<float4>();
+				}
 			}
 			const int quantum = std::max(1, (int)std::min<size_t>(passPaths / (size_t)pixels, (size_t)(j->spp - j->next)));
 			const int s1 = std::min(j->spp, j->next + quantum);
@@ -132,10 +146,21 @@ void RenderServer::loop()
 			j->next = s1;
 			if (j->next == j->spp)
 			{
-				check(launch_finalize(j->acc32.as<float4>(), pixels, 1.0f / (float)j->spp, j->image.as<float4>(), stream), "finalize");
+				if (j->denoise)
+				{	/* FinalizeRender's filter path for a first frame (RenderCore_Optix7Filter/rendercore.cpp:897-935) */
+					auto& f = filters[{ j->w, j->h }];
+					if (!f) f.reset(new Svgf(j->w, j->h));
+					else f->Reset(stream);
+					f->taa = 0;
+					const lighthouse2::ViewPyramid none = {};
+					f->Frame(j->acc32.as<float4>(), j->gFeatures.as<uint4>(), j->gWorldPos.as<float4>(), j->gDeltaDepth.as<float4>(), none,
+						0.0f, 0.0f, 0.0f, 0.0f, j->spp, core.clampDirectSetting(), core.clampIndirectSetting(), 0, j->image.as<float4>(), stream);
+					j->gFeatures.release(), j->gWorldPos.release(), j->gDeltaDepth.release();
+				}
+				else check(launch_finalize(j->acc32.as<float4>(), pixels, 1.0f / (float)j->spp, j->image.as<float4>(), stream), "finalize");
 				j->hostImage.resize((size_t)pixels * 4), j->hostAcc.resize((size_t)pixels * 4);
 				check(hipMemcpyAsync(j->hostImage.data(), j->image.p, j->image.bytes, hipMemcpyDeviceToHost, stream), "D2H");
-				check(hipMemcpyAsync(j->hostAcc.data(), j->acc64.p, j->acc64.bytes, hipMemcpyDeviceToHost, stream), "D2H");
+				check(hipMemcpyAsync(j->hostAcc.data(), j->acc64.p, (size_t)pixels * 4 * sizeof(double), hipMemcpyDeviceToHost, stream), "D2H");
 				check(hipStreamSynchronize(stream), "sync");
 				j->acc32.release(), j->acc64.release(), j->image.release();
 				std::lock_guard<std::mutex> g(lock);

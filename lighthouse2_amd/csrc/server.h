@@ -23,7 +23,10 @@ public:
 	 * quantum (a job runs at most this many paths before another job may run). */
 	RenderServer(RenderCore& core, uint64_t budgetBytes, size_t passPaths);
 	~RenderServer();
-	uint64_t Submit(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int maxDepth, uint32_t seed);
+	/* denoise 0: the core's estimator, image = accumulator / spp; 1: the
+	 * filter estimator (G-buffer per job) and SVGF on the job's samples, as
+	 * RenderCore::Render draws a first frame with the filter on and TAA off. */
+	uint64_t Submit(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int maxDepth, uint32_t seed, int denoise = 0);
 	bool Done(uint64_t id);
 	/* Blocks until the job is done (timeout < 0: no limit); false on timeout. */
 	bool Wait(uint64_t id, double timeoutSeconds);
@@ -38,17 +41,19 @@ private:
 	{
 		uint64_t id;
 		lighthouse2::ViewPyramid view;
-		int w, h, spp, depth;
+		int w, h, spp, depth, denoise = 0;
 		uint32_t seed;
 		int next = 0;                   /* next sample to render */
 		bool admitted = false, done = false;
 		std::string error;
-		DevBuf acc32, acc64, image;
+		DevBuf acc32, acc64, image, gFeatures, gWorldPos, gDeltaDepth;
 		std::vector<float> hostImage;
 		std::vector<double> hostAcc;
 		JobTimes t = { 0, 0, 0 };
 		FrameSetup frame;
-		uint64_t bytes() const { return (uint64_t)w * h * 64; }
+		/* accumulators + image: 64 B/px; with SVGF, direct and indirect halves
+		 * (another 48 B/px) and the G-buffer (48 B/px) */
+		uint64_t bytes() const { return (uint64_t)w * h * (denoise ? 160 : 64); }
 		uint64_t remaining() const { return (uint64_t)(spp - next) * (uint64_t)w * h; }
 	};
 	void loop();
@@ -67,6 +72,7 @@ private:
 	hipEvent_t laneEv[2] = { nullptr, nullptr };
 	int device = 0;
 	Wavefront wf;
+	std::map<std::pair<int, int>, std::unique_ptr<Svgf>> filters;   /* per job size, reused */
 	DevBuf queue, ovf, laneScratch, laneOvf;
 	std::chrono::steady_clock::time_point t0;
 	std::thread worker;

@@ -24,6 +24,8 @@ public:
 	 * count in bits 0..3 of w is kept by the filter); worldPos: device float4
 	 * {position, packed normal | specular bits}; deltaDepth: device float4
 	 * (z, w = depth derivatives). out: device float4 [w * h], the display image. */
+	/* Clears every buffer, history included: the next Frame is a first frame. */
+	void Reset(hipStream_t s);
 	void Frame(const float4* accumulator, const uint4* features, const float4* worldPos, const float4* deltaDepth,
 		const lighthouse2::ViewPyramid& prevView, float j0, float j1, float prevj0, float prevj1, int samplesTaken,
 		float clampDirect, float clampIndirect, int camIsStationary, float4* out, hipStream_t s);

@@ -500,6 +500,13 @@ Svgf::Svgf(int w_, int h_) : w(w_), h(h_)
 		chk(hipMemset(b->p, 0, b->bytes), "memset");
 }
 
+void Svgf::Reset(hipStream_t s)
+{
+	for (DevBuf* b : { &features, &worldPos, &prevWorldPos, &deltaDepth, &motion, &moments, &prevMoments, &shading, &filteredIn, &filteredOut, &lastPass,
+		&prevPixels, &taaOut })
+		chk(hipMemsetAsync(b->p, 0, b->bytes, s), "memset");
+}
+
 void Svgf::Frame(const float4* acc, const uint4* feat, const float4* wp, const float4* dd, const lighthouse2::ViewPyramid& prevView,
 	float j0, float j1, float pj0, float pj1, int samplesTaken, float clampD, float clampI, int stationary, float4* out, hipStream_t s)
 {

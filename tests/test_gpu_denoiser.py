@@ -1,4 +1,4 @@
-"""Learned denoiser (SURVEY row A17) on the gfx950 matrix cores against the
+"""Learned denoiser on the gfx950 matrix cores against the
 FP64 reference (oracle/denoiser_ref.py) with bf16 rounding emulated where
 the kernels store bf16. Tolerances: fp32-accumulated outputs within 1e-5 of
 the output scale; bf16-stored outputs within 1 bf16 ulp (2^-7 relative);

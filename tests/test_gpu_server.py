@@ -1,6 +1,8 @@
-"""Render job server: jobs of mixed resolution, spp, depth
-and seed, scheduled in quanta under an HBM budget, must each produce exactly
-the image RenderCore.Render(view, Restart) produces for them alone."""
+"""Render job server: jobs of mixed resolution, spp, depth and seed,
+scheduled in quanta under an HBM budget, must each produce exactly the image
+RenderCore.Render(view, Restart) produces for them alone; a job denoised by
+SVGF exactly the filtered first frame of RenderCore.Render with the filter
+estimator and TAA off."""
 import numpy as np
 import pytest
 
@@ -78,5 +80,39 @@ def test_server_errors(torch_cuda):
         srv.read(j)                                # already released
     with pytest.raises(lh.LH2HIPError):
         srv.done(12345)
+    srv.close()
+    core.Shutdown()
+
+
+def solo_filtered(scene, view, w, h, spp, seed):
+    core = lh.RenderCore()
+    lh.load_scene(core, scene)
+    core.SetTarget(w, h, spp)
+    core.Setting("estimator", 2)
+    core.Setting("TAA", 0)
+    core.Setting("seed", seed)
+    core.Render(view, lh.LH2_RESTART)
+    img, acc = core.read_image(), core.read_accumulator_f64()
+    core.Shutdown()
+    return img, acc
+
+
+def test_svgf_jobs_match_filtered_render(torch_cuda):
+    s = scenes.cornell_box(sphere_tris=2000)
+    core = lh.RenderCore()
+    lh.load_scene(core, s)
+    srv = lh.RenderServer(core, budget_bytes=1 << 30, pass_paths=3000)
+    jobs = [(64, 48, 2, 21, 1), (40, 40, 1, 22, 0), (64, 48, 3, 23, 1)]     # two SVGF jobs of one size share a filter
+    ids = [srv.submit(scenes.cornell_view(w, h), w, h, spp, 4, seed, denoise=dn) for (w, h, spp, seed, dn) in jobs]
+    for j in ids:
+        srv.wait(j, timeout=120)
+    out = [srv.read(j, with_acc=True) for j in ids]
+    for (w, h, spp, seed, dn), (img, acc) in zip(jobs, out):
+        ri, ra = solo_filtered(s, scenes.cornell_view(w, h), w, h, spp, seed) if dn else solo(s, scenes.cornell_view(w, h), w, h, spp, 4, seed)
+        assert np.array_equal(acc, ra), (w, h, spp, seed, dn)
+        assert np.array_equal(img, ri), (w, h, spp, seed, dn)
+        assert np.isfinite(img).all() and img[..., :3].sum() > 0
+    with pytest.raises(lh.LH2HIPError):
+        srv.submit(scenes.cornell_view(2, 2), 2, 2, 1, 4, 1, denoise=1)
     srv.close()
     core.Shutdown()

@@ -1,4 +1,4 @@
-"""SVGF on the GPU (SURVEY row A14) against the CPU restatement
+"""SVGF on the GPU against the CPU restatement
 (oracle/oracle_svgf.cpp), bit for bit: a three-frame sequence (fresh
 history, static camera, moving jittered camera with a specular strip that
 takes the diamond search), comparing the display image, the last-pass

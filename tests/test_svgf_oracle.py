@@ -1,4 +1,4 @@
-"""SVGF oracle (oracle/oracle_svgf.cpp, SURVEY row A14) pinned by the
+"""SVGF oracle (oracle/oracle_svgf.cpp) pinned by the
 invariants of the filter: a constant image stays constant, the history count
 grows under a static camera, a static camera reprojects every diffuse pixel
 onto itself, and the filter output stays within the range of its inputs. CPU only."""

@@ -1,4 +1,4 @@
-"""Texture maps and skydome of the shading path (SURVEY rows A15, A16): the
+"""Texture maps and skydome of the shading path: the
 oracle's texel fetch, MIP chain, sky lookup and self-written transcendental
 functions, pinned against closed forms and float64 numpy. CPU only."""
 import numpy as np
