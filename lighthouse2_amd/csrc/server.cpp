@@ -136,6 +136,13 @@ void RenderServer::loop()
 					j->frame.P.gDeltaDepth = j->gDeltaDepth.as<float4>();
 				}
 			}
+			const size_t need = trace_overflow_bytes(core.traceBlocks());
+			if (ovf.bytes < need || laneOvf.bytes < need)
+			{	/* the core's persistent block count grew (traceBlocksPerCU): grow the overflow stacks */
+				check(hipStreamSynchronize(stream), "sync");
+				check(hipStreamSynchronize(stream2), "sync");
+				ovf.ensure(need), laneOvf.ensure(need);
+			}
 			const int quantum = std::max(1, (int)std::min<size_t>(passPaths / (size_t)pixels, (size_t)(j->spp - j->next)));
 			const int s1 = std::min(j->spp, j->next + quantum);
 			RenderCore::SideLane lane;
@@ -148,6 +155,11 @@ void RenderServer::loop()
 			{
 				if (j->denoise)
 				{	/* FinalizeRender's filter path for a first frame (RenderCore_Optix7Filter/rendercore.cpp:897-935) */
+					if (!filters.count({ j->w, j->h }) && filters.size() >= 8)
+					{	/* at most 8 job sizes keep a filter */
+						check(hipStreamSynchronize(stream), "sync");
+						filters.clear();
+					}
 					auto& f = filters[{ j->w, j->h }];
 					if (!f) f.reset(new Svgf(j->w, j->h));
 					else f->Reset(stream);

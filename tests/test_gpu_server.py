@@ -116,3 +116,21 @@ def test_svgf_jobs_match_filtered_render(torch_cuda):
         srv.submit(scenes.cornell_view(2, 2), 2, 2, 1, 4, 1, denoise=1)
     srv.close()
     core.Shutdown()
+
+
+def test_more_persistent_blocks_after_server_start(torch_cuda):
+    """traceBlocksPerCU raised after the server was created: the server grows
+    its traversal overflow stacks before the next quantum; the job still
+    equals its solo render."""
+    s = scenes.cornell_box(sphere_tris=2000)
+    core = lh.RenderCore()
+    lh.load_scene(core, s)
+    srv = lh.RenderServer(core, budget_bytes=1 << 30, pass_paths=1 << 16)
+    core.Setting("traceBlocksPerCU", 12)
+    j = srv.submit(scenes.cornell_view(48, 40), 48, 40, spp=2, max_depth=4, seed=77)
+    srv.wait(j, timeout=60)
+    img, acc = srv.read(j, with_acc=True)
+    ri, ra = solo(s, scenes.cornell_view(48, 40), 48, 40, 2, 4, 77)
+    assert np.array_equal(acc, ra) and np.array_equal(img, ri)
+    srv.close()
+    core.Shutdown()
