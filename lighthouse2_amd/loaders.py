@@ -904,8 +904,21 @@ class GltfScene:
     (HostAnimation::Update) and re-placing the instances and the world-space
     lights of emissive meshes (HostNode::Update / UpdateLights)."""
 
-    def __init__(self, scene, doc, bufs, mesh_ids, node_base, root):
+    def __init__(self, scene, doc, bufs, mesh_ids, node_base, root, skin_data=None):
         self.scene, self.mesh_ids, self.node_base = scene, mesh_ids, node_base
+        # skins (HostSkin::ConvertFromGLTFSkin, host_mesh.cpp:46-66): joint nodes and
+        # inverse bind matrices, column-major in the file, transposed into row-major cells
+        self.skins = []
+        for sk in doc.get("skins", []):
+            ibm = None
+            if sk.get("inverseBindMatrices", -1) > -1:
+                ibm = gltf_accessor(doc, bufs, sk["inverseBindMatrices"]).astype(f32).reshape(-1, 4, 4).transpose(0, 2, 1).reshape(-1, 16)
+            self.skins.append((list(sk.get("joints", [])), ibm))
+        self.skinned = {}                              # mesh -> (original soup, original vertex normals, joints, weights)
+        for m, (J, Wt) in (skin_data or {}).items():
+            v, t = scene.meshes[m]
+            vn = np.stack([t["vN0"], t["vN1"], t["vN2"]], 1).reshape(-1, 3)
+            self.skinned[m] = (v.copy(), vn.astype(f32), J, Wt)
         self.docnodes = doc.get("nodes", [])
         self.state = [_node_state(n) for n in self.docnodes]
         self.root = mat4_mul(F.IDENTITY, root)
@@ -921,6 +934,45 @@ class GltfScene:
         self.lights = {}
         for i in sorted(order):                      # lights in node order (PrepareLights at construction)
             self.lights[i] = instance_lights(scene, mesh_ids[self.docnodes[i]["mesh"]], combined[i], node_base + i)
+        self._pose(combined)
+
+    def _pose(self, combined):
+        """HostNode::Update's skinning step (host_node.cpp:176-186) and
+        HostMesh::SetPose(skin) (host_mesh.cpp:717-911, its scalar form):
+        joint matrices inverse(mesh node) * joint node * inverse bind; per
+        vertex the weighted sum of its four joint matrices moves the
+        original position and vertex normal (normal renormalised), then the
+        face normals are recomputed. The reference's vector path uses
+        approximate reciprocal square roots, so skinned normals agree with it
+        to about 12 bits, not bit for bit."""
+        for i in self.order:
+            node = self.docnodes[i]
+            m = self.mesh_ids[node["mesh"]]
+            if "skin" not in node or m not in self.skinned:
+                continue
+            joints, ibm = self.skins[node["skin"]]
+            inv = np.linalg.inv(np.asarray(combined[i], np.float64).reshape(4, 4)).astype(f32).reshape(16)
+            jm = np.stack([mat4_mul(mat4_mul(inv, combined.get(jn, F.IDENTITY)), ibm[k] if ibm is not None else F.IDENTITY)
+                           for k, jn in enumerate(joints)]).reshape(-1, 4, 4)
+            v0, vn0, J, Wt = self.skinned[m]
+            S = (((Wt[:, 0, None, None] * jm[J[:, 0]] + Wt[:, 1, None, None] * jm[J[:, 1]]) + Wt[:, 2, None, None] * jm[J[:, 2]])
+                 + Wt[:, 3, None, None] * jm[J[:, 3]]).astype(f32)
+            p = np.einsum("nij,nj->ni", S.astype(np.float64), v0.astype(np.float64)).astype(f32)
+            n4 = np.concatenate([vn0, np.zeros((len(vn0), 1), f32)], 1)
+            nn = np.einsum("nij,nj->ni", S.astype(np.float64), n4.astype(np.float64))[:, :3].astype(f32)
+            nn = _normalize(nn)
+            _, t = self.scene.meshes[m]
+            t = t.copy()
+            q = p[:, :3].reshape(-1, 3, 3)
+            t["vertex0"], t["vertex1"], t["vertex2"] = q[:, 0], q[:, 1], q[:, 2]
+            with np.errstate(invalid="ignore", divide="ignore"):
+                N = _normalize(_cross((q[:, 1] - q[:, 0]).astype(f32), (q[:, 2] - q[:, 0]).astype(f32)))
+            nn = nn.reshape(-1, 3, 3)
+            t["vN0"], t["vN1"], t["vN2"] = nn[:, 0], nn[:, 1], nn[:, 2]
+            t["Nx"], t["Ny"], t["Nz"] = N[:, 0], N[:, 1], N[:, 2]
+            verts = p.copy()
+            verts[:, 3] = 1
+            self.scene.meshes[m] = (verts, t)
 
     def _local(self, i):
         st = self.state[i]
@@ -949,6 +1001,7 @@ class GltfScene:
         combined, _ = self._walk()
         for inst, i in zip(self.instances, self.order):
             self.scene.instances[inst] = (self.scene.instances[inst][0], combined[i])
+        self._pose(combined)
         for i, slot in self.lights.items():
             if slot is not None:
                 write_instance_lights(self.scene, self.mesh_ids[self.docnodes[i]["mesh"]], combined[i], self.node_base + i, *slot)
@@ -966,9 +1019,10 @@ def load_gltf(scene, path, transform=None, animated=False):
     list of instance indices. Without materials in the file every triangle
     uses scene material 0 (a grey Lambert material is added when the scene
     has none); a primitive without a material gets a grey Lambert one.
-    Skins and morph targets are not applied (the base pose is read); with
-    animated=True the GltfScene is returned, whose update(dt) plays the
-    file's translation / rotation / scale animations."""
+    Skinned meshes are posed by their skins; morph targets are not applied
+    (the base pose is read). With animated=True the GltfScene is returned,
+    whose update(dt) plays the file's translation / rotation / scale
+    animations and re-poses skinned meshes."""
     doc, bufs, directory = read_gltf(path)
     tex_ids = []
     for t in doc.get("textures", []):
@@ -991,9 +1045,9 @@ def load_gltf(scene, path, transform=None, animated=False):
                 default[0] = scene.add_material(F.lambert_material((0.5, 0.5, 0.5)))
             return default[0]
         return mat_ids[prim["material"]]
-    mesh_ids = []
+    mesh_ids, skin_data = [], {}
     for mesh in doc.get("meshes", []):
-        parts = []
+        parts, skin_parts = [], []
         for prim in mesh.get("primitives", []):
             idx = gltf_primitive_indices(doc, bufs, prim)
             if idx is None:
@@ -1008,16 +1062,25 @@ def load_gltf(scene, path, transform=None, animated=False):
             if len(idx) and (idx.min() < 0 or idx.max() >= len(P)):
                 raise ValueError("glTF indices outside the vertex range")
             parts.append(indexed_triangles(scene, idx, P, Nv, UV, UV2, prim_material(prim)))
+            if "JOINTS_0" in at and "WEIGHTS_0" in at:
+                J = gltf_accessor(doc, bufs, at["JOINTS_0"]).astype(np.int64)
+                Wt = _gltf_float_attribute(doc, bufs, at["WEIGHTS_0"], "VEC4", "WEIGHTS_0")
+                norm = f32(1) / (((Wt[:, 0] + Wt[:, 1]) + Wt[:, 2]) + Wt[:, 3])     # weights scaled to sum 1
+                skin_parts.append((J[idx.reshape(-1)], (Wt * norm[:, None]).astype(f32)[idx.reshape(-1)]))
+            else:
+                skin_parts.append(None)
         if parts:
             verts = np.concatenate([p[0] for p in parts])
             tris = np.concatenate([p[1] for p in parts])
         else:
             verts, tris = np.zeros((0, 4), f32), F.new_tris(0)
         mesh_ids.append(scene.add_mesh(verts, tris))
+        if skin_parts and all(x is not None for x in skin_parts):
+            skin_data[mesh_ids[-1]] = (np.concatenate([x[0] for x in skin_parts]), np.concatenate([x[1] for x in skin_parts]))
     node_base = getattr(scene, "node_count", len(scene.instances)) + 1
     scene.node_count = node_base + len(doc.get("nodes", []))
     root = F.IDENTITY.copy() if transform is None else np.asarray(transform, f32).reshape(16)
-    g = GltfScene(scene, doc, bufs, mesh_ids, node_base, root)
+    g = GltfScene(scene, doc, bufs, mesh_ids, node_base, root, skin_data)
     return g.instances if not animated else g
 
 

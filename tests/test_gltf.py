@@ -295,3 +295,45 @@ def test_animated_lamp_moves_its_lights(tmp_path):
     g.update(0.5)
     assert np.allclose(s.tri_lights["vertex0"][0], [0, 1.5, 0]) and np.allclose(s.tri_lights["vertex2"][0], [0, 2.5, 0])
     assert abs(s.tri_lights["area"][0] - 0.5) < 1e-6
+
+
+def test_skinned_strip_follows_a_step_rotation(tmp_path):
+    """Two joints (node 1 at the origin, node 2 one unit up), inverse binds
+    identity and translation (0, -1, 0); the top row weighted to joint 2.
+    A STEP rotation of 90 degrees about z on joint 2 takes a top vertex
+    (x, 2, 0) to T(0, 1, 0) R T(0, -1, 0) (x, 2, 0) = (-1, 1 + x, 0); the
+    lower rows stay put; face normals follow."""
+    g = GltfBuilder()
+    m = g.material(pbrMetallicRoughness={"baseColorFactor": [0.5, 0.5, 0.5, 1]})
+    P = np.float32([[-0.5, y, 0] for y in (0, 1, 2)] + [[0.5, y, 0] for y in (0, 1, 2)])
+    idx = [0, 3, 4, 0, 4, 1, 1, 4, 5, 1, 5, 2]
+    prim = g.primitive(P, idx, material=m)
+    prim["attributes"]["JOINTS_0"] = g.accessor(np.array([[0, 0, 0, 0], [0, 0, 0, 0], [1, 0, 0, 0]] * 2, "u2"))
+    prim["attributes"]["WEIGHTS_0"] = g.accessor(np.float32([[1, 0, 0, 0]] * 6))
+    g.mesh([prim])
+    ibm = np.stack([np.eye(4, dtype=f32), np.eye(4, dtype=f32)])
+    ibm[1, 1, 3] = -1
+    view = g.view(ibm.transpose(0, 2, 1).astype(f32).tobytes())          # column-major in the file
+    g.doc["accessors"].append({"bufferView": view, "componentType": 5126, "count": 2, "type": "MAT4"})
+    g.doc["skins"] = [{"joints": [1, 2], "inverseBindMatrices": len(g.doc["accessors"]) - 1}]
+    g.node(root=True, mesh=0, skin=0)
+    g.node(root=True, children=[2])
+    g.node(translation=[0, 1, 0])
+    h = np.sqrt(0.5)
+    tin = g.accessor(np.float32([0, 1, 2]))
+    rot = g.accessor(np.float32([[0, 0, 0, 1], [0, 0, h, h], [0, 0, h, h]]))
+    g.doc["animations"] = [{"samplers": [{"input": tin, "output": rot, "interpolation": "STEP"}],
+                            "channels": [{"sampler": 0, "target": {"node": 2, "path": "rotation"}}]}]
+    s = scenes.Scene()
+    anim = L.load_gltf(s, str(g.write_glb(tmp_path / "skin.glb")), animated=True)
+    v, t = s.meshes[0]
+    assert np.allclose(v[:, :3], P[idx], atol=1e-6)                      # bind pose
+    anim.update(1.5)
+    v, t = s.meshes[0]
+    want = P[idx].copy()
+    top = want[:, 1] == 2
+    want[top] = np.stack([np.full(top.sum(), -1.0), 1 + want[top, 0], np.zeros(top.sum())], 1)
+    assert np.allclose(v[:, :3], want, atol=1e-5)
+    assert np.allclose(t["vertex2"], v[2::3, :3])
+    N = np.stack([t["Nx"], t["Ny"], t["Nz"]], -1)
+    assert np.allclose(np.linalg.norm(N, axis=1), 1, atol=1e-5)

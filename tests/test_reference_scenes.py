@@ -75,15 +75,18 @@ def test_reference_scene_invariants(name, ntris, nlights):
 
 def test_strip_sample_soup_matches_its_buffers():
     """SimpleSkin's strip: 24 u16 indices, 10 float3 positions in the first
-    data: buffer; the soup must be positions[indices] (skin and animation are
-    not applied: the base pose)."""
+    data: buffer, posed by its skin at load (HostNode::Update poses skinned
+    meshes every update): both joints sit at translation (0, 1, 0) and both
+    inverse bind matrices are translations by (-0.5, -1, 0), so every joint
+    matrix is a shift by -0.5 in x and, the weights summing to one, the soup
+    is positions[indices] shifted by (-0.5, 0, 0)."""
     doc = json.load(open(os.path.join(SHARED, "strip.gltf")))
     buf = base64.b64decode(doc["buffers"][0]["uri"].split(",", 1)[1])
     idx = struct.unpack("<24H", buf[0:48])
     pos = np.array(struct.unpack("<30f", buf[48:168]), np.float32).reshape(10, 3)
     s = load("strip.gltf")
     verts, t = s.meshes[0]
-    assert np.array_equal(verts[:, :3], pos[list(idx)])
+    assert np.allclose(verts[:, :3], pos[list(idx)] + np.float32([-0.5, 0, 0]), atol=1e-6)
     assert len(s.instances) == 1 and np.array_equal(s.instances[0][1], np.eye(4, dtype=np.float32).reshape(16))
 
 
@@ -111,3 +114,16 @@ def test_reference_animations_play():
     g2 = L.load_gltf(s2, os.path.join(SHARED, "strip.gltf"), animated=True)
     g2.update(0.5)
     assert np.array_equal(s2.instances[0][1], np.eye(4, dtype=np.float32).reshape(16))   # only joint nodes animate
+
+
+def test_strip_sample_skin_follows_its_joint_animation():
+    """SimpleSkin's joint 2 rotates about z: vertices weighted to it move,
+    those weighted only to joint 1 keep their bind pose; the strip stays finite."""
+    s = scenes.Scene()
+    g = L.load_gltf(s, os.path.join(SHARED, "strip.gltf"), animated=True)
+    v0 = s.meshes[0][0].copy()
+    g.update(0.25)
+    v1 = s.meshes[0][0]
+    moved = np.abs(v1 - v0).max(1) > 1e-4
+    assert np.isfinite(v1).all() and moved.any() and not moved.all()
+    assert np.all(v0[~moved][:, 1] <= 1.0 + 1e-6)     # the unmoved vertices are the strip's lower half
