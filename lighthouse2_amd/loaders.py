@@ -840,6 +840,27 @@ class AnimationSampler:
                      5122: np.maximum(sb / f32(32767), f32(-1)), 5123: sb / f32(65535)}[ct].astype(f32)
             self.keys = v if acc["type"] == "VEC4" else v.reshape(-1)
 
+    def sample_float(self, time, k, i, count):
+        """SampleFloat (host_anim.cpp:95-118): weight i of count per key."""
+        K, spline = self.keys, self.kind == "spline"
+        if k == 0 and time < self.t[0]:
+            return K[1] if spline else K[0]
+        t0, t1 = self.t[k], self.t[k + 1]
+        f = f32(f32(f32(time) - t0) / f32(t1 - t0))
+        if f <= 0:
+            return K[0]
+        if spline:
+            t2 = f * f
+            t3 = t2 * f
+            dt = f32(t1 - t0)
+            p0, m0 = K[(k * count + i) * 3 + 1], f32(dt * K[(k * count + i) * 3 + 2])
+            p1, m1 = K[((k + 1) * count + i) * 3 + 1], f32(dt * K[((k + 1) * count + i) * 3])
+            return f32(f32(f32(m0 * f32((t3 - f32(2) * t2) + f) + p0 * f32((f32(2) * t3 - f32(3) * t2) + f32(1)))
+                           + p1 * f32(f32(-2) * t3 + f32(3) * t2)) + m1 * f32(t3 - t2))
+        if self.kind == "step":
+            return K[k]
+        return f32(f32((f32(1) - f) * K[k * count + i]) + f32(f * K[(k + 1) * count + i]))
+
     def sample(self, time, k, quat):
         """SampleVec3 / SampleQuat (:120-181)."""
         K, spline = self.keys, self.kind == "spline"
@@ -879,8 +900,25 @@ class AnimationChannel:
 
     def update(self, dt, smp, state):
         self.t = f32(self.t + f32(dt))
-        if self.path not in ("translation", "rotation", "scale") or self.node < 0:
+        if self.node < 0 or self.path not in ("translation", "rotation", "scale", "weights"):
             return False
+        if self.path == "weights":
+            w = state[self.node].get("w")
+            if w is None:
+                return False
+            n = len(smp.t)
+            dur = smp.t[n - 1]
+            if dur == 0 or n == 1:
+                w[:] = smp.keys[0]
+            else:
+                while self.t >= dur:
+                    self.t, self.k = f32(self.t - dur), 0
+                while self.t >= smp.t[(self.k + 1) % n]:
+                    self.k += 1
+                for i in range(len(w)):
+                    w[i] = smp.sample_float(self.t, self.k, i, len(w))
+            state[self.node]["morphed"] = True
+            return True
         key = {"translation": "t", "rotation": "q", "scale": "s"}[self.path]
         n = len(smp.t)
         dur = smp.t[n - 1]
@@ -904,8 +942,9 @@ class GltfScene:
     (HostAnimation::Update) and re-placing the instances and the world-space
     lights of emissive meshes (HostNode::Update / UpdateLights)."""
 
-    def __init__(self, scene, doc, bufs, mesh_ids, node_base, root, skin_data=None):
+    def __init__(self, scene, doc, bufs, mesh_ids, node_base, root, skin_data=None, morph_data=None):
         self.scene, self.mesh_ids, self.node_base = scene, mesh_ids, node_base
+        self.morph = morph_data or {}                  # mesh -> (base positions, base normals, target offsets, target normals)
         # skins (HostSkin::ConvertFromGLTFSkin, host_mesh.cpp:46-66): joint nodes and
         # inverse bind matrices, column-major in the file, transposed into row-major cells
         self.skins = []
@@ -921,6 +960,10 @@ class GltfScene:
             self.skinned[m] = (v.copy(), vn.astype(f32), J, Wt)
         self.docnodes = doc.get("nodes", [])
         self.state = [_node_state(n) for n in self.docnodes]
+        for i, n in enumerate(self.docnodes):         # morph weights start at 0 (HostNode::ConvertFromGLTFNode)
+            m = mesh_ids[n["mesh"]] if "mesh" in n else None
+            self.state[i]["w"] = np.zeros(self.morph[m][2].shape[0], f32) if m in self.morph else None
+            self.state[i]["morphed"] = False
         self.root = mat4_mul(F.IDENTITY, root)
         sc = doc.get("scenes", [])
         self.roots = sc[0].get("nodes", []) if sc else []
@@ -1001,10 +1044,37 @@ class GltfScene:
         combined, _ = self._walk()
         for inst, i in zip(self.instances, self.order):
             self.scene.instances[inst] = (self.scene.instances[inst][0], combined[i])
+        for i in self.order:
+            st = self.state[i]
+            if st["morphed"]:
+                st["morphed"] = False
+                self._morph(self.mesh_ids[self.docnodes[i]["mesh"]], st["w"])
         self._pose(combined)
         for i, slot in self.lights.items():
             if slot is not None:
                 write_instance_lights(self.scene, self.mesh_ids[self.docnodes[i]["mesh"]], combined[i], self.node_base + i, *slot)
+
+    def _morph(self, m, w):
+        """HostMesh::SetPose(weights) (host_mesh.cpp:711-738): positions =
+        base + sum of weight x target offset; vertex normals = normalise(base
+        + every target's normal offset), the offsets added unweighted as the
+        reference adds them; face normals are left as they were."""
+        bp, bn, tp, tn = self.morph[m]
+        pos = bp.copy()
+        nrm = bn.copy()
+        for j in range(len(w)):
+            pos = (pos + f32(w[j]) * tp[j]).astype(f32)
+            nrm = (nrm + tn[j]).astype(f32)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            nrm = _normalize(nrm)
+        verts, t = self.scene.meshes[m]
+        verts, t = verts.copy(), t.copy()
+        verts[:, :3] = pos
+        q = pos.reshape(-1, 3, 3)
+        n3 = nrm.reshape(-1, 3, 3)
+        t["vertex0"], t["vertex1"], t["vertex2"] = q[:, 0], q[:, 1], q[:, 2]
+        t["vN0"], t["vN1"], t["vN2"] = n3[:, 0], n3[:, 1], n3[:, 2]
+        self.scene.meshes[m] = (verts, t)
 
 
 def load_gltf(scene, path, transform=None, animated=False):
@@ -1019,10 +1089,10 @@ def load_gltf(scene, path, transform=None, animated=False):
     list of instance indices. Without materials in the file every triangle
     uses scene material 0 (a grey Lambert material is added when the scene
     has none); a primitive without a material gets a grey Lambert one.
-    Skinned meshes are posed by their skins; morph targets are not applied
-    (the base pose is read). With animated=True the GltfScene is returned,
+    Skinned meshes are posed by their skins; morph targets start at weight 0. With animated=True the GltfScene is returned,
     whose update(dt) plays the file's translation / rotation / scale
-    animations and re-poses skinned meshes."""
+    animations (morph weights included) and re-poses skinned and morphed
+    meshes."""
     doc, bufs, directory = read_gltf(path)
     tex_ids = []
     for t in doc.get("textures", []):
@@ -1045,9 +1115,9 @@ def load_gltf(scene, path, transform=None, animated=False):
                 default[0] = scene.add_material(F.lambert_material((0.5, 0.5, 0.5)))
             return default[0]
         return mat_ids[prim["material"]]
-    mesh_ids, skin_data = [], {}
+    mesh_ids, skin_data, morph_data = [], {}, {}
     for mesh in doc.get("meshes", []):
-        parts, skin_parts = [], []
+        parts, skin_parts, morph_parts = [], [], []
         for prim in mesh.get("primitives", []):
             idx = gltf_primitive_indices(doc, bufs, prim)
             if idx is None:
@@ -1062,6 +1132,18 @@ def load_gltf(scene, path, transform=None, animated=False):
             if len(idx) and (idx.min() < 0 or idx.max() >= len(P)):
                 raise ValueError("glTF indices outside the vertex range")
             parts.append(indexed_triangles(scene, idx, P, Nv, UV, UV2, prim_material(prim)))
+            nt = len(mesh.get("weights", []))
+            if nt and len(prim.get("targets", [])) >= nt:
+                # morph targets (host_mesh.cpp:462-508): per target, position and normal offsets per corner
+                flat = idx.reshape(-1)
+                tp = np.stack([_gltf_float_attribute(doc, bufs, prim["targets"][i]["POSITION"], "VEC3", "target POSITION")[flat]
+                               if "POSITION" in prim["targets"][i] else np.zeros((len(flat), 3), f32) for i in range(nt)])
+                tn = np.stack([_gltf_float_attribute(doc, bufs, prim["targets"][i]["NORMAL"], "VEC3", "target NORMAL")[flat]
+                               if "NORMAL" in prim["targets"][i] else np.zeros((len(flat), 3), f32) for i in range(nt)])
+                bn = Nv[flat] if Nv is not None else np.zeros((len(flat), 3), f32)
+                morph_parts.append((P[flat], bn, tp, tn))
+            else:
+                morph_parts.append(None)
             if "JOINTS_0" in at and "WEIGHTS_0" in at:
                 J = gltf_accessor(doc, bufs, at["JOINTS_0"]).astype(np.int64)
                 Wt = _gltf_float_attribute(doc, bufs, at["WEIGHTS_0"], "VEC4", "WEIGHTS_0")
@@ -1075,12 +1157,14 @@ def load_gltf(scene, path, transform=None, animated=False):
         else:
             verts, tris = np.zeros((0, 4), f32), F.new_tris(0)
         mesh_ids.append(scene.add_mesh(verts, tris))
+        if morph_parts and all(x is not None for x in morph_parts):
+            morph_data[mesh_ids[-1]] = tuple(np.concatenate([x[c] for x in morph_parts], axis=1 if c >= 2 else 0) for c in range(4))
         if skin_parts and all(x is not None for x in skin_parts):
             skin_data[mesh_ids[-1]] = (np.concatenate([x[0] for x in skin_parts]), np.concatenate([x[1] for x in skin_parts]))
     node_base = getattr(scene, "node_count", len(scene.instances)) + 1
     scene.node_count = node_base + len(doc.get("nodes", []))
     root = F.IDENTITY.copy() if transform is None else np.asarray(transform, f32).reshape(16)
-    g = GltfScene(scene, doc, bufs, mesh_ids, node_base, root, skin_data)
+    g = GltfScene(scene, doc, bufs, mesh_ids, node_base, root, skin_data, morph_data)
     return g.instances if not animated else g
 
 

@@ -337,3 +337,28 @@ def test_skinned_strip_follows_a_step_rotation(tmp_path):
     assert np.allclose(t["vertex2"], v[2::3, :3])
     N = np.stack([t["Nx"], t["Ny"], t["Nz"]], -1)
     assert np.allclose(np.linalg.norm(N, axis=1), 1, atol=1e-5)
+
+
+def test_morph_target_weight_animation(tmp_path):
+    """One triangle with a morph target moving every corner by (0, 0, 1) and
+    a LINEAR weights channel 0 -> 1 over one second: at t = 0.5 the corners
+    sit at z = 0.5; vertex normals are the normalised sum of base and target
+    normals (added unweighted, as the reference adds them)."""
+    g = GltfBuilder()
+    m = g.material(pbrMetallicRoughness={"baseColorFactor": [0.5, 0.5, 0.5, 1]})
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], f32)
+    prim = g.primitive(tri, [0, 1, 2], np.tile([[0, 0, 1]], (3, 1)).astype(f32), None, m)
+    prim["targets"] = [{"POSITION": g.accessor(np.tile([[0, 0, 1]], (3, 1)).astype(f32)),
+                        "NORMAL": g.accessor(np.tile([[1, 0, 0]], (3, 1)).astype(f32))}]
+    g.doc["meshes"].append({"primitives": [prim], "weights": [0.0]})
+    g.node(root=True, mesh=0)
+    tin = g.accessor(np.float32([0, 1]))
+    wout = g.accessor(np.float32([0, 1]))
+    g.doc["animations"] = [{"samplers": [{"input": tin, "output": wout}], "channels": [{"sampler": 0, "target": {"node": 0, "path": "weights"}}]}]
+    s = scenes.Scene()
+    anim = L.load_gltf(s, str(g.write_glb(tmp_path / "morph.glb")), animated=True)
+    assert np.all(s.meshes[0][0][:, 2] == 0)
+    anim.update(0.5)
+    v, t = s.meshes[0]
+    assert np.allclose(v[:, 2], 0.5) and np.allclose(t["vertex1"], [1, 0, 0.5])
+    assert np.allclose(t["vN0"], [np.sqrt(0.5), 0, np.sqrt(0.5)], atol=1e-6)

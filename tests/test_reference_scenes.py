@@ -127,3 +127,13 @@ def test_strip_sample_skin_follows_its_joint_animation():
     moved = np.abs(v1 - v0).max(1) > 1e-4
     assert np.isfinite(v1).all() and moved.any() and not moved.all()
     assert np.all(v0[~moved][:, 1] <= 1.0 + 1e-6)     # the unmoved vertices are the strip's lower half
+
+
+def test_morph_cube_weights_animate():
+    """AnimatedMorphCube: its weights channel moves the cube's vertices."""
+    s = scenes.Scene()
+    g = L.load_gltf(s, os.path.join(SHARED, "AnimatedMorphCube.glb"), animated=True)
+    v0 = s.meshes[0][0].copy()
+    g.update(0.6)
+    v1 = s.meshes[0][0]
+    assert np.isfinite(v1).all() and np.abs(v1 - v0).max() > 1e-3
