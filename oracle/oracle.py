@@ -42,6 +42,7 @@ def lib():
             "orc_svgf_frame": (None, [P, P, P, P, P, P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, I,
                                       ctypes.c_float, ctypes.c_float, I, P, I]),
             "orc_svgf_read": (None, [P, I, P]),
+            "orc_svgf_pass": (None, [P, I, P, P, P, P, P, P, I]),
             "orc_set_blue_noise": (None, [P, P, P, P, U]),
             "orc_blue_noise4": (None, [P, P, I, P]), "orc_normals": (None, [I, P, P, I]),
             "orc_disney_eval": (None, [P, P, P, P, P, P, P, I, P]), "orc_disney_sample": (None, [P, P, P, P, P, P, P, I, P]),
@@ -272,6 +273,19 @@ class OracleSvgf:
         o = np.zeros((self.h, self.w, 4), np.float32) if out is None else out
         lib().orc_svgf_frame(self.hd, _p(a), _p(f), _p(wp), _p(dd), _p(v), j[0], j[1], prev_j[0], prev_j[1], samples,
                              clamp_direct, clamp_indirect, stationary, _p(o), 1 if taa else 0)
+        return o
+
+    def apply_pass(self, phase, combined, features, world_pos, delta_depth, moments, last=False):
+        """One a-trous pass (applyFilterKernel) on the given inputs, without
+        history; returns the (h, w, 4) float32 output (combined 5:11 format,
+        or the sqrt image with last=True)."""
+        a = np.ascontiguousarray(combined, np.float32)
+        f = np.ascontiguousarray(features, np.uint32)
+        wp = np.ascontiguousarray(world_pos, np.float32)
+        dd = np.ascontiguousarray(delta_depth, np.float32)
+        m = np.ascontiguousarray(moments, np.float32)
+        o = np.zeros((self.h, self.w, 4), np.float32)
+        lib().orc_svgf_pass(self.hd, phase, _p(a), _p(f), _p(wp), _p(dd), _p(m), _p(o), 1 if last else 0)
         return o
 
     def read(self, which):

@@ -460,6 +460,29 @@ OAPI void orc_svgf_frame(void* sp, const float* acc, const uint* feat, const flo
 	std::swap(s.moments, s.prevMoments);
 }
 
+/* One a-trous pass of applyFilterKernel (finalize_shared.h:320-484) on given
+ * inputs, for closed-form checks of the edge-stopping weights. A, C: w*h
+ * float4 in the 5:11 combined format (C: sqrt image with last = 1); feat,
+ * wp, dd, moments as in orc_svgf_frame. Motion vectors point outside the
+ * image, so phase 1 takes no history and reduces to the spatial pass. */
+OAPI void orc_svgf_pass(void* sp, int phase, const float* A, const uint* feat, const float* wp, const float* dd, const float* mom, float* C, int last)
+{
+	Svgf& s = *(Svgf*)sp;
+	const int n = s.w * s.h;
+	std::vector<f4> a(n), c(n, f4{ 0, 0, 0, 0 });
+	for (int i = 0; i < n; i++)
+	{
+		s.features[i] = u4{ feat[4 * i], feat[4 * i + 1], feat[4 * i + 2], feat[4 * i + 3] };
+		s.worldPos[i] = f4{ wp[4 * i], wp[4 * i + 1], wp[4 * i + 2], wp[4 * i + 3] };
+		s.deltaDepth[i] = f4{ dd[4 * i], dd[4 * i + 1], dd[4 * i + 2], dd[4 * i + 3] };
+		s.moments[i] = f4{ mom[4 * i], mom[4 * i + 1], mom[4 * i + 2], mom[4 * i + 3] };
+		s.motion[i] = fl2{ -1e9f, -1e9f };
+		a[i] = f4{ A[4 * i], A[4 * i + 1], A[4 * i + 2], A[4 * i + 3] };
+	}
+	s.apply(phase, a, &s.filteredIn, c, last);
+	memcpy(C, c.data(), (size_t)n * 16);
+}
+
 OAPI void orc_svgf_read(void* sp, int which, void* out)
 {
 	Svgf& s = *(Svgf*)sp;
