@@ -108,6 +108,15 @@ LH2HIP_API int lh2hip_read_buffer(lh2hip_core* core, int which, void* hostOut, s
 /* CoreAPI_Base::GetCoreStats (core_api_base.h:87). */
 LH2HIP_API int lh2hip_get_stats(lh2hip_core* core, CoreStats* out);
 
+/* Headless readback for a core obtained through the plugin entry point
+ * CreateCore (lib/RenderSystem/core_api_base.cpp:119-130 loads it; the
+ * reference's cores write the image into the GL texture of SetTarget,
+ * lib/CUDA/shared_host_code/interoptexture.cpp:32-73): `core` is the
+ * CoreAPI_Base* CreateCore returned. Same output as lh2hip_read_image and
+ * lh2hip_read_accumulator_f64. */
+LH2HIP_API int lh2hip_plugin_read_image(void* core, lh2_float4* hostOut);
+LH2HIP_API int lh2hip_plugin_read_accumulator_f64(void* core, double* hostOut);
+
 /* ---- traversal queries (OptiX Prime rtpQueryExecute replacement) ---------- */
 
 /* Closest hit, RTP_QUERY_TYPE_CLOSEST (lib/RenderCore_PrimeRef/rendercore.cpp:559-573;
@@ -208,6 +217,16 @@ LH2HIP_API int lh2hip_dn_upcat(const void* coarse, int C0, const void* skip, int
 LH2HIP_API int lh2hip_dn_upcat_bwd(const void* dcat, int C0, int C1, int H, int W, void* dcoarse, void* dskip, void* stream);
 LH2HIP_API int lh2hip_dn_pack_input(const lh2_float4* radiance, const lh2_float4* albedo, const lh2_float4* normal, const float* depth,
 	int H, int W, void* out, void* stream);
+/* Network input of a filter-estimator frame: acc32 float4 [2 w h] (direct,
+ * then indirect sums), features uint4 [w h] (the SVGF G-buffer); out bf16
+ * [Hp][Wp][16] = log(1 + radiance invSpp / max(albedo, 0.02)), albedo,
+ * normal, 0.1 log(1 + min(depth, 1e4)), zero padding beyond w x h. */
+LH2HIP_API int lh2hip_dn_frame_input(const lh2_float4* acc32, const uint32_t* features, float invSpp, int w, int h, int Hp, int Wp, void* out,
+	void* stream);
+/* Denoised display image from the network's fp32 output [Hp][Wp][16]:
+ * (exp(o) - 1) max(albedo, 0.02) per channel, raised to 1/4 (the filter
+ * core's display convention: the sqrt of SVGF's sqrt last pass). */
+LH2HIP_API int lh2hip_dn_frame_output(const float* out, int Hp, int Wp, const uint32_t* features, int w, int h, lh2_float4* image, void* stream);
 LH2HIP_API int lh2hip_dn_relmse(const float* out, int Cout, const lh2_float4* target, int H, int W, void* dout, float* loss, void* stream);
 /* Adam step (g NULL: no step, only the bf16 copies); wfwd / wbwd (NULL to
  * skip): bf16 weights in the forward layout and flipped for the input gradient. */
@@ -219,10 +238,16 @@ LH2HIP_API int lh2hip_dn_adam(float* p, float* m, float* v, const float* g, uint
  * RenderThread of lib/rendercore_optix7/rendercore.cpp:803-847). A queue of
  * render jobs over the core's current scene: mixed resolutions, spp and
  * depths, admitted while their buffers (64 B per pixel) fit budgetBytes of
- * device memory, run by a worker thread in quanta of at most passPaths paths,
- * least remaining work first. A job's image equals RenderCore::Render(view,
- * Restart) with the same size, spp, depth and seed, bit for bit. Submit from
- * any thread; the scene must not change while jobs are queued. */
+ * device memory (a job that does not fit lets smaller later ones pass, at
+ * most 8 times), run by a worker thread in passes of at most passPaths paths
+ * that carry up to 16 jobs' paths together, least remaining work first. A
+ * job's image equals RenderCore::Render(view, Restart) with the same size,
+ * spp, depth and seed, bit for bit. Submit from any thread. While jobs are
+ * pending, the calls that change the scene (SetGeometry, SetInstance,
+ * FinalizeInstances, SetMaterials, SetLights, SetTextures, SetSkyData,
+ * SetBlueNoise, the nodeFormat and refit settings) return LH2HIP_ERR_STATE,
+ * and lh2hip_destroy of the core fails while a server of it exists. w * h *
+ * spp is at most 2^32 (so is SetTarget's). */
 typedef struct lh2hip_server lh2hip_server;
 LH2HIP_API int lh2hip_server_create(lh2hip_core* core, uint64_t budgetBytes, uint64_t passPaths, lh2hip_server** out);
 LH2HIP_API int lh2hip_server_destroy(lh2hip_server* server);
@@ -232,11 +257,28 @@ LH2HIP_API int lh2hip_server_submit(lh2hip_server* server, const ViewPyramid* vi
  * estimator with its own G-buffer and filtered by SVGF, its image equal to
  * RenderCore::Render(view, Restart) with the filter estimator, TAA off and
  * camera seed `seed` on a fresh core (RenderCore_Optix7Filter's
- * FinalizeRender for a first frame); budget 160 B per pixel. denoise 0 is
- * lh2hip_server_submit. No counterpart in the reference (its cores have no
- * job server). */
+ * FinalizeRender for a first frame); budget 160 B per pixel. denoise 2: the
+ * same frame denoised by the learned U-Net (lh2hip_server_set_denoiser).
+ * denoise 0 is lh2hip_server_submit. No counterpart in the reference (its
+ * cores have no job server). */
 LH2HIP_API int lh2hip_server_submit_denoised(lh2hip_server* server, const ViewPyramid* view, int width, int height, int spp,
 	int maxDepth, uint32_t seed, int denoise, uint64_t* jobId);
+/* Weights of the learned denoiser for jobs submitted with denoise 2
+ * (lh2hip_dn_weights_bytes() bytes: per layer of the U-Net in forward order,
+ * bf16 weights [9][Cout][Cin] then fp32 bias [Cout]). With denoise 2 a job is
+ * rendered as with denoise 1 and its frame goes through lh2hip_dn_frame_input,
+ * the U-Net forward pass and lh2hip_dn_frame_output instead of SVGF. No
+ * counterpart in the reference. */
+LH2HIP_API int lh2hip_server_set_denoiser(lh2hip_server* server, const void* weights, uint64_t bytes);
+LH2HIP_API uint64_t lh2hip_dn_weights_bytes(void);
+/* paused 1: the worker finishes the pass it runs and starts no other until
+ * paused 0 (a burst of jobs submitted meanwhile shares passes from the start). */
+LH2HIP_API int lh2hip_server_pause(lh2hip_server* server, int paused);
+/* Batching counters (out8): passes run, job parts over all passes, passes
+ * shared by more than one job, the most jobs in one pass, paths (eye rays),
+ * extension rays traced over all waves, and the device time of the
+ * extension-traversal and shade launches in microseconds. */
+LH2HIP_API int lh2hip_server_stats(lh2hip_server* server, uint64_t* out8);
 /* *done = 1 once the job finished (or failed: lh2hip_server_read reports it). */
 LH2HIP_API int lh2hip_server_done(lh2hip_server* server, uint64_t jobId, int* done);
 /* Blocks until the job is done; timeoutSeconds < 0 waits without limit.

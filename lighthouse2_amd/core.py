@@ -86,6 +86,14 @@ _SIGS = {
     "lh2hip_server_wait": (I, [P, ctypes.c_uint64, ctypes.c_double]),
     "lh2hip_server_read": (I, [P, ctypes.c_uint64, P, P]),
     "lh2hip_server_times": (I, [P, ctypes.c_uint64, P]),
+    "lh2hip_server_set_denoiser": (I, [P, P, ctypes.c_uint64]),
+    "lh2hip_plugin_read_image": (I, [P, P]),
+    "lh2hip_plugin_read_accumulator_f64": (I, [P, P]),
+    "lh2hip_server_stats": (I, [P, P]),
+    "lh2hip_server_pause": (I, [P, I]),
+    "lh2hip_dn_weights_bytes": (ctypes.c_uint64, []),
+    "lh2hip_dn_frame_input": (I, [P, P, ctypes.c_float, I, I, I, I, P, P]),
+    "lh2hip_dn_frame_output": (I, [P, I, I, P, I, I, P, P]),
 }
 EXPORTED = sorted(_SIGS) + ["CreateCore"]
 
@@ -152,8 +160,11 @@ class RenderCore:
 
     def SetGeometry(self, meshIdx, vertexData, triangles=None):
         v = np.ascontiguousarray(vertexData, np.float32)
-        assert v.ndim == 2 and v.shape[1] == 4 and len(v) % 3 == 0
+        if v.ndim != 2 or v.shape[1] != 4 or len(v) % 3 != 0:
+            raise ValueError("SetGeometry: vertexData must be (3n, 4) float32 (three float4 per triangle)")
         t = None if triangles is None else np.ascontiguousarray(triangles, F.CoreTri)
+        if t is not None and (t.ndim != 1 or len(t) != len(v) // 3):
+            raise ValueError(f"SetGeometry: {len(v) // 3} triangles in vertexData but {t.size} CoreTri records")
         _check(library().lh2hip_set_geometry(self._h, meshIdx, _ptr(v), len(v), len(v) // 3, _ptr(t)))
 
     def SetInstance(self, instanceIdx, meshIdx, transform=None):
@@ -278,9 +289,11 @@ class RenderServer:
         h = P()
         _check(library().lh2hip_server_create(core._h, budget_bytes, pass_paths, ctypes.byref(h)))
         self._h, self._sizes = h, {}
+        self._core = core          # the server reads the core's scene: keep the core alive as long as the server
 
     def submit(self, view, width, height, spp=1, max_depth=3, seed=0x12345678, denoise=0):
-        """denoise 1: filter estimator + SVGF for this job."""
+        """denoise 1: filter estimator + SVGF for this job; 2: the same frame
+        through the learned denoiser (set_denoiser first)."""
         v = np.ascontiguousarray(view, F.ViewPyramid)
         jid = ctypes.c_uint64()
         if denoise:
@@ -289,6 +302,25 @@ class RenderServer:
             _check(library().lh2hip_server_submit(self._h, _ptr(v), width, height, spp, max_depth, seed, ctypes.byref(jid)))
         self._sizes[jid.value] = (height, width)
         return jid.value
+
+    def set_denoiser(self, blob):
+        """Weights of the learned denoiser (UNetDenoiser.weights_blob())."""
+        b = np.ascontiguousarray(np.frombuffer(blob, np.uint8) if isinstance(blob, (bytes, bytearray)) else blob)
+        _check(library().lh2hip_server_set_denoiser(self._h, _ptr(b), b.nbytes))
+
+    def pause(self, paused=True):
+        """Paused: no new pass starts (the running one finishes) until pause(False)."""
+        _check(library().lh2hip_server_pause(self._h, 1 if paused else 0))
+
+    def stats(self):
+        """Batching counters: passes, job parts, passes shared by several jobs,
+        most jobs in one pass, paths (eye rays), extension rays traced, and
+        the device ms of the extension-traversal and shade launches."""
+        o = np.zeros(8, np.uint64)
+        _check(library().lh2hip_server_stats(self._h, _ptr(o)))
+        d = dict(zip(("passes", "parts", "shared_passes", "max_parts", "paths", "rays"), (int(x) for x in o[:6])))
+        d["trace_ms"], d["shade_ms"] = float(o[6]) * 1e-3, float(o[7]) * 1e-3
+        return d
 
     def done(self, jid):
         d = ctypes.c_int()

@@ -76,6 +76,7 @@ LH2HIP_API int lh2hip_create(lh2hip_core** out)
 LH2HIP_API int lh2hip_destroy(lh2hip_core* core)
 {
 	if (!core) return LH2HIP_OK;
+	if (core->core.serverCount.load() > 0) return fail(LH2HIP_ERR_STATE, "destroy: destroy the job servers of this core first");
 	int r = guarded(core, [](RenderCore& c) { c.Shutdown(); });
 	delete core;
 	return r;
@@ -299,6 +300,28 @@ LH2HIP_API int lh2hip_server_read(lh2hip_server* server, uint64_t jobId, lh2_flo
 	return guarded_server(server, [&](lh::RenderServer& s) { s.Read(jobId, image, acc64); });
 }
 
+LH2HIP_API int lh2hip_server_set_denoiser(lh2hip_server* server, const void* weights, uint64_t bytes)
+{
+	return guarded_server(server, [&](lh::RenderServer& s) { s.SetDenoiser(weights, (size_t)bytes); });
+}
+
+LH2HIP_API int lh2hip_server_pause(lh2hip_server* server, int paused)
+{
+	return guarded_server(server, [&](lh::RenderServer& s) { s.Pause(paused != 0); });
+}
+
+LH2HIP_API uint64_t lh2hip_dn_weights_bytes(void) { return (uint64_t)lh::UNetForward::BlobBytes(); }
+
+LH2HIP_API int lh2hip_server_stats(lh2hip_server* server, uint64_t* out8)
+{
+	return guarded_server(server, [&](lh::RenderServer& s) {
+		if (!out8) throw CoreError(LH2HIP_ERR_INVALID, "null output");
+		const lh::ServerStats t = s.Stats();
+		out8[0] = t.passes, out8[1] = t.parts, out8[2] = t.sharedPasses, out8[3] = t.maxParts, out8[4] = t.paths, out8[5] = t.rays;
+		out8[6] = (uint64_t)(t.traceMs * 1e3), out8[7] = (uint64_t)(t.shadeMs * 1e3);
+	});
+}
+
 LH2HIP_API int lh2hip_server_times(lh2hip_server* server, uint64_t jobId, double* out3)
 {
 	return guarded_server(server, [&](lh::RenderServer& s) {
@@ -382,6 +405,21 @@ LH2HIP_API int lh2hip_dn_pack_input(const lh2_float4* radiance, const lh2_float4
 		(hipStream_t)stream), "dn_pack_input");
 }
 
+LH2HIP_API int lh2hip_dn_frame_input(const lh2_float4* acc32, const uint32_t* features, float invSpp, int w, int h, int Hp, int Wp, void* out,
+	void* stream)
+{
+	if (!acc32 || !features || !out) return fail(LH2HIP_ERR_INVALID, "dn_frame_input: null pointer");
+	return dn_status(lh::dn_frame_input((const float4*)acc32, (const uint4*)features, invSpp, w, h, Hp, Wp, (lh::bf16_t*)out, (hipStream_t)stream),
+		"dn_frame_input (Hp >= h, Wp >= w)");
+}
+
+LH2HIP_API int lh2hip_dn_frame_output(const float* out, int Hp, int Wp, const uint32_t* features, int w, int h, lh2_float4* image, void* stream)
+{
+	if (!out || !features || !image) return fail(LH2HIP_ERR_INVALID, "dn_frame_output: null pointer");
+	return dn_status(lh::dn_frame_output(out, Hp, Wp, (const uint4*)features, w, h, (float4*)image, (hipStream_t)stream),
+		"dn_frame_output (Hp >= h, Wp >= w)");
+}
+
 LH2HIP_API int lh2hip_dn_relmse(const float* out, int Cout, const lh2_float4* target, int H, int W, void* dout, float* loss, void* stream)
 {
 	return dn_status(lh::dn_relmse(out, Cout, (const float4*)target, H, W, (lh::bf16_t*)dout, loss, (hipStream_t)stream), "dn_relmse");
@@ -456,6 +494,34 @@ LH2HIP_API int lh2hip_svgf_read(lh2hip_svgf* f, int which, void* hostOut)
 }
 
 } /* extern "C" */
+
+/* Headless readback for a core made by CreateCore (the plugin path has no C
+ * handle): the finalized image and the fp64 accumulator, as
+ * lh2hip_read_image / lh2hip_read_accumulator_f64 read them. */
+template <typename F>
+static int guarded_plugin(void* core, F f)
+{
+	lh::RenderCore* c = core ? dynamic_cast<lh::RenderCore*>((lighthouse2::CoreAPI_Base*)core) : nullptr;
+	if (!c) return fail(LH2HIP_ERR_INVALID, "not a core made by CreateCore");
+	try
+	{
+		f(*c);
+		return LH2HIP_OK;
+	}
+	catch (const CoreError& e) { return fail(e.code, e.what()); }
+	catch (const std::exception& e) { return fail(LH2HIP_ERR_DEVICE, e.what()); }
+}
+
+extern "C" {
+LH2HIP_API int lh2hip_plugin_read_image(void* core, lh2_float4* hostOut)
+{
+	return guarded_plugin(core, [&](RenderCore& c) { c.ReadImage(hostOut); });
+}
+LH2HIP_API int lh2hip_plugin_read_accumulator_f64(void* core, double* hostOut)
+{
+	return guarded_plugin(core, [&](RenderCore& c) { c.ReadAccumulatorF64(hostOut); });
+}
+}
 
 /* Plugin entry point: RenderSystem dlopens the core and calls Init itself
  * (lib/RenderSystem/core_api_base.cpp:119-130). */

@@ -29,8 +29,7 @@ hipError_t dn_conv3x3_cat(const bf16_t* coarse, const bf16_t* skip, const bf16_t
 /* Weight gradient: dw[9][Cout][Cin] = sum_p x[p + tap] dy[p], db[Cout] = sum_p dy[p]
  * (both overwritten). ws: workspace of dn_dw_workspace_bytes(Cin, Cout). */
 size_t dn_dw_workspace_bytes(int Cin, int Cout);
-hipError_t dn_conv3x3_dw(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, int H, int W, float* dw, float* db, void* ws, hipStream_t s);
-hipError_t dn_relu_bwd(const bf16_t* y, const bf16_t* dy, bf16_t* dyPre, size_t n, hipStream_t s);
+hipError_t dn_conv3x3_dw(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, int H, int W, float* dw, float* db, void* ws, hipStream_t s);hipError_t dn_relu_bwd(const bf16_t* y, const bf16_t* dy, bf16_t* dyPre, size_t n, hipStream_t s);
 /* 2x2 average pool (H, W even) and its gradient (dx = or += dy / 4). */
 hipError_t dn_pool2(const bf16_t* x, int C, int H, int W, bf16_t* y, hipStream_t s);
 hipError_t dn_pool2_bwd(const bf16_t* dy, int C, int H, int W, bf16_t* dx, int accumulate, hipStream_t s);
@@ -48,5 +47,17 @@ hipError_t dn_relmse(const float* out, int Cout, const float4* target, int H, in
  * wbwd[8 - tap][Cin][Cout] used by the input-gradient convolution. */
 hipError_t dn_adam(float* p, float* m, float* v, const float* g, size_t n, float lr, float b1, float b2, float eps, int t,
 	bf16_t* wfwd, bf16_t* wbwd, int Cout, int Cin, hipStream_t s);
+
+/* Network input from a filter-estimator frame: radiance = (direct + indirect)
+ * sums * invSpp, demodulated by the G-buffer albedo and range-compressed,
+ * log(1 + radiance / max(albedo, 0.02)); albedo, normal and 0.1 log(1 + depth)
+ * (depth clamped to 1e4, misses at 1e4) from the packed features; zero in the
+ * padding beyond w x h. acc32: float4 [2 w h]; features: uint4 [w h]. */
+hipError_t dn_frame_input(const float4* acc32, const uint4* features, float invSpp, int w, int h, int Hp, int Wp, bf16_t* out, hipStream_t s);
+/* Denoised image from the network output: radiance = (exp(o) - 1) *
+ * max(albedo, 0.02) per channel (o clamped at 0), written as the filter
+ * core's display value radiance^(1/4) (the sqrt of the sqrt SVGF's last pass
+ * writes), w = 0. */
+hipError_t dn_frame_output(const float* out, int Hp, int Wp, const uint4* features, int w, int h, float4* image, hipStream_t s);
 
 } // namespace lh

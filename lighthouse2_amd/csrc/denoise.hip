@@ -33,7 +33,7 @@ __device__ __forceinline__ bf16_t f2bf(float f)
 	return (bf16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
-static const int DW_BLOCKS = 256;
+static const int DW_BLOCKS = 512;
 
 /* ---- 3x3 convolution, forward and input gradient ------------------------ */
 template <int NB, bool RELU, bool F32OUT>
@@ -338,78 +338,126 @@ hipError_t dn_conv3x3(const bf16_t* x, int Cin, const bf16_t* w, const float* bi
 	}
 }
 
-/* ---- weight gradient --------------------------------------------------- */
-template <int CIN, int COUT>
-__global__ void __launch_bounds__(256) k_conv3x3_dw(const bf16_t* __restrict__ x, const bf16_t* __restrict__ dy, int H, int W,
-	float* __restrict__ partial)
+/* ---- weight gradient ---------------------------------------------------
+ * dW[tap][co][ci] = sum_p dy[p][co] x[p + tap - (1,1)][ci] as a GEMM over
+ * pixels on v_mfma_f32_16x16x32_bf16 (M = output channels, N = input
+ * channels, K = 32 pixels). Writing q = p + (dx - 1) for the input pixel, a
+ * tap pairs input row ty + r - 1 at pixels q with the output-gradient row ty
+ * at pixels q + 1 - dx, so a block keeps, per 64-pixel segment of a row:
+ * the three input rows (a ring: one new row per step) and the gradient row
+ * at its three horizontal shifts, all channel-major ([channel][pixel]) so
+ * that every MFMA fragment (8 consecutive pixels of one channel) is one
+ * aligned 16-byte LDS read. Staging lanes load 4 pixels x 8 channels with
+ * 16-byte global loads and write them transposed as 8 rows of 4 pixels; the
+ * next row's loads are in flight while the current row's products run. Nine waves, one per tap, accumulate all Cout x Cin tiles of
+ * their tap in registers over a column segment of rows; per-block partial
+ * sums are reduced in block order (deterministic). */
+typedef __bf16 bf8v_dw __attribute__((ext_vector_type(8)));
+
+/* Channel c of four pixels: v[p] holds channels 0..7 of pixel p (bf16
+ * pairs per dword); the result holds pixels 0..3 of channel c. */
+__device__ __forceinline__ uint2 column4(const uint4 (&v)[4], int c)
 {
-	constexpr int MB = COUT / 16, NBK = CIN / 16, XP = 68;
-	__shared__ __attribute__((aligned(16))) bf16_t dys[COUT][64];
-	__shared__ __attribute__((aligned(16))) bf16_t xs[3][CIN][XP];
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const int ml = lane & 15, kq = (lane >> 4) * 4;
-	/* taps of this wave: wave, wave + 4, wave + 8 */
-	f4v acc[3][MB][NBK];
+	uint32_t d[4];
 #pragma unroll
-	for (int t = 0; t < 3; t++)
+	for (int p = 0; p < 4; p++)
+		d[p] = (c >> 1) == 0 ? v[p].x : (c >> 1) == 1 ? v[p].y : (c >> 1) == 2 ? v[p].z : v[p].w;
+	if (c & 1) return make_uint2((d[0] >> 16) | (d[1] & 0xffff0000u), (d[2] >> 16) | (d[3] & 0xffff0000u));
+	return make_uint2((d[0] & 0xffffu) | (d[1] << 16), (d[2] & 0xffffu) | (d[3] << 16));
+}
+
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(576) k_conv3x3_dw(const bf16_t* __restrict__ x, const bf16_t* __restrict__ dy, int H, int W,
+	int rowsPerChunk, float* __restrict__ partial)
+{
+	constexpr int MB = COUT / 16, NB = CIN / 16, KP = 64 + 8;
+	constexpr int XL = 2 * CIN, DL = 6 * COUT;   /* staging lanes: 16 groups of 4 pixels x CIN / 8 channel groups; 3 shifts x 16 x COUT / 8 */
+	__shared__ __attribute__((aligned(16))) bf16_t xs[3][CIN][KP];
+	__shared__ __attribute__((aligned(16))) bf16_t ds[3][COUT][KP];
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int r = wave / 3, dxo = wave % 3, sh = 2 - dxo;
+	const int tilesX = (W + 63) / 64;
+	const int x0 = (int)(blockIdx.x % tilesX) * 64, y0 = (int)(blockIdx.x / tilesX) * rowsPerChunk;
+	const int y1 = y0 + rowsPerChunk < H ? y0 + rowsPerChunk : H;
+	/* staging role: x (tid < XL) or dy (XL <= tid < XL + DL) */
+	const bool isX = tid < XL, isD = !isX && tid < XL + DL;
+	const int t = isX ? tid : tid - XL;
+	const int cg = isX ? t % (CIN / 8) : t % (COUT / 8);
+	const int q = isX ? t / (CIN / 8) : (t / (COUT / 8)) % 16;
+	const int shift = isX ? 0 : t / (2 * COUT);    /* dy copy s holds pixels x0 + j + s - 1 */
+	uint4 v[4];
+	const auto load = [&](int row) {
+		const bf16_t* base = isX ? x : dy;
+		const int C = isX ? CIN : COUT;
+		const bool rowOk = row >= 0 && row < H;
 #pragma unroll
-		for (int mb = 0; mb < MB; mb++)
+		for (int p = 0; p < 4; p++)
+		{
+			const int px = x0 + 4 * q + p + (isX ? 0 : shift - 1);
+			v[p] = (rowOk && px >= 0 && px < W) ? *(const uint4*)(base + ((size_t)row * W + px) * C + cg * 8) : make_uint4(0, 0, 0, 0);
+		}
+	};
+	const auto store = [&](int slot) {
 #pragma unroll
-			for (int nb = 0; nb < NBK; nb++) acc[t][mb][nb] = (f4v){ 0.0f, 0.0f, 0.0f, 0.0f };
-	const int tilesX = (W + 63) / 64, tiles = tilesX * H;
-	for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x)
+		for (int c = 0; c < 8; c++)
+		{
+			bf16_t* dst = isX ? &xs[slot][cg * 8 + c][4 * q] : &ds[shift][cg * 8 + c][4 * q];
+			*(uint2*)dst = column4(v, c);
+		}
+	};
+	const auto slotOf = [](int row) { return (row % 3 + 3) % 3; };
+	f4v acc[MB][NB];
+#pragma unroll
+	for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+		for (int nb = 0; nb < NB; nb++) acc[mb][nb] = (f4v){ 0.0f, 0.0f, 0.0f, 0.0f };
+	if (y0 < y1)
 	{
-		const int ty = tile / tilesX, x0 = (tile % tilesX) * 64;
-		__syncthreads();
-		for (int i = threadIdx.x; i < 64 * COUT; i += 256)
-		{
-			const int px = i / COUT, c = i % COUT;
-			dys[c][px] = (x0 + px < W) ? dy[((size_t)ty * W + x0 + px) * COUT + c] : (bf16_t)0;
+		if (isX)
+		{	/* the two rows above the first step's new row */
+			load(y0 - 1), store(slotOf(y0 - 1));
+			load(y0), store(slotOf(y0));
 		}
-		for (int i = threadIdx.x; i < 3 * 66 * CIN; i += 256)
+		if (isX) load(y0 + 1);
+		else if (isD) load(y0);
+		for (int ty = y0; ty < y1; ty++)
 		{
-			const int r = i / (66 * CIN), rem = i % (66 * CIN), px = rem / CIN, c = rem % CIN;
-			const int yy = ty + r - 1, xx = x0 + px - 1;
-			xs[r][c][px] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? x[((size_t)yy * W + xx) * CIN + c] : (bf16_t)0;
-		}
-		__syncthreads();
-#pragma unroll
-		for (int t = 0; t < 3; t++)
-		{
-			const int tap = wave + 4 * t;
-			if (tap >= 9) break;
-			const int r = tap / 3, dxo = tap % 3;
-#pragma unroll
-			for (int kk = 0; kk < 4; kk++)
-			{
-				s4v a[MB];
-#pragma unroll
-				for (int mb = 0; mb < MB; mb++) a[mb] = *(const s4v*)&dys[mb * 16 + ml][kk * 16 + kq];
-#pragma unroll
-				for (int nb = 0; nb < NBK; nb++)
-				{
-					const bf16_t* src = &xs[r][nb * 16 + ml][kk * 16 + kq + dxo];
-					const s4v b = (s4v){ (short)src[0], (short)src[1], (short)src[2], (short)src[3] };
-#pragma unroll
-					for (int mb = 0; mb < MB; mb++) acc[t][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mb], b, acc[t][mb][nb], 0, 0, 0);
-				}
+			if (isX) store(slotOf(ty + 1));
+			else if (isD) store(0);
+			__syncthreads();
+			if (ty + 1 < y1)
+			{	/* the next step's rows, in flight during this step's products */
+				if (isX) load(ty + 2);
+				else if (isD) load(ty + 1);
 			}
+			const int xslot = slotOf(ty - 1 + r);
+#pragma unroll
+			for (int kc = 0; kc < 2; kc++)
+			{
+				const int koff = kc * 32 + (lane >> 4) * 8;
+				bf8v_dw A[MB], B[NB];
+#pragma unroll
+				for (int mb = 0; mb < MB; mb++) A[mb] = *(const bf8v_dw*)&ds[sh][mb * 16 + (lane & 15)][koff];
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++) B[nb] = *(const bf8v_dw*)&xs[xslot][nb * 16 + (lane & 15)][koff];
+#pragma unroll
+				for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+					for (int nb = 0; nb < NB; nb++) acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mb], B[nb], acc[mb][nb], 0, 0, 0);
+			}
+			__syncthreads();
 		}
 	}
+	/* D layout: lane holds rows 4 (lane / 16) + i, column lane % 16 */
 	float* out = partial + (size_t)blockIdx.x * 9 * COUT * CIN;
+	const int tap = wave;
 #pragma unroll
-	for (int t = 0; t < 3; t++)
-	{
-		const int tap = wave + 4 * t;
-		if (tap >= 9) break;
+	for (int mb = 0; mb < MB; mb++)
 #pragma unroll
-		for (int mb = 0; mb < MB; mb++)
+		for (int nb = 0; nb < NB; nb++)
 #pragma unroll
-			for (int nb = 0; nb < NBK; nb++)
-#pragma unroll
-				for (int i = 0; i < 4; i++)
-					out[((size_t)tap * COUT + mb * 16 + kq + i) * CIN + nb * 16 + ml] = acc[t][mb][nb][i];
-	}
+			for (int i = 0; i < 4; i++)
+				out[((size_t)tap * COUT + mb * 16 + (lane >> 4) * 4 + i) * CIN + nb * 16 + (lane & 15)] = acc[mb][nb][i];
 }
 
 /* per-block partial sums of dy over channels (bias gradient) */
@@ -447,22 +495,28 @@ __global__ void k_reduce_partials(const float* __restrict__ partial, int blocks,
 size_t dn_dw_workspace_bytes(int Cin, int Cout) { return (size_t)DW_BLOCKS * (9 * Cout * Cin + Cout) * sizeof(float); }
 
 template <int CIN, int COUT>
-static void launch_dw(const bf16_t* x, const bf16_t* dy, int H, int W, float* partial, int blocks, hipStream_t s)
+static void launch_dw(const bf16_t* x, const bf16_t* dy, int H, int W, int rowsPerChunk, float* partial, int blocks, hipStream_t s)
 {
-	k_conv3x3_dw<CIN, COUT><<<blocks, 256, 0, s>>>(x, dy, H, W, partial);
+	k_conv3x3_dw<CIN, COUT><<<blocks, 576, 0, s>>>(x, dy, H, W, rowsPerChunk, partial);
 }
 
 hipError_t dn_conv3x3_dw(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, int H, int W, float* dw, float* db, void* ws, hipStream_t s)
 {
 	if (H <= 0 || W <= 0 || !ws) return hipErrorInvalidValue;
-	const int tiles = ((W + 63) / 64) * H;
-	const int blocks = tiles < DW_BLOCKS ? tiles : DW_BLOCKS;
+	/* blocks = 64-pixel column segments x row chunks (at least 8 rows each), at most DW_BLOCKS */
+	const int tilesX = (W + 63) / 64;
+	int chunks = DW_BLOCKS / tilesX;
+	if (chunks < 1) chunks = 1;
+	if (chunks > (H + 7) / 8) chunks = (H + 7) / 8;
+	const int rowsPerChunk = (H + chunks - 1) / chunks;
+	const int blocks = tilesX * ((H + rowsPerChunk - 1) / rowsPerChunk);
+	if (blocks > DW_BLOCKS) return hipErrorInvalidValue;   /* W beyond DW_BLOCKS * 64 pixels */
 	float* partial = (float*)ws;
-	if (Cin == 16 && Cout == 32) launch_dw<16, 32>(x, dy, H, W, partial, blocks, s);
-	else if (Cin == 32 && Cout == 32) launch_dw<32, 32>(x, dy, H, W, partial, blocks, s);
-	else if (Cin == 64 && Cout == 32) launch_dw<64, 32>(x, dy, H, W, partial, blocks, s);
-	else if (Cin == 32 && Cout == 16) launch_dw<32, 16>(x, dy, H, W, partial, blocks, s);
-	else if (Cin == 16 && Cout == 16) launch_dw<16, 16>(x, dy, H, W, partial, blocks, s);
+	if (Cin == 16 && Cout == 32) launch_dw<16, 32>(x, dy, H, W, rowsPerChunk, partial, blocks, s);
+	else if (Cin == 32 && Cout == 32) launch_dw<32, 32>(x, dy, H, W, rowsPerChunk, partial, blocks, s);
+	else if (Cin == 64 && Cout == 32) launch_dw<64, 32>(x, dy, H, W, rowsPerChunk, partial, blocks, s);
+	else if (Cin == 32 && Cout == 16) launch_dw<32, 16>(x, dy, H, W, rowsPerChunk, partial, blocks, s);
+	else if (Cin == 16 && Cout == 16) launch_dw<16, 16>(x, dy, H, W, rowsPerChunk, partial, blocks, s);
 	else return hipErrorInvalidValue;
 	const int n = 9 * Cout * Cin;
 	k_reduce_partials<<<(n + 255) / 256, 256, 0, s>>>(partial, blocks, n, dw);
@@ -601,6 +655,81 @@ __global__ void k_pack_input(const float4* __restrict__ rad, const float4* __res
 hipError_t dn_pack_input(const float4* radiance, const float4* albedo, const float4* normal, const float* depth, int H, int W, bf16_t* out, hipStream_t s)
 {
 	k_pack_input<<<grid_for((size_t)H * W), 256, 0, s>>>(radiance, albedo, normal, depth, (size_t)H * W, out);
+	return hipGetLastError();
+}
+
+/* Decoders of the SVGF feature words (tools_shared.h:130-136,169-176):
+ * albedo HDRtoRGB32 (10:11:11), normal PackNormal2, depth as float bits. */
+__device__ __forceinline__ float3 feat_albedo(uint32_t c)
+{
+	return make_float3((float)(c >> 22) * (1.0f / 1023.0f), (float)((c >> 11) & 2047u) * (1.0f / 2047.0f), (float)(c & 2047u) * (1.0f / 2047.0f));
+}
+__device__ __forceinline__ float3 feat_normal(uint32_t p)
+{
+	return make_float3((float)((p >> 2) & 1023u) * (1.0f / 511.0f) - 1.0f, (float)((p >> 12) & 1023u) * (1.0f / 511.0f) - 1.0f,
+		(float)(p >> 22) * (1.0f / 511.0f) - 1.0f);
+}
+
+static const float DN_ALBEDO_FLOOR = 0.02f;
+
+__global__ void k_frame_input(const float4* __restrict__ acc, const uint4* __restrict__ feat, float invSpp, int w, int h, int Hp, int Wp,
+	bf16_t* __restrict__ out)
+{
+	const size_t pixels = (size_t)Hp * Wp;
+	for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < pixels; p += (size_t)gridDim.x * blockDim.x)
+	{
+		const int x = (int)(p % (size_t)Wp), y = (int)(p / (size_t)Wp);
+		uint4* o = (uint4*)(out + p * 16);
+		if (x >= w || y >= h)
+		{
+			o[0] = make_uint4(0, 0, 0, 0), o[1] = make_uint4(0, 0, 0, 0);
+			continue;
+		}
+		const size_t q = (size_t)x + (size_t)y * w;
+		const float4 d = acc[q], i = acc[q + (size_t)w * h];
+		const uint4 f = feat[q];
+		const float3 a = feat_albedo(f.x), n = feat_normal(f.y);
+		float depth = __uint_as_float(f.z);
+		depth = __builtin_isfinite(depth) ? fminf(depth, 1.0e4f) : 1.0e4f;
+		const float r[3] = { (d.x + i.x) * invSpp, (d.y + i.y) * invSpp, (d.z + i.z) * invSpp };
+		const float al[3] = { a.x, a.y, a.z };
+		float c[3];
+		for (int k = 0; k < 3; k++) c[k] = log1pf(fmaxf(r[k], 0.0f) / fmaxf(al[k], DN_ALBEDO_FLOOR));
+		const float dz = log1pf(depth) * 0.1f;
+		auto pk = [](float lo, float hi) { return (uint32_t)f2bf(lo) | (uint32_t)f2bf(hi) << 16; };
+		o[0] = make_uint4(pk(c[0], c[1]), pk(c[2], a.x), pk(a.y, a.z), pk(n.x, n.y));
+		o[1] = make_uint4(pk(n.z, dz), 0, 0, 0);
+	}
+}
+
+hipError_t dn_frame_input(const float4* acc32, const uint4* features, float invSpp, int w, int h, int Hp, int Wp, bf16_t* out, hipStream_t s)
+{
+	if (w <= 0 || h <= 0 || Hp < h || Wp < w) return hipErrorInvalidValue;
+	const size_t px = (size_t)Hp * Wp;
+	k_frame_input<<<grid_for(px) < 4096 ? grid_for(px) : 4096, 256, 0, s>>>(acc32, features, invSpp, w, h, Hp, Wp, out);
+	return hipGetLastError();
+}
+
+__global__ void k_frame_output(const float* __restrict__ net, int Wp, const uint4* __restrict__ feat, int w, int h, float4* __restrict__ image)
+{
+	const size_t pixels = (size_t)w * h;
+	for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < pixels; q += (size_t)gridDim.x * blockDim.x)
+	{
+		const int x = (int)(q % (size_t)w), y = (int)(q / (size_t)w);
+		const float* o = net + ((size_t)x + (size_t)y * Wp) * 16;
+		const float3 a = feat_albedo(feat[q].x);
+		const float al[3] = { a.x, a.y, a.z };
+		float v[3];
+		for (int k = 0; k < 3; k++) v[k] = sqrtf(sqrtf(expm1f(fmaxf(o[k], 0.0f)) * fmaxf(al[k], DN_ALBEDO_FLOOR)));
+		image[q] = make_float4(v[0], v[1], v[2], 0.0f);
+	}
+}
+
+hipError_t dn_frame_output(const float* out, int Hp, int Wp, const uint4* features, int w, int h, float4* image, hipStream_t s)
+{
+	if (w <= 0 || h <= 0 || Hp < h || Wp < w) return hipErrorInvalidValue;
+	const size_t px = (size_t)w * h;
+	k_frame_output<<<grid_for(px) < 4096 ? grid_for(px) : 4096, 256, 0, s>>>(out, Wp, features, w, h, image);
 	return hipGetLastError();
 }
 

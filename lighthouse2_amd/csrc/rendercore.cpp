@@ -151,7 +151,11 @@ PathBuffers Wavefront::buffers()
 }
 
 RenderCore::RenderCore() {}
-RenderCore::~RenderCore() { Shutdown(); }
+RenderCore::~RenderCore()
+{
+	try { Shutdown(); }
+	catch (...) {}   /* a server still holds the core: its owner destroys it first */
+}
 
 void RenderCore::Init()
 {
@@ -187,6 +191,7 @@ void RenderCore::Init()
 void RenderCore::Shutdown()
 {
 	if (!stream) return;
+	if (serverCount.load() > 0) throw CoreError(LH2HIP_ERR_STATE, "Shutdown: destroy the job servers of this core first");
 	(void)hipStreamSynchronize(stream);
 	(void)hipStreamSynchronize(stream2);
 	meshes.clear();
@@ -202,6 +207,12 @@ void RenderCore::Shutdown()
 	stream = stream2 = nullptr;
 }
 
+void RenderCore::requireNoJobs(const char* what) const
+{
+	if (pendingJobs.load() > 0)
+		throw CoreError(LH2HIP_ERR_STATE, std::string(what) + ": a job server has jobs pending on this scene (wait for them first)");
+}
+
 void RenderCore::SetProbePos(const int2 pos) { probePos = pos; }
 
 void RenderCore::SetTarget(GLTexture* target, const uint32_t spp)
@@ -212,8 +223,12 @@ void RenderCore::SetTarget(GLTexture* target, const uint32_t spp)
 
 void RenderCore::SetTargetSize(int w, int h, int spp)
 {
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);
 	if (w <= 0 || h <= 0 || spp <= 0 || spp > 65535 || (int64_t)w * h > (1 << 28))
 		throw CoreError(LH2HIP_ERR_INVALID, "SetTarget: bad size");
+	/* path indices (eye-ray jitter, RNG seeds, path slots) are 32-bit */
+	if ((uint64_t)w * (uint64_t)h * (uint64_t)spp > ((uint64_t)1 << 32))
+		throw CoreError(LH2HIP_ERR_INVALID, "SetTarget: width * height * spp must not exceed 2^32");
 	scrw = w, scrh = h, scrspp = spp;
 	wf.resizeTarget(w * h, stream);
 	wf.clearAccumulators(stream);
@@ -223,6 +238,8 @@ void RenderCore::SetTargetSize(int w, int h, int spp)
 void RenderCore::Setting(const char* name, float value)
 {
 	if (!name) throw CoreError(LH2HIP_ERR_INVALID, "Setting: null name");
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);
+	if (!strcmp(name, "nodeFormat") || !strcmp(name, "refit")) requireNoJobs("Setting");
 	if (!strcmp(name, "epsilon")) geometryEpsilon = value;
 	else if (!strcmp(name, "clampValue")) clampValue = value;
 	else if (!strcmp(name, "maxDepth"))
@@ -449,6 +466,8 @@ void RenderCore::layoutMesh(MeshData& m)
 
 void RenderCore::SetGeometry(const int meshIdx, const lighthouse2::float4* vertexData, const int vertexCount, const int triangleCount, const CoreTri* triangles)
 {
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);
+	requireNoJobs("SetGeometry");
 	if (meshIdx < 0 || meshIdx > (int)meshes.size()) throw CoreError(LH2HIP_ERR_INVALID, "SetGeometry: meshes must be added in sequential order");
 	if (triangleCount < 0 || vertexCount != 3 * triangleCount || (triangleCount > 0 && !vertexData))
 		throw CoreError(LH2HIP_ERR_INVALID, "SetGeometry: vertexCount must be 3 * triangleCount");
@@ -475,6 +494,8 @@ void RenderCore::SetGeometry(const int meshIdx, const lighthouse2::float4* verte
 
 void RenderCore::SetInstance(const int instanceIdx, const int modelIdx, const mat4& transform)
 {
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);
+	requireNoJobs("SetInstance");
 	if (modelIdx == -1)
 	{
 		if ((int)instances.size() > instanceIdx && instanceIdx >= 0) instances.resize(instanceIdx);
@@ -520,6 +541,8 @@ static void invert(const float* c, float* out)
 
 void RenderCore::FinalizeInstances()
 {
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);
+	requireNoJobs("FinalizeInstances");
 	instancesDirty = true;
 	updateTopLevel();
 }
@@ -707,6 +730,8 @@ static uint64_t mip_texels(uint32_t w, uint32_t h, int levels)
 
 void RenderCore::SetMaterials(CoreMaterial* mat, const int materialCount)
 {
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);
+	requireNoJobs("SetMaterials");
 	if (materialCount < 0 || (materialCount > 0 && !mat)) throw CoreError(LH2HIP_ERR_INVALID, "SetMaterials: bad arguments");
 	std::vector<GpuMaterial> g(std::max(1, materialCount));
 	memset(g.data(), 0, g.size() * sizeof(GpuMaterial));
@@ -760,6 +785,8 @@ void RenderCore::SetMaterials(CoreMaterial* mat, const int materialCount)
 void RenderCore::SetLights(const CoreLightTri* tl, const int nt, const CorePointLight* pl, const int np,
 	const CoreSpotLight* sl, const int ns, const CoreDirectionalLight* dl, const int nd)
 {
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);
+	requireNoJobs("SetLights");
 	if (nt < 0 || np < 0 || ns < 0 || nd < 0) throw CoreError(LH2HIP_ERR_INVALID, "SetLights: negative count");
 	if (nt) triLights.upload(tl, nt * sizeof(CoreLightTri), stream);
 	if (np) pointLights.upload(pl, np * sizeof(CorePointLight), stream);
@@ -770,6 +797,8 @@ void RenderCore::SetLights(const CoreLightTri* tl, const int nt, const CorePoint
 
 void RenderCore::SetTextures(const CoreTexDesc* tex, const int textureCount)
 {
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);
+	requireNoJobs("SetTextures");
 	if (textureCount < 0 || (textureCount > 0 && !tex)) throw CoreError(LH2HIP_ERR_INVALID, "SetTextures: bad arguments");
 	/* SyncStorageType (rendercore.cpp:310-355): one continuous array per storage
 	 * kind, textures in order, firstPixel = start in that array. ARGB128 texels
@@ -813,6 +842,8 @@ void RenderCore::SetTextures(const CoreTexDesc* tex, const int textureCount)
 
 void RenderCore::SetSkyData(const lighthouse2::float3* pixels, const uint32_t width, const uint32_t height, const mat4& worldToLight)
 {
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);
+	requireNoJobs("SetSkyData");
 	/* rendercore.cpp:466-478: float3 texels stored as float4 (w = 0), followed
 	 * by the 64x64-box downsampled dome the optix7 estimator samples after a
 	 * diffuse bounce (rendercore_optix7/rendercore.cpp:716-740; float sums, rows
@@ -842,6 +873,8 @@ void RenderCore::SetSkyData(const lighthouse2::float3* pixels, const uint32_t wi
 
 void RenderCore::SetBlueNoise(const uint8_t* sequence, const uint8_t* scrambling, const uint8_t* ranking)
 {
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);
+	requireNoJobs("SetBlueNoise");
 	if (!sequence || !scrambling || !ranking) throw CoreError(LH2HIP_ERR_INVALID, "SetBlueNoise: null table");
 	std::vector<uint32_t> bn(BLUE_NOISE_WORDS, 0u);
 	for (int i = 0; i < 65536; i++) bn[i] = sequence[i];
@@ -891,6 +924,7 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
 	f.estimator = estimator;
 	if (estimator == ESTIMATOR_OPTIX7 || estimator == ESTIMATOR_FILTER)
 	{
+		std::lock_guard<std::mutex> lockNoise(blueNoiseMutex);   /* job servers call this from their submitting threads */
 		if (!blueNoise.p)
 		{
 			std::vector<uint32_t> bn(BLUE_NOISE_WORDS);
@@ -938,111 +972,17 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
  * resolved into acc32/acc64 in sample order. The image depends on neither
  * the pass size nor on how the samples are split between calls: a path's
  * random numbers are keyed by its index within the frame. */
-int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
+int RenderCore::runSamples(const FrameSetup& f, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
 	uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing, const SideLane* side)
 {
-	FrameSetup f = f0;
-	const TraceScene& sc = scene;
 	const int pixels = f.pixels;
-	const bool o7 = f.estimator == ESTIMATOR_OPTIX7, fl = f.estimator == ESTIMATOR_FILTER, mis = o7 || fl;
-	const bool gbuf = f.P.gFeatures != nullptr;
-	/* optix7 cores: R0 of shade wave k is RandomUInt(camRNGseed) + k * 91771,
-	 * the seed advancing once per wave the frame runs (rendercore.cpp:881-905) */
-	uint32_t R0wave[MAXPATHLENGTH_FILTER + 1] = { 0 };
-	if (mis)
-	{
-		uint32_t cs = f.camSeed;
-		for (int k = 1; k <= f.maxDepth; k++) R0wave[k] = xor32_host(cs) + (uint32_t)k * 91771u;
-	}
-	int deepest = 0;
-	bool sidePending = false;
-	int sideTimed = 0;
-	/* the main stream waits for the shadow lane before anything that writes
-	 * what the lane reads (shade: shadow rays, potentials, path slots) or
-	 * reads what it writes (resolve: path slots) */
-	const auto joinSide = [&]() {
-		if (sidePending) hipCheck(hipStreamWaitEvent(st, side->done, 0), "wait"), sidePending = false;
-	};
 	const int passSamples = std::max(1, std::min(s1 - s0, (int)(std::min(passCap, Wavefront::PASS_CAP) / (size_t)pixels)));
 	w.resizePaths((size_t)passSamples * pixels, st);
+	int deepest = 0, sideTimed = 0;
 	for (int a = s0; a < s1; a += passSamples)
 	{
-		const int k = std::min(passSamples, s1 - a);
-		const int n = k * pixels;
-		PathBuffers B = w.buffers();
-		f.P.jobBase = (uint32_t)a * (uint32_t)pixels;
-		f.P.passPaths = (uint32_t)n;
-		if (o7) hipCheck(launch_generate7(f.cam, f.P, f.P.jobBase, n, geometryEpsilon, B, 0, st), "generate");
-		else if (fl) hipCheck(launch_generateF(f.cam, f.P, f.P.jobBase, n, geometryEpsilon, B, 0, st), "generate");
-		else hipCheck(launch_generate(f.cam, f.R0gen, f.P.jobBase, n, geometryEpsilon, B, 0, st), "generate");
-		int in = 0;
-		for (int pathLength = 1; pathLength <= f.maxDepth; pathLength++)
-		{
-			if (evs) hipCheck(hipEventRecord(evs[0], st), "event");
-			hipCheck(launch_trace_dev(sc, MODE_CLOSEST, B.rays[in], B.hits, n, B.counters + 0, queue, ovf, persistentBlocks, st), "extend");
-			if (evs) hipCheck(hipEventRecord(evs[1], st), "event");
-			f.P.pathLength = pathLength;
-			joinSide();
-			if (mis)
-			{
-				f.P.R0 = R0wave[pathLength];
-				if (o7) hipCheck(launch_shade7(f.P, B, in, n, st), "shade");
-				else hipCheck(launch_shadeF(f.P, B, in, n, st), "shade");
-			}
-			else
-			{
-				f.P.R0 = (uint32_t)f.samplesTaken * 7907u + (uint32_t)pathLength * 91771u;
-				hipCheck(launch_shade(f.P, B, in, n, st), "shade");
-			}
-			if (evs) hipCheck(hipEventRecord(evs[2], st), "event");
-			/* the reference estimator connects from every vertex but the last;
-			 * optix7's from every vertex (its NEE precedes the depth cut) */
-			if ((mis || pathLength < f.maxDepth) && side)
-			{	/* snapshot the counters (advance clears the shadow count), then
-				 * trace and connect on the side stream */
-				hipCheck(hipMemcpyAsync(side->counters, B.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "D2D");
-				hipCheck(hipEventRecord(side->shaded, st), "event");
-				hipCheck(hipStreamWaitEvent(side->st, side->shaded, 0), "wait");
-				const bool timed = side->timing && sideTimed < SIDE_TIMED;
-				if (timed) hipCheck(hipEventRecord(side->timing[2 * sideTimed], side->st), "event");
-				PathBuffers Bs = B;
-				Bs.counters = side->counters;
-				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, side->counters + 2, side->queue, side->ovf, persistentBlocks,
-					side->st), "connect");
-				hipCheck(launch_connect(Bs, n, side->st, mis ? 1.0f : 0.0f), "connect");
-				if (timed) hipCheck(hipEventRecord(side->timing[2 * sideTimed + 1], side->st), "event"), sideTimed++;
-				hipCheck(hipEventRecord(side->done, side->st), "event");
-				sidePending = true;
-			}
-			else if (mis || pathLength < f.maxDepth)
-			{
-				hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, n, B.counters + 2, queue, ovf, persistentBlocks, st), "connect");
-				hipCheck(launch_connect(B, n, st, mis ? 1.0f : 0.0f), "connect");
-			}
-			if (evs) hipCheck(hipEventRecord(evs[3], st), "event");
-			hipCheck(launch_advance(B.counters, w.countLog() ? w.countLog() + pathLength : nullptr, st), "advance");
-			in ^= 1;
-			deepest = std::max(deepest, pathLength);
-			/* stop early once every path has terminated (checked every 4 waves;
-			 * after every wave for optix7, whose loop reads the count back each time) */
-			if (mis || (pathLength & 3) == 0 || pathLength == f.maxDepth)
-			{
-				uint32_t live = 0;
-				hipCheck(hipMemcpyAsync(&live, B.counters + 0, 4, hipMemcpyDeviceToHost, st), "D2H");
-				hipCheck(hipStreamSynchronize(st), "sync");
-				if (evs && timing)
-				{
-					float a1, b2, c;
-					(void)hipEventElapsedTime(&a1, evs[0], evs[1]), (void)hipEventElapsedTime(&b2, evs[1], evs[2]), (void)hipEventElapsedTime(&c, evs[2], evs[3]);
-					timing[std::min(pathLength - 1, 2)] += a1, timing[3] += b2;
-					if (!side) timing[4] += c;
-				}
-				if (live == 0) break;
-			}
-		}
-		joinSide();
-		hipCheck(launch_resolve(B.pathAcc, pixels, k, acc32, acc64, st), "resolve");
-		if (gbuf) hipCheck(launch_resolve(B.pathAcc + n, pixels, k, acc32 + pixels, acc64 + 4 * (size_t)pixels, st), "resolve");
+		PassPart part = { &f, a, std::min(s1, a + passSamples), acc32, acc64 };
+		deepest = std::max(deepest, runPass(&part, 1, w, queue, ovf, st, evs, timing, side, nullptr, nullptr, &sideTimed));
 	}
 	if (side && side->timing && timing && sideTimed > 0)
 	{	/* shadow-lane durations, read once the lane has finished */
@@ -1056,8 +996,171 @@ int RenderCore::runSamples(const FrameSetup& f0, int s0, int s1, size_t passCap,
 	return deepest;
 }
 
+/* One pass of the wavefront loop over the paths of one or several jobs:
+ * generate -> (extend -> shade -> connect) per wave -> resolve. With several
+ * parts the jobs' paths share every launch: part i holds pass slots
+ * [slotBase_i, slotBase_i + (s1 - s0) * pixels_i), the path buffers carry
+ * pass slots, and the shade kernels look each path's job up in a segment
+ * table (segsHost staged, copied to segsDev on the stream; every pass reads
+ * a wave count back before the next pass stages its table). All parts use
+ * one estimator. A job's paths compute exactly what they compute in a pass
+ * of their own: every random number is keyed by the path's index within its
+ * job's frame, and each path slot is written by its own path only. */
+int RenderCore::runPass(const PassPart* parts, int np, Wavefront& w, uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs,
+	float* timing, const SideLane* side, const JobSeg* segsDev, JobSeg* segsHost, int* sideTimedIo)
+{
+	if (np < 1 || np > JOB_SEGS_MAX) throw CoreError(LH2HIP_ERR_INVALID, "runPass: 1..16 parts");
+	const bool multi = np > 1;
+	if (multi && (!segsDev || !segsHost)) throw CoreError(LH2HIP_ERR_INVALID, "runPass: several parts need a segment table");
+	const FrameSetup& f0 = *parts[0].f;
+	const TraceScene& sc = scene;
+	const bool o7 = f0.estimator == ESTIMATOR_OPTIX7, fl = f0.estimator == ESTIMATOR_FILTER, mis = o7 || fl;
+	uint32_t slotBase[JOB_SEGS_MAX];
+	uint32_t n = 0;
+	int maxDepthAll = 0;
+	for (int i = 0; i < np; i++)
+	{
+		const FrameSetup& f = *parts[i].f;
+		if (f.estimator != f0.estimator) throw CoreError(LH2HIP_ERR_INVALID, "runPass: parts of one estimator only");
+		slotBase[i] = n;
+		n += (uint32_t)(parts[i].s1 - parts[i].s0) * (uint32_t)f.pixels;
+		maxDepthAll = std::max(maxDepthAll, f.maxDepth);
+	}
+	if (n > Wavefront::PASS_CAP) throw CoreError(LH2HIP_ERR_INVALID, "runPass: more than 2^24 paths");
+	w.resizePaths(n, st);
+	PathBuffers B = w.buffers();
+	/* optix7 cores: R0 of shade wave k is RandomUInt(camRNGseed) + k * 91771,
+	 * the seed advancing once per wave the frame runs (rendercore.cpp:881-905) */
+	uint32_t R0wave[JOB_SEGS_MAX][MAXPATHLENGTH_FILTER + 1] = {};
+	if (mis)
+		for (int i = 0; i < np; i++)
+		{
+			uint32_t cs = parts[i].f->camSeed;
+			for (int k = 1; k <= parts[i].f->maxDepth && k <= MAXPATHLENGTH_FILTER; k++) R0wave[i][k] = xor32_host(cs) + (uint32_t)k * 91771u;
+		}
+	ShadeParams P = f0.P;
+	P.jobBase = (uint32_t)parts[0].s0 * (uint32_t)f0.pixels;
+	P.passPaths = n;
+	P.segs = nullptr, P.nseg = 0;
+	if (multi)
+	{
+		for (int i = 0; i < np; i++)
+		{
+			const FrameSetup& f = *parts[i].f;
+			JobSeg& S = segsHost[i];
+			memset(&S, 0, sizeof(S));
+			S.slotBase = slotBase[i], S.jobBase = (uint32_t)parts[i].s0 * (uint32_t)f.pixels;
+			S.w = f.P.w, S.h = f.P.h, S.pass = f.P.pass, S.maxDepth = f.P.maxDepth, S.spreadAngle = f.P.spreadAngle;
+			S.shift = f.P.shift, S.blueSlot = f.P.blueSlot, S.R0add = (uint32_t)f.samplesTaken * 7907u;
+			S.gFeatures = f.P.gFeatures, S.gWorldPos = f.P.gWorldPos, S.gDeltaDepth = f.P.gDeltaDepth;
+			S.camPos = f.P.camPos, S.camP1 = f.P.camP1, S.camRight = f.P.camRight, S.camUp = f.P.camUp;
+			for (int k = 0; k < 8; k++) S.R0w[k] = R0wave[i][k];
+		}
+		hipCheck(hipMemcpyAsync((void*)segsDev, segsHost, sizeof(JobSeg) * np, hipMemcpyHostToDevice, st), "H2D");
+		P.segs = segsDev, P.nseg = np, P.probe = nullptr;
+	}
+	for (int i = 0; i < np; i++)
+	{
+		const FrameSetup& f = *parts[i].f;
+		const int k = parts[i].s1 - parts[i].s0;
+		const uint32_t jb = (uint32_t)parts[i].s0 * (uint32_t)f.pixels, cnt = (uint32_t)k * (uint32_t)f.pixels;
+		ShadeParams Pi = f.P;
+		Pi.jobBase = jb, Pi.passPaths = n;
+		if (o7) hipCheck(launch_generate7(f.cam, Pi, jb, (int)cnt, geometryEpsilon, B, 0, st, slotBase[i], n, multi), "generate");
+		else if (fl) hipCheck(launch_generateF(f.cam, Pi, jb, (int)cnt, geometryEpsilon, B, 0, st, slotBase[i], n, multi), "generate");
+		else hipCheck(launch_generate(f.cam, f.R0gen, jb, (int)cnt, geometryEpsilon, B, 0, st, slotBase[i], n, multi), "generate");
+	}
+	int deepest = 0;
+	bool sidePending = false;
+	int sideTimed = sideTimedIo ? *sideTimedIo : 0;
+	/* the main stream waits for the shadow lane before anything that writes
+	 * what the lane reads (shade: shadow rays, potentials, path slots) or
+	 * reads what it writes (resolve: path slots) */
+	const auto joinSide = [&]() {
+		if (sidePending) hipCheck(hipStreamWaitEvent(st, side->done, 0), "wait"), sidePending = false;
+	};
+	int in = 0;
+	for (int pathLength = 1; pathLength <= maxDepthAll; pathLength++)
+	{
+		if (evs) hipCheck(hipEventRecord(evs[0], st), "event");
+		hipCheck(launch_trace_dev(sc, MODE_CLOSEST, B.rays[in], B.hits, (int)n, B.counters + 0, queue, ovf, persistentBlocks, st), "extend");
+		if (evs) hipCheck(hipEventRecord(evs[1], st), "event");
+		P.pathLength = pathLength;
+		joinSide();
+		if (mis)
+		{
+			P.R0 = R0wave[0][std::min(pathLength, MAXPATHLENGTH_FILTER)];
+			if (o7) hipCheck(launch_shade7(P, B, in, (int)n, st), "shade");
+			else hipCheck(launch_shadeF(P, B, in, (int)n, st), "shade");
+		}
+		else
+		{	/* several parts: each job adds its samplesTaken * 7907 (JobSeg::R0add) */
+			P.R0 = multi ? (uint32_t)pathLength * 91771u : (uint32_t)f0.samplesTaken * 7907u + (uint32_t)pathLength * 91771u;
+			hipCheck(launch_shade(P, B, in, (int)n, st), "shade");
+		}
+		if (evs) hipCheck(hipEventRecord(evs[2], st), "event");
+		/* the reference estimator connects from every vertex but the last;
+		 * optix7's from every vertex (its NEE precedes the depth cut) */
+		if ((mis || pathLength < maxDepthAll) && side)
+		{	/* snapshot the counters (advance clears the shadow count), then
+			 * trace and connect on the side stream */
+			hipCheck(hipMemcpyAsync(side->counters, B.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "D2D");
+			hipCheck(hipEventRecord(side->shaded, st), "event");
+			hipCheck(hipStreamWaitEvent(side->st, side->shaded, 0), "wait");
+			const bool timed = side->timing && sideTimed < SIDE_TIMED;
+			if (timed) hipCheck(hipEventRecord(side->timing[2 * sideTimed], side->st), "event");
+			PathBuffers Bs = B;
+			Bs.counters = side->counters;
+			hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, (int)n, side->counters + 2, side->queue, side->ovf, persistentBlocks,
+				side->st), "connect");
+			hipCheck(launch_connect(Bs, (int)n, side->st, mis ? 1.0f : 0.0f), "connect");
+			if (timed) hipCheck(hipEventRecord(side->timing[2 * sideTimed + 1], side->st), "event"), sideTimed++;
+			hipCheck(hipEventRecord(side->done, side->st), "event");
+			sidePending = true;
+		}
+		else if (mis || pathLength < maxDepthAll)
+		{
+			hipCheck(launch_trace_dev(sc, MODE_ANY_FLAG, B.shadowRays, B.occluded, (int)n, B.counters + 2, queue, ovf, persistentBlocks, st), "connect");
+			hipCheck(launch_connect(B, (int)n, st, mis ? 1.0f : 0.0f), "connect");
+		}
+		if (evs) hipCheck(hipEventRecord(evs[3], st), "event");
+		hipCheck(launch_advance(B.counters, w.countLog() ? w.countLog() + pathLength : nullptr, st), "advance");
+		in ^= 1;
+		deepest = std::max(deepest, pathLength);
+		/* stop early once every path has terminated (checked every 4 waves;
+		 * after every wave for optix7, whose loop reads the count back each time) */
+		if (mis || (pathLength & 3) == 0 || pathLength == maxDepthAll)
+		{
+			uint32_t live = 0;
+			hipCheck(hipMemcpyAsync(&live, B.counters + 0, 4, hipMemcpyDeviceToHost, st), "D2H");
+			hipCheck(hipStreamSynchronize(st), "sync");
+			if (evs && timing)
+			{
+				float a1, b2, c;
+				(void)hipEventElapsedTime(&a1, evs[0], evs[1]), (void)hipEventElapsedTime(&b2, evs[1], evs[2]), (void)hipEventElapsedTime(&c, evs[2], evs[3]);
+				timing[std::min(pathLength - 1, 2)] += a1, timing[3] += b2;
+				if (!side) timing[4] += c;
+			}
+			if (live == 0) break;
+		}
+	}
+	joinSide();
+	for (int i = 0; i < np; i++)
+	{
+		const FrameSetup& f = *parts[i].f;
+		const int k = parts[i].s1 - parts[i].s0;
+		hipCheck(launch_resolve(B.pathAcc + slotBase[i], f.pixels, k, parts[i].acc32, parts[i].acc64, st), "resolve");
+		if (f.P.gFeatures)
+			hipCheck(launch_resolve(B.pathAcc + n + slotBase[i], f.pixels, k, parts[i].acc32 + f.pixels, parts[i].acc64 + 4 * (size_t)f.pixels, st),
+				"resolve");
+	}
+	if (sideTimedIo) *sideTimedIo = sideTimed;
+	return deepest;
+}
+
 void RenderCore::Render(const ViewPyramid& view, const Convergence converge, bool async)
 {
+	std::unique_lock<std::shared_mutex> lockScene(sceneMutex);   /* a server's pass reads the same scene buffers */
 	if (scrw == 0) throw CoreError(LH2HIP_ERR_STATE, "Render: SetTarget first");
 	updateTopLevel();
 	if (!sceneReady) return; /* no geometry yet: nothing to render (rendercore_optix7/rendercore.cpp:821) */
@@ -1123,6 +1226,13 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 		memcpy(&stats.probedDist, &probe[2], 4);
 		float ms = 0;
 		if (filtered && hipEventElapsedTime(&ms, ev[4], ev[5]) == hipSuccess) stats.filterTime = ms * 1e-3f;
+		/* wave sizes of the (last) pass: the advance after wave k logged the size of wave k + 1 */
+		uint32_t logged[MAXPATHLENGTH_FILTER + 2] = {};
+		const int nlog = std::min(waves, MAXPATHLENGTH_FILTER + 1);
+		if (nlog > 1) hipCheck(hipMemcpy(logged, wf.countLog(), sizeof(uint32_t) * (nlog + 1), hipMemcpyDeviceToHost), "D2H");
+		stats.bounce1RayCount = nlog > 1 ? logged[1] : 0;
+		stats.deepRayCount = 0;
+		for (int k = 2; k < nlog; k++) stats.deepRayCount += logged[k];
 	}
 	const auto t1 = std::chrono::high_resolution_clock::now();
 	stats.renderTime = std::chrono::duration<float>(t1 - t0).count();

@@ -12,6 +12,9 @@
 #include <vector>
 #include <stdexcept>
 #include <memory>
+#include <mutex>
+#include <shared_mutex>
+#include <atomic>
 
 namespace lh {
 
@@ -100,6 +103,16 @@ struct FrameSetup
 
 enum { ESTIMATOR_REFERENCE = 0, ESTIMATOR_OPTIX7 = 1, ESTIMATOR_FILTER = 2 };
 
+/* One job's samples [s0, s1) in a pass of the wavefront loop, resolved into
+ * its accumulators (a pass may carry several jobs' paths, see runPass). */
+struct PassPart
+{
+	const FrameSetup* f;
+	int s0, s1;
+	float4* acc32;
+	double* acc64;
+};
+
 class Svgf;
 
 class RenderCore : public lighthouse2::CoreAPI_Base
@@ -167,6 +180,11 @@ public:
 	/* Returns the number of shade waves the deepest pass ran. */
 	int runSamples(const FrameSetup& f, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
 		uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing, const SideLane* side = nullptr);
+	/* One pass over the paths of 1..JOB_SEGS_MAX parts (at most 2^24 paths in
+	 * all, one estimator); several parts need a staging segment table on the
+	 * host and its device copy. Returns the number of waves run. */
+	int runPass(const PassPart* parts, int np, Wavefront& w, uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing,
+		const SideLane* side, const JobSeg* segsDev, JobSeg* segsHost, int* sideTimed = nullptr);
 	int estimatorSetting() const { return estimator; }
 	bool sceneIsReady() const { return sceneReady; }
 	int traceBlocks() const { return persistentBlocks; }
@@ -175,6 +193,15 @@ public:
 
 	hipStream_t stream = nullptr;
 	int deviceCUs = 0;
+	/* Job servers (server.cpp) read this core's scene from their threads
+	 * (submission and passes, shared locks); the calls that change the scene
+	 * or render take it exclusively. While a server has jobs pending the
+	 * calls that change the scene fail with LH2HIP_ERR_STATE, and the core
+	 * cannot shut down while a server exists. */
+	std::shared_mutex sceneMutex;
+	std::mutex blueNoiseMutex;
+	std::atomic<int> serverCount{ 0 }, pendingJobs{ 0 };
+	void requireNoJobs(const char* what) const;
 
 private:
 	void buildMesh(MeshData& m);

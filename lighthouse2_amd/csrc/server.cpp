@@ -2,14 +2,24 @@
  *
  * One worker thread owns a HIP stream (plus a second one for the shadow
  * lane, as in RenderCore::Render), its own path buffers and traversal
- * scratch. Jobs wait in arrival order until their per-job buffers fit the
- * HBM budget; among admitted jobs the one with the least remaining work
- * (samples x pixels) runs next, one quantum of at most passPaths paths at a
- * time, so small jobs overtake large ones (shortest remaining processing time
- * keeps p50 and p99 latency low under mixed load). A job renders exactly what
+ * scratch. Admission: waiting jobs are admitted in arrival order while their
+ * per-job buffers fit the HBM budget; a job that does not fit lets later,
+ * smaller jobs pass, until it has been passed MAX_BYPASS times, after which
+ * it holds the line until memory frees up (no starvation of large jobs).
+ * Batching: every pass of the wavefront loop carries the paths of several
+ * admitted jobs at once (up to JOB_SEGS_MAX jobs of one estimator and
+ * passPaths paths): the job with the least remaining work (samples x pixels)
+ * takes all it needs, the next ones fill the rest, whole samples each; the
+ * jobs' paths share every generate / extend / shade / connect launch and
+ * are resolved into each job's own accumulators. A job renders exactly what
  * RenderCore::Render(view, Restart) with its size, spp, depth and seed
- * renders: the quanta split its samples, and a path's random numbers depend on
- * its index in the frame only. The scene must not change while jobs run. */
+ * renders: a path's random numbers depend on its index in its job's frame
+ * only, and each path slot has one writer.
+ *
+ * The scene is shared with the core: calls that change it fail while jobs
+ * are pending (RenderCore::requireNoJobs); submission and the worker's
+ * passes hold the core's scene lock shared, the calls that change the scene
+ * (or render on the core) hold it exclusively. */
 #include "server.h"
 #include "svgf.h"
 #include "../../include/lh2hip.h"
@@ -17,6 +27,8 @@
 #include <cstring>
 
 namespace lh {
+
+static const int MAX_BYPASS = 8;
 
 static inline void check(hipError_t e, const char* what)
 {
@@ -28,6 +40,7 @@ static uint32_t xor32_seed(uint32_t s) { s ^= s << 13; s ^= s >> 17; s ^= s << 5
 
 RenderServer::RenderServer(RenderCore& c, uint64_t budgetBytes, size_t pp) : core(c), budget(budgetBytes), passPaths(std::max<size_t>(pp, 1))
 {
+	std::shared_lock<std::shared_mutex> g(core.sceneMutex);
 	if (!core.sceneIsReady()) throw CoreError(LH2HIP_ERR_STATE, "server: load and finalize a scene first");
 	core.checkRenderable();
 	check(hipGetDevice(&device), "hipGetDevice");
@@ -36,9 +49,13 @@ RenderServer::RenderServer(RenderCore& c, uint64_t budgetBytes, size_t pp) : cor
 	ovf.ensure(trace_overflow_bytes(core.traceBlocks()));
 	check(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking), "hipStreamCreate");
 	for (int i = 0; i < 2; i++) check(hipEventCreateWithFlags(&laneEv[i], hipEventDisableTiming), "hipEventCreate");
+	for (int i = 0; i < 4; i++) check(hipEventCreate(&passEv[i]), "hipEventCreate");
 	laneScratch.ensure((4 + TRACE_QUEUE_WORDS) * sizeof(uint32_t));
 	laneOvf.ensure(trace_overflow_bytes(core.traceBlocks()));
+	segs.ensure(sizeof(JobSeg) * JOB_SEGS_MAX);
+	check(hipHostMalloc((void**)&segsHost, sizeof(JobSeg) * JOB_SEGS_MAX, hipHostMallocDefault), "hipHostMalloc");
 	t0 = std::chrono::steady_clock::now();
+	core.serverCount++;
 	worker = std::thread([this] { loop(); });
 }
 
@@ -50,11 +67,16 @@ RenderServer::~RenderServer()
 	}
 	wake.notify_all();
 	if (worker.joinable()) worker.join();
+	for (auto& kv : jobs) if (!kv.second->done) core.pendingJobs--;
 	jobs.clear(), waiting.clear();
-	wf.release(), queue.release(), ovf.release(), laneScratch.release(), laneOvf.release();
+	filters.clear();
+	wf.release(), queue.release(), ovf.release(), laneScratch.release(), laneOvf.release(), segs.release();
+	if (segsHost) (void)hipHostFree(segsHost);
 	for (int i = 0; i < 2; i++) if (laneEv[i]) (void)hipEventDestroy(laneEv[i]);
+	for (int i = 0; i < 4; i++) if (passEv[i]) (void)hipEventDestroy(passEv[i]);
 	if (stream2) (void)hipStreamDestroy(stream2);
 	if (stream) (void)hipStreamDestroy(stream);
+	core.serverCount--;
 }
 
 double RenderServer::now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
@@ -63,17 +85,31 @@ uint64_t RenderServer::Submit(const lighthouse2::ViewPyramid& view, int w, int h
 {
 	if (w <= 0 || h <= 0 || spp <= 0 || maxDepth < 1 || maxDepth > 1024 || (int64_t)w * h > (1 << 24))
 		throw CoreError(LH2HIP_ERR_INVALID, "server submit: bad job size (w*h <= 2^24, spp >= 1, depth 1..1024)");
-	if (denoise != 0 && denoise != 1) throw CoreError(LH2HIP_ERR_INVALID, "server submit: denoise 0 (none) or 1 (SVGF)");
-	if (denoise && (w < 3 || h < 3)) throw CoreError(LH2HIP_ERR_INVALID, "server submit: SVGF needs at least 3 x 3 pixels");
+	if ((uint64_t)w * (uint64_t)h * (uint64_t)spp > ((uint64_t)1 << 32))
+		throw CoreError(LH2HIP_ERR_INVALID, "server submit: w * h * spp must not exceed 2^32 (path indices are 32-bit)");
+	if (denoise < DENOISE_NONE || denoise > DENOISE_LEARNED) throw CoreError(LH2HIP_ERR_INVALID, "server submit: denoise 0 (none), 1 (SVGF) or 2 (learned)");
+	if (denoise && (w < 3 || h < 3)) throw CoreError(LH2HIP_ERR_INVALID, "server submit: denoising needs at least 3 x 3 pixels");
+	if (denoise == DENOISE_LEARNED)
+	{
+		std::lock_guard<std::mutex> g(unetLock);
+		if (!unet.Ready()) throw CoreError(LH2HIP_ERR_STATE, "server submit: learned denoising needs the denoiser's weights (SetDenoiser)");
+	}
 	auto j = std::make_shared<Job>();
 	j->view = view, j->w = w, j->h = h, j->spp = spp, j->depth = maxDepth, j->seed = seed, j->denoise = denoise;
 	if (j->bytes() > budget) throw CoreError(LH2HIP_ERR_INVALID, "server submit: job needs more device memory than the budget");
-	/* the frame constants only read scene state, which must stay fixed */
-	j->frame = core.frameParams(view, w, h, spp, maxDepth, xor32_seed(seed), 0, 0.0f, 0.0f, denoise ? ESTIMATOR_FILTER : -1);
-	/* the core's G-buffer belongs to Render's target; an SVGF job gets its own
-	 * when admitted (rendercore.cpp:780-797: aperture 0 while filtering) */
+	{
+		/* the frame constants point at scene buffers: taken under the scene
+		 * lock, and the job counts as pending before the lock is released */
+		std::shared_lock<std::shared_mutex> g(core.sceneMutex);
+		j->frame = core.frameParams(view, w, h, spp, maxDepth, xor32_seed(seed), 0, 0.0f, 0.0f, denoise ? ESTIMATOR_FILTER : -1);
+		core.pendingJobs++;
+	}
+	/* the core's G-buffer belongs to Render's target; a denoised job gets its
+	 * own when it starts (rendercore.cpp:780-797: aperture 0 while filtering) */
 	j->frame.P.gFeatures = nullptr, j->frame.P.gWorldPos = nullptr, j->frame.P.gDeltaDepth = nullptr;
-	if (denoise) j->frame.cam.aperture = 0, j->frame.camSeed = seed;
+	j->frame.P.probe = nullptr;
+	j->frame.camSeed = seed;   /* Render's camera seed after Setting("seed", seed) and a restart */
+	if (denoise) j->frame.cam.aperture = 0;
 	std::lock_guard<std::mutex> g(lock);
 	j->id = nextId++;
 	j->t.queued = now();
@@ -83,59 +119,146 @@ uint64_t RenderServer::Submit(const lighthouse2::ViewPyramid& view, int w, int h
 	return j->id;
 }
 
+void RenderServer::Pause(bool p)
+{
+	{
+		std::lock_guard<std::mutex> g(lock);
+		paused = p;
+	}
+	wake.notify_all();
+}
+
+void RenderServer::SetDenoiser(const void* blob, size_t bytes)
+{
+	std::lock_guard<std::mutex> g(unetLock);
+	unet.SetWeights(blob, bytes, stream);
+}
+
 void RenderServer::admit()
 {	/* called with the lock held */
-	while (!waiting.empty() && inUse + waiting.front()->bytes() <= budget)
+	for (auto it = waiting.begin(); it != waiting.end();)
 	{
-		auto j = waiting.front();
-		waiting.pop_front();
-		inUse += j->bytes();
-		j->admitted = true;
-		j->t.admitted = now();
+		Job& j = **it;
+		if (inUse + j.bytes() <= budget)
+		{
+			inUse += j.bytes();
+			j.admitted = true;
+			j.t.admitted = now();
+			for (auto e = waiting.begin(); e != it; ++e) (*e)->bypassed++;
+			it = waiting.erase(it);
+			continue;
+		}
+		if (j.bypassed >= MAX_BYPASS) break;   /* it has waited long enough: no more overtaking */
+		++it;
 	}
+}
+
+void RenderServer::startJob(Job& j)
+{	/* per-job buffers, cleared (worker thread, on the server's stream) */
+	const int pixels = j.w * j.h;
+	const int halves = j.denoise ? 2 : 1;
+	j.acc32.ensure((size_t)halves * pixels * sizeof(float4)), j.acc64.ensure((size_t)halves * pixels * 4 * sizeof(double));
+	j.image.ensure((size_t)pixels * sizeof(float4));
+	check(hipMemsetAsync(j.acc32.p, 0, j.acc32.bytes, stream), "memset");
+	check(hipMemsetAsync(j.acc64.p, 0, j.acc64.bytes, stream), "memset");
+	if (j.denoise)
+	{	/* SVGF writes the interior pixels only: the border stays 0, as in a fresh render target */
+		check(hipMemsetAsync(j.image.p, 0, j.image.bytes, stream), "memset");
+		j.gFeatures.ensure((size_t)pixels * 16), j.gWorldPos.ensure((size_t)pixels * 16), j.gDeltaDepth.ensure((size_t)pixels * 16);
+		check(hipMemsetAsync(j.gFeatures.p, 0, j.gFeatures.bytes, stream), "memset");
+		j.frame.P.gFeatures = j.gFeatures.as<uint4>(), j.frame.P.gWorldPos = j.gWorldPos.as<float4>();
+		j.frame.P.gDeltaDepth = j.gDeltaDepth.as<float4>();
+	}
+	j.started = true;
+}
+
+void RenderServer::finishJob(Job& j)
+{	/* all samples resolved: denoise or finalize, copy to the host */
+	const int pixels = j.w * j.h;
+	if (j.denoise == DENOISE_SVGF)
+	{	/* FinalizeRender's filter path for a first frame (RenderCore_Optix7Filter/rendercore.cpp:897-935) */
+		if (!filters.count({ j.w, j.h }) && filters.size() >= 8)
+		{	/* at most 8 job sizes keep a filter */
+			check(hipStreamSynchronize(stream), "sync");
+			filters.clear();
+		}
+		auto& f = filters[{ j.w, j.h }];
+		if (!f) f.reset(new Svgf(j.w, j.h));
+		else f->Reset(stream);
+		f->taa = 0;
+		const lighthouse2::ViewPyramid none = {};
+		f->Frame(j.acc32.as<float4>(), j.gFeatures.as<uint4>(), j.gWorldPos.as<float4>(), j.gDeltaDepth.as<float4>(), none,
+			0.0f, 0.0f, 0.0f, 0.0f, j.spp, core.clampDirectSetting(), core.clampIndirectSetting(), 0, j.image.as<float4>(), stream);
+	}
+	else if (j.denoise == DENOISE_LEARNED)
+	{	/* the U-Net on the job's frame, padded to multiples of 16 */
+		std::lock_guard<std::mutex> g(unetLock);
+		const int Hp = (j.h + 15) & ~15, Wp = (j.w + 15) & ~15;
+		bf16_t* in = unet.Input(Hp, Wp);
+		check(dn_frame_input(j.acc32.as<float4>(), j.gFeatures.as<uint4>(), 1.0f / (float)j.spp, j.w, j.h, Hp, Wp, in, stream), "denoiser input");
+		const float* out = unet.Forward(stream);
+		check(dn_frame_output(out, Hp, Wp, j.gFeatures.as<uint4>(), j.w, j.h, j.image.as<float4>(), stream), "denoiser output");
+	}
+	else check(launch_finalize(j.acc32.as<float4>(), pixels, 1.0f / (float)j.spp, j.image.as<float4>(), stream), "finalize");
+	const size_t accWords = (size_t)pixels * 4;   /* denoised jobs: the direct half, as RenderCore::ReadAccumulatorF64 */
+	j.hostImage.resize((size_t)pixels * 4), j.hostAcc.resize(accWords);
+	check(hipMemcpyAsync(j.hostImage.data(), j.image.p, j.image.bytes, hipMemcpyDeviceToHost, stream), "D2H");
+	check(hipMemcpyAsync(j.hostAcc.data(), j.acc64.p, accWords * sizeof(double), hipMemcpyDeviceToHost, stream), "D2H");
+	check(hipStreamSynchronize(stream), "sync");
+}
+
+void RenderServer::releaseJob(Job& j)
+{
+	j.acc32.release(), j.acc64.release(), j.image.release();
+	j.gFeatures.release(), j.gWorldPos.release(), j.gDeltaDepth.release();
 }
 
 void RenderServer::loop()
 {
 	check(hipSetDevice(device), "hipSetDevice");
+	std::vector<std::shared_ptr<Job>> batch;
+	std::vector<PassPart> parts;
 	while (true)
 	{
-		std::shared_ptr<Job> j;
+		batch.clear(), parts.clear();
 		{
 			std::unique_lock<std::mutex> g(lock);
 			wake.wait(g, [&] {
 				if (stopping) return true;
+				if (paused) return false;
 				admit();
 				for (auto& kv : jobs) if (kv.second->admitted && !kv.second->done) return true;
 				return false;
 			});
 			if (stopping) return;
-			for (auto& kv : jobs)
+			/* least remaining work first; the pass takes jobs of the first one's estimator */
+			std::vector<std::shared_ptr<Job>> ready;
+			for (auto& kv : jobs) if (kv.second->admitted && !kv.second->done) ready.push_back(kv.second);
+			std::stable_sort(ready.begin(), ready.end(), [](const std::shared_ptr<Job>& a, const std::shared_ptr<Job>& b) {
+				return a->remaining() < b->remaining();
+			});
+			const int est = ready[0]->estimator();
+			size_t room = std::min(passPaths, Wavefront::PASS_CAP);
+			for (auto& c : ready)
 			{
-				Job* c = kv.second.get();
-				if (!c->admitted || c->done) continue;
-				if (!j || c->remaining() < j->remaining()) j = kv.second;
+				if ((int)batch.size() == JOB_SEGS_MAX) break;
+				if (c->estimator() != est) continue;
+				const size_t px = (size_t)c->w * c->h;
+				int k = (int)std::min<size_t>((size_t)(c->spp - c->next), room / px);
+				if (k == 0)
+				{
+					if (!batch.empty()) continue;
+					k = 1;   /* one sample larger than a pass: it runs alone */
+				}
+				batch.push_back(c);
+				room -= std::min(room, (size_t)k * px);
+				parts.push_back(PassPart{ &c->frame, c->next, c->next + k, nullptr, nullptr });
 			}
 		}
+		std::vector<Job*> completed;
 		try
 		{
-			const int pixels = j->w * j->h;
-			const int halves = j->denoise ? 2 : 1;
-			if (j->next == 0)
-			{	/* first quantum: per-job buffers, cleared */
-				j->acc32.ensure((size_t)halves * pixels * sizeof(float4)), j->acc64.ensure((size_t)halves * pixels * 4 * sizeof(double));
-				j->image.ensure((size_t)pixels * sizeof(float4));
-				check(hipMemsetAsync(j->acc32.p, 0, j->acc32.bytes, stream), "memset");
-				check(hipMemsetAsync(j->acc64.p, 0, j->acc64.bytes, stream), "memset");
-				if (j->denoise)
-				{	/* SVGF writes the interior pixels only: the border stays 0, as in a fresh render target */
-					check(hipMemsetAsync(j->image.p, 0, j->image.bytes, stream), "memset");
-					j->gFeatures.ensure((size_t)pixels * 16), j->gWorldPos.ensure((size_t)pixels * 16), j->gDeltaDepth.ensure((size_t)pixels * 16);
-					check(hipMemsetAsync(j->gFeatures.p, 0, j->gFeatures.bytes, stream), "memset");
-					j->frame.P.gFeatures = j->gFeatures.as<uint4>(), j->frame.P.gWorldPos = j->gWorldPos.as<float4>();
-					j->frame.P.gDeltaDepth = j->gDeltaDepth.as<float4>();
-				}
-			}
+			std::shared_lock<std::shared_mutex> sg(core.sceneMutex);
 			const size_t need = trace_overflow_bytes(core.traceBlocks());
 			if (ovf.bytes < need || laneOvf.bytes < need)
 			{	/* the core's persistent block count grew (traceBlocksPerCU): grow the overflow stacks */
@@ -143,52 +266,62 @@ void RenderServer::loop()
 				check(hipStreamSynchronize(stream2), "sync");
 				ovf.ensure(need), laneOvf.ensure(need);
 			}
-			const int quantum = std::max(1, (int)std::min<size_t>(passPaths / (size_t)pixels, (size_t)(j->spp - j->next)));
-			const int s1 = std::min(j->spp, j->next + quantum);
+			for (size_t i = 0; i < batch.size(); i++)
+			{
+				if (!batch[i]->started) startJob(*batch[i]);
+				parts[i].f = &batch[i]->frame;
+				parts[i].acc32 = batch[i]->acc32.as<float4>(), parts[i].acc64 = batch[i]->acc64.as<double>();
+			}
 			RenderCore::SideLane lane;
 			lane.st = stream2, lane.counters = laneScratch.as<uint32_t>(), lane.queue = laneScratch.as<uint32_t>() + 4;
 			lane.ovf = laneOvf.as<uint32_t>(), lane.shaded = laneEv[0], lane.done = laneEv[1];
-			core.runSamples(j->frame, j->next, s1, passPaths, wf, j->acc32.as<float4>(), j->acc64.as<double>(),
-				queue.as<uint32_t>(), ovf.as<uint32_t>(), stream, nullptr, nullptr, core.shadowLaneOn() ? &lane : nullptr);
-			j->next = s1;
-			if (j->next == j->spp)
+			float timing[5] = { 0, 0, 0, 0, 0 };
+			const int waves = core.runPass(parts.data(), (int)parts.size(), wf, queue.as<uint32_t>(), ovf.as<uint32_t>(), stream, passEv, timing,
+				core.shadowLaneOn() ? &lane : nullptr, segs.as<JobSeg>(), segsHost);
+			/* wave sizes: the advance after wave k logged the size of wave k + 1 */
+			std::vector<uint32_t> logged((size_t)waves + 1, 0u);
+			check(hipMemcpyAsync(logged.data(), wf.countLog(), sizeof(uint32_t) * logged.size(), hipMemcpyDeviceToHost, stream), "D2H");
+			check(hipStreamSynchronize(stream), "sync");
+			uint64_t paths = 0, rays = 0;
+			for (int k = 1; k < waves; k++) rays += logged[k];
+			for (size_t i = 0; i < batch.size(); i++)
 			{
-				if (j->denoise)
-				{	/* FinalizeRender's filter path for a first frame (RenderCore_Optix7Filter/rendercore.cpp:897-935) */
-					if (!filters.count({ j->w, j->h }) && filters.size() >= 8)
-					{	/* at most 8 job sizes keep a filter */
-						check(hipStreamSynchronize(stream), "sync");
-						filters.clear();
-					}
-					auto& f = filters[{ j->w, j->h }];
-					if (!f) f.reset(new Svgf(j->w, j->h));
-					else f->Reset(stream);
-					f->taa = 0;
-					const lighthouse2::ViewPyramid none = {};
-					f->Frame(j->acc32.as<float4>(), j->gFeatures.as<uint4>(), j->gWorldPos.as<float4>(), j->gDeltaDepth.as<float4>(), none,
-						0.0f, 0.0f, 0.0f, 0.0f, j->spp, core.clampDirectSetting(), core.clampIndirectSetting(), 0, j->image.as<float4>(), stream);
-					j->gFeatures.release(), j->gWorldPos.release(), j->gDeltaDepth.release();
+				Job& j = *batch[i];
+				paths += (uint64_t)(parts[i].s1 - parts[i].s0) * (uint64_t)j.w * j.h;
+				j.next = parts[i].s1;
+				if (j.next == j.spp)
+				{
+					try { finishJob(j); }
+					catch (const std::exception& e) { j.error = e.what(); }
+					completed.push_back(&j);
 				}
-				else check(launch_finalize(j->acc32.as<float4>(), pixels, 1.0f / (float)j->spp, j->image.as<float4>(), stream), "finalize");
-				j->hostImage.resize((size_t)pixels * 4), j->hostAcc.resize((size_t)pixels * 4);
-				check(hipMemcpyAsync(j->hostImage.data(), j->image.p, j->image.bytes, hipMemcpyDeviceToHost, stream), "D2H");
-				check(hipMemcpyAsync(j->hostAcc.data(), j->acc64.p, (size_t)pixels * 4 * sizeof(double), hipMemcpyDeviceToHost, stream), "D2H");
-				check(hipStreamSynchronize(stream), "sync");
-				j->acc32.release(), j->acc64.release(), j->image.release();
-				std::lock_guard<std::mutex> g(lock);
-				j->done = true;
-				j->t.done = now();
-				inUse -= j->bytes();
 			}
+			std::lock_guard<std::mutex> g(lock);
+			stats.passes++, stats.parts += batch.size(), stats.paths += paths, stats.rays += paths + rays;
+			stats.traceMs += timing[0] + timing[1] + timing[2], stats.shadeMs += timing[3];
+			if (batch.size() > 1) stats.sharedPasses++;
+			stats.maxParts = std::max<uint64_t>(stats.maxParts, batch.size());
 		}
 		catch (const std::exception& e)
-		{
-			std::lock_guard<std::mutex> g(lock);
-			j->error = e.what(), j->done = true, j->t.done = now();
-			j->acc32.release(), j->acc64.release(), j->image.release();
-			inUse -= j->bytes();
+		{	/* the pass failed: every job in it fails */
+			completed.clear();
+			for (auto& c : batch) if (!c->done) c->error = e.what(), completed.push_back(c.get());
 		}
-		if (j->done) finished.notify_all();
+		if (!completed.empty())
+		{
+			for (Job* j : completed) releaseJob(*j);
+			{
+				std::lock_guard<std::mutex> g(lock);
+				for (Job* j : completed)
+				{
+					j->done = true;
+					j->t.done = now();
+					inUse -= j->bytes();
+					core.pendingJobs--;
+				}
+			}
+			finished.notify_all();
+		}
 	}
 }
 
@@ -238,6 +371,12 @@ uint64_t RenderServer::BytesInUse()
 {
 	std::lock_guard<std::mutex> g(lock);
 	return inUse;
+}
+
+ServerStats RenderServer::Stats()
+{
+	std::lock_guard<std::mutex> g(lock);
+	return stats;
 }
 
 } // namespace lh

@@ -4,6 +4,7 @@
  * asynchronous Render + RenderThread of lib/rendercore_optix7/rendercore.cpp:803-847. */
 #pragma once
 #include "rendercore.h"
+#include "unet.h"
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -15,26 +16,43 @@ namespace lh {
 
 struct JobTimes { double queued, admitted, done; };   /* seconds since the server started */
 
+/* Counters of the batching: passes run, job parts over all passes, passes
+ * that carried more than one job, the most jobs one pass carried, paths
+ * (eye rays) over all passes, extension rays traced (every wave), and the
+ * device time of the extension traversal and shade launches (ms). */
+struct ServerStats { uint64_t passes, parts, sharedPasses, maxParts, paths, rays; double traceMs, shadeMs; };
+
+enum { DENOISE_NONE = 0, DENOISE_SVGF = 1, DENOISE_LEARNED = 2 };
+
 class RenderServer
 {
 public:
 	/* budgetBytes: device memory the per-job buffers of admitted jobs may hold
-	 * (accumulators + image, 64 B per pixel); passPaths: paths per scheduling
-	 * quantum (a job runs at most this many paths before another job may run). */
+	 * (accumulators + image: 64 B per pixel; denoised jobs 160 B per pixel);
+	 * passPaths: paths per pass (the admitted jobs with the least remaining
+	 * work share a pass up to this many paths; a job whose single sample is
+	 * larger runs a pass of its own). */
 	RenderServer(RenderCore& core, uint64_t budgetBytes, size_t passPaths);
 	~RenderServer();
 	/* denoise 0: the core's estimator, image = accumulator / spp; 1: the
 	 * filter estimator (G-buffer per job) and SVGF on the job's samples, as
-	 * RenderCore::Render draws a first frame with the filter on and TAA off. */
+	 * RenderCore::Render draws a first frame with the filter on and TAA off;
+	 * 2: the same frame denoised by the learned U-Net (SetDenoiser). */
 	uint64_t Submit(const lighthouse2::ViewPyramid& view, int w, int h, int spp, int maxDepth, uint32_t seed, int denoise = 0);
+	/* Paused: the worker finishes its pass and starts no other until resumed
+	 * (jobs submitted meanwhile then share passes from the start). */
+	void Pause(bool paused);
+	/* Weights of the learned denoiser (UNetForward::BlobBytes() bytes). */
+	void SetDenoiser(const void* blob, size_t bytes);
 	bool Done(uint64_t id);
 	/* Blocks until the job is done (timeout < 0: no limit); false on timeout. */
 	bool Wait(uint64_t id, double timeoutSeconds);
 	/* Copies the final image (w*h float4) and, optionally, the fp64 accumulator
-	 * (w*h*4 doubles), then forgets the job. */
+	 * (w*h*4 doubles; denoised jobs: the direct half), then forgets the job. */
 	void Read(uint64_t id, void* image, double* acc64);
 	JobTimes Times(uint64_t id);
 	uint64_t BytesInUse();
+	ServerStats Stats();
 
 private:
 	struct Job
@@ -44,20 +62,25 @@ private:
 		int w, h, spp, depth, denoise = 0;
 		uint32_t seed;
 		int next = 0;                   /* next sample to render */
-		bool admitted = false, done = false;
+		int bypassed = 0;               /* later jobs admitted ahead of this one */
+		bool admitted = false, started = false, done = false;
 		std::string error;
 		DevBuf acc32, acc64, image, gFeatures, gWorldPos, gDeltaDepth;
 		std::vector<float> hostImage;
 		std::vector<double> hostAcc;
 		JobTimes t = { 0, 0, 0 };
 		FrameSetup frame;
-		/* accumulators + image: 64 B/px; with SVGF, direct and indirect halves
+		/* accumulators + image: 64 B/px; denoised: direct and indirect halves
 		 * (another 48 B/px) and the G-buffer (48 B/px) */
 		uint64_t bytes() const { return (uint64_t)w * h * (denoise ? 160 : 64); }
 		uint64_t remaining() const { return (uint64_t)(spp - next) * (uint64_t)w * h; }
+		int estimator() const { return frame.estimator; }
 	};
 	void loop();
 	void admit();
+	void startJob(Job& j);
+	void finishJob(Job& j);
+	void releaseJob(Job& j);
 	double now() const;
 	RenderCore& core;
 	uint64_t budget;
@@ -67,13 +90,18 @@ private:
 	std::deque<std::shared_ptr<Job>> waiting;            /* not yet admitted, arrival order */
 	std::map<uint64_t, std::shared_ptr<Job>> jobs;       /* every job not yet read */
 	uint64_t nextId = 1, inUse = 0;
-	bool stopping = false;
+	bool stopping = false, paused = false;
+	ServerStats stats = { 0, 0, 0, 0, 0, 0, 0.0, 0.0 };
+	hipEvent_t passEv[4] = { nullptr, nullptr, nullptr, nullptr };   /* per-wave timing of the pass launches */
 	hipStream_t stream = nullptr, stream2 = nullptr;   /* stream2: the shadow lane */
 	hipEvent_t laneEv[2] = { nullptr, nullptr };
 	int device = 0;
 	Wavefront wf;
 	std::map<std::pair<int, int>, std::unique_ptr<Svgf>> filters;   /* per job size, reused */
-	DevBuf queue, ovf, laneScratch, laneOvf;
+	DevBuf queue, ovf, laneScratch, laneOvf, segs;
+	JobSeg* segsHost = nullptr;                          /* pinned staging of the pass's segment table */
+	UNetForward unet;                                   /* learned denoiser (worker thread only after SetDenoiser) */
+	std::mutex unetLock;
 	std::chrono::steady_clock::time_point t0;
 	std::thread worker;
 };

@@ -24,6 +24,26 @@ enum MaterialFlags
 	MF_HASSMOOTHNORMALS = 2048, MF_HASALPHA = 4096
 };
 
+/* One job's share of a pass that several render jobs share (the job
+ * server's batches): its paths hold path slots [slotBase, slotBase + n) of
+ * the pass, slot slotBase being path jobBase of the job's frame; the rest is
+ * what the job's own ShadeParams would hold. */
+struct JobSeg
+{
+	uint32_t slotBase, jobBase;
+	int w, h, pass, maxDepth;
+	float spreadAngle;
+	uint32_t shift;               /* optix7 blue-noise shift */
+	int blueSlot;
+	uint32_t R0add;               /* reference estimator: added to the pass-wide R0 (samplesTaken * 7907) */
+	uint4* gFeatures;
+	float4* gWorldPos;
+	float4* gDeltaDepth;
+	float4 camPos, camP1, camRight, camUp;
+	uint32_t R0w[8];              /* optix7 / filter estimators: shade seed per wave (index pathLength) */
+};
+static const int JOB_SEGS_MAX = 16;
+
 /* Per-wave constants of the shade kernel (replaces the reference's
  * __constant__ globals, lib/RenderCore_PrimeRef/kernels/.cuda.cu:20-45, so
  * several jobs can be in flight). */
@@ -68,6 +88,10 @@ struct ShadeParams
 	int blueSlot;
 	uint32_t passPaths;
 	float4 camPos, camP1, camRight, camUp;
+	/* several jobs in one pass: nseg > 0 segments (device table, ascending
+	 * slotBase); the path buffers then hold pass slots instead of path indices */
+	const JobSeg* segs;
+	int nseg;
 };
 
 /* Path-state buffers of one pass (SoA). */
@@ -85,7 +109,12 @@ struct PathBuffers
 	float2* aux[2];        /* optix7 estimator: postponed bsdf pdf, packed normal of the last vertex */
 };
 
-hipError_t launch_generate(const CamParams& c, uint32_t R0, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s);
+/* Eye rays of paths jobBase .. jobBase + n - 1 of a frame into pass slots
+ * slotBase .. slotBase + n - 1 (total: paths of the whole pass, 0 = n; the
+ * launch with slotBase 0 resets the wave counters to it). storeSlot: the path
+ * buffers hold the pass slot (passes shared by several jobs), else the path index. */
+hipError_t launch_generate(const CamParams& c, uint32_t R0, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s,
+	uint32_t slotBase = 0, uint32_t total = 0, bool storeSlot = false);
 hipError_t launch_shade(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s);
 /* Adds each unoccluded connection to its path slot; countW is added to w
  * (optix7's connections count 1 there, rendercore_optix7/optix/.optix.cu:153). */
@@ -96,7 +125,8 @@ hipError_t launch_set_u32(uint32_t* p, int n, uint32_t v, hipStream_t s);
 /* optix7 estimator (lib/rendercore_optix7): eye rays with blue-noise pixel and
  * lens samples (optix/.optix.cu:85-129) and the MIS shade kernel
  * (kernels/pathtracer.h:54-238) with the Lambert BSDF (sharedBSDFs/lambert.h). */
-hipError_t launch_generate7(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s);
+hipError_t launch_generate7(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s,
+	uint32_t slotBase = 0, uint32_t total = 0, bool storeSlot = false);
 hipError_t launch_shade7(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s);
 /* Blue-noise tables in the layout of Heitz et al.'s sampler: [0, 65536)
  * sequence values dimension + 256 * index, [65536, 196608) scrambling tile
@@ -112,7 +142,8 @@ static const int MAXPATHLENGTH_OPTIX7 = 3;
  * rays with the filter on, blue-noise jittered thin-lens rays with it off
  * (optix/.optix.cu:66-129), and the shade kernel that also writes the SVGF
  * G-buffer and splits direct and indirect radiance (kernels/pathtracer.h:76-306). */
-hipError_t launch_generateF(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s);
+hipError_t launch_generateF(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s,
+	uint32_t slotBase = 0, uint32_t total = 0, bool storeSlot = false);
 hipError_t launch_shadeF(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s);
 /* MAXPATHLENGTH of RenderCore_Optix7Filter (core_settings.h:25) */
 static const int MAXPATHLENGTH_FILTER = 7;

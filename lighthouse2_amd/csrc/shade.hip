@@ -429,32 +429,51 @@ __device__ __forceinline__ void acc_add(float4* a, v3 c)
 }
 
 __global__ void k_generate(CamParams c, uint32_t R0, uint32_t jobBase, int n, float eps, float4* __restrict__ rays,
-	float4* __restrict__ state, uint32_t* __restrict__ pathId, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
+	float4* __restrict__ state, uint32_t* __restrict__ pathId, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters,
+	uint32_t slotBase, uint32_t total, int storeSlot)
 {
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j == 0) counters[0] = (uint32_t)n, counters[1] = 0, counters[2] = 0;
+	if (j == 0 && slotBase == 0) counters[0] = total, counters[1] = 0, counters[2] = 0;
 	if (j >= n) return;
 	const uint32_t job = jobBase + (uint32_t)j;
+	const uint32_t o = slotBase + (uint32_t)j;
 	v3 O, D;
 	eye_ray(c, (int)job, R0, O, D);
-	rays[2 * j] = make_float4(O.x, O.y, O.z, eps);
-	rays[2 * j + 1] = make_float4(D.x, D.y, D.z, 1e34f);
-	state[j] = make_float4(1.0f, 1.0f, 1.0f, bitsf(S_SPECULAR));
-	pathId[j] = job;
-	pathAcc[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	rays[2 * o] = make_float4(O.x, O.y, O.z, eps);
+	rays[2 * o + 1] = make_float4(D.x, D.y, D.z, 1e34f);
+	state[o] = make_float4(1.0f, 1.0f, 1.0f, bitsf(S_SPECULAR));
+	pathId[o] = storeSlot ? o : job;
+	pathAcc[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-__global__ void __launch_bounds__(128) k_shade(const ShadeParams P, const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
+/* The constants of the job a pass slot belongs to (passes shared by several
+ * jobs): binary search of the segment table by slot, then the job's own
+ * size, sample offset, seeds, camera and G-buffer in place of the pass's. */
+template <bool MIS>
+__device__ __forceinline__ void job_view(ShadeParams& P, uint32_t slot, uint32_t& pathIdx, uint32_t& slotOut)
+{
+	int g = 0;
+	for (int step = JOB_SEGS_MAX / 2; step > 0; step >>= 1)
+		if (g + step < P.nseg && P.segs[g + step].slotBase <= slot) g += step;
+	const JobSeg& S = P.segs[g];
+	pathIdx = S.jobBase + (slot - S.slotBase);
+	slotOut = slot;
+	P.w = S.w, P.h = S.h, P.pass = S.pass, P.maxDepth = S.maxDepth, P.spreadAngle = S.spreadAngle;
+	P.R0 = MIS ? S.R0w[P.pathLength & 7] : P.R0 + S.R0add;
+	P.shift = S.shift, P.blueSlot = S.blueSlot;
+	P.gFeatures = S.gFeatures, P.gWorldPos = S.gWorldPos, P.gDeltaDepth = S.gDeltaDepth;
+	P.camPos = S.camPos, P.camP1 = S.camP1, P.camRight = S.camRight, P.camUp = S.camUp;
+	P.probe = nullptr;
+}
+
+__device__ __forceinline__ void shade_path(const ShadeParams& P, const int j, const uint32_t pathIdx, const uint32_t slot, const uint32_t stored,
+	const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
 	const uint32_t* __restrict__ pathIn, const float4* __restrict__ hits, float4* __restrict__ raysOut, float4* __restrict__ stateOut,
 	uint32_t* __restrict__ pathOut, float4* __restrict__ shadowRays, float4* __restrict__ shadowPot, float4* __restrict__ pathAcc,
 	uint32_t* __restrict__ counters)
 {
-	const int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= (int)counters[0]) return;
 	const float4 O4 = raysIn[2 * j], D4 = raysIn[2 * j + 1], T4 = stateIn[j];
 	const float4 hd = hits[j];
-	const uint32_t pathIdx = pathIn[j];
-	const uint32_t slot = pathIdx - P.jobBase;
 	float4* acc = pathAcc + slot;
 	const int wh = P.w * P.h;
 	const uint32_t pixelIdx = pathIdx % (uint32_t)wh;
@@ -531,7 +550,26 @@ __global__ void __launch_bounds__(128) k_shade(const ShadeParams P, const float4
 	raysOut[2 * k] = make_float4(no.x, no.y, no.z, 0.0f);
 	raysOut[2 * k + 1] = make_float4(R.x, R.y, R.z, 1e34f);
 	stateOut[k] = make_float4(throughput.x, throughput.y, throughput.z, bitsf(flags));
-	pathOut[k] = pathIdx;
+	pathOut[k] = stored;
+}
+
+template <bool JOBS>
+__global__ void __launch_bounds__(128) k_shade(const ShadeParams P, const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
+	const uint32_t* __restrict__ pathIn, const float4* __restrict__ hits, float4* __restrict__ raysOut, float4* __restrict__ stateOut,
+	uint32_t* __restrict__ pathOut, float4* __restrict__ shadowRays, float4* __restrict__ shadowPot, float4* __restrict__ pathAcc,
+	uint32_t* __restrict__ counters)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= (int)counters[0]) return;
+	const uint32_t stored = pathIn[j];
+	if (JOBS)
+	{	/* a path of one of several jobs sharing the pass: that job's constants */
+		ShadeParams Q = P;
+		uint32_t pathIdx, slot;
+		job_view<false>(Q, stored, pathIdx, slot);
+		shade_path(Q, j, pathIdx, slot, stored, raysIn, stateIn, pathIn, hits, raysOut, stateOut, pathOut, shadowRays, shadowPot, pathAcc, counters);
+	}
+	else shade_path(P, j, stored, stored - P.jobBase, stored, raysIn, stateIn, pathIn, hits, raysOut, stateOut, pathOut, shadowRays, shadowPot, pathAcc, counters);
 }
 
 __global__ void k_connect(const float4* __restrict__ shadowPot, const uint32_t* __restrict__ occluded, float4* __restrict__ pathAcc,
@@ -690,12 +728,14 @@ __device__ __forceinline__ v3 fixnan(v3 c)
 }
 
 __global__ void k_generate7(CamParams c, ShadeParams P, uint32_t jobBase, int n, float eps, float4* __restrict__ rays,
-	float4* __restrict__ state, uint32_t* __restrict__ pathId, float2* __restrict__ aux, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
+	float4* __restrict__ state, uint32_t* __restrict__ pathId, float2* __restrict__ aux, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters,
+	uint32_t slotBase, uint32_t total, int storeSlot)
 {
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j == 0) counters[0] = (uint32_t)n, counters[1] = 0, counters[2] = 0;
+	if (j == 0 && slotBase == 0) counters[0] = total, counters[1] = 0, counters[2] = 0;
 	if (j >= n) return;
 	const uint32_t pathIdx = jobBase + (uint32_t)j;
+	const uint32_t o = slotBase + (uint32_t)j;
 	const uint32_t wh = (uint32_t)(c.w * c.h);
 	const uint32_t pixelIdx = pathIdx % wh, sampleIdx = pathIdx / wh + (uint32_t)P.pass;
 	uint32_t seed = wang_hash(pathIdx * 16789u + (uint32_t)P.pass * 1791u);
@@ -729,26 +769,23 @@ __global__ void k_generate7(CamParams c, ShadeParams P, uint32_t jobBase, int n,
 		target = p1 + (bx + r4.y) * (right / (float)c.w) + (by + r4.w) * (up / (float)c.h);
 	}
 	const v3 D = normalize(target - O);
-	rays[2 * j] = make_float4(O.x, O.y, O.z, eps);
-	rays[2 * j + 1] = make_float4(D.x, D.y, D.z, 1e34f);
-	state[j] = make_float4(1.0f, 1.0f, 1.0f, bitsf(S7_SPECULAR));
-	aux[j] = make_float2(1.0f, bitsf(0u));
-	pathId[j] = pathIdx;
-	pathAcc[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	rays[2 * o] = make_float4(O.x, O.y, O.z, eps);
+	rays[2 * o + 1] = make_float4(D.x, D.y, D.z, 1e34f);
+	state[o] = make_float4(1.0f, 1.0f, 1.0f, bitsf(S7_SPECULAR));
+	aux[o] = make_float2(1.0f, bitsf(0u));
+	pathId[o] = storeSlot ? o : pathIdx;
+	pathAcc[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-__global__ void __launch_bounds__(128) k_shade7(const ShadeParams P, const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
+__device__ __forceinline__ void shade_path7(const ShadeParams& P, const int j, const uint32_t pathIdx, const uint32_t slot, const uint32_t stored,
+	const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
 	const float2* __restrict__ auxIn, const uint32_t* __restrict__ pathIn, const float4* __restrict__ hits, float4* __restrict__ raysOut,
 	float4* __restrict__ stateOut, float2* __restrict__ auxOut, uint32_t* __restrict__ pathOut, float4* __restrict__ shadowRays,
 	float4* __restrict__ shadowPot, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
 {
-	const int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= (int)counters[0]) return;
 	const float4 O4 = raysIn[2 * j], D4 = raysIn[2 * j + 1], T4 = stateIn[j];
 	const float2 A2 = auxIn[j];
 	const float4 hd = hits[j];
-	const uint32_t pathIdx = pathIn[j];
-	const uint32_t slot = pathIdx - P.jobBase;
 	float4* acc = pathAcc + slot;
 	const uint32_t wh = (uint32_t)(P.w * P.h);
 	const uint32_t pixelIdx = pathIdx % wh;
@@ -792,7 +829,7 @@ __global__ void __launch_bounds__(128) k_shade7(const ShadeParams P, const float
 			if (!__builtin_isfinite(T4.x + T4.y + T4.z)) st = make_float4(0, 0, 0, T4.w);
 			stateOut[k] = st;
 			auxOut[k] = A2;
-			pathOut[k] = pathIdx;
+			pathOut[k] = stored;
 		}
 		return;
 	}
@@ -880,20 +917,43 @@ __global__ void __launch_bounds__(128) k_shade7(const ShadeParams P, const float
 	const v3 tp = fixnan(throughput) * bsdf * fabsf(dot(fN, R));
 	stateOut[k] = make_float4(tp.x, tp.y, tp.z, bitsf(flags));
 	auxOut[k] = make_float2(newBsdfPdf, bitsf(packedNormal));
-	pathOut[k] = pathIdx;
+	pathOut[k] = stored;
 }
 
-hipError_t launch_generate7(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s)
+template <bool JOBS>
+__global__ void __launch_bounds__(128) k_shade7(const ShadeParams P, const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
+	const float2* __restrict__ auxIn, const uint32_t* __restrict__ pathIn, const float4* __restrict__ hits, float4* __restrict__ raysOut,
+	float4* __restrict__ stateOut, float2* __restrict__ auxOut, uint32_t* __restrict__ pathOut, float4* __restrict__ shadowRays,
+	float4* __restrict__ shadowPot, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= (int)counters[0]) return;
+	const uint32_t stored = pathIn[j];
+	if (JOBS)
+	{	/* a path of one of several jobs sharing the pass: that job's constants */
+		ShadeParams Q = P;
+		uint32_t pathIdx, slot;
+		job_view<true>(Q, stored, pathIdx, slot);
+		shade_path7(Q, j, pathIdx, slot, stored, raysIn, stateIn, auxIn, pathIn, hits, raysOut, stateOut, auxOut, pathOut, shadowRays, shadowPot, pathAcc, counters);
+	}
+	else shade_path7(P, j, stored, stored - P.jobBase, stored, raysIn, stateIn, auxIn, pathIn, hits, raysOut, stateOut, auxOut, pathOut, shadowRays, shadowPot, pathAcc, counters);
+}
+
+hipError_t launch_generate7(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s,
+	uint32_t slotBase, uint32_t total, bool storeSlot)
 {
 	if (n <= 0) return hipSuccess;
-	k_generate7<<<(n + 255) / 256, 256, 0, s>>>(c, p, jobBase, n, eps, b.rays[out], b.state[out], b.pathId[out], b.aux[out], b.pathAcc, b.counters);
+	k_generate7<<<(n + 255) / 256, 256, 0, s>>>(c, p, jobBase, n, eps, b.rays[out], b.state[out], b.pathId[out], b.aux[out], b.pathAcc, b.counters,
+		slotBase, total ? total : (uint32_t)n, storeSlot ? 1 : 0);
 	return hipGetLastError();
 }
 
 hipError_t launch_shade7(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s)
 {
 	if (maxCount <= 0) return hipSuccess;
-	k_shade7<<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.aux[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
+	if (p.nseg > 0) k_shade7<true><<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.aux[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
+		b.aux[in ^ 1], b.pathId[in ^ 1], b.shadowRays, b.shadowPot, b.pathAcc, b.counters);
+	else k_shade7<false><<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.aux[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
 		b.aux[in ^ 1], b.pathId[in ^ 1], b.shadowRays, b.shadowPot, b.pathAcc, b.counters);
 	return hipGetLastError();
 }
@@ -953,12 +1013,14 @@ __device__ void depth_derivatives(const ShadeParams& P, int x, int y, float dept
 }
 
 __global__ void k_generateF(CamParams c, ShadeParams P, uint32_t jobBase, int n, float eps, float4* __restrict__ rays,
-	float4* __restrict__ state, uint32_t* __restrict__ pathId, float2* __restrict__ aux, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
+	float4* __restrict__ state, uint32_t* __restrict__ pathId, float2* __restrict__ aux, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters,
+	uint32_t slotBase, uint32_t total, int storeSlot)
 {
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j == 0) counters[0] = (uint32_t)n, counters[1] = 0, counters[2] = 0;
+	if (j == 0 && slotBase == 0) counters[0] = total, counters[1] = 0, counters[2] = 0;
 	if (j >= n) return;
 	const uint32_t pathIdx = jobBase + (uint32_t)j;
+	const uint32_t o = slotBase + (uint32_t)j;
 	const uint32_t wh = (uint32_t)(c.w * c.h);
 	const uint32_t pixelIdx = pathIdx % wh, sampleIdx = pathIdx / wh + (uint32_t)P.pass;
 	const int sx = (int)(pixelIdx % (uint32_t)c.w), sy = (int)(pixelIdx / (uint32_t)c.w);
@@ -981,32 +1043,30 @@ __global__ void k_generateF(CamParams c, ShadeParams P, uint32_t jobBase, int n,
 	const float u = ((float)sx + r0) * (1.0f / (float)c.w), v = ((float)sy + r1) * (1.0f / (float)c.h);
 	const v3 target = p1 + u * right + v * up;
 	const v3 D = normalize(target - O);
-	rays[2 * j] = make_float4(O.x, O.y, O.z, eps);
-	rays[2 * j + 1] = make_float4(D.x, D.y, D.z, 1e34f);
-	state[j] = make_float4(1.0f, 1.0f, 1.0f, bitsf(SF_SPECULAR));
-	aux[j] = make_float2(1.0f, bitsf(0u));
-	pathId[j] = pathIdx;
-	pathAcc[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-	pathAcc[j + n] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	rays[2 * o] = make_float4(O.x, O.y, O.z, eps);
+	rays[2 * o + 1] = make_float4(D.x, D.y, D.z, 1e34f);
+	state[o] = make_float4(1.0f, 1.0f, 1.0f, bitsf(SF_SPECULAR));
+	aux[o] = make_float2(1.0f, bitsf(0u));
+	pathId[o] = storeSlot ? o : pathIdx;
+	pathAcc[o] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	pathAcc[o + total] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-__global__ void __launch_bounds__(128) k_shadeF(const ShadeParams P, const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
+__device__ __forceinline__ void shade_pathF(const ShadeParams& P, const int j, const uint32_t pathIdx, const uint32_t slot0, const uint32_t stored,
+	const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
 	const float2* __restrict__ auxIn, const uint32_t* __restrict__ pathIn, const float4* __restrict__ hits, float4* __restrict__ raysOut,
 	float4* __restrict__ stateOut, float2* __restrict__ auxOut, uint32_t* __restrict__ pathOut, float4* __restrict__ shadowRays,
 	float4* __restrict__ shadowPot, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
 {
-	const int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= (int)counters[0]) return;
 	const float4 O4 = raysIn[2 * j], D4 = raysIn[2 * j + 1], T4 = stateIn[j];
 	const float2 A2 = auxIn[j];
 	const float4 hd = hits[j];
-	const uint32_t pathIdx = pathIn[j];
 	uint32_t flags = fbits(T4.w);
 	const bool gbuf = P.gFeatures != nullptr;
 	const uint32_t wh = (uint32_t)(P.w * P.h);
 	const uint32_t pixelIdx = (pathIdx % wh) + ((gbuf && (flags & SF_BOUNCED)) ? wh : 0u);
 	const uint32_t sampleIdx = pathIdx / wh + (uint32_t)P.pass;
-	const uint32_t slot = (pathIdx - P.jobBase) + ((gbuf && (flags & SF_BOUNCED)) ? P.passPaths : 0u);
+	const uint32_t slot = slot0 + ((gbuf && (flags & SF_BOUNCED)) ? P.passPaths : 0u);
 	float4* acc = pathAcc + slot;
 	const v3 D = xyz(D4);
 	v3 throughput = xyz(T4);
@@ -1077,7 +1137,7 @@ __global__ void __launch_bounds__(128) k_shadeF(const ShadeParams P, const float
 			if (!__builtin_isfinite(T4.x + T4.y + T4.z)) st = make_float4(0, 0, 0, T4.w);
 			stateOut[k] = st;
 			auxOut[k] = A2;
-			pathOut[k] = pathIdx;
+			pathOut[k] = stored;
 		}
 		return;
 	}
@@ -1199,20 +1259,43 @@ __global__ void __launch_bounds__(128) k_shadeF(const ShadeParams P, const float
 	const v3 tp = fixnan(throughput) * bsdf * fabsf(dot(fN, R));
 	stateOut[k] = make_float4(tp.x, tp.y, tp.z, bitsf(flags));
 	auxOut[k] = make_float2(newBsdfPdf, bitsf(packedNormal));
-	pathOut[k] = pathIdx;
+	pathOut[k] = stored;
 }
 
-hipError_t launch_generateF(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s)
+template <bool JOBS>
+__global__ void __launch_bounds__(128) k_shadeF(const ShadeParams P, const float4* __restrict__ raysIn, const float4* __restrict__ stateIn,
+	const float2* __restrict__ auxIn, const uint32_t* __restrict__ pathIn, const float4* __restrict__ hits, float4* __restrict__ raysOut,
+	float4* __restrict__ stateOut, float2* __restrict__ auxOut, uint32_t* __restrict__ pathOut, float4* __restrict__ shadowRays,
+	float4* __restrict__ shadowPot, float4* __restrict__ pathAcc, uint32_t* __restrict__ counters)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= (int)counters[0]) return;
+	const uint32_t stored = pathIn[j];
+	if (JOBS)
+	{	/* a path of one of several jobs sharing the pass: that job's constants */
+		ShadeParams Q = P;
+		uint32_t pathIdx, slot;
+		job_view<true>(Q, stored, pathIdx, slot);
+		shade_pathF(Q, j, pathIdx, slot, stored, raysIn, stateIn, auxIn, pathIn, hits, raysOut, stateOut, auxOut, pathOut, shadowRays, shadowPot, pathAcc, counters);
+	}
+	else shade_pathF(P, j, stored, stored - P.jobBase, stored, raysIn, stateIn, auxIn, pathIn, hits, raysOut, stateOut, auxOut, pathOut, shadowRays, shadowPot, pathAcc, counters);
+}
+
+hipError_t launch_generateF(const CamParams& c, const ShadeParams& p, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s,
+	uint32_t slotBase, uint32_t total, bool storeSlot)
 {
 	if (n <= 0) return hipSuccess;
-	k_generateF<<<(n + 255) / 256, 256, 0, s>>>(c, p, jobBase, n, eps, b.rays[out], b.state[out], b.pathId[out], b.aux[out], b.pathAcc, b.counters);
+	k_generateF<<<(n + 255) / 256, 256, 0, s>>>(c, p, jobBase, n, eps, b.rays[out], b.state[out], b.pathId[out], b.aux[out], b.pathAcc, b.counters,
+		slotBase, total ? total : (uint32_t)n, storeSlot ? 1 : 0);
 	return hipGetLastError();
 }
 
 hipError_t launch_shadeF(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s)
 {
 	if (maxCount <= 0) return hipSuccess;
-	k_shadeF<<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.aux[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
+	if (p.nseg > 0) k_shadeF<true><<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.aux[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
+		b.aux[in ^ 1], b.pathId[in ^ 1], b.shadowRays, b.shadowPot, b.pathAcc, b.counters);
+	else k_shadeF<false><<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.aux[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
 		b.aux[in ^ 1], b.pathId[in ^ 1], b.shadowRays, b.shadowPot, b.pathAcc, b.counters);
 	return hipGetLastError();
 }
@@ -1229,16 +1312,20 @@ void blue_noise_tables(uint32_t* out, uint32_t seed)
 	for (int i = 0; i < 128 * 128 * 8; i++) out[3 * 65536 + i] = next() & 255u;
 }
 
-hipError_t launch_generate(const CamParams& c, uint32_t R0, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s)
+hipError_t launch_generate(const CamParams& c, uint32_t R0, uint32_t jobBase, int n, float eps, const PathBuffers& b, int out, hipStream_t s,
+	uint32_t slotBase, uint32_t total, bool storeSlot)
 {
-	k_generate<<<(n + 255) / 256, 256, 0, s>>>(c, R0, jobBase, n, eps, b.rays[out], b.state[out], b.pathId[out], b.pathAcc, b.counters);
+	k_generate<<<(n + 255) / 256, 256, 0, s>>>(c, R0, jobBase, n, eps, b.rays[out], b.state[out], b.pathId[out], b.pathAcc, b.counters,
+		slotBase, total ? total : (uint32_t)n, storeSlot ? 1 : 0);
 	return hipGetLastError();
 }
 
 hipError_t launch_shade(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s)
 {
 	if (maxCount <= 0) return hipSuccess;
-	k_shade<<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
+	if (p.nseg > 0) k_shade<true><<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
+		b.pathId[in ^ 1], b.shadowRays, b.shadowPot, b.pathAcc, b.counters);
+	else k_shade<false><<<(maxCount + 127) / 128, 128, 0, s>>>(p, b.rays[in], b.state[in], b.pathId[in], b.hits, b.rays[in ^ 1], b.state[in ^ 1],
 		b.pathId[in ^ 1], b.shadowRays, b.shadowPot, b.pathAcc, b.counters);
 	return hipGetLastError();
 }

@@ -634,7 +634,7 @@ static int resident_blocks(const void* fn)
 	static std::mutex lock;
 	static std::map<std::pair<const void*, int>, int> cache;
 	int dev = 0;
-	hipGetDevice(&dev);
+	if (hipGetDevice(&dev) != hipSuccess) dev = 0;
 	std::lock_guard<std::mutex> g(lock);
 	auto it = cache.find({ fn, dev });
 	if (it != cache.end()) return it->second;

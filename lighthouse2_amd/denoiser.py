@@ -201,6 +201,31 @@ class UNetDenoiser:
             _check(L.lh2hip_dn_adam(d["b"].data_ptr(), d["mb"].data_ptr(), d["vb"].data_ptr(), d["gb"].data_ptr(), d["b"].numel(),
                                     lr, b1, b2, eps, self.t, None, None, 0, 0, self.sh))
 
+    def weights_blob(self):
+        """The network's inference weights as one blob (the job server's
+        lh2hip_server_set_denoiser layout): per layer in forward order, the
+        bf16 forward weights [9][Cout][Cin] then the fp32 bias [Cout]."""
+        self.torch.cuda.synchronize()
+        parts = []
+        for name, ci, co in layer_specs():
+            d = self.p[name]
+            parts.append(d["wf"].cpu().numpy().astype(np.int16).tobytes())
+            parts.append(d["b"].cpu().numpy().astype(np.float32).tobytes())
+        blob = b"".join(parts)
+        assert len(blob) == library().lh2hip_dn_weights_bytes()
+        return blob
+
+    def set_input_frame(self, acc32, features, spp, w, h):
+        """Input from a filter-estimator frame on the device (acc32 float4
+        [2 h w], features uint4 [h w]; the frame's w x h inside this network's
+        H x W): the same kernel the job server runs."""
+        _check(library().lh2hip_dn_frame_input(acc32.data_ptr(), features.data_ptr(), 1.0 / spp, w, h, self.H, self.W,
+                                               self.a["in_0"].data_ptr(), self.sh))
+
+    def output_frame(self, features, w, h, image):
+        """Display image (w x h float4, device) from the last forward pass, as the job server writes it."""
+        _check(library().lh2hip_dn_frame_output(self.out.data_ptr(), self.H, self.W, features.data_ptr(), w, h, image.data_ptr(), self.sh))
+
     def params_numpy(self):
         return {k: (d["w"].cpu().numpy(), d["b"].cpu().numpy()) for k, d in self.p.items()}
 
