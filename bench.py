@@ -1,17 +1,21 @@
 #!/usr/bin/env python3
-"""Incoherent-extend benchmark of RenderCore_HIP on one MI355X.
+"""Benchmarks of RenderCore_HIP on one MI355X; the default run prints one JSON
+line whose headline is the incoherent-extend rate, with the other two metrics
+of the workload set and the parity run as sub-objects.
 
-Workload (incoherent traversal microbenchmark): 4,000,000 random triangles
+Headline (incoherent traversal microbenchmark): 4,000,000 random triangles
 (centroids U[0,1)^3, vertex offsets N(0, 0.005^2), seed 1), binned-SAH BVH
 built on the GPU; 1920x1080 primary rays from (0.5,0.5,-1.2) towards
 (0.5,0.5,0.5), FOV 40, then 3 cosine-weighted diffuse bounces (misses die).
 One step = primary generation + 4 extend waves + 3 bounce kernels.
-
 metric: "Mrays/s incoherent extend" = rays of waves 2..4 / time of their extend
 launches (HIP events on the launch stream), the reference's rays/traceTime
 definition (lib/RenderCore_PrimeRef/rendercore.cpp:568-576,
-apps/benchmarkapp/main.cpp:479-482). Prints one JSON line.
-"""
+apps/benchmarkapp/main.cpp:479-482); q8 nodes, fp32 nodes reported beside.
+
+Sub-objects: "frame" (ms/frame, 1080p 1 spp + SVGF on the instanced scene),
+"serve" (p50/p99 job latency of a Poisson job stream through the job
+server), "c1_parity" (the Cornell-box parity run against the FP64 oracle)."""
 import argparse
 import json
 import os
@@ -50,7 +54,9 @@ def parse():
                     help="extend: the headline traversal metric; denoise: U-Net forward/train at 1088x1920; "
                          "svgf: filter ms/frame at 1920x1080; serve: job-server latency under Poisson arrivals; "
                          "frame: instanced scene, 1080p 1 spp + SVGF; learned: denoiser trained on rendered pairs vs SVGF")
-    ap.add_argument("--serve-jobs", type=int, default=40)
+    ap.add_argument("--serve-jobs", type=int, default=0, help="jobs of the serving stream (0: 200 in the default run, 2000 alone)")
+    ap.add_argument("--headline-only", action="store_true", help="skip the frame, serve and parity sub-objects")
+    ap.add_argument("--serve-weights", default="", help="learned-denoiser weight blob for the serving stream (default: seeded init)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (FETCH_SIZE/WRITE_SIZE calibration)")
     return ap.parse_args()
@@ -173,14 +179,23 @@ def bench_svgf(a, np, torch, lh, scenes):
                                    "traffic": None, "bytes_per_pixel": bpp}}))
 
 
-def bench_frame(a, np, torch, lh, scenes):
+# algorithmic bytes of one filter-estimator shade launch per path: in 76 (ray 32, state 16, aux 8,
+# path id 4, hit 16), triangle 128 (8 of 13 float4), material 128, texels 32 (a trilinear fetch),
+# out 140 (extension ray + state + aux + id 60, shadow ray + potential 48, path-slot read-modify-write
+# 32); the first wave also writes the 48-byte G-buffer (features, world position, depth derivatives)
+SHADE_BYTES_WAVE1 = 552
+SHADE_BYTES = 504
+
+
+def frame_section(a, np, torch, lh, scenes):
     """One real-time frame of the instanced workload at 1920x1080, 1 spp:
     10 noise-displaced spheres (50k-200k triangles), 1000 instances with
     random affine transforms (two-level BVH), ten 2048^2 textures, Disney
     materials (Lambert diffuse + GGX, roughness U(0.05, 1)), 64 area lights;
-    the filter estimator (eye rays, shade with G-buffer, NEE, one diffuse
-    bounce) followed by SVGF. Camera fixed; each frame restarts accumulation,
-    as a moving camera would. ms/frame is wall time around Render."""
+    the filter estimator (eye rays, shade with G-buffer, NEE, its path cap)
+    followed by SVGF and TAA, the reference's defaults. Camera fixed; each
+    frame restarts accumulation, as a moving camera would. ms/frame is wall
+    time around Render; the roofline is the shade kernel's (HIP events)."""
     t0 = time.time()
     s = scenes.instanced_scene()
     t_scene = time.time() - t0
@@ -197,27 +212,37 @@ def bench_frame(a, np, torch, lh, scenes):
     t_load = time.time() - t0
     for _ in range(a.warmup):
         core.Render(view, lh.LH2_RESTART)
-    parts = {k: 0.0 for k in ("traceTime0", "traceTime1", "traceTimeX", "shadeTime", "shadowTraceTime", "filterTime")}
-    rays = 0
+    keys = ("traceTime0", "traceTime1", "traceTimeX", "shadeTime", "shadowTraceTime", "filterTime")
+    parts = {k: 0.0 for k in keys}
+    shaded = [0, 0]          # first-wave paths, later-wave paths
     torch.cuda.synchronize()
     t0 = time.time()
     for _ in range(a.steps):
         core.Render(view, lh.LH2_RESTART)
         st = core.GetCoreStats()
-        for k in parts:
+        for k in keys:
             parts[k] += float(st[k])
+        shaded[0] += int(st["primaryRayCount"])
+        shaded[1] += int(st["bounce1RayCount"]) + int(st["deepRayCount"])
     core.WaitForRender()
     ms = (time.time() - t0) * 1e3 / a.steps
+    shade_s = parts["shadeTime"]
     parts = {k: round(v * 1e3 / a.steps, 3) for k, v in parts.items()}
     img = core.read_image()
     tris = sum(len(v) // 3 for v, _ in s.meshes)
+    gbs = (shaded[0] * SHADE_BYTES_WAVE1 + shaded[1] * SHADE_BYTES) / max(shade_s, 1e-9) / 1e9
     line = {"metric": "ms/frame 1080p 1spp+denoise", "value": round(ms, 3), "unit": "ms", "higher_is_better": False, "n_gpus": 1,
             "steps": a.steps, "warmup": a.warmup, "dtype": "f32", "data": "synthetic scene", "vs_baseline": None,
             "config": {"workload": "instanced scene: 10 displaced spheres 50k-200k tris, 1000 instances, 2048^2 textures, "
                                    "Disney (diffuse + GGX), 64 area lights; 1920x1080, 1 spp, filter estimator + SVGF + TAA",
-                       "mesh_triangles": int(tris), "instances": len(s.instances)},
-            "breakdown_ms": parts, "image_mean": float(img[1:-1, 1:-1, :3].mean()),
+                       "mesh_triangles": int(tris), "instances": len(s.instances), "path_cap": "the filter core's MAXPATHLENGTH (7)"},
+            "breakdown_ms": parts, "paths_shaded_per_frame": int(sum(shaded) / a.steps),
+            "roofline": {"bound": "hbm", "kernel": "k_shadeF", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "bytes": "algorithmic",
+                         "bytes_per_path": {"first_wave": SHADE_BYTES_WAVE1, "later_waves": SHADE_BYTES}},
+            "image_mean": float(img[1:-1, 1:-1, :3].mean()),
             "scene_build_s": round(t_scene, 1), "load_and_first_frame_s": round(t_load, 1)}
+    core.Shutdown()
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
@@ -227,10 +252,13 @@ def bench_frame(a, np, torch, lh, scenes):
         t0 = time.time()
         ref.frame(scenes.instanced_view(w, h), orc.RESTART)
         cpu_s = time.time() - t0
-        line["cpu_baseline"] = {"value": round(cpu_s * 64 * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+        line["cpu_baseline"] = {"value": round(cpu_s * 64 * 1e3, 1), "unit": "ms", "cores": threads, "cpu": cpu_model(), "kind": "port",
                                 "sample": f"oracle filter estimator + SVGF + TAA at {w}x{h} (1/64 of the pixels), scaled by 64"}
-    print(json.dumps(line))
-    core.Shutdown()
+    return line
+
+
+def bench_frame(a, np, torch, lh, scenes):
+    print(json.dumps(frame_section(a, np, torch, lh, scenes)))
 
 
 def bench_learned(a, np, torch, lh, scenes):
@@ -283,52 +311,284 @@ def bench_learned(a, np, torch, lh, scenes):
                       "data_s": round(t_data, 1), "train_s": round(t_train, 1)}))
 
 
-def bench_serve(a, np, torch, lh, scenes):
-    """Job server under Poisson arrivals (lambda 20 jobs/s, seed 7):
-    resolutions {256^2, 512^2, 1920x1080, 3840x2160}, spp {1, 4, 16, 64},
-    on the instanced scene (10 displaced spheres, 1,000 instances, 2048^2
-    textures, 64 area lights); every job is rendered by the filter estimator
-    with its own G-buffer and denoised by SVGF (a first frame, TAA off);
-    latency = completion - arrival (the jobs' depth draws, U{4..8}, are kept
-    in the stream but the filter estimator's path length is its own)."""
+def serve_section(a, np, torch, lh, scenes, n_jobs):
+    """Job server under Poisson arrivals (lambda 20 jobs/s, seed 7), the
+    first n_jobs of the stream: resolutions {256^2, 512^2, 1920x1080,
+    3840x2160}, spp {1, 4, 16, 64}, depth U{4..8}, half the jobs denoised by
+    SVGF and half by the learned U-Net, on the instanced scene (10 displaced
+    spheres, 1,000 instances, 2048^2 textures, 64 area lights); admission by a
+    64 GB budget, passes of up to 2^22 paths shared by up to 16 jobs.
+    latency = completion - arrival. Every job is rendered by the filter
+    estimator (its path cap replaces the drawn depth)."""
     import threading
+    from lighthouse2_amd.denoiser import UNetDenoiser
     s = scenes.instanced_scene()
     core = lh.RenderCore()
     lh.load_scene(core, s)
-    srv = lh.RenderServer(core, budget_bytes=64 << 30, pass_paths=1 << 21)
+    pass_paths = 1 << 22
+    srv = lh.RenderServer(core, budget_bytes=64 << 30, pass_paths=pass_paths)
+    if a.serve_weights:
+        blob, wsrc = open(a.serve_weights, "rb").read(), os.path.basename(a.serve_weights)
+    else:
+        blob, wsrc = UNetDenoiser(16, 16, seed=0).weights_blob(), "seeded initial weights (forward cost does not depend on them)"
+    srv.set_denoiser(blob)
     rng = np.random.default_rng(7)
     res = [(256, 256), (512, 512), (1920, 1080), (3840, 2160)]
-    jobs = [(res[rng.integers(4)], int(rng.choice([1, 4, 16, 64])), int(rng.integers(4, 9))) for _ in range(a.serve_jobs)]
-    gaps = rng.exponential(1 / 20.0, a.serve_jobs)
+    jobs = [(res[rng.integers(4)], int(rng.choice([1, 4, 16, 64])), int(rng.integers(4, 9)), int(1 + rng.integers(2)))
+            for _ in range(n_jobs)]
+    gaps = rng.exponential(1 / 20.0, n_jobs)
     ids = []
     t_start = time.time()
 
     def submitter():
         t = time.time()
-        for (wh, spp, d), gap in zip(jobs, gaps):
+        for k, (((w, h), spp, d, dn), gap) in enumerate(zip(jobs, gaps)):
             t += gap
             time.sleep(max(0.0, t - time.time()))
-            ids.append(srv.submit(scenes.instanced_view(*wh), wh[0], wh[1], spp, d, 1234, denoise=1))
+            ids.append(srv.submit(scenes.instanced_view(w, h), w, h, spp, d, 1000 + k, denoise=dn))
     th = threading.Thread(target=submitter)
     th.start()
     th.join()
     lat, paths = [], 0
-    for j, ((w, h), spp, d) in zip(ids, jobs):
+    for j, ((w, h), spp, d, dn) in zip(ids, jobs):
         srv.wait(j)
         t = srv.times(j)
         lat.append(t[2] - t[0])
         paths += w * h * spp
         srv.read(j)
     makespan = time.time() - t_start
-    lat = np.array(lat)
-    print(json.dumps({"metric": "p50/p99 job latency", "value": round(float(np.percentile(lat, 50)), 3), "unit": "s",
-                      "p99_s": round(float(np.percentile(lat, 99)), 3), "higher_is_better": False, "n_gpus": 1,
-                      "jobs": len(lat), "lambda_jobs_per_s": 20, "makespan_s": round(makespan, 2),
-                      "mpaths_per_s": round(paths / makespan / 1e6, 1), "dtype": "f32", "data": "synthetic", "vs_baseline": None,
-                      "config": {"workload": "Poisson job stream on the instanced scene (10 displaced spheres 50k-200k tris, 1000 instances, "
-                                             "2048^2 textures, 64 area lights), mixed resolution/spp, filter estimator + SVGF per job",
-                                 "budget_gb": 64, "pass_paths": 1 << 21}}))
+    st = srv.stats()
     srv.close()
+    core.Shutdown()
+    lat = np.array(lat)
+    later = st["rays"] - st["paths"]
+    gbs = (st["paths"] * SHADE_BYTES_WAVE1 + later * SHADE_BYTES) / max(st["shade_ms"] * 1e-3, 1e-9) / 1e9
+    line = {"metric": "p50/p99 job latency", "value": round(float(np.percentile(lat, 50)), 3), "unit": "s",
+            "p99_s": round(float(np.percentile(lat, 99)), 3), "higher_is_better": False, "n_gpus": 1,
+            "jobs": len(lat), "lambda_jobs_per_s": 20, "makespan_s": round(makespan, 2),
+            "mrays_s": round(st["rays"] / makespan / 1e6, 1), "mpaths_s": round(paths / makespan / 1e6, 1),
+            "dtype": "f32", "data": "synthetic", "vs_baseline": None,
+            "batching": {k: st[k] for k in ("passes", "parts", "shared_passes", "max_parts")},
+            "device_ms": {"extend": round(st["trace_ms"], 1), "shade": round(st["shade_ms"], 1)},
+            "roofline": {"bound": "hbm", "kernel": "k_shadeF (passes shared by several jobs)", "achieved": round(gbs, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "bytes": "algorithmic"},
+            "config": {"workload": f"Poisson job stream (first {n_jobs} jobs of seed 7) on the instanced scene (10 displaced spheres "
+                                   "50k-200k tris, 1000 instances, 2048^2 textures, 64 area lights), mixed resolution/spp, "
+                                   "filter estimator per job, SVGF or learned denoiser 50/50",
+                       "budget_gb": 64, "pass_paths": pass_paths, "denoiser_weights": wsrc}}
+    if not a.no_cpu:
+        # CPU: 1% of the jobs (at least 2), each rendered by the oracle filter estimator + SVGF at 1/64 of its pixels, scaled by 64
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        threads = min(16, os.cpu_count() or 1)
+        pick = np.random.default_rng(8).choice(len(jobs), size=max(2, len(jobs) // 100), replace=False)
+        o = orc.OracleScene(s)
+        times = []
+        for k in pick:
+            (w, h), spp, d, dn = jobs[k]
+            ref = orc.RenderF(o, w // 8, h // 8, spp=spp, threads=threads, taa=False, seed=1000 + int(k))
+            t0 = time.time()
+            ref.frame(scenes.instanced_view(w // 8, h // 8), orc.RESTART)
+            times.append((time.time() - t0) * 64)
+        line["cpu_baseline"] = {"value": round(float(np.median(times)), 2), "unit": "s per job (median)", "cores": threads, "cpu": cpu_model(),
+                                "kind": "port", "sample": f"{len(pick)} jobs of the stream ({', '.join(f'{jobs[k][0][0]}x{jobs[k][0][1]} {jobs[k][1]} spp' for k in pick)}): "
+                                                         "oracle filter estimator + SVGF at 1/64 of the pixels, scaled by 64"}
+    return line
+
+
+def bench_serve(a, np, torch, lh, scenes):
+    print(json.dumps(serve_section(a, np, torch, lh, scenes, a.serve_jobs or 2000)))
+
+
+def c1_section(a, np, torch, lh, scenes):
+    """The parity run: Cornell box (5 wall quads + a 20,000-triangle sphere,
+    0.2 x 0.2 light of radiance 15), 256 x 256, 1 spp x 16 frames, max depth 3,
+    seed 1234, the reference estimator. Every image (the accumulated image
+    after each frame) against the FP64 oracle: mean and max relative error;
+    the fp64 accumulators compared bit for bit."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    s = scenes.cornell_box(seed=1234, sphere_tris=20000)
+    W = H = 256
+    view = scenes.cornell_view(W, H)
+    core = lh.RenderCore()
+    lh.load_scene(core, s)
+    core.SetTarget(W, H, 1)
+    core.Setting("maxDepth", 3)
+    core.Setting("epsilon", 1e-3)
+    core.Setting("seed", 1234)
+    threads = min(16, os.cpu_count() or 1)
+    o = orc.OracleScene(s)
+    modes = [orc.RESTART] + [orc.CONVERGE] * 16
+    frames, gpu_ms, cpu_s, identical = [], [], 0.0, True
+    gen = orc.render_frames(o, view, W, H, 1, 16, 3, 1e-3, seed=1234, threads=threads, modes=modes)
+    for k, m in enumerate(modes):
+        torch.cuda.synchronize()
+        t0 = time.time()
+        core.Render(view, lh.LH2_RESTART if m == orc.RESTART else lh.LH2_CONVERGE)
+        core.WaitForRender()
+        gpu_ms.append((time.time() - t0) * 1e3)
+        img, acc = core.read_image(), core.read_accumulator_f64()
+        t0 = time.time()
+        _, r64, n = next(gen)
+        cpu_s += time.time() - t0
+        if k == 0:
+            continue                      # the preview frame before the converging sequence
+        ref = r64[..., :3] / n
+        rel = np.abs(img[..., :3].astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-6)
+        frames.append({"frame": k, "samples": int(n), "mean_rel": float(f"{rel.mean():.3e}"), "max_rel": float(f"{rel.max():.3e}")})
+        identical = identical and bool(np.array_equal(acc, r64))
+    core.Shutdown()
+    return {"metric": "per-image relative error vs FP64 oracle", "value": max(f["max_rel"] for f in frames), "unit": "max rel",
+            "tolerance": 1e-4, "fp64_accumulators_identical": identical, "images": frames,
+            "config": {"workload": "Cornell box 256x256, 1 spp x 16 frames, max depth 3, seed 1234, 20,000-triangle sphere, reference estimator"},
+            "gpu_ms_per_frame": round(float(np.median(gpu_ms[1:])), 3),
+            "cpu_baseline": {"value": round(cpu_s / len(modes) * 1e3, 1), "unit": "ms/frame", "cores": threads, "cpu": cpu_model(), "kind": "port",
+                             "sample": "the same 17 frames (preview + 16) on the oracle: fp32 estimator, fp64 accumulation"}}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def measure_extend(a, np, torch, core, wv, steps, warmup):
+    """Times `steps` steps after `warmup`; returns per-step ray counts and the
+    per-wave extend times (ms) from HIP events on the launch stream."""
+    for _ in range(warmup):
+        wv.step()
+    torch.cuda.synchronize()
+    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(4)] for _ in range(steps)]
+    per_step = []
+    torch.cuda.synchronize()
+    tw0 = time.time()
+    for s in range(steps):
+        per_step.append(wv.step(ev[s]))
+    torch.cuda.synchronize()
+    wall = time.time() - tw0
+    ms = [[ev[s][w][0].elapsed_time(ev[s][w][1]) for w in range(4)] for s in range(steps)]
+    return per_step, ms, wall
+
+
+def bench_extend(a, np, torch, lh, scenes):
+    t0 = time.time()
+    verts = scenes.random_triangles(a.tris, seed=1)
+    t_gen = time.time() - t0
+    core = lh.RenderCore()
+    core.Setting("nodeFormat", 1 if a.format == "q8" else 0)
+    core.Setting("traceLoop", {"merged": 0, "ww": 1, "greedy": 2}[a.loop])
+    core.Setting("traceTriBias", a.tri_bias)
+    core.Setting("traceRefillLanes", a.refill_lanes)
+    core.Setting("traceChunk", a.chunk)
+    if a.blocks_per_cu:
+        core.Setting("traceBlocksPerCU", a.blocks_per_cu)
+    if a.min_rays_per_lane:
+        core.Setting("traceMinRaysPerLane", a.min_rays_per_lane)
+    t0 = time.time()
+    core.SetGeometry(0, verts, None)
+    core.SetInstance(0, 0)
+    core.FinalizeInstances()
+    t_build = time.time() - t0
+    info = core.bvh_info(0)
+    view = scenes.microbench_view(a.width, a.height)
+    wv = Waves(torch, core, view, a.width, a.height)
+
+    per_step, ms, wall = measure_extend(a, np, torch, core, wv, a.steps, a.warmup)
+    inco_rays = sum(sum(c[1:]) for c in per_step)
+    inco_ms = sum(sum(m[1:]) for m in ms)
+    prim_rays = sum(c[0] for c in per_step)
+    prim_ms = sum(m[0] for m in ms)
+    value = inco_rays / (inco_ms * 1e-3) / 1e6
+
+    # algorithmic work per incoherent ray, counted by the kernel on the same rays (untimed)
+    keep = []
+    wv.step(keep=keep)
+    torch.cuda.synchronize()
+    nodes = tris = nrays = 0
+    it = np.zeros(10)
+    scratch = torch.zeros_like(wv.hits)
+    for r, _h in keep:
+        c = core.trace_closest_counted(r.data_ptr(), scratch.data_ptr(), len(r), wv.sh)
+        nodes, tris, nrays = nodes + c[0], tris + c[1], nrays + len(r)
+        it += np.array(c[2:], np.float64)
+    nodes_per_ray, tris_per_ray = nodes / nrays, tris / nrays
+    bytes_per_ray = RAY_IO_BYTES + nodes_per_ray * NODE_BYTES[a.format] + tris_per_ray * TRI_BYTES
+    launch_ms = [m for st in ms for m in st[1:]]
+    launch_rays = [c for st in per_step for c in st[1:]]
+    achieved = float(np.mean([r * bytes_per_ray / (m * 1e-3) / 1e9 for r, m in zip(launch_rays, launch_ms)]))
+
+    # the other node format on the same triangles and rays: fp32 BVH2 nodes (q8 BVH4 is the default)
+    other = "f32" if a.format == "q8" else "q8"
+    core.Setting("nodeFormat", 0 if other == "f32" else 1)
+    core.FinalizeInstances()
+    ps2, ms2, _ = measure_extend(a, np, torch, core, wv, max(3, a.steps // 2), 2)
+    other_value = sum(sum(c[1:]) for c in ps2) / (sum(sum(m[1:]) for m in ms2) * 1e-3) / 1e6
+    other_keep = []
+    wv.step(keep=other_keep)
+    torch.cuda.synchronize()
+    other_equal = all(torch.equal(h1, h2) for (_, h1), (_, h2) in zip(keep, other_keep))
+
+    traffic, traffic_src = None, None
+    tf = os.path.join(ROOT, "profiles", "k_trace_merged_incoherent_4m_traffic.json")
+    if os.path.exists(tf) and a.format == "q8":
+        tj = json.load(open(tf))
+        traffic, traffic_src = tj.get("bytes_per_launch"), tj.get("source")
+
+    cpu = None
+    parity = None
+    if not a.no_cpu:
+        cpu, parity = cpu_baseline(a, verts, keep, np)
+    out = {
+        "metric": "Mrays/s incoherent extend",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random triangles, generated on the host)",
+        "config": {"workload": "incoherent traversal: 4M random triangles, 1920x1080 primary + 3 diffuse bounces",
+                   "triangles": a.tris, "width": a.width, "height": a.height, "node_format": a.format,
+                   "bvh": "binned SAH 16 bins, leaf <= 4, GPU build", "seed": 1,
+                   "blocks_per_cu": f"{a.blocks_per_cu or 8} requested, capped at the kernel occupancy", "min_rays_per_lane": a.min_rays_per_lane,
+                   "loop": a.loop, "ray_chunk": a.chunk, "refill_lanes": a.refill_lanes},
+        # achieved = algorithmic bytes per ray (ray + hit, nodes and triangles the kernel visits on these
+        # rays) x rays / launch time; the kernel's limiter is VALU issue (SQ counters in profiles/), and
+        # the HBM traffic that reaches memory is far below the algorithmic bytes (MALL and L2 hits)
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                     "bytes": "algorithmic", "limiter": "VALU issue",
+                     "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 2),
+                     "tris_per_ray": round(tris_per_ray, 2)},
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "node_formats": {a.format: round(value, 2), other: round(other_value, 2), "hits_equal": bool(other_equal)},
+        "primary_mrays_s": round(prim_rays / (prim_ms * 1e-3) / 1e6, 2),
+        "incoherent_rays_per_step": int(inco_rays / a.steps),
+        "extend_ms_per_launch": round(float(np.mean(launch_ms)), 4),
+        "per_wave": [{"wave": w + 1, "rays": int(np.mean([c[w] for c in per_step])),
+                      "ms": round(float(np.mean([m[w] for m in ms])), 4),
+                      "mrays_s": round(float(np.mean([c[w] for c in per_step]) / np.mean([m[w] for m in ms]) / 1e3), 1)}
+                     for w in range(4)],
+        "simd_utilisation": {"node_loop": round(it[1] / max(64 * it[0], 1), 3), "leaf_loop": round(it[3] / max(64 * it[2], 1), 3),
+                             "node_iters_per_ray": round(it[0] * 64 / nrays, 2), "leaf_iters_per_ray": round(it[2] * 64 / nrays, 2),
+                             "lanes_holding_ray": round(it[5] / max(64 * it[4], 1), 3), "loop_iters_per_ray": round(it[4] * 64 / nrays, 2),
+                             "tail_lanes_holding_ray": round(it[7] / max(64 * it[6], 1), 3), "tail_iters_per_ray": round(it[6] * 64 / nrays, 2),
+                             "tail_time_share": round(1.0 - it[8] / max(it[9], 1), 3)},
+        "wall_mrays_s_all_waves": round((inco_rays + prim_rays) / wall / 1e6, 2),
+        "bvh": {"build_s": round(t_build, 3), "scene_gen_s": round(t_gen, 3), **info},
+    }
+    core.Shutdown()
+    return out
 
 
 def main():
@@ -355,113 +615,14 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": a.copy_bytes, "dispatches": 3}))
         return
-    t0 = time.time()
-    verts = scenes.random_triangles(a.tris, seed=1)
-    t_gen = time.time() - t0
-    core = lh.RenderCore()
-    core.Setting("nodeFormat", 1 if a.format == "q8" else 0)
-    core.Setting("traceLoop", {"merged": 0, "ww": 1, "greedy": 2}[a.loop])
-    core.Setting("traceTriBias", a.tri_bias)
-    core.Setting("traceRefillLanes", a.refill_lanes)
-    core.Setting("traceChunk", a.chunk)
-    if a.blocks_per_cu:
-        core.Setting("traceBlocksPerCU", a.blocks_per_cu)
-    if a.min_rays_per_lane:
-        core.Setting("traceMinRaysPerLane", a.min_rays_per_lane)
-    t0 = time.time()
-    core.SetGeometry(0, verts, None)
-    core.SetInstance(0, 0)
-    core.FinalizeInstances()
-    t_build = time.time() - t0
-    info = core.bvh_info(0)
-    view = scenes.microbench_view(a.width, a.height)
-    wv = Waves(torch, core, view, a.width, a.height)
-
-    for _ in range(a.warmup):
-        wv.step()
-    torch.cuda.synchronize()
-    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(4)] for _ in range(a.steps)]
-    per_step = []
-    torch.cuda.synchronize()
-    tw0 = time.time()
-    for s in range(a.steps):
-        per_step.append(wv.step(ev[s]))
-    torch.cuda.synchronize()
-    wall = time.time() - tw0
-    ms = [[ev[s][w][0].elapsed_time(ev[s][w][1]) for w in range(4)] for s in range(a.steps)]
-    inco_rays = sum(sum(c[1:]) for c in per_step)
-    inco_ms = sum(sum(m[1:]) for m in ms)
-    prim_rays = sum(c[0] for c in per_step)
-    prim_ms = sum(m[0] for m in ms)
-    value = inco_rays / (inco_ms * 1e-3) / 1e6
-
-    # algorithmic work per incoherent ray, counted by the kernel on the same rays (untimed)
-    keep = []
-    wv.step(keep=keep)
-    torch.cuda.synchronize()
-    nodes = tris = nrays = 0
-    it = np.zeros(10)
-    scratch = torch.zeros_like(wv.hits)
-    for r, _h in keep:
-        c = core.trace_closest_counted(r.data_ptr(), scratch.data_ptr(), len(r), wv.sh)
-        nodes, tris, nrays = nodes + c[0], tris + c[1], nrays + len(r)
-        it += np.array(c[2:], np.float64)
-    nodes_per_ray, tris_per_ray = nodes / nrays, tris / nrays
-    bytes_per_ray = RAY_IO_BYTES + nodes_per_ray * NODE_BYTES[a.format] + tris_per_ray * TRI_BYTES
-    launch_ms = [m for st in ms for m in st[1:]]
-    launch_rays = [c for st in per_step for c in st[1:]]
-    achieved = float(np.mean([r * bytes_per_ray / (m * 1e-3) / 1e9 for r, m in zip(launch_rays, launch_ms)]))
-
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "k_trace_merged_incoherent_4m_traffic.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
-
-    cpu = None
-    parity = None
-    if not a.no_cpu:
-        cpu, parity = cpu_baseline(a, verts, keep, np)
-
-    out = {
-        "metric": "Mrays/s incoherent extend",
-        "value": round(value, 2),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(wall / a.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded random triangles, generated on the host)",
-        "config": {"workload": "incoherent traversal: 4M random triangles, 1920x1080 primary + 3 diffuse bounces",
-                   "triangles": a.tris, "width": a.width, "height": a.height, "node_format": a.format,
-                   "bvh": "binned SAH 16 bins, leaf <= 4, GPU build", "seed": 1,
-                   "blocks_per_cu": f"{a.blocks_per_cu or 8} requested, capped at the kernel occupancy", "min_rays_per_lane": a.min_rays_per_lane,
-                   "loop": a.loop, "ray_chunk": a.chunk, "refill_lanes": a.refill_lanes},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 2),
-                     "tris_per_ray": round(tris_per_ray, 2)},
-        "cpu_baseline": cpu,
-        "parity": parity,
-        "primary_mrays_s": round(prim_rays / (prim_ms * 1e-3) / 1e6, 2),
-        "incoherent_rays_per_step": int(inco_rays / a.steps),
-        "extend_ms_per_launch": round(float(np.mean(launch_ms)), 4),
-        "per_wave": [{"wave": w + 1, "rays": int(np.mean([c[w] for c in per_step])),
-                      "ms": round(float(np.mean([m[w] for m in ms])), 4),
-                      "mrays_s": round(float(np.mean([c[w] for c in per_step]) / np.mean([m[w] for m in ms]) / 1e3), 1)}
-                     for w in range(4)],
-        "simd_utilisation": {"node_loop": round(it[1] / max(64 * it[0], 1), 3), "leaf_loop": round(it[3] / max(64 * it[2], 1), 3),
-                             "node_iters_per_ray": round(it[0] * 64 / nrays, 2), "leaf_iters_per_ray": round(it[2] * 64 / nrays, 2),
-                             "lanes_holding_ray": round(it[5] / max(64 * it[4], 1), 3), "loop_iters_per_ray": round(it[4] * 64 / nrays, 2),
-                             "tail_lanes_holding_ray": round(it[7] / max(64 * it[6], 1), 3), "tail_iters_per_ray": round(it[6] * 64 / nrays, 2),
-                             "tail_time_share": round(1.0 - it[8] / max(it[9], 1), 3)},
-        "wall_mrays_s_all_waves": round((inco_rays + prim_rays) / wall / 1e6, 2),
-        "bvh": {"build_s": round(t_build, 3), "scene_gen_s": round(t_gen, 3), **info},
-    }
-    core.Shutdown()
+    out = bench_extend(a, np, torch, lh, scenes)
+    print("headline done: %.1f Mrays/s" % out["value"], file=sys.stderr, flush=True)
+    if not a.headline_only:
+        out["frame"] = frame_section(a, np, torch, lh, scenes)
+        print("frame done: %.3f ms" % out["frame"]["value"], file=sys.stderr, flush=True)
+        out["serve"] = serve_section(a, np, torch, lh, scenes, a.serve_jobs or 200)
+        print("serve done: p50 %.3f s" % out["serve"]["value"], file=sys.stderr, flush=True)
+        out["c1_parity"] = c1_section(a, np, torch, lh, scenes)
     print(json.dumps(out), flush=True)
 
 
@@ -497,7 +658,7 @@ def cpu_baseline(a, verts, keep, np):
         n_total += len(idx)
         if t_total > a.cpu_seconds:
             break
-    cpu = {"value": round(n_total / t_total / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+    cpu = {"value": round(n_total / t_total / 1e6, 3), "unit": "Mrays/s", "cores": threads, "cpu": cpu_model(), "kind": "port",
            "sample": f"{n_total} incoherent rays (random 200k of waves 2-4, repeated) traced by the oracle BVH on "
                      f"{threads} threads in {t_total:.1f} s; oracle BVH build {t_build:.1f} s excluded"}
     return cpu, {"checked": int(len(idx)), "mismatches": mism}

@@ -220,14 +220,14 @@ def xor32_host(seed):
 RESTART, CONVERGE = 1, 0
 
 
-def render(scene_or_oracle, view, w, h, spp=1, frames=1, max_depth=3, eps=1e-3, seed=0x12345678, threads=8, modes=None):
-    """Oracle of a sequence of Render calls. `modes` lists Restart/Converge per
-    call (default: one Restart, then `frames` Converge calls). Restates the
-    convergence state machine of RenderCore::Render
+def render_frames(scene_or_oracle, view, w, h, spp=1, frames=1, max_depth=3, eps=1e-3, seed=0x12345678, threads=8, modes=None):
+    """Oracle of a sequence of Render calls, one result per call. `modes` lists
+    Restart/Converge per call (default: one Restart, then `frames` Converge
+    calls). Restates the convergence state machine of RenderCore::Render
     (lib/RenderCore_PrimeRef/rendercore.cpp:543-550; rendercore.h:129-134):
     a Restart, or the first Converge call after it, clears the accumulator and
-    reseeds the camera RNG. Returns (acc32 (h,w,4) float32, acc64 float64,
-    samplesTaken) as they stand after the last call."""
+    reseeds the camera RNG. Yields (acc32 (h,w,4) float32, acc64 float64,
+    samplesTaken) after each call (the arrays are updated in place)."""
     o = scene_or_oracle if isinstance(scene_or_oracle, OracleScene) else OracleScene(scene_or_oracle)
     if modes is None:
         modes = [RESTART] + [CONVERGE] * frames
@@ -248,7 +248,16 @@ def render(scene_or_oracle, view, w, h, spp=1, frames=1, max_depth=3, eps=1e-3, 
             acc32 += blk
             acc64 += blk.astype(np.float64)
         taken += spp
-    return acc32.reshape(h, w, 4), acc64.reshape(h, w, 4), taken
+        yield acc32.reshape(h, w, 4), acc64.reshape(h, w, 4), taken
+
+
+def render(scene_or_oracle, view, w, h, spp=1, frames=1, max_depth=3, eps=1e-3, seed=0x12345678, threads=8, modes=None):
+    """The state after the last call of render_frames: (acc32 (h,w,4) float32,
+    acc64 float64, samplesTaken)."""
+    out = None
+    for out in render_frames(scene_or_oracle, view, w, h, spp, frames, max_depth, eps, seed, threads, modes):
+        pass
+    return out[0].copy(), out[1].copy(), out[2]
 
 
 class OracleSvgf:
