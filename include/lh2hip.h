@@ -158,6 +158,14 @@ LH2HIP_API int lh2hip_bvh_info(lh2hip_core* core, int meshIdx, int64_t* out8);
  * 8 x 4 bytes, and the leaf primitive list. Returns node count in *nodeCount. */
 LH2HIP_API int lh2hip_bvh_download(lh2hip_core* core, int meshIdx, void* nodesOut, int maxNodes,
 	uint32_t* primsOut, int maxPrims, int* nodeCount);
+/* Downloads the traversal layout the device built for a mesh (inspection of
+ * the build that replaces OptiX's, lib/rendercore_optix7/core_mesh.cpp:105-123):
+ * the f32 binary nodes (64 B each, bvh_info out[3] of them), the 4-wide q8
+ * nodes (64 B each, out[4]) and the leaf triangles (3 x float4 per triangle,
+ * v0.w = primitive id | bit 31 on a leaf's last triangle). A null pointer skips
+ * that array; each array must hold the full count. */
+LH2HIP_API int lh2hip_bvh_layout_download(lh2hip_core* core, int meshIdx, void* f32Out, int maxF32,
+	void* q8Out, int maxQ8, void* trisOut, int maxTris);
 
 /* ---- SVGF ----------------------------------------------------------------
  * prepareFilter + applyFilter x 3 + finalizeNoTAA of RenderCore_Optix7Filter

@@ -58,6 +58,7 @@ _SIGS = {
     "lh2hip_bounce_diffuse": (I, [P, P, P, P, I, ctypes.c_uint32, ctypes.c_float, P, P, P, P]),
     "lh2hip_bvh_info": (I, [P, I, P]),
     "lh2hip_bvh_download": (I, [P, I, P, I, P, I, P]),
+    "lh2hip_bvh_layout_download": (I, [P, I, P, I, P, I, P, I]),
     "lh2hip_dn_conv3x3": (I, [P, I, P, P, P, I, I, I, I, P]),
     "lh2hip_dn_dw_workspace_bytes": (ctypes.c_uint64, [I, I]),
     "lh2hip_dn_conv3x3_dw": (I, [P, I, P, I, I, I, P, P, P, P]),
@@ -278,6 +279,16 @@ class RenderCore:
         _check(library().lh2hip_bvh_download(self._h, mesh, _ptr(nodes), len(nodes), _ptr(prims), len(prims),
                                              ctypes.byref(cnt)))
         return nodes, prims
+
+    def bvh_layout_download(self, mesh=0):
+        """The device-built traversal layout: f32 nodes and q8 nodes (64 bytes
+        each, as uint32 rows of 16) and the leaf triangles (float32 rows of 12)."""
+        info = self.bvh_info(mesh)
+        f32 = np.zeros((info["nodes_f32"], 16), np.uint32)
+        q8 = np.zeros((info["nodes_q8"], 16), np.uint32)
+        tris = np.zeros((info["triangles"], 12), np.float32)
+        _check(library().lh2hip_bvh_layout_download(self._h, mesh, _ptr(f32), len(f32), _ptr(q8), len(q8), _ptr(tris), len(tris)))
+        return f32, q8, tris
 
 
 class RenderServer:

@@ -62,6 +62,7 @@ struct DeviceBuild
 	BuildNode* nodes = nullptr;
 	uint32_t* prims = nullptr;
 	int nodeCount = 0, primCount = 0, depth = 0;
+	std::vector<int> levelStart;   /* build ids of depth d: [levelStart[d], levelStart[d + 1]) */
 };
 
 /* Binned SAH (16 bins) over primitive boxes lo[i].xyz / hi[i].xyz on the GPU.
@@ -71,34 +72,39 @@ hipError_t gpu_build(const float4* dLo, const float4* dHi, int n, int maxLeaf, i
 	DeviceBuild& out, hipStream_t stream);
 void free_build(DeviceBuild& b);
 
-/* Refit on the device, topology unchanged. Boxes of the build nodes listed in
- * ids[first .. first+count) (one tree level: leaves take the min / max of their
+/* Refit on the device, topology unchanged. Boxes of the build nodes
+ * first .. first+count-1 (one tree level: leaves take the min / max of their
  * triangles' vertices, interior nodes the union of their two children, which
  * must already be refitted); then the laid-out f32 / q8 nodes are rewritten
  * from the refitted build boxes through the layout maps (refs unchanged). */
-hipError_t refit_level(BuildNode* dNodes, const uint32_t* dPrims, const float4* dVerts, const int32_t* dIds, int first, int count, hipStream_t stream);
+hipError_t refit_level(BuildNode* dNodes, const uint32_t* dPrims, const float4* dVerts, int first, int count, hipStream_t stream);
 hipError_t refit_layout(const BuildNode* dNodes, NodeF32* dF32, const int32_t* dF32Kids, int nF32, NodeQ8* dQ8, const int32_t* dQ8Map, int nQ8, hipStream_t stream);
 
 /* Triangle soup (3 float4 per triangle) -> primitive boxes. */
 hipError_t tri_bounds(const float4* dVerts, int n, float4* dLo, float4* dHi, hipStream_t stream);
 
-/* Host-side layout of a downloaded build tree into the traversal formats.
- * order[i] = primitive stored in leaf triangle slot i (depth-first leaf order);
- * last[i] = 1 on the final slot of each leaf. */
-struct Layout
+/* Device layout of a build tree into the traversal formats (bvh_layout.hip).
+ * Arrays are caller-owned device memory sized for the worst case: a build
+ * tree of N nodes has (N - 1) / 2 interior nodes, so at most that many f32
+ * and q8 nodes; f32Kids holds 2 and q8Map 5 ints per node; order / last one
+ * entry per primitive. q8 (with q8Map) null: binary layout only. f32Kids,
+ * q8Map null: no refit maps. */
+struct LayoutOut
 {
-	std::vector<NodeF32> f32;
-	std::vector<NodeQ8> q8;
-	std::vector<uint32_t> order;
-	std::vector<uint8_t> last;
-	/* build-tree nodes behind each laid-out node, for refits that keep the
-	 * topology: f32Kids[2i..2i+1] = children of f32 node i; q8Map[5i] = the
-	 * node itself, q8Map[5i+1..5i+4] = its children (-1 past the last) */
-	std::vector<int32_t> f32Kids, q8Map;
-	uint32_t rootF32 = 0, rootQ8 = 0;
-	int depthF32 = 0, depthQ8 = 0, stackF32 = 0, stackQ8 = 0;
+	NodeF32* f32 = nullptr;
+	int32_t* f32Kids = nullptr;       /* children (build ids) of f32 node i: [2i], [2i+1] */
+	NodeQ8* q8 = nullptr;
+	int32_t* q8Map = nullptr;         /* [5i] = build node of q8 node i, [5i+1..5i+4] = its children (-1 past the last) */
+	uint32_t* order = nullptr;        /* primitive in leaf triangle slot i */
+	uint8_t* last = nullptr;          /* 1 on the final slot of each leaf */
 };
-void layout_bvh(const std::vector<BuildNode>& nodes, const std::vector<uint32_t>& prims, bool wantQ8, Layout& out);
+/* levelStart[d]..levelStart[d+1]: the build ids of depth d (gpu_build's breadth-first
+ * order). leafRefMap non-null (top level, one primitive per leaf): leaf refs are
+ * REF_LEAF | leafRefMap[primitive] and no triangle order is written.
+ * countsOut (host, may be null; the call then waits for the stream): [0] f32
+ * nodes, [1] q8 nodes, [2] q8 tree depth. */
+hipError_t layout_device(const BuildNode* dNodes, const uint32_t* dPrims, int nodeCount, const std::vector<int>& levelStart,
+	const uint32_t* leafRefMap, const LayoutOut& out, int* countsOut, hipStream_t stream);
 
 /* Gathers leaf triangles: tris[3*i+k] = verts[3*order[i]+k], v0.w = prim | last bit. */
 hipError_t gather_tris(const float4* dVerts, const uint32_t* dOrder, const uint8_t* dLast, int n, float4* dTris, hipStream_t stream);

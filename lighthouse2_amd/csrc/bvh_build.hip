@@ -382,6 +382,7 @@ hipError_t gpu_build(const float4* dLo, const float4* dHi, int n, int maxLeaf, i
 {
 	hipError_t err = hipSuccess;
 	out.primCount = n, out.nodeCount = 0, out.depth = 0;
+	out.levelStart.clear();
 	if (n <= 0) return hipSuccess;
 	const int chunkCap = std::min(n, 1 << 20);
 	int* posNode = nullptr; int* tmpNode = nullptr; uint32_t* tmpPrims = nullptr;
@@ -410,6 +411,7 @@ hipError_t gpu_build(const float4* dLo, const float4* dHi, int n, int maxLeaf, i
 	while (levelEnd > levelBegin)
 	{
 		out.depth++;
+		out.levelStart.push_back(levelBegin);
 		for (int cb = levelBegin; cb < levelEnd; cb += chunkCap)
 		{
 			const int cc = std::min(chunkCap, levelEnd - cb);
@@ -427,6 +429,7 @@ hipError_t gpu_build(const float4* dLo, const float4* dHi, int n, int maxLeaf, i
 		levelBegin = levelEnd, levelEnd = total;
 	}
 	out.nodeCount = total;
+	out.levelStart.push_back(total);
 done:
 	(void)hipFree(posNode), (void)hipFree(tmpNode), (void)hipFree(tmpPrims), (void)hipFree(scr), (void)hipFree(dec), (void)hipFree(cursors), (void)hipFree(counter);
 	if (err != hipSuccess) free_build(out);
@@ -440,17 +443,17 @@ void free_build(DeviceBuild& b)
 	b.nodes = nullptr, b.prims = nullptr, b.nodeCount = 0;
 }
 
-/* std::min / std::max of the host refit and layout: (b < a) ? b : a and
- * (a < b) ? b : a, so a NaN operand resolves the same way on both sides. */
+/* min / max in std::min / std::max form, (b < a) ? b : a and (a < b) ? b : a:
+ * a NaN operand resolves the same way as in the oracle's builder. */
 __device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
 
 __global__ void k_refit_level(BuildNode* __restrict__ nodes, const uint32_t* __restrict__ prims, const float4* __restrict__ v,
-	const int32_t* __restrict__ ids, int first, int count)
+	int first, int count)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= count) return;
-	const int x = ids[first + i];
+	const int x = first + i;
 	BuildNode b = nodes[x];
 	float lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
 	if (b.count > 0)
@@ -471,98 +474,10 @@ __global__ void k_refit_level(BuildNode* __restrict__ nodes, const uint32_t* __r
 	for (int c = 0; c < 3; c++) nodes[x].lo[c] = lo[c], nodes[x].hi[c] = hi[c];
 }
 
-hipError_t refit_level(BuildNode* dNodes, const uint32_t* dPrims, const float4* dVerts, const int32_t* dIds, int first, int count, hipStream_t stream)
+hipError_t refit_level(BuildNode* dNodes, const uint32_t* dPrims, const float4* dVerts, int first, int count, hipStream_t stream)
 {
 	if (count <= 0) return hipSuccess;
-	k_refit_level<<<(count + 255) / 256, 256, 0, stream>>>(dNodes, dPrims, dVerts, dIds, first, count);
-	return hipGetLastError();
-}
-
-/* The host layout's quantise_axis: the smallest power-of-two step from
- * ceil(log2(extent / 255)) up for which every child's lo / hi rounds outward
- * to a byte offset from the parent's lo. The step search ends by exponent 128
- * (an infinite step makes every test false, i.e. accepted). */
-__device__ uint32_t quantise_axis_dev(float lo, float hi, const float* clo, const float* chi, int cnt, uint32_t* qlo, uint32_t* qhi)
-{
-	const float ext = hi - lo;
-	int e = -126;
-	if (ext > 0) e = max(-126, (int)ceil(log2((double)ext / 255.0)));
-	for (; e <= 128; e++)
-	{
-		const float s = __uint_as_float((uint32_t)(e + 127) << 23);
-		bool ok = true;
-		for (int c = 0; c < cnt && ok; c++)
-		{
-			const double ql = floor(((double)clo[c] - (double)lo) / (double)s);
-			const double qlc = (ql < 255.0) ? ql : 255.0;
-			int a = (int)((0.0 < qlc) ? qlc : 0.0);
-			while (a > 0 && __fadd_rn(lo, __fmul_rn((float)a, s)) > clo[c]) a--;
-			const double qh = ceil(((double)chi[c] - (double)lo) / (double)s);
-			const double qhc = (qh < 255.0) ? qh : 255.0;
-			int b = (int)((0.0 < qhc) ? qhc : 0.0);
-			while (b < 255 && __fadd_rn(lo, __fmul_rn((float)b, s)) < chi[c]) b++;
-			if (__fadd_rn(lo, __fmul_rn((float)b, s)) < chi[c] || __fadd_rn(lo, __fmul_rn((float)a, s)) > clo[c]) { ok = false; break; }
-			qlo[c] = (uint32_t)a, qhi[c] = (uint32_t)b;
-		}
-		if (ok) return (uint32_t)(e + 127);
-	}
-	return 255u;
-}
-
-__global__ void k_refit_f32(const BuildNode* __restrict__ nodes, NodeF32* __restrict__ out, const int32_t* __restrict__ kids, int n)
-{
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const BuildNode L = nodes[kids[2 * i]], R = nodes[kids[2 * i + 1]];
-	out[i].lxy = make_float4(L.lo[0], L.hi[0], L.lo[1], L.hi[1]);
-	out[i].rxy = make_float4(R.lo[0], R.hi[0], R.lo[1], R.hi[1]);
-	out[i].lrz = make_float4(L.lo[2], L.hi[2], R.lo[2], R.hi[2]);
-}
-
-__global__ void k_refit_q8(const BuildNode* __restrict__ nodes, NodeQ8* __restrict__ out, const int32_t* __restrict__ map, int n)
-{
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const BuildNode B = nodes[map[5 * i]];
-	int cnt = 0;
-	int kid[4];
-	for (int c = 0; c < 4; c++) { kid[c] = map[5 * i + 1 + c]; if (kid[c] >= 0) cnt = c + 1; }
-	uint32_t ex[3], q[6][4] = {};
-	for (int a = 0; a < 3; a++)
-	{
-		float clo[4], chi[4];
-		for (int c = 0; c < cnt; c++) { const BuildNode& K = nodes[kid[c]]; clo[c] = K.lo[a], chi[c] = K.hi[a]; }
-		ex[a] = quantise_axis_dev(B.lo[a], B.hi[a], clo, chi, cnt, q[a], q[3 + a]);
-	}
-	auto pack = [](const uint32_t* v) { return v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24); };
-	out[i].origin = make_float4(B.lo[0], B.lo[1], B.lo[2], __uint_as_float(ex[0] | (ex[1] << 8) | (ex[2] << 16) | ((uint32_t)cnt << 24)));
-	out[i].qa = make_uint4(pack(q[0]), pack(q[1]), pack(q[2]), pack(q[3]));
-	out[i].qb = make_uint4(pack(q[4]), pack(q[5]), 0, 0);
-}
-
-hipError_t refit_layout(const BuildNode* dNodes, NodeF32* dF32, const int32_t* dF32Kids, int nF32, NodeQ8* dQ8, const int32_t* dQ8Map, int nQ8, hipStream_t stream)
-{
-	if (nF32 > 0) k_refit_f32<<<(nF32 + 255) / 256, 256, 0, stream>>>(dNodes, dF32, dF32Kids, nF32);
-	if (nQ8 > 0) k_refit_q8<<<(nQ8 + 127) / 128, 128, 0, stream>>>(dNodes, dQ8, dQ8Map, nQ8);
-	return hipGetLastError();
-}
-
-__global__ void k_gather_tris(const float4* __restrict__ v, const uint32_t* __restrict__ order, const uint8_t* __restrict__ last, int n, float4* __restrict__ tris)
-{
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const uint32_t p = order[i];
-	float4 a = v[3 * p];
-	a.w = bitsf(p | (last[i] ? TRI_LAST : 0u));
-	tris[3 * i] = a;
-	tris[3 * i + 1] = v[3 * p + 1];
-	tris[3 * i + 2] = v[3 * p + 2];
-}
-
-hipError_t gather_tris(const float4* dVerts, const uint32_t* dOrder, const uint8_t* dLast, int n, float4* dTris, hipStream_t stream)
-{
-	if (n <= 0) return hipSuccess;
-	k_gather_tris<<<(n + 255) / 256, 256, 0, stream>>>(dVerts, dOrder, dLast, n, dTris);
+	k_refit_level<<<(count + 255) / 256, 256, 0, stream>>>(dNodes, dPrims, dVerts, first, count);
 	return hipGetLastError();
 }
 

@@ -237,6 +237,11 @@ LH2HIP_API int lh2hip_bvh_download(lh2hip_core* core, int meshIdx, void* nodesOu
 	});
 }
 
+LH2HIP_API int lh2hip_bvh_layout_download(lh2hip_core* core, int meshIdx, void* f32Out, int maxF32, void* q8Out, int maxQ8, void* trisOut, int maxTris)
+{
+	return guarded(core, [&](RenderCore& c) { c.BvhLayoutDownload(meshIdx, f32Out, maxF32, q8Out, maxQ8, trisOut, maxTris); });
+}
+
 
 LH2HIP_API int lh2hip_server_create(lh2hip_core* core, uint64_t budgetBytes, uint64_t passPaths, lh2hip_server** out)
 {

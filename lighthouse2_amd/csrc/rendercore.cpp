@@ -323,6 +323,10 @@ void RenderCore::Setting(const char* name, float value)
 	/* unknown keys are ignored, as in the reference cores */
 }
 
+/* BLAS build, entirely on the device: primitive boxes, binned SAH level by
+ * level, then the layout passes (depth-first f32 nodes, 4-wide q8 nodes, leaf
+ * triangle order) and the leaf-triangle gather. The host reads back only the
+ * per-level node counts, the node counts of both formats and the root box. */
 void RenderCore::buildMesh(MeshData& m)
 {
 	const auto t0 = std::chrono::high_resolution_clock::now();
@@ -332,136 +336,65 @@ void RenderCore::buildMesh(MeshData& m)
 	hipCheck(tri_bounds(m.verts.as<float4>(), n, lo.as<float4>(), hi.as<float4>(), stream), "tri_bounds");
 	DeviceBuild b;
 	hipCheck(gpu_build(lo.as<float4>(), hi.as<float4>(), n, 4, 255, b, stream), "gpu_build");
-	m.buildNodes.resize(b.nodeCount);
-	m.buildPrims.resize(n);
-	hipCheck(hipMemcpyAsync(m.buildNodes.data(), b.nodes, sizeof(BuildNode) * b.nodeCount, hipMemcpyDeviceToHost, stream), "D2H nodes");
-	hipCheck(hipMemcpyAsync(m.buildPrims.data(), b.prims, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream), "D2H prims");
-	hipCheck(hipStreamSynchronize(stream), "sync");
+	lo.release(), hi.release();
+	m.dBuildNodes.release(), m.dBuildPrims.release();
+	m.dBuildNodes.p = b.nodes, m.dBuildNodes.bytes = sizeof(BuildNode) * (size_t)(2 * n);
+	m.dBuildPrims.p = b.prims, m.dBuildPrims.bytes = sizeof(uint32_t) * (size_t)n;
+	m.buildNodeCount = b.nodeCount;
+	m.levelStart = b.levelStart;
 	m.buildDepth = b.depth;
-	free_build(b);
-	layoutMesh(m);
+	const size_t interior = std::max(1, (b.nodeCount - 1) / 2);
+	m.nodesF32.ensure(sizeof(NodeF32) * interior);
+	m.nodesQ8.ensure(sizeof(NodeQ8) * interior);
+	m.dF32Kids.ensure(sizeof(int32_t) * 2 * interior);
+	m.dQ8Map.ensure(sizeof(int32_t) * 5 * interior);
+	m.dOrder.ensure(sizeof(uint32_t) * n);
+	m.dLast.ensure((size_t)n);
+	LayoutOut L;
+	L.f32 = m.nodesF32.as<NodeF32>(), L.f32Kids = m.dF32Kids.as<int32_t>();
+	L.q8 = m.nodesQ8.as<NodeQ8>(), L.q8Map = m.dQ8Map.as<int32_t>();
+	L.order = m.dOrder.as<uint32_t>(), L.last = m.dLast.as<uint8_t>();
+	int counts[3];
+	hipCheck(layout_device(m.dBuildNodes.as<BuildNode>(), m.dBuildPrims.as<uint32_t>(), b.nodeCount, b.levelStart, nullptr, L, counts, stream), "bvh layout");
+	m.tris.ensure(sizeof(float4) * 3 * (size_t)n);
+	hipCheck(gather_tris(m.verts.as<float4>(), m.dOrder.as<uint32_t>(), m.dLast.as<uint8_t>(), n, m.tris.as<float4>(), stream), "gather_tris");
+	BuildNode root;
+	hipCheck(hipMemcpyAsync(&root, m.dBuildNodes.p, sizeof(BuildNode), hipMemcpyDeviceToHost, stream), "D2H root");
+	hipCheck(hipStreamSynchronize(stream), "sync");
+	const bool rootLeaf = root.count > 0;
+	m.nodeCountF32 = (size_t)counts[0], m.nodeCountQ8 = (size_t)counts[1];
+	m.rootF32 = rootLeaf ? REF_LEAF : 0u, m.rootQ8 = rootLeaf ? REF_LEAF : 0u;
+	m.depthF32 = b.depth, m.stackF32 = b.depth + 1;
+	m.depthQ8 = counts[2], m.stackQ8 = 3 * counts[2] + 1;
+	m.leafCount = (size_t)(b.nodeCount + 1) / 2;
+	for (int c = 0; c < 3; c++) m.lo[c] = root.lo[c], m.hi[c] = root.hi[c];
 	const auto t1 = std::chrono::high_resolution_clock::now();
 	m.buildMs = std::chrono::duration<float, std::milli>(t1 - t0).count();
 	stats.bvhBuildTime = m.buildMs * 0.001f;
 }
 
 /* Refit keeps the build tree's topology and recomputes its boxes from the new
- * vertices. On the device (once the mesh has been laid out with the refit
- * setting on): one launch per tree level from the deepest up (a leaf box is
- * the min / max of its triangles' vertices, as tri_bounds computes them, an
- * interior box the union of its two children), then the f32 and q8 nodes are
- * rewritten from the refitted boxes through the layout maps and the leaf
- * triangles gathered again. Otherwise the same sweep runs on the host copy
- * (children follow parents in the build array) and the mesh is laid out anew. */
-void RenderCore::refitMesh(MeshData& m, const lighthouse2::float4* v)
+ * vertices on the device: one launch per tree level from the deepest up (a
+ * leaf box is the min / max of its triangles' vertices, as tri_bounds computes
+ * them, an interior box the union of its two children), then the f32 and q8
+ * nodes are rewritten from the refitted boxes through the layout maps and the
+ * leaf triangles gathered again. */
+void RenderCore::refitMesh(MeshData& m)
 {
 	const auto t0 = std::chrono::high_resolution_clock::now();
-	if (m.deviceRefit)
-	{
-		for (int d = (int)m.levelStart.size() - 2; d >= 0; d--)
-			hipCheck(refit_level(m.dBuildNodes.as<BuildNode>(), m.dBuildPrims.as<uint32_t>(), m.verts.as<float4>(), m.dLevelIds.as<int32_t>(),
-				m.levelStart[d], m.levelStart[d + 1] - m.levelStart[d], stream), "refit_level");
-		hipCheck(refit_layout(m.dBuildNodes.as<BuildNode>(), m.nodesF32.as<NodeF32>(), m.dF32Kids.as<int32_t>(), (int)m.nodeCountF32,
-			m.nodesQ8.as<NodeQ8>(), m.dQ8Map.as<int32_t>(), (int)m.nodeCountQ8, stream), "refit_layout");
-		hipCheck(gather_tris(m.verts.as<float4>(), m.dOrder.as<uint32_t>(), m.dLast.as<uint8_t>(), m.triCount, m.tris.as<float4>(), stream), "gather_tris");
-		BuildNode root;
-		hipCheck(hipMemcpyAsync(&root, m.dBuildNodes.p, sizeof(BuildNode), hipMemcpyDeviceToHost, stream), "D2H root");
-		hipCheck(hipStreamSynchronize(stream), "sync");
-		for (int c = 0; c < 3; c++) m.lo[c] = root.lo[c], m.hi[c] = root.hi[c];
-	}
-	else
-	{
-		std::vector<BuildNode>& nodes = m.buildNodes;
-		for (int i = (int)nodes.size() - 1; i >= 0; i--)
-		{
-			BuildNode& b = nodes[i];
-			float lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
-			if (b.count > 0)
-				for (int k = 0; k < b.count; k++)
-				{
-					const lighthouse2::float4* t = v + 3 * (size_t)m.buildPrims[b.leftFirst + k];
-					for (int c = 0; c < 3; c++)
-					{
-						const float x0 = (&t[0].x)[c], x1 = (&t[1].x)[c], x2 = (&t[2].x)[c];
-						lo[c] = std::min(lo[c], std::min(std::min(x0, x1), x2));
-						hi[c] = std::max(hi[c], std::max(std::max(x0, x1), x2));
-					}
-				}
-			else
-				for (int k = 0; k < 2; k++)
-				{
-					const BuildNode& ch = nodes[b.leftFirst + k];
-					for (int c = 0; c < 3; c++) lo[c] = std::min(lo[c], ch.lo[c]), hi[c] = std::max(hi[c], ch.hi[c]);
-				}
-			for (int c = 0; c < 3; c++) b.lo[c] = lo[c], b.hi[c] = hi[c];
-		}
-		layoutMesh(m);
-	}
+	for (int d = (int)m.levelStart.size() - 2; d >= 0; d--)
+		hipCheck(refit_level(m.dBuildNodes.as<BuildNode>(), m.dBuildPrims.as<uint32_t>(), m.verts.as<float4>(),
+			m.levelStart[d], m.levelStart[d + 1] - m.levelStart[d], stream), "refit_level");
+	hipCheck(refit_layout(m.dBuildNodes.as<BuildNode>(), m.nodesF32.as<NodeF32>(), m.dF32Kids.as<int32_t>(), (int)m.nodeCountF32,
+		m.nodesQ8.as<NodeQ8>(), m.dQ8Map.as<int32_t>(), (int)m.nodeCountQ8, stream), "refit_layout");
+	hipCheck(gather_tris(m.verts.as<float4>(), m.dOrder.as<uint32_t>(), m.dLast.as<uint8_t>(), m.triCount, m.tris.as<float4>(), stream), "gather_tris");
+	BuildNode root;
+	hipCheck(hipMemcpyAsync(&root, m.dBuildNodes.p, sizeof(BuildNode), hipMemcpyDeviceToHost, stream), "D2H root");
+	hipCheck(hipStreamSynchronize(stream), "sync");
+	for (int c = 0; c < 3; c++) m.lo[c] = root.lo[c], m.hi[c] = root.hi[c];
 	const auto t1 = std::chrono::high_resolution_clock::now();
 	m.buildMs = std::chrono::duration<float, std::milli>(t1 - t0).count();
 	stats.bvhBuildTime = m.buildMs * 0.001f;
-}
-
-/* Device copies of the build tree, its nodes grouped by depth (children lie
- * one level below their parent) and the layout maps, for refit. */
-void RenderCore::prepareDeviceRefit(MeshData& m, const Layout& L)
-{
-	const std::vector<BuildNode>& nodes = m.buildNodes;
-	const int n = (int)nodes.size();
-	std::vector<int> depth(n, -1);
-	depth[0] = 0;
-	int maxDepth = 0;
-	for (int i = 0; i < n; i++)
-	{
-		if (depth[i] < 0) throw CoreError(LH2HIP_ERR_STATE, "refit: build node before its parent");
-		maxDepth = std::max(maxDepth, depth[i]);
-		if (nodes[i].count == 0)
-			for (int k = 0; k < 2; k++)
-			{
-				const int c = nodes[i].leftFirst + k;
-				if (c <= i || c >= n) throw CoreError(LH2HIP_ERR_STATE, "refit: child index out of order");
-				depth[c] = depth[i] + 1;
-			}
-	}
-	m.levelStart.assign(maxDepth + 2, 0);
-	for (int i = 0; i < n; i++) m.levelStart[depth[i] + 1]++;
-	for (int d = 0; d <= maxDepth; d++) m.levelStart[d + 1] += m.levelStart[d];
-	std::vector<int32_t> ids(n);
-	std::vector<int> fill(m.levelStart.begin(), m.levelStart.end() - 1);
-	for (int i = 0; i < n; i++) ids[fill[depth[i]]++] = i;
-	m.dBuildNodes.upload(nodes.data(), sizeof(BuildNode) * n, stream);
-	m.dBuildPrims.upload(m.buildPrims.data(), sizeof(uint32_t) * m.buildPrims.size(), stream);
-	m.dLevelIds.upload(ids.data(), sizeof(int32_t) * n, stream);
-	m.dF32Kids.upload(L.f32Kids.data(), sizeof(int32_t) * L.f32Kids.size(), stream);
-	m.dQ8Map.upload(L.q8Map.data(), sizeof(int32_t) * L.q8Map.size(), stream);
-	m.dOrder.upload(L.order.data(), sizeof(uint32_t) * L.order.size(), stream);
-	m.dLast.upload(L.last.data(), L.last.size(), stream);
-	m.deviceRefit = true;
-}
-
-void RenderCore::layoutMesh(MeshData& m)
-{
-	const int n = m.triCount;
-	Layout L;
-	layout_bvh(m.buildNodes, m.buildPrims, true, L);
-	m.rootF32 = L.rootF32, m.rootQ8 = L.rootQ8;
-	m.depthF32 = L.depthF32, m.depthQ8 = L.depthQ8, m.stackF32 = L.stackF32, m.stackQ8 = L.stackQ8;
-	m.nodeCountF32 = L.f32.size(), m.nodeCountQ8 = L.q8.size();
-	m.leafCount = 0;
-	for (uint8_t v : L.last) m.leafCount += v;
-	for (int c = 0; c < 3; c++) m.lo[c] = m.buildNodes[0].lo[c], m.hi[c] = m.buildNodes[0].hi[c];
-	m.nodesF32.upload(L.f32.data(), L.f32.size() * sizeof(NodeF32), stream);
-	m.nodesQ8.upload(L.q8.data(), L.q8.size() * sizeof(NodeQ8), stream);
-	DevBuf order, last;
-	order.upload(L.order.data(), L.order.size() * 4, stream);
-	last.upload(L.last.data(), L.last.size(), stream);
-	m.tris.ensure(sizeof(float4) * 3 * (size_t)n);
-	hipCheck(gather_tris(m.verts.as<float4>(), order.as<uint32_t>(), last.as<uint8_t>(), n, m.tris.as<float4>(), stream), "gather_tris");
-	m.deviceRefit = false;
-	m.dBuildNodes.release(), m.dBuildPrims.release(), m.dLevelIds.release(), m.dF32Kids.release(), m.dQ8Map.release();
-	m.dOrder.release(), m.dLast.release();
-	if (refitSetting) prepareDeviceRefit(m, L);
-	hipCheck(hipStreamSynchronize(stream), "sync");
 }
 
 void RenderCore::SetGeometry(const int meshIdx, const lighthouse2::float4* vertexData, const int vertexCount, const int triangleCount, const CoreTri* triangles)
@@ -476,7 +409,7 @@ void RenderCore::SetGeometry(const int meshIdx, const lighthouse2::float4* verte
 	MeshData& m = meshes[meshIdx];
 	/* animation (rendercore_optix7/core_mesh.cpp:34-61 rebuilds): with the refit setting and an
 	 * unchanged triangle count the existing tree is refitted */
-	const bool refit = refitSetting && triangleCount > 0 && m.triCount == triangleCount && !m.buildNodes.empty();
+	const bool refit = refitSetting && triangleCount > 0 && m.triCount == triangleCount && m.buildNodeCount > 0;
 	m.triCount = triangleCount;
 	m.verts.upload(vertexData, sizeof(float4) * 3 * (size_t)triangleCount, stream);
 	m.maxMaterial = -1;
@@ -486,9 +419,13 @@ void RenderCore::SetGeometry(const int meshIdx, const lighthouse2::float4* verte
 		for (int i = 0; i < triangleCount; i++) m.maxMaterial = std::max(m.maxMaterial, (int)std::min<uint32_t>(triangles[i].material, 0x7fffffffu));
 	}
 	else m.coreTris.release();
-	if (refit) refitMesh(m, vertexData);
+	if (refit) refitMesh(m);
 	else if (triangleCount > 0) buildMesh(m);
-	else m.buildNodes.clear(), m.buildPrims.clear(), m.nodeCountF32 = m.nodeCountQ8 = 0;
+	else
+	{
+		m.dBuildNodes.release(), m.dBuildPrims.release(), m.dF32Kids.release(), m.dQ8Map.release(), m.dOrder.release(), m.dLast.release();
+		m.buildNodeCount = 0, m.levelStart.clear(), m.nodeCountF32 = m.nodeCountQ8 = 0;
+	}
 	instancesDirty = true;
 }
 
@@ -619,25 +556,22 @@ void RenderCore::updateTopLevel()
 		DevBuf dlo, dhi;
 		dlo.upload(blo.data(), n * sizeof(float4), stream);
 		dhi.upload(bhi.data(), n * sizeof(float4), stream);
+		DevBuf liveMap;
+		liveMap.upload(live.data(), n * sizeof(int), stream);
 		DeviceBuild b;
 		hipCheck(gpu_build(dlo.as<float4>(), dhi.as<float4>(), n, 1, 1, b, stream), "tlas build");
-		std::vector<BuildNode> nodes(b.nodeCount);
-		std::vector<uint32_t> prims(n);
-		hipCheck(hipMemcpyAsync(nodes.data(), b.nodes, sizeof(BuildNode) * b.nodeCount, hipMemcpyDeviceToHost, stream), "D2H");
-		hipCheck(hipMemcpyAsync(prims.data(), b.prims, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream), "D2H");
-		hipCheck(hipStreamSynchronize(stream), "sync");
+		/* one instance per leaf: leaf refs point straight at the instance id */
+		tlasNodes.ensure(sizeof(NodeF32) * std::max(1, (b.nodeCount - 1) / 2));
+		LayoutOut L;
+		L.f32 = tlasNodes.as<NodeF32>();
+		const hipError_t e = layout_device(b.nodes, b.prims, b.nodeCount, b.levelStart, liveMap.as<uint32_t>(), L, nullptr, stream);
+		const int depth = b.depth;
 		free_build(b);
-		for (auto& p : prims) p = (uint32_t)live[p]; /* TLAS primitive -> instance id */
-		Layout L;
-		layout_bvh(nodes, prims, false, L);
-		/* one instance per leaf: point leaf refs straight at the instance id */
-		auto fix = [&](uint32_t r) { return (r & REF_LEAF) ? (REF_LEAF | L.order[r & REF_MASK]) : r; };
-		for (auto& nd : L.f32) nd.refs.x = fix(nd.refs.x), nd.refs.y = fix(nd.refs.y);
-		tlasNodes.upload(L.f32.data(), std::max<size_t>(1, L.f32.size()) * sizeof(NodeF32), stream);
+		hipCheck(e, "tlas layout");
 		scene.twoLevel = 1;
 		scene.tlas = tlasNodes.as<NodeF32>();
-		scene.tlasRoot = fix(L.rootF32);
-		scene.maxStack = maxStack + L.stackF32;
+		scene.tlasRoot = n == 1 ? (REF_LEAF | (uint32_t)live[0]) : 0u;
+		scene.maxStack = maxStack + depth + 1;
 	}
 	if (scene.maxStack > TRACE_MAX_STACK) throw CoreError(LH2HIP_ERR_INVALID, "BVH too deep for the traversal stack");
 	hipCheck(hipStreamSynchronize(stream), "sync");
@@ -707,7 +641,7 @@ void RenderCore::BvhInfo(int mesh, int64_t* o) const
 {
 	if (mesh < 0 || mesh >= (int)meshes.size()) throw CoreError(LH2HIP_ERR_INVALID, "bvh_info: unknown mesh");
 	const MeshData& m = meshes[mesh];
-	o[0] = (int64_t)m.buildNodes.size(), o[1] = (int64_t)m.leafCount, o[2] = m.buildDepth, o[3] = (int64_t)m.nodeCountF32;
+	o[0] = (int64_t)m.buildNodeCount, o[1] = (int64_t)m.leafCount, o[2] = m.buildDepth, o[3] = (int64_t)m.nodeCountF32;
 	o[4] = (int64_t)m.nodeCountQ8, o[5] = m.depthQ8, o[6] = (int64_t)(m.buildMs * 1000.0f), o[7] = m.triCount;
 }
 
@@ -715,9 +649,24 @@ int RenderCore::BvhDownload(int mesh, void* nodesOut, int maxNodes, uint32_t* pr
 {
 	if (mesh < 0 || mesh >= (int)meshes.size()) throw CoreError(LH2HIP_ERR_INVALID, "bvh_download: unknown mesh");
 	const MeshData& m = meshes[mesh];
-	if (nodesOut && maxNodes >= (int)m.buildNodes.size()) memcpy(nodesOut, m.buildNodes.data(), m.buildNodes.size() * sizeof(BuildNode));
-	if (primsOut && maxPrims >= (int)m.buildPrims.size()) memcpy(primsOut, m.buildPrims.data(), m.buildPrims.size() * 4);
-	return (int)m.buildNodes.size();
+	if (nodesOut && maxNodes >= m.buildNodeCount && m.buildNodeCount > 0)
+		hipCheck(hipMemcpyAsync(nodesOut, m.dBuildNodes.p, (size_t)m.buildNodeCount * sizeof(BuildNode), hipMemcpyDeviceToHost, stream), "D2H");
+	if (primsOut && maxPrims >= m.triCount && m.buildNodeCount > 0)
+		hipCheck(hipMemcpyAsync(primsOut, m.dBuildPrims.p, (size_t)m.triCount * 4, hipMemcpyDeviceToHost, stream), "D2H");
+	hipCheck(hipStreamSynchronize(stream), "sync");
+	return m.buildNodeCount;
+}
+
+void RenderCore::BvhLayoutDownload(int mesh, void* f32Out, int maxF32, void* q8Out, int maxQ8, void* trisOut, int maxTris) const
+{
+	if (mesh < 0 || mesh >= (int)meshes.size()) throw CoreError(LH2HIP_ERR_INVALID, "bvh_layout_download: unknown mesh");
+	const MeshData& m = meshes[mesh];
+	if ((f32Out && maxF32 < (int)m.nodeCountF32) || (q8Out && maxQ8 < (int)m.nodeCountQ8) || (trisOut && maxTris < m.triCount))
+		throw CoreError(LH2HIP_ERR_INVALID, "bvh_layout_download: output array too small");
+	if (f32Out && m.nodeCountF32) hipCheck(hipMemcpyAsync(f32Out, m.nodesF32.p, m.nodeCountF32 * sizeof(NodeF32), hipMemcpyDeviceToHost, stream), "D2H");
+	if (q8Out && m.nodeCountQ8) hipCheck(hipMemcpyAsync(q8Out, m.nodesQ8.p, m.nodeCountQ8 * sizeof(NodeQ8), hipMemcpyDeviceToHost, stream), "D2H");
+	if (trisOut && m.triCount) hipCheck(hipMemcpyAsync(trisOut, m.tris.p, (size_t)m.triCount * 3 * sizeof(float4), hipMemcpyDeviceToHost, stream), "D2H");
+	hipCheck(hipStreamSynchronize(stream), "sync");
 }
 
 /* HostTexture::PixelsNeeded (host_texture.cpp:117-122). */

@@ -76,13 +76,11 @@ struct MeshData
 	int depthF32 = 0, depthQ8 = 0, stackF32 = 0, stackQ8 = 0, buildDepth = 0;
 	size_t nodeCountF32 = 0, nodeCountQ8 = 0, leafCount = 0;
 	float lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
-	std::vector<BuildNode> buildNodes;   /* canonical build tree (host copy; stale after a device refit) */
-	std::vector<uint32_t> buildPrims;
-	/* device refit state (kept while the refit setting is on): the build tree,
-	 * its nodes grouped by depth, the layout maps and the leaf slot order */
-	DevBuf dBuildNodes, dBuildPrims, dLevelIds, dF32Kids, dQ8Map, dOrder, dLast;
-	std::vector<int> levelStart;         /* ids of depth d: [levelStart[d], levelStart[d + 1]) */
-	bool deviceRefit = false;
+	/* the build tree stays on the device (refit, inspection): breadth-first
+	 * nodes, primitive ranges, the layout maps and the leaf slot order */
+	DevBuf dBuildNodes, dBuildPrims, dF32Kids, dQ8Map, dOrder, dLast;
+	int buildNodeCount = 0;
+	std::vector<int> levelStart;         /* build ids of depth d: [levelStart[d], levelStart[d + 1]) */
 	float buildMs = 0;
 	int maxMaterial = -1;                /* largest CoreTri::material, -1 without CoreTri data */
 };
@@ -157,6 +155,7 @@ public:
 	void ReadBuffer(int which, void* hostOut, size_t bytes);
 	void BvhInfo(int mesh, int64_t* out8) const;
 	int BvhDownload(int mesh, void* nodesOut, int maxNodes, uint32_t* primsOut, int maxPrims) const;
+	void BvhLayoutDownload(int mesh, void* f32Out, int maxF32, void* q8Out, int maxQ8, void* trisOut, int maxTris) const;
 	const TraceScene& Scene();
 	/* Render building blocks, shared with the job server (server.cpp). */
 	void checkRenderable();
@@ -206,10 +205,9 @@ public:
 private:
 	void buildMesh(MeshData& m);
 	/* Refit: the build tree keeps its topology, node boxes are recomputed
-	 * bottom-up from the new vertices (host copy), then laid out again. */
-	void refitMesh(MeshData& m, const lighthouse2::float4* vertexData);
-	void layoutMesh(MeshData& m);
-	void prepareDeviceRefit(MeshData& m, const Layout& L);
+	 * bottom-up from the new vertices and the laid-out nodes rewritten, all
+	 * on the device. */
+	void refitMesh(MeshData& m);
 	void ensureTraceBuffers();
 	void updateTopLevel();
 

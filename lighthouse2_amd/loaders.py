@@ -1140,7 +1140,13 @@ def load_gltf(scene, path, transform=None, animated=False):
                                if "POSITION" in prim["targets"][i] else np.zeros((len(flat), 3), f32) for i in range(nt)])
                 tn = np.stack([_gltf_float_attribute(doc, bufs, prim["targets"][i]["NORMAL"], "VEC3", "target NORMAL")[flat]
                                if "NORMAL" in prim["targets"][i] else np.zeros((len(flat), 3), f32) for i in range(nt)])
-                bn = Nv[flat] if Nv is not None else np.zeros((len(flat), 3), f32)
+                if Nv is not None:
+                    bn = Nv[flat]
+                else:
+                    # no NORMAL attribute: the base normals are the corner normals the load produced
+                    # (the face normal), so a morph update never normalises a zero vector
+                    lt = parts[-1][1]
+                    bn = np.stack([lt["vN0"], lt["vN1"], lt["vN2"]], axis=1).reshape(-1, 3).astype(f32)
                 morph_parts.append((P[flat], bn, tp, tn))
             else:
                 morph_parts.append(None)

@@ -27,6 +27,7 @@ def lib():
             "orc_set_materials": (None, [P, P, I]), "orc_set_lights": (None, [P, P, I, P, I, P, I, P, I]),
             "orc_trace": (None, [P, P, I, P, I, I, P, I]),
             "orc_bvh_nodes": (I, [P, I, P, P, P]), "orc_tlas_nodes": (I, [P, P, P]),
+            "orc_bvh_layout": (None, [P, I, P, I, P, P, P, P, P, P]),
             "orc_render_frame": (None, [P, P, I, I, I, I, ctypes.c_float, U, I, P, I]),
             "orc_primary_rays": (None, [P, I, I, U, ctypes.c_float, P]),
             "orc_bounce": (I, [P, P, P, P, I, U, ctypes.c_float, P, P]),
@@ -61,6 +62,25 @@ def lib():
 
 def _p(a):
     return None if a is None else a.ctypes.data
+
+
+def bvh_layout(nodes, prims):
+    """The traversal layout (oracle_layout.cpp) of a build tree as the device
+    downloads it: (f32 nodes as uint32 rows of 16, q8 nodes likewise, leaf
+    triangle order, last-of-leaf flags, counts [f32, q8, f32 root, q8 root,
+    binary depth, 4-wide depth], q8 map: per q8 node its build node and its
+    children's build nodes, -1 past the last)."""
+    nodes = np.ascontiguousarray(nodes)
+    prims = np.ascontiguousarray(prims, np.uint32)
+    cap = max(1, (len(nodes) - 1) // 2)
+    f32 = np.zeros((cap, 16), np.uint32)
+    q8 = np.zeros((cap, 16), np.uint32)
+    order = np.zeros(len(prims), np.uint32)
+    last = np.zeros(len(prims), np.uint8)
+    q8map = np.zeros((cap, 5), np.int32)
+    counts = np.zeros(6, np.int32)
+    lib().orc_bvh_layout(_p(nodes), len(nodes), _p(prims), len(prims), _p(f32), _p(q8), _p(q8map), _p(order), _p(last), _p(counts))
+    return f32[:counts[0]], q8[:counts[1]], order, last, counts, q8map[:counts[1]]
 
 
 class OracleScene:

@@ -1,21 +1,45 @@
-/* bvh_layout.cpp - converts a binned-SAH build tree into the traversal formats.
+/* oracle_layout.cpp - CPU restatement of RenderCore_HIP's BVH layout.
+ * TEST INFRASTRUCTURE: loaded only by tests/. There is no reference
+ * counterpart (OptiX lays its trees out inside its closed builder); this is
+ * the checker for the device layout passes (bvh_layout.hip), which must
+ * produce the same arrays from the same build tree.
  *
- * Depth-first ordering puts a node's subtree next to it (cache locality) and
- * lays leaf triangles out in visiting order. The 4-wide format collapses the
- * binary tree (a node adopts the grandchildren of its largest-area interior
- * child until it has 4 children) and stores child boxes as 8-bit offsets from
- * the node's lower corner in power-of-two steps per axis. Quantisation is
- * conservative in the arithmetic the kernel uses: decoded lo = origin + q*step
- * never exceeds the true lo and decoded hi never falls below the true hi
- * (q*step is exact, so a fused multiply-add decodes to the same value), hence
- * the quantised tree reaches every triangle the fp32 tree reaches.
+ * Converts a binned-SAH build tree (breadth-first BuildNode array, root 0,
+ * children in pairs) into the traversal formats. Depth-first ordering puts a
+ * node's subtree next to it and lays leaf triangles out in visiting order. The
+ * 4-wide format collapses the binary tree (a node adopts the grandchildren of
+ * its largest-area interior child until it has 4 children) and stores child
+ * boxes as 8-bit offsets from the node's lower corner in power-of-two steps
+ * per axis, rounded outward.
  */
-#include "bvh.h"
 #include <cmath>
 #include <cstring>
+#include <cstdint>
+#include <vector>
 #include <algorithm>
 
-namespace lh {
+namespace orc_layout {
+
+struct BuildNode { float lo[3]; int32_t leftFirst; float hi[3]; int32_t count; };
+struct float4 { float x, y, z, w; };
+struct uint4 { uint32_t x, y, z, w; };
+static inline float4 make_float4(float x, float y, float z, float w) { return { x, y, z, w }; }
+static inline uint4 make_uint4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { return { x, y, z, w }; }
+struct NodeF32 { float4 lxy, rxy, lrz; uint4 refs; };
+struct NodeQ8 { float4 origin; uint4 qa, qb, refs; };
+static const uint32_t REF_LEAF = 0x80000000u;
+struct Layout
+{
+	std::vector<NodeF32> f32;
+	std::vector<NodeQ8> q8;
+	std::vector<uint32_t> order;
+	std::vector<uint8_t> last;
+	std::vector<int32_t> f32Kids, q8Map;
+	uint32_t rootF32 = 0, rootQ8 = 0;
+	int depthF32 = 0, depthQ8 = 0, stackF32 = 0, stackQ8 = 0;
+};
+
+
 
 static inline float bits_to_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 static inline uint32_t float_to_bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
@@ -243,4 +267,23 @@ void layout_bvh(const std::vector<BuildNode>& nodes, const std::vector<uint32_t>
 	}
 }
 
-} // namespace lh
+} // namespace orc_layout
+
+/* counts: [0] f32 nodes, [1] q8 nodes, [2] f32 root ref, [3] q8 root ref, [4] binary depth, [5] 4-wide depth.
+ * Output arrays sized by the caller for (nodeCount - 1) / 2 nodes (q8MapOut: 5 ints per node) and n triangles. */
+extern "C" __attribute__((visibility("default"))) void orc_bvh_layout(const void* nodes, int nodeCount, const uint32_t* prims, int n,
+	void* f32Out, void* q8Out, int32_t* q8MapOut, uint32_t* orderOut, uint8_t* lastOut, int* counts)
+{
+	using namespace orc_layout;
+	std::vector<BuildNode> nv((const BuildNode*)nodes, (const BuildNode*)nodes + nodeCount);
+	std::vector<uint32_t> pv(prims, prims + n);
+	Layout L;
+	layout_bvh(nv, pv, true, L);
+	memcpy(f32Out, L.f32.data(), L.f32.size() * sizeof(NodeF32));
+	memcpy(q8Out, L.q8.data(), L.q8.size() * sizeof(NodeQ8));
+	memcpy(q8MapOut, L.q8Map.data(), L.q8Map.size() * sizeof(int32_t));
+	memcpy(orderOut, L.order.data(), L.order.size() * 4);
+	memcpy(lastOut, L.last.data(), L.last.size());
+	counts[0] = (int)L.f32.size(), counts[1] = (int)L.q8.size();
+	counts[2] = (int)L.rootF32, counts[3] = (int)L.rootQ8, counts[4] = L.depthF32, counts[5] = L.depthQ8;
+}

@@ -91,3 +91,20 @@ def test_scene_producers():
     c = (view["p1"] + (view["p2"] - view["p1"]) * 0.5 + (view["p3"] - view["p1"]) * 0.5)[0]
     d = c - view["pos"][0]
     assert np.allclose(d / np.linalg.norm(d), (0, 0, 1), atol=1e-6)  # image centre looks at the target
+
+
+def test_set_geometry_rejects_mismatched_arrays():
+    """SetGeometry checks its arrays on the host before the C call reads
+    them: three float4 per triangle and exactly one CoreTri per triangle
+    (a shorter CoreTri array would be over-read by the upload)."""
+    class Dummy:
+        _h = None
+    v = np.zeros((6, 4), np.float32)
+    with pytest.raises(ValueError):
+        lh.RenderCore.SetGeometry(Dummy(), 0, np.zeros((5, 4), np.float32))
+    with pytest.raises(ValueError):
+        lh.RenderCore.SetGeometry(Dummy(), 0, np.zeros((6, 3), np.float32))
+    with pytest.raises(ValueError):
+        lh.RenderCore.SetGeometry(Dummy(), 0, v, F.new_tris(1))
+    with pytest.raises(ValueError):
+        lh.RenderCore.SetGeometry(Dummy(), 0, v, F.new_tris(3))

@@ -362,3 +362,28 @@ def test_morph_target_weight_animation(tmp_path):
     v, t = s.meshes[0]
     assert np.allclose(v[:, 2], 0.5) and np.allclose(t["vertex1"], [1, 0, 0.5])
     assert np.allclose(t["vN0"], [np.sqrt(0.5), 0, np.sqrt(0.5)], atol=1e-6)
+
+
+def test_morph_without_normal_attribute_stays_finite(tmp_path):
+    """A morphed triangle whose primitive has no NORMAL attribute (and whose
+    target has none either) keeps its face normal through a weights update:
+    the base normals are the ones the load produced, never a zero vector."""
+    g = GltfBuilder()
+    m = g.material(pbrMetallicRoughness={"baseColorFactor": [0.5, 0.5, 0.5, 1]})
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], f32)
+    prim = g.primitive(tri, [0, 1, 2], None, None, m)
+    prim["targets"] = [{"POSITION": g.accessor(np.tile([[0, 0, 1]], (3, 1)).astype(f32))}]
+    g.doc["meshes"].append({"primitives": [prim], "weights": [0.0]})
+    g.node(root=True, mesh=0)
+    tin = g.accessor(np.float32([0, 1]))
+    wout = g.accessor(np.float32([0, 1]))
+    g.doc["animations"] = [{"samplers": [{"input": tin, "output": wout}], "channels": [{"sampler": 0, "target": {"node": 0, "path": "weights"}}]}]
+    s = scenes.Scene()
+    anim = L.load_gltf(s, str(g.write_glb(tmp_path / "morph_nonormal.glb")), animated=True)
+    n_load = s.meshes[0][1]["vN0"].copy()
+    anim.update(0.5)
+    v, t = s.meshes[0]
+    assert np.allclose(v[:, 2], 0.5)
+    for c in ("vN0", "vN1", "vN2"):
+        assert np.all(np.isfinite(t[c]))
+    assert np.allclose(t["vN0"], n_load) and np.allclose(np.abs(t["vN0"][:, 2]), 1)
