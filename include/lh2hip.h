@@ -218,6 +218,15 @@ LH2HIP_API uint64_t lh2hip_dn_dw_workspace_bytes(int Cin, int Cout);
 /* dw [9][Cout][Cin] and db [Cout] (fp32, overwritten) of a 3x3 convolution
  * from its input x and output gradient dy; ws: workspace bytes above. */
 LH2HIP_API int lh2hip_dn_conv3x3_dw(const void* x, int Cin, const void* dy, int Cout, int H, int W, float* dw, float* db, void* ws, void* stream);
+/* No counterpart in the reference: lh2hip_dn_conv3x3_dw for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops). The workspace is a file-scope device array of
+ * lh2hip_dn_dw_scratch_bytes() (37.8 MB, each device its own copy, held
+ * until the process ends), enough for every supported shape; shapes beyond
+ * it are rejected. No hipMalloc / hipFree. Calls that overlap on one device
+ * share the array and race; results equal lh2hip_dn_conv3x3_dw's. */
+LH2HIP_API uint64_t lh2hip_dn_dw_scratch_bytes(void);
+LH2HIP_API int lh2hip_dn_conv3x3_dw_noalloc(const void* x, int Cin, const void* dy, int Cout, int H, int W, float* dw, float* db, void* stream);
 LH2HIP_API int lh2hip_dn_relu_bwd(const void* y, const void* dy, void* dyPre, uint64_t n, void* stream);
 LH2HIP_API int lh2hip_dn_pool2(const void* x, int C, int H, int W, void* y, void* stream);
 LH2HIP_API int lh2hip_dn_pool2_bwd(const void* dy, int C, int H, int W, void* dx, int accumulate, void* stream);

@@ -62,6 +62,8 @@ _SIGS = {
     "lh2hip_dn_conv3x3": (I, [P, I, P, P, P, I, I, I, I, P]),
     "lh2hip_dn_dw_workspace_bytes": (ctypes.c_uint64, [I, I]),
     "lh2hip_dn_conv3x3_dw": (I, [P, I, P, I, I, I, P, P, P, P]),
+    "lh2hip_dn_dw_scratch_bytes": (ctypes.c_uint64, []),
+    "lh2hip_dn_conv3x3_dw_noalloc": (I, [P, I, P, I, I, I, P, P, P]),
     "lh2hip_dn_relu_bwd": (I, [P, P, P, ctypes.c_uint64, P]),
     "lh2hip_dn_pool2": (I, [P, I, I, I, P, P]),
     "lh2hip_dn_pool2_bwd": (I, [P, I, I, I, P, I, P]),

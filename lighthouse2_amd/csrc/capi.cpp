@@ -377,6 +377,17 @@ LH2HIP_API int lh2hip_dn_conv3x3_dw(const void* x, int Cin, const void* dy, int 
 		"dn_conv3x3_dw (Cin x Cout in 16x32, 32x32, 64x32, 32x16, 16x16)");
 }
 
+LH2HIP_API uint64_t lh2hip_dn_dw_scratch_bytes(void) { return lh::dn_dw_scratch_bytes(); }
+
+LH2HIP_API int lh2hip_dn_conv3x3_dw_noalloc(const void* x, int Cin, const void* dy, int Cout, int H, int W, float* dw, float* db, void* stream)
+{
+	if (!x || !dy || !dw) return fail(LH2HIP_ERR_INVALID, "dn_conv3x3_dw_noalloc: null pointer");
+	if (lh::dn_dw_workspace_bytes(Cin, Cout) > lh::dn_dw_scratch_bytes())
+		return fail(LH2HIP_ERR_INVALID, "dn_conv3x3_dw_noalloc: the shape's workspace exceeds the built-in scratch array");
+	return dn_status(lh::dn_conv3x3_dw_noalloc((const lh::bf16_t*)x, Cin, (const lh::bf16_t*)dy, Cout, H, W, dw, db, (hipStream_t)stream),
+		"dn_conv3x3_dw_noalloc (Cin x Cout in 16x32, 32x32, 64x32, 32x16, 16x16; W <= 32768)");
+}
+
 LH2HIP_API int lh2hip_dn_relu_bwd(const void* y, const void* dy, void* dyPre, uint64_t n, void* stream)
 {
 	return dn_status(lh::dn_relu_bwd((const lh::bf16_t*)y, (const lh::bf16_t*)dy, (lh::bf16_t*)dyPre, (size_t)n, (hipStream_t)stream), "dn_relu_bwd");

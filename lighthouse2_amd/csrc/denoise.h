@@ -29,7 +29,12 @@ hipError_t dn_conv3x3_cat(const bf16_t* coarse, const bf16_t* skip, const bf16_t
 /* Weight gradient: dw[9][Cout][Cin] = sum_p x[p + tap] dy[p], db[Cout] = sum_p dy[p]
  * (both overwritten). ws: workspace of dn_dw_workspace_bytes(Cin, Cout). */
 size_t dn_dw_workspace_bytes(int Cin, int Cout);
-hipError_t dn_conv3x3_dw(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, int H, int W, float* dw, float* db, void* ws, hipStream_t s);hipError_t dn_relu_bwd(const bf16_t* y, const bf16_t* dy, bf16_t* dyPre, size_t n, hipStream_t s);
+hipError_t dn_conv3x3_dw(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, int H, int W, float* dw, float* db, void* ws, hipStream_t s);
+/* dn_conv3x3_dw with the workspace in a file-scope device array of
+ * dn_dw_scratch_bytes() (no allocation; calls overlapping on one device race). */
+size_t dn_dw_scratch_bytes();
+hipError_t dn_conv3x3_dw_noalloc(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, int H, int W, float* dw, float* db, hipStream_t s);
+hipError_t dn_relu_bwd(const bf16_t* y, const bf16_t* dy, bf16_t* dyPre, size_t n, hipStream_t s);
 /* 2x2 average pool (H, W even) and its gradient (dx = or += dy / 4). */
 hipError_t dn_pool2(const bf16_t* x, int C, int H, int W, bf16_t* y, hipStream_t s);
 hipError_t dn_pool2_bwd(const bf16_t* dy, int C, int H, int W, bf16_t* dx, int accumulate, hipStream_t s);

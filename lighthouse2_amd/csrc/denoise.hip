@@ -19,6 +19,8 @@
  */
 #include "denoise.h"
 
+#include <mutex>
+
 namespace lh {
 
 typedef short s4v __attribute__((__vector_size__(4 * sizeof(short))));
@@ -527,6 +529,46 @@ hipError_t dn_conv3x3_dw(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, i
 		k_reduce_partials<<<1, 64, 0, s>>>(bp, DW_BLOCKS, Cout, db);
 	}
 	return hipGetLastError();
+}
+
+/* ---- weight gradient without allocation --------------------------------
+ * The workspace of dn_conv3x3_dw_noalloc is a file-scope device array sized
+ * for the largest supported shape (Cin 64, Cout 32: DW_BLOCKS x (9 x 32 x 64
+ * + 32) floats, 37.8 MB); each device holds its own copy until the process
+ * ends. Every block partial that the reduction reads is written by the same
+ * call first, so nothing needs clearing between calls; calls that overlap
+ * on one device share the array and race. */
+static constexpr size_t DW_SCRATCH_FLOATS = (size_t)DW_BLOCKS * (9 * 32 * 64 + 32);
+static __device__ __align__(256) float scratch[DW_SCRATCH_FLOATS];
+
+static constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratchLock;
+static void* g_scratchPtr[MAX_DEVICES];   /* device address of scratch per device, looked up once */
+
+size_t dn_dw_scratch_bytes() { return DW_SCRATCH_FLOATS * sizeof(float); }
+
+hipError_t dn_conv3x3_dw_noalloc(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, int H, int W, float* dw, float* db, hipStream_t s)
+{
+	if (dn_dw_workspace_bytes(Cin, Cout) > dn_dw_scratch_bytes()) return hipErrorInvalidValue;
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e != hipSuccess) return e;
+	if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+	void* ptr = nullptr;
+	{
+		std::lock_guard<std::mutex> g(g_scratchLock);
+		if (!g_scratchPtr[dev])
+		{	/* the host-side name is not a device address: ask the runtime for this device's copy */
+			e = hipGetSymbolAddress(&g_scratchPtr[dev], HIP_SYMBOL(scratch));
+			if (e != hipSuccess)
+			{
+				g_scratchPtr[dev] = nullptr;
+				return e;
+			}
+		}
+		ptr = g_scratchPtr[dev];
+	}
+	return dn_conv3x3_dw(x, Cin, dy, Cout, H, W, dw, db, ptr, s);
 }
 
 /* ---- elementwise and layout kernels ------------------------------------ */
