@@ -275,13 +275,34 @@ __device__ v3 random_point_on_light(const ShadeParams& P, float r0, float r1, v3
 {
 	const int lightCount = P.nTri + P.nPoint + P.nSpot + P.nDir;
 	const v3 bary = random_barycentrics(r0);
+	/* One pass sums the potentials and keeps the running sum at the end of
+	 * each of 8 equal segments of the light list. The search for the first
+	 * light whose running sum reaches r1 then starts from the checkpoint
+	 * before the segment that holds it: the sums are formed by the same
+	 * additions in the same order as a full second pass (potentials are
+	 * non-negative, so running sums never decrease), at most one segment of
+	 * potentials is evaluated again. */
+	const int seg = (lightCount + 7) >> 3;
+	float ck[8];
 	float sum = 0;
-	for (int i = 0; i < lightCount; i++) sum += potential(P, i, I, N, bary);
+#pragma unroll
+	for (int j = 0; j < 8; j++)
+	{
+		const int e = min((j + 1) * seg, lightCount);
+		for (int i = j * seg; i < e; i++) sum += potential(P, i, I, N, bary);
+		ck[j] = sum;
+	}
 	if (sum <= 0) { lightPdf = 0; return mk3(1, 1, 1); }
 	r1 *= sum;
+	int first = 7;
+	float total = ck[6];
+#pragma unroll
+	for (int j = 6; j >= 0; j--)
+		if (ck[j] >= r1) first = j, total = j > 0 ? ck[j - 1] : 0.0f;
 	int lightIdx = 0;
-	float total = 0, pl = 0;
-	for (int i = 0; i < lightCount; i++)
+	float pl = 0;
+	const int e = min((first + 1) * seg, lightCount);
+	for (int i = first * seg; i < e; i++)
 	{
 		const float c = potential(P, i, I, N, bary);
 		total += c;
