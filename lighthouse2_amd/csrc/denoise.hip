@@ -14,8 +14,9 @@
  * Weight gradient: M = output channels, N = input channels, K = pixels;
  * a 64-pixel tile of dy and the 3 x 66 pixel input halo are staged in LDS
  * channel-major so that both operands read consecutive pixels; each block
- * keeps partial sums for its tiles and a second kernel adds the partials in
- * block order (deterministic).
+ * keeps partial sums for its tiles (the bias gradient from the same staged
+ * rows) and a second kernel adds the partials in a fixed order
+ * (deterministic).
  */
 #include "denoise.h"
 
@@ -368,14 +369,21 @@ __device__ __forceinline__ uint2 column4(const uint4 (&v)[4], int c)
 	return make_uint2((d[0] & 0xffffu) | (d[1] << 16), (d[2] & 0xffffu) | (d[3] << 16));
 }
 
+template <int COUT>
+__device__ __forceinline__ bool tid_in_bias(int tid) { return tid < 16 * COUT; }
+
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(576) k_conv3x3_dw(const bf16_t* __restrict__ x, const bf16_t* __restrict__ dy, int H, int W,
-	int rowsPerChunk, float* __restrict__ partial)
+	int rowsPerChunk, float* __restrict__ partial, float* __restrict__ partialB)
 {
 	constexpr int MB = COUT / 16, NB = CIN / 16, KP = 64 + 8;
 	constexpr int XL = 2 * CIN, DL = 6 * COUT;   /* staging lanes: 16 groups of 4 pixels x CIN / 8 channel groups; 3 shifts x 16 x COUT / 8 */
 	__shared__ __attribute__((aligned(16))) bf16_t xs[3][CIN][KP];
 	__shared__ __attribute__((aligned(16))) bf16_t ds[3][COUT][KP];
+	__shared__ float bred[16][COUT];
+	/* bias gradient: lane (channel tid % COUT, pixel group tid / COUT) adds 4 pixels of each row */
+	const bool isB = partialB != nullptr && tid_in_bias<COUT>(threadIdx.x);
+	float bsum = 0.0f;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int r = wave / 3, dxo = wave % 3, sh = 2 - dxo;
 	const int tilesX = (W + 63) / 64;
@@ -432,6 +440,12 @@ __global__ void __launch_bounds__(576) k_conv3x3_dw(const bf16_t* __restrict__ x
 				if (isX) load(ty + 2);
 				else if (isD) load(ty + 1);
 			}
+			if (isB)
+			{	/* the unshifted copy holds row ty at pixels x0 .. x0 + 63 (zero beyond W) */
+				const uint2 v = *(const uint2*)&ds[1][tid % COUT][(tid / COUT) * 4];
+				bsum += bf2f((bf16_t)(v.x & 0xffffu)), bsum += bf2f((bf16_t)(v.x >> 16));
+				bsum += bf2f((bf16_t)(v.y & 0xffffu)), bsum += bf2f((bf16_t)(v.y >> 16));
+			}
 			const int xslot = slotOf(ty - 1 + r);
 #pragma unroll
 			for (int kc = 0; kc < 2; kc++)
@@ -450,6 +464,17 @@ __global__ void __launch_bounds__(576) k_conv3x3_dw(const bf16_t* __restrict__ x
 			__syncthreads();
 		}
 	}
+	if (partialB)
+	{	/* this block's bias partial: the 16 pixel groups of each channel added in group order */
+		if (isB) bred[tid / COUT][tid % COUT] = bsum;
+		__syncthreads();
+		if (tid < COUT)
+		{
+			float t = 0.0f;
+			for (int g = 0; g < 16; g++) t += bred[g][tid];
+			partialB[(size_t)blockIdx.x * COUT + tid] = t;
+		}
+	}
 	/* D layout: lane holds rows 4 (lane / 16) + i, column lane % 16 */
 	float* out = partial + (size_t)blockIdx.x * 9 * COUT * CIN;
 	const int tap = wave;
@@ -462,44 +487,34 @@ __global__ void __launch_bounds__(576) k_conv3x3_dw(const bf16_t* __restrict__ x
 				out[((size_t)tap * COUT + mb * 16 + (lane >> 4) * 4 + i) * CIN + nb * 16 + (lane & 15)] = acc[mb][nb][i];
 }
 
-/* per-block partial sums of dy over channels (bias gradient) */
-/* Bias gradient partial sums: block b sums a contiguous range of pixels;
- * 256 threads = (256 / C) pixel lanes x C channels read whole pixels
- * (coalesced); the pixel lanes are then added in lane order. */
-__global__ void __launch_bounds__(256) k_bias_partial(const bf16_t* __restrict__ dy, int C, size_t pixels, float* __restrict__ partial)
+/* out[i] = sum over b < blocks of partial[b][i] in a fixed order: 8 lanes
+ * per output each add every 8th block partial (32 consecutive outputs per
+ * block row, coalesced), then the 8 sums are added in lane order. */
+__global__ void __launch_bounds__(256) k_reduce_partials(const float* __restrict__ partial, int blocks, int n, float* __restrict__ out)
 {
-	__shared__ float red[256];
-	const int P = 256 / C, c = threadIdx.x % C, pl = threadIdx.x / C;
-	const size_t chunk = (pixels + gridDim.x - 1) / gridDim.x;
-	const size_t p0 = blockIdx.x * chunk, p1 = p0 + chunk < pixels ? p0 + chunk : pixels;
+	__shared__ float red[8][33];
+	const int lane = threadIdx.x & 31, part = threadIdx.x >> 5;
+	const int i = blockIdx.x * 32 + lane;
 	float s = 0.0f;
-	if (pl < P)
-		for (size_t p = p0 + pl; p < p1; p += P) s += bf2f(dy[p * C + c]);
-	red[threadIdx.x] = s;
+	if (i < n)
+		for (int b = part; b < blocks; b += 8) s += partial[(size_t)b * n + i];
+	red[part][lane] = s;
 	__syncthreads();
-	if (threadIdx.x < C)
+	if (part == 0 && i < n)
 	{
 		float t = 0.0f;
-		for (int k = 0; k < P; k++) t += red[k * C + threadIdx.x];
-		partial[(size_t)blockIdx.x * C + threadIdx.x] = t;
+#pragma unroll
+		for (int k = 0; k < 8; k++) t += red[k][lane];
+		out[i] = t;
 	}
-}
-
-__global__ void k_reduce_partials(const float* __restrict__ partial, int blocks, int n, float* __restrict__ out)
-{
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	float s = 0.0f;
-	for (int b = 0; b < blocks; b++) s += partial[(size_t)b * n + i];
-	out[i] = s;
 }
 
 size_t dn_dw_workspace_bytes(int Cin, int Cout) { return (size_t)DW_BLOCKS * (9 * Cout * Cin + Cout) * sizeof(float); }
 
 template <int CIN, int COUT>
-static void launch_dw(const bf16_t* x, const bf16_t* dy, int H, int W, int rowsPerChunk, float* partial, int blocks, hipStream_t s)
+static void launch_dw(const bf16_t* x, const bf16_t* dy, int H, int W, int rowsPerChunk, float* partial, float* partialB, int blocks, hipStream_t s)
 {
-	k_conv3x3_dw<CIN, COUT><<<blocks, 576, 0, s>>>(x, dy, H, W, rowsPerChunk, partial);
+	k_conv3x3_dw<CIN, COUT><<<blocks, 576, 0, s>>>(x, dy, H, W, rowsPerChunk, partial, partialB);
 }
 
 hipError_t dn_conv3x3_dw(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, int H, int W, float* dw, float* db, void* ws, hipStream_t s)
@@ -514,20 +529,16 @@ hipError_t dn_conv3x3_dw(const bf16_t* x, int Cin, const bf16_t* dy, int Cout, i
 	const int blocks = tilesX * ((H + rowsPerChunk - 1) / rowsPerChunk);
 	if (blocks > DW_BLOCKS) return hipErrorInvalidValue;   /* W beyond DW_BLOCKS * 64 pixels */
 	float* partial = (float*)ws;
-	if (Cin == 16 && Cout == 32) launch_dw<16, 32>(x, dy, H, W, rowsPerChunk, partial, blocks, s);
-	else if (Cin == 32 && Cout == 32) launch_dw<32, 32>(x, dy, H, W, rowsPerChunk, partial, blocks, s);
-	else if (Cin == 64 && Cout == 32) launch_dw<64, 32>(x, dy, H, W, rowsPerChunk, partial, blocks, s);
-	else if (Cin == 32 && Cout == 16) launch_dw<32, 16>(x, dy, H, W, rowsPerChunk, partial, blocks, s);
-	else if (Cin == 16 && Cout == 16) launch_dw<16, 16>(x, dy, H, W, rowsPerChunk, partial, blocks, s);
-	else return hipErrorInvalidValue;
 	const int n = 9 * Cout * Cin;
-	k_reduce_partials<<<(n + 255) / 256, 256, 0, s>>>(partial, blocks, n, dw);
-	if (db)
-	{
-		float* bp = partial + (size_t)DW_BLOCKS * n;
-		k_bias_partial<<<DW_BLOCKS, 256, 0, s>>>(dy, Cout, (size_t)H * W, bp);
-		k_reduce_partials<<<1, 64, 0, s>>>(bp, DW_BLOCKS, Cout, db);
-	}
+	float* bp = db ? partial + (size_t)DW_BLOCKS * n : nullptr;   /* bias partials of the same blocks */
+	if (Cin == 16 && Cout == 32) launch_dw<16, 32>(x, dy, H, W, rowsPerChunk, partial, bp, blocks, s);
+	else if (Cin == 32 && Cout == 32) launch_dw<32, 32>(x, dy, H, W, rowsPerChunk, partial, bp, blocks, s);
+	else if (Cin == 64 && Cout == 32) launch_dw<64, 32>(x, dy, H, W, rowsPerChunk, partial, bp, blocks, s);
+	else if (Cin == 32 && Cout == 16) launch_dw<32, 16>(x, dy, H, W, rowsPerChunk, partial, bp, blocks, s);
+	else if (Cin == 16 && Cout == 16) launch_dw<16, 16>(x, dy, H, W, rowsPerChunk, partial, bp, blocks, s);
+	else return hipErrorInvalidValue;
+	k_reduce_partials<<<(n + 31) / 32, 256, 0, s>>>(partial, blocks, n, dw);
+	if (db) k_reduce_partials<<<(Cout + 31) / 32, 256, 0, s>>>(bp, blocks, Cout, db);
 	return hipGetLastError();
 }
 
