@@ -57,6 +57,10 @@ def parse():
     ap.add_argument("--serve-jobs", type=int, default=0, help="jobs of the serving stream (0: 200 in the default run, 2000 alone)")
     ap.add_argument("--headline-only", action="store_true", help="skip the frame, serve and parity sub-objects")
     ap.add_argument("--serve-weights", default="", help="learned-denoiser weight blob for the serving stream (default: seeded init)")
+    ap.add_argument("--poses", type=int, default=2000, help="learned: training poses (held-out poses: 16)")
+    ap.add_argument("--train-steps", type=int, default=8000, help="learned: Adam steps (batch 1)")
+    ap.add_argument("--oracle-crops", type=int, default=4, help="learned: held-out poses checked against FP64-oracle crops")
+    ap.add_argument("--weights-out", default="", help="learned: write the trained weight blob here")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (FETCH_SIZE/WRITE_SIZE calibration)")
     return ap.parse_args()
@@ -261,54 +265,101 @@ def bench_frame(a, np, torch, lh, scenes):
     print(json.dumps(frame_section(a, np, torch, lh, scenes)))
 
 
+def crop_view(np, view, W, H, x0, y0, cw, ch):
+    """ViewPyramid of the pixels [x0, x0 + cw) x [y0, y0 + ch) of a W x H view
+    (same pixel size and spread angle, screen corners moved)."""
+    v = view.copy()
+    p1, p2, p3 = (np.asarray(view[k][0], np.float64) for k in ("p1", "p2", "p3"))
+    at = lambda x, y: p1 + (x / W) * (p2 - p1) + (y / H) * (p3 - p1)
+    v["p1"], v["p2"], v["p3"] = at(x0, y0), at(x0 + cw, y0), at(x0, y0 + ch)
+    return v
+
+
 def bench_learned(a, np, torch, lh, scenes):
     """The learned denoiser against SVGF on the instanced workload: training
     pairs rendered by the core (1 spp filter-estimator frames with their
-    G-buffer vs 1024-spp renders of the same poses, 256x256), trained with the
-    library's own kernels; relMSE on held-out poses for the 1-spp frame, SVGF
-    (one frame, and after 4 still frames of history) and the network; and
-    ms/frame of SVGF and of the network's forward pass at 1080p."""
+    G-buffer vs 1024-spp renders of the same poses, 256x256; 2,000 training
+    poses by default), trained with the library's own kernels; relMSE on
+    held-out poses for the 1-spp frame, SVGF (one frame, and after 4 still
+    frames of history) and the network against the 1024-spp renders, and, on
+    the CPU leg, against FP64-oracle renders of 64x64 crops of held-out poses;
+    ms/frame of the network's forward pass at 1080p."""
     from lighthouse2_amd import denoiser as D
     from lighthouse2_amd import training as T
     t0 = time.time()
     s = scenes.instanced_scene()
     w = h = 256
-    n_train, n_eval = 96, 16
+    n_train, n_eval = a.poses, 16
     pr = T.PairRenderer(s, w, h, ref_spp=1024, spp_per_frame=128)
     views = T.views_near_instances(s, n_train + n_eval, w, h, seed=6)
-    data = [pr.render(v) for v in views]
+    samples = []
+    for k, v in enumerate(views[:n_train]):
+        d = pr.render(v)
+        samples.append((T.network_inputs(torch, d), T.network_target(torch, d)))
+        if (k + 1) % 100 == 0:
+            print(f"rendered {k + 1} / {n_train} training pairs ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
+    data = [pr.render(v) for v in views[n_train:]]
     hist = [pr.render(v, history=4)["svgf"] for v in views[n_train:]]
     pr.close()
     t_data = time.time() - t0
-    samples = [(T.network_inputs(torch, d), T.network_target(torch, d)) for d in data[:n_train]]
     net = D.UNetDenoiser(h, w, seed=0)
     t0 = time.time()
-    steps = max(200, a.steps * 200)
+    steps = a.train_steps
     T.train(net, torch, samples, steps, lr=1e-3)
     torch.cuda.synchronize()
     t_train = time.time() - t0
+    print(f"trained {steps} steps in {t_train:.0f} s", file=sys.stderr, flush=True)
     err = {"1spp": [], "svgf": [], "svgf_4_frames": [], "learned": []}
-    for d, sv4 in zip(data[n_train:], hist):
+    outs = []
+    for d, sv4 in zip(data, hist):
         net.set_input(*T.network_inputs(torch, d))
         out = T.network_radiance(net.forward().cpu().numpy(), d)
+        outs.append(out)
         err["1spp"].append(T.relmse(d["radiance"], d["ref"]))
         err["svgf"].append(T.relmse(d["svgf"][1:-1, 1:-1], d["ref"][1:-1, 1:-1]))
         err["svgf_4_frames"].append(T.relmse(sv4[1:-1, 1:-1], d["ref"][1:-1, 1:-1]))
         err["learned"].append(T.relmse(out, d["ref"]))
     err = {k: round(float(np.mean(v)), 5) for k, v in err.items()}
-    # ms/frame at 1080p: network forward (1088x1920 padded) and SVGF
+    if a.weights_out:
+        open(a.weights_out, "wb").write(net.weights_blob())
+    # ms/frame at 1080p: network forward (1088x1920 padded)
     net2 = D.UNetDenoiser(1088, 1920, seed=0, params=net.params_numpy())
     z = torch.zeros((1088, 1920, 4), dtype=torch.float32, device="cuda")
     net2.set_input(z, z, z, z[..., 0].contiguous())
     for _ in range(3):
         net2.forward()
     fwd_ms = _events(torch, net2.stream, net2.forward, 10)
-    print(json.dumps({"metric": "relMSE vs 1024 spp, held-out poses", "value": err["learned"], "unit": "relMSE", "higher_is_better": False,
-                      "n_gpus": 1, "steps": steps, "warmup": 0, "dtype": "bf16 (fp32 accumulate)", "data": "rendered by the core",
-                      "vs_baseline": None, "relmse": err, "learned_forward_ms_1080p": round(fwd_ms, 3),
-                      "config": {"workload": f"instanced scene, {w}x{h}, {n_train} training / {n_eval} held-out poses, "
-                                             f"1 spp inputs vs 1024 spp targets, {steps} Adam steps (batch 1)"},
-                      "data_s": round(t_data, 1), "train_s": round(t_train, 1)}))
+    line = {"metric": "relMSE vs 1024 spp, held-out poses", "value": err["learned"], "unit": "relMSE", "higher_is_better": False,
+            "n_gpus": 1, "steps": steps, "warmup": 0, "dtype": "bf16 (fp32 accumulate)", "data": "rendered by the core",
+            "vs_baseline": None, "relmse": err, "learned_forward_ms_1080p": round(fwd_ms, 3),
+            "config": {"workload": f"instanced scene, {w}x{h}, {n_train} training / {n_eval} held-out poses, "
+                                   f"1 spp inputs vs 1024 spp targets, {steps} Adam steps (batch 1)"},
+            "data_s": round(t_data, 1), "train_s": round(t_train, 1)}
+    if not a.no_cpu:
+        # FP64 oracle: 64x64 centre crops of held-out poses at 1024 spp (filter estimator, no SVGF)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        threads = min(16, os.cpu_count() or 1)
+        cw = ch = 64
+        x0 = y0 = (w - cw) // 2
+        o = orc.OracleScene(s)
+        crop = {"1spp": [], "svgf": [], "learned": [], "gpu_1024spp_target": []}
+        t0 = time.time()
+        for d, out, v in list(zip(data, outs, views[n_train:]))[:a.oracle_crops]:
+            ref = orc.RenderF(o, cw, ch, spp=1024, filter_on=False, taa=False, threads=threads)
+            ref.frame(crop_view(np, v, w, h, x0, y0, cw, ch), orc.RESTART)
+            r64 = ((ref.acc64[0] + ref.acc64[1]) / 1024.0).reshape(ch, cw, 4)[..., :3]
+            sl = (slice(y0, y0 + ch), slice(x0, x0 + cw))
+            crop["1spp"].append(T.relmse(d["radiance"][sl], r64))
+            crop["svgf"].append(T.relmse(d["svgf"][sl], r64))
+            crop["learned"].append(T.relmse(out[sl], r64))
+            crop["gpu_1024spp_target"].append(T.relmse(d["ref"][sl], r64))
+            print(f"oracle crop {len(crop['learned'])} done ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
+        line["relmse_vs_fp64_oracle_crops"] = {k: round(float(np.mean(v)), 5) for k, v in crop.items()}
+        line["oracle_crops"] = f"{len(crop['learned'])} held-out poses, {cw}x{ch} centre crop, 1024 spp, FP64 accumulation, {threads} threads"
+        line["cpu_baseline"] = {"value": round((time.time() - t0) / max(1, len(crop["learned"])), 1), "unit": "s per 64x64 crop at 1024 spp",
+                                "cores": threads, "cpu": cpu_model(), "kind": "port", "sample": line["oracle_crops"]}
+    print(json.dumps(line))
 
 
 def serve_section(a, np, torch, lh, scenes, n_jobs):
@@ -346,12 +397,16 @@ def serve_section(a, np, torch, lh, scenes, n_jobs):
             t += gap
             time.sleep(max(0.0, t - time.time()))
             ids.append(srv.submit(scenes.instanced_view(w, h), w, h, spp, d, 1000 + k, denoise=dn))
+            if (k + 1) % 200 == 0:
+                print(f"serve: {k + 1} / {n_jobs} jobs submitted ({time.time() - t_start:.0f} s)", file=sys.stderr, flush=True)
     th = threading.Thread(target=submitter)
     th.start()
     th.join()
     lat, paths = [], 0
-    for j, ((w, h), spp, d, dn) in zip(ids, jobs):
+    for k, (j, ((w, h), spp, d, dn)) in enumerate(zip(ids, jobs)):
         srv.wait(j)
+        if (k + 1) % 100 == 0:
+            print(f"serve: {k + 1} / {n_jobs} jobs done ({time.time() - t_start:.0f} s)", file=sys.stderr, flush=True)
         t = srv.times(j)
         lat.append(t[2] - t[0])
         paths += w * h * spp
