@@ -38,6 +38,10 @@ __device__ __forceinline__ bf16_t f2bf(float f)
 
 static const int DW_BLOCKS = 512;
 
+/* Rows per block of the forward / input-gradient convolution's tiles on
+ * large images (a wave per row, 64 pixels wide). */
+static constexpr int TILE_ROWS = 8;
+
 /* ---- 3x3 convolution, forward and input gradient ------------------------ */
 template <int NB, bool RELU, bool F32OUT>
 __global__ void __launch_bounds__(256) k_conv3x3(const bf16_t* __restrict__ x, int Cin, const bf16_t* __restrict__ w,
@@ -96,38 +100,42 @@ __global__ void __launch_bounds__(256) k_conv3x3(const bf16_t* __restrict__ x, i
 	}
 }
 
-/* Tiled variant for the network's shapes: a block of 4 waves computes 4 rows
- * x 64 pixels (one row per wave). The 6 x 66 pixel input halo is staged once
+/* Tiled variant for the network's shapes: a block of ROWS waves computes
+ * ROWS rows x 16 MBW pixels (one row per wave; 8 x 64 on large images, 4 x 64
+ * with 64 input channels, 4 x 16 on small ones). The (ROWS + 2) x (16 MBW + 2)
+ * pixel input halo is staged once
  * in LDS with 16-byte loads, all of a thread's loads in flight together, CP
  * channels per pixel (8 of padding against bank conflicts). Each wave then
  * runs the 9 taps as v_mfma_f32_16x16x32_bf16 products with
  * A = 16 output channels x 32 input channels of the [tap][Cout][Cin] weights
  * (read from memory one tap ahead of their use) and B = 32 input channels x
  * 16 pixels from LDS (lane: pixel lane % 16, channels 8 (lane / 16) + 0..7);
- * D then gives each lane 4 consecutive output channels of one pixel, stored
- * as one 8-byte (bf16) or 16-byte (fp32) write.
+ * D then gives each lane 4 consecutive output channels of one pixel; bf16
+ * outputs pair channel blocks across lanes for 16-byte writes (below).
  * CIN = 16 is padded to K = 32 with zero channels. CAT: the input is the
  * 64-channel concatenation [nearest 2x upsample of coarse (32 channels),
- * x (32 channels)]; the 4 x 34 coarse pixels under the tile are staged
+ * x (32 channels)]; the coarse pixels under the tile (and their halo) are staged
  * separately and upsampled by the fragment addressing. */
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
-template <int CIN, int NB, bool RELU, bool F32OUT, bool CAT, bool POOL = false>
-__global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x, const bf16_t* __restrict__ coarse,
+template <int CIN, int NB, bool RELU, bool F32OUT, bool CAT, bool POOL = false, int MBW = 4, int ROWS = 4>
+__global__ void __launch_bounds__(64 * ROWS) k_conv3x3_t(const bf16_t* __restrict__ x, const bf16_t* __restrict__ coarse,
 	const bf16_t* __restrict__ w, const float* __restrict__ bias, void* __restrict__ y, int H, int W, bf16_t* __restrict__ ypool)
 {
 	constexpr int CX = CAT ? 32 : CIN;                  /* channels of x */
 	constexpr int CKX = CX < 32 ? 32 : CX;              /* channels per pixel of the x tile (K, padded) */
-	constexpr int CP = CKX + 8, TW = 66, TR = 6, CH8 = CX / 8, CHP = CKX / 8;
+	constexpr int CP = CKX + 8, TW = 16 * MBW + 2, TR = ROWS + 2, CH8 = CX / 8, CHP = CKX / 8;
 	constexpr int KS = CAT ? 2 : CKX / 32;              /* 32-channel steps of K per tap */
 	constexpr int WCIN = CAT ? 64 : CIN;                /* weight row length */
-	constexpr int CPC = 40, TWC = 34, TRC = 4;          /* coarse tile */
+	constexpr int CPC = 40, TWC = 8 * MBW + 2, TRC = ROWS / 2 + 2; /* coarse tile */
+	constexpr int NT = 64 * ROWS;                       /* threads per block: a wave per output row */
 	constexpr int Cout = NB * 16;
 	__shared__ __attribute__((aligned(16))) bf16_t xs[TR * TW * CP];
 	__shared__ __attribute__((aligned(16))) bf16_t cs[CAT ? TRC * TWC * CPC : 8];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, ml = lane & 15, kg = lane >> 4;
-	const int x0 = blockIdx.x * 64, r0 = blockIdx.y * 4;
+	const int tx = blockIdx.x, ty = blockIdx.y;
+	const int x0 = tx * (16 * MBW), r0 = ty * ROWS;
 	{
-		constexpr int N = TR * TW * CHP, IT = (N + 255) / 256, BATCH = 8;
+		constexpr int N = TR * TW * CHP, IT = (N + NT - 1) / NT, BATCH = 8;
 #pragma unroll
 		for (int it0 = 0; it0 < IT; it0 += BATCH)
 		{
@@ -135,7 +143,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 #pragma unroll
 			for (int j = 0; j < BATCH; j++)
 			{
-				const int i = tid + (it0 + j) * 256;
+				const int i = tid + (it0 + j) * NT;
 				const int c8 = i % CHP, p = (i / CHP) % TW, gy = r0 - 1 + i / (CHP * TW), gx = x0 - 1 + p;
 				v[j] = it0 + j < IT && i < N && c8 < CH8 && gy >= 0 && gy < H && gx >= 0 && gx < W ?
 					*(const uint4*)(x + ((size_t)gy * W + gx) * CX + c8 * 8) : make_uint4(0, 0, 0, 0);
@@ -143,26 +151,26 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 #pragma unroll
 			for (int j = 0; j < BATCH; j++)
 			{
-				const int i = tid + (it0 + j) * 256;
+				const int i = tid + (it0 + j) * NT;
 				if (it0 + j < IT && i < N) *(uint4*)(xs + (i / CHP) * CP + (i % CHP) * 8) = v[j];
 			}
 		}
 		if (CAT)
 		{
-			constexpr int NC = TRC * TWC * 4, ITC = (NC + 255) / 256;
+			constexpr int NC = TRC * TWC * 4, ITC = (NC + NT - 1) / NT;
 			const int Hc = H >> 1, Wc = W >> 1;
 			uint4 v[ITC];
 #pragma unroll
 			for (int j = 0; j < ITC; j++)
 			{
-				const int i = tid + j * 256;
+				const int i = tid + j * NT;
 				const int c8 = i % 4, p = (i / 4) % TWC, cy = (r0 >> 1) - 1 + i / (4 * TWC), cx = (x0 >> 1) - 1 + p;
 				v[j] = i < NC && cy >= 0 && cy < Hc && cx >= 0 && cx < Wc ? *(const uint4*)(coarse + ((size_t)cy * Wc + cx) * 32 + c8 * 8) : make_uint4(0, 0, 0, 0);
 			}
 #pragma unroll
 			for (int j = 0; j < ITC; j++)
 			{
-				const int i = tid + j * 256;
+				const int i = tid + j * NT;
 				if (i < NC) *(uint4*)(cs + (i / 4) * CPC + (i % 4) * 8) = v[j];
 			}
 		}
@@ -176,9 +184,9 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 	for (int ks = 0; ks < KS; ks++)
 #pragma unroll
 		for (int nb = 0; nb < NB; nb++) wc[ks][nb] = wfrag(0, ks, nb);
-	f4v acc[4][NB];
+	f4v acc[MBW][NB];
 #pragma unroll
-	for (int mb = 0; mb < 4; mb++)
+	for (int mb = 0; mb < MBW; mb++)
 #pragma unroll
 		for (int nb = 0; nb < NB; nb++) acc[mb][nb] = (f4v){ 0.0f, 0.0f, 0.0f, 0.0f };
 	__syncthreads();
@@ -194,7 +202,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 #pragma unroll
 		for (int ks = 0; ks < KS; ks++)
 #pragma unroll
-			for (int mb = 0; mb < 4; mb++)
+			for (int mb = 0; mb < MBW; mb++)
 			{
 				const int m = mb * 16 + ml;
 				bf8v a;
@@ -209,7 +217,14 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 			for (int nb = 0; nb < NB; nb++) wc[ks][nb] = wn[ks][nb];
 	}
 	const int ry = r0 + wave;
-	float hs[POOL ? 4 : 1][NB][4];                       /* POOL: sums of horizontal pixel pairs */
+	float hs[POOL ? MBW : 1][NB][4];                     /* POOL: sums of horizontal pixel pairs */
+	/* bf16 output with channel blocks in pairs: lanes kg and kg ^ 1 swap half
+	 * their fragments so that each lane writes 8 consecutive channels (16 B)
+	 * and one store instruction covers whole 64 B pixel rows; kg even keeps
+	 * block nb (channels 4 kg .. 4 kg + 7), kg odd block nb + 1 (channels
+	 * 16 + 4 (kg - 1) .. + 7). */
+	constexpr bool PAIRED = !F32OUT && (NB % 2) == 0;
+	uint32_t pk[MBW][NB][2];
 #pragma unroll
 	for (int nb = 0; nb < NB; nb++)
 	{
@@ -218,7 +233,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 #pragma unroll
 		for (int i = 0; i < 4; i++) bv[i] = bias ? bias[ch + i] : 0.0f;
 #pragma unroll
-		for (int mb = 0; mb < 4; mb++)
+		for (int mb = 0; mb < MBW; mb++)
 		{
 			const int px = x0 + mb * 16 + ml;
 			float v[4];
@@ -229,7 +244,9 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 				if (RELU) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
 			}
 			const size_t o = ((size_t)ry * W + px) * Cout + ch;
-			if (ry < H && px < W)
+			if (PAIRED)
+				pk[mb][nb][0] = (uint32_t)f2bf(v[0]) | (uint32_t)f2bf(v[1]) << 16, pk[mb][nb][1] = (uint32_t)f2bf(v[2]) | (uint32_t)f2bf(v[3]) << 16;
+			else if (ry < H && px < W)
 			{
 				if (F32OUT) *(float4*)((float*)y + o) = make_float4(v[0], v[1], v[2], v[3]);
 				else *(uint2*)((bf16_t*)y + o) = make_uint2((uint32_t)f2bf(v[0]) | (uint32_t)f2bf(v[1]) << 16,
@@ -244,6 +261,24 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 				}
 		}
 	}
+	if (PAIRED)
+	{
+		const bool even = !(kg & 1);
+#pragma unroll
+		for (int mb = 0; mb < MBW; mb++)
+		{
+			const int px = x0 + mb * 16 + ml;
+#pragma unroll
+			for (int nb = 0; nb < NB; nb += 2)
+			{
+				const uint32_t s0 = even ? pk[mb][nb + 1][0] : pk[mb][nb][0], s1 = even ? pk[mb][nb + 1][1] : pk[mb][nb][1];
+				const uint32_t r0 = (uint32_t)__shfl_xor((int)s0, 16), r1 = (uint32_t)__shfl_xor((int)s1, 16);
+				const uint4 q = even ? make_uint4(pk[mb][nb][0], pk[mb][nb][1], r0, r1) : make_uint4(r0, r1, pk[mb][nb + 1][0], pk[mb][nb + 1][1]);
+				const int ch = even ? nb * 16 + kg * 4 : (nb + 1) * 16 + (kg - 1) * 4;
+				if (ry < H && px < W) *(uint4*)((bf16_t*)y + ((size_t)ry * W + px) * Cout + ch) = q;
+			}
+		}
+	}
 	if (POOL)
 	{
 		/* 2x2 average: (top pair) + (bottom pair), as dn_pool2 on the stored tensor.
@@ -252,16 +287,16 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 		__syncthreads();                                  /* every wave is done with the staged tile */
 		if (wave & 1)
 #pragma unroll
-			for (int mb = 0; mb < 4; mb++)
+			for (int mb = 0; mb < MBW; mb++)
 #pragma unroll
 				for (int nb = 0; nb < NB; nb++)
 #pragma unroll
-					for (int i = 0; i < 4; i++) ex[(((wave >> 1) * 4 + mb) * NB * 4 + nb * 4 + i) * 64 + lane] = hs[POOL ? mb : 0][nb][i];
+					for (int i = 0; i < 4; i++) ex[(((wave >> 1) * MBW + mb) * NB * 4 + nb * 4 + i) * 64 + lane] = hs[POOL ? mb : 0][nb][i];
 		__syncthreads();
 		const int Wp = W >> 1, pr = (r0 >> 1) + (wave >> 1);
 		if (!(wave & 1) && ry + 1 < H && !(ml & 1))
 #pragma unroll
-			for (int mb = 0; mb < 4; mb++)
+			for (int mb = 0; mb < MBW; mb++)
 			{
 				const int px = x0 + mb * 16 + ml;
 				if (px >= W) continue;
@@ -271,7 +306,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 					bf16_t q[4];
 #pragma unroll
 					for (int i = 0; i < 4; i++)
-						q[i] = f2bf((hs[POOL ? mb : 0][nb][i] + ex[(((wave >> 1) * 4 + mb) * NB * 4 + nb * 4 + i) * 64 + lane]) * 0.25f);
+						q[i] = f2bf((hs[POOL ? mb : 0][nb][i] + ex[(((wave >> 1) * MBW + mb) * NB * 4 + nb * 4 + i) * 64 + lane]) * 0.25f);
 					*(uint2*)(ypool + ((size_t)pr * Wp + (px >> 1)) * Cout + nb * 16 + kg * 4) =
 						make_uint2((uint32_t)q[0] | (uint32_t)q[1] << 16, (uint32_t)q[2] | (uint32_t)q[3] << 16);
 				}
@@ -279,15 +314,33 @@ __global__ void __launch_bounds__(256) k_conv3x3_t(const bf16_t* __restrict__ x,
 	}
 }
 
+/* Images with fewer than SMALL_TILES 64-pixel-wide tiles of 4 rows (levels
+ * 3 and 4 of the network at 1080p) run 4 x 16 tiles: four times the blocks,
+ * so that a small layer spreads over more compute units instead of running
+ * at the latency of a few blocks. (At level 2, 4 x 16 tiles were slower: a
+ * block's nine dependent weight loads then outweigh its products.) */
+static const int SMALL_TILES = 256;
+static bool small_image(int H, int W) { return ((W + 63) / 64) * ((H + 3) / 4) < SMALL_TILES; }
+
+
+template <int CIN, int NB, bool CAT, int MBW, int ROWS>
+static void conv_t_w(const bf16_t* x, const bf16_t* coarse, const bf16_t* w, const float* bias, void* y, int H, int W, int flags, hipStream_t s)
+{
+	const dim3 grid((W + 16 * MBW - 1) / (16 * MBW), (H + ROWS - 1) / ROWS), block(64 * ROWS);
+	const bool relu = flags & DN_RELU, f32 = flags & DN_OUT_F32;
+	if (relu && f32) k_conv3x3_t<CIN, NB, true, true, CAT, false, MBW, ROWS><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
+	else if (relu) k_conv3x3_t<CIN, NB, true, false, CAT, false, MBW, ROWS><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
+	else if (f32) k_conv3x3_t<CIN, NB, false, true, CAT, false, MBW, ROWS><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
+	else k_conv3x3_t<CIN, NB, false, false, CAT, false, MBW, ROWS><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
+}
+
 template <int CIN, int NB, bool CAT>
 static hipError_t conv_t(const bf16_t* x, const bf16_t* coarse, const bf16_t* w, const float* bias, void* y, int H, int W, int flags, hipStream_t s)
 {
-	const dim3 grid((W + 63) / 64, (H + 3) / 4), block(256);
-	const bool relu = flags & DN_RELU, f32 = flags & DN_OUT_F32;
-	if (relu && f32) k_conv3x3_t<CIN, NB, true, true, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
-	else if (relu) k_conv3x3_t<CIN, NB, true, false, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
-	else if (f32) k_conv3x3_t<CIN, NB, false, true, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
-	else k_conv3x3_t<CIN, NB, false, false, CAT><<<grid, block, 0, s>>>(x, coarse, w, bias, y, H, W, nullptr);
+	/* 64 input channels without the concatenation keep 4-row tiles (an 8-row tile would not leave room for two blocks per CU) */
+	constexpr int ROWS = (CIN == 64 && !CAT) ? 4 : TILE_ROWS;
+	if (small_image(H, W)) conv_t_w<CIN, NB, CAT, 1, 4>(x, coarse, w, bias, y, H, W, flags, s);
+	else conv_t_w<CIN, NB, CAT, 4, ROWS>(x, coarse, w, bias, y, H, W, flags, s);
 	return hipGetLastError();
 }
 
@@ -295,9 +348,19 @@ hipError_t dn_conv3x3_pool(const bf16_t* x, int Cin, const bf16_t* w, const floa
 	bf16_t* ypool, hipStream_t s)
 {
 	if (Cout != 32 || (Cin != 16 && Cin != 32) || H <= 0 || W <= 0 || H % 2 || W % 2) return hipErrorInvalidValue;
-	const dim3 grid((W + 63) / 64, (H + 3) / 4), block(256);
-	if (Cin == 16) k_conv3x3_t<16, 2, true, false, false, true><<<grid, block, 0, s>>>(x, nullptr, w, bias, y, H, W, ypool);
-	else k_conv3x3_t<32, 2, true, false, false, true><<<grid, block, 0, s>>>(x, nullptr, w, bias, y, H, W, ypool);
+	const dim3 block(256);
+	if (small_image(H, W))
+	{
+		const dim3 grid((W + 15) / 16, (H + 3) / 4);
+		if (Cin == 16) k_conv3x3_t<16, 2, true, false, false, true, 1><<<grid, block, 0, s>>>(x, nullptr, w, bias, y, H, W, ypool);
+		else k_conv3x3_t<32, 2, true, false, false, true, 1><<<grid, block, 0, s>>>(x, nullptr, w, bias, y, H, W, ypool);
+	}
+	else
+	{
+		const dim3 grid((W + 63) / 64, (H + TILE_ROWS - 1) / TILE_ROWS), blockL(64 * TILE_ROWS);
+		if (Cin == 16) k_conv3x3_t<16, 2, true, false, false, true, 4, TILE_ROWS><<<grid, blockL, 0, s>>>(x, nullptr, w, bias, y, H, W, ypool);
+		else k_conv3x3_t<32, 2, true, false, false, true, 4, TILE_ROWS><<<grid, blockL, 0, s>>>(x, nullptr, w, bias, y, H, W, ypool);
+	}
 	return hipGetLastError();
 }
 

@@ -195,14 +195,15 @@ def test_tall_image_walks_several_bands_per_block(torch_cuda, cin, cout, cat):
     assert err.max() <= 1e-5 * np.abs(ref).max(), np.unravel_index(err.argmax(), err.shape)
 
 
-@pytest.mark.parametrize("cin", [16, 32])
-def test_conv_with_fused_pool_equals_conv_then_pool(torch_cuda, cin):
+@pytest.mark.parametrize("cin,H", [(16, 22), (32, 22), (16, 2050), (32, 2050)])
+def test_conv_with_fused_pool_equals_conv_then_pool(torch_cuda, cin, H):
     """The encoder convolution that also writes the 2x2 average of its output
-    gives exactly dn_conv3x3 (relu, bf16) followed by dn_pool2."""
+    gives exactly dn_conv3x3 (relu, bf16) followed by dn_pool2, with 16-pixel
+    tiles (22 rows: a small image) and 64-pixel tiles (2050 rows)."""
     torch = torch_cuda
     L = lh.library()
     rng = np.random.default_rng(40 + cin)
-    H, W = 22, 70                                   # the last 4-row tile has only 2 rows; ragged strip
+    W = 70                                          # the last 4-row tile has only 2 rows; ragged strip
     x = R.bf16(rng.standard_normal((H, W, cin)))
     w = R.bf16(rng.standard_normal((9, 32, cin)) * 0.3)
     b = (rng.standard_normal(32) * 0.1).astype(np.float32)
@@ -218,6 +219,8 @@ def test_conv_with_fused_pool_equals_conv_then_pool(torch_cuda, cin):
     torch.cuda.synchronize()
     assert np.array_equal(y.cpu().numpy(), y2.cpu().numpy())
     assert np.array_equal(yp.cpu().numpy(), yp2.cpu().numpy())
+    if H > 100:
+        return                                      # the unfused pair is checked against FP64 at this height elsewhere
     ref = R.pool2(R.bf16(np.maximum(R.conv(x, w, b), 0)))
     got = D.bf16_to_f32(yp.cpu().numpy()).reshape(H // 2, W // 2, 32)
     assert np.all(np.abs(got - R.bf16(ref)) <= np.abs(R.bf16(ref)) * 2 ** -7 + 1e-30)
