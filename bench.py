@@ -143,24 +143,48 @@ def bench_denoise(a, np, torch, lh, scenes):
     train_ms = _events(torch, st, train, max(2, a.steps // 2))
     flops = net.flops_forward()
     tf = flops / (fwd_ms * 1e-3) / 1e12
+    # a whole denoised 1080p frame from the filter core's buffers (input packing + forward + display image),
+    # against SVGF (prepare + 3 a-trous passes + display, no TAA) on the same synthetic frame
+    g0, v1 = svgf_gbuffer(np, torch, scenes)
+    img = torch.zeros((1080, 1920, 4), dtype=torch.float32, device="cuda")
+
+    def unfused_frame():
+        net.set_input_frame(g0[0], g0[1], 1, 1920, 1080)
+        net.forward()
+        net.output_frame(g0[1], 1920, 1080, img)
+
+    def learned_frame():            # the output convolution writes the display image
+        net.denoise_frame(g0[0], g0[1], 1, 1920, 1080, img)
+    for _ in range(2):
+        unfused_frame()
+        learned_frame()
+    unfused_ms = _events(torch, st, unfused_frame, a.steps)
+    frame_ms = _events(torch, st, learned_frame, a.steps)
+    sv_ms = svgf_ms(a, np, torch, lh, scenes, g0, v1)
     print(json.dumps({"metric": "ms/frame learned denoiser forward 1080p", "value": round(fwd_ms, 3), "unit": "ms",
                       "higher_is_better": False, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "dtype": "bf16 (fp32 accumulate)",
                       "data": "synthetic", "vs_baseline": None,
                       "config": {"workload": "5-level 3x3 conv U-Net, 32 channels, 1088x1920", "layers": len(D.layer_specs())},
+                      "learned_frame_ms": round(frame_ms, 3), "learned_frame_unfused_ms": round(unfused_ms, 3), "svgf_frame_ms": round(sv_ms, 3),
                       "train_step_ms": round(train_ms, 3), "forward_gflop": round(flops / 1e9, 1),
                       "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tf / 2500.0, 4),
                                    "traffic": None}}))
 
 
-def bench_svgf(a, np, torch, lh, scenes):
-    """SVGF filter (prepare + 3 a-trous passes + display) on a 1920x1080
-    synthetic G-buffer, moving camera: ms per frame."""
-    W, H = 1920, 1080
+def svgf_gbuffer(np, torch, scenes, W=1920, H=1080):
+    """A synthetic 1080p filter-core frame on the device: accumulators
+    (direct, indirect), packed features, world position, depth derivatives,
+    and the view of the next (moved) camera."""
     v0 = scenes.camera_view((0.5, 0.6, -1.0), (0.5, 0.5, 1.0), 50.0, W, H)
     v1 = scenes.camera_view((0.505, 0.6, -0.999), (0.5, 0.5, 1.0), 50.0, W, H)
     dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
-    g0 = [dev(x) for x in scenes.gbuffer(v0, W, H, seed=1)]
-    g0[1] = dev(scenes.gbuffer(v0, W, H, seed=1)[1].view(np.int32))
+    gb = scenes.gbuffer(v0, W, H, seed=1)
+    g0 = [dev(x) for x in gb]
+    g0[1] = dev(gb[1].view(np.int32))
+    return g0, v1
+
+
+def svgf_ms(a, np, torch, lh, scenes, g0, v1, W=1920, H=1080):
     out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
     f = lh.Svgf(W, H)
     st = torch.cuda.current_stream()
@@ -170,7 +194,15 @@ def bench_svgf(a, np, torch, lh, scenes):
                 samples=1, stationary=0, stream=st.cuda_stream)
     for _ in range(a.warmup):
         frame()
-    ms = _events(torch, st, frame, a.steps)
+    return _events(torch, st, frame, a.steps)
+
+
+def bench_svgf(a, np, torch, lh, scenes):
+    """SVGF filter (prepare + 3 a-trous passes + display) on a 1920x1080
+    synthetic G-buffer, moving camera: ms per frame."""
+    W, H = 1920, 1080
+    g0, v1 = svgf_gbuffer(np, torch, scenes, W, H)
+    ms = svgf_ms(a, np, torch, lh, scenes, g0, v1, W, H)
     # compulsory bytes per pixel over all passes: prepare reads 2x16 acc + 16 feat + 16 pos + 16 prevpos + 16 moments (history),
     # writes 16 shading + 8 motion + 16 moments + 16 feat; each a-trous pass reads ~16 A + 16 feat + 16 moments + 16 depth;
     # phase 1 adds 16 motion-history; plus 16 out: about 240 B/px

@@ -244,6 +244,13 @@ LH2HIP_API int lh2hip_dn_frame_input(const lh2_float4* acc32, const uint32_t* fe
  * (exp(o) - 1) max(albedo, 0.02) per channel, raised to 1/4 (the filter
  * core's display convention: the sqrt of SVGF's sqrt last pass). */
 LH2HIP_API int lh2hip_dn_frame_output(const float* out, int Hp, int Wp, const uint32_t* features, int w, int h, lh2_float4* image, void* stream);
+/* The output convolution fused with the frame conversion, for inference:
+ * conv3x3(x) + bias (32 -> 16 channels) written as the display image of a
+ * w x h frame inside the Hp x Wp network image, as lh2hip_dn_conv3x3 (fp32
+ * out) then lh2hip_dn_frame_output compute it (bit for bit). No
+ * counterpart in the reference. */
+LH2HIP_API int lh2hip_dn_conv3x3_display(const void* x, const void* wts, const float* bias, const uint32_t* features, int w, int h,
+	lh2_float4* image, int Hp, int Wp, void* stream);
 LH2HIP_API int lh2hip_dn_relmse(const float* out, int Cout, const lh2_float4* target, int H, int W, void* dout, float* loss, void* stream);
 /* Adam step (g NULL: no step, only the bf16 copies); wfwd / wbwd (NULL to
  * skip): bf16 weights in the forward layout and flipped for the input gradient. */

@@ -96,6 +96,7 @@ _SIGS = {
     "lh2hip_server_pause": (I, [P, I]),
     "lh2hip_dn_weights_bytes": (ctypes.c_uint64, []),
     "lh2hip_dn_frame_input": (I, [P, P, ctypes.c_float, I, I, I, I, P, P]),
+    "lh2hip_dn_conv3x3_display": (I, [P, P, P, P, I, I, P, I, I, P]),
     "lh2hip_dn_frame_output": (I, [P, I, I, P, I, I, P, P]),
 }
 EXPORTED = sorted(_SIGS) + ["CreateCore"]

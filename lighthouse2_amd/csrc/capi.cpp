@@ -377,6 +377,14 @@ LH2HIP_API int lh2hip_dn_conv3x3_dw(const void* x, int Cin, const void* dy, int 
 		"dn_conv3x3_dw (Cin x Cout in 16x32, 32x32, 64x32, 32x16, 16x16)");
 }
 
+LH2HIP_API int lh2hip_dn_conv3x3_display(const void* x, const void* wts, const float* bias, const uint32_t* features, int w, int h,
+	lh2_float4* image, int Hp, int Wp, void* stream)
+{
+	if (!x || !wts || !features || !image) return fail(LH2HIP_ERR_INVALID, "dn_conv3x3_display: null pointer");
+	return dn_status(lh::dn_conv3x3_display((const lh::bf16_t*)x, (const lh::bf16_t*)wts, bias, (const uint4*)features, w, h, (float4*)image,
+		Hp, Wp, (hipStream_t)stream), "dn_conv3x3_display (frame inside the Hp x Wp image)");
+}
+
 LH2HIP_API uint64_t lh2hip_dn_dw_scratch_bytes(void) { return lh::dn_dw_scratch_bytes(); }
 
 LH2HIP_API int lh2hip_dn_conv3x3_dw_noalloc(const void* x, int Cin, const void* dy, int Cout, int H, int W, float* dw, float* db, void* stream)

@@ -26,6 +26,11 @@ hipError_t dn_conv3x3_pool(const bf16_t* x, int Cin, const bf16_t* w, const floa
  * the two tensors (H, W even); equal to dn_upcat followed by dn_conv3x3. */
 hipError_t dn_conv3x3_cat(const bf16_t* coarse, const bf16_t* skip, const bf16_t* w, const float* bias, void* y, int Cout, int H, int W,
 	int flags, hipStream_t s);
+/* image = the display image of conv3x3(x) + bias (32 -> 16 channels) for a
+ * w x h frame inside the Hp x Wp network image: what dn_conv3x3 (fp32 out)
+ * followed by dn_frame_output computes, without the fp32 output tensor. */
+hipError_t dn_conv3x3_display(const bf16_t* x, const bf16_t* wts, const float* bias, const uint4* features, int w, int h, float4* image,
+	int Hp, int Wp, hipStream_t s);
 /* Weight gradient: dw[9][Cout][Cin] = sum_p x[p + tap] dy[p], db[Cout] = sum_p dy[p]
  * (both overwritten). ws: workspace of dn_dw_workspace_bytes(Cin, Cout). */
 size_t dn_dw_workspace_bytes(int Cin, int Cout);

@@ -116,11 +116,49 @@ __global__ void __launch_bounds__(256) k_conv3x3(const bf16_t* __restrict__ x, i
  * 64-channel concatenation [nearest 2x upsample of coarse (32 channels),
  * x (32 channels)]; the coarse pixels under the tile (and their halo) are staged
  * separately and upsampled by the fragment addressing. */
-typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
-template <int CIN, int NB, bool RELU, bool F32OUT, bool CAT, bool POOL = false, int MBW = 4, int ROWS = 4>
-__global__ void __launch_bounds__(64 * ROWS) k_conv3x3_t(const bf16_t* __restrict__ x, const bf16_t* __restrict__ coarse,
-	const bf16_t* __restrict__ w, const float* __restrict__ bias, void* __restrict__ y, int H, int W, bf16_t* __restrict__ ypool)
+/* Decoders of the SVGF feature words (tools_shared.h:130-136,169-176):
+ * albedo HDRtoRGB32 (10:11:11), normal PackNormal2, depth as float bits. */
+__device__ __forceinline__ float3 feat_albedo(uint32_t c)
 {
+	return make_float3((float)(c >> 22) * (1.0f / 1023.0f), (float)((c >> 11) & 2047u) * (1.0f / 2047.0f), (float)(c & 2047u) * (1.0f / 2047.0f));
+}
+__device__ __forceinline__ float3 feat_normal(uint32_t p)
+{
+	return make_float3((float)((p >> 2) & 1023u) * (1.0f / 511.0f) - 1.0f, (float)((p >> 12) & 1023u) * (1.0f / 511.0f) - 1.0f,
+		(float)(p >> 22) * (1.0f / 511.0f) - 1.0f);
+}
+
+static const float DN_ALBEDO_FLOOR = 0.02f;
+
+/* The frame the output convolution's display variant (dn_conv3x3_display)
+ * writes: features of the frame and its display image, w x h inside the
+ * network's padded image. */
+struct FrameIO
+{
+	const uint4* feat;
+	int w, h;
+	float4* image;
+};
+
+/* The display value of one pixel from the network's first 3 output channels,
+ * as k_frame_output computes it. */
+__device__ __forceinline__ float4 frame_display(float o0, float o1, float o2, uint32_t featX)
+{
+	const float3 a = feat_albedo(featX);
+	const float o[3] = { o0, o1, o2 }, al[3] = { a.x, a.y, a.z };
+	float v[3];
+	for (int k = 0; k < 3; k++) v[k] = sqrtf(sqrtf(expm1f(fmaxf(o[k], 0.0f)) * fmaxf(al[k], DN_ALBEDO_FLOOR)));
+	return make_float4(v[0], v[1], v[2], 0.0f);
+}
+
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+enum { IO_NONE = 0, IO_DISPLAY = 1 };
+template <int CIN, int NB, bool RELU, bool F32OUT, bool CAT, bool POOL = false, int MBW = 4, int ROWS = 4, int IO = IO_NONE>
+__global__ void __launch_bounds__(64 * ROWS) k_conv3x3_t(const bf16_t* __restrict__ x, const bf16_t* __restrict__ coarse,
+	const bf16_t* __restrict__ w, const float* __restrict__ bias, void* __restrict__ y, int H, int W, bf16_t* __restrict__ ypool,
+	FrameIO io = FrameIO{})
+{
+	static_assert(IO != IO_DISPLAY || (NB == 1 && F32OUT && !RELU), "the display epilogue reads the first output channels");
 	constexpr int CX = CAT ? 32 : CIN;                  /* channels of x */
 	constexpr int CKX = CX < 32 ? 32 : CX;              /* channels per pixel of the x tile (K, padded) */
 	constexpr int CP = CKX + 8, TW = 16 * MBW + 2, TR = ROWS + 2, CH8 = CX / 8, CHP = CKX / 8;
@@ -246,6 +284,14 @@ __global__ void __launch_bounds__(64 * ROWS) k_conv3x3_t(const bf16_t* __restric
 			const size_t o = ((size_t)ry * W + px) * Cout + ch;
 			if (PAIRED)
 				pk[mb][nb][0] = (uint32_t)f2bf(v[0]) | (uint32_t)f2bf(v[1]) << 16, pk[mb][nb][1] = (uint32_t)f2bf(v[2]) | (uint32_t)f2bf(v[3]) << 16;
+			else if (IO == IO_DISPLAY)
+			{	/* lanes holding channels 0..3 write the display pixel of the frame */
+				if (kg == 0 && ry < io.h && px < io.w)
+				{
+					const size_t q = (size_t)px + (size_t)ry * io.w;
+					io.image[q] = frame_display(v[0], v[1], v[2], io.feat[q].x);
+				}
+			}
 			else if (ry < H && px < W)
 			{
 				if (F32OUT) *(float4*)((float*)y + o) = make_float4(v[0], v[1], v[2], v[3]);
@@ -402,6 +448,31 @@ hipError_t dn_conv3x3(const bf16_t* x, int Cin, const bf16_t* w, const float* bi
 	case 3: return conv_nb<3>(x, Cin, w, bias, y, H, W, flags, s);
 	default: return conv_nb<4>(x, Cin, w, bias, y, H, W, flags, s);
 	}
+}
+
+/* The output convolution (32 -> 16) writing the display image of a
+ * filter-core frame from its epilogue: equal to dn_conv3x3 (fp32 out) +
+ * dn_frame_output bit for bit, without the fp32 output tensor. (Reading the
+ * frame into the first convolution's tile the same way was measured slower
+ * than dn_frame_input + dn_conv3x3: 181 against 99 µs at 1080p.) */
+template <int CIN, int NB, bool RELU, bool F32OUT, int IO>
+static void conv_io(const bf16_t* x, const bf16_t* w, const float* bias, void* y, int H, int W, const FrameIO& io, hipStream_t s)
+{
+	if (small_image(H, W))
+		k_conv3x3_t<CIN, NB, RELU, F32OUT, false, false, 1, 4, IO><<<dim3((W + 15) / 16, (H + 3) / 4), 256, 0, s>>>(x, nullptr, w, bias, y, H, W,
+			nullptr, io);
+	else
+		k_conv3x3_t<CIN, NB, RELU, F32OUT, false, false, 4, TILE_ROWS, IO><<<dim3((W + 63) / 64, (H + TILE_ROWS - 1) / TILE_ROWS), 64 * TILE_ROWS, 0,
+			s>>>(x, nullptr, w, bias, y, H, W, nullptr, io);
+}
+
+hipError_t dn_conv3x3_display(const bf16_t* x, const bf16_t* wts, const float* bias, const uint4* features, int w, int h, float4* image,
+	int Hp, int Wp, hipStream_t s)
+{
+	if (w <= 0 || h <= 0 || Hp < h || Wp < w) return hipErrorInvalidValue;
+	const FrameIO io{ features, w, h, image };
+	conv_io<32, 1, false, true, IO_DISPLAY>(x, wts, bias, nullptr, Hp, Wp, io, s);
+	return hipGetLastError();
 }
 
 /* ---- weight gradient ---------------------------------------------------
@@ -774,19 +845,6 @@ hipError_t dn_pack_input(const float4* radiance, const float4* albedo, const flo
 	return hipGetLastError();
 }
 
-/* Decoders of the SVGF feature words (tools_shared.h:130-136,169-176):
- * albedo HDRtoRGB32 (10:11:11), normal PackNormal2, depth as float bits. */
-__device__ __forceinline__ float3 feat_albedo(uint32_t c)
-{
-	return make_float3((float)(c >> 22) * (1.0f / 1023.0f), (float)((c >> 11) & 2047u) * (1.0f / 2047.0f), (float)(c & 2047u) * (1.0f / 2047.0f));
-}
-__device__ __forceinline__ float3 feat_normal(uint32_t p)
-{
-	return make_float3((float)((p >> 2) & 1023u) * (1.0f / 511.0f) - 1.0f, (float)((p >> 12) & 1023u) * (1.0f / 511.0f) - 1.0f,
-		(float)(p >> 22) * (1.0f / 511.0f) - 1.0f);
-}
-
-static const float DN_ALBEDO_FLOOR = 0.02f;
 
 __global__ void k_frame_input(const float4* __restrict__ acc, const uint4* __restrict__ feat, float invSpp, int w, int h, int Hp, int Wp,
 	bf16_t* __restrict__ out)

@@ -191,13 +191,10 @@ void RenderServer::finishJob(Job& j)
 			0.0f, 0.0f, 0.0f, 0.0f, j.spp, core.clampDirectSetting(), core.clampIndirectSetting(), 0, j.image.as<float4>(), stream);
 	}
 	else if (j.denoise == DENOISE_LEARNED)
-	{	/* the U-Net on the job's frame, padded to multiples of 16 */
+	{	/* the U-Net on the job's frame, padded to multiples of 16; the frame is read by the
+		 * first convolution and the display image written by the last one */
 		std::lock_guard<std::mutex> g(unetLock);
-		const int Hp = (j.h + 15) & ~15, Wp = (j.w + 15) & ~15;
-		bf16_t* in = unet.Input(Hp, Wp);
-		check(dn_frame_input(j.acc32.as<float4>(), j.gFeatures.as<uint4>(), 1.0f / (float)j.spp, j.w, j.h, Hp, Wp, in, stream), "denoiser input");
-		const float* out = unet.Forward(stream);
-		check(dn_frame_output(out, Hp, Wp, j.gFeatures.as<uint4>(), j.w, j.h, j.image.as<float4>(), stream), "denoiser output");
+		unet.DenoiseFrame(j.acc32.as<float4>(), j.gFeatures.as<uint4>(), 1.0f / (float)j.spp, j.w, j.h, j.image.as<float4>(), stream);
 	}
 	else check(launch_finalize(j.acc32.as<float4>(), pixels, 1.0f / (float)j.spp, j.image.as<float4>(), stream), "finalize");
 	const size_t accWords = (size_t)pixels * 4;   /* denoised jobs: the direct half, as RenderCore::ReadAccumulatorF64 */

@@ -71,12 +71,29 @@ const float* UNetForward::Forward(hipStream_t s)
 {
 	if (!ready) throw CoreError(LH2HIP_ERR_STATE, "denoiser: no weights");
 	if (!H) throw CoreError(LH2HIP_ERR_STATE, "denoiser: no input");
-	const auto Wt = [&](int i) { return (const bf16_t*)((const char*)weights.p + wOff[i]); };
-	const auto Bi = [&](int i) { return (const float*)((const char*)weights.p + bOff[i]); };
+	dnCheck(dn_conv3x3(in[0].as<bf16_t>(), CIN, Wt(0), Bi(0), a[0].p, CH, H, W, DN_RELU, s), "conv");
+	Middle(s);
+	dnCheck(dn_conv3x3(d[0].as<bf16_t>(), CH, Wt(LAYERS - 1), Bi(LAYERS - 1), out.p, COUT, H, W, DN_OUT_F32, s), "conv");
+	return out.as<float>();
+}
+
+void UNetForward::DenoiseFrame(const float4* acc32, const uint4* features, float invSpp, int w, int h, float4* image, hipStream_t s)
+{
+	if (!ready) throw CoreError(LH2HIP_ERR_STATE, "denoiser: no weights");
+	const int Hp = (h + 15) & ~15, Wp = (w + 15) & ~15;
+	bf16_t* x = Input(Hp, Wp);
+	dnCheck(dn_frame_input(acc32, features, invSpp, w, h, Hp, Wp, x, s), "denoiser input");
+	dnCheck(dn_conv3x3(x, CIN, Wt(0), Bi(0), a[0].p, CH, H, W, DN_RELU, s), "conv");
+	Middle(s);
+	dnCheck(dn_conv3x3_display(d[0].as<bf16_t>(), Wt(LAYERS - 1), Bi(LAYERS - 1), features, w, h, image, Hp, Wp, s), "denoiser output conv");
+}
+
+void UNetForward::Middle(hipStream_t s)
+{
 	for (int l = 0; l < LEVELS; l++)
 	{
 		const int h = H >> l, w = W >> l;
-		dnCheck(dn_conv3x3(in[l].as<bf16_t>(), l == 0 ? CIN : CH, Wt(2 * l), Bi(2 * l), a[l].p, CH, h, w, DN_RELU, s), "conv");
+		if (l > 0) dnCheck(dn_conv3x3(in[l].as<bf16_t>(), CH, Wt(2 * l), Bi(2 * l), a[l].p, CH, h, w, DN_RELU, s), "conv");
 		if (l < LEVELS - 1)
 			dnCheck(dn_conv3x3_pool(a[l].as<bf16_t>(), CH, Wt(2 * l + 1), Bi(2 * l + 1), b[l].as<bf16_t>(), CH, h, w, in[l + 1].as<bf16_t>(), s), "conv");
 		else dnCheck(dn_conv3x3(a[l].as<bf16_t>(), CH, Wt(2 * l + 1), Bi(2 * l + 1), b[l].p, CH, h, w, DN_RELU, s), "conv");
@@ -89,8 +106,6 @@ const float* UNetForward::Forward(hipStream_t s)
 		dnCheck(dn_conv3x3(c[l].as<bf16_t>(), CH, Wt(i + 1), Bi(i + 1), d[l].p, CH, h, w, DN_RELU, s), "conv");
 		prev = d[l].as<bf16_t>();
 	}
-	dnCheck(dn_conv3x3(d[0].as<bf16_t>(), CH, Wt(LAYERS - 1), Bi(LAYERS - 1), out.p, COUT, H, W, DN_OUT_F32, s), "conv");
-	return out.as<float>();
 }
 
 } // namespace lh

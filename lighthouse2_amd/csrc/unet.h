@@ -23,8 +23,15 @@ public:
 	bf16_t* Input(int Hp, int Wp);
 	/* Runs the network on the input; returns the fp32 output [Hp][Wp][16]. */
 	const float* Forward(hipStream_t s);
+	/* The display image [h w] of a filter-core frame (acc32 float4 [2 h w],
+	 * features uint4 [h w]), the output convolution writing it directly; equal
+	 * to Input + dn_frame_input + Forward + dn_frame_output. */
+	void DenoiseFrame(const float4* acc32, const uint4* features, float invSpp, int w, int h, float4* image, hipStream_t s);
 
 private:
+	void Middle(hipStream_t s);   /* every layer after the first and before the output convolution */
+	const bf16_t* Wt(int i) const { return (const bf16_t*)((const char*)weights.p + wOff[i]); }
+	const float* Bi(int i) const { return (const float*)((const char*)weights.p + bOff[i]); }
 	bool ready = false;
 	int H = 0, W = 0;
 	DevBuf weights;

@@ -134,11 +134,19 @@ class UNetDenoiser:
 
     def forward(self):
         """Runs the network on the current input; returns the fp32 output (H, W, 16)."""
+        self._conv(self.a["in_0"], "e1_0", self.a["a_0"], self.H, self.W, 1)
+        self._forward_from_a0()
+        self._conv(self.a["d_0"], "out", self.out, self.H, self.W, 2)
+        return self.out.view(self.H, self.W, COUT)
+
+    def _forward_from_a0(self):
+        """Every layer after e1_0 and before the output convolution."""
         L = library()
         H, W = self.H, self.W
         for lvl in range(LEVELS):
             h, w = H >> lvl, W >> lvl
-            self._conv(self.a[f"in_{lvl}"], f"e1_{lvl}", self.a[f"a_{lvl}"], h, w, 1)
+            if lvl > 0:
+                self._conv(self.a[f"in_{lvl}"], f"e1_{lvl}", self.a[f"a_{lvl}"], h, w, 1)
             if lvl < LEVELS - 1:                   # b_l and its 2x2 average in_{l+1} from one kernel
                 d = self.p[f"e2_{lvl}"]
                 _check(L.lh2hip_dn_conv3x3_pool(self.a[f"a_{lvl}"].data_ptr(), CH, d["wf"].data_ptr(), d["b"].data_ptr(),
@@ -153,8 +161,6 @@ class UNetDenoiser:
                                            self.a[f"c_{lvl}"].data_ptr(), CH, h, w, 1, self.sh))
             self._conv(self.a[f"c_{lvl}"], f"d2_{lvl}", self.a[f"d_{lvl}"], h, w, 1)
             prev = self.a[f"d_{lvl}"]
-        self._conv(self.a["d_0"], "out", self.out, H, W, 2)
-        return self.out.view(H, W, COUT)
 
     def loss_backward(self, target):
         """relMSE against target (device float32 (H, W, 4)); fills every
@@ -221,6 +227,18 @@ class UNetDenoiser:
         H x W): the same kernel the job server runs."""
         _check(library().lh2hip_dn_frame_input(acc32.data_ptr(), features.data_ptr(), 1.0 / spp, w, h, self.H, self.W,
                                                self.a["in_0"].data_ptr(), self.sh))
+
+    def denoise_frame(self, acc32, features, spp, w, h, image):
+        """Inference on a filter-estimator frame, the output convolution writing
+        the display image itself (no fp32 output tensor): the image equals
+        set_input_frame + forward + output_frame's."""
+        L, H, W = library(), self.H, self.W
+        self.set_input_frame(acc32, features, spp, w, h)
+        self._conv(self.a["in_0"], "e1_0", self.a["a_0"], H, W, 1)
+        self._forward_from_a0()
+        d = self.p["out"]
+        _check(L.lh2hip_dn_conv3x3_display(self.a["d_0"].data_ptr(), d["wf"].data_ptr(), d["b"].data_ptr(), features.data_ptr(), w, h,
+                                           image.data_ptr(), H, W, self.sh))
 
     def output_frame(self, features, w, h, image):
         """Display image (w x h float4, device) from the last forward pass, as the job server writes it."""

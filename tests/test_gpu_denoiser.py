@@ -225,3 +225,31 @@ def test_conv_with_fused_pool_equals_conv_then_pool(torch_cuda, cin, H):
     got = D.bf16_to_f32(yp.cpu().numpy()).reshape(H // 2, W // 2, 32)
     assert np.all(np.abs(got - R.bf16(ref)) <= np.abs(R.bf16(ref)) * 2 ** -7 + 1e-30)
     assert L.lh2hip_dn_conv3x3_pool(dx_.data_ptr(), cin, dw_.data_ptr(), None, y.data_ptr(), 32, H - 1, W, yp.data_ptr(), sh) != 0
+
+
+@pytest.mark.parametrize("w,h", [(70, 40), (1920, 1080)])
+def test_fused_display_equals_unfused(torch_cuda, w, h):
+    """Inference with the output convolution writing the display image
+    (lh2hip_dn_conv3x3_display) gives the image of set_input_frame + forward
+    + output_frame bit for bit: a frame inside a padded image with 16-pixel
+    tiles (70 x 40 in 48 x 80) and with 64-pixel, 8-row tiles (1080p in
+    1088 x 1920)."""
+    torch = torch_cuda
+    rng = np.random.default_rng(w + h)
+    Hp, Wp = (h + 15) // 16 * 16, (w + 15) // 16 * 16
+    net = D.UNetDenoiser(Hp, Wp, seed=5)
+    acc = torch.from_numpy(rng.uniform(0, 2, (2, h, w, 4)).astype(np.float32)).cuda()
+    feat = rng.integers(0, 2 ** 32, (h, w, 4), dtype=np.uint64).astype(np.uint32)
+    feat[..., 2] = rng.uniform(0.1, 50, (h, w)).astype(np.float32).view(np.uint32)   # hit distance
+    feat[0, 0, 2] = np.float32(np.inf).view(np.uint32)                                 # a miss
+    f = torch.from_numpy(feat.view(np.int32)).cuda()
+    ref = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
+    net.set_input_frame(acc, f, 4, w, h)
+    net.forward()
+    net.output_frame(f, w, h, ref)
+    got = torch.full((h, w, 4), -1.0, dtype=torch.float32, device="cuda")
+    net.denoise_frame(acc, f, 4, w, h, got)
+    torch.cuda.synchronize()
+    r, g = ref.cpu().numpy(), got.cpu().numpy()
+    assert np.isfinite(r).all() and (r[..., :3] > 0).any()
+    assert np.array_equal(r.view(np.uint32), g.view(np.uint32))
