@@ -18,7 +18,9 @@ calls a core:
 The file parsers are this project's (the reference uses tinyobjloader,
 tinygltf and FreeImage): polygons are split as a fan, which is what tinyobjloader's ear
 clipping gives for convex polygons; PNG is decoded with zlib (8-bit grey,
-grey + alpha, RGB, RGBA and palette images, not interlaced). All float
+grey + alpha, RGB, RGBA and palette images, not interlaced); baseline and
+extended-sequential JPEG by this project's decoder (jpeg.py), which follows
+libjpeg's default arithmetic, the library behind FreeImage's JPEG plugin. All float
 arithmetic of the producers is float32 in the reference's operation order.
 """
 import os
@@ -29,6 +31,7 @@ import numpy as np
 
 from . import formats as F
 from . import scenes
+from .jpeg import decode_jpeg
 
 f32 = np.float32
 
@@ -225,12 +228,14 @@ def mark_normal_map(tex):
 
 
 def decode_image(data, name=""):
-    """PNG or TGA bytes (by signature, then by file name) -> RGBA, top row first."""
+    """PNG, JPEG or TGA bytes (by signature, then by file name) -> RGBA, top row first."""
     if data[:8] == b"\x89PNG\r\n\x1a\n":
         return decode_png(data)
+    if data[:3] == b"\xff\xd8\xff":
+        return decode_jpeg(data)
     if name.lower().endswith(".tga"):
         return decode_tga(data)
-    raise ValueError(f"{name or 'image'}: only PNG and TGA images are read")
+    raise ValueError(f"{name or 'image'}: only PNG, JPEG and TGA images are read")
 
 
 def resolve_path(path):
@@ -256,6 +261,76 @@ def load_texture(path, flipped=False, linearized=False, normal_map=False):
     path = resolve_path(path)
     with open(path, "rb") as fh:
         return texture_from_rgba(decode_image(fh.read(), path), flipped, linearized, normal_map)
+
+
+# ---- skydome (Radiance HDR) ---------------------------------------------------
+
+
+def decode_hdr(data):
+    """Radiance RGBE image (the .hdr skydomes HostSkyDome::Load reads through
+    FreeImage, host_skydome.cpp:103-133) -> (h, w, 3) float32, top row first.
+    Scanlines are flat or new-style run-length (per channel: a count above
+    128 repeats the next byte count - 128 times, otherwise count bytes
+    follow); a pixel (r, g, b, e) is (r, g, b) * 2^(e - 136), zero for e = 0."""
+    data = bytes(data)
+    if not (data.startswith(b"#?RADIANCE") or data.startswith(b"#?RGBE")):
+        raise ValueError("not a Radiance HDR file")
+    pos = data.index(b"\n\n") + 2
+    if b"FORMAT=" in data[:pos] and b"32-bit_rle_rgbe" not in data[:pos]:
+        raise ValueError("Radiance HDR: only the RGBE pixel format is read")
+    eol = data.index(b"\n", pos)
+    res = data[pos:eol].split()
+    if len(res) != 4 or res[0] != b"-Y" or res[2] != b"+X":
+        raise ValueError("Radiance HDR: only the standard -Y H +X W orientation is read")
+    h, w = int(res[1]), int(res[3])
+    pos = eol + 1
+    rgbe = np.zeros((h, w, 4), np.uint8)
+    for y in range(h):
+        if 8 <= w < 32768 and data[pos] == 2 and data[pos + 1] == 2 and ((data[pos + 2] << 8) | data[pos + 3]) == w and not data[pos + 2] & 0x80:
+            pos += 4
+            for c in range(4):
+                x = 0
+                while x < w:
+                    n = data[pos]
+                    if n > 128:
+                        n -= 128
+                        rgbe[y, x:x + n, c] = data[pos + 1]
+                        pos += 2
+                    else:
+                        if n == 0:
+                            raise ValueError("Radiance HDR: empty run")
+                        rgbe[y, x:x + n, c] = np.frombuffer(data, np.uint8, n, pos + 1)
+                        pos += 1 + n
+                    x += n
+                if x != w:
+                    raise ValueError("Radiance HDR: run-length scanline overruns the width")
+        else:
+            rgbe[y] = np.frombuffer(data, np.uint8, 4 * w, pos).reshape(w, 4)
+            pos += 4 * w
+    e = rgbe[..., 3].astype(np.int32)
+    f = np.where(e > 0, np.ldexp(np.float32(1.0), e - 136), 0).astype(f32)
+    return (rgbe[..., :3].astype(f32) * f[..., None]).astype(f32)
+
+
+def load_skydome(scene, path, scale=(1.0, 1.0, 1.0), transform=None):
+    """HostSkyDome::Load (host_skydome.cpp:80-139): a .hdr file or the
+    reference's cache of it (.bin: uint32 width, height, then float3 pixels,
+    top row first), multiplied by scale; becomes the scene's sky
+    (SetSkyData's pixels, width, height and world-to-sky transform,
+    identity by default as HostSkyDome's worldToLight)."""
+    path = resolve_path(path)
+    with open(path, "rb") as fh:
+        data = fh.read()
+    if path.lower().endswith(".bin"):
+        w, h = struct.unpack("<II", data[:8])
+        px = np.frombuffer(data, f32, w * h * 3, 8).reshape(h, w, 3).copy()
+    else:
+        px = decode_hdr(data)
+    sc = np.asarray(scale, f32)
+    if np.any(sc != 1):
+        px = (px * sc).astype(f32)
+    scene.sky = (px, (np.eye(4, dtype=f32) if transform is None else np.asarray(transform, f32)).reshape(16))
+    return px
 
 
 # ---- OBJ / MTL ----------------------------------------------------------------

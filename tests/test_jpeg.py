@@ -1,0 +1,163 @@
+"""JPEG textures (lighthouse2_amd/jpeg.py), the formats HostTexture::Load
+reads through FreeImage (lib/RenderSystem/host_texture.cpp:200-241). CPU only.
+
+Pinned by closed forms that do not depend on any image library:
+* a block with only a DC coefficient D decodes to 128 + floor((D + 4) / 8)
+  everywhere (the integer IDCT's scaling and rounding);
+* random coefficient blocks decode within one level of the exact float64
+  inverse DCT (a wrong butterfly constant or sign is off by far more);
+* YCbCr -> RGB within one level of the float formula, exact on the grey axis;
+* the triangle upsampling keeps constants and interpolates ramps;
+then cross-checked bit for bit against Pillow's libjpeg decoder (the library
+FreeImage's JPEG plugin is built on) for 4:4:4, 4:2:2, 4:2:0 and grey images,
+with and without restart intervals, and on the reference's own JPEG textures
+when the reference tree is present.
+"""
+import glob
+import io
+import os
+
+import numpy as np
+import pytest
+
+from lighthouse2_amd import jpeg as J
+from lighthouse2_amd import loaders
+
+REF_DATA = "/root/reference/apps/_shareddata"
+
+
+def _zz_block(coef_rowmajor):
+    return np.asarray(coef_rowmajor, np.int64).reshape(1, 64)
+
+
+@pytest.mark.parametrize("D", [-1024, -300, -4, -3, 0, 3, 4, 5, 11, 12, 1016, 1023])
+def test_dc_only_block_is_flat(D):
+    c = np.zeros(64, np.int64)
+    c[0] = D
+    out = J.idct_blocks(_zz_block(c))[0]
+    assert np.all(out == np.clip(128 + ((D + 4) >> 3), 0, 255))
+
+
+def _float_idct(c):
+    """Exact 2-D inverse DCT of an 8x8 block (T.81 A.3.3), float64."""
+    k = np.arange(8)
+    a = np.where(k == 0, 1 / np.sqrt(2), 1.0)
+    T = a[None, :] * np.cos((2 * k[:, None] + 1) * k[None, :] * np.pi / 16) / 2   # [x][u]
+    return T @ c @ T.T
+
+
+def test_integer_idct_is_within_one_level_of_the_exact_transform():
+    rng = np.random.default_rng(1)
+    blocks = np.zeros((400, 8, 8), np.int64)
+    # realistic spectra: energy falling with frequency
+    for b in blocks:
+        u, v = np.mgrid[0:8, 0:8]
+        b[:] = np.round(rng.normal(0, 300 / (1 + u + v), (8, 8)))
+    got = J.idct_blocks(blocks.reshape(-1, 64)).astype(np.int64)
+    for g, c in zip(got, blocks):
+        ref = np.clip(np.round(_float_idct(c.astype(np.float64)) + 128), 0, 255)
+        assert np.max(np.abs(g - ref)) <= 1
+
+
+def test_ycc_to_rgb_matches_the_formula():
+    rng = np.random.default_rng(2)
+    y, cb, cr = (rng.integers(0, 256, 5000) for _ in range(3))
+    r, g, b = J.ycc_to_rgb(y, cb, cr)
+    fr = np.clip(y + 1.402 * (cr - 128), 0, 255)
+    fg = np.clip(y - 0.34414 * (cb - 128) - 0.71414 * (cr - 128), 0, 255)
+    fb = np.clip(y + 1.772 * (cb - 128), 0, 255)
+    for got, ref in ((r, fr), (g, fg), (b, fb)):
+        assert np.max(np.abs(got.astype(float) - ref)) <= 1.0
+    grey = np.arange(256)
+    r, g, b = J.ycc_to_rgb(grey, np.full(256, 128), np.full(256, 128))
+    assert np.array_equal(r, grey) and np.array_equal(g, grey) and np.array_equal(b, grey)
+
+
+def test_upsampling_keeps_constants_and_interpolates():
+    c = np.full((3, 5), 77)
+    assert np.all(J._upsample_h2(c) == 77) and np.all(J._upsample_h2v2(c) == 77)
+    ramp = np.array([[0, 40, 80]])
+    # out[2i] = (3 p[i] + p[i-1] + 1) >> 2, out[2i+1] = (3 p[i] + p[i+1] + 2) >> 2, edges copied
+    assert J._upsample_h2(ramp).tolist() == [[0, 10, 30, 50, 70, 80]]
+
+
+def test_not_a_jpeg_and_unsupported_modes():
+    with pytest.raises(ValueError):
+        J.decode_jpeg(b"\x89PNG\r\n\x1a\n")
+    # a progressive frame header (SOF2) right after SOI
+    sof2 = b"\xff\xd8\xff\xc2\x00\x11\x08\x00\x08\x00\x08\x03\x01\x11\x00\x02\x11\x00\x03\x11\x00"
+    with pytest.raises(ValueError, match="progressive"):
+        J.decode_jpeg(sof2)
+
+
+def _pil():
+    return pytest.importorskip("PIL.Image")
+
+
+def _pil_rgb(Image, b):
+    return np.asarray(Image.open(io.BytesIO(b)).convert("RGB"))
+
+
+def _test_image(h=67, w=93, seed=0):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = np.stack([xx * 255 / w, yy * 255 / h, (xx + yy) * 2 % 256], -1).astype(np.float64)
+    return np.clip(img + rng.normal(0, 20, img.shape), 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("subsampling", [0, 1, 2])        # 4:4:4, 4:2:2, 4:2:0
+@pytest.mark.parametrize("quality", [40, 95])
+@pytest.mark.parametrize("restart", [None, 3])
+def test_equals_libjpeg(subsampling, quality, restart):
+    Image = _pil()
+    buf = io.BytesIO()
+    kw = {"restart_marker_blocks": restart} if restart else {}
+    Image.fromarray(_test_image()).save(buf, "JPEG", quality=quality, subsampling=subsampling, **kw)
+    b = buf.getvalue()
+    if restart:
+        assert b.find(b"\xff\xdd") >= 0
+    got = J.decode_jpeg(b)
+    assert got.shape == (67, 93, 4) and np.all(got[..., 3] == 255)
+    assert np.array_equal(got[..., :3], _pil_rgb(Image, b))
+
+
+def test_grey_equals_libjpeg():
+    Image = _pil()
+    buf = io.BytesIO()
+    Image.fromarray(_test_image()[..., 1]).save(buf, "JPEG", quality=80)
+    got = J.decode_jpeg(buf.getvalue())
+    ref = np.asarray(Image.open(io.BytesIO(buf.getvalue())).convert("L"))
+    assert all(np.array_equal(got[..., k], ref) for k in range(3))
+
+
+def test_texture_loader_reads_jpeg(tmp_path):
+    Image = _pil()
+    img = _test_image(16, 24, seed=3)
+    p = tmp_path / "t.jpg"
+    Image.fromarray(img).save(p, "JPEG", quality=90)
+    t = loaders.load_texture(str(p))
+    ref = _pil_rgb(Image, p.read_bytes()).astype(np.uint32)
+    px = t["pixels"][:16 * 24].reshape(16, 24)
+    assert t["width"] == 24 and t["height"] == 16
+    assert np.array_equal(px, (255 << 24) | (ref[..., 2] << 16) | (ref[..., 1] << 8) | ref[..., 0])
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DATA), reason="reference tree not present")
+def test_reference_textures_equal_libjpeg():
+    """The reference's scene textures (book, escher, knight): baseline JPEGs
+    decode as libjpeg decodes them; its one progressive file raises."""
+    Image = _pil()
+    files = sorted(glob.glob(os.path.join(REF_DATA, "**", "*.jp*g"), recursive=True))
+    assert files
+    done = 0
+    for f in files:
+        b = open(f, "rb").read()
+        if b"\xff\xc2" in b[:2000]:
+            with pytest.raises(ValueError, match="progressive"):
+                J.decode_jpeg(b)
+            continue
+        if len(b) > 400_000:
+            continue                      # the 2048^2 receiver textures: same code path, seconds each
+        assert np.array_equal(J.decode_jpeg(b)[..., :3], _pil_rgb(Image, b)), f
+        done += 1
+    assert done >= 10
