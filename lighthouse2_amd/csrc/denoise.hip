@@ -717,16 +717,116 @@ hipError_t dn_conv3x3_dw_noalloc(const bf16_t* x, int Cin, const bf16_t* dy, int
 }
 
 /* ---- elementwise and layout kernels ------------------------------------ */
+/* 16-byte versions of the elementwise kernels below: a thread handles 8
+ * channels of one pixel (C a multiple of 8, 16-byte aligned tensors), a grid
+ * row (blockIdx.y) one image row, so no per-element index division; the
+ * per-element arithmetic is the scalar kernels' (identical results). */
+typedef __attribute__((ext_vector_type(8))) unsigned short u16x8;
+
+__device__ __forceinline__ u16x8 ld8(const bf16_t* p) { return *(const u16x8*)p; }
+__device__ __forceinline__ void st8(bf16_t* p, u16x8 v) { *(u16x8*)p = v; }
+
+__global__ void k_relu_bwd8(const bf16_t* __restrict__ y, const bf16_t* __restrict__ dy, bf16_t* __restrict__ out, size_t n8)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n8; i += (size_t)gridDim.x * blockDim.x)
+	{
+		const u16x8 a = ld8(y + 8 * i), g = ld8(dy + 8 * i);
+		u16x8 o;
+#pragma unroll
+		for (int k = 0; k < 8; k++) o[k] = bf2f(a[k]) > 0.0f ? g[k] : (unsigned short)0;
+		st8(out + 8 * i, o);
+	}
+}
+
+/* pixel row y of a W-wide image: groups j = px * G + c8 of 8 channels (G = C / 8 = 1 << gs) */
+__global__ void k_pool2_bwd8(const bf16_t* __restrict__ dy, int gs, int W, bf16_t* __restrict__ dx, int acc)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x, G = 1 << gs, py = blockIdx.y;
+	if (j >= W * G) return;
+	const int px = j >> gs, c8 = j & (G - 1);
+	const u16x8 d = ld8(dy + ((size_t)(py >> 1) * (W >> 1) + (px >> 1)) * (8 * G) + 8 * c8);
+	bf16_t* o = dx + ((size_t)py * W + px) * (8 * G) + 8 * c8;
+	const u16x8 prev = acc ? ld8(o) : u16x8{};
+	u16x8 r;
+#pragma unroll
+	for (int k = 0; k < 8; k++)
+	{
+		float v = bf2f(d[k]) * 0.25f;
+		if (acc) v += bf2f(prev[k]);
+		r[k] = f2bf(v);
+	}
+	st8(o, r);
+}
+
+__global__ void k_pool2_8(const bf16_t* __restrict__ x, int gs, int W, bf16_t* __restrict__ y)
+{
+	const int w2 = W >> 1, j = blockIdx.x * blockDim.x + threadIdx.x, G = 1 << gs, py = blockIdx.y;
+	if (j >= w2 * G) return;
+	const int px = j >> gs, c8 = j & (G - 1), C = 8 * G;
+	const bf16_t* b = x + ((size_t)(2 * py) * W + 2 * px) * C + 8 * c8;
+	const u16x8 a = ld8(b), c = ld8(b + C), d = ld8(b + (size_t)W * C), e = ld8(b + (size_t)W * C + C);
+	u16x8 r;
+#pragma unroll
+	for (int k = 0; k < 8; k++) r[k] = f2bf(((bf2f(a[k]) + bf2f(c[k])) + (bf2f(d[k]) + bf2f(e[k]))) * 0.25f);
+	st8(y + ((size_t)py * w2 + px) * C + 8 * c8, r);
+}
+
+/* cat row: groups j = px * (G0 + G1) + g; the first G0 from the coarse pixel */
+__global__ void k_upcat8(const bf16_t* __restrict__ coarse, int G0, const bf16_t* __restrict__ skip, int G1, int W, bf16_t* __restrict__ out)
+{
+	const int G = G0 + G1, j = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y;
+	if (j >= W * G) return;
+	const int px = j / G, g = j - px * G;
+	const u16x8 v = g < G0 ? ld8(coarse + ((size_t)(py >> 1) * (W >> 1) + (px >> 1)) * (8 * G0) + 8 * g)
+		: ld8(skip + ((size_t)py * W + px) * (8 * G1) + 8 * (g - G0));
+	st8(out + ((size_t)py * W + px) * (8 * G) + 8 * g, v);
+}
+
+__global__ void k_upcat_bwd_coarse8(const bf16_t* __restrict__ dcat, int G0, int G1, int W, bf16_t* __restrict__ dcoarse)
+{
+	const int G = G0 + G1, w2 = W >> 1, j = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y;
+	if (j >= w2 * G0) return;
+	const int px = j / G0, g = j - px * G0, C = 8 * G;
+	const bf16_t* b = dcat + ((size_t)(2 * py) * W + 2 * px) * C + 8 * g;
+	const u16x8 a = ld8(b), c = ld8(b + C), d = ld8(b + (size_t)W * C), e = ld8(b + (size_t)W * C + C);
+	u16x8 r;
+#pragma unroll
+	for (int k = 0; k < 8; k++) r[k] = f2bf((bf2f(a[k]) + bf2f(c[k])) + (bf2f(d[k]) + bf2f(e[k])));
+	st8(dcoarse + ((size_t)py * w2 + px) * (8 * G0) + 8 * g, r);
+}
+
+__global__ void k_upcat_bwd_skip8(const bf16_t* __restrict__ dcat, int G0, int G1, int W, bf16_t* __restrict__ dskip)
+{
+	const int G = G0 + G1, j = blockIdx.x * blockDim.x + threadIdx.x, py = blockIdx.y;
+	if (j >= W * G1) return;
+	const int px = j / G1, g = j - px * G1;
+	st8(dskip + ((size_t)py * W + px) * (8 * G1) + 8 * g, ld8(dcat + ((size_t)py * W + px) * (8 * G) + 8 * (G0 + g)));
+}
+
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+static int log2_exact(int v) { int k = 0; while ((1 << k) < v) k++; return (1 << k) == v ? k : -1; }
+static dim3 row_grid(int perRow, int rows) { return dim3((unsigned)((perRow + 255) / 256), (unsigned)rows); }
+
 __global__ void k_relu_bwd(const bf16_t* __restrict__ y, const bf16_t* __restrict__ dy, bf16_t* __restrict__ out, size_t n)
 {
 	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
 		out[i] = bf2f(y[i]) > 0.0f ? dy[i] : (bf16_t)0;
 }
 
+static bool aligned16_3(const void* a, const void* b, const void* c) { return (((uintptr_t)a | (uintptr_t)b | (uintptr_t)c) & 15) == 0; }
+
 hipError_t dn_relu_bwd(const bf16_t* y, const bf16_t* dy, bf16_t* dyPre, size_t n, hipStream_t s)
 {
+	if (!n) return hipSuccess;
+	if (n % 8 == 0 && aligned16_3(y, dy, dyPre))
+	{
+		const size_t n8 = n / 8;
+		const int blocks = (int)((n8 + 255) / 256 < 65536 ? (n8 + 255) / 256 : 65536);
+		k_relu_bwd8<<<blocks, 256, 0, s>>>(y, dy, dyPre, n8);
+		return hipGetLastError();
+	}
 	const int blocks = (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
-	if (n) k_relu_bwd<<<blocks, 256, 0, s>>>(y, dy, dyPre, n);
+	k_relu_bwd<<<blocks, 256, 0, s>>>(y, dy, dyPre, n);
 	return hipGetLastError();
 }
 
@@ -799,27 +899,45 @@ static int grid_for(size_t n) { const size_t b = (n + 255) / 256; return (int)(b
 hipError_t dn_pool2(const bf16_t* x, int C, int H, int W, bf16_t* y, hipStream_t s)
 {
 	if (H % 2 || W % 2) return hipErrorInvalidValue;
-	k_pool2<<<grid_for((size_t)H * W * C / 4), 256, 0, s>>>(x, C, H, W, y);
+	if (!H || !W || !C) return hipSuccess;
+	const int gs = C % 8 ? -1 : log2_exact(C / 8);
+	if (gs >= 0 && aligned16(x) && aligned16(y) && H / 2 <= 65535)
+		k_pool2_8<<<row_grid((W / 2) << gs, H / 2), 256, 0, s>>>(x, gs, W, y);
+	else k_pool2<<<grid_for((size_t)H * W * C / 4), 256, 0, s>>>(x, C, H, W, y);
 	return hipGetLastError();
 }
 
 hipError_t dn_pool2_bwd(const bf16_t* dy, int C, int H, int W, bf16_t* dx, int accumulate, hipStream_t s)
 {
 	if (H % 2 || W % 2) return hipErrorInvalidValue;
-	k_pool2_bwd<<<grid_for((size_t)H * W * C), 256, 0, s>>>(dy, C, H, W, dx, accumulate);
+	if (!H || !W || !C) return hipSuccess;
+	const int gs = C % 8 ? -1 : log2_exact(C / 8);
+	if (gs >= 0 && aligned16(dy) && aligned16(dx) && H <= 65535)
+		k_pool2_bwd8<<<row_grid(W << gs, H), 256, 0, s>>>(dy, gs, W, dx, accumulate);
+	else k_pool2_bwd<<<grid_for((size_t)H * W * C), 256, 0, s>>>(dy, C, H, W, dx, accumulate);
 	return hipGetLastError();
 }
 
 hipError_t dn_upcat(const bf16_t* coarse, int C0, const bf16_t* skip, int C1, int H, int W, bf16_t* out, hipStream_t s)
 {
 	if (H % 2 || W % 2) return hipErrorInvalidValue;
-	k_upcat<<<grid_for((size_t)H * W * (C0 + C1)), 256, 0, s>>>(coarse, C0, skip, C1, H, W, out);
+	if (!H || !W || !(C0 + C1)) return hipSuccess;
+	if (C0 > 0 && C1 > 0 && C0 % 8 == 0 && C1 % 8 == 0 && aligned16_3(coarse, skip, out) && H <= 65535)
+		k_upcat8<<<row_grid(W * ((C0 + C1) / 8), H), 256, 0, s>>>(coarse, C0 / 8, skip, C1 / 8, W, out);
+	else k_upcat<<<grid_for((size_t)H * W * (C0 + C1)), 256, 0, s>>>(coarse, C0, skip, C1, H, W, out);
 	return hipGetLastError();
 }
 
 hipError_t dn_upcat_bwd(const bf16_t* dcat, int C0, int C1, int H, int W, bf16_t* dcoarse, bf16_t* dskip, hipStream_t s)
 {
 	if (H % 2 || W % 2) return hipErrorInvalidValue;
+	if (!H || !W || !(C0 + C1)) return hipSuccess;
+	if (C0 > 0 && C1 > 0 && C0 % 8 == 0 && C1 % 8 == 0 && aligned16_3(dcat, dcoarse, dskip) && H <= 65535)
+	{
+		k_upcat_bwd_coarse8<<<row_grid((W / 2) * (C0 / 8), H / 2), 256, 0, s>>>(dcat, C0 / 8, C1 / 8, W, dcoarse);
+		k_upcat_bwd_skip8<<<row_grid(W * (C1 / 8), H), 256, 0, s>>>(dcat, C0 / 8, C1 / 8, W, dskip);
+		return hipGetLastError();
+	}
 	k_upcat_bwd_coarse<<<grid_for((size_t)H * W * C0 / 4), 256, 0, s>>>(dcat, C0, C1, H, W, dcoarse);
 	k_upcat_bwd_skip<<<grid_for((size_t)H * W * C1), 256, 0, s>>>(dcat, C0, C1, (size_t)H * W, dskip);
 	return hipGetLastError();

@@ -253,3 +253,57 @@ def test_fused_display_equals_unfused(torch_cuda, w, h):
     r, g = ref.cpu().numpy(), got.cpu().numpy()
     assert np.isfinite(r).all() and (r[..., :3] > 0).any()
     assert np.array_equal(r.view(np.uint32), g.view(np.uint32))
+
+
+def test_elementwise_kernels_vector_and_scalar_paths_agree(torch_cuda):
+    """relu_bwd, pool2, pool2_bwd (= and +=), upcat and upcat_bwd on 16-byte
+    aligned tensors (8 channels per thread) and on tensors one element off
+    alignment (the per-element kernels): identical bits."""
+    torch = torch_cuda
+    L = lh.library()
+    rng = np.random.default_rng(77)
+    H, W, C = 22, 70, 32
+    sh = torch.cuda.current_stream().cuda_stream
+
+    def dev(a, off):
+        """bf16 bits of a on the device starting `off` elements into a buffer."""
+        bits = torch.from_numpy(D.bf16_bits(a).reshape(-1).copy())
+        buf = torch.zeros(bits.numel() + 8, dtype=torch.int16, device="cuda")
+        buf[off:off + bits.numel()] = bits.cuda()
+        return buf[off:off + bits.numel()]
+
+    def run(off):
+        x = dev(rng_state["x"], off)
+        g = dev(rng_state["g"], off)
+        co = dev(rng_state["co"], off)
+        out = {}
+        y = dev(np.zeros((H, W, C), np.float32), off)
+        assert L.lh2hip_dn_relu_bwd(x.data_ptr(), g.data_ptr(), y.data_ptr(), H * W * C, sh) == 0
+        out["relu"] = y
+        p = dev(np.zeros((H // 2, W // 2, C), np.float32), off)
+        assert L.lh2hip_dn_pool2(x.data_ptr(), C, H, W, p.data_ptr(), sh) == 0
+        out["pool"] = p
+        dxs = dev(np.zeros((H, W, C), np.float32), off)
+        assert L.lh2hip_dn_pool2_bwd(co.data_ptr(), C, H, W, dxs.data_ptr(), 0, sh) == 0
+        out["pool_bwd"] = dxs.clone()
+        assert L.lh2hip_dn_pool2_bwd(co.data_ptr(), C, H, W, dxs.data_ptr(), 1, sh) == 0
+        out["pool_bwd_acc"] = dxs
+        cat = dev(np.zeros((H, W, 2 * C), np.float32), off)
+        assert L.lh2hip_dn_upcat(co.data_ptr(), C, x.data_ptr(), C, H, W, cat.data_ptr(), sh) == 0
+        out["upcat"] = cat
+        dco = dev(np.zeros((H // 2, W // 2, C), np.float32), off)
+        dsk = dev(np.zeros((H, W, C), np.float32), off)
+        assert L.lh2hip_dn_upcat_bwd(cat.data_ptr(), C, C, H, W, dco.data_ptr(), dsk.data_ptr(), sh) == 0
+        out["upcat_bwd_coarse"], out["upcat_bwd_skip"] = dco, dsk
+        torch.cuda.synchronize()
+        return {k: v.cpu().numpy() for k, v in out.items()}
+
+    rng_state = {"x": R.bf16(rng.standard_normal((H, W, C))), "g": R.bf16(rng.standard_normal((H, W, C))),
+                 "co": R.bf16(rng.standard_normal((H // 2, W // 2, C)))}
+    vec, sca = run(0), run(1)
+    for k in vec:
+        assert np.array_equal(vec[k], sca[k]), k
+    # the scalar path itself against the definitions
+    x, g = rng_state["x"], rng_state["g"]
+    assert np.array_equal(D.bf16_to_f32(vec["relu"]).reshape(H, W, C), np.where(x > 0, g, 0).astype(np.float32))
+    assert np.array_equal(D.bf16_to_f32(vec["upcat_bwd_skip"]).reshape(H, W, C), x)
