@@ -12,7 +12,8 @@
  *   NodeF32  binary node holding both child boxes in fp32;
  *   NodeQ8   4-wide node, child boxes quantised to 8 bits per plane against
  *            the node box (lo rounded down, hi rounded up: conservative).
- * Child refs: bit 31 = leaf, low 30 bits = node index or first leaf triangle.
+ * Child refs: bit 31 = leaf, low 30 bits = node index or first leaf triangle;
+ * an unused q8 child slot holds Q8_EMPTY.
  * Leaf triangles: 3 float4 (vertex 0..2); v0.w = primitive id | bit 31 on the
  * last triangle of a leaf.
  */
@@ -35,16 +36,20 @@ struct NodeF32
 
 struct NodeQ8
 {
-	float4 origin;    /* xyz = quantisation origin; w = bits: ex | ey << 8 | ez << 16 | nchild << 24 */
+	float4 origin;    /* xyz = quantisation origin; w = step of z (the float with exponent field ez) */
 	uint4 qa;         /* lo.x[4], lo.y[4], lo.z[4], hi.x[4] (one byte per child) */
-	uint4 qb;         /* hi.y[4], hi.z[4], unused, unused */
-	uint4 refs;       /* child refs */
+	uint4 qb;         /* hi.y[4], hi.z[4], step of x, step of y (float bits, exponent fields ex, ey) */
+	uint4 refs;       /* child refs, Q8_EMPTY for unused slots */
 };
 
 static const uint32_t REF_LEAF = 0x80000000u;
 static const uint32_t REF_TLAS = 0x40000000u;
 static const uint32_t REF_MASK = 0x3fffffffu;
 static const uint32_t TRI_LAST = 0x80000000u;
+static const uint32_t Q8_EMPTY = 0xffffffffu;
+/* A q8 plane lies at origin + q * step with step = 2^(e - 127) stored as the
+ * float whose exponent field is e (e = 0 gives 0, e = 255 infinity). */
+__host__ __device__ inline uint32_t q8_step_bits(uint32_t e) { return e << 23; }
 
 /* Per-instance record for the two-level traversal (64 bytes). */
 struct InstanceRec

@@ -246,9 +246,9 @@ __global__ void k_refit_q8(const BuildNode* __restrict__ nodes, NodeQ8* __restri
 		ex[a] = quantise_axis_dev(B.lo[a], B.hi[a], clo, chi, cnt, q[a], q[3 + a]);
 	}
 	auto pack = [](const uint32_t* v) { return v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24); };
-	out[i].origin = make_float4(B.lo[0], B.lo[1], B.lo[2], __uint_as_float(ex[0] | (ex[1] << 8) | (ex[2] << 16) | ((uint32_t)cnt << 24)));
+	out[i].origin = make_float4(B.lo[0], B.lo[1], B.lo[2], __uint_as_float(q8_step_bits(ex[2])));
 	out[i].qa = make_uint4(pack(q[0]), pack(q[1]), pack(q[2]), pack(q[3]));
-	out[i].qb = make_uint4(pack(q[4]), pack(q[5]), 0, 0);
+	out[i].qb = make_uint4(pack(q[4]), pack(q[5]), q8_step_bits(ex[0]), q8_step_bits(ex[1]));
 }
 
 hipError_t refit_layout(const BuildNode* dNodes, NodeF32* dF32, const int32_t* dF32Kids, int nF32, NodeQ8* dQ8, const int32_t* dQ8Map, int nQ8, hipStream_t stream)
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(LBLOCK) k_lay_down2(LayoutArgs a, int b, int e
 		if (k[j] >= 0) cnt = j + 1, ik += a.nodes[k[j]].count == 0 ? 1 : 0;
 	const int A = fld(a, F_AQ8, x), s = fld(a, F_SQ8, x);
 	int run = A + ik, j2 = 0;
-	uint32_t refs[4] = { 0, 0, 0, 0 };
+	uint32_t refs[4] = { Q8_EMPTY, Q8_EMPTY, Q8_EMPTY, Q8_EMPTY };
 	for (int j = 0; j < cnt; j++)
 	{
 		const BuildNode& C = a.nodes[k[j]];
@@ -292,9 +292,9 @@ __global__ void __launch_bounds__(LBLOCK) k_lay_down2(LayoutArgs a, int b, int e
 	}
 	auto pack = [](const uint32_t* v) { return v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24); };
 	NodeQ8 o;
-	o.origin = make_float4(B.lo[0], B.lo[1], B.lo[2], __uint_as_float(ex[0] | (ex[1] << 8) | (ex[2] << 16) | ((uint32_t)cnt << 24)));
+	o.origin = make_float4(B.lo[0], B.lo[1], B.lo[2], __uint_as_float(q8_step_bits(ex[2])));
 	o.qa = make_uint4(pack(q[0]), pack(q[1]), pack(q[2]), pack(q[3]));
-	o.qb = make_uint4(pack(q[4]), pack(q[5]), 0, 0);
+	o.qb = make_uint4(pack(q[4]), pack(q[5]), q8_step_bits(ex[0]), q8_step_bits(ex[1]));
 	o.refs = make_uint4(refs[0], refs[1], refs[2], refs[3]);
 	a.q8[s] = o;
 	if (a.q8Map)

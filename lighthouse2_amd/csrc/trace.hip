@@ -54,10 +54,13 @@ __device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, flo
 	return tn <= tf;
 }
 
+/* An LDS pointer typed as such, so that loads through it stay LDS loads. */
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 template <int BLOCK>
 struct Stack
 {
-	uint32_t* lds;      /* TRACE_LDS_STACK x BLOCK */
+	lds_u32* lds;       /* TRACE_LDS_STACK x BLOCK */
 	uint32_t* ovf;      /* this thread's overflow area */
 	int tid, sp;
 	/* The host checks that the tree depth fits TRACE_MAX_STACK; a push beyond
@@ -88,10 +91,14 @@ struct Stack
 			if (ha) push(a);
 		}
 	}
+	/* LDS read for every lane (the row clamped into the LDS part), the
+	 * overflow area read only by lanes below it. */
 	__device__ __forceinline__ uint32_t pop()
 	{
 		sp--;
-		return sp < TRACE_LDS_STACK ? lds[sp * BLOCK + tid] : ovf[sp - TRACE_LDS_STACK];
+		uint32_t v = lds[min(sp, TRACE_LDS_STACK - 1) * BLOCK + tid];
+		if (sp >= TRACE_LDS_STACK) v = ovf[sp - TRACE_LDS_STACK];
+		return v;
 	}
 };
 
@@ -126,22 +133,20 @@ __device__ __forceinline__ bool is_top(uint32_t r) { return (r & REF_TLAS) && r 
 template <int BLOCK>
 __device__ __forceinline__ void visit_q8(const NodeQ8& n, const Lane& L, Stack<BLOCK>& st, uint32_t& ref)
 {
-	const uint32_t e = fbits(n.origin.w);
-	const float dx = bitsf((e & 255u) << 23) * L.inv.x;
-	const float dy = bitsf(((e >> 8) & 255u) << 23) * L.inv.y;
-	const float dz = bitsf(((e >> 16) & 255u) << 23) * L.inv.z;
+	const float dx = bitsf(n.qb.z) * L.inv.x;
+	const float dy = bitsf(n.qb.w) * L.inv.y;
+	const float dz = n.origin.w * L.inv.z;
 	const float bx = (n.origin.x - L.O.x) * L.inv.x;
 	const float by = (n.origin.y - L.O.y) * L.inv.y;
 	const float bz = (n.origin.z - L.O.z) * L.inv.z;
-	const float ex = (fabsf(bx) + 255.0f * fabsf(dx)) * 9.5367431640625e-7f;
-	const float ey = (fabsf(by) + 255.0f * fabsf(dy)) * 9.5367431640625e-7f;
-	const float ez = (fabsf(bz) + 255.0f * fabsf(dz)) * 9.5367431640625e-7f;
+	const float ex = fmaf(fabsf(dx), 2.43186950683593750e-4f, fabsf(bx) * 9.5367431640625e-7f);   /* 255 * 2^-20, 2^-20 */
+	const float ey = fmaf(fabsf(dy), 2.43186950683593750e-4f, fabsf(by) * 9.5367431640625e-7f);
+	const float ez = fmaf(fabsf(dz), 2.43186950683593750e-4f, fabsf(bz) * 9.5367431640625e-7f);
 	const float bnx = bx - ex, bfx = bx + ex, bny = by - ey, bfy = by + ey, bnz = bz - ez, bfz = bz + ez;
 	const bool px = L.inv.x >= 0.0f, py = L.inv.y >= 0.0f, pz = L.inv.z >= 0.0f;
 	const uint32_t qnx = px ? n.qa.x : n.qa.w, qfx = px ? n.qa.w : n.qa.x;
 	const uint32_t qny = py ? n.qa.y : n.qb.x, qfy = py ? n.qb.x : n.qa.y;
 	const uint32_t qnz = pz ? n.qa.z : n.qb.y, qfz = pz ? n.qb.y : n.qa.z;
-	const int nc = (int)(e >> 24);
 	float tk[4];
 	uint32_t rk[4] = { n.refs.x, n.refs.y, n.refs.z, n.refs.w };
 #pragma unroll
@@ -149,7 +154,7 @@ __device__ __forceinline__ void visit_q8(const NodeQ8& n, const Lane& L, Stack<B
 	{
 		const float tn = fmaxf(fmaxf(fmaf(ubyte(qnx, c), dx, bnx), fmaf(ubyte(qny, c), dy, bny)), fmaxf(fmaf(ubyte(qnz, c), dz, bnz), L.tmin));
 		const float tf = fminf(fminf(fmaf(ubyte(qfx, c), dx, bfx), fmaf(ubyte(qfy, c), dy, bfy)), fminf(fmaf(ubyte(qfz, c), dz, bfz), L.tbest));
-		tk[c] = (c < nc && tn <= tf) ? tn : __builtin_huge_valf();
+		tk[c] = (rk[c] != Q8_EMPTY && tn <= tf) ? tn : __builtin_huge_valf();
 	}
 	cswap(tk[0], tk[1], rk[0], rk[1]);
 	cswap(tk[2], tk[3], rk[2], rk[3]);
@@ -330,7 +335,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 5 : 6) k_trace_merged(const
 	__shared__ uint32_t sstack[TRACE_LDS_STACK * TRACE_BLOCK];
 	const int count = countDev ? (int)*countDev : maxCount;
 	Stack<TRACE_BLOCK> st;
-	st.lds = sstack, st.ovf = overflow + ((size_t)blockIdx.x * TRACE_BLOCK + threadIdx.x) * TRACE_OVF_STACK, st.tid = threadIdx.x, st.sp = 0;
+	st.lds = (lds_u32*)sstack, st.ovf = overflow + ((size_t)blockIdx.x * TRACE_BLOCK + threadIdx.x) * TRACE_OVF_STACK, st.tid = threadIdx.x, st.sp = 0;
 	Lane L;
 	L.inst = -1;
 	L.nodes = sc.nodes, L.tris = sc.tris;
@@ -432,7 +437,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace_greedy(const
 	__shared__ uint32_t sstack[TRACE_LDS_STACK * TRACE_BLOCK];
 	const int count = countDev ? (int)*countDev : maxCount;
 	Stack<TRACE_BLOCK> st;
-	st.lds = sstack, st.ovf = overflow + ((size_t)blockIdx.x * TRACE_BLOCK + threadIdx.x) * TRACE_OVF_STACK, st.tid = threadIdx.x, st.sp = 0;
+	st.lds = (lds_u32*)sstack, st.ovf = overflow + ((size_t)blockIdx.x * TRACE_BLOCK + threadIdx.x) * TRACE_OVF_STACK, st.tid = threadIdx.x, st.sp = 0;
 	Lane L;
 	L.inst = -1;
 	L.nodes = sc.nodes, L.tris = sc.tris;
@@ -528,7 +533,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK, TWO ? 6 : 8) k_trace_ww(const Tra
 	__shared__ uint32_t sstack[TRACE_LDS_STACK * TRACE_BLOCK];
 	const int count = countDev ? (int)*countDev : maxCount;
 	Stack<TRACE_BLOCK> st;
-	st.lds = sstack, st.ovf = overflow + ((size_t)blockIdx.x * TRACE_BLOCK + threadIdx.x) * TRACE_OVF_STACK, st.tid = threadIdx.x, st.sp = 0;
+	st.lds = (lds_u32*)sstack, st.ovf = overflow + ((size_t)blockIdx.x * TRACE_BLOCK + threadIdx.x) * TRACE_OVF_STACK, st.tid = threadIdx.x, st.sp = 0;
 	Lane L;
 	L.inst = -1;
 	L.nodes = sc.nodes, L.tris = sc.tris;

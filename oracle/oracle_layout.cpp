@@ -207,11 +207,12 @@ struct Walker
 			}
 			NodeQ8 nd;
 			memset(&nd, 0, sizeof(nd));
-			nd.origin = make_float4(B.lo[0], B.lo[1], B.lo[2], bits_to_float((uint32_t)ex[0] | ((uint32_t)ex[1] << 8) | ((uint32_t)ex[2] << 16) | ((uint32_t)cnt << 24)));
+			/* plane step 2^(e - 127) stored as the float with exponent field e */
+			nd.origin = make_float4(B.lo[0], B.lo[1], B.lo[2], bits_to_float((uint32_t)ex[2] << 23));
 			auto pack = [&](const uint8_t* q) { return (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16) | ((uint32_t)q[3] << 24); };
 			nd.qa = make_uint4(pack(qlo[0]), pack(qlo[1]), pack(qlo[2]), pack(qhi[0]));
-			nd.qb = make_uint4(pack(qhi[1]), pack(qhi[2]), 0, 0);
-			uint32_t refs[4] = { 0, 0, 0, 0 };
+			nd.qb = make_uint4(pack(qhi[1]), pack(qhi[2]), (uint32_t)ex[0] << 23, (uint32_t)ex[1] << 23);
+			uint32_t refs[4] = { 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu };   /* unused slots */
 			std::vector<Item> pend;
 			for (int c = 0; c < cnt; c++)
 			{

@@ -89,13 +89,21 @@ def check_f32(nodes, f32):
     assert len(seen) == len(f32)
 
 
+Q8_EMPTY = 0xFFFFFFFF
+
+
 def decode_q8(row):
+    """origin.xyz, plane steps as floats whose exponent fields are ex, ey, ez
+    (qb.z, qb.w, origin.w), the child count from the unused slots' refs."""
     origin = f32v(row[0:3])
-    w = int(row[3])
-    ex = [w & 255, (w >> 8) & 255, (w >> 16) & 255]
-    cnt = w >> 24
+    steps = [int(row[10]), int(row[11]), int(row[3])]
+    assert all(b & 0x807FFFFF == 0 for b in steps)          # exactly 2^(e - 127), 0 or inf
+    ex = [b >> 23 for b in steps]
+    refs = [int(r) for r in row[12:16]]
+    cnt = sum(r != Q8_EMPTY for r in refs)
+    assert all(r != Q8_EMPTY for r in refs[:cnt]) and all(r == Q8_EMPTY for r in refs[cnt:])
     q = np.array([[(int(row[4 + k]) >> (8 * c)) & 255 for c in range(4)] for k in range(6)])   # lo.xyz, hi.xyz
-    return origin, ex, cnt, q, [int(r) for r in row[12:16]]
+    return origin, ex, cnt, q, refs
 
 
 def contains(lo, hi, origin, e, qlo, qhi):
