@@ -654,11 +654,12 @@ def bench_extend(a, np, torch, lh, scenes):
                    "blocks_per_cu": f"{a.blocks_per_cu or 8} requested, capped at the kernel occupancy", "min_rays_per_lane": a.min_rays_per_lane,
                    "loop": a.loop, "ray_chunk": a.chunk, "refill_lanes": a.refill_lanes},
         # achieved = algorithmic bytes per ray (ray + hit, nodes and triangles the kernel visits on these
-        # rays) x rays / launch time; the kernel's limiter is VALU issue (SQ counters in profiles/), and
+        # rays) x rays / launch time; the kernel is co-limited by VALU issue and the vector-memory address
+        # and data units (SQ and TA/TD counters in profiles/), and
         # the HBM traffic that reaches memory is far below the algorithmic bytes (MALL and L2 hits)
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                     "bytes": "algorithmic", "limiter": "VALU issue",
+                     "bytes": "algorithmic", "limiter": "VALU issue + vector-memory units",
                      "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 2),
                      "tris_per_ray": round(tris_per_ray, 2)},
         "cpu_baseline": cpu,
