@@ -24,6 +24,7 @@ libjpeg's default arithmetic, the library behind FreeImage's JPEG plugin. All fl
 arithmetic of the producers is float32 in the reference's operation order.
 """
 import os
+import re
 import struct
 import zlib
 
@@ -220,6 +221,38 @@ def texture_from_rgba(rgba, flipped=False, linearized=False, normal_map=False):
     return t
 
 
+def bump_to_normal_map(tex, height_scale=1.0):
+    """HostTexture::BumpToNormalMap (host_texture.cpp:320-343): the red byte
+    of level 0 is a height h = r * (1/255); per texel n = normalize((h(x-1) -
+    h(x+1)) * s, (h(y+1) - h(y-1)) * s, 1) with the texel itself standing in
+    for a missing neighbour at the borders (rows in stored order), in float
+    arithmetic (normalize multiplies by 1 / sqrtf of the dot product); the
+    texel becomes round((n * 0.5 + 0.5) * 255) per component (in double,
+    halves away from zero) with alpha 255. Only level 0 is rewritten: the
+    MIP levels keep what the loader built from the height image, as in the
+    reference, which converts after loading."""
+    w, h = tex["width"], tex["height"]
+    px = tex["pixels"]
+    f32 = np.float32
+    step = f32(1.0) / f32(255.0)
+    z = (px[:w * h] & 255).reshape(h, w).astype(f32) * step
+    x_prev = np.concatenate([z[:, :1], z[:, :-1]], axis=1)
+    x_next = np.concatenate([z[:, 1:], z[:, -1:]], axis=1)
+    y_prev = np.concatenate([z[1:], z[-1:]], axis=0)
+    y_next = np.concatenate([z[:1], z[:-1]], axis=0)
+    s = f32(height_scale)
+    nx = (x_prev - x_next) * s
+    ny = (y_prev - y_next) * s
+    inv = f32(1.0) / np.sqrt((nx * nx + ny * ny) + f32(1.0))
+    out = []
+    for c in (nx * inv, ny * inv, inv):
+        v = (c.astype(np.float64) * 0.5 + 0.5) * 255.0
+        out.append(np.floor(v + 0.5).astype(np.uint32))        # round(): v >= 0 and v + 0.5 is exact
+    px[:w * h] = (out[0] | (out[1] << 8) | (out[2] << 16) | np.uint32(255 << 24)).reshape(-1)
+    mark_normal_map(tex)
+    return tex
+
+
 def mark_normal_map(tex):
     """HostMaterial sets NORMALMAP on a texture it uses as a normal map; the
     core then receives it as NRM32 texels (host_texture.cpp:61)."""
@@ -334,6 +367,12 @@ def load_skydome(scene, path, scale=(1.0, 1.0, 1.0), transform=None):
 
 
 # ---- OBJ / MTL ----------------------------------------------------------------
+def _atof(text):
+    """C atof: the longest leading decimal number, 0.0 if there is none."""
+    m = re.match(r"\s*([+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?)", text)
+    return float(m.group(1)) if m else 0.0
+
+
 def parse_mtl(text):
     """MTL text -> list of dicts (name, Kd, Kt, Ni, map_Kd, map_bump/norm, map_Ks, shading)."""
     mats, cur = [], None
@@ -355,8 +394,9 @@ def parse_mtl(text):
         elif k in ("map_Kd", "map_Ks", "norm", "map_bump", "bump"):
             name = args[-1]                           # options (-bm 1 ...) come before the file name
             cur["map_bump" if k in ("bump", "map_bump") else k] = name
-            if k in ("map_bump", "bump") and "-bm" in args:
-                cur["bump_height"] = float(args[args.index("-bm") + 1])
+        elif k == "bump_height":
+            # an unknown MTL parameter, read with atof (host_material.cpp:57-59); a -bm option is not used
+            cur["bump_height"] = _atof(" ".join(args))
         elif k == "shading":
             cur["shading"] = args[0] if args else ""
     return mats
@@ -519,7 +559,10 @@ def _obj_material(m, scene, directory, tex_cache):
     if m.get("norm"):
         mat["normals"]["textureID"] = tex(m["norm"], flipped=True, normal_map=True)[0]
     elif m.get("map_bump"):
-        raise ValueError(f"{m['name']}: bump maps (height to normal conversion) are not read; give a normal map (norm)")
+        # a texture of its own per material (CreateTexture, not FindOrCreate: the height scale may differ)
+        t = load_texture(os.path.join(directory, m["map_bump"]), flipped=True)
+        bump_to_normal_map(t, m.get("bump_height", 1.0))
+        mat["normals"]["textureID"] = scene.add_texture(t)
     if m.get("map_Ks"):
         mat["roughness"]["textureID"] = tex(m["map_Ks"], flipped=True)[0]
         mat["roughness"]["value"] = 1.0
