@@ -14,7 +14,10 @@ every arithmetic step, so this decoder follows the same published methods:
   1/4 of the farther sample with alternating rounding biases, for 2x1 and
   2x2 subsampling.
 
-Progressive, lossless, arithmetic-coded and 12-bit images raise ValueError.
+Progressive images (T.81 Annex G: spectral selection and successive
+approximation, DC and AC first and refinement scans with end-of-band runs)
+are decoded into the same coefficient arrays and then follow the same path.
+Lossless, hierarchical, arithmetic-coded and 12-bit images raise ValueError.
 """
 import struct
 
@@ -260,7 +263,7 @@ def decode_jpeg(data):
                 syms = list(seg[i + 17:i + 17 + n])
                 (dc_t if tc == 0 else ac_t)[th] = _Huffman(counts, syms)
                 i += 17 + n
-        elif m in (0xC0, 0xC1):                        # baseline / extended sequential, Huffman
+        elif m in (0xC0, 0xC1, 0xC2):                  # baseline / extended sequential / progressive, Huffman
             prec, h, w, nc = seg[0], *struct.unpack(">HH", seg[1:5]), seg[5]
             if prec != 8:
                 raise ValueError(f"JPEG with {prec}-bit samples: only 8-bit images are read")
@@ -268,15 +271,15 @@ def decode_jpeg(data):
             for k in range(nc):
                 cid, hv, tq = seg[6 + 3 * k], seg[7 + 3 * k], seg[8 + 3 * k]
                 comps.append({"id": cid, "h": hv >> 4, "v": hv & 15, "tq": tq})
-            frame = {"h": h, "w": w, "comps": comps}
+            frame = {"h": h, "w": w, "comps": comps, "progressive": m == 0xC2}
             hmax, vmax = max(c["h"] for c in comps), max(c["v"] for c in comps)
             mcux, mcuy = -(-w // (8 * hmax)), -(-h // (8 * vmax))
             for c in comps:
                 c["bw"], c["bh"] = mcux * c["h"], mcuy * c["v"]
                 c["coef"] = np.zeros((c["bh"], c["bw"], 64), np.int64)
             frame.update(hmax=hmax, vmax=vmax, mcux=mcux, mcuy=mcuy)
-        elif m in (0xC2, 0xC3, 0xC5, 0xC6, 0xC7, 0xC9, 0xCA, 0xCB, 0xCD, 0xCE, 0xCF):
-            raise ValueError("progressive, lossless, hierarchical or arithmetic-coded JPEG images are not read")
+        elif m in (0xC3, 0xC5, 0xC6, 0xC7, 0xC9, 0xCA, 0xCB, 0xCD, 0xCE, 0xCF):
+            raise ValueError("lossless, hierarchical or arithmetic-coded JPEG images are not read")
         elif m == 0xDD:                                # DRI
             restart = struct.unpack(">H", seg[:2])[0]
         elif m == 0xEE and seg[:5] == b"Adobe":
@@ -289,8 +292,14 @@ def decode_jpeg(data):
             for k in range(ns):
                 cid, t = seg[1 + 2 * k], seg[2 + 2 * k]
                 c = next(c for c in frame["comps"] if c["id"] == cid)
-                sc.append((c, dc_t[t >> 4], ac_t[t & 15]))
-            pos = _decode_scan(data, pos + 2 + seg_len, frame, sc, restart)
+                sc.append((c, dc_t.get(t >> 4), ac_t.get(t & 15)))
+            if frame["progressive"]:
+                ss, se, ah, al = seg[1 + 2 * ns], seg[2 + 2 * ns], seg[3 + 2 * ns] >> 4, seg[3 + 2 * ns] & 15
+                if not (ss <= se <= 63) or (ss == 0) != (se == 0) or (ss > 0 and ns != 1):
+                    raise ValueError("corrupt JPEG data: bad progressive scan parameters")
+                pos = _decode_scan_progressive(data, pos + 2 + seg_len, frame, sc, restart, ss, se, ah, al)
+            else:
+                pos = _decode_scan(data, pos + 2 + seg_len, frame, sc, restart)
             continue
         pos += 2 + seg_len
     if frame is None:
@@ -328,25 +337,40 @@ def decode_jpeg(data):
     return rgba
 
 
+def _scan_units(frame, sc):
+    """The scan's units in coding order (T.81 A.2): lists of (scan component,
+    block row, block column). A single-component scan codes the component's
+    own blocks (ceil(component width / 8) per row); an interleaved scan codes
+    MCUs of h x v blocks per component."""
+    if len(sc) == 1:
+        c = sc[0][0]
+        ncols, nrows = -(-frame["w"] * c["h"] // (8 * frame["hmax"])), -(-frame["h"] * c["v"] // (8 * frame["vmax"]))
+        return [[(0, r, q)] for r in range(nrows) for q in range(ncols)]
+    units = []
+    for my in range(frame["mcuy"]):
+        for mx in range(frame["mcux"]):
+            u = []
+            for k, (c, _, _) in enumerate(sc):
+                for by in range(c["v"]):
+                    for bx in range(c["h"]):
+                        u.append((k, my * c["v"] + by, mx * c["h"] + bx))
+            units.append(u)
+    return units
+
+
+def _scan_end(data, p):
+    """Steps over what remains of a scan's entropy-coded data, up to the next marker."""
+    while p + 1 < len(data) and not (data[p] == 0xFF and data[p + 1] not in (0x00,) and not (0xD0 <= data[p + 1] <= 0xD7)):
+        p += 1
+    return p
+
+
 def _decode_scan(data, pos, frame, sc, restart):
     """Huffman-decodes one sequential scan into the components' coefficient
     arrays; returns the position after its entropy-coded data."""
     bits = _Bits(data, pos)
     pred = [0] * len(sc)
-    if len(sc) == 1:                                     # non-interleaved: blocks of the component itself
-        c = sc[0][0]
-        ncols, nrows = -(-frame["w"] * c["h"] // (8 * frame["hmax"])), -(-frame["h"] * c["v"] // (8 * frame["vmax"]))
-        units = [[(0, r, q)] for r in range(nrows) for q in range(ncols)]
-    else:
-        units = []
-        for my in range(frame["mcuy"]):
-            for mx in range(frame["mcux"]):
-                u = []
-                for k, (c, _, _) in enumerate(sc):
-                    for by in range(c["v"]):
-                        for bx in range(c["h"]):
-                            u.append((k, my * c["v"] + by, mx * c["h"] + bx))
-                units.append(u)
+    units = _scan_units(frame, sc)
     zz = _ZZ.tolist()
     for n, u in enumerate(units):
         if restart and n and n % restart == 0:
@@ -372,8 +396,103 @@ def _decode_scan(data, pos, frame, sc, restart):
                     raise ValueError("corrupt JPEG data: coefficient index beyond 63")
                 blk[zz[i]] = _extend(bits.get(s), s)
                 i += 1
-    # step over what remains of the entropy-coded data, up to the next marker
-    p = bits.p
-    while p + 1 < len(data) and not (data[p] == 0xFF and data[p + 1] not in (0x00,) and not (0xD0 <= data[p + 1] <= 0xD7)):
-        p += 1
-    return p
+    return _scan_end(data, bits.p)
+
+
+def _decode_scan_progressive(data, pos, frame, sc, restart, ss, se, ah, al):
+    """One progressive scan (T.81 G.1.2) over coefficients ss..se (zigzag
+    positions) at bit position al; ah = 0 for a first scan, else the bit
+    position of the previous scan of this band (a refinement adds one bit).
+
+    * DC first: the difference code as in a sequential scan, the coefficient
+      becomes the running prediction times 2^al.
+    * DC refinement: one raw bit per block, ORed in at bit al.
+    * AC first: run/size codes as in a sequential scan, values times 2^al;
+      a code with size 0 and run r < 15 starts an end-of-band run of
+      2^r + (r raw bits) blocks (this one included) with no more coefficients.
+    * AC refinement: size is 0 or 1; a new coefficient is +-2^al (one sign
+      bit) and lands on the (run + 1)-th still-zero position; every already
+      nonzero coefficient passed on the way, and every one after the last code
+      of a block in an end-of-band run, reads one correction bit that moves
+      it 2^al away from zero unless that bit is already set.
+    Returns the position after the scan's entropy-coded data."""
+    bits = _Bits(data, pos)
+    units = _scan_units(frame, sc)
+    zz = _ZZ.tolist()
+    pred = [0] * len(sc)
+    eobrun = 0
+    p1, m1 = 1 << al, -1 << al
+    for n, u in enumerate(units):
+        if restart and n and n % restart == 0:
+            bits.restart()
+            pred = [0] * len(sc)
+            eobrun = 0
+        for k, r, q in u:
+            c, dct, act = sc[k]
+            blk = c["coef"][r, q]
+            if ss == 0:                                   # DC scans (possibly interleaved)
+                if ah == 0:
+                    s = bits.symbol(dct)
+                    pred[k] += _extend(bits.get(s), s)
+                    blk[0] = pred[k] * p1
+                elif bits.get(1):
+                    blk[0] |= p1
+                continue
+            if ah == 0:                                   # AC first
+                if eobrun:
+                    eobrun -= 1
+                    continue
+                i = ss
+                while i <= se:
+                    rs = bits.symbol(act)
+                    run, s = rs >> 4, rs & 15
+                    if s:
+                        i += run
+                        if i > se:
+                            raise ValueError("corrupt JPEG data: coefficient index beyond the band")
+                        blk[zz[i]] = _extend(bits.get(s), s) * p1
+                        i += 1
+                    elif run == 15:
+                        i += 16
+                    else:
+                        eobrun = (1 << run) - 1 + bits.get(run)
+                        break
+                continue
+            i = ss                                        # AC refinement
+            if eobrun == 0:
+                while i <= se:
+                    rs = bits.symbol(act)
+                    run, s = rs >> 4, rs & 15
+                    val = 0
+                    if s:
+                        if s != 1:
+                            raise ValueError("corrupt JPEG data: refinement coefficient of size above 1")
+                        val = p1 if bits.get(1) else m1
+                    elif run != 15:
+                        eobrun = (1 << run) + bits.get(run)
+                        break
+                    while i <= se:                        # correction bits up to the target zero position
+                        z = zz[i]
+                        cf = blk[z]
+                        if cf:
+                            if bits.get(1) and not (cf & p1):
+                                blk[z] = cf + p1 if cf >= 0 else cf + m1
+                        else:
+                            if run == 0:
+                                break
+                            run -= 1
+                        i += 1
+                    if val:
+                        if i > se:
+                            raise ValueError("corrupt JPEG data: coefficient index beyond the band")
+                        blk[zz[i]] = val
+                    i += 1
+            if eobrun:
+                while i <= se:                            # correction bits for the rest of the band
+                    z = zz[i]
+                    cf = blk[z]
+                    if cf and bits.get(1) and not (cf & p1):
+                        blk[z] = cf + p1 if cf >= 0 else cf + m1
+                    i += 1
+                eobrun -= 1
+    return _scan_end(data, bits.p)

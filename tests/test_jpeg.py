@@ -10,8 +10,10 @@ Pinned by closed forms that do not depend on any image library:
 * the triangle upsampling keeps constants and interpolates ramps;
 then cross-checked bit for bit against Pillow's libjpeg decoder (the library
 FreeImage's JPEG plugin is built on) for 4:4:4, 4:2:2, 4:2:0 and grey images,
-with and without restart intervals, and on the reference's own JPEG textures
-when the reference tree is present.
+sequential and progressive (libjpeg's default progressive script: spectral
+selection and successive approximation, DC and AC refinement scans), with and
+without restart intervals, and on the reference's own JPEG textures when the
+reference tree is present (its one progressive file included).
 """
 import glob
 import io
@@ -84,10 +86,10 @@ def test_upsampling_keeps_constants_and_interpolates():
 def test_not_a_jpeg_and_unsupported_modes():
     with pytest.raises(ValueError):
         J.decode_jpeg(b"\x89PNG\r\n\x1a\n")
-    # a progressive frame header (SOF2) right after SOI
-    sof2 = b"\xff\xd8\xff\xc2\x00\x11\x08\x00\x08\x00\x08\x03\x01\x11\x00\x02\x11\x00\x03\x11\x00"
-    with pytest.raises(ValueError, match="progressive"):
-        J.decode_jpeg(sof2)
+    # a lossless frame header (SOF3) right after SOI
+    sof3 = b"\xff\xd8\xff\xc3\x00\x11\x08\x00\x08\x00\x08\x03\x01\x11\x00\x02\x11\x00\x03\x11\x00"
+    with pytest.raises(ValueError, match="lossless"):
+        J.decode_jpeg(sof3)
 
 
 def _pil():
@@ -121,6 +123,32 @@ def test_equals_libjpeg(subsampling, quality, restart):
     assert np.array_equal(got[..., :3], _pil_rgb(Image, b))
 
 
+@pytest.mark.parametrize("subsampling", [0, 1, 2])        # 4:4:4, 4:2:2, 4:2:0
+@pytest.mark.parametrize("quality", [30, 92])
+@pytest.mark.parametrize("restart", [None, 2])
+def test_progressive_equals_libjpeg(subsampling, quality, restart):
+    Image = _pil()
+    buf = io.BytesIO()
+    kw = {"restart_marker_blocks": restart} if restart else {}
+    Image.fromarray(_test_image(61, 83, seed=4)).save(buf, "JPEG", quality=quality, subsampling=subsampling,
+                                                     progressive=True, **kw)
+    b = buf.getvalue()
+    assert b.find(b"\xff\xc2") >= 0 and b.count(b"\xff\xda") > 3      # a progressive frame, several scans
+    if restart:
+        assert b.find(b"\xff\xdd") >= 0
+    got = J.decode_jpeg(b)
+    assert np.array_equal(got[..., :3], _pil_rgb(Image, b))
+
+
+def test_progressive_grey_equals_libjpeg():
+    Image = _pil()
+    buf = io.BytesIO()
+    Image.fromarray(_test_image(40, 70, seed=6)[..., 0]).save(buf, "JPEG", quality=75, progressive=True)
+    got = J.decode_jpeg(buf.getvalue())
+    ref = np.asarray(Image.open(io.BytesIO(buf.getvalue())).convert("L"))
+    assert all(np.array_equal(got[..., k], ref) for k in range(3))
+
+
 def test_grey_equals_libjpeg():
     Image = _pil()
     buf = io.BytesIO()
@@ -144,20 +172,17 @@ def test_texture_loader_reads_jpeg(tmp_path):
 
 @pytest.mark.skipif(not os.path.isdir(REF_DATA), reason="reference tree not present")
 def test_reference_textures_equal_libjpeg():
-    """The reference's scene textures (book, escher, knight): baseline JPEGs
-    decode as libjpeg decodes them; its one progressive file raises."""
+    """The reference's scene textures (book, escher, knight, the material-test
+    scene's progressive brick normal map) decode as libjpeg decodes them."""
     Image = _pil()
     files = sorted(glob.glob(os.path.join(REF_DATA, "**", "*.jp*g"), recursive=True))
     assert files
-    done = 0
+    done = progressive = 0
     for f in files:
         b = open(f, "rb").read()
-        if b"\xff\xc2" in b[:2000]:
-            with pytest.raises(ValueError, match="progressive"):
-                J.decode_jpeg(b)
-            continue
+        progressive += b"\xff\xc2" in b[:4000]
         if len(b) > 400_000:
             continue                      # the 2048^2 receiver textures: same code path, seconds each
         assert np.array_equal(J.decode_jpeg(b)[..., :3], _pil_rgb(Image, b)), f
         done += 1
-    assert done >= 10
+    assert done >= 11 and progressive >= 1
