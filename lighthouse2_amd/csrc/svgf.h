@@ -23,7 +23,9 @@ public:
 	 * PackNormal2 normal, depth bits, specular << 4 | matid << 6} (the history
 	 * count in bits 0..3 of w is kept by the filter); worldPos: device float4
 	 * {position, packed normal | specular bits}; deltaDepth: device float4
-	 * (z, w = depth derivatives). out: device float4 [w * h], the display image. */
+	 * (z, w = depth derivatives). worldPos and deltaDepth are read in place by
+	 * the frame's kernels (stream-ordered on s), not copied. out: device float4
+	 * [w * h], the display image. */
 	/* Clears every buffer, history included: the next Frame is a first frame. */
 	void Reset(hipStream_t s);
 	void Frame(const float4* accumulator, const uint4* features, const float4* worldPos, const float4* deltaDepth,
@@ -33,7 +35,7 @@ public:
 	 * image (finalize_shared.h:490-575); otherwise finalizeNoTAA. The TAA
 	 * history (prevPixels) is kept only while taa is on. */
 	int taa = 0;
-	DevBuf features, worldPos, prevWorldPos, deltaDepth, motion, moments, prevMoments, shading, filteredIn, filteredOut, lastPass;
+	DevBuf features, worldPos, prevWorldPos, motion, moments, prevMoments, shading, filteredIn, filteredOut, lastPass;
 	DevBuf prevPixels, taaOut;
 };
 

@@ -175,9 +175,11 @@ __global__ void k_merge_features(const uint4* __restrict__ in, uint4* __restrict
 }
 
 /* prepareFilterKernel (finalize_shared.h:226-294) */
+/* Also keeps this frame's world positions (the next frame's history) in
+ * worldPosOut, so the G-buffer needs no separate copy. */
 __global__ void k_prepare(const float4* __restrict__ acc, uint4* __restrict__ features, const float4* __restrict__ worldPos,
-	const float4* __restrict__ prevWorldPos, float4* __restrict__ shading, float2* __restrict__ motion, float4* __restrict__ moments,
-	const float4* __restrict__ prevMoments, const float4* __restrict__ deltaDepth, SvgfCam cam, float j0, float j1, float pj0, float pj1,
+	float4* __restrict__ worldPosOut, const float4* __restrict__ prevWorldPos, float4* __restrict__ shading, float2* __restrict__ motion,
+	float4* __restrict__ moments, const float4* __restrict__ prevMoments, SvgfCam cam, float j0, float j1, float pj0, float pj1,
 	int w, int h, float pvs, float clampD, float clampI, int stationary)
 {
 	const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -187,6 +189,7 @@ __global__ void k_prepare(const float4* __restrict__ acc, uint4* __restrict__ fe
 	const v3 direct = mk3(a0.x, a0.y, a0.z) * pvs;
 	const uint4 lf = features[p];
 	const float4 lwp = worldPos[p];
+	worldPosOut[p] = lwp;
 	const v3 albedo = rgb32_to_hdr_min1(lf.x);
 	const v3 indirect = mk3(a1.x, a1.y, a1.z) * pvs;
 	const v3 reci = mk3(1.0f / albedo.x, 1.0f / albedo.y, 1.0f / albedo.z);
@@ -492,17 +495,17 @@ Svgf::Svgf(int w_, int h_) : w(w_), h(h_)
 {
 	if (w < 3 || h < 3 || (int64_t)w * h > (1 << 26)) throw CoreError(LH2HIP_ERR_INVALID, "svgf: size 3x3 .. 2^26 pixels");
 	const size_t n = (size_t)w * h;
-	features.ensure(n * 16), worldPos.ensure(n * 16), prevWorldPos.ensure(n * 16), deltaDepth.ensure(n * 16);
+	features.ensure(n * 16), worldPos.ensure(n * 16), prevWorldPos.ensure(n * 16);
 	motion.ensure(n * 8), moments.ensure(n * 16), prevMoments.ensure(n * 16), shading.ensure(n * 16);
 	filteredIn.ensure(n * 16), filteredOut.ensure(n * 16), lastPass.ensure(n * 16), prevPixels.ensure(n * 16), taaOut.ensure(n * 16);
-	for (DevBuf* b : { &features, &worldPos, &prevWorldPos, &deltaDepth, &motion, &moments, &prevMoments, &shading, &filteredIn, &filteredOut, &lastPass,
+	for (DevBuf* b : { &features, &worldPos, &prevWorldPos, &motion, &moments, &prevMoments, &shading, &filteredIn, &filteredOut, &lastPass,
 		&prevPixels, &taaOut })
 		chk(hipMemset(b->p, 0, b->bytes), "memset");
 }
 
 void Svgf::Reset(hipStream_t s)
 {
-	for (DevBuf* b : { &features, &worldPos, &prevWorldPos, &deltaDepth, &motion, &moments, &prevMoments, &shading, &filteredIn, &filteredOut, &lastPass,
+	for (DevBuf* b : { &features, &worldPos, &prevWorldPos, &motion, &moments, &prevMoments, &shading, &filteredIn, &filteredOut, &lastPass,
 		&prevPixels, &taaOut })
 		chk(hipMemsetAsync(b->p, 0, b->bytes, s), "memset");
 }
@@ -513,14 +516,15 @@ void Svgf::Frame(const float4* acc, const uint4* feat, const float4* wp, const f
 	if (samplesTaken <= 0) throw CoreError(LH2HIP_ERR_INVALID, "svgf frame: samplesTaken >= 1");
 	const int n = w * h;
 	k_merge_features<<<(n + 255) / 256, 256, 0, s>>>(feat, features.as<uint4>(), n);
-	chk(hipMemcpyAsync(worldPos.p, wp, (size_t)n * 16, hipMemcpyDeviceToDevice, s), "copy");
-	chk(hipMemcpyAsync(deltaDepth.p, dd, (size_t)n * 16, hipMemcpyDeviceToDevice, s), "copy");
 	const dim3 b(8, 8), g((w + 7) / 8, (h + 7) / 8);
 	const SvgfCam cam = svgf_camera(prevView, h);
-	k_prepare<<<g, b, 0, s>>>(acc, features.as<uint4>(), worldPos.as<float4>(), prevWorldPos.as<float4>(), shading.as<float4>(), motion.as<float2>(),
-		moments.as<float4>(), prevMoments.as<float4>(), deltaDepth.as<float4>(), cam, j0, j1, pj0, pj1, w, h, 1.0f / (float)samplesTaken, clampD, clampI, stationary);
+	/* wp and dd are read in place (stream order keeps them valid until the
+	 * last pass); k_prepare keeps wp as the next frame's history */
+	k_prepare<<<g, b, 0, s>>>(acc, features.as<uint4>(), wp, worldPos.as<float4>(), prevWorldPos.as<float4>(), shading.as<float4>(),
+		motion.as<float2>(), moments.as<float4>(), prevMoments.as<float4>(), cam, j0, j1, pj0, pj1, w, h, 1.0f / (float)samplesTaken, clampD,
+		clampI, stationary);
 	const auto apply = [&](int phase, DevBuf& A, const float4* B, DevBuf& C, int last) {
-		k_apply<<<g, b, 0, s>>>(features.as<uint4>(), prevWorldPos.as<float4>(), worldPos.as<float4>(), deltaDepth.as<float4>(), motion.as<float2>(),
+		k_apply<<<g, b, 0, s>>>(features.as<uint4>(), prevWorldPos.as<float4>(), wp, dd, motion.as<float2>(),
 			moments.as<float4>(), A.as<float4>(), B, C.as<float4>(), w, h, phase, last);
 	};
 	apply(1, shading, filteredIn.as<float4>(), filteredOut, 0);
