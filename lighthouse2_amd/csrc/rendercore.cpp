@@ -105,7 +105,11 @@ void Wavefront::resizeTarget(int pixels, hipStream_t s)
 	acc32B.ensure(sizeof(float4) * 2 * pixelCap);
 	acc64B.ensure(sizeof(double) * 8 * pixelCap);
 	gFeaturesB.ensure(16 * (size_t)pixelCap), gWorldPosB.ensure(16 * (size_t)pixelCap), gDeltaDepthB.ensure(16 * (size_t)pixelCap);
+	/* the shade kernel writes the G-buffer only where a path reaches a diffuse
+	 * surface; the filter still reads the other pixels, so they start at 0 */
 	hipCheck(hipMemsetAsync(gFeaturesB.p, 0, gFeaturesB.bytes, s), "memset");
+	hipCheck(hipMemsetAsync(gWorldPosB.p, 0, gWorldPosB.bytes, s), "memset");
+	hipCheck(hipMemsetAsync(gDeltaDepthB.p, 0, gDeltaDepthB.bytes, s), "memset");
 	imageB.ensure(sizeof(float4) * pixelCap);
 	probeB.ensure(16);
 	countLogB.ensure(sizeof(uint32_t) * 1100);

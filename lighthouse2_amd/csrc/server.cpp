@@ -165,7 +165,10 @@ void RenderServer::startJob(Job& j)
 	{	/* SVGF writes the interior pixels only: the border stays 0, as in a fresh render target */
 		check(hipMemsetAsync(j.image.p, 0, j.image.bytes, stream), "memset");
 		j.gFeatures.ensure((size_t)pixels * 16), j.gWorldPos.ensure((size_t)pixels * 16), j.gDeltaDepth.ensure((size_t)pixels * 16);
+		/* pixels whose path reaches no diffuse surface keep these zeros, as in a fresh render target */
 		check(hipMemsetAsync(j.gFeatures.p, 0, j.gFeatures.bytes, stream), "memset");
+		check(hipMemsetAsync(j.gWorldPos.p, 0, j.gWorldPos.bytes, stream), "memset");
+		check(hipMemsetAsync(j.gDeltaDepth.p, 0, j.gDeltaDepth.bytes, stream), "memset");
 		j.frame.P.gFeatures = j.gFeatures.as<uint4>(), j.frame.P.gWorldPos = j.gWorldPos.as<float4>();
 		j.frame.P.gDeltaDepth = j.gDeltaDepth.as<float4>();
 	}
