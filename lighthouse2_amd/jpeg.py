@@ -297,6 +297,12 @@ def decode_jpeg(data):
                 ss, se, ah, al = seg[1 + 2 * ns], seg[2 + 2 * ns], seg[3 + 2 * ns] >> 4, seg[3 + 2 * ns] & 15
                 if not (ss <= se <= 63) or (ss == 0) != (se == 0) or (ss > 0 and ns != 1):
                     raise ValueError("corrupt JPEG data: bad progressive scan parameters")
+                need_dc, need_ac = ss == 0 and ah == 0, ss > 0
+            else:
+                need_dc = need_ac = True
+            if any((need_dc and d is None) or (need_ac and a is None) for _, d, a in sc):
+                raise ValueError("corrupt JPEG data: a scan uses an undefined Huffman table")
+            if frame["progressive"]:
                 pos = _decode_scan_progressive(data, pos + 2 + seg_len, frame, sc, restart, ss, se, ah, al)
             else:
                 pos = _decode_scan(data, pos + 2 + seg_len, frame, sc, restart)

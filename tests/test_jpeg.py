@@ -123,6 +123,17 @@ def test_equals_libjpeg(subsampling, quality, restart):
     assert np.array_equal(got[..., :3], _pil_rgb(Image, b))
 
 
+def test_scan_with_undefined_huffman_table_raises():
+    Image = _pil()
+    buf = io.BytesIO()
+    Image.fromarray(_test_image(16, 16)).save(buf, "JPEG", quality=80)
+    b = bytearray(buf.getvalue())
+    sos = b.find(b"\xff\xda")
+    b[sos + 6] = 0x33                                   # first component: DC and AC table 3, never defined
+    with pytest.raises(ValueError, match="Huffman table"):
+        J.decode_jpeg(bytes(b))
+
+
 @pytest.mark.parametrize("subsampling", [0, 1, 2])        # 4:4:4, 4:2:2, 4:2:0
 @pytest.mark.parametrize("quality", [30, 92])
 @pytest.mark.parametrize("restart", [None, 2])
