@@ -217,26 +217,34 @@ __global__ void __launch_bounds__(64 * ROWS) k_conv3x3_t(const bf16_t* __restric
 		const int k = ks * 32 + kg * 8;
 		return k < WCIN ? *(const bf8v*)(w + ((size_t)(tap * Cout + nb * 16 + ml) * WCIN + k)) : (bf8v){ 0, 0, 0, 0, 0, 0, 0, 0 };
 	};
-	bf8v wc[KS][NB], wn[KS][NB];
+	/* bias now, beside the tile's loads, so that the epilogue waits on nothing;
+	 * weight fragments two taps ahead of their products */
+	float bv[NB][4];
+#pragma unroll
+	for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+		for (int i = 0; i < 4; i++) bv[nb][i] = bias ? bias[nb * 16 + kg * 4 + i] : 0.0f;
+	/* weight fragments PD taps ahead of their products: two where the third
+	 * set of registers costs no occupancy (64-pixel tiles, 32 input channels,
+	 * up to 32 outputs), else one with the tap loop kept rolled */
+	constexpr int PD = MBW > 1 && (CIN == 32 || CAT) && NB <= 2 ? 2 : 1;
+	bf8v wc[KS][NB], w1[KS][NB], w2[KS][NB];
 #pragma unroll
 	for (int ks = 0; ks < KS; ks++)
 #pragma unroll
-		for (int nb = 0; nb < NB; nb++) wc[ks][nb] = wfrag(0, ks, nb);
+		for (int nb = 0; nb < NB; nb++)
+		{
+			wc[ks][nb] = wfrag(0, ks, nb);
+			if (PD == 2) w1[ks][nb] = wfrag(1, ks, nb);
+		}
 	f4v acc[MBW][NB];
 #pragma unroll
 	for (int mb = 0; mb < MBW; mb++)
 #pragma unroll
 		for (int nb = 0; nb < NB; nb++) acc[mb][nb] = (f4v){ 0.0f, 0.0f, 0.0f, 0.0f };
 	__syncthreads();
-#pragma unroll 1
-	for (int tap = 0; tap < 9; tap++)
-	{
+	const auto products = [&](int tap) {
 		const int dy = tap / 3, dx = tap % 3;
-		if (tap < 8)
-#pragma unroll
-			for (int ks = 0; ks < KS; ks++)
-#pragma unroll
-				for (int nb = 0; nb < NB; nb++) wn[ks][nb] = wfrag(tap + 1, ks, nb);
 #pragma unroll
 		for (int ks = 0; ks < KS; ks++)
 #pragma unroll
@@ -249,10 +257,40 @@ __global__ void __launch_bounds__(64 * ROWS) k_conv3x3_t(const bf16_t* __restric
 #pragma unroll
 				for (int nb = 0; nb < NB; nb++) acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wc[ks][nb], a, acc[mb][nb], 0, 0, 0);
 			}
+	};
+	if constexpr (PD == 2)
+	{
 #pragma unroll
-		for (int ks = 0; ks < KS; ks++)
+		for (int tap = 0; tap < 9; tap++)
+		{
+			if (tap < 7)
 #pragma unroll
-			for (int nb = 0; nb < NB; nb++) wc[ks][nb] = wn[ks][nb];
+				for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+					for (int nb = 0; nb < NB; nb++) w2[ks][nb] = wfrag(tap + 2, ks, nb);
+			products(tap);
+#pragma unroll
+			for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++) wc[ks][nb] = w1[ks][nb], w1[ks][nb] = w2[ks][nb];
+		}
+	}
+	else
+	{
+#pragma unroll 1
+		for (int tap = 0; tap < 9; tap++)
+		{
+			if (tap < 8)
+#pragma unroll
+				for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+					for (int nb = 0; nb < NB; nb++) w1[ks][nb] = wfrag(tap + 1, ks, nb);
+			products(tap);
+#pragma unroll
+			for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+				for (int nb = 0; nb < NB; nb++) wc[ks][nb] = w1[ks][nb];
+		}
 	}
 	const int ry = r0 + wave;
 	float hs[POOL ? MBW : 1][NB][4];                     /* POOL: sums of horizontal pixel pairs */
@@ -267,9 +305,6 @@ __global__ void __launch_bounds__(64 * ROWS) k_conv3x3_t(const bf16_t* __restric
 	for (int nb = 0; nb < NB; nb++)
 	{
 		const int ch = nb * 16 + kg * 4;
-		float bv[4];
-#pragma unroll
-		for (int i = 0; i < 4; i++) bv[i] = bias ? bias[ch + i] : 0.0f;
 #pragma unroll
 		for (int mb = 0; mb < MBW; mb++)
 		{
@@ -278,7 +313,7 @@ __global__ void __launch_bounds__(64 * ROWS) k_conv3x3_t(const bf16_t* __restric
 #pragma unroll
 			for (int i = 0; i < 4; i++)
 			{
-				v[i] = acc[mb][nb][i] + bv[i];
+				v[i] = acc[mb][nb][i] + bv[nb][i];
 				if (RELU) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
 			}
 			const size_t o = ((size_t)ry * W + px) * Cout + ch;

@@ -307,3 +307,47 @@ def test_elementwise_kernels_vector_and_scalar_paths_agree(torch_cuda):
     x, g = rng_state["x"], rng_state["g"]
     assert np.array_equal(D.bf16_to_f32(vec["relu"]).reshape(H, W, C), np.where(x > 0, g, 0).astype(np.float32))
     assert np.array_equal(D.bf16_to_f32(vec["upcat_bwd_skip"]).reshape(H, W, C), x)
+
+
+@pytest.mark.parametrize("cin,cout", [(16, 32), (32, 32), (32, 16)])
+def test_large_tiles_equal_fp64_and_small_tiles(torch_cuda, cin, cout):
+    """1930 x 202 (ragged in both directions) runs the 8 x 64 tiles with the
+    weights fetched two taps ahead (32 input channels). Checked against FP64
+    dot products on sampled pixels, and bit for bit against crops of the same
+    input run through the 4 x 16 tiles (same tap and channel order of the
+    products, so the same fp32 sums)."""
+    torch = torch_cuda
+    L = lh.library()
+    rng = np.random.default_rng(cin + 3 * cout)
+    H, W = 202, 1930
+    x = R.bf16(rng.standard_normal((H, W, cin)))
+    w = R.bf16(rng.standard_normal((9, cout, cin)) * 0.2)
+    b = rng.standard_normal(cout).astype(np.float32)
+    sh = torch.cuda.current_stream().cuda_stream
+    dw_, db_ = dev_bf16(torch, w), torch.from_numpy(b).cuda()
+
+    def run(xa, flags):
+        h, wd = xa.shape[:2]
+        xd = dev_bf16(torch, xa)
+        y = torch.zeros(h * wd * cout, dtype=torch.float32 if flags & 2 else torch.int16, device="cuda")
+        assert L.lh2hip_dn_conv3x3(xd.data_ptr(), cin, dw_.data_ptr(), db_.data_ptr(), y.data_ptr(), cout, h, wd, flags, sh) == 0
+        torch.cuda.synchronize()
+        y = y.cpu().numpy()
+        return (y if flags & 2 else D.bf16_to_f32(y)).reshape(h, wd, cout)
+
+    full = run(x, 2)                                     # fp32 out, no ReLU
+    py, px = rng.integers(0, H, 3000), rng.integers(0, W, 3000)
+    xp = np.zeros((H + 2, W + 2, cin))
+    xp[1:-1, 1:-1] = x
+    ref = np.zeros((3000, cout)) + b
+    for t in range(9):
+        ref += xp[py + t // 3, px + t % 3] @ w[t].T
+    assert np.max(np.abs(full[py, px] - ref)) <= 1e-5 * np.max(np.abs(ref))
+    fullb = run(x, 1)                                    # bf16 out with ReLU
+    for (r0, r1, c0, c1) in ((0, 40, 0, 60), (150, 202, 1868, 1930), (91, 130, 700, 760)):
+        a0, a1, b0, b1 = max(r0 - 1, 0), min(r1 + 1, H), max(c0 - 1, 0), min(c1 + 1, W)
+        crop = x[a0:a1, b0:b1]
+        for flags, whole in ((2, full), (1, fullb)):
+            part = run(crop, flags)
+            assert np.array_equal(part[r0 - a0:r1 - a0, c0 - b0:c1 - b0].view(np.uint32),
+                                  np.ascontiguousarray(whole[r0:r1, c0:c1]).view(np.uint32))
