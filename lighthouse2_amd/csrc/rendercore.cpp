@@ -111,10 +111,18 @@ void Wavefront::resizeTarget(int pixels, hipStream_t s)
 	hipCheck(hipMemsetAsync(gWorldPosB.p, 0, gWorldPosB.bytes, s), "memset");
 	hipCheck(hipMemsetAsync(gDeltaDepthB.p, 0, gDeltaDepthB.bytes, s), "memset");
 	imageB.ensure(sizeof(float4) * pixelCap);
-	probeB.ensure(16);
-	countLogB.ensure(sizeof(uint32_t) * 1100);
+	ensureHost();
 	hipCheck(hipMemsetAsync(imageB.p, 0, imageB.bytes, s), "memset");
-	hipCheck(hipMemsetAsync(probeB.p, 0xff, 16, s), "memset");
+}
+
+void Wavefront::ensureHost()
+{
+	if (hostW) return;
+	void* p = nullptr;
+	hipCheck(hipHostMalloc(&p, sizeof(uint32_t) * HOST_WORDS, hipHostMallocDefault), "hipHostMalloc");
+	hostW = (uint32_t*)p;
+	memset(hostW, 0, sizeof(uint32_t) * HOST_WORDS);
+	memset(hostW + 4, 0xff, 3 * sizeof(uint32_t));   /* no probe hit yet */
 }
 
 void Wavefront::clearAccumulators(hipStream_t s)
@@ -131,7 +139,7 @@ void Wavefront::resizePaths(size_t n, hipStream_t s)
 		rays[i].ensure(32 * n), state[i].ensure(16 * n), pathIdB[i].ensure(4 * n), auxB[i].ensure(8 * n);
 	hitsB.ensure(16 * n), shadowRaysB.ensure(32 * n), shadowPotB.ensure(16 * n), occludedB.ensure(4 * n), pathAccB.ensure(32 * n);
 	countersB.ensure(64);
-	countLogB.ensure(sizeof(uint32_t) * 1100);   /* one slot per wave, maxDepth <= 1024 */
+	ensureHost();                                 /* one log slot per wave, maxDepth <= 1024 */
 	hipCheck(hipMemsetAsync(countersB.p, 0, 64, s), "memset");
 }
 
@@ -139,7 +147,8 @@ void Wavefront::release()
 {
 	for (int i = 0; i < 2; i++) rays[i].release(), state[i].release(), pathIdB[i].release(), auxB[i].release();
 	hitsB.release(), shadowRaysB.release(), shadowPotB.release(), occludedB.release(), pathAccB.release();
-	countersB.release(), countLogB.release(), probeB.release(), acc32B.release(), acc64B.release(), imageB.release();
+	countersB.release(), acc32B.release(), acc64B.release(), imageB.release();
+	if (hostW) (void)hipHostFree(hostW), hostW = nullptr;
 	gFeaturesB.release(), gWorldPosB.release(), gDeltaDepthB.release();
 	pathCap = 0, pixelCap = 0;
 }
@@ -1054,10 +1063,13 @@ int RenderCore::runPass(const PassPart* parts, int np, Wavefront& w, uint32_t* q
 		if (evs) hipCheck(hipEventRecord(evs[2], st), "event");
 		/* the reference estimator connects from every vertex but the last;
 		 * optix7's from every vertex (its NEE precedes the depth cut) */
+		uint32_t* const logSlot = w.countLog() + pathLength;
+		bool advanced = false;
 		if ((mis || pathLength < maxDepthAll) && side)
-		{	/* snapshot the counters (advance clears the shadow count), then
-			 * trace and connect on the side stream */
-			hipCheck(hipMemcpyAsync(side->counters, B.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), "D2D");
+		{	/* the advance (which clears the shadow count) snapshots the counters
+			 * for the side stream, which then traces and connects */
+			hipCheck(launch_advance(B.counters, logSlot, st, side->counters, w.hostLive()), "advance");
+			advanced = true;
 			hipCheck(hipEventRecord(side->shaded, st), "event");
 			hipCheck(hipStreamWaitEvent(side->st, side->shaded, 0), "wait");
 			const bool timed = side->timing && sideTimed < SIDE_TIMED;
@@ -1077,16 +1089,15 @@ int RenderCore::runPass(const PassPart* parts, int np, Wavefront& w, uint32_t* q
 			hipCheck(launch_connect(B, (int)n, st, mis ? 1.0f : 0.0f), "connect");
 		}
 		if (evs) hipCheck(hipEventRecord(evs[3], st), "event");
-		hipCheck(launch_advance(B.counters, w.countLog() ? w.countLog() + pathLength : nullptr, st), "advance");
+		if (!advanced) hipCheck(launch_advance(B.counters, logSlot, st, nullptr, w.hostLive()), "advance");
 		in ^= 1;
 		deepest = std::max(deepest, pathLength);
 		/* stop early once every path has terminated (checked every 4 waves;
 		 * after every wave for optix7, whose loop reads the count back each time) */
 		if (mis || (pathLength & 3) == 0 || pathLength == maxDepthAll)
 		{
-			uint32_t live = 0;
-			hipCheck(hipMemcpyAsync(&live, B.counters + 0, 4, hipMemcpyDeviceToHost, st), "D2H");
 			hipCheck(hipStreamSynchronize(st), "sync");
+			const uint32_t live = *(volatile uint32_t*)w.hostLive();   /* written by the advance */
 			if (evs && timing)
 			{
 				float a1, b2, c;
@@ -1174,7 +1185,7 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	{
 		hipCheck(hipStreamSynchronize(stream), "sync");
 		int probe[3];
-		hipCheck(hipMemcpy(probe, wf.probe(), 12, hipMemcpyDeviceToHost), "probe");
+		memcpy(probe, (const void*)wf.probe(), 12);   /* pinned: written by the shade kernel */
 		stats.probedInstid = probe[0], stats.probedTriid = probe[1];
 		memcpy(&stats.probedDist, &probe[2], 4);
 		float ms = 0;
@@ -1182,7 +1193,7 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 		/* wave sizes of the (last) pass: the advance after wave k logged the size of wave k + 1 */
 		uint32_t logged[MAXPATHLENGTH_FILTER + 2] = {};
 		const int nlog = std::min(waves, MAXPATHLENGTH_FILTER + 1);
-		if (nlog > 1) hipCheck(hipMemcpy(logged, wf.countLog(), sizeof(uint32_t) * (nlog + 1), hipMemcpyDeviceToHost), "D2H");
+		if (nlog > 1) memcpy(logged, (const void*)wf.countLog(), sizeof(uint32_t) * (nlog + 1));   /* pinned: written by the advances */
 		stats.bounce1RayCount = nlog > 1 ? logged[1] : 0;
 		stats.deepRayCount = 0;
 		for (int k = 2; k < nlog; k++) stats.deepRayCount += logged[k];

@@ -48,8 +48,15 @@ struct DevBuf
 struct Wavefront
 {
 	static constexpr size_t PASS_CAP = (size_t)1 << 24;
-	DevBuf rays[2], state[2], pathIdB[2], auxB[2], hitsB, shadowRaysB, shadowPotB, occludedB, pathAccB, countersB, countLogB, probeB;
+	DevBuf rays[2], state[2], pathIdB[2], auxB[2], hitsB, shadowRaysB, shadowPotB, occludedB, pathAccB, countersB;
 	DevBuf acc32B, acc64B, imageB;
+	/* Pinned host words the kernels write directly, read by the host after a
+	 * stream synchronisation (no copy on the stream): [0] live path count after
+	 * the last advance, [4..6] probe hit (instance, primitive, distance bits),
+	 * [HOST_LOG..] paths per wave. */
+	static constexpr int HOST_LOG = 16, HOST_WORDS = HOST_LOG + 1100;
+	uint32_t* hostW = nullptr;
+	void ensureHost();
 	DevBuf gFeaturesB, gWorldPosB, gDeltaDepthB;   /* SVGF G-buffer of the filter estimator */
 	size_t pathCap = 0;
 	int pixelCap = 0;
@@ -58,11 +65,12 @@ struct Wavefront
 	void resizePaths(size_t n, hipStream_t s);
 	void release();
 	PathBuffers buffers();
-	int* probe() { return probeB.as<int>(); }
+	int* probe() { return (int*)(hostW + 4); }
+	uint32_t* hostLive() { return hostW; }
 	float4* acc32() { return acc32B.as<float4>(); }
 	double* acc64() { return acc64B.as<double>(); }
 	float4* image() { return imageB.as<float4>(); }
-	uint32_t* countLog() { return countLogB.as<uint32_t>(); }
+	uint32_t* countLog() { return hostW + HOST_LOG; }
 	uint4* gFeatures() { return gFeaturesB.as<uint4>(); }
 	float4* gWorldPos() { return gWorldPosB.as<float4>(); }
 	float4* gDeltaDepth() { return gDeltaDepthB.as<float4>(); }

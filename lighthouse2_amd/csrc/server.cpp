@@ -280,8 +280,8 @@ void RenderServer::loop()
 				core.shadowLaneOn() ? &lane : nullptr, segs.as<JobSeg>(), segsHost);
 			/* wave sizes: the advance after wave k logged the size of wave k + 1 */
 			std::vector<uint32_t> logged((size_t)waves + 1, 0u);
-			check(hipMemcpyAsync(logged.data(), wf.countLog(), sizeof(uint32_t) * logged.size(), hipMemcpyDeviceToHost, stream), "D2H");
 			check(hipStreamSynchronize(stream), "sync");
+			memcpy(logged.data(), (const void*)wf.countLog(), sizeof(uint32_t) * logged.size());   /* pinned, written by the advances */
 			uint64_t paths = 0, rays = 0;
 			for (int k = 1; k < waves; k++) rays += logged[k];
 			for (size_t i = 0; i < batch.size(); i++)

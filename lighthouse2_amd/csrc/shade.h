@@ -147,8 +147,10 @@ hipError_t launch_generateF(const CamParams& c, const ShadeParams& p, uint32_t j
 hipError_t launch_shadeF(const ShadeParams& p, const PathBuffers& b, int in, int maxCount, hipStream_t s);
 /* MAXPATHLENGTH of RenderCore_Optix7Filter (core_settings.h:25) */
 static const int MAXPATHLENGTH_FILTER = 7;
-/* counters[0] = counters[1] (next wave size, also logged), counters[1] = counters[2] = 0. */
-hipError_t launch_advance(uint32_t* counters, uint32_t* logSlot, hipStream_t s);
+/* counters[0] = counters[1] (next wave size, also logged), counters[1] = counters[2] = 0.
+ * snapshot (if given) receives counters[0..3] as they were before; live (if
+ * given, may be pinned host memory) the next wave size. */
+hipError_t launch_advance(uint32_t* counters, uint32_t* logSlot, hipStream_t s, uint32_t* snapshot = nullptr, uint32_t* live = nullptr);
 hipError_t launch_copy_u32(uint32_t* dst, const uint32_t* src, hipStream_t s);
 
 } // namespace lh

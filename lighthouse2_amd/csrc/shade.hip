@@ -639,16 +639,18 @@ __global__ void k_set_u32(uint32_t* p, int n, uint32_t v)
 
 __global__ void k_copy_u32(uint32_t* dst, const uint32_t* src) { *dst = *src; }
 
-__global__ void k_advance(uint32_t* c, uint32_t* logSlot)
+__global__ void k_advance(uint32_t* c, uint32_t* logSlot, uint32_t* snapshot, uint32_t* live)
 {
 	const uint32_t next = c[1];
+	if (snapshot) snapshot[0] = c[0], snapshot[1] = c[1], snapshot[2] = c[2], snapshot[3] = c[3];
 	c[0] = next, c[1] = 0u, c[2] = 0u;
 	if (logSlot) *logSlot = next;
+	if (live) *live = next;
 }
 
-hipError_t launch_advance(uint32_t* counters, uint32_t* logSlot, hipStream_t s)
+hipError_t launch_advance(uint32_t* counters, uint32_t* logSlot, hipStream_t s, uint32_t* snapshot, uint32_t* live)
 {
-	k_advance<<<1, 1, 0, s>>>(counters, logSlot);
+	k_advance<<<1, 1, 0, s>>>(counters, logSlot, snapshot, live);
 	return hipGetLastError();
 }
 
