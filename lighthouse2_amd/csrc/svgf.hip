@@ -18,7 +18,7 @@ struct f2 { float x, y; };
 __device__ __forceinline__ v3 unpack_normal2(uint32_t p)
 {
 	const uint32_t x = (p >> 2u) & 1023u, y = (p >> 12u) & 1023u, z = p >> 22u;
-	return mk3((float)x * (1.0f / 511.0f) - 1, (float)y * (1.0f / 511.0f) - 1, (float)z * (1.0f / 511.0f) - 1);
+	return mk3(fmaf((float)x, 1.0f / 511.0f, -1.0f), fmaf((float)y, 1.0f / 511.0f, -1.0f), fmaf((float)z, 1.0f / 511.0f, -1.0f));
 }
 __device__ __forceinline__ v3 rgb32_to_hdr(uint32_t c)
 {
@@ -31,8 +31,10 @@ __device__ __forceinline__ v3 rgb32_to_hdr_min1(uint32_t c)
 }
 __device__ __forceinline__ float luminance(v3 c)
 {
-	return 0.299f * fminf(c.x, 10.0f) + 0.587f * fminf(c.y, 10.0f) + 0.114f * fminf(c.z, 10.0f);
+	return fmaf(0.114f, fminf(c.z, 10.0f), fmaf(0.587f, fminf(c.y, 10.0f), 0.299f * fminf(c.x, 10.0f)));
 }
+/* dot product with its two adds fused into the products */
+__device__ __forceinline__ float fdot(v3 a, v3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 __device__ __forceinline__ float4 combine(v3 A, v3 B)
 {
 	const uint32_t Ar = f2u_sat(fminf(A.x, 31.999f) * 2048.0f), Ag = f2u_sat(fminf(A.y, 31.999f) * 2048.0f), Ab = f2u_sat(fminf(A.z, 31.999f) * 2048.0f);
@@ -284,19 +286,19 @@ __global__ void k_apply(const uint4* __restrict__ features, const float4* __rest
 			const uint4 nf = features[q];
 			const float wdist = (float)(uu * uu + vv * vv) * (-1.0f / 7.5f);
 			const v3 nd = get_direct(c), ni = get_indirect(c), nn = unpack_normal2(nf.y);
-			float wn = pow128(fmaxf(0.0f, dot(nn, ln)));
-			const float expected = ldepth + ddx * (float)(uu * step) + ddy * (float)(vv * step);
+			float wn = pow128(fmaxf(0.0f, fdot(nn, ln)));
+			const float expected = fmaf(ddy, (float)(vv * step), fmaf(ddx, (float)(uu * step), ldepth));
 			const float err = fabsf(expected - bitsf(nf.z));
 			const float expDiff = fabsf(expected - ldepth);
 			const float wdepth = err / fmaxf(0.00001f, (0.5f + (float)phase * 0.5f) * expDiff);
 			if ((nf.w >> 4) != lmat) wn = (float)((double)wn * 0.0001);
-			else wn = wn * dot(lc, rgb32_to_hdr(nf.x));
-			float wd = wn * cw_expf(fabsf(lD - luminance(nd)) * rD + wdist - wdepth);
-			float wi = wn * cw_expf(fabsf(lI - luminance(ni)) * rI + wdist - wdepth);
+			else wn = wn * fdot(lc, rgb32_to_hdr(nf.x));
+			float wd = wn * cw_expf(fmaf(fabsf(lD - luminance(nd)), rD, wdist) - wdepth);
+			float wi = wn * cw_expf(fmaf(fabsf(lI - luminance(ni)), rI, wdist) - wdepth);
 			if (!__builtin_isfinite(wd)) wd = 0;
 			if (!__builtin_isfinite(wi)) wi = 0;
-			dsum = dsum + nd * wd, dws += wd;
-			isum = isum + ni * wi, iws += wi;
+			dsum = mk3(fmaf(nd.x, wd, dsum.x), fmaf(nd.y, wd, dsum.y), fmaf(nd.z, wd, dsum.z)), dws += wd;
+			isum = mk3(fmaf(ni.x, wi, isum.x), fmaf(ni.y, wi, isum.y), fmaf(ni.z, wi, isum.z)), iws += wi;
 		}
 	}
 	v3 df = dsum * (1.0f / fmaxf(0.0001f, dws));

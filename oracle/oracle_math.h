@@ -95,7 +95,8 @@ static inline float lh_atan2f(float y, float x)
 /* acos(x), |x| <= 1: atan2(sqrt((1 - x)(1 + x)), x). */
 static inline float lh_acosf(float x) { return lh_atan2f(sqrtf((1.0f - x) * (1.0f + x)), x); }
 /* log2(x): x = m 2^e with m in [sqrt(1/2), sqrt(2)), ln(m) = 2 atanh(s),
- * s = (m - 1) / (m + 1), |s| <= 0.1716, series to s^9 (truncation < 2e-9). */
+ * s = (m - 1) / (m + 1), |s| <= 0.1716, series to s^9 (truncation < 2e-9),
+ * its multiply-adds fused (fmaf, as the kernels' fma). */
 static inline float lh_log2f(float x)
 {
 	if (!(x > 0.0f)) return x == 0.0f ? -HUGE_VALF : NAN;
@@ -107,8 +108,11 @@ static inline float lh_log2f(float x)
 	float m = u2f((b & 0x007fffffu) | 0x3f800000u);
 	if (m > 1.4142135f) { m = m * 0.5f; e += 1; }
 	const float s = (m - 1.0f) / (m + 1.0f), z = s * s;
-	const float ln = 2.0f * s * (1.0f + z * (0.33333333f + z * (0.2f + z * (0.14285714f + z * 0.11111111f))));
-	return (float)e + ln * 1.4426950f;
+	float q = fmaf(z, 0.11111111f, 0.14285714f);
+	q = fmaf(q, z, 0.2f);
+	q = fmaf(q, z, 0.33333333f);
+	q = fmaf(q, z, 1.0f);
+	return fmaf(2.0f * s * q, 1.4426950f, (float)e);
 }
 /* float -> uint32 with the saturation of the GPU's conversion (NaN and
  * negatives give 0). */
@@ -117,14 +121,21 @@ static inline uint32_t f2u_sat(float x) { return x > 0.0f ? (x >= 4294967040.0f 
 static inline int32_t f2i_sat(float x) { return x >= 2147483520.0f ? 0x7fffffff : (x <= -2147483648.0f ? (int32_t)0x80000000 : (x == x ? (int32_t)x : 0)); }
 
 /* exp(x) as the kernels compute it: k = round(x / ln2), Cody-Waite split
- * of ln2, Taylor series of exp(r) to r^7, scaled by 2^k. */
+ * of ln2, Taylor series of exp(r) to r^7, scaled by 2^k; every multiply-add
+ * fused (fmaf is the singly rounded IEEE operation, as the kernels' fma). */
 static inline float cw_expf(float x)
 {
 	if (x < -87.0f) return 0.0f;
 	if (x > 88.0f) return HUGE_VALF;
-	const float k = floorf(x * 1.44269504f + 0.5f);
-	const float r = (x - k * 0.693359375f) - k * -2.12194440e-4f;
-	const float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.16666667f + r * (4.1666668e-2f + r * (8.3333333e-3f + r * (1.3888889e-3f + r * 1.9841270e-4f))))));
+	const float k = floorf(fmaf(x, 1.44269504f, 0.5f));
+	const float r = fmaf(k, 2.12194440e-4f, fmaf(-k, 0.693359375f, x));
+	float p = fmaf(r, 1.9841270e-4f, 1.3888889e-3f);
+	p = fmaf(p, r, 8.3333333e-3f);
+	p = fmaf(p, r, 4.1666668e-2f);
+	p = fmaf(p, r, 0.16666667f);
+	p = fmaf(p, r, 0.5f);
+	p = fmaf(p, r, 1.0f);
+	p = fmaf(p, r, 1.0f);
 	return p * u2f((uint)((int)k + 127) << 23);
 }
 

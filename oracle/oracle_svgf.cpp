@@ -22,7 +22,7 @@ struct fl2 { float x, y; };
 static f3 unpack_normal2(uint p)
 {
 	const uint x = (p >> 2u) & 1023u, y = (p >> 12u) & 1023u, z = p >> 22u;
-	return mk3((float)x * (1.0f / 511.0f) - 1, (float)y * (1.0f / 511.0f) - 1, (float)z * (1.0f / 511.0f) - 1);
+	return mk3(fmaf((float)x, 1.0f / 511.0f, -1.0f), fmaf((float)y, 1.0f / 511.0f, -1.0f), fmaf((float)z, 1.0f / 511.0f, -1.0f));
 }
 static f3 rgb32_to_hdr(uint c) { return mk3((float)(c >> 22) * (1.0f / 1023.0f), (float)((c >> 11) & 2047u) * (1.0f / 2047.0f), (float)(c & 2047u) * (1.0f / 2047.0f)); }
 static f3 rgb32_to_hdr_min1(uint c)
@@ -30,7 +30,9 @@ static f3 rgb32_to_hdr_min1(uint c)
 	return mk3((float)std::max(1u, c >> 22) * (1.0f / 1023.0f), (float)std::max(1u, (c >> 11) & 2047u) * (1.0f / 2047.0f),
 		(float)std::max(1u, c & 2047u) * (1.0f / 2047.0f));
 }
-static float luminance(f3 c) { return 0.299f * std::min(c.x, 10.0f) + 0.587f * std::min(c.y, 10.0f) + 0.114f * std::min(c.z, 10.0f); }
+static float luminance(f3 c) { return fmaf(0.114f, std::min(c.z, 10.0f), fmaf(0.587f, std::min(c.y, 10.0f), 0.299f * std::min(c.x, 10.0f))); }
+/* dot product with its two adds fused into the products (fmaf, as the kernels' fma) */
+static float fdot(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 static f4 combine(f3 A, f3 B)
 {
 	const uint Ar = f2u_sat(std::min(A.x, 31.999f) * 2048.0f), Ag = f2u_sat(std::min(A.y, 31.999f) * 2048.0f), Ab = f2u_sat(std::min(A.z, 31.999f) * 2048.0f);
@@ -339,19 +341,19 @@ struct Svgf
 					const u4 nf = features[q];
 					const float wdist = (float)(uu * uu + vv * vv) * (-1.0f / 7.5f);
 					const f3 nd = get_direct(c), ni = get_indirect(c);
-					float wn = pow128(std::max(0.0f, dot(unpack_normal2(nf.y), ln)));
-					const float expected = ldepth + ddx * (float)(uu * step) + ddy * (float)(vv * step);
+					float wn = pow128(std::max(0.0f, fdot(unpack_normal2(nf.y), ln)));
+					const float expected = fmaf(ddy, (float)(vv * step), fmaf(ddx, (float)(uu * step), ldepth));
 					const float err = fabsf(expected - u2f(nf.z));
 					const float expDiff = fabsf(expected - ldepth);
 					const float wdepth = err / std::max(0.00001f, (0.5f + (float)phase * 0.5f) * expDiff);
 					if ((nf.w >> 4) != lmat) wn = (float)((double)wn * 0.0001);
-					else wn = wn * dot(lc, rgb32_to_hdr(nf.x));
-					float wd = wn * cw_expf(fabsf(lD - luminance(nd)) * rD + wdist - wdepth);
-					float wi = wn * cw_expf(fabsf(lI - luminance(ni)) * rI + wdist - wdepth);
+					else wn = wn * fdot(lc, rgb32_to_hdr(nf.x));
+					float wd = wn * cw_expf(fmaf(fabsf(lD - luminance(nd)), rD, wdist) - wdepth);
+					float wi = wn * cw_expf(fmaf(fabsf(lI - luminance(ni)), rI, wdist) - wdepth);
 					if (!std::isfinite(wd)) wd = 0;
 					if (!std::isfinite(wi)) wi = 0;
-					dsum = dsum + nd * wd, dws += wd;
-					isum = isum + ni * wi, iws += wi;
+					dsum = mk3(fmaf(nd.x, wd, dsum.x), fmaf(nd.y, wd, dsum.y), fmaf(nd.z, wd, dsum.z)), dws += wd;
+					isum = mk3(fmaf(ni.x, wi, isum.x), fmaf(ni.y, wi, isum.y), fmaf(ni.z, wi, isum.z)), iws += wi;
 				}
 			}
 			f3 df = dsum * (1.0f / std::max(0.0001f, dws));
