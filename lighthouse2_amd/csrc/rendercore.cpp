@@ -1166,7 +1166,7 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	if (estimator == ESTIMATOR_FILTER && f.P.gFeatures)
 	{	/* FinalizeRender (RenderCore_Optix7Filter/rendercore.cpp:897-935) */
 		blueSlot = (blueSlot + 1) & 255;
-		if (!svgf || svgf->w != scrw || svgf->h != scrh) svgf.reset(new Svgf(scrw, scrh));
+		if (!svgf || svgf->w != scrw || svgf->h != scrh) svgf.reset(new Svgf(scrw, scrh, stream));
 		svgf->taa = taaOn;
 		hipCheck(hipEventRecord(ev[4], stream), "event");
 		svgf->Frame(wf.acc32(), wf.gFeatures(), wf.gWorldPos(), wf.gDeltaDepth(), prevView, j0, j1, prevj0, prevj1, samplesTaken, clampDirect,

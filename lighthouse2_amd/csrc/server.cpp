@@ -11,7 +11,10 @@
  * passPaths paths): the job with the least remaining work (samples x pixels)
  * takes all it needs, the next ones fill the rest, whole samples each; the
  * jobs' paths share every generate / extend / shade / connect launch and
- * are resolved into each job's own accumulators. A job renders exactly what
+ * are resolved into each job's own accumulators. A finished job's results
+ * reach the host through a drain thread (its own stream and pinned staging),
+ * so the worker never waits on a read-back; freed job buffers are pooled by
+ * size, so admission rarely calls hipMalloc and never hipFree. A job renders exactly what
  * RenderCore::Render(view, Restart) with its size, spp, depth and seed
  * renders: a path's random numbers depend on its index in its job's frame
  * only, and each path slot has one writer.
@@ -54,8 +57,15 @@ RenderServer::RenderServer(RenderCore& c, uint64_t budgetBytes, size_t pp) : cor
 	laneOvf.ensure(trace_overflow_bytes(core.traceBlocks()));
 	segs.ensure(sizeof(JobSeg) * JOB_SEGS_MAX);
 	check(hipHostMalloc((void**)&segsHost, sizeof(JobSeg) * JOB_SEGS_MAX, hipHostMallocDefault), "hipHostMalloc");
+	check(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking), "hipStreamCreate");
+	for (int i = 0; i < 2; i++)
+	{
+		check(hipHostMalloc(&stage[i], STAGE_BYTES, hipHostMallocDefault), "hipHostMalloc");
+		check(hipEventCreateWithFlags(&stageEv[i], hipEventBlockingSync | hipEventDisableTiming), "hipEventCreate");
+	}
 	t0 = std::chrono::steady_clock::now();
 	core.serverCount++;
+	drainer = std::thread([this] { drainLoop(); });
 	worker = std::thread([this] { loop(); });
 }
 
@@ -67,9 +77,22 @@ RenderServer::~RenderServer()
 	}
 	wake.notify_all();
 	if (worker.joinable()) worker.join();
+	{	/* the drain thread empties its queue, then ends */
+		std::lock_guard<std::mutex> g(drainLock);
+		drainStop = true;
+	}
+	drainWake.notify_all();
+	if (drainer.joinable()) drainer.join();
 	for (auto& kv : jobs) if (!kv.second->done) core.pendingJobs--;
 	jobs.clear(), waiting.clear();
 	filters.clear();
+	pool.clear();
+	for (int i = 0; i < 2; i++)
+	{
+		if (stage[i]) (void)hipHostFree(stage[i]);
+		if (stageEv[i]) (void)hipEventDestroy(stageEv[i]);
+	}
+	if (cstream) (void)hipStreamDestroy(cstream);
 	wf.release(), queue.release(), ovf.release(), laneScratch.release(), laneOvf.release(), segs.release();
 	if (segsHost) (void)hipHostFree(segsHost);
 	for (int i = 0; i < 2; i++) if (laneEv[i]) (void)hipEventDestroy(laneEv[i]);
@@ -157,14 +180,14 @@ void RenderServer::startJob(Job& j)
 {	/* per-job buffers, cleared (worker thread, on the server's stream) */
 	const int pixels = j.w * j.h;
 	const int halves = j.denoise ? 2 : 1;
-	j.acc32.ensure((size_t)halves * pixels * sizeof(float4)), j.acc64.ensure((size_t)halves * pixels * 4 * sizeof(double));
-	j.image.ensure((size_t)pixels * sizeof(float4));
+	takeBuf(j.acc32, (size_t)halves * pixels * sizeof(float4)), takeBuf(j.acc64, (size_t)halves * pixels * 4 * sizeof(double));
+	takeBuf(j.image, (size_t)pixels * sizeof(float4));
 	check(hipMemsetAsync(j.acc32.p, 0, j.acc32.bytes, stream), "memset");
 	check(hipMemsetAsync(j.acc64.p, 0, j.acc64.bytes, stream), "memset");
 	if (j.denoise)
 	{	/* SVGF writes the interior pixels only: the border stays 0, as in a fresh render target */
 		check(hipMemsetAsync(j.image.p, 0, j.image.bytes, stream), "memset");
-		j.gFeatures.ensure((size_t)pixels * 16), j.gWorldPos.ensure((size_t)pixels * 16), j.gDeltaDepth.ensure((size_t)pixels * 16);
+		takeBuf(j.gFeatures, (size_t)pixels * 16), takeBuf(j.gWorldPos, (size_t)pixels * 16), takeBuf(j.gDeltaDepth, (size_t)pixels * 16);
 		/* pixels whose path reaches no diffuse surface keep these zeros, as in a fresh render target */
 		check(hipMemsetAsync(j.gFeatures.p, 0, j.gFeatures.bytes, stream), "memset");
 		check(hipMemsetAsync(j.gWorldPos.p, 0, j.gWorldPos.bytes, stream), "memset");
@@ -176,7 +199,7 @@ void RenderServer::startJob(Job& j)
 }
 
 void RenderServer::finishJob(Job& j)
-{	/* all samples resolved: denoise or finalize, copy to the host */
+{	/* all samples resolved: denoise or finalize; the drain thread copies the results once `ready` is reached */
 	const int pixels = j.w * j.h;
 	if (j.denoise == DENOISE_SVGF)
 	{	/* FinalizeRender's filter path for a first frame (RenderCore_Optix7Filter/rendercore.cpp:897-935) */
@@ -186,7 +209,7 @@ void RenderServer::finishJob(Job& j)
 			filters.clear();
 		}
 		auto& f = filters[{ j.w, j.h }];
-		if (!f) f.reset(new Svgf(j.w, j.h));
+		if (!f) f.reset(new Svgf(j.w, j.h, stream));
 		else f->Reset(stream);
 		f->taa = 0;
 		const lighthouse2::ViewPyramid none = {};
@@ -200,17 +223,105 @@ void RenderServer::finishJob(Job& j)
 		unet.DenoiseFrame(j.acc32.as<float4>(), j.gFeatures.as<uint4>(), 1.0f / (float)j.spp, j.w, j.h, j.image.as<float4>(), stream);
 	}
 	else check(launch_finalize(j.acc32.as<float4>(), pixels, 1.0f / (float)j.spp, j.image.as<float4>(), stream), "finalize");
-	const size_t accWords = (size_t)pixels * 4;   /* denoised jobs: the direct half, as RenderCore::ReadAccumulatorF64 */
-	j.hostImage.resize((size_t)pixels * 4), j.hostAcc.resize(accWords);
-	check(hipMemcpyAsync(j.hostImage.data(), j.image.p, j.image.bytes, hipMemcpyDeviceToHost, stream), "D2H");
-	check(hipMemcpyAsync(j.hostAcc.data(), j.acc64.p, accWords * sizeof(double), hipMemcpyDeviceToHost, stream), "D2H");
-	check(hipStreamSynchronize(stream), "sync");
+	check(hipEventCreateWithFlags(&j.ready, hipEventDisableTiming), "hipEventCreate");
+	check(hipEventRecord(j.ready, stream), "hipEventRecord");
 }
 
 void RenderServer::releaseJob(Job& j)
 {
-	j.acc32.release(), j.acc64.release(), j.image.release();
-	j.gFeatures.release(), j.gWorldPos.release(), j.gDeltaDepth.release();
+	giveBuf(j.acc32), giveBuf(j.acc64), giveBuf(j.image);
+	giveBuf(j.gFeatures), giveBuf(j.gWorldPos), giveBuf(j.gDeltaDepth);
+}
+
+void RenderServer::takeBuf(DevBuf& b, size_t bytes)
+{
+	{
+		std::lock_guard<std::mutex> g(poolLock);
+		auto it = pool.find(bytes);
+		if (it != pool.end())
+		{
+			b = std::move(it->second);
+			pool.erase(it);
+			pooled -= bytes;
+			return;
+		}
+	}
+	if (hipMalloc(&b.p, bytes) == hipSuccess) { b.bytes = bytes; return; }
+	(void)hipGetLastError();
+	b.p = nullptr;
+	{	/* out of memory: give the pooled buffers back and try once more */
+		std::lock_guard<std::mutex> g(poolLock);
+		pool.clear(), pooled = 0;
+	}
+	b.ensure(bytes);
+}
+
+void RenderServer::giveBuf(DevBuf& b)
+{
+	if (!b.p) return;
+	std::lock_guard<std::mutex> g(poolLock);
+	if (pooled + b.bytes > budget) { b.release(); return; }
+	pooled += b.bytes;
+	const size_t n = b.bytes;
+	pool.emplace(n, std::move(b));
+}
+
+void RenderServer::drainCopy(void* dst, const void* src, size_t bytes)
+{	/* the copy engine fills one staging buffer while this thread empties the other */
+	size_t next = 0, off[2] = { 0, 0 }, len[2] = { 0, 0 };
+	for (int k = 0; next < bytes || len[0] || len[1]; k ^= 1)
+	{
+		if (len[k])
+		{
+			check(hipEventSynchronize(stageEv[k]), "hipEventSynchronize");
+			memcpy((char*)dst + off[k], stage[k], len[k]);
+			len[k] = 0;
+		}
+		if (next < bytes)
+		{
+			const size_t n = std::min(STAGE_BYTES, bytes - next);
+			check(hipMemcpyAsync(stage[k], (const char*)src + next, n, hipMemcpyDeviceToHost, cstream), "D2H");
+			check(hipEventRecord(stageEv[k], cstream), "hipEventRecord");
+			off[k] = next, len[k] = n, next += n;
+		}
+	}
+}
+
+void RenderServer::drainLoop()
+{
+	(void)hipSetDevice(device);
+	while (true)
+	{
+		std::shared_ptr<Job> j;
+		{
+			std::unique_lock<std::mutex> g(drainLock);
+			drainWake.wait(g, [&] { return drainStop || !draining.empty(); });
+			if (draining.empty()) return;
+			j = draining.front();
+			draining.pop_front();
+		}
+		try
+		{
+			check(hipStreamWaitEvent(cstream, j->ready, 0), "hipStreamWaitEvent");
+			const size_t pixels = (size_t)j->w * j->h;
+			j->hostImage.resize(pixels * 4), j->hostAcc.resize(pixels * 4);   /* denoised jobs: the direct half, as RenderCore::ReadAccumulatorF64 */
+			drainCopy(j->hostImage.data(), j->image.p, pixels * sizeof(float4));
+			drainCopy(j->hostAcc.data(), j->acc64.p, pixels * 4 * sizeof(double));
+		}
+		catch (const std::exception& e) { j->error = e.what(); }
+		(void)hipEventDestroy(j->ready);
+		j->ready = nullptr;
+		releaseJob(*j);
+		{
+			std::lock_guard<std::mutex> g(lock);
+			j->done = true;
+			j->t.done = now();
+			inUse -= j->bytes();
+			core.pendingJobs--;
+		}
+		finished.notify_all();
+		wake.notify_all();   /* the freed budget may admit waiting jobs */
+	}
 }
 
 void RenderServer::loop()
@@ -227,13 +338,13 @@ void RenderServer::loop()
 				if (stopping) return true;
 				if (paused) return false;
 				admit();
-				for (auto& kv : jobs) if (kv.second->admitted && !kv.second->done) return true;
+				for (auto& kv : jobs) if (kv.second->admitted && !kv.second->rendered) return true;
 				return false;
 			});
 			if (stopping) return;
 			/* least remaining work first; the pass takes jobs of the first one's estimator */
 			std::vector<std::shared_ptr<Job>> ready;
-			for (auto& kv : jobs) if (kv.second->admitted && !kv.second->done) ready.push_back(kv.second);
+			for (auto& kv : jobs) if (kv.second->admitted && !kv.second->rendered) ready.push_back(kv.second);
 			std::stable_sort(ready.begin(), ready.end(), [](const std::shared_ptr<Job>& a, const std::shared_ptr<Job>& b) {
 				return a->remaining() < b->remaining();
 			});
@@ -255,7 +366,7 @@ void RenderServer::loop()
 				parts.push_back(PassPart{ &c->frame, c->next, c->next + k, nullptr, nullptr });
 			}
 		}
-		std::vector<Job*> completed;
+		std::vector<std::shared_ptr<Job>> completed;   /* finish enqueued, or failed (error set) */
 		try
 		{
 			std::shared_lock<std::shared_mutex> sg(core.sceneMutex);
@@ -293,7 +404,7 @@ void RenderServer::loop()
 				{
 					try { finishJob(j); }
 					catch (const std::exception& e) { j.error = e.what(); }
-					completed.push_back(&j);
+					completed.push_back(batch[i]);
 				}
 			}
 			std::lock_guard<std::mutex> g(lock);
@@ -305,22 +416,46 @@ void RenderServer::loop()
 		catch (const std::exception& e)
 		{	/* the pass failed: every job in it fails */
 			completed.clear();
-			for (auto& c : batch) if (!c->done) c->error = e.what(), completed.push_back(c.get());
+			for (auto& c : batch) if (!c->rendered) c->error = e.what(), completed.push_back(c);
 		}
 		if (!completed.empty())
 		{
-			for (Job* j : completed) releaseJob(*j);
+			std::vector<std::shared_ptr<Job>> failed;
 			{
 				std::lock_guard<std::mutex> g(lock);
-				for (Job* j : completed)
+				for (auto& j : completed)
 				{
-					j->done = true;
-					j->t.done = now();
-					inUse -= j->bytes();
-					core.pendingJobs--;
+					j->rendered = true;
+					if (!j->error.empty()) failed.push_back(j);
 				}
 			}
-			finished.notify_all();
+			if (!failed.empty())
+			{	/* kernels of a failed pass may still use its jobs' buffers: let the streams drain before they are pooled */
+				(void)hipStreamSynchronize(stream), (void)hipStreamSynchronize(stream2);
+				(void)hipGetLastError();
+				for (auto& j : failed)
+				{
+					if (j->ready) (void)hipEventDestroy(j->ready), j->ready = nullptr;
+					releaseJob(*j);
+				}
+				{
+					std::lock_guard<std::mutex> g(lock);
+					for (auto& j : failed)
+					{
+						j->done = true;
+						j->t.done = now();
+						inUse -= j->bytes();
+						core.pendingJobs--;
+					}
+				}
+				finished.notify_all();
+				wake.notify_all();
+			}
+			{
+				std::lock_guard<std::mutex> g(drainLock);
+				for (auto& j : completed) if (j->error.empty()) draining.push_back(j);
+			}
+			drainWake.notify_all();
 		}
 	}
 }

@@ -64,6 +64,8 @@ private:
 		int next = 0;                   /* next sample to render */
 		int bypassed = 0;               /* later jobs admitted ahead of this one */
 		bool admitted = false, started = false, done = false;
+		bool rendered = false;          /* every sample resolved and the finish enqueued (or failed) */
+		hipEvent_t ready = nullptr;     /* recorded after the finish: the results may be copied */
 		std::string error;
 		DevBuf acc32, acc64, image, gFeatures, gWorldPos, gDeltaDepth;
 		std::vector<float> hostImage;
@@ -81,6 +83,16 @@ private:
 	void startJob(Job& j);
 	void finishJob(Job& j);
 	void releaseJob(Job& j);
+	/* The drain thread: copies a finished job's image and fp64 accumulator
+	 * to the host on its own stream (the copy engine) through two pinned
+	 * staging buffers, returns the job's buffers to the pool and marks the
+	 * job done, while the worker goes on with the next pass. */
+	void drainLoop();
+	void drainCopy(void* dst, const void* src, size_t bytes);
+	/* Per-job device buffers are kept for reuse, keyed by size, up to the
+	 * budget: hipFree synchronises the whole device. */
+	void takeBuf(DevBuf& b, size_t bytes);
+	void giveBuf(DevBuf& b);
 	double now() const;
 	RenderCore& core;
 	uint64_t budget;
@@ -104,6 +116,17 @@ private:
 	std::mutex unetLock;
 	std::chrono::steady_clock::time_point t0;
 	std::thread worker;
+	std::mutex drainLock, poolLock;
+	std::condition_variable drainWake;
+	std::deque<std::shared_ptr<Job>> draining;          /* rendered, results not yet on the host */
+	bool drainStop = false;
+	std::multimap<size_t, DevBuf> pool;                  /* freed job buffers by size */
+	uint64_t pooled = 0;
+	static constexpr size_t STAGE_BYTES = (size_t)32 << 20;
+	hipStream_t cstream = nullptr;                       /* the drain thread's copies */
+	void* stage[2] = { nullptr, nullptr };               /* pinned staging of the read-back */
+	hipEvent_t stageEv[2] = { nullptr, nullptr };
+	std::thread drainer;
 };
 
 } // namespace lh

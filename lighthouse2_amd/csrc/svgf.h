@@ -16,7 +16,9 @@ SvgfCam svgf_camera(const lighthouse2::ViewPyramid& prevView, int h);
 class Svgf
 {
 public:
-	Svgf(int w, int h);
+	/* The buffers are cleared on stream s, ahead of the frames drawn on it;
+	 * without a stream the constructor returns once they are clear. */
+	Svgf(int w, int h, hipStream_t s = nullptr);
 	int w, h;
 	/* One frame. accumulator: device float4 [2 * w * h] (direct, then indirect
 	 * radiance sums); features: device uint4 [w * h] {albedo HDRtoRGB32,

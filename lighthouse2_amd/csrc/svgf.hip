@@ -493,7 +493,7 @@ static inline void chk(hipError_t e, const char* what)
 	if (e != hipSuccess) throw CoreError(e == hipErrorOutOfMemory ? LH2HIP_ERR_OOM : LH2HIP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-Svgf::Svgf(int w_, int h_) : w(w_), h(h_)
+Svgf::Svgf(int w_, int h_, hipStream_t s) : w(w_), h(h_)
 {
 	if (w < 3 || h < 3 || (int64_t)w * h > (1 << 26)) throw CoreError(LH2HIP_ERR_INVALID, "svgf: size 3x3 .. 2^26 pixels");
 	const size_t n = (size_t)w * h;
@@ -502,7 +502,8 @@ Svgf::Svgf(int w_, int h_) : w(w_), h(h_)
 	filteredIn.ensure(n * 16), filteredOut.ensure(n * 16), lastPass.ensure(n * 16), prevPixels.ensure(n * 16), taaOut.ensure(n * 16);
 	for (DevBuf* b : { &features, &worldPos, &prevWorldPos, &motion, &moments, &prevMoments, &shading, &filteredIn, &filteredOut, &lastPass,
 		&prevPixels, &taaOut })
-		chk(hipMemset(b->p, 0, b->bytes), "memset");
+		chk(hipMemsetAsync(b->p, 0, b->bytes, s), "memset");
+	if (!s) chk(hipStreamSynchronize(nullptr), "sync");
 }
 
 void Svgf::Reset(hipStream_t s)
