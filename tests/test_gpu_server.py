@@ -118,6 +118,31 @@ def test_svgf_jobs_match_filtered_render(torch_cuda):
     core.Shutdown()
 
 
+def test_jobs_on_reused_buffers_match_their_solo_images(torch_cuda):
+    """Finished jobs' buffers go back to a pool keyed by size and serve later
+    jobs, also in another role: at 64 x 48 a plain job's fp64 accumulator has
+    the size of a denoised job's fp32 accumulator, and its image and fp32
+    accumulator the size of each G-buffer plane. Rounds of jobs that wait for
+    the previous round must still equal their solo renders exactly."""
+    s = scenes.cornell_box(sphere_tris=2000)
+    core = lh.RenderCore()
+    lh.load_scene(core, s)
+    srv = lh.RenderServer(core, budget_bytes=1 << 30, pass_paths=1 << 20)
+    view = scenes.cornell_view(64, 48)
+    rounds = [[(2, 31, 1), (3, 32, 0)], [(2, 33, 0), (3, 34, 1)], [(1, 35, 1), (4, 36, 0), (2, 37, 0)]]
+    for jobs in rounds:
+        ids = [srv.submit(view, 64, 48, spp, 4, seed, denoise=dn) for (spp, seed, dn) in jobs]
+        for j in ids:
+            srv.wait(j, timeout=120)
+        for (spp, seed, dn), j in zip(jobs, ids):
+            img, acc = srv.read(j, with_acc=True)
+            ri, ra = solo_filtered(s, view, 64, 48, spp, seed) if dn else solo(s, view, 64, 48, spp, 4, seed)
+            assert np.array_equal(acc, ra), (spp, seed, dn)
+            assert np.array_equal(img, ri), (spp, seed, dn)
+    srv.close()
+    core.Shutdown()
+
+
 def test_more_persistent_blocks_after_server_start(torch_cuda):
     """traceBlocksPerCU raised after the server was created: the server grows
     its traversal overflow stacks before the next quantum; the job still
