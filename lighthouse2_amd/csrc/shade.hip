@@ -229,11 +229,21 @@ __device__ v3 random_barycentrics(float r0)
 	return mk3(rx, ry, 1 - rx - ry);
 }
 
+/* Light records do not change during a launch. Read through the constant
+ * address space, the potential loops (every lane at the same light) fetch
+ * them with scalar loads instead of one vector load per lane. */
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) float4* ConstF4;
+#else
+typedef const float4* ConstF4;   /* the host pass only parses the device functions */
+#endif
+__device__ __forceinline__ ConstF4 as_const(const float4* p) { return (ConstF4)p; }
+
 __device__ __forceinline__ float potential(const ShadeParams& P, int i, v3 I, v3 N, v3 bary)
 {
 	if (i < P.nTri)
 	{
-		const float4* L = P.triLights + 6 * i;
+		const ConstF4 L = as_const(P.triLights) + 6 * i;
 		const float4 d0 = L[0], d1 = L[1];
 		v3 Lv = I;
 		if (bary.x >= 0) Lv = bary.x * xyz(L[3]) + bary.y * xyz(L[4]) + bary.z * xyz(L[5]);
@@ -247,7 +257,7 @@ __device__ __forceinline__ float potential(const ShadeParams& P, int i, v3 I, v3
 	i -= P.nTri;
 	if (i < P.nPoint)
 	{
-		const float4 d0 = P.pointLights[2 * i];
+		const float4 d0 = as_const(P.pointLights)[2 * i];
 		const v3 Lv = xyz(d0) - I;
 		const float NdotL = fmaxf(0.0f, dot(N, normalize(Lv)));
 		const float att = 1.0f / dot(Lv, Lv);
@@ -256,7 +266,8 @@ __device__ __forceinline__ float potential(const ShadeParams& P, int i, v3 I, v3
 	i -= P.nPoint;
 	if (i < P.nSpot)
 	{
-		const float4 d0 = P.spotLights[3 * i], d1 = P.spotLights[3 * i + 1], d2 = P.spotLights[3 * i + 2];
+		const ConstF4 S = as_const(P.spotLights) + 3 * i;
+		const float4 d0 = S[0], d1 = S[1], d2 = S[2];
 		v3 Lv = xyz(d0) - I;
 		const float att = 1.0f / dot(Lv, Lv);
 		Lv = normalize(Lv);
@@ -266,7 +277,7 @@ __device__ __forceinline__ float potential(const ShadeParams& P, int i, v3 I, v3
 		return (d1.x + d1.y + d1.z) * LNdotL * NdotL * att;
 	}
 	i -= P.nSpot;
-	const float4 d0 = P.dirLights[2 * i];
+	const float4 d0 = as_const(P.dirLights)[2 * i];
 	const float LNdotL = fmaxf(0.0f, -(d0.x * N.x + d0.y * N.y + d0.z * N.z));
 	return d0.w * LNdotL;
 }
@@ -708,7 +719,7 @@ __device__ float light_pick_prob(const ShadeParams& P, int idx, v3 O, v3 N, v3 I
 		float c;
 		if (i < P.nTri)
 		{
-			const float4* L = P.triLights + 6 * i;
+			const ConstF4 L = as_const(P.triLights) + 6 * i;
 			v3 Lv = I - O;
 			const float att = 1.0f / dot(Lv, Lv);
 			Lv = normalize(Lv);
