@@ -239,6 +239,19 @@ typedef const float4* ConstF4;   /* the host pass only parses the device functio
 #endif
 __device__ __forceinline__ ConstF4 as_const(const float4* p) { return (ConstF4)p; }
 
+/* dot product with its two adds fused into the products */
+__device__ __forceinline__ float fdot3(v3 a, v3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+/* A triangle light's potential seen along Lv (light point - shading point):
+ * energy max(0, -LN.L) max(0, N.L) / |Lv|^2 with L = Lv / |Lv|, its dot
+ * products fused. */
+__device__ __forceinline__ float tri_potential(float energy, v3 LN, v3 N, v3 Lv)
+{
+	const float d2 = fdot3(Lv, Lv);
+	const float att = 1.0f / d2;
+	Lv = Lv * (1.0f / sqrtf(d2));
+	return energy * fmaxf(0.0f, -fdot3(LN, Lv)) * fmaxf(0.0f, fdot3(N, Lv)) * att;
+}
+
 __device__ __forceinline__ float potential(const ShadeParams& P, int i, v3 I, v3 N, v3 bary)
 {
 	if (i < P.nTri)
@@ -246,13 +259,13 @@ __device__ __forceinline__ float potential(const ShadeParams& P, int i, v3 I, v3
 		const ConstF4 L = as_const(P.triLights) + 6 * i;
 		const float4 d0 = L[0], d1 = L[1];
 		v3 Lv = I;
-		if (bary.x >= 0) Lv = bary.x * xyz(L[3]) + bary.y * xyz(L[4]) + bary.z * xyz(L[5]);
-		Lv = Lv - I;
-		const float att = 1.0f / dot(Lv, Lv);
-		Lv = normalize(Lv);
-		const float LNdotL = fmaxf(0.0f, -dot(xyz(d1), Lv));
-		const float NdotL = fmaxf(0.0f, dot(N, Lv));
-		return d0.w * LNdotL * NdotL * att;
+		if (bary.x >= 0)
+		{	/* the barycentric point, fused */
+			const float4 a = L[3], b = L[4], c = L[5];
+			Lv = mk3(fmaf(bary.z, c.x, fmaf(bary.y, b.x, bary.x * a.x)), fmaf(bary.z, c.y, fmaf(bary.y, b.y, bary.x * a.y)),
+				fmaf(bary.z, c.z, fmaf(bary.y, b.z, bary.x * a.z)));
+		}
+		return tri_potential(d0.w, xyz(d1), N, Lv - I);
 	}
 	i -= P.nTri;
 	if (i < P.nPoint)
@@ -720,12 +733,7 @@ __device__ float light_pick_prob(const ShadeParams& P, int idx, v3 O, v3 N, v3 I
 		if (i < P.nTri)
 		{
 			const ConstF4 L = as_const(P.triLights) + 6 * i;
-			v3 Lv = I - O;
-			const float att = 1.0f / dot(Lv, Lv);
-			Lv = normalize(Lv);
-			const float LNdotL = fmaxf(0.0f, -dot(xyz(L[1]), Lv));
-			const float NdotL = fmaxf(0.0f, dot(N, Lv));
-			c = L[0].w * LNdotL * NdotL * att;
+			c = tri_potential(L[0].w, xyz(L[1]), N, I - O);
 		}
 		else c = potential(P, i, O, N, mk3(-1, -1, -1));
 		sum += c;

@@ -307,20 +307,25 @@ static f3 random_barycentrics(float r0)
 	return mk3(rx, ry, 1 - rx - ry);
 }
 
-/* Potential contributions (lib/CUDA/shared_kernel_code/lights_shared.h:37-108). */
+/* Potential contributions (lib/CUDA/shared_kernel_code/lights_shared.h:37-108).
+ * The triangle light's barycentric point and dot products are fused (fmaf,
+ * as the kernels' fma; nvcc contracts the same expressions by default). */
+static float fdot3(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 static float potential_tri(const CoreLightTri& L, f3 O, f3 N, f3 I, f3 bary)
 {
 	f3 Lv = I;
 	if (bary.x >= 0)
 	{
 		const f3 V0 = f3of(L.vertex0), V1 = f3of(L.vertex1), V2 = f3of(L.vertex2);
-		Lv = bary.x * V0 + bary.y * V1 + bary.z * V2;
+		Lv = mk3(fmaf(bary.z, V2.x, fmaf(bary.y, V1.x, bary.x * V0.x)), fmaf(bary.z, V2.y, fmaf(bary.y, V1.y, bary.x * V0.y)),
+			fmaf(bary.z, V2.z, fmaf(bary.y, V1.z, bary.x * V0.z)));
 	}
 	Lv = Lv - O;
-	const float att = 1.0f / dot(Lv, Lv);
-	Lv = normalize(Lv);
-	const float LNdotL = std::max(0.0f, -dot(f3of(L.N), Lv));
-	const float NdotL = std::max(0.0f, dot(N, Lv));
+	const float d2 = fdot3(Lv, Lv);
+	const float att = 1.0f / d2;
+	Lv = Lv * (1.0f / sqrtf(d2));
+	const float LNdotL = std::max(0.0f, -fdot3(f3of(L.N), Lv));
+	const float NdotL = std::max(0.0f, fdot3(N, Lv));
 	return L.energy * LNdotL * NdotL * att;
 }
 static float potential_point(const CorePointLight& L, f3 I, f3 N)
