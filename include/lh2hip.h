@@ -265,6 +265,9 @@ LH2HIP_API int lh2hip_dn_adam(float* p, float* m, float* v, const float* g, uint
  * device memory (a job that does not fit lets smaller later ones pass, at
  * most 8 times), run by a worker thread in passes of at most passPaths paths
  * that carry up to 16 jobs' paths together, least remaining work first. A
+ * finished job's results are copied to host memory by a second server thread
+ * while the next passes run; its device buffers are then kept in a pool for
+ * later jobs (at most budgetBytes more, freed with the server). A
  * job's image equals RenderCore::Render(view, Restart) with the same size,
  * spp, depth and seed, bit for bit. Submit from any thread. While jobs are
  * pending, the calls that change the scene (SetGeometry, SetInstance,
