@@ -935,7 +935,7 @@ FrameSetup RenderCore::frameParams(const ViewPyramid& view, int w, int h, int sp
  * the pass size nor on how the samples are split between calls: a path's
  * random numbers are keyed by its index within the frame. */
 int RenderCore::runSamples(const FrameSetup& f, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
-	uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing, const SideLane* side)
+	uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing, const SideLane* side, bool fresh)
 {
 	const int pixels = f.pixels;
 	const int passSamples = std::max(1, std::min(s1 - s0, (int)(std::min(passCap, Wavefront::PASS_CAP) / (size_t)pixels)));
@@ -943,7 +943,7 @@ int RenderCore::runSamples(const FrameSetup& f, int s0, int s1, size_t passCap, 
 	int deepest = 0, sideTimed = 0;
 	for (int a = s0; a < s1; a += passSamples)
 	{
-		PassPart part = { &f, a, std::min(s1, a + passSamples), acc32, acc64 };
+		PassPart part = { &f, a, std::min(s1, a + passSamples), acc32, acc64, fresh && a == s0 };
 		deepest = std::max(deepest, runPass(&part, 1, w, queue, ovf, st, evs, timing, side, nullptr, nullptr, &sideTimed));
 	}
 	if (side && side->timing && timing && sideTimed > 0)
@@ -1113,10 +1113,10 @@ int RenderCore::runPass(const PassPart* parts, int np, Wavefront& w, uint32_t* q
 	{
 		const FrameSetup& f = *parts[i].f;
 		const int k = parts[i].s1 - parts[i].s0;
-		hipCheck(launch_resolve(B.pathAcc + slotBase[i], f.pixels, k, parts[i].acc32, parts[i].acc64, st), "resolve");
+		hipCheck(launch_resolve(B.pathAcc + slotBase[i], f.pixels, k, parts[i].acc32, parts[i].acc64, st, parts[i].fresh), "resolve");
 		if (f.P.gFeatures)
-			hipCheck(launch_resolve(B.pathAcc + n + slotBase[i], f.pixels, k, parts[i].acc32 + f.pixels, parts[i].acc64 + 4 * (size_t)f.pixels, st),
-				"resolve");
+			hipCheck(launch_resolve(B.pathAcc + n + slotBase[i], f.pixels, k, parts[i].acc32 + f.pixels, parts[i].acc64 + 4 * (size_t)f.pixels, st,
+				parts[i].fresh), "resolve");
 	}
 	if (sideTimedIo) *sideTimedIo = sideTimed;
 	return deepest;
@@ -1130,9 +1130,10 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	if (!sceneReady) return; /* no geometry yet: nothing to render (rendercore_optix7/rendercore.cpp:821) */
 	checkRenderable();
 	const auto t0 = std::chrono::high_resolution_clock::now();
+	bool fresh = false;   /* the frame's first pass writes the accumulators (no clear needed) */
 	if (converge == Restart || firstConvergingFrame)
 	{
-		wf.clearAccumulators(stream);
+		fresh = true;
 		samplesTaken = 0;
 		firstConvergingFrame = true;
 		if (estimator != ESTIMATOR_FILTER) camSeed = seedSetting;   /* the filter core keeps its seed (RenderCore_Optix7Filter/rendercore.cpp:749) */
@@ -1158,7 +1159,7 @@ void RenderCore::Render(const ViewPyramid& view, const Convergence converge, boo
 	lane.st = stream2, lane.counters = sideCounters.as<uint32_t>(), lane.queue = sideCounters.as<uint32_t>() + 4, lane.ovf = overflow2.as<uint32_t>();
 	lane.shaded = sideEv[0], lane.done = sideEv[1], lane.timing = sideEv + 2;
 	const int waves = runSamples(f, 0, scrspp, Wavefront::PASS_CAP, wf, wf.acc32(), wf.acc64(), counters.as<uint32_t>() + TRACE_QUEUE_WORDS,
-		overflow.as<uint32_t>(), stream, ev, timing, shadowLane ? &lane : nullptr);
+		overflow.as<uint32_t>(), stream, ev, timing, shadowLane ? &lane : nullptr, fresh);
 	if (estimator != ESTIMATOR_REFERENCE) for (int k = 0; k < waves; k++) xor32_host(camSeed);
 	const int pixels = scrw * scrh;
 	samplesTaken += scrspp;

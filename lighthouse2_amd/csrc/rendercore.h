@@ -117,6 +117,7 @@ struct PassPart
 	int s0, s1;
 	float4* acc32;
 	double* acc64;
+	bool fresh = false;   /* the part's first samples since a restart: the resolve writes instead of adding */
 };
 
 class Svgf;
@@ -186,7 +187,7 @@ public:
 	static const int SIDE_TIMED = 16;
 	/* Returns the number of shade waves the deepest pass ran. */
 	int runSamples(const FrameSetup& f, int s0, int s1, size_t passCap, Wavefront& w, float4* acc32, double* acc64,
-		uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing, const SideLane* side = nullptr);
+		uint32_t* queue, uint32_t* ovf, hipStream_t st, hipEvent_t* evs, float* timing, const SideLane* side = nullptr, bool fresh = false);
 	/* One pass over the paths of 1..JOB_SEGS_MAX parts (at most 2^24 paths in
 	 * all, one estimator); several parts need a staging segment table on the
 	 * host and its device copy. Returns the number of waves run. */

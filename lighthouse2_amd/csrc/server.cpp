@@ -182,8 +182,7 @@ void RenderServer::startJob(Job& j)
 	const int halves = j.denoise ? 2 : 1;
 	takeBuf(j.acc32, (size_t)halves * pixels * sizeof(float4)), takeBuf(j.acc64, (size_t)halves * pixels * 4 * sizeof(double));
 	takeBuf(j.image, (size_t)pixels * sizeof(float4));
-	check(hipMemsetAsync(j.acc32.p, 0, j.acc32.bytes, stream), "memset");
-	check(hipMemsetAsync(j.acc64.p, 0, j.acc64.bytes, stream), "memset");
+	/* no clear of the accumulators: the job's first pass writes them (PassPart::fresh) */
 	if (j.denoise)
 	{	/* SVGF writes the interior pixels only: the border stays 0, as in a fresh render target */
 		check(hipMemsetAsync(j.image.p, 0, j.image.bytes, stream), "memset");
@@ -363,7 +362,7 @@ void RenderServer::loop()
 				}
 				batch.push_back(c);
 				room -= std::min(room, (size_t)k * px);
-				parts.push_back(PassPart{ &c->frame, c->next, c->next + k, nullptr, nullptr });
+				parts.push_back(PassPart{ &c->frame, c->next, c->next + k, nullptr, nullptr, c->next == 0 });
 			}
 		}
 		std::vector<std::shared_ptr<Job>> completed;   /* finish enqueued, or failed (error set) */

@@ -119,7 +119,9 @@ hipError_t launch_shade(const ShadeParams& p, const PathBuffers& b, int in, int 
 /* Adds each unoccluded connection to its path slot; countW is added to w
  * (optix7's connections count 1 there, rendercore_optix7/optix/.optix.cu:153). */
 hipError_t launch_connect(const PathBuffers& b, int maxCount, hipStream_t s, float countW = 0.0f);
-hipError_t launch_resolve(const float4* pathAcc, int pixels, int samples, float4* acc32, double* acc64, hipStream_t s);
+/* fresh: write the sums instead of adding them to the accumulators (the
+ * first pass of a frame after a restart; every pixel gets >= 1 sample). */
+hipError_t launch_resolve(const float4* pathAcc, int pixels, int samples, float4* acc32, double* acc64, hipStream_t s, bool fresh = false);
 hipError_t launch_finalize(const float4* acc32, int pixels, float scale, float4* image, hipStream_t s);
 hipError_t launch_set_u32(uint32_t* p, int n, uint32_t v, hipStream_t s);
 /* optix7 estimator (lib/rendercore_optix7): eye rays with blue-noise pixel and

@@ -630,13 +630,18 @@ __global__ void k_connect(const float4* __restrict__ shadowPot, const uint32_t* 
 	*a = v;
 }
 
-/* Sample-ordered reduction of the path slots into the pixel accumulators. */
-__global__ void k_resolve(const float4* __restrict__ pathAcc, int pixels, int samples, float4* __restrict__ acc32, double* __restrict__ acc64)
+/* Sample-ordered reduction of the path slots into the pixel accumulators.
+ * fresh (a frame's first pass after a restart): the sums start from zero
+ * instead of the stored values, the same additions as after clearing the
+ * accumulators, without the clear and its reads. */
+__global__ void k_resolve(const float4* __restrict__ pathAcc, int pixels, int samples, float4* __restrict__ acc32, double* __restrict__ acc64,
+	int fresh)
 {
 	const int px = blockIdx.x * blockDim.x + threadIdx.x;
 	if (px >= pixels) return;
-	float4 a = acc32[px];
-	double ax = acc64[4 * px], ay = acc64[4 * px + 1], az = acc64[4 * px + 2], aw = acc64[4 * px + 3];
+	float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	double ax = 0.0, ay = 0.0, az = 0.0, aw = 0.0;
+	if (!fresh) a = acc32[px], ax = acc64[4 * px], ay = acc64[4 * px + 1], az = acc64[4 * px + 2], aw = acc64[4 * px + 3];
 	for (int s = 0; s < samples; s++)
 	{
 		const float4 c = pathAcc[(size_t)s * pixels + px];
@@ -1379,9 +1384,9 @@ hipError_t launch_connect(const PathBuffers& b, int maxCount, hipStream_t s, flo
 	return hipGetLastError();
 }
 
-hipError_t launch_resolve(const float4* pathAcc, int pixels, int samples, float4* acc32, double* acc64, hipStream_t s)
+hipError_t launch_resolve(const float4* pathAcc, int pixels, int samples, float4* acc32, double* acc64, hipStream_t s, bool fresh)
 {
-	k_resolve<<<(pixels + 255) / 256, 256, 0, s>>>(pathAcc, pixels, samples, acc32, acc64);
+	k_resolve<<<(pixels + 255) / 256, 256, 0, s>>>(pathAcc, pixels, samples, acc32, acc64, fresh ? 1 : 0);
 	return hipGetLastError();
 }
 
