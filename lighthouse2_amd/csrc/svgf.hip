@@ -168,18 +168,13 @@ __device__ void read_texel_consistent2(const float4* buf, const float4* prevWP, 
 		(wt[0] * pi[0].z + wt[1] * pi[1].z + wt[2] * pi[2].z + wt[3] * pi[3].z) * r);
 }
 
-__global__ void k_merge_features(const uint4* __restrict__ in, uint4* __restrict__ f, int n)
-{
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const uint4 a = in[i];
-	f[i] = make_uint4(a.x, a.y, a.z, (a.w & ~15u) | (f[i].w & 15u));
-}
 
 /* prepareFilterKernel (finalize_shared.h:226-294) */
 /* Also keeps this frame's world positions (the next frame's history) in
- * worldPosOut, so the G-buffer needs no separate copy. */
-__global__ void k_prepare(const float4* __restrict__ acc, uint4* __restrict__ features, const float4* __restrict__ worldPos,
+ * worldPosOut, so the G-buffer needs no separate copy, and merges the frame's
+ * G-buffer features (feat) into the filter's copy: x, y, z and the flags of w
+ * from the frame, the history count (bits 0..3 of w) kept and updated. */
+__global__ void k_prepare(const float4* __restrict__ acc, const uint4* __restrict__ feat, uint4* __restrict__ features, const float4* __restrict__ worldPos,
 	float4* __restrict__ worldPosOut, const float4* __restrict__ prevWorldPos, float4* __restrict__ shading, float2* __restrict__ motion,
 	float4* __restrict__ moments, const float4* __restrict__ prevMoments, SvgfCam cam, float j0, float j1, float pj0, float pj1,
 	int w, int h, float pvs, float clampD, float clampI, int stationary)
@@ -189,7 +184,9 @@ __global__ void k_prepare(const float4* __restrict__ acc, uint4* __restrict__ fe
 	const int p = x + y * w;
 	const float4 a0 = acc[p], a1 = acc[p + w * h];
 	const v3 direct = mk3(a0.x, a0.y, a0.z) * pvs;
-	const uint4 lf = features[p];
+	const uint4 g = feat[p];
+	const uint4 lf = make_uint4(g.x, g.y, g.z, (g.w & ~15u) | (features[p].w & 15u));
+	uint32_t fw = lf.w;                                  /* w with the history count this frame leaves */
 	const float4 lwp = worldPos[p];
 	worldPosOut[p] = lwp;
 	const v3 albedo = rgb32_to_hdr_min1(lf.x);
@@ -239,11 +236,12 @@ __global__ void k_prepare(const float4* __restrict__ acc, uint4* __restrict__ fe
 			lumD2 = 0.2f * lumD2 + 0.8f * hist.y;
 			lumI = 0.2f * lumI + 0.8f * hist.z;
 			lumI2 = 0.2f * lumI2 + 0.8f * hist.w;
-			if ((features[p].w & 15u) < 15u) features[p].w++;
+			if ((fw & 15u) < 15u) fw++;
 		}
-		else features[p].w &= 0xfffffff0u;
+		else fw &= 0xfffffff0u;
 	}
-	else features[p].w &= 0xfffffff0u;
+	else fw &= 0xfffffff0u;
+	features[p] = make_uint4(lf.x, lf.y, lf.z, fw);
 	motion[p] = make_float2(pp.x, pp.y);
 	moments[p] = make_float4(lumD, lumD2, lumI, lumI2);
 }
@@ -517,13 +515,11 @@ void Svgf::Frame(const float4* acc, const uint4* feat, const float4* wp, const f
 	float j0, float j1, float pj0, float pj1, int samplesTaken, float clampD, float clampI, int stationary, float4* out, hipStream_t s)
 {
 	if (samplesTaken <= 0) throw CoreError(LH2HIP_ERR_INVALID, "svgf frame: samplesTaken >= 1");
-	const int n = w * h;
-	k_merge_features<<<(n + 255) / 256, 256, 0, s>>>(feat, features.as<uint4>(), n);
 	const dim3 b(8, 8), g((w + 7) / 8, (h + 7) / 8);
 	const SvgfCam cam = svgf_camera(prevView, h);
 	/* wp and dd are read in place (stream order keeps them valid until the
 	 * last pass); k_prepare keeps wp as the next frame's history */
-	k_prepare<<<g, b, 0, s>>>(acc, features.as<uint4>(), wp, worldPos.as<float4>(), prevWorldPos.as<float4>(), shading.as<float4>(),
+	k_prepare<<<g, b, 0, s>>>(acc, feat, features.as<uint4>(), wp, worldPos.as<float4>(), prevWorldPos.as<float4>(), shading.as<float4>(),
 		motion.as<float2>(), moments.as<float4>(), prevMoments.as<float4>(), cam, j0, j1, pj0, pj1, w, h, 1.0f / (float)samplesTaken, clampD,
 		clampI, stationary);
 	const auto apply = [&](int phase, DevBuf& A, const float4* B, DevBuf& C, int last) {
