@@ -303,19 +303,22 @@ void RenderServer::drainLoop()
 		{
 			check(hipStreamWaitEvent(cstream, j->ready, 0), "hipStreamWaitEvent");
 			const size_t pixels = (size_t)j->w * j->h;
-			j->hostImage.resize(pixels * 4), j->hostAcc.resize(pixels * 4);   /* denoised jobs: the direct half, as RenderCore::ReadAccumulatorF64 */
+			j->hostImage.resize(pixels * 4);
 			drainCopy(j->hostImage.data(), j->image.p, pixels * sizeof(float4));
-			drainCopy(j->hostAcc.data(), j->acc64.p, pixels * 4 * sizeof(double));
 		}
 		catch (const std::exception& e) { j->error = e.what(); }
 		(void)hipEventDestroy(j->ready);
 		j->ready = nullptr;
+		DevBuf acc = std::move(j->acc64);     /* kept for Read */
 		releaseJob(*j);
+		if (j->error.empty()) j->acc64 = std::move(acc);
+		else giveBuf(acc);
 		{
 			std::lock_guard<std::mutex> g(lock);
 			j->done = true;
 			j->t.done = now();
-			inUse -= j->bytes();
+			j->held = j->acc64.bytes;
+			inUse -= j->bytes() - j->held;
 			core.pendingJobs--;
 		}
 		finished.notify_all();
@@ -488,9 +491,19 @@ void RenderServer::Read(uint64_t id, void* image, double* acc64)
 		j = it->second;
 		jobs.erase(it);
 	}
+	hipError_t e = hipSuccess;
+	if (j->error.empty() && acc64)   /* denoised jobs: the direct half, as RenderCore::ReadAccumulatorF64 */
+		e = hipMemcpy(acc64, j->acc64.p, (size_t)j->w * j->h * 4 * sizeof(double), hipMemcpyDeviceToHost);
+	giveBuf(j->acc64);
+	{
+		std::lock_guard<std::mutex> g(lock);
+		inUse -= j->held;
+		j->held = 0;
+	}
+	wake.notify_all();   /* the freed budget may admit waiting jobs */
 	if (!j->error.empty()) throw CoreError(LH2HIP_ERR_DEVICE, "server job failed: " + j->error);
+	check(e, "D2H");
 	if (image) memcpy(image, j->hostImage.data(), j->hostImage.size() * sizeof(float));
-	if (acc64) memcpy(acc64, j->hostAcc.data(), j->hostAcc.size() * sizeof(double));
 }
 
 JobTimes RenderServer::Times(uint64_t id)

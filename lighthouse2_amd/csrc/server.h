@@ -69,7 +69,7 @@ private:
 		std::string error;
 		DevBuf acc32, acc64, image, gFeatures, gWorldPos, gDeltaDepth;
 		std::vector<float> hostImage;
-		std::vector<double> hostAcc;
+		uint64_t held = 0;              /* budget bytes a drained job still holds: its fp64 accumulator, until read */
 		JobTimes t = { 0, 0, 0 };
 		FrameSetup frame;
 		/* accumulators + image: 64 B/px; denoised: direct and indirect halves
@@ -83,10 +83,11 @@ private:
 	void startJob(Job& j);
 	void finishJob(Job& j);
 	void releaseJob(Job& j);
-	/* The drain thread: copies a finished job's image and fp64 accumulator
-	 * to the host on its own stream (the copy engine) through two pinned
-	 * staging buffers, returns the job's buffers to the pool and marks the
-	 * job done, while the worker goes on with the next pass. */
+	/* The drain thread: copies a finished job's image to the host on its own
+	 * stream through two pinned staging buffers, returns the job's buffers
+	 * but its fp64 accumulator to the pool and marks the job done, while the
+	 * worker goes on with the next pass. The accumulator stays on the device
+	 * (and in the budget) until Read copies it or releases the job. */
 	void drainLoop();
 	void drainCopy(void* dst, const void* src, size_t bytes);
 	/* Per-job device buffers are kept for reuse, keyed by size, up to the
