@@ -265,12 +265,10 @@ LH2HIP_API int lh2hip_dn_adam(float* p, float* m, float* v, const float* g, uint
  * device memory (a job that does not fit lets smaller later ones pass, at
  * most 8 times), run by a worker thread in passes of at most passPaths paths
  * that carry up to 16 jobs' paths together, least remaining work first. A
- * finished job's image is copied to host memory by a second server thread
- * while the next passes run; its device buffers are then kept in a pool for
- * later jobs (at most budgetBytes more, freed with the server), except its
- * fp64 accumulator (32 B per pixel, 64 for denoised jobs), which stays on the
- * device and in the budget until lh2hip_server_read copies it or releases the
- * job. A
+ * finished job's image, then its fp64 accumulator, are copied to host
+ * memory by a second server thread while the next passes run (the job is
+ * done once the image is there); its device buffers are then kept in a pool
+ * for later jobs (at most budgetBytes more, freed with the server). A
  * job's image equals RenderCore::Render(view, Restart) with the same size,
  * spp, depth and seed, bit for bit. Submit from any thread. While jobs are
  * pending, the calls that change the scene (SetGeometry, SetInstance,
@@ -315,8 +313,8 @@ LH2HIP_API int lh2hip_server_done(lh2hip_server* server, uint64_t jobId, int* do
  * Returns LH2HIP_ERR_STATE on timeout. */
 LH2HIP_API int lh2hip_server_wait(lh2hip_server* server, uint64_t jobId, double timeoutSeconds);
 /* Copies the finished job's image (w*h float4, host) and optionally its fp64
- * accumulator (w*h*4 doubles, may be NULL; copied from device memory now),
- * then releases the job and the budget it still holds. */
+ * accumulator (w*h*4 doubles, may be NULL; waits until it is on the host),
+ * then releases the job. */
 LH2HIP_API int lh2hip_server_read(lh2hip_server* server, uint64_t jobId, lh2_float4* image, double* acc64);
 /* Seconds since the server started at which the job was queued, admitted
  * and finished (out3). */

@@ -307,19 +307,28 @@ void RenderServer::drainLoop()
 			drainCopy(j->hostImage.data(), j->image.p, pixels * sizeof(float4));
 		}
 		catch (const std::exception& e) { j->error = e.what(); }
-		(void)hipEventDestroy(j->ready);
-		j->ready = nullptr;
-		DevBuf acc = std::move(j->acc64);     /* kept for Read */
-		releaseJob(*j);
-		if (j->error.empty()) j->acc64 = std::move(acc);
-		else giveBuf(acc);
-		{
+		{	/* the image is on the host: the job is done and reads the scene no more */
 			std::lock_guard<std::mutex> g(lock);
 			j->done = true;
 			j->t.done = now();
-			j->held = j->acc64.bytes;
-			inUse -= j->bytes() - j->held;
 			core.pendingJobs--;
+		}
+		finished.notify_all();
+		if (j->error.empty())
+			try
+			{	/* denoised jobs: the direct half, as RenderCore::ReadAccumulatorF64 */
+				const size_t pixels = (size_t)j->w * j->h;
+				j->hostAcc.resize(pixels * 4);
+				drainCopy(j->hostAcc.data(), j->acc64.p, pixels * 4 * sizeof(double));
+			}
+			catch (const std::exception& e) { j->accError = e.what(); }
+		(void)hipEventDestroy(j->ready);
+		j->ready = nullptr;
+		releaseJob(*j);
+		{
+			std::lock_guard<std::mutex> g(lock);
+			j->accReady = true;
+			inUse -= j->bytes();
 		}
 		finished.notify_all();
 		wake.notify_all();   /* the freed budget may admit waiting jobs */
@@ -444,7 +453,7 @@ void RenderServer::loop()
 					std::lock_guard<std::mutex> g(lock);
 					for (auto& j : failed)
 					{
-						j->done = true;
+						j->done = j->accReady = true;
 						j->t.done = now();
 						inUse -= j->bytes();
 						core.pendingJobs--;
@@ -491,19 +500,16 @@ void RenderServer::Read(uint64_t id, void* image, double* acc64)
 		j = it->second;
 		jobs.erase(it);
 	}
-	hipError_t e = hipSuccess;
-	if (j->error.empty() && acc64)   /* denoised jobs: the direct half, as RenderCore::ReadAccumulatorF64 */
-		e = hipMemcpy(acc64, j->acc64.p, (size_t)j->w * j->h * 4 * sizeof(double), hipMemcpyDeviceToHost);
-	giveBuf(j->acc64);
-	{
-		std::lock_guard<std::mutex> g(lock);
-		inUse -= j->held;
-		j->held = 0;
-	}
-	wake.notify_all();   /* the freed budget may admit waiting jobs */
 	if (!j->error.empty()) throw CoreError(LH2HIP_ERR_DEVICE, "server job failed: " + j->error);
-	check(e, "D2H");
 	if (image) memcpy(image, j->hostImage.data(), j->hostImage.size() * sizeof(float));
+	if (acc64)
+	{	/* the accumulator reaches the host shortly after the image */
+		std::unique_lock<std::mutex> g(lock);
+		finished.wait(g, [&] { return j->accReady; });
+		g.unlock();
+		if (!j->accError.empty()) throw CoreError(LH2HIP_ERR_DEVICE, "server job failed: " + j->accError);
+		memcpy(acc64, j->hostAcc.data(), j->hostAcc.size() * sizeof(double));
+	}
 }
 
 JobTimes RenderServer::Times(uint64_t id)

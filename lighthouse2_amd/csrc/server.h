@@ -69,7 +69,9 @@ private:
 		std::string error;
 		DevBuf acc32, acc64, image, gFeatures, gWorldPos, gDeltaDepth;
 		std::vector<float> hostImage;
-		uint64_t held = 0;              /* budget bytes a drained job still holds: its fp64 accumulator, until read */
+		std::vector<double> hostAcc;
+		bool accReady = false;          /* hostAcc filled (after done), the job's buffers and budget released */
+		std::string accError;
 		JobTimes t = { 0, 0, 0 };
 		FrameSetup frame;
 		/* accumulators + image: 64 B/px; denoised: direct and indirect halves
@@ -84,10 +86,9 @@ private:
 	void finishJob(Job& j);
 	void releaseJob(Job& j);
 	/* The drain thread: copies a finished job's image to the host on its own
-	 * stream through two pinned staging buffers, returns the job's buffers
-	 * but its fp64 accumulator to the pool and marks the job done, while the
-	 * worker goes on with the next pass. The accumulator stays on the device
-	 * (and in the budget) until Read copies it or releases the job. */
+	 * stream through two pinned staging buffers and marks the job done, then
+	 * copies its fp64 accumulator, returns its buffers to the pool and
+	 * releases its budget, while the worker goes on with the next pass. */
 	void drainLoop();
 	void drainCopy(void* dst, const void* src, size_t bytes);
 	/* Per-job device buffers are kept for reuse, keyed by size, up to the
