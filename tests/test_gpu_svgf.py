@@ -10,7 +10,7 @@ import pytest
 import lighthouse2_amd as lh
 from lighthouse2_amd import scenes
 import oracle as orc
-from svgf_inputs import gbuffer
+from lighthouse2_amd.scenes import gbuffer
 
 pytestmark = pytest.mark.gpu
 

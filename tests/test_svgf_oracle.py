@@ -6,7 +6,7 @@ import numpy as np
 
 from lighthouse2_amd import scenes
 import oracle as orc
-from svgf_inputs import gbuffer
+from lighthouse2_amd.scenes import gbuffer
 
 
 def test_constant_image_stays_constant():
